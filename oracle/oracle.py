@@ -1,0 +1,265 @@
+"""ctypes binding of the oracle (CPU FP64 restatement of the reference slice loop).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
+
+LAYOUT_NAMES = ["ncomp", "next_jx_beam", "next_jy_beam", "chi", "Sy", "Sx", "ExmBy", "EypBx", "Ez", "Bx",
+                "By", "Bz", "Psi", "jx_beam", "jy_beam", "jz_beam", "jx", "jy", "rhomjz", "rho",
+                "prev_jx_beam", "prev_jy_beam", "ions_rhomjz", "next_jx", "next_jy", "jz", "prev_Bx",
+                "prev_By", "prev_jx", "prev_jy", "pciter_Bx", "pciter_By"]
+
+PLASMA_REALS = ["x", "y", "w", "ux", "uy", "psi", "x_prev", "y_prev", "ux_half_step", "uy_half_step",
+                "psi_half_step"]
+
+
+class OrcConfig(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("xlo", C.c_double), ("xhi", C.c_double), ("ylo", C.c_double), ("yhi", C.c_double),
+                ("zlo", C.c_double), ("zhi", C.c_double),
+                ("depos_order", C.c_int), ("derivative_type", C.c_int), ("explicit_solver", C.c_int),
+                ("deposit_rho", C.c_int), ("do_beam_jx_jy", C.c_int),
+                ("mg_tol_rel", C.c_double), ("mg_tol_abs", C.c_double), ("mg_maxiter", C.c_int),
+                ("particle_bc", C.c_int), ("plo", C.c_double * 2), ("phi", C.c_double * 2),
+                ("pc_tol", C.c_double), ("pc_maxiter", C.c_int), ("pc_mix", C.c_double),
+                ("has_plasma", C.c_int), ("ppc", C.c_int * 2),
+                ("density", C.c_double), ("ramp_a", C.c_double), ("plasma_radius", C.c_double),
+                ("u_mean", C.c_double * 3),
+                ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double),
+                ("neutralize", C.c_int), ("nthreads", C.c_int)]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc); returns the library path."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "-j8"], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.orc_sim_create.restype = vp
+        L.orc_sim_create.argtypes = [C.POINTER(OrcConfig)]
+        L.orc_sim_destroy.argtypes = [vp]
+        L.orc_sim_ncomp.argtypes = [vp]
+        L.orc_sim_layout.argtypes = [vp, ip]
+        L.orc_sim_set_beam.argtypes = [vp, C.c_int64] + [dp] * 7 + [C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                                             C.c_double, C.c_double]
+        L.orc_sim_add_capture.argtypes = [vp, C.c_int, dp]
+        L.orc_sim_clear_captures.argtypes = [vp]
+        L.orc_sim_begin_step.argtypes = [vp]
+        L.orc_sim_solve_slice.argtypes = [vp, C.c_int]
+        L.orc_sim_run_step.argtypes = [vp, C.c_int, C.c_int]
+        L.orc_sim_run_step.restype = C.c_double
+        L.orc_sim_checksums.argtypes = [vp, dp]
+        L.orc_sim_stats.argtypes = [vp, dp]
+        L.orc_sim_get_store.argtypes = [vp, dp]
+        L.orc_sim_set_store.argtypes = [vp, dp]
+        L.orc_sim_plasma_size.argtypes = [vp]
+        L.orc_sim_plasma_size.restype = C.c_int64
+        L.orc_sim_get_plasma.argtypes = [vp, dp, C.POINTER(C.c_int32)]
+        L.orc_sim_set_plasma.argtypes = [vp, C.c_int64, dp, C.POINTER(C.c_int32)]
+        L.orc_deposit_plasma.argtypes = [vp, ip, C.c_int, C.POINTER(C.c_int64)]
+        L.orc_explicit_deposit.argtypes = [vp]
+        L.orc_deposit_beam.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_push_plasma.argtypes = [vp, C.c_int]
+        L.orc_solve_poisson_psi_ez_bz.argtypes = [vp]
+        L.orc_solve_poisson_bxby.argtypes = [vp]
+        L.orc_initialize_sxsy_with_beam.argtypes = [vp]
+        L.orc_mg_solve1_store.argtypes = [vp, C.c_double, C.c_double, C.c_int, ip]
+        L.orc_dst1.argtypes = [dp, dp, C.c_int]
+        L.orc_poisson.argtypes = [C.POINTER(OrcConfig), dp, dp]
+        L.orc_shape.argtypes = [C.c_int, C.c_double, C.c_int, dp]
+        L.orc_dshape.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_config(cfg, nthreads: int = 0) -> OrcConfig:
+    from hipace2_amd.inputs import BC_CODES
+    (plx, ply), (phx, phy) = cfg.bounds
+    c = OrcConfig()
+    c.nx, c.ny, c.nz = cfg.nx, cfg.ny, cfg.nz
+    c.xlo, c.xhi, c.ylo, c.yhi, c.zlo, c.zhi = cfg.xlo, cfg.xhi, cfg.ylo, cfg.yhi, cfg.zlo, cfg.zhi
+    c.depos_order, c.derivative_type = cfg.depos_order, cfg.derivative_type
+    c.explicit_solver, c.deposit_rho = int(cfg.explicit_solver), int(cfg.deposit_rho)
+    c.do_beam_jx_jy = int(cfg.do_beam_jx_jy)
+    c.mg_tol_rel, c.mg_tol_abs, c.mg_maxiter = cfg.mg_tol_rel, cfg.mg_tol_abs, cfg.mg_maxiter
+    c.particle_bc = BC_CODES[cfg.particle_bc]
+    c.plo[0], c.plo[1], c.phi[0], c.phi[1] = plx, ply, phx, phy
+    c.pc_tol, c.pc_maxiter, c.pc_mix = cfg.pc_tol, cfg.pc_maxiter, cfg.pc_mix
+    c.has_plasma = int(cfg.has_plasma)
+    c.ppc[0], c.ppc[1] = cfg.ppc
+    c.density, c.ramp_a, c.plasma_radius = cfg.density, cfg.ramp_a, cfg.plasma_radius
+    for k in range(3):
+        c.u_mean[k] = cfg.u_mean[k]
+    c.plasma_charge, c.plasma_mass, c.max_qsa = cfg.plasma_charge, cfg.plasma_mass, cfg.max_qsa
+    c.neutralize = int(cfg.neutralize)
+    c.nthreads = nthreads
+    return c
+
+
+class OracleSim:
+    """One slice-loop simulation on the CPU (the checker)."""
+
+    def __init__(self, cfg, beam=None, nthreads: int = 0):
+        self.cfg = cfg
+        self._c = make_config(cfg, nthreads)
+        self.h = lib().orc_sim_create(C.byref(self._c))
+        arr = (C.c_int * 32)()
+        lib().orc_sim_layout(self.h, arr)
+        self.layout = dict(zip(LAYOUT_NAMES, list(arr)))
+        self.ncomp = self.layout["ncomp"]
+        self.ng = 2
+        self.ldx, self.ldy = cfg.nx + 4, cfg.ny + 4
+        self._captures = {}
+        self._beam = None
+        if beam is not None:
+            self.set_beam(beam)
+
+    def __del__(self):
+        try:
+            lib().orc_sim_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_beam(self, b):
+        self._beam = [np.ascontiguousarray(a, dtype=np.float64) for a in (b.x, b.y, b.z, b.ux, b.uy, b.uz, b.w)]
+        self._bvalid = np.ascontiguousarray(b.valid, dtype=np.int32)
+        self._boff = np.ascontiguousarray(b.off, dtype=np.int64)
+        lib().orc_sim_set_beam(self.h, b.n, *[_dp(a) for a in self._beam],
+                               self._bvalid.ctypes.data_as(C.POINTER(C.c_int32)),
+                               self._boff.ctypes.data_as(C.POINTER(C.c_int64)), b.charge, b.mass)
+
+    def capture(self, name: str) -> np.ndarray:
+        """Record the valid region of component `name` at the diagnostics point of every slice."""
+        buf = np.zeros((self.cfg.nz, self.cfg.ny, self.cfg.nx))
+        self._captures[name] = buf
+        lib().orc_sim_add_capture(self.h, self.layout[name], _dp(buf))
+        return buf
+
+    def run_step(self, first=None, last=0) -> float:
+        first = self.cfg.nz - 1 if first is None else first
+        return lib().orc_sim_run_step(self.h, first, last)
+
+    def begin_step(self):
+        lib().orc_sim_begin_step(self.h)
+
+    def solve_slice(self, islice: int):
+        lib().orc_sim_solve_slice(self.h, islice)
+
+    def checksums(self) -> dict:
+        out = np.zeros(self.ncomp)
+        lib().orc_sim_checksums(self.h, _dp(out))
+        return {name: out[idx] for name, idx in self.layout.items() if name != "ncomp" and idx >= 0}
+
+    def stats(self) -> dict:
+        out = np.zeros(6)
+        lib().orc_sim_stats(self.h, _dp(out))
+        return dict(n_qsa=out[0], mg_iters=out[1], mg_status=out[2], pc_iters=out[3], pushes=out[4],
+                    np=out[5])
+
+    def store(self) -> np.ndarray:
+        out = np.zeros((self.ncomp, self.ldy, self.ldx))
+        lib().orc_sim_get_store(self.h, _dp(out))
+        return out
+
+    def set_store(self, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.size == self.ncomp * self.ldx * self.ldy
+        lib().orc_sim_set_store(self.h, _dp(a))
+
+    def plasma(self) -> dict:
+        n = lib().orc_sim_plasma_size(self.h)
+        reals = np.zeros((11, n))
+        valid = np.zeros(n, np.int32)
+        lib().orc_sim_get_plasma(self.h, _dp(reals), valid.ctypes.data_as(C.POINTER(C.c_int32)))
+        d = {k: reals[i] for i, k in enumerate(PLASMA_REALS)}
+        d["valid"] = valid
+        return d
+
+    def set_plasma(self, d: dict):
+        n = len(d["x"])
+        reals = np.ascontiguousarray(np.stack([np.asarray(d[k], np.float64) for k in PLASMA_REALS]))
+        valid = np.ascontiguousarray(d["valid"], dtype=np.int32)
+        lib().orc_sim_set_plasma(self.h, n, _dp(reals), valid.ctypes.data_as(C.POINTER(C.c_int32)))
+
+    # kernel-level calls on the current store / plasma
+    def deposit_plasma(self, comps, ions=False) -> int:
+        arr = (C.c_int * 6)(*[self.layout.get(c, -1) if isinstance(c, str) else c for c in comps])
+        nq = C.c_int64(0)
+        lib().orc_deposit_plasma(self.h, arr, int(ions), C.byref(nq))
+        return nq.value
+
+    def explicit_deposit(self):
+        lib().orc_explicit_deposit(self.h)
+
+    def deposit_beam(self, islice, cjx, cjy, cjz):
+        f = lambda c: self.layout.get(c, -1) if isinstance(c, str) else c
+        lib().orc_deposit_beam(self.h, islice, f(cjx), f(cjy), f(cjz))
+
+    def push_plasma(self, temp_slice=False):
+        lib().orc_push_plasma(self.h, int(temp_slice))
+
+    def solve_poisson_psi_ez_bz(self):
+        lib().orc_solve_poisson_psi_ez_bz(self.h)
+
+    def solve_poisson_bxby(self):
+        lib().orc_solve_poisson_bxby(self.h)
+
+    def initialize_sxsy_with_beam(self):
+        lib().orc_initialize_sxsy_with_beam(self.h)
+
+    def mg_solve1(self, tol_rel=1e-4, tol_abs=2.2250738585072014e-308, maxiter=200):
+        it = C.c_int(0)
+        st = lib().orc_mg_solve1_store(self.h, tol_rel, tol_abs, maxiter, C.byref(it))
+        return st, it.value
+
+
+def dst1(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    lib().orc_dst1(_dp(x), _dp(out), x.shape[0])
+    return out
+
+
+def poisson(cfg, staging: np.ndarray) -> np.ndarray:
+    c = make_config(cfg)
+    st = np.ascontiguousarray(staging, dtype=np.float64).reshape(cfg.ny, cfg.nx)
+    lhs = np.zeros((cfg.ny + 4, cfg.nx + 4))
+    lib().orc_poisson(C.byref(c), _dp(st), _dp(lhs))
+    return lhs
+
+
+def shape(order, xmid, ix):
+    out = np.zeros(2)
+    lib().orc_shape(order, xmid, ix, _dp(out))
+    return out[0], int(out[1])
+
+
+def dshape(dtype, order, xmid, ix):
+    out = np.zeros(3)
+    lib().orc_dshape(dtype, order, xmid, ix, _dp(out))
+    return out[0], out[1], int(out[2])
