@@ -1,0 +1,111 @@
+"""ctypes binding of the C ABI in include/hpq.h (libhpq.so, built in-tree by build.py).
+
+The library is the only compute path: if it is missing this module raises, there is
+no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhpq.so")
+
+HPQ_OK = 0
+ERRORS = {1: "HPQ_E_INVALID", 2: "HPQ_E_HIP", 3: "HPQ_E_MG_NOT_CONVERGED", 4: "HPQ_E_MG_DIVERGED",
+          5: "HPQ_E_UNSUPPORTED", 6: "HPQ_E_NOMEM"}
+
+LAYOUT_NAMES = ["ncomp", "next_jx_beam", "next_jy_beam", "chi", "Sy", "Sx", "ExmBy", "EypBx", "Ez", "Bx",
+                "By", "Bz", "Psi", "jx_beam", "jy_beam", "jz_beam", "jx", "jy", "rhomjz", "rho",
+                "prev_jx_beam", "prev_jy_beam", "ions_rhomjz", "next_jx", "next_jy", "jz", "prev_Bx",
+                "prev_By", "prev_jx", "prev_jy", "pciter_Bx", "pciter_By"]
+
+EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq_layout", "hpq_store_doubles",
+           "hpq_deposit_current", "hpq_explicit_deposition", "hpq_deposit_current_slice",
+           "hpq_advance_plasma_particles", "hpq_poisson_dirichlet", "hpq_solve_poisson_psi_exmby_eypbx_ez_bz",
+           "hpq_solve_poisson_bxby", "hpq_initialize_sxsy_with_beam", "hpq_mg_solve1", "hpq_plasma_count",
+           "hpq_init_plasma", "hpq_begin_step", "hpq_solve_one_slice", "hpq_run_slices", "hpq_set_checksums",
+           "hpq_add_capture", "hpq_clear_captures", "hpq_stats"]
+
+
+class Params(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("xlo", C.c_double), ("xhi", C.c_double), ("ylo", C.c_double), ("yhi", C.c_double),
+                ("zlo", C.c_double), ("zhi", C.c_double),
+                ("depos_order", C.c_int), ("derivative_type", C.c_int), ("explicit_solver", C.c_int),
+                ("deposit_rho", C.c_int), ("do_beam_jx_jy", C.c_int), ("neutralize", C.c_int),
+                ("mg_tol_rel", C.c_double), ("mg_tol_abs", C.c_double), ("mg_maxiter", C.c_int),
+                ("particle_bc", C.c_int), ("plo", C.c_double * 2), ("phi", C.c_double * 2),
+                ("pc_tol", C.c_double), ("pc_maxiter", C.c_int), ("pc_mix", C.c_double),
+                ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double)]
+
+
+class Plasma(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("x", "y", "w", "ux", "uy", "psi", "x_prev", "y_prev", "ux_half_step",
+                                           "uy_half_step", "psi_half_step", "valid")] + [("n", C.c_int64)]
+
+
+class Beam(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("x", "y", "z", "ux", "uy", "uz", "w", "valid")] + \
+               [("off", C.POINTER(C.c_int64)), ("n", C.c_int64), ("charge", C.c_double), ("mass", C.c_double)]
+
+
+class PlasmaProfile(C.Structure):
+    _fields_ = [("ppc", C.c_int * 2), ("density", C.c_double), ("ramp_a", C.c_double), ("radius", C.c_double),
+                ("u_mean", C.c_double * 3)]
+
+
+class HpqError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libhpq.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python hipace2_amd/build.py` "
+                               "(or __graft_entry__.build()) before using hipace2_amd")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.c_int, C.POINTER(C.c_double)
+        i32p = C.POINTER(C.c_int32)
+        L.hpq_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+        L.hpq_destroy.argtypes = [vp]
+        L.hpq_last_error.argtypes = [vp]
+        L.hpq_last_error.restype = C.c_char_p
+        L.hpq_set_stream.argtypes = [vp, vp]
+        L.hpq_layout.argtypes = [vp, i32p]
+        L.hpq_store_doubles.argtypes = [vp]
+        L.hpq_store_doubles.restype = C.c_int64
+        L.hpq_deposit_current.argtypes = [vp, C.POINTER(Plasma), vp, i32p, ip, C.POINTER(C.c_int64)]
+        L.hpq_explicit_deposition.argtypes = [vp, C.POINTER(Plasma), vp]
+        L.hpq_deposit_current_slice.argtypes = [vp, C.POINTER(Beam), ip, vp, i32p]
+        L.hpq_advance_plasma_particles.argtypes = [vp, C.POINTER(Plasma), vp, ip]
+        L.hpq_poisson_dirichlet.argtypes = [vp, vp, vp]
+        L.hpq_solve_poisson_psi_exmby_eypbx_ez_bz.argtypes = [vp, vp]
+        L.hpq_solve_poisson_bxby.argtypes = [vp, vp]
+        L.hpq_initialize_sxsy_with_beam.argtypes = [vp, vp]
+        L.hpq_mg_solve1.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_double, C.c_double, ip, C.POINTER(ip)]
+        L.hpq_plasma_count.argtypes = [vp, C.POINTER(PlasmaProfile)]
+        L.hpq_plasma_count.restype = C.c_int64
+        L.hpq_init_plasma.argtypes = [vp, C.POINTER(PlasmaProfile), C.POINTER(Plasma)]
+        L.hpq_begin_step.argtypes = [vp, C.POINTER(Plasma), vp]
+        L.hpq_solve_one_slice.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, ip]
+        L.hpq_run_slices.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, ip, ip]
+        L.hpq_set_checksums.argtypes = [vp, vp]
+        L.hpq_add_capture.argtypes = [vp, ip, vp]
+        L.hpq_clear_captures.argtypes = [vp]
+        L.hpq_stats.argtypes = [vp, dp]
+        _lib = L
+    return _lib
+
+
+def check(ctx, rc):
+    if rc != HPQ_OK:
+        msg = lib().hpq_last_error(ctx)
+        raise HpqError(rc, msg.decode() if msg else "")

@@ -1,0 +1,553 @@
+// Host side of the slice loop, mirroring the reference's call surface:
+//   Fields (InitializeSlices, ShiftSlices, AddRhoIons, SolvePoissonPsiExmByEypBxEzBz,
+//           SolvePoissonBxBy, InitialBfieldGuess, MixAndShiftBfields, ComputeRelBFieldError)
+//   FFTPoissonSolverDirichlet (SolvePoissonEquation on the staging area)
+//   hpmg::MultiGrid::solve1
+//   DepositCurrent, ExplicitDeposition, DepositCurrentSlice, AdvancePlasmaParticles
+//   Hipace (SolveOneSlice, InitializeSxSyWithBeam, ExplicitMGSolveBxBy,
+//           PredictorCorrectorLoopToSolveBxBy)
+// and the C ABI of include/hpq.h on top of them. Everything runs on the context's
+// stream; the only host synchronisations are the ones the reference also has
+// (MG residual norm per V-cycle, PC-loop B-field error, QSA counter on request).
+#include <cmath>
+#include <cstring>
+
+#include "internal.hpp"
+
+namespace hpq {
+
+void layout_init(hpq_ctx* c) {
+    Layout L;
+    int n = 0;
+    const bool ions = c->p.neutralize != 0;
+    if (c->p.explicit_solver) {
+        L.next_jx_beam = n++; L.next_jy_beam = n++;
+        L.chi = n++; L.Sy = n++; L.Sx = n++; L.ExmBy = n++; L.EypBx = n++; L.Ez = n++;
+        L.Bx = n++; L.By = n++; L.Bz = n++; L.Psi = n++;
+        L.jx_beam = n++; L.jy_beam = n++; L.jz_beam = n++; L.jx = n++; L.jy = n++; L.rhomjz = n++;
+        if (c->p.deposit_rho) L.rho = n++;
+        L.prev_jx_beam = n++; L.prev_jy_beam = n++;
+        if (ions) L.ions_rhomjz = n++;
+    } else {
+        L.next_jx = n++; L.next_jy = n++;
+        L.ExmBy = n++; L.EypBx = n++; L.Ez = n++; L.Bx = n++; L.By = n++; L.Bz = n++; L.Psi = n++;
+        L.jx = n++; L.jy = n++; L.jz = n++; L.rhomjz = n++;
+        if (c->p.deposit_rho) L.rho = n++;
+        L.prev_Bx = n++; L.prev_By = n++; L.prev_jx = n++; L.prev_jy = n++;
+        if (ions) L.ions_rhomjz = n++;
+        L.pciter_Bx = n++; L.pciter_By = n++;
+        L.pcprev_Bx = n++; L.pcprev_By = n++;
+    }
+    L.ncomp = n;
+    c->L = L;
+}
+
+// ---------------------------------------------------------------------------------
+// Fields (src/fields/Fields.cpp)
+// ---------------------------------------------------------------------------------
+class Fields {
+public:
+    Fields(hpq_ctx* c, double* store) : c_(c), s_(store), L_(c->L) {}
+
+    // Fields::InitializeSlices, :536-586 (level 0)
+    void InitializeSlices() {
+        if (c_->p.explicit_solver) {
+            const int z[] = {L_.chi, L_.Sy, L_.Sx, L_.ExmBy, L_.EypBx, L_.jz_beam, L_.rhomjz, L_.next_jx_beam};
+            plane_fill(c_, s_, z, 8, 0.0);
+            const int z2[] = {L_.next_jy_beam, L_.rho};
+            plane_fill(c_, s_, z2, 2, 0.0);
+        } else {
+            const int z[] = {L_.ExmBy, L_.EypBx, L_.jx, L_.jy, L_.jz, L_.rhomjz, L_.rho};
+            plane_fill(c_, s_, z, 7, 0.0);
+        }
+    }
+    // Fields::ShiftSlices, :589-604
+    void ShiftSlices() {
+        if (c_->p.explicit_solver) {
+            const int d1[] = {L_.prev_jx_beam, L_.prev_jy_beam};
+            const int s1[] = {L_.jx_beam, L_.jy_beam};
+            plane_copy(c_, s_, d1, s1, 2);
+            const int d2[] = {L_.jx_beam, L_.jy_beam, L_.jx, L_.jy};
+            const int s2[] = {L_.next_jx_beam, L_.next_jy_beam, L_.next_jx_beam, L_.next_jy_beam};
+            plane_copy(c_, s_, d2, s2, 4);
+        } else {
+            const int d1[] = {L_.pcprev_Bx, L_.pcprev_By};
+            const int s1[] = {L_.prev_Bx, L_.prev_By};
+            plane_copy(c_, s_, d1, s1, 2);
+            const int d2[] = {L_.prev_Bx, L_.prev_By, L_.prev_jx, L_.prev_jy};
+            const int s2[] = {L_.Bx, L_.By, L_.jx, L_.jy};
+            plane_copy(c_, s_, d2, s2, 4);
+        }
+    }
+    // Fields::AddRhoIons, :607-615
+    void AddRhoIons() {
+        if (L_.ions_rhomjz < 0) return;
+        plane_add(c_, s_, L_.rhomjz, L_.ions_rhomjz);
+        if (L_.rho >= 0) plane_add(c_, s_, L_.rho, L_.ions_rhomjz);
+    }
+    // Fields::SolvePoissonPsiExmByEypBxEzBz, :841-957
+    void SolvePoissonPsiExmByEypBxEzBz() {
+        build_rhs_psi_ez_bz(c_, s_);
+        double* lhs[3] = {s_ + long(L_.Psi) * c_->g.plane, s_ + long(L_.Ez) * c_->g.plane,
+                          s_ + long(L_.Bz) * c_->g.plane};
+        poisson_solve_staged(c_, 3, lhs);
+        exmby_eypbx(c_, s_);
+    }
+    // Fields::SolvePoissonBxBy, :1009-1078 (into PCIter)
+    void SolvePoissonBxBy() {
+        build_rhs_bxby(c_, s_);
+        double* lhs[3] = {s_ + long(L_.pciter_Bx) * c_->g.plane, s_ + long(L_.pciter_By) * c_->g.plane, nullptr};
+        poisson_solve_staged(c_, 2, lhs);
+    }
+    // Fields::InitialBfieldGuess, :1150-1171
+    void InitialBfieldGuess(double err, double tol) {
+        const double m = std::exp(-0.5 * std::pow(err / (2.5 * tol), 2));
+        plane_lincomb(c_, s_, L_.Bx, 1. + m, L_.prev_Bx, -m, L_.pcprev_Bx);
+        plane_lincomb(c_, s_, L_.By, 1. + m, L_.prev_By, -m, L_.pcprev_By);
+    }
+    // Fields::MixAndShiftBfields, :1174-1226
+    void MixAndShiftBfields(double err, double err_prev, double mix) {
+        double wi, wp;
+        if (err != 0. || err_prev != 0.) {
+            wi = err_prev / (err + err_prev);
+            wp = err / (err + err_prev);
+        } else {
+            wi = 0.5;
+            wp = 0.5;
+        }
+        plane_lincomb(c_, s_, L_.pcprev_Bx, wi, L_.pciter_Bx, wp, L_.pcprev_Bx);
+        plane_lincomb(c_, s_, L_.pcprev_By, wi, L_.pciter_By, wp, L_.pcprev_By);
+        plane_lincomb(c_, s_, L_.Bx, 1. - mix, L_.Bx, mix, L_.pcprev_Bx);
+        plane_lincomb(c_, s_, L_.By, 1. - mix, L_.By, mix, L_.pcprev_By);
+        const int d[] = {L_.pcprev_Bx, L_.pcprev_By};
+        const int s[] = {L_.pciter_Bx, L_.pciter_By};
+        plane_copy(c_, s_, d, s, 2);
+    }
+    // Fields::ComputeRelBFieldError, :1229-1286
+    double ComputeRelBFieldError(int bx, int by, int bxi, int byi) { return rel_b_error(c_, s_, bx, by, bxi, byi); }
+
+private:
+    hpq_ctx* c_;
+    double* s_;
+    const Layout& L_;
+};
+
+namespace hpmg {
+// hpmg::MultiGrid (src/mg_solver/HpMultiGrid.H:37-268), type I; the hierarchy lives in
+// the context and is built once per grid.
+class MultiGrid {
+public:
+    explicit MultiGrid(hpq_ctx* c) : c_(c) {}
+    int solve1(double* sol, const double* rhs, const double* acf, long comp_stride, double tol_rel, double tol_abs,
+               int nummaxiter, int* iters) {
+        return mg_solve1(c_, sol, rhs, acf, comp_stride, tol_rel, tol_abs, nummaxiter, iters);
+    }
+
+private:
+    hpq_ctx* c_;
+};
+}  // namespace hpmg
+
+// ::DepositCurrent (src/particles/deposition/PlasmaDepositCurrent.cpp:22)
+void DepositCurrent(hpq_ctx* c, const hpq_plasma* p, double* store, int jx, int jy, int jz, int rho, int chi,
+                    int rhomjz, bool ions = false) {
+    const int32_t comps[6] = {jx, jy, jz, rho, chi, rhomjz};
+    deposit_current(c, p, store, comps, ions);
+}
+
+// ::DepositCurrentSlice (src/particles/deposition/BeamDepositCurrent.cpp:22)
+void DepositCurrentSlice(hpq_ctx* c, const hpq_beam* b, int slice, double* store, int jx, int jy, int jz) {
+    const int32_t comps[3] = {jx, jy, jz};
+    deposit_beam(c, b, slice, store, comps);
+}
+
+// ---------------------------------------------------------------------------------
+// Hipace (src/Hipace.cpp)
+// ---------------------------------------------------------------------------------
+class Hipace {
+public:
+    Hipace(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store)
+        : c_(c), p_(p), b_(b), s_(store), fields_(c, store), L_(c->L) {}
+
+    // Hipace::SolveOneSlice, :557-728 (level 0; no laser, SALAME, ionisation, collisions)
+    void SolveOneSlice(int islice) {
+        const hpq_params& P = c_->p;
+        fields_.InitializeSlices();
+        if (P.explicit_solver) {
+            DepositCurrent(c_, p_, s_, L_.jx, L_.jy, -1, L_.rho, L_.chi, L_.rhomjz);
+            DepositCurrentSlice(c_, b_, islice, s_, -1, -1, L_.jz_beam);
+        } else {
+            DepositCurrent(c_, p_, s_, L_.jx, L_.jy, L_.jz, L_.rho, -1, L_.rhomjz);
+            DepositCurrentSlice(c_, b_, islice, s_, P.do_beam_jx_jy ? L_.jx : -1, P.do_beam_jx_jy ? L_.jy : -1,
+                                L_.jz);
+        }
+        fields_.AddRhoIons();
+        fields_.SolvePoissonPsiExmByEypBxEzBz();
+        if (P.explicit_solver) {
+            if (P.do_beam_jx_jy) DepositCurrentSlice(c_, b_, islice - 1, s_, L_.next_jx_beam, L_.next_jy_beam, -1);
+            InitializeSxSyWithBeam();
+            explicit_deposition(c_, p_, s_);
+            ExplicitMGSolveBxBy();
+        } else {
+            PredictorCorrectorLoopToSolveBxBy(islice);
+        }
+        diag_accumulate(c_, s_, islice);
+        advance_plasma(c_, p_, s_, false);
+        c_->stats[3] += double(p_->n);
+        fields_.ShiftSlices();
+    }
+
+    // Hipace::InitializeSxSyWithBeam, :745-790
+    void InitializeSxSyWithBeam() { sxsy_from_beam(c_, s_); }
+
+    // Hipace::ExplicitMGSolveBxBy, :794-933 (hpmg branch, level 0)
+    void ExplicitMGSolveBxBy() {
+        hpmg::MultiGrid mg(c_);
+        const long ps = c_->g.plane;
+        int iters = 0;
+        const int st = mg.solve1(s_ + long(L_.Bx) * ps, s_ + long(L_.Sy) * ps, s_ + long(L_.chi) * ps, ps,
+                                 c_->p.mg_tol_rel, c_->p.mg_tol_abs, c_->p.mg_maxiter, &iters);
+        c_->stats[1] += iters;
+        if (st == HPQ_E_MG_DIVERGED) throw HpqError(st, "hpmg failing so lets stop here");
+        if (st == HPQ_E_MG_NOT_CONVERGED) throw HpqError(st, "hpmg failed");
+    }
+
+    // Hipace::PredictorCorrectorLoopToSolveBxBy, :936-1031
+    void PredictorCorrectorLoopToSolveBxBy(int islice) {
+        const hpq_params& P = c_->p;
+        double err_prev = 1.0;
+        double err = fields_.ComputeRelBFieldError(L_.prev_Bx, L_.prev_By, L_.pcprev_Bx, L_.pcprev_By);
+        fields_.InitialBfieldGuess(err, P.pc_tol);
+        const int z[] = {L_.pciter_Bx, L_.pciter_By};
+        plane_fill(c_, s_, z, 2, 0.0);
+        const int d[] = {L_.pcprev_Bx, L_.pcprev_By};
+        const int s[] = {L_.Bx, L_.By};
+        plane_copy(c_, s_, d, s, 2);
+        int it = 0;
+        err = 1.0;
+        while (err > P.pc_tol && it < P.pc_maxiter) {
+            ++it;
+            advance_plasma(c_, p_, s_, true);
+            DepositCurrent(c_, p_, s_, L_.next_jx, L_.next_jy, -1, -1, -1, -1);
+            if (P.do_beam_jx_jy) DepositCurrentSlice(c_, b_, islice - 1, s_, L_.next_jx, L_.next_jy, -1);
+            fields_.SolvePoissonBxBy();
+            err = fields_.ComputeRelBFieldError(L_.Bx, L_.By, L_.pciter_Bx, L_.pciter_By);
+            if (it == 1) err_prev = err;
+            fields_.MixAndShiftBfields(err, err_prev, P.pc_mix);
+            const int zz[] = {L_.next_jx, L_.next_jy};
+            plane_fill(c_, s_, zz, 2, 0.0);
+            err_prev = err;
+        }
+        c_->stats[2] += it;
+    }
+
+private:
+    hpq_ctx* c_;
+    const hpq_plasma* p_;
+    const hpq_beam* b_;
+    double* s_;
+    Fields fields_;
+    const Layout& L_;
+};
+
+}  // namespace hpq
+
+// ---------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------
+using namespace hpq;
+
+namespace {
+
+template <class F>
+int guarded(hpq_ctx* c, F&& f) {
+    try {
+        f();
+        return HPQ_OK;
+    } catch (const HpqError& e) {
+        if (c) c->err = e.msg;
+        return e.code;
+    } catch (const HipError& e) {
+        if (c) c->err = e.msg;
+        return HPQ_E_HIP;
+    } catch (const std::bad_alloc&) {
+        if (c) c->err = "out of host memory";
+        return HPQ_E_NOMEM;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        return HPQ_E_INVALID;
+    }
+}
+
+void check_store(const hpq_ctx* c, const void* store) {
+    if (!store) throw HpqError(HPQ_E_INVALID, "slice store pointer is null");
+}
+void check_plasma(const hpq_plasma* p) {
+    if (!p) throw HpqError(HPQ_E_INVALID, "plasma descriptor is null");
+    if (p->n > 0 && (!p->x || !p->y || !p->w || !p->ux || !p->uy || !p->psi || !p->x_prev || !p->y_prev ||
+                     !p->ux_half_step || !p->uy_half_step || !p->psi_half_step || !p->valid))
+        throw HpqError(HPQ_E_INVALID, "plasma descriptor has a null array");
+}
+void check_comp(const hpq_ctx* c, int k) {
+    if (k < -1 || k >= c->L.ncomp) throw HpqError(HPQ_E_INVALID, "component index out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+int hpq_create(const hpq_params* p, hpq_ctx** out) {
+    if (!p || !out) return HPQ_E_INVALID;
+    *out = nullptr;
+    hpq_ctx* c = new hpq_ctx();
+    const int rc = guarded(c, [&] {
+        if (p->nx < 4 || p->ny < 4 || p->nz < 1) throw HpqError(HPQ_E_INVALID, "grid too small");
+        if (!(p->xhi > p->xlo && p->yhi > p->ylo && p->zhi > p->zlo)) throw HpqError(HPQ_E_INVALID, "empty domain");
+        if (p->depos_order < 0 || p->depos_order > 3) throw HpqError(HPQ_E_UNSUPPORTED, "depos_order_xy must be 0..3");
+        if (p->derivative_type < 0 || p->derivative_type > 2)
+            throw HpqError(HPQ_E_UNSUPPORTED, "depos_derivative_type must be 0..2");
+        if (p->depos_order == 0 && p->derivative_type == 0)
+            throw HpqError(HPQ_E_INVALID, "Analytic derivative with depos_order=0 would vanish");
+        c->p = *p;
+        Geom& g = c->g;
+        g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.ng = 2;
+        g.ldx = g.nx + 2 * g.ng; g.ldy = g.ny + 2 * g.ng; g.plane = long(g.ldx) * g.ldy;
+        g.xlo = p->xlo; g.xhi = p->xhi; g.ylo = p->ylo; g.yhi = p->yhi; g.zlo = p->zlo; g.zhi = p->zhi;
+        g.dx = (g.xhi - g.xlo) / double(g.nx);
+        g.dy = (g.yhi - g.ylo) / double(g.ny);
+        g.dz = (g.zhi - g.zlo) / double(g.nz);
+        g.dxi = 1.0 / g.dx; g.dyi = 1.0 / g.dy; g.dzi = 1.0 / g.dz;
+        g.xoff = 0.5 * (g.xlo + g.xhi - g.dx * double(g.nx - 1));
+        g.yoff = 0.5 * (g.ylo + g.yhi - g.dy * double(g.ny - 1));
+        layout_init(c);
+        HPQ_HIP(hipMalloc(&c->d_count, sizeof(int) * 4));
+        HPQ_HIP(hipHostMalloc(&c->h_count, sizeof(int) * 4, hipHostMallocDefault));
+        HPQ_HIP(hipMalloc(&c->d_red, sizeof(double) * 8));
+        HPQ_HIP(hipHostMalloc(&c->h_red, sizeof(double) * 8, hipHostMallocDefault));
+        HPQ_HIP(hipMemset(c->d_count, 0, sizeof(int) * 4));
+        poisson_init(c);
+        mg_init(c);
+        HPQ_HIP(hipDeviceSynchronize());
+    });
+    if (rc != HPQ_OK) {
+        // keep the context so the caller can read the message; it owns partial state
+        *out = c;
+        return rc;
+    }
+    *out = c;
+    return HPQ_OK;
+}
+
+void hpq_destroy(hpq_ctx* c) {
+    if (!c) return;
+    (void)hipStreamSynchronize(c->stream);
+    poisson_free(c);
+    mg_free(c);
+    if (c->d_count) (void)hipFree(c->d_count);
+    if (c->h_count) (void)hipHostFree(c->h_count);
+    if (c->d_red) (void)hipFree(c->d_red);
+    if (c->h_red) (void)hipHostFree(c->h_red);
+    delete c;
+}
+
+const char* hpq_last_error(const hpq_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int hpq_set_stream(hpq_ctx* c, void* s) {
+    if (!c) return HPQ_E_INVALID;
+    c->stream = static_cast<hipStream_t>(s);
+    return HPQ_OK;
+}
+
+int hpq_layout(const hpq_ctx* c, int32_t* out) {
+    if (!c || !out) return HPQ_E_INVALID;
+    const Layout& L = c->L;
+    const int v[HPQ_LAYOUT_N] = {L.ncomp, L.next_jx_beam, L.next_jy_beam, L.chi, L.Sy, L.Sx, L.ExmBy, L.EypBx,
+                                 L.Ez, L.Bx, L.By, L.Bz, L.Psi, L.jx_beam, L.jy_beam, L.jz_beam, L.jx, L.jy,
+                                 L.rhomjz, L.rho, L.prev_jx_beam, L.prev_jy_beam, L.ions_rhomjz, L.next_jx,
+                                 L.next_jy, L.jz, L.prev_Bx, L.prev_By, L.prev_jx, L.prev_jy, L.pciter_Bx,
+                                 L.pciter_By};
+    for (int k = 0; k < HPQ_LAYOUT_N; ++k) out[k] = v[k];
+    return HPQ_OK;
+}
+
+int64_t hpq_store_doubles(const hpq_ctx* c) { return c ? int64_t(c->L.ncomp) * c->g.plane : -1; }
+
+int hpq_deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32_t* comps, int ions,
+                        int64_t* n_qsa) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_plasma(p);
+        if (!comps) throw HpqError(HPQ_E_INVALID, "comps is null");
+        for (int k = 0; k < 6; ++k) check_comp(c, comps[k]);
+        if (n_qsa) HPQ_HIP(hipMemsetAsync(c->d_count, 0, sizeof(int), c->stream));
+        deposit_current(c, p, store, comps, ions != 0);
+        if (n_qsa) {
+            HPQ_HIP(hipMemcpyAsync(c->h_count, c->d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HPQ_HIP(hipStreamSynchronize(c->stream));
+            *n_qsa = c->h_count[0];
+        }
+    });
+}
+
+int hpq_explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_plasma(p);
+        if (!c->p.explicit_solver) throw HpqError(HPQ_E_UNSUPPORTED, "explicit deposition needs the explicit solver");
+        explicit_deposition(c, p, store);
+    });
+}
+
+int hpq_deposit_current_slice(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const int32_t* comps) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        if (!comps) throw HpqError(HPQ_E_INVALID, "comps is null");
+        for (int k = 0; k < 3; ++k) check_comp(c, comps[k]);
+        deposit_beam(c, b, slice, store, comps);
+    });
+}
+
+int hpq_advance_plasma_particles(hpq_ctx* c, const hpq_plasma* p, const double* store, int temp_slice) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_plasma(p);
+        advance_plasma(c, p, store, temp_slice != 0);
+    });
+}
+
+int hpq_poisson_dirichlet(hpq_ctx* c, const double* staging, double* lhs) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        if (!staging || !lhs) throw HpqError(HPQ_E_INVALID, "null pointer");
+        HPQ_HIP(hipMemcpyAsync(c->ps.bufA, staging, sizeof(double) * c->g.nx * c->g.ny, hipMemcpyDeviceToDevice,
+                               c->stream));
+        double* planes[3] = {lhs, lhs, lhs};
+        poisson_solve_staged(c, 1, planes);
+    });
+}
+
+int hpq_solve_poisson_psi_exmby_eypbx_ez_bz(hpq_ctx* c, double* store) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        Fields(c, store).SolvePoissonPsiExmByEypBxEzBz();
+    });
+}
+
+int hpq_solve_poisson_bxby(hpq_ctx* c, double* store) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        if (c->p.explicit_solver) throw HpqError(HPQ_E_UNSUPPORTED, "SolvePoissonBxBy needs the predictor-corrector layout");
+        Fields(c, store).SolvePoissonBxBy();
+    });
+}
+
+int hpq_initialize_sxsy_with_beam(hpq_ctx* c, double* store) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        if (!c->p.explicit_solver) throw HpqError(HPQ_E_UNSUPPORTED, "Sx/Sy exist only with the explicit solver");
+        sxsy_from_beam(c, store);
+    });
+}
+
+int hpq_mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, int64_t comp_stride, double tol_rel,
+                  double tol_abs, int maxiter, int* iters) {
+    if (!c) return HPQ_E_INVALID;
+    int st = HPQ_OK;
+    const int rc = guarded(c, [&] {
+        if (!sol || !rhs || !acf) throw HpqError(HPQ_E_INVALID, "null pointer");
+        if (comp_stride < c->g.plane) throw HpqError(HPQ_E_INVALID, "comp_stride smaller than a plane");
+        st = hpmg::MultiGrid(c).solve1(sol, rhs, acf, comp_stride, tol_rel, tol_abs, maxiter, iters);
+        if (st == HPQ_E_MG_NOT_CONVERGED) c->err = "hpmg failed";
+        if (st == HPQ_E_MG_DIVERGED) c->err = "hpmg failing so lets stop here";
+    });
+    return rc != HPQ_OK ? rc : st;
+}
+
+int64_t hpq_plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof) {
+    if (!c || !prof) return -1;
+    return plasma_count(c, prof);
+}
+
+int hpq_init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        if (!prof) throw HpqError(HPQ_E_INVALID, "profile is null");
+        check_plasma(p);
+        if (prof->ppc[0] < 1 || prof->ppc[1] < 1) throw HpqError(HPQ_E_INVALID, "ppc must be >= 1");
+        init_plasma(c, prof, p);
+    });
+}
+
+int hpq_begin_step(hpq_ctx* c, const hpq_plasma* p, double* store) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_plasma(p);
+        HPQ_HIP(hipMemsetAsync(store, 0, sizeof(double) * c->L.ncomp * c->g.plane, c->stream));
+        HPQ_HIP(hipMemsetAsync(c->d_count, 0, sizeof(int) * 4, c->stream));
+        for (double& s : c->stats) s = 0;
+        if (c->L.ions_rhomjz >= 0) DepositCurrent(c, p, store, -1, -1, -1, -1, -1, c->L.ions_rhomjz, true);
+    });
+}
+
+int hpq_solve_one_slice(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store, int islice) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_plasma(p);
+        if (islice < 0 || islice >= c->g.nz) throw HpqError(HPQ_E_INVALID, "slice index out of range");
+        Hipace(c, p, b, store).SolveOneSlice(islice);
+    });
+}
+
+int hpq_run_slices(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store, int first, int last) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_plasma(p);
+        if (first >= c->g.nz || last < 0 || first < last) throw HpqError(HPQ_E_INVALID, "bad slice range");
+        Hipace h(c, p, b, store);
+        for (int isl = first; isl >= last; --isl) h.SolveOneSlice(isl);
+    });
+}
+
+int hpq_set_checksums(hpq_ctx* c, double* checksums) {
+    if (!c) return HPQ_E_INVALID;
+    c->checksums = checksums;
+    return HPQ_OK;
+}
+
+int hpq_add_capture(hpq_ctx* c, int comp, double* dst) {
+    if (!c || !dst || comp < 0 || comp >= c->L.ncomp) return HPQ_E_INVALID;
+    c->captures.push_back({comp, dst});
+    return HPQ_OK;
+}
+
+int hpq_clear_captures(hpq_ctx* c) {
+    if (!c) return HPQ_E_INVALID;
+    c->captures.clear();
+    return HPQ_OK;
+}
+
+int hpq_stats(const hpq_ctx* c, double* out4) {
+    if (!c || !out4) return HPQ_E_INVALID;
+    hpq_ctx* mc = const_cast<hpq_ctx*>(c);
+    const int rc = guarded(mc, [&] {
+        HPQ_HIP(hipMemcpyAsync(mc->h_count, mc->d_count, sizeof(int), hipMemcpyDeviceToHost, mc->stream));
+        HPQ_HIP(hipStreamSynchronize(mc->stream));
+    });
+    out4[0] = rc == HPQ_OK ? double(mc->h_count[0]) : -1.0;
+    out4[1] = c->stats[1];
+    out4[2] = c->stats[2];
+    out4[3] = c->stats[3];
+    return rc;
+}
+
+}  // extern "C"

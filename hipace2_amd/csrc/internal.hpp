@@ -1,0 +1,154 @@
+// Internal state of an hpq context: geometry, slice-store layout, DST plans,
+// multigrid hierarchy and scratch, all resident in HBM for the life of the context.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hpq.h"
+
+namespace hpq {
+
+#define HPQ_HIP(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct HipError : std::exception {
+    std::string msg;
+    explicit HipError(std::string m) : msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+struct HpqError : std::exception {
+    int code;
+    std::string msg;
+    HpqError(int c, std::string m) : code(c), msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+
+// Slice geometry. Positions follow GetPosOffset (src/fields/Fields.H:72-77).
+struct Geom {
+    int nx = 0, ny = 0, nz = 0, ng = 2;
+    int ldx = 0, ldy = 0;
+    long plane = 0;
+    double xlo = 0, xhi = 0, ylo = 0, yhi = 0, zlo = 0, zhi = 0;
+    double dx = 0, dy = 0, dz = 0, dxi = 0, dyi = 0, dzi = 0;
+    double xoff = 0, yoff = 0;
+    __host__ __device__ long idx(int i, int j) const { return long(j + ng) * ldx + (i + ng); }
+};
+
+// Component indices of the slice store (src/fields/Fields.cpp:70-165).
+struct Layout {
+    int ncomp = 0;
+    int next_jx_beam = -1, next_jy_beam = -1;
+    int chi = -1, Sy = -1, Sx = -1, ExmBy = -1, EypBx = -1, Ez = -1, Bx = -1, By = -1, Bz = -1,
+        Psi = -1, jx_beam = -1, jy_beam = -1, jz_beam = -1, jx = -1, jy = -1, rhomjz = -1, rho = -1;
+    int prev_jx_beam = -1, prev_jy_beam = -1;
+    int ions_rhomjz = -1;
+    int next_jx = -1, next_jy = -1, jz = -1;
+    int prev_Bx = -1, prev_By = -1, prev_jx = -1, prev_jy = -1;
+    int pciter_Bx = -1, pciter_By = -1, pcprev_Bx = -1, pcprev_By = -1;
+};
+
+// Batched 1-D DST-I (RODFT00) along contiguous rows of length n.
+struct DstPlan {
+    int n = 0;          // transform length
+    int N = 0;          // n + 1
+    int M = 0;          // FFT length 2N (power-of-two path)
+    int logM = 0;
+    bool fft = false;   // power-of-two FFT in LDS, else direct sum with a sine table
+    double2* tw = nullptr;    // M/2 twiddles exp(-2 pi i k / M)
+    double* sintab = nullptr; // 2N entries sin(pi m / N) (direct path)
+};
+
+// Dirichlet Poisson solver on one slice (FFTPoissonSolverDirichlet*).
+struct Poisson {
+    int nx = 0, ny = 0;
+    DstPlan px, py;
+    double* eigT = nullptr;   // [i][j] eigenvalue matrix with the RODFT00 normalisation
+    double* bufA = nullptr;   // 3 * nx * ny
+    double* bufB = nullptr;
+};
+
+// One multigrid level; arrays cover the level box plus one ghost ring.
+struct MGLevel {
+    bool cc = true;
+    int ilo = 0, ihi = 0, jlo = 0, jhi = 0;
+    int vilo = 0, vihi = 0, vjlo = 0, vjhi = 0;
+    int ldx = 0, ldy = 0;
+    long cs = 0;   // component stride
+    double dx = 0, dy = 0;
+    double *acf = nullptr, *res = nullptr, *cor = nullptr, *rescor = nullptr;
+};
+
+struct MultiGrid {
+    bool cc = true;
+    int max_level = 0;
+    int coarse_begin = 0;   // first level handled by the single-workgroup coarse kernel
+    double dx = 0, dy = 0;
+    std::vector<MGLevel> lev;
+    MGLevel* d_lev = nullptr;        // device copy of lev (for the coarse kernel)
+    double* msol = nullptr;          // level-0 solution (2 comps)
+    double* mrhs = nullptr;          // level-0 right-hand side (2 comps)
+    unsigned long long* d_norm = nullptr;  // 2 max-norm accumulators (bit patterns)
+    unsigned long long* h_norm = nullptr;  // pinned host mirror
+};
+
+struct Capture {
+    int comp;
+    double* dst;
+};
+
+}  // namespace hpq
+
+struct hpq_ctx {
+    hpq_params p{};
+    hpq::Geom g;
+    hpq::Layout L;
+    hipStream_t stream = nullptr;
+    hpq::Poisson ps;
+    hpq::MultiGrid mg;
+    int* d_count = nullptr;          // QSA counter
+    int* h_count = nullptr;          // pinned
+    double* d_red = nullptr;         // reduction scratch (8 doubles)
+    double* h_red = nullptr;         // pinned
+    double* checksums = nullptr;     // caller-owned device accumulator (may be null)
+    std::vector<hpq::Capture> captures;
+    double stats[4] = {0, 0, 0, 0};
+    std::string err;
+};
+
+namespace hpq {
+
+// kernels and helpers implemented in the .hip translation units
+void layout_init(hpq_ctx* c);
+void poisson_init(hpq_ctx* c);
+void poisson_free(hpq_ctx* c);
+void poisson_solve_staged(hpq_ctx* c, int nfields, double* const* lhs_planes);
+void build_rhs_psi_ez_bz(hpq_ctx* c, const double* store);
+void build_rhs_bxby(hpq_ctx* c, const double* store);
+void exmby_eypbx(hpq_ctx* c, double* store);
+void mg_init(hpq_ctx* c);
+void mg_free(hpq_ctx* c);
+int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, long comp_stride, double tol_rel,
+              double tol_abs, int maxiter, int* iters);
+void deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32_t* comps, bool ions);
+void explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store);
+void deposit_beam(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const int32_t* comps);
+void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool temp_slice);
+void plane_fill(hpq_ctx* c, double* store, const int* comps, int ncomps, double v);
+void plane_copy(hpq_ctx* c, double* store, const int* dst, const int* src, int ncomps);
+void plane_add(hpq_ctx* c, double* store, int dst, int src);
+void plane_lincomb(hpq_ctx* c, double* store, int dst, double a, int sa, double b, int sb);
+void sxsy_from_beam(hpq_ctx* c, double* store);
+void diag_accumulate(hpq_ctx* c, const double* store, int islice);
+double rel_b_error(hpq_ctx* c, const double* store, int bx, int by, int bxi, int byi);
+void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p);
+int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof);
+
+inline dim3 grid1d(long n, int block) { return dim3(unsigned((n + block - 1) / block)); }
+
+}  // namespace hpq

@@ -1,0 +1,486 @@
+// Particle kernels on gfx950: plasma current/chi deposition, explicit Sx/Sy deposition,
+// beam current deposition, field gather + leapfrog push, plasma injection.
+//
+// One thread per particle over coalesced SoA arrays; stencils unrolled at compile time
+// (shape order and derivative type are template parameters); deposits go to the slice
+// store with hardware FP64 atomics (global_atomic_add_f64, -munsafe-fp-atomics on
+// coarse-grained device memory). Every expression keeps the reference's operation order,
+// so FP64 results equal the reference CPU build's up to the order of the atomic sums.
+#include <cmath>
+
+#include "internal.hpp"
+#include "shape.hpp"
+
+namespace hpq {
+
+namespace {
+
+__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+__device__ __forceinline__ bool in_store(const Geom& g, int i, int j) {
+    return i >= -g.ng && i <= g.nx - 1 + g.ng && j >= -g.ng && j <= g.ny - 1 + g.ng;
+}
+
+// ::DepositCurrent deposit functor, src/particles/deposition/PlasmaDepositCurrent.cpp:182-251
+template <int O>
+__global__ __launch_bounds__(256) void k_deposit_plasma(Geom g, hpq_plasma p, double* __restrict__ store, int cjx,
+                                                        int cjy, int cjz, int crho, int cchi, int crm,
+                                                        double charge_invvol, double charge_mu0_mass_ratio,
+                                                        double max_qsa, int* __restrict__ qsa_count) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= p.n || !p.valid[ip]) return;
+    const double clight = 1.0, clightinv = 1.0 / 1.0;
+    const double psi_inv = 1.0 / p.psi[ip];
+    const double xp = p.x[ip], yp = p.y[ip];
+    const double vx_c = p.ux[ip] * psi_inv;
+    const double vy_c = p.uy[ip] * psi_inv;
+    const double q_invvol = charge_invvol * p.w[ip];
+    const double xmid = (xp - g.xoff) * g.dxi;
+    const double ymid = (yp - g.yoff) * g.dyi;
+    const double Aabssqp = 0.0;
+    const double gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * psi_inv * psi_inv + vx_c * vx_c * clightinv * clightinv +
+                                    vy_c * vy_c * clightinv * clightinv + 1.0);
+    if (gamma_psi < 0.0 || gamma_psi > max_qsa || psi_inv < 0.0) {
+        atomicAdd(qsa_count, 1);
+        p.w[ip] = 0.0;
+        p.valid[ip] = 0;
+        return;
+    }
+    Sf sx[O + 1], sy[O + 1];
+#pragma unroll
+    for (int k = 0; k <= O; ++k) {
+        sx[k] = shape<O>(xmid, k);
+        sy[k] = shape<O>(ymid, k);
+    }
+    double* pjx = store + long(cjx) * g.plane;
+    double* pjy = store + long(cjy) * g.plane;
+    double* pjz = store + long(cjz) * g.plane;
+    double* prho = store + long(crho) * g.plane;
+    double* pchi = store + long(cchi) * g.plane;
+    double* prm = store + long(crm) * g.plane;
+#pragma unroll
+    for (int iy = 0; iy <= O; ++iy) {
+#pragma unroll
+        for (int ix = 0; ix <= O; ++ix) {
+            if (!in_store(g, sx[ix].cell, sy[iy].cell)) continue;
+            const long k = g.idx(sx[ix].cell, sy[iy].cell);
+            const double cd = q_invvol * sx[ix].s * sy[iy].s;
+            if (cjx >= 0) {
+                atomic_add(pjx + k, cd * vx_c);
+                atomic_add(pjy + k, cd * vy_c);
+            }
+            if (cjz >= 0) atomic_add(pjz + k, cd * (gamma_psi - 1.0) * clight);
+            if (crho >= 0) atomic_add(prho + k, cd * gamma_psi);
+            if (cchi >= 0) atomic_add(pchi + k, cd * charge_mu0_mass_ratio * psi_inv);
+            if (crm >= 0) atomic_add(prm + k, cd);
+        }
+    }
+}
+
+// ::ExplicitDeposition deposit functor, src/particles/deposition/ExplicitDeposition.cpp:166-262
+template <int O, int T>
+__global__ __launch_bounds__(256) void k_explicit_deposit(Geom g, hpq_plasma p, double* __restrict__ store, int cBz,
+                                                          int cEz, int cExmBy, int cEypBx, int cSy, int cSx,
+                                                          double charge_invvol_mu0, double charge_mass_ratio) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= p.n || !p.valid[ip]) return;
+    const double clight = 1.0, clight_inv = 1.0 / 1.0;
+    const double psi_inv = 1.0 / p.psi[ip];
+    const double xp = p.x[ip], yp = p.y[ip];
+    const double vx = p.ux[ip] * psi_inv * clight_inv;
+    const double vy = p.uy[ip] * psi_inv * clight_inv;
+    const double q_mass_ratio = charge_mass_ratio;
+    const double charge_density_mu0 = charge_invvol_mu0 * p.w[ip];
+    const double xmid = (xp - g.xoff) * g.dxi;
+    const double ymid = (yp - g.yoff) * g.dyi;
+    const double Aabssqp = 0.0;
+    const double gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * psi_inv * psi_inv + vx * vx + vy * vy + 1.0);
+    constexpr int W = dshape_width<T, O>();
+    Dsf sx[W], sy[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        sx[k] = dshape<T, O>(xmid, k);
+        sy[k] = dshape<T, O>(ymid, k);
+    }
+    const double* Bz = store + long(cBz) * g.plane;
+    const double* Ez = store + long(cEz) * g.plane;
+    const double* ExmBy = store + long(cExmBy) * g.plane;
+    const double* EypBx = store + long(cEypBx) * g.plane;
+    double* Sy = store + long(cSy) * g.plane;
+    double* Sx = store + long(cSx) * g.plane;
+    const double AabssqDxp = 0.0, AabssqDyp = 0.0;
+#pragma unroll
+    for (int iy = 0; iy < W; ++iy) {
+#pragma unroll
+        for (int ix = 0; ix < W; ++ix) {
+            if (T == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) continue;
+            const Dsf X = sx[ix], Y = sy[iy];
+            if (!in_store(g, X.cell, Y.cell)) continue;
+            const long k = g.idx(X.cell, Y.cell);
+            const double Bz_v = Bz[k], Ez_v = Ez[k], ExmBy_v = ExmBy[k], EypBx_v = EypBx[k];
+            atomic_add(Sy + k, charge_density_mu0 * (
+                -X.s * Y.s * (
+                    -Bz_v * vx
+                    + (Ez_v * vy
+                    + ExmBy_v * (-vx * vy)
+                    + EypBx_v * (gamma_psi - vy * vy)) * clight_inv
+                    - 0.25 * AabssqDyp * q_mass_ratio * psi_inv
+                ) * q_mass_ratio * psi_inv
+                + (-X.ds * Y.s * g.dxi * (-vx * vy)
+                   - X.s * Y.ds * g.dyi * (gamma_psi - vy * vy - 1.0)) * clight));
+            atomic_add(Sx + k, charge_density_mu0 * (
+                +X.s * Y.s * (
+                    +Bz_v * vy
+                    + (Ez_v * vx
+                    + ExmBy_v * (gamma_psi - vx * vx)
+                    + EypBx_v * (-vx * vy)) * clight_inv
+                    - 0.25 * AabssqDxp * q_mass_ratio * psi_inv
+                ) * q_mass_ratio * psi_inv
+                + (+X.ds * Y.s * g.dxi * (gamma_psi - vx * vx - 1.0)
+                   + X.s * Y.ds * g.dyi * (-vx * vy)) * clight));
+        }
+    }
+}
+
+// ::DepositCurrentSlice deposit functor, src/particles/deposition/BeamDepositCurrent.cpp:133-193
+template <int O>
+__global__ __launch_bounds__(256) void k_deposit_beam(Geom g, hpq_beam b, long first, long count,
+                                                      double* __restrict__ store, int cjx, int cjy, int cjz) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const long ip = first + t;
+    if (!b.valid[ip]) return;
+    const double clightsq = 1.0 / (1.0 * 1.0);
+    const double ux = b.ux[ip], uy = b.uy[ip], uz = b.uz[ip];
+    const double gaminv = 1.0 / sqrt(1.0 + ux * ux * clightsq + uy * uy * clightsq + uz * uz * clightsq);
+    const double wq = b.charge * b.w[ip] * 1.0;
+    const double wqx = wq * (ux * gaminv), wqy = wq * (uy * gaminv), wqz = wq * (uz * gaminv);
+    const double xmid = (b.x[ip] - g.xoff) * g.dxi;
+    const double ymid = (b.y[ip] - g.yoff) * g.dyi;
+    Sf sx[O + 1], sy[O + 1];
+#pragma unroll
+    for (int k = 0; k <= O; ++k) {
+        sx[k] = shape<O>(xmid, k);
+        sy[k] = shape<O>(ymid, k);
+    }
+#pragma unroll
+    for (int iy = 0; iy <= O; ++iy) {
+#pragma unroll
+        for (int ix = 0; ix <= O; ++ix) {
+            if (!in_store(g, sx[ix].cell, sy[iy].cell)) continue;
+            const long k = g.idx(sx[ix].cell, sy[iy].cell);
+            const double s = sx[ix].s * sy[iy].s;
+            if (cjx >= 0) {
+                atomic_add(store + long(cjx) * g.plane + k, s * wqx);
+                atomic_add(store + long(cjy) * g.plane + k, s * wqy);
+            }
+            if (cjz >= 0) atomic_add(store + long(cjz) * g.plane + k, s * wqz);
+        }
+    }
+}
+
+// Forward-mode dual number for the second-order Taylor step (src/utils/DualNumbers.H:13-43)
+struct Dual {
+    double v, e;
+    __device__ Dual(double a) : v(a), e(0) {}
+    __device__ Dual(double a, double b) : v(a), e(b) {}
+};
+__device__ __forceinline__ Dual operator+(const Dual& a, const Dual& b) { return {a.v + b.v, a.e + b.e}; }
+__device__ __forceinline__ Dual operator-(const Dual& a, const Dual& b) { return {a.v - b.v, a.e - b.e}; }
+__device__ __forceinline__ Dual operator*(const Dual& a, const Dual& b) { return {a.v * b.v, a.e * b.v + a.v * b.e}; }
+
+// PlasmaMomentumPush, src/particles/pusher/PushPlasmaParticles.H:37-75 (no laser terms)
+template <class T>
+__device__ __forceinline__ void momentum_rhs(const T& ux, const T& uy, const T& psi_inv, double ExmBy, double EypBx,
+                                             double Ez, double Bx_c, double By_c, double Bz, double ci, double qmc,
+                                             T& dux, T& duy, T& dpsi) {
+    const double A = 0.0, ADx = 0.0, ADy = 0.0;
+    const T gamma_psi =
+        T(0.5) * psi_inv * psi_inv * (T(1.0) + T(A) + ux * ux * T(ci * ci) + uy * uy * T(ci * ci)) + T(0.5);
+    dux = (T(qmc) * (gamma_psi * T(ExmBy) + T(By_c) + (uy * T(Bz)) * psi_inv) - T(ADx) * psi_inv);
+    duy = (T(qmc) * (gamma_psi * T(EypBx) - T(Bx_c) - (ux * T(Bz)) * psi_inv) - T(ADy) * psi_inv);
+    dpsi = (T(qmc) * T(ci) * ((ux * T(ExmBy) + uy * T(EypBx)) * T(ci) * psi_inv - T(Ez)));
+}
+
+struct BC {
+    int type;
+    double plo0, plo1, phi0, phi1;
+};
+
+// EnforceBC, src/particles/pusher/GetAndSetPosition.H:29-99; returns true if removed
+__device__ __forceinline__ bool enforce_bc(const BC& bc, double& x, double& y, double& ux, double& uy) {
+    if (x < bc.plo0 || y < bc.plo1 || x > bc.phi0 || y > bc.phi1) {
+        const double lx = bc.phi0 - bc.plo0, ly = bc.phi1 - bc.plo1;
+        if (bc.type == HPQ_BC_REFLECTING) {
+            x = fmod(x - bc.plo0, 2 * lx); if (x < 0) x += 2 * lx; x += bc.plo0;
+            if (x > bc.phi0) { x = 2 * bc.phi0 - x; ux = -ux; }
+            y = fmod(y - bc.plo1, 2 * ly); if (y < 0) y += 2 * ly; y += bc.plo1;
+            if (y > bc.phi1) { y = 2 * bc.phi1 - y; uy = -uy; }
+            return false;
+        } else if (bc.type == HPQ_BC_PERIODIC) {
+            x = fmod(x - bc.plo0, lx); if (x < 0) x += lx; x += bc.plo0;
+            y = fmod(y - bc.plo1, ly); if (y < 0) y += ly; y += bc.plo1;
+            return false;
+        }
+        return true;
+    }
+    return false;
+}
+
+// AdvancePlasmaParticles, src/particles/pusher/PlasmaParticleAdvance.cpp:84-240 (leapfrog)
+template <int O>
+__global__ __launch_bounds__(256) void k_push_plasma(Geom g, hpq_plasma p, const double* __restrict__ store, int cPsi,
+                                                     int cEz, int cBx, int cBy, int cBz, double qmc, BC bc,
+                                                     int temp_slice) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= p.n || !p.valid[ip]) return;
+    const double clight = 1.0, clight_inv = 1.0 / 1.0;
+    const double dz = g.dz / 1;
+    double xp = p.x_prev[ip], yp = p.y_prev[ip];
+    double ExmByp = 0, EypBxp = 0, Ezp = 0, Bxp = 0, Byp = 0, Bzp = 0;
+    {
+        const double xm = (xp - g.xoff) * g.dxi, ym = (yp - g.yoff) * g.dyi;
+        const double* Psi = store + long(cPsi) * g.plane;
+        const double* Ez = store + long(cEz) * g.plane;
+        const double* Bx = store + long(cBx) * g.plane;
+        const double* By = store + long(cBy) * g.plane;
+        const double* Bz = store + long(cBz) * g.plane;
+        constexpr int W = O + 2;
+        Dsf sx[W], sy[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            sx[k] = dshape<1, O>(xm, k);
+            sy[k] = dshape<1, O>(ym, k);
+        }
+#pragma unroll
+        for (int iy = 0; iy < W; ++iy) {
+#pragma unroll
+            for (int ix = 0; ix < W; ++ix) {
+                const Dsf X = sx[ix], Y = sy[iy];
+                if (!in_store(g, X.cell, Y.cell)) continue;
+                const long k = g.idx(X.cell, Y.cell);
+                const double ps = Psi[k];
+                ExmByp += (X.ds * Y.s) * ps * g.dxi;
+                EypBxp += (X.s * Y.ds) * ps * g.dyi;
+                Ezp += (X.s * Y.s) * Ez[k];
+                Bxp += (X.s * Y.s) * Bx[k];
+                Byp += (X.s * Y.s) * By[k];
+                Bzp += (X.s * Y.s) * Bz[k];
+            }
+        }
+    }
+    Bxp *= clight;
+    Byp *= clight;
+    constexpr int nsub = 4;
+    const double sdz = dz / nsub;
+    double ux = p.ux_half_step[ip], uy = p.uy_half_step[ip], psi = p.psi_half_step[ip];
+    auto substep = [&]() {
+        const double psi_inv = 1.0 / psi;
+        double dux, duy, dpsi;
+        momentum_rhs<double>(ux, uy, psi_inv, ExmByp, EypBxp, Ezp, Bxp, Byp, Bzp, clight_inv, qmc, dux, duy, dpsi);
+        const Dual uxd{ux, dux}, uyd{uy, duy}, pid{psi_inv, -psi_inv * psi_inv * dpsi};
+        Dual ddux(0.0), dduy(0.0), ddpsi(0.0);
+        momentum_rhs<Dual>(uxd, uyd, pid, ExmByp, EypBxp, Ezp, Bxp, Byp, Bzp, clight_inv, qmc, ddux, dduy, ddpsi);
+        ux += sdz * dux + 0.5 * sdz * sdz * ddux.e;
+        uy += sdz * duy + 0.5 * sdz * sdz * dduy.e;
+        psi += sdz * dpsi + 0.5 * sdz * sdz * ddpsi.e;
+    };
+#pragma unroll 1
+    for (int s = 0; s < nsub; ++s) substep();
+    xp += dz * clight_inv * (ux * (1.0 / psi));
+    yp += dz * clight_inv * (uy * (1.0 / psi));
+    if (enforce_bc(bc, xp, yp, ux, uy)) {
+        p.w[ip] = 0.0;
+        p.valid[ip] = 0;
+        return;
+    }
+    p.x[ip] = xp;
+    p.y[ip] = yp;
+    if (!temp_slice) {
+        p.ux_half_step[ip] = ux;
+        p.uy_half_step[ip] = uy;
+        p.psi_half_step[ip] = psi;
+        p.x_prev[ip] = xp;
+        p.y_prev[ip] = yp;
+    }
+#pragma unroll 1
+    for (int s = 0; s < nsub / 2; ++s) substep();
+    p.ux[ip] = ux;
+    p.uy[ip] = uy;
+    p.psi[ip] = psi;
+}
+
+template <class F>
+void dispatch_order(int order, F&& f) {
+    switch (order) {
+        case 0: f(std::integral_constant<int, 0>{}); break;
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        default: throw HpqError(HPQ_E_UNSUPPORTED, "depos_order_xy must be 0..3");
+    }
+}
+
+}  // namespace
+
+void deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32_t* comps, bool ions) {
+    if (p->n <= 0) return;
+    const Geom& g = c->g;
+    const double charge = ions ? -c->p.plasma_charge : c->p.plasma_charge;
+    const double mu0 = 1.0;
+    const double invvol = g.dx * g.dy * g.dxi * g.dyi;
+    const double charge_invvol = charge * invvol;
+    const double cmr = charge * mu0 / c->p.plasma_mass;
+    const int zero = 0;
+    (void)zero;
+    auto cm = [&](int k) { return comps[k] >= 0 ? comps[k] : -1; };
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        hipLaunchKernelGGL((k_deposit_plasma<decltype(O)::value>), grid1d(p->n, 256), dim3(256), 0, c->stream, g, *p,
+                           store, cm(0), cm(1), cm(2), cm(3), cm(4), cm(5), charge_invvol, cmr, c->p.max_qsa,
+                           c->d_count);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+void explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store) {
+    if (p->n <= 0) return;
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    const double mu0 = 1.0;
+    const double invvol = g.dx * g.dy * g.dxi * g.dyi;
+    const double qimu0 = c->p.plasma_charge * invvol * mu0;
+    const double qmr = c->p.plasma_charge / c->p.plasma_mass;
+    const int T = c->p.derivative_type;
+    if (T < 0 || T > 2) throw HpqError(HPQ_E_UNSUPPORTED, "depos_derivative_type must be 0..2");
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        constexpr int o = decltype(O)::value;
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid1d(p->n, 256), dim3(256), 0, c->stream, g, *p, store, L.Bz, L.Ez, L.ExmBy,
+                               L.EypBx, L.Sy, L.Sx, qimu0, qmr);
+        };
+        if (T == 0) launch(k_explicit_deposit<o, 0>);
+        else if (T == 1) launch(k_explicit_deposit<o, 1>);
+        else launch(k_explicit_deposit<o, 2>);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+void deposit_beam(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const int32_t* comps) {
+    if (!b || slice < 0 || slice >= c->g.nz || !b->off) return;
+    const long first = b->off[slice];
+    const long count = b->off[slice + 1] - first;
+    if (count <= 0) return;
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        hipLaunchKernelGGL((k_deposit_beam<decltype(O)::value>), grid1d(count, 256), dim3(256), 0, c->stream, c->g, *b,
+                           first, count, store, comps[0], comps[1], comps[2]);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool temp_slice) {
+    if (p->n <= 0) return;
+    const Layout& L = c->L;
+    const double qmc = c->p.plasma_charge / (c->p.plasma_mass * 1.0);
+    BC bc{c->p.particle_bc, c->p.plo[0], c->p.plo[1], c->p.phi[0], c->p.phi[1]};
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        hipLaunchKernelGGL((k_push_plasma<decltype(O)::value>), grid1d(p->n, 256), dim3(256), 0, c->stream, c->g, *p,
+                           store, L.Psi, L.Ez, L.Bx, L.By, L.Bz, qmc, bc, int(temp_slice));
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------
+// plasma injection, PlasmaParticleContainer::InitParticles
+// (src/particles/plasma/PlasmaParticleContainerInit.cpp:18-378): lattice positions
+// (0.5 + k)/ppc per cell, weight n(x,y)/ppc, order i_part outer and cells inner.
+// ---------------------------------------------------------------------------------
+namespace {
+
+struct Prof {
+    int ppc0, ppc1;
+    double density, ramp_a, radius;
+    double u0, u1, u2;
+    double blo0, blo1, bhi0, bhi1;
+};
+
+__host__ __device__ __forceinline__ bool inject(const Geom& g, const Prof& pr, int i, int j, int ipart, double& x,
+                                                double& y, double& w) {
+    const int ixp = ipart % pr.ppc0, iyp = ipart / pr.ppc0;
+    const double r0 = (0.5 + ixp) / pr.ppc0, r1 = (0.5 + iyp) / pr.ppc1;
+    x = g.xlo + (i + r0 + 0.0) * g.dx;
+    y = g.ylo + (j + r1 + 0.0) * g.dy;
+    const double rsq = x * x + y * y;
+    const double dens = pr.density * (1.0 + pr.ramp_a * rsq);
+    if (x >= pr.bhi0 || x < pr.blo0 || y >= pr.bhi1 || y < pr.blo1 || rsq > pr.radius * pr.radius || dens <= 0.0)
+        return false;
+    w = dens * (1.0 / (pr.ppc0 * pr.ppc1));
+    return true;
+}
+
+// one thread per cell of one i_part; slot = offset of the cell among injected cells
+__global__ void k_init_plasma(Geom g, Prof pr, int ipart, const int* __restrict__ slot, long base, hpq_plasma p) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nx * g.ny) return;
+    const int j = t / g.nx, i = t - j * g.nx;
+    double x, y, w;
+    if (!inject(g, pr, i, j, ipart, x, y, w)) return;
+    const long k = base + slot[t];
+    const double psi0 = sqrt(1. + pr.u0 * pr.u0 + pr.u1 * pr.u1 + pr.u2 * pr.u2) - pr.u2;
+    p.x[k] = x; p.y[k] = y; p.w[k] = w;
+    p.ux[k] = pr.u0 * 1.0; p.uy[k] = pr.u1 * 1.0; p.psi[k] = psi0;
+    p.x_prev[k] = x; p.y_prev[k] = y;
+    p.ux_half_step[k] = pr.u0 * 1.0; p.uy_half_step[k] = pr.u1 * 1.0; p.psi_half_step[k] = psi0;
+    p.valid[k] = 1;
+}
+
+Prof make_prof(const hpq_ctx* c, const hpq_plasma_profile* prof) {
+    Prof pr;
+    pr.ppc0 = prof->ppc[0]; pr.ppc1 = prof->ppc[1];
+    pr.density = prof->density; pr.ramp_a = prof->ramp_a; pr.radius = prof->radius;
+    pr.u0 = prof->u_mean[0]; pr.u1 = prof->u_mean[1]; pr.u2 = prof->u_mean[2];
+    pr.blo0 = c->p.plo[0]; pr.blo1 = c->p.plo[1]; pr.bhi0 = c->p.phi[0]; pr.bhi1 = c->p.phi[1];
+    return pr;
+}
+
+}  // namespace
+
+int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof) {
+    const Prof pr = make_prof(c, prof);
+    int64_t n = 0;
+    double x, y, w;
+    for (int ip = 0; ip < pr.ppc0 * pr.ppc1; ++ip)
+        for (int j = 0; j < c->g.ny; ++j)
+            for (int i = 0; i < c->g.nx; ++i) n += inject(c->g, pr, i, j, ip, x, y, w) ? 1 : 0;
+    return n;
+}
+
+void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p) {
+    const Prof pr = make_prof(c, prof);
+    const Geom& g = c->g;
+    if (plasma_count(c, prof) != p->n)
+        throw HpqError(HPQ_E_INVALID, "hpq_init_plasma: plasma capacity does not match the profile");
+    // slots computed on the host (exclusive scan of the injection mask, x fastest)
+    std::vector<int> slot(size_t(g.nx) * g.ny);
+    int* d_slot = nullptr;
+    HPQ_HIP(hipMalloc(&d_slot, sizeof(int) * slot.size()));
+    long base = 0;
+    for (int ip = 0; ip < pr.ppc0 * pr.ppc1; ++ip) {
+        int cnt = 0;
+        double x, y, w;
+        for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i) {
+                slot[size_t(j) * g.nx + i] = cnt;
+                cnt += inject(g, pr, i, j, ip, x, y, w) ? 1 : 0;
+            }
+        HPQ_HIP(hipMemcpy(d_slot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_init_plasma, grid1d(long(g.nx) * g.ny, 256), dim3(256), 0, c->stream, g, pr, ip, d_slot,
+                           base, *p);
+        HPQ_HIP(hipGetLastError());
+        HPQ_HIP(hipStreamSynchronize(c->stream));
+        base += cnt;
+    }
+    HPQ_HIP(hipFree(d_slot));
+    if (base != p->n) throw HpqError(HPQ_E_INVALID, "hpq_init_plasma: plasma capacity does not match the profile");
+}
+
+}  // namespace hpq

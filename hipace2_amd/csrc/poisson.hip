@@ -1,0 +1,304 @@
+// Transverse Dirichlet Poisson solver on gfx950: batched DST-I along rows in LDS,
+// LDS-tiled transposes between the x and y passes, eigenvalue scaling fused into the
+// middle pass. Solves up to three fields (Psi, Ez, Bz) per call so each launch has
+// 3 x n rows of work.
+//
+// Semantics follow FFTPoissonSolverDirichletDirect (src/fields/fft_poisson_solver/
+// FFTPoissonSolverDirichletDirect.cpp:43-139): forward 2-D DST-I (FFTW RODFT00
+// normalisation), multiply by the eigenvalue matrix (:60-82), inverse 2-D DST-I,
+// copy the valid region into the destination plane.
+//
+// Row DST, power-of-two n+1: two real rows a, b are packed as a + i b, odd-extended to
+// length M = 2(n+1) and transformed with a radix-2 FFT held in LDS (M <= 4096,
+// 64 KiB). With F = FFT(a + i b), RODFT00(a)_{k-1} = -Im F_k and RODFT00(b)_{k-1} = Re F_k.
+// Other n: direct sum with an exactly reduced sine table, one row per workgroup.
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace hpq {
+
+struct Rows {
+    double* base[3];
+    long stride;
+    int rows_per_field;
+};
+
+__device__ __forceinline__ double* row_ptr(const Rows& R, int r) {
+    const int f = r / R.rows_per_field;
+    const int j = r - f * R.rows_per_field;
+    return R.base[f] + long(j) * R.stride;
+}
+
+__global__ __launch_bounds__(256) void k_dst_rows_fft(Rows in, Rows out, int nrows, int n, int M, int logM,
+                                                      const double2* __restrict__ tw,
+                                                      const double* __restrict__ scale) {
+    extern __shared__ double2 a[];
+    const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
+    const bool has1 = r1 < nrows;
+    const double* p0 = row_ptr(in, r0);
+    const double* p1 = has1 ? row_ptr(in, r1) : p0;
+    const int N = M >> 1;
+    for (int m = threadIdx.x; m < M; m += blockDim.x) {
+        double va = 0.0, vb = 0.0;
+        if (m >= 1 && m <= n) {
+            va = p0[m - 1];
+            vb = has1 ? p1[m - 1] : 0.0;
+        } else if (m > N) {
+            const int q = M - m;
+            va = -p0[q - 1];
+            vb = has1 ? -p1[q - 1] : 0.0;
+        }
+        a[__brev(unsigned(m)) >> (32 - logM)] = make_double2(va, vb);
+    }
+    __syncthreads();
+    for (int s = 1; s <= logM; ++s) {
+        const int half = 1 << (s - 1);
+        const int step = M >> s;
+        for (int b = threadIdx.x; b < (M >> 1); b += blockDim.x) {
+            const int k = b & (half - 1);
+            const int i0 = ((b >> (s - 1)) << s) + k;
+            const int i1 = i0 + half;
+            const double2 w = tw[k * step];
+            const double2 u = a[i0];
+            const double2 x = a[i1];
+            const double2 v = make_double2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+            a[i0] = make_double2(u.x + v.x, u.y + v.y);
+            a[i1] = make_double2(u.x - v.x, u.y - v.y);
+        }
+        __syncthreads();
+    }
+    double* q0 = row_ptr(out, r0);
+    double* q1 = has1 ? row_ptr(out, r1) : nullptr;
+    const double* s0 = scale ? scale + long(r0 % in.rows_per_field) * n : nullptr;
+    const double* s1 = (scale && has1) ? scale + long(r1 % in.rows_per_field) * n : nullptr;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const double2 F = a[k + 1];
+        double o0 = -F.y, o1 = F.x;
+        if (s0) o0 *= s0[k];
+        if (s1) o1 *= s1[k];
+        q0[k] = o0;
+        if (has1) q1[k] = o1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dst_rows_direct(Rows in, Rows out, int n, const double* __restrict__ tab,
+                                                         const double* __restrict__ scale) {
+    extern __shared__ double row[];
+    const int r = blockIdx.x;
+    const double* p = row_ptr(in, r);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) row[j] = p[j];
+    __syncthreads();
+    const int N2 = 2 * (n + 1);
+    double* q = row_ptr(out, r);
+    const double* sc = scale ? scale + long(r % in.rows_per_field) * n : nullptr;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        double s = 0.0;
+        int idx = k + 1;
+        for (int j = 0; j < n; ++j) {
+            s += row[j] * tab[idx];
+            idx += k + 1;
+            if (idx >= N2) idx -= N2;
+        }
+        double o = 2.0 * s;
+        if (sc) o *= sc[k];
+        q[k] = o;
+    }
+}
+
+// out[f][c][r] = in[f][r][c] for f in [0, nf), in: R x C rows
+__global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ in, double* __restrict__ out, int R,
+                                                   int C) {
+    __shared__ double t[32][33];
+    const long fo = long(blockIdx.z) * R * C;
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < R && c < C) t[k][threadIdx.x] = in[fo + long(r) * C + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + threadIdx.x;
+        if (r < R && c < C) out[fo + long(c) * R + r] = t[threadIdx.x][k];
+    }
+}
+
+static void dst_rows(hpq_ctx* c, const DstPlan& P, const Rows& in, const Rows& out, int nrows, const double* scale) {
+    if (P.fft) {
+        const size_t lds = size_t(P.M) * sizeof(double2);
+        hipLaunchKernelGGL(k_dst_rows_fft, dim3((nrows + 1) / 2), dim3(256), lds, c->stream, in, out, nrows, P.n, P.M,
+                           P.logM, P.tw, scale);
+    } else {
+        const size_t lds = size_t(P.n) * sizeof(double);
+        hipLaunchKernelGGL(k_dst_rows_direct, dim3(nrows), dim3(256), lds, c->stream, in, out, P.n, P.sintab, scale);
+    }
+    HPQ_HIP(hipGetLastError());
+}
+
+static double sin_pi_frac(long m, long N) {
+    m %= 2 * N;
+    if (m < 0) m += 2 * N;
+    double sgn = 1.0;
+    if (m >= N) { m -= N; sgn = -1.0; }
+    if (2 * m > N) m = N - m;
+    return sgn * std::sin(M_PI * double(m) / double(N));
+}
+
+static void plan_init(DstPlan& P, int n) {
+    P.n = n;
+    P.N = n + 1;
+    P.M = 2 * P.N;
+    P.fft = (P.N & (P.N - 1)) == 0 && P.M <= 4096 && P.N >= 2;
+    if (P.fft) {
+        P.logM = 0;
+        while ((1 << P.logM) < P.M) ++P.logM;
+        std::vector<double2> tw(P.M / 2);
+        for (int k = 0; k < P.M / 2; ++k)
+            tw[k] = make_double2(sin_pi_frac(k + P.N / 2, P.N), -sin_pi_frac(k, P.N));
+        HPQ_HIP(hipMalloc(&P.tw, sizeof(double2) * tw.size()));
+        HPQ_HIP(hipMemcpy(P.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> tab(2 * P.N);
+        for (int m = 0; m < 2 * P.N; ++m) tab[m] = sin_pi_frac(m, P.N);
+        HPQ_HIP(hipMalloc(&P.sintab, sizeof(double) * tab.size()));
+        HPQ_HIP(hipMemcpy(P.sintab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    }
+}
+
+void poisson_init(hpq_ctx* c) {
+    Poisson& S = c->ps;
+    const int nx = c->g.nx, ny = c->g.ny;
+    S.nx = nx;
+    S.ny = ny;
+    plan_init(S.px, nx);
+    plan_init(S.py, ny);
+    // eigenvalue matrix (FFTPoissonSolverDirichletDirect.cpp:60-82), stored [i][j]
+    std::vector<double> ev(size_t(nx) * ny);
+    const double dxsq = c->g.dx * c->g.dx, dysq = c->g.dy * c->g.dy;
+    const double sxf = M_PI / (2. * (nx + 1)), syf = M_PI / (2. * (ny + 1));
+    const double norm_fac = 0.5 / (2 * ((nx + 1) * (ny + 1)));
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            const double sx = std::sin((i + 1) * sxf) * std::sin((i + 1) * sxf);
+            const double sy = std::sin((j + 1) * syf) * std::sin((j + 1) * syf);
+            double e = 0.0;
+            if (sx != 0 && sy != 0) e = norm_fac / (-4.0 * (sx / dxsq + sy / dysq));
+            ev[size_t(i) * ny + j] = e;
+        }
+    HPQ_HIP(hipMalloc(&S.eigT, sizeof(double) * ev.size()));
+    HPQ_HIP(hipMemcpy(S.eigT, ev.data(), sizeof(double) * ev.size(), hipMemcpyHostToDevice));
+    HPQ_HIP(hipMalloc(&S.bufA, sizeof(double) * 3 * size_t(nx) * ny));
+    HPQ_HIP(hipMalloc(&S.bufB, sizeof(double) * 3 * size_t(nx) * ny));
+}
+
+void poisson_free(hpq_ctx* c) {
+    Poisson& S = c->ps;
+    for (void* p : {(void*)S.px.tw, (void*)S.px.sintab, (void*)S.py.tw, (void*)S.py.sintab, (void*)S.eigT,
+                    (void*)S.bufA, (void*)S.bufB})
+        if (p) (void)hipFree(p);
+    S = Poisson{};
+}
+
+// Solves nfields staged right-hand sides held in ps.bufA as [f][j][i] and writes the
+// valid region of field f into lhs_planes[f] (guarded planes).
+void poisson_solve_staged(hpq_ctx* c, int nf, double* const* lhs_planes) {
+    Poisson& S = c->ps;
+    const int nx = S.nx, ny = S.ny;
+    const long fs = long(nx) * ny;
+    auto rows = [&](double* base, long stride, int rpf) {
+        Rows R;
+        for (int f = 0; f < 3; ++f) R.base[f] = base + f * fs;
+        R.stride = stride;
+        R.rows_per_field = rpf;
+        return R;
+    };
+    // x pass on rows j
+    dst_rows(c, S.px, rows(S.bufA, nx, ny), rows(S.bufB, nx, ny), nf * ny, nullptr);
+    hipLaunchKernelGGL(k_transpose, dim3((nx + 31) / 32, (ny + 31) / 32, nf), dim3(32, 8), 0, c->stream, S.bufB,
+                       S.bufA, ny, nx);
+    // y pass, eigenvalue scaling, inverse y pass on rows i
+    dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufB, ny, nx), nf * nx, S.eigT);
+    dst_rows(c, S.py, rows(S.bufB, ny, nx), rows(S.bufA, ny, nx), nf * nx, nullptr);
+    hipLaunchKernelGGL(k_transpose, dim3((ny + 31) / 32, (nx + 31) / 32, nf), dim3(32, 8), 0, c->stream, S.bufA,
+                       S.bufB, nx, ny);
+    // inverse x pass straight into the destination planes
+    Rows out;
+    for (int f = 0; f < 3; ++f) out.base[f] = lhs_planes[f < nf ? f : 0] + c->g.idx(0, 0);
+    out.stride = c->g.ldx;
+    out.rows_per_field = ny;
+    dst_rows(c, S.px, rows(S.bufB, nx, ny), out, nf * ny, nullptr);
+}
+
+// Right-hand sides of Fields::SolvePoissonPsiExmByEypBxEzBz (src/fields/Fields.cpp:886-917):
+// Psi: -rhomjz/ep0; Ez: (d_x jx + d_y jy)/(ep0 c); Bz: mu0 (d_y jx - d_x jy), centred
+// differences (derivative<>, :223-277) that read the guard cells of jx, jy.
+__global__ void k_rhs_psi_ez_bz(Geom g, const double* __restrict__ rhomjz, const double* __restrict__ jx,
+                                const double* __restrict__ jy, double* __restrict__ st) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= g.nx) return;
+    const double ep0 = 1.0, mu0 = 1.0, cl = 1.0;
+    const double dxih = 0.5 * g.dxi, dyih = 0.5 * g.dyi;
+    const long fs = long(g.nx) * g.ny;
+    const long o = long(j) * g.nx + i;
+    st[o] = (-1.0 / ep0) * rhomjz[g.idx(i, j)];
+    const double fe = 1.0 / (ep0 * cl);
+    st[fs + o] = fe * ((jx[g.idx(i + 1, j)] - jx[g.idx(i - 1, j)]) * dxih) +
+                 fe * ((jy[g.idx(i, j + 1)] - jy[g.idx(i, j - 1)]) * dyih);
+    st[2 * fs + o] = mu0 * ((jx[g.idx(i, j + 1)] - jx[g.idx(i, j - 1)]) * dyih) +
+                     (-mu0) * ((jy[g.idx(i + 1, j)] - jy[g.idx(i - 1, j)]) * dxih);
+}
+
+void build_rhs_psi_ez_bz(hpq_ctx* c, const double* store) {
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    hipLaunchKernelGGL(k_rhs_psi_ez_bz, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g,
+                       store + L.rhomjz * g.plane, store + L.jx * g.plane, store + L.jy * g.plane, c->ps.bufA);
+    HPQ_HIP(hipGetLastError());
+}
+
+// Right-hand sides of Fields::SolvePoissonBxBy (src/fields/Fields.cpp:1044-1068).
+__global__ void k_rhs_bxby(Geom g, const double* __restrict__ jz, const double* __restrict__ pjx,
+                           const double* __restrict__ pjy, const double* __restrict__ njx,
+                           const double* __restrict__ njy, double* __restrict__ st) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= g.nx) return;
+    const double mu0 = 1.0;
+    const double dxih = 0.5 * g.dxi, dyih = 0.5 * g.dyi, dzih = 0.5 * g.dzi;
+    const long fs = long(g.nx) * g.ny;
+    const long o = long(j) * g.nx + i;
+    const long k = g.idx(i, j);
+    st[o] = -mu0 * ((jz[g.idx(i, j + 1)] - jz[g.idx(i, j - 1)]) * dyih) + mu0 * ((pjy[k] - njy[k]) * dzih);
+    st[fs + o] = mu0 * ((jz[g.idx(i + 1, j)] - jz[g.idx(i - 1, j)]) * dxih) + (-mu0) * ((pjx[k] - njx[k]) * dzih);
+}
+
+void build_rhs_bxby(hpq_ctx* c, const double* store) {
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    hipLaunchKernelGGL(k_rhs_bxby, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g, store + L.jz * g.plane,
+                       store + L.prev_jx * g.plane, store + L.prev_jy * g.plane, store + L.next_jx * g.plane,
+                       store + L.next_jy * g.plane, c->ps.bufA);
+    HPQ_HIP(hipGetLastError());
+}
+
+// ExmBy = -d_x Psi, EypBx = -d_y Psi on the valid box plus one guard ring (Fields.cpp:931-956)
+__global__ void k_exmby_eypbx(Geom g, const double* __restrict__ psi, double* __restrict__ exmby,
+                              double* __restrict__ eypbx) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x) - 1;
+    const int j = int(blockIdx.y) - 1;
+    if (i > g.nx) return;
+    const double dxih = 0.5 * g.dxi, dyih = 0.5 * g.dyi;
+    exmby[g.idx(i, j)] = -(psi[g.idx(i + 1, j)] - psi[g.idx(i - 1, j)]) * dxih;
+    eypbx[g.idx(i, j)] = -(psi[g.idx(i, j + 1)] - psi[g.idx(i, j - 1)]) * dyih;
+}
+
+void exmby_eypbx(hpq_ctx* c, double* store) {
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    hipLaunchKernelGGL(k_exmby_eypbx, dim3((g.nx + 2 + 127) / 128, g.ny + 2), dim3(128), 0, c->stream, g,
+                       store + L.Psi * g.plane, store + L.ExmBy * g.plane, store + L.EypBx * g.plane);
+    HPQ_HIP(hipGetLastError());
+}
+
+}  // namespace hpq

@@ -1,0 +1,209 @@
+"""Host driver of the slice loop over the C ABI.
+
+`Simulation` owns the device buffers of one run (slice store, plasma SoA, beam arrays,
+diagnostics), allocated through torch on the GPU; every computation is a call into
+libhpq.so on torch's current HIP stream. It mirrors the reference's Hipace object for
+this path: InitData (plasma and beam), Evolve's per-step reset and neutralising
+background, SolveOneSlice over the ζ-slices from head to tail.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+
+from . import _abi
+from .inputs import BC_CODES, BeamArrays, SliceConfig
+
+PLASMA_REALS = ["x", "y", "w", "ux", "uy", "psi", "x_prev", "y_prev", "ux_half_step", "uy_half_step",
+                "psi_half_step"]
+
+
+def make_params(cfg: SliceConfig) -> _abi.Params:
+    (plx, ply), (phx, phy) = cfg.bounds
+    p = _abi.Params()
+    p.nx, p.ny, p.nz = cfg.nx, cfg.ny, cfg.nz
+    p.xlo, p.xhi, p.ylo, p.yhi, p.zlo, p.zhi = cfg.xlo, cfg.xhi, cfg.ylo, cfg.yhi, cfg.zlo, cfg.zhi
+    p.depos_order, p.derivative_type = cfg.depos_order, cfg.derivative_type
+    p.explicit_solver, p.deposit_rho = int(cfg.explicit_solver), int(cfg.deposit_rho)
+    p.do_beam_jx_jy = int(cfg.do_beam_jx_jy)
+    p.neutralize = int(cfg.neutralize and cfg.has_plasma)
+    p.mg_tol_rel, p.mg_tol_abs, p.mg_maxiter = cfg.mg_tol_rel, cfg.mg_tol_abs, cfg.mg_maxiter
+    p.particle_bc = BC_CODES[cfg.particle_bc]
+    p.plo[0], p.plo[1], p.phi[0], p.phi[1] = plx, ply, phx, phy
+    p.pc_tol, p.pc_maxiter, p.pc_mix = cfg.pc_tol, cfg.pc_maxiter, cfg.pc_mix
+    p.plasma_charge, p.plasma_mass, p.max_qsa = cfg.plasma_charge, cfg.plasma_mass, cfg.max_qsa
+    return p
+
+
+def make_profile(cfg: SliceConfig) -> _abi.PlasmaProfile:
+    pr = _abi.PlasmaProfile()
+    pr.ppc[0], pr.ppc[1] = cfg.ppc
+    pr.density, pr.ramp_a = cfg.density, cfg.ramp_a
+    pr.radius = cfg.plasma_radius if math.isfinite(cfg.plasma_radius) else 1e300
+    for k in range(3):
+        pr.u_mean[k] = cfg.u_mean[k]
+    return pr
+
+
+class Simulation:
+    def __init__(self, cfg: SliceConfig, beam: Optional[BeamArrays] = None, device: str = "cuda:0"):
+        if not torch.cuda.is_available():
+            raise RuntimeError("hipace2_amd needs a GPU (HIP device)")
+        self.cfg = cfg
+        self.device = torch.device(device)
+        self.L = _abi.lib()
+        self._params = make_params(cfg)
+        h = C.c_void_p()
+        rc = self.L.hpq_create(C.byref(self._params), C.byref(h))
+        self.ctx = h
+        _abi.check(self.ctx, rc)
+        self.set_stream(torch.cuda.current_stream(self.device))
+        arr = (C.c_int32 * 32)()
+        self.L.hpq_layout(self.ctx, arr)
+        self.layout = dict(zip(_abi.LAYOUT_NAMES, list(arr)))
+        self.ncomp = self.layout["ncomp"]
+        self.ldx, self.ldy = cfg.nx + 4, cfg.ny + 4
+        self.store = torch.zeros((self.ncomp, self.ldy, self.ldx), dtype=torch.float64, device=self.device)
+        # plasma SoA
+        self._profile = make_profile(cfg)
+        n = int(self.L.hpq_plasma_count(self.ctx, C.byref(self._profile))) if cfg.has_plasma else 0
+        self.np = n
+        self.plasma = {k: torch.zeros(max(n, 1), dtype=torch.float64, device=self.device) for k in PLASMA_REALS}
+        self.plasma["valid"] = torch.zeros(max(n, 1), dtype=torch.int32, device=self.device)
+        self._pdesc = _abi.Plasma(*[self.plasma[k].data_ptr() for k in PLASMA_REALS + ["valid"]], n)
+        self.checksum_buf = torch.zeros(self.ncomp, dtype=torch.float64, device=self.device)
+        _abi.check(self.ctx, self.L.hpq_set_checksums(self.ctx, C.c_void_p(self.checksum_buf.data_ptr())))
+        self._captures: Dict[str, torch.Tensor] = {}
+        self.set_beam(beam if beam is not None else BeamArrays.empty(cfg.nz))
+
+    def __del__(self):
+        try:
+            torch.cuda.synchronize(self.device)
+            self.L.hpq_destroy(self.ctx)
+        except Exception:
+            pass
+
+    # ---- setup ----------------------------------------------------------------------
+    def set_stream(self, stream):
+        self.stream = stream
+        _abi.check(self.ctx, self.L.hpq_set_stream(self.ctx, C.c_void_p(stream.cuda_stream)))
+
+    def set_beam(self, b: BeamArrays):
+        self.beam_host = b
+        dev = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(self.device)
+        n = b.n
+        self.beam = {k: dev(getattr(b, k) if n else np.zeros(1)) for k in ("x", "y", "z", "ux", "uy", "uz", "w")}
+        self.beam["valid"] = dev(b.valid if n else np.zeros(1, np.int32), torch.int32)
+        self._boff = (C.c_int64 * (self.cfg.nz + 1))(*[int(v) for v in b.off])
+        self._bdesc = _abi.Beam(*[self.beam[k].data_ptr() for k in ("x", "y", "z", "ux", "uy", "uz", "w", "valid")],
+                                C.cast(self._boff, C.POINTER(C.c_int64)), n, b.charge, b.mass)
+
+    def init_plasma(self):
+        if self.np:
+            _abi.check(self.ctx, self.L.hpq_init_plasma(self.ctx, C.byref(self._profile), C.byref(self._pdesc)))
+
+    def set_plasma(self, d: Dict[str, np.ndarray]):
+        for k in PLASMA_REALS + ["valid"]:
+            src = torch.as_tensor(np.ascontiguousarray(d[k]))
+            self.plasma[k][: self.np].copy_(src.to(self.plasma[k].dtype))
+
+    def capture(self, name: str) -> torch.Tensor:
+        """Record the valid region of component `name` at every slice's diagnostics point."""
+        buf = torch.zeros((self.cfg.nz, self.cfg.ny, self.cfg.nx), dtype=torch.float64, device=self.device)
+        self._captures[name] = buf
+        _abi.check(self.ctx, self.L.hpq_add_capture(self.ctx, self.layout[name], C.c_void_p(buf.data_ptr())))
+        return buf
+
+    # ---- slice loop -------------------------------------------------------------------
+    def begin_step(self, init_plasma: bool = True):
+        if init_plasma:
+            self.init_plasma()
+        self.checksum_buf.zero_()
+        _abi.check(self.ctx, self.L.hpq_begin_step(self.ctx, C.byref(self._pdesc),
+                                                   C.c_void_p(self.store.data_ptr())))
+
+    def solve_slice(self, islice: int):
+        _abi.check(self.ctx, self.L.hpq_solve_one_slice(self.ctx, C.byref(self._pdesc), C.byref(self._bdesc),
+                                                        C.c_void_p(self.store.data_ptr()), islice))
+
+    def run_slices(self, first: Optional[int] = None, last: int = 0):
+        first = self.cfg.nz - 1 if first is None else first
+        _abi.check(self.ctx, self.L.hpq_run_slices(self.ctx, C.byref(self._pdesc), C.byref(self._bdesc),
+                                                   C.c_void_p(self.store.data_ptr()), first, last))
+
+    def run_step(self, first: Optional[int] = None, last: int = 0):
+        self.begin_step()
+        self.run_slices(first, last)
+
+    # ---- results ----------------------------------------------------------------------
+    def checksums(self) -> Dict[str, float]:
+        torch.cuda.synchronize(self.device)
+        v = self.checksum_buf.cpu().numpy()
+        return {k: float(v[i]) for k, i in self.layout.items() if k != "ncomp" and i >= 0}
+
+    def stats(self) -> Dict[str, float]:
+        out = (C.c_double * 4)()
+        _abi.check(self.ctx, self.L.hpq_stats(self.ctx, out))
+        return dict(n_qsa=out[0], mg_iters=out[1], pc_iters=out[2], pushes=out[3])
+
+    def plasma_host(self) -> Dict[str, np.ndarray]:
+        torch.cuda.synchronize(self.device)
+        return {k: v[: self.np].cpu().numpy() for k, v in self.plasma.items()}
+
+    def store_host(self) -> np.ndarray:
+        torch.cuda.synchronize(self.device)
+        return self.store.cpu().numpy()
+
+    # ---- kernel-level entry points (the reference's free functions) -----------------------
+    def deposit_current(self, comps, ions: bool = False) -> int:
+        arr = (C.c_int32 * 6)(*[self.layout.get(c, -1) if isinstance(c, str) else c for c in comps])
+        nq = C.c_int64(0)
+        _abi.check(self.ctx, self.L.hpq_deposit_current(self.ctx, C.byref(self._pdesc),
+                                                        C.c_void_p(self.store.data_ptr()), arr, int(ions),
+                                                        C.byref(nq)))
+        return nq.value
+
+    def explicit_deposition(self):
+        _abi.check(self.ctx, self.L.hpq_explicit_deposition(self.ctx, C.byref(self._pdesc),
+                                                            C.c_void_p(self.store.data_ptr())))
+
+    def deposit_current_slice(self, islice: int, cjx, cjy, cjz):
+        f = lambda c: self.layout.get(c, -1) if isinstance(c, str) else c
+        arr = (C.c_int32 * 3)(f(cjx), f(cjy), f(cjz))
+        _abi.check(self.ctx, self.L.hpq_deposit_current_slice(self.ctx, C.byref(self._bdesc), islice,
+                                                              C.c_void_p(self.store.data_ptr()), arr))
+
+    def advance_plasma_particles(self, temp_slice: bool = False):
+        _abi.check(self.ctx, self.L.hpq_advance_plasma_particles(self.ctx, C.byref(self._pdesc),
+                                                                 C.c_void_p(self.store.data_ptr()),
+                                                                 int(temp_slice)))
+
+    def solve_poisson_psi_exmby_eypbx_ez_bz(self):
+        _abi.check(self.ctx, self.L.hpq_solve_poisson_psi_exmby_eypbx_ez_bz(self.ctx,
+                                                                            C.c_void_p(self.store.data_ptr())))
+
+    def solve_poisson_bxby(self):
+        _abi.check(self.ctx, self.L.hpq_solve_poisson_bxby(self.ctx, C.c_void_p(self.store.data_ptr())))
+
+    def initialize_sxsy_with_beam(self):
+        _abi.check(self.ctx, self.L.hpq_initialize_sxsy_with_beam(self.ctx, C.c_void_p(self.store.data_ptr())))
+
+    def mg_solve1(self, tol_rel=1e-4, tol_abs=2.2250738585072014e-308, maxiter=200):
+        plane = self.ldx * self.ldy
+        base = self.store.data_ptr()
+        it = C.c_int(0)
+        rc = self.L.hpq_mg_solve1(self.ctx, C.c_void_p(base + 8 * plane * self.layout["Bx"]),
+                                  C.c_void_p(base + 8 * plane * self.layout["Sy"]),
+                                  C.c_void_p(base + 8 * plane * self.layout["chi"]), plane, tol_rel, tol_abs,
+                                  maxiter, C.byref(it))
+        return rc, it.value
+
+    def poisson_dirichlet(self, staging: torch.Tensor, lhs: torch.Tensor):
+        assert staging.is_contiguous() and staging.numel() == self.cfg.nx * self.cfg.ny
+        assert lhs.is_contiguous() and lhs.numel() == self.ldx * self.ldy
+        _abi.check(self.ctx, self.L.hpq_poisson_dirichlet(self.ctx, C.c_void_p(staging.data_ptr()),
+                                                          C.c_void_p(lhs.data_ptr())))

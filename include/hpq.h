@@ -1,0 +1,186 @@
+/* hpq — MI355X-native per-slice quasi-static loop (gfx950 HIP), C ABI.
+ *
+ * This is the drop-in boundary for HiPACE++'s per-slice loop. Each entry point
+ * replaces one reference function (cited below as path:line inside the reference
+ * source tree). All particle and field buffers are caller-owned device pointers;
+ * the context owns only scratch (DST work arrays, multigrid hierarchy, counters).
+ * Calls are ordered on the context's HIP stream (hpq_set_stream); entry points that
+ * must read back a scalar (MG convergence, QSA count) synchronise that stream.
+ * Return value: 0 (HPQ_OK) or an HPQ_E* code; hpq_last_error() gives the message.
+ * A context is not re-entrant: use one context per stream.
+ *
+ * Slice store layout (the reference's Fields::m_slices, src/fields/Fields.cpp:50-220):
+ * ncomp planes of (nx+4) x (ny+4) doubles, x fastest, 2 guard cells on each side,
+ * plane k at store + k*(nx+4)*(ny+4). Component indices: hpq_layout().
+ */
+#ifndef HPQ_H
+#define HPQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hpq_ctx hpq_ctx;
+
+enum {
+    HPQ_OK = 0,
+    HPQ_E_INVALID = 1,          /* bad argument or size */
+    HPQ_E_HIP = 2,              /* HIP runtime error */
+    HPQ_E_MG_NOT_CONVERGED = 3, /* hpmg: Failed to converge (HpMultiGrid.cpp:1408-1416) */
+    HPQ_E_MG_DIVERGED = 4,      /* hpmg: failing (HpMultiGrid.cpp:1397-1405) */
+    HPQ_E_UNSUPPORTED = 5,      /* option outside this path */
+    HPQ_E_NOMEM = 6
+};
+
+enum { HPQ_BC_REFLECTING = 0, HPQ_BC_PERIODIC = 1, HPQ_BC_ABSORBING = 2 };
+
+/* Run parameters: the reference input keys this path honours
+ * (src/Hipace.cpp:97-213, src/Hipace.H:208-252). Units are normalised (c=ep0=mu0=1). */
+typedef struct hpq_params {
+    int nx, ny, nz;                      /* amr.n_cell */
+    double xlo, xhi, ylo, yhi, zlo, zhi; /* geometry.prob_lo / prob_hi */
+    int depos_order;                     /* hipace.depos_order_xy: 0..3 */
+    int derivative_type;                 /* hipace.depos_derivative_type: 0..2 */
+    int explicit_solver;                 /* hipace.bxby_solver: 1 explicit, 0 predictor-corrector */
+    int deposit_rho;                     /* allocate and deposit rho */
+    int do_beam_jx_jy;                   /* hipace.do_beam_jx_jy_deposition */
+    int neutralize;                      /* plasma.neutralize_background */
+    double mg_tol_rel, mg_tol_abs;       /* hipace.MG_tolerance_rel / _abs */
+    int mg_maxiter;                      /* 200 in Hipace::ExplicitMGSolveBxBy */
+    int particle_bc;                     /* boundary.particle: HPQ_BC_* */
+    double plo[2], phi[2];               /* boundary.particle_lo / particle_hi */
+    double pc_tol;                       /* hipace.predcorr_B_error_tolerance */
+    int pc_maxiter;                      /* hipace.predcorr_max_iterations */
+    double pc_mix;                       /* hipace.predcorr_B_mixing_factor */
+    double plasma_charge, plasma_mass;   /* species charge and mass */
+    double max_qsa;                      /* plasma.max_qsa_weighting_factor */
+} hpq_params;
+
+/* Plasma species, SoA device arrays in PlasmaIdx order
+ * (src/particles/plasma/PlasmaParticleContainer.H:21-46). valid: 1 live, 0 removed. */
+typedef struct hpq_plasma {
+    double *x, *y, *w, *ux, *uy, *psi;
+    double *x_prev, *y_prev, *ux_half_step, *uy_half_step, *psi_half_step;
+    int32_t *valid;
+    int64_t n;
+} hpq_plasma;
+
+/* Beam particles grouped by ζ-slice (BoxSort.cpp:14-78): the particles of slice s are
+ * [off[s], off[s+1]). Device arrays; `off` is a host array of nz+1 offsets. */
+typedef struct hpq_beam {
+    const double *x, *y, *z, *ux, *uy, *uz, *w;
+    const int32_t *valid;
+    const int64_t *off;
+    int64_t n;
+    double charge, mass;
+} hpq_beam;
+
+/* Plasma injection profile: n(x,y) = density * (1 + ramp_a (x^2+y^2)) inside `radius`. */
+typedef struct hpq_plasma_profile {
+    int ppc[2];
+    double density, ramp_a, radius;
+    double u_mean[3];
+} hpq_plasma_profile;
+
+/* Names of the hpq_layout() entries, in order. */
+#define HPQ_LAYOUT_N 32
+
+/* ---- context ---------------------------------------------------------------- */
+int hpq_create(const hpq_params *p, hpq_ctx **out);
+void hpq_destroy(hpq_ctx *ctx);
+const char *hpq_last_error(const hpq_ctx *ctx);
+int hpq_set_stream(hpq_ctx *ctx, void *hip_stream);
+/* out[HPQ_LAYOUT_N]: ncomp, next_jx_beam, next_jy_beam, chi, Sy, Sx, ExmBy, EypBx, Ez, Bx,
+ * By, Bz, Psi, jx_beam, jy_beam, jz_beam, jx, jy, rhomjz, rho, prev_jx_beam, prev_jy_beam,
+ * ions_rhomjz, next_jx, next_jy, jz, prev_Bx, prev_By, prev_jx, prev_jy, pciter_Bx, pciter_By
+ * (-1: not allocated). Replaces the Comps map, src/fields/Fields.H:27-56. */
+int hpq_layout(const hpq_ctx *ctx, int32_t *out);
+int64_t hpq_store_doubles(const hpq_ctx *ctx);
+
+/* ---- per-slice operations (reference free functions and methods) -------------- */
+
+/* ::DepositCurrent, src/particles/deposition/PlasmaDepositCurrent.cpp:22-259.
+ * comps[6] = jx, jy, jz, rho, chi, rhomjz component indices (-1: skip). ions != 0 flips
+ * the charge (RhomJzIons background). QSA violators are invalidated; their count is
+ * written to *n_qsa (host) when n_qsa is not NULL (synchronises the stream). */
+int hpq_deposit_current(hpq_ctx *ctx, const hpq_plasma *p, double *store, const int32_t *comps,
+                        int ions, int64_t *n_qsa);
+
+/* ::ExplicitDeposition, src/particles/deposition/ExplicitDeposition.cpp:20-265:
+ * adds the plasma part of Sy, Sx from Bz, Ez, ExmBy, EypBx. */
+int hpq_explicit_deposition(hpq_ctx *ctx, const hpq_plasma *p, double *store);
+
+/* ::DepositCurrentSlice, src/particles/deposition/BeamDepositCurrent.cpp:22-196:
+ * beam particles of slice `slice` deposit into comps[3] = jx, jy, jz (-1: skip). */
+int hpq_deposit_current_slice(hpq_ctx *ctx, const hpq_beam *b, int slice, double *store,
+                              const int32_t *comps);
+
+/* AdvancePlasmaParticles, src/particles/pusher/PlasmaParticleAdvance.cpp:29-240
+ * (leapfrog, gather FieldGather.H:45-96, EnforceBC GetAndSetPosition.H:29-99). */
+int hpq_advance_plasma_particles(hpq_ctx *ctx, const hpq_plasma *p, const double *store,
+                                 int temp_slice);
+
+/* FFTPoissonSolverDirichlet*::SolvePoissonEquation
+ * (src/fields/fft_poisson_solver/FFTPoissonSolverDirichletDirect.cpp:101-139,
+ * ...DirichletFast.cpp:291-328): solves Lap(lhs) = staging with zero Dirichlet values
+ * one cell outside the box. staging: nx*ny contiguous; lhs: a guarded plane, valid cells
+ * written. */
+int hpq_poisson_dirichlet(hpq_ctx *ctx, const double *staging, double *lhs);
+
+/* Fields::SolvePoissonPsiExmByEypBxEzBz, src/fields/Fields.cpp:841-957. */
+int hpq_solve_poisson_psi_exmby_eypbx_ez_bz(hpq_ctx *ctx, double *store);
+
+/* Fields::SolvePoissonBxBy, src/fields/Fields.cpp:1009-1078 (into PCIter Bx, By). */
+int hpq_solve_poisson_bxby(hpq_ctx *ctx, double *store);
+
+/* Hipace::InitializeSxSyWithBeam, src/Hipace.cpp:745-790. */
+int hpq_initialize_sxsy_with_beam(hpq_ctx *ctx, double *store);
+
+/* hpmg::MultiGrid::solve1, src/mg_solver/HpMultiGrid.cpp:1170-1190 + solve_doit
+ * :1308-1427: solves -acf*sol + Lap(sol) = rhs (two components comp_stride apart,
+ * guarded planes as in the slice store). sol holds the initial guess. *iters (host)
+ * receives the number of V-cycles. Synchronises the stream once per V-cycle. */
+int hpq_mg_solve1(hpq_ctx *ctx, double *sol, const double *rhs, const double *acf,
+                  int64_t comp_stride, double tol_rel, double tol_abs, int maxiter, int *iters);
+
+/* ---- slice loop (Hipace::Evolve / SolveOneSlice) ---------------------------------- */
+
+/* Number of plasma particles the profile injects (PlasmaParticleContainerInit.cpp:143-176). */
+int64_t hpq_plasma_count(const hpq_ctx *ctx, const hpq_plasma_profile *prof);
+/* PlasmaParticleContainer::InitParticles (PlasmaParticleContainerInit.cpp:18-378): fills
+ * p (capacity p->n must equal hpq_plasma_count) with the lattice in GPU order. */
+int hpq_init_plasma(hpq_ctx *ctx, const hpq_plasma_profile *prof, const hpq_plasma *p);
+
+/* Start of a time step (src/Hipace.cpp:403-471): zero the slice store and deposit the
+ * neutralising ion background of p (MultiPlasma::DepositNeutralizingBackground). */
+int hpq_begin_step(hpq_ctx *ctx, const hpq_plasma *p, double *store);
+
+/* Hipace::SolveOneSlice, src/Hipace.cpp:557-728, for slice `islice`. */
+int hpq_solve_one_slice(hpq_ctx *ctx, const hpq_plasma *p, const hpq_beam *b, double *store,
+                        int islice);
+
+/* Slices first..last (descending) of one step. */
+int hpq_run_slices(hpq_ctx *ctx, const hpq_plasma *p, const hpq_beam *b, double *store,
+                   int first, int last);
+
+/* Diagnostics at the field-output point of each slice (Hipace::FillFieldDiagnostics):
+ * checksums (device, ncomp doubles) accumulates sum |F| over valid cells per component
+ * (the reference's checksum, tests/checksum/backend/openpmd_backend.py:40-45); NULL
+ * disables. A capture copies the valid region of component `comp` of every slice into
+ * dst (device, nz*ny*nx doubles, slice-major). */
+int hpq_set_checksums(hpq_ctx *ctx, double *checksums);
+int hpq_add_capture(hpq_ctx *ctx, int comp, double *dst);
+int hpq_clear_captures(hpq_ctx *ctx);
+
+/* Counters of the last hpq_begin_step..hpq_run_slices sequence:
+ * out[0] QSA removals, out[1] MG V-cycles, out[2] PC iterations, out[3] plasma pushes
+ * (non-temporary, as src/particles/pusher/PlasmaParticleAdvance.cpp:70-73 counts). */
+int hpq_stats(const hpq_ctx *ctx, double *out4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPQ_H */

@@ -1,0 +1,68 @@
+"""Shared helpers for the parity tests: configurations, seeded random states, norms."""
+import json
+import math
+import os
+
+import numpy as np
+
+from hipace2_amd.inputs import SliceConfig, parse_deck, make_beam, merge_beams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden", "reference_checksums.json")
+
+PLASMA_REALS = ["x", "y", "w", "ux", "uy", "psi", "x_prev", "y_prev", "ux_half_step", "uy_half_step",
+                "psi_half_step"]
+
+
+def golden_cases():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def deck_config(case):
+    """(cfg, beam) for one golden case of tests/golden/reference_checksums.json."""
+    text = "\n".join(f"{k} = {v}" for k, v in case["deck"].items())
+    cfg, beams, _ = parse_deck(text, case["overrides"])
+    beam = merge_beams(cfg, [make_beam(cfg, b) for b in beams])
+    return cfg, beam
+
+
+def small_config(n=63, nz=16, order=2, dtype=2, explicit=True, L=6.0):
+    return SliceConfig(nx=n, ny=n, nz=nz, xlo=-L, xhi=L, ylo=-L, yhi=L, zlo=-8.0, zhi=4.0,
+                       depos_order=order, derivative_type=dtype, explicit_solver=explicit)
+
+
+def random_plasma(cfg, n, seed=0, spread=0.2):
+    """Seeded plasma state with every particle strictly inside the domain."""
+    rng = np.random.default_rng(seed)
+    d = {}
+    m = 0.5 * cfg.dx
+    d["x"] = rng.uniform(cfg.xlo + m, cfg.xhi - m, n)
+    d["y"] = rng.uniform(cfg.ylo + m, cfg.yhi - m, n)
+    d["w"] = rng.uniform(0.5, 1.5, n)
+    d["ux"] = rng.normal(0, spread, n)
+    d["uy"] = rng.normal(0, spread, n)
+    d["psi"] = 1.0 + rng.uniform(-0.3, 0.3, n)
+    d["x_prev"] = np.clip(d["x"] + rng.normal(0, 0.1 * cfg.dx, n), cfg.xlo + m, cfg.xhi - m)
+    d["y_prev"] = np.clip(d["y"] + rng.normal(0, 0.1 * cfg.dy, n), cfg.ylo + m, cfg.yhi - m)
+    d["ux_half_step"] = d["ux"] + rng.normal(0, 0.01, n)
+    d["uy_half_step"] = d["uy"] + rng.normal(0, 0.01, n)
+    d["psi_half_step"] = d["psi"] + rng.uniform(-0.01, 0.01, n)
+    d["valid"] = (rng.uniform(0, 1, n) > 0.05).astype(np.int32)
+    return d
+
+
+def random_store(ncomp, ldy, ldx, seed=1, scale=1.0):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-scale, scale, (ncomp, ldy, ldx))
+
+
+def rel_err(a, b):
+    """max |a - b| / max |b| (norm-wise relative error, 0 if both vanish)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.max(np.abs(b)) if b.size else 0.0
+    num = np.max(np.abs(a - b)) if a.size else 0.0
+    if den == 0.0:
+        return num
+    return num / den
