@@ -1,0 +1,192 @@
+"""Throughput of the per-slice quasi-static loop on one MI355X.
+
+Workload (one step = one pass of the slice loop over all ζ-slices, inputs resident in HBM):
+  511 x 511 transverse grid over [-8, 8]^2, 512 slices over ζ in [-8, 4], uniform plasma
+  n0 = 1 at 2 x 2 particles per cell (1 044 484 plasma particles per slice), shape order 2,
+  explicit Bx/By solver with multigrid; Gaussian electron drive beam with peak density
+  0.01 n0, sigma_r = 0.3, sigma_z = 1.41, 4e6 macro-particles (seeded Philox), gamma = 1000.
+  Arithmetic in FP64 (the plain path).
+
+Prints one JSON line: slices/s (value), plasma particle pushes/s, the roofline of the
+dominant kernel (HIP-event timed inside the timed region) and the oracle timed on the host
+cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = "511x511x512 blowout-geometry slice loop, 2x2 ppc, shape 2, explicit MG, FP64"
+
+
+def make_workload(n=511, nz=512, ppc=2, nbeam=4_000_000):
+    from hipace2_amd.inputs import BeamSpec, SliceConfig, make_beam
+    cfg = SliceConfig(nx=n, ny=n, nz=nz, xlo=-8.0, xhi=8.0, ylo=-8.0, yhi=8.0, zlo=-8.0, zhi=4.0,
+                      depos_order=2, derivative_type=2, explicit_solver=True, ppc=(ppc, ppc))
+    spec = BeamSpec(injection_type="fixed_weight", profile="gaussian", density=0.01,
+                    position_std=(0.3, 0.3, 1.41), num_particles=nbeam, u_mean=(0.0, 0.0, 1000.0), seed=1234)
+    return cfg, make_beam(cfg, spec)
+
+
+def bytes_per_launch(kind, cfg, n_plasma):
+    """Algorithmic HBM bytes of one launch (each operand read or written once)."""
+    G = cfg.nx * cfg.ny
+    if kind == "push":
+        # read x_prev, y_prev, ux/uy/psi half-step (5 x 8 B) + valid (4 B); write x, y, ux, uy, psi
+        # and the five half-step values (10 x 8 B); gather reads Psi, Ez, Bx, By, Bz once
+        return n_plasma * (5 * 8 + 4 + 10 * 8) + 5 * G * 8
+    if kind == "deposit":
+        # read x, y, w, ux, uy, psi (6 x 8 B) + valid; write jx, jy, chi, rhomjz once
+        return n_plasma * (6 * 8 + 4) + 4 * G * 8
+    if kind == "explicit_deposit":
+        # read x, y, w, ux, uy, psi + valid; read Bz, Ez, ExmBy, EypBx; write Sy, Sx
+        return n_plasma * (6 * 8 + 4) + 6 * G * 8
+    return None
+
+
+def run_gpu(args):
+    import torch
+    from hipace2_amd.simulation import PLASMA_REALS, Simulation
+
+    cfg, beam = make_workload(args.n, args.nz, args.ppc, args.nbeam)
+    sim = Simulation(cfg, beam)
+    sim.init_plasma()
+    pristine = {k: v.clone() for k, v in sim.plasma.items()}
+    kinds = ["push", "deposit", "explicit_deposit", "mg_smooth", "mg_coarse", "dst", "beam_deposit"]
+
+    def step():
+        for k, v in pristine.items():
+            sim.plasma[k].copy_(v)
+        sim.begin_step(init_plasma=False)
+        sim.run_slices()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    sim.trace(kinds, True)
+    sim.trace_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    traces = {k: sim.trace_read(k) for k in kinds}
+    stats = sim.stats()
+    return cfg, beam, sim, elapsed, traces, stats
+
+
+def cpu_baseline(args, cfg, beam, sim):
+    """The oracle on the host cores, started from the GPU state at slice s0 (inside the beam)."""
+    import numpy as np
+    import torch
+    from oracle.oracle import OracleSim
+    from hipace2_amd.simulation import PLASMA_REALS
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    s0 = min(cfg.nz - 1, int(round((0.44 - cfg.zlo) / cfg.dz)))
+    # GPU state entering slice s0
+    sim.trace(["push"], False)
+    sim.init_plasma()
+    sim.begin_step(init_plasma=False)
+    sim.run_slices(cfg.nz - 1, s0 + 1)
+    torch.cuda.synchronize()
+    orc = OracleSim(cfg, beam, nthreads=threads)
+    orc.begin_step()
+    orc.set_store(sim.store_host())
+    p = sim.plasma_host()
+    orc.set_plasma(p)
+    done = 0
+    t0 = time.perf_counter()
+    budget = args.cpu_seconds
+    while done < args.cpu_max_slices:
+        orc.solve_slice(s0 - done)
+        done += 1
+        if time.perf_counter() - t0 > budget and done >= 2:
+            break
+    t = time.perf_counter() - t0
+    return {"value": done / t, "unit": "slices/s", "cores": threads, "kind": "port",
+            "sample": f"{done} consecutive slices of the same step starting at slice {s0} (zeta=+0.44, inside "
+                      f"the drive beam) from the GPU's state, oracle (C++/OpenMP FP64) on {threads} host threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=511)
+    ap.add_argument("--nz", type=int, default=512)
+    ap.add_argument("--ppc", type=int, default=2)
+    ap.add_argument("--nbeam", type=int, default=4_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-max-slices", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the slice loop is not sharded across GPUs "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    cfg, beam, sim, elapsed, traces, stats = run_gpu(args)
+    n_plasma = sim.np
+    slices = cfg.nz * args.steps
+    slices_per_s = slices / elapsed
+    # counters are per step (reset by begin_step); the reference counts every plasma particle of
+    # a slice as one push (src/particles/pusher/PlasmaParticleAdvance.cpp:70-73)
+    pushes_per_s = stats["pushes"] * args.steps / elapsed
+
+    # dominant kernel among those with an algorithmic byte count
+    rows = {}
+    for k, (ms, launches) in traces.items():
+        rows[k] = {"total_ms": ms, "launches": launches, "avg_us": 1e3 * ms / launches if launches else 0.0}
+    bounded = {k: v for k, v in rows.items() if bytes_per_launch(k, cfg, n_plasma) is not None and v["launches"]}
+    dom = max(bounded, key=lambda k: bounded[k]["total_ms"])
+    bpl = bytes_per_launch(dom, cfg, n_plasma)
+    avg_s = rows[dom]["avg_us"] * 1e-6
+    achieved = bpl / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "bytes_per_launch": bpl, "avg_launch_us": round(rows[dom]["avg_us"], 2)}
+    print("kernel times over the timed region: " + json.dumps(rows), file=sys.stderr)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, cfg, beam, sim)
+
+    line = {
+        "metric": "slices/s + plasma pushes/s",
+        "value": round(slices_per_s, 2),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (lattice plasma, seeded Philox Gaussian beam)",
+        "config": {"workload": WORKLOAD if (args.n, args.nz, args.ppc) == (511, 512, 2) else
+                   f"{args.n}x{args.n}x{args.nz} slice loop, {args.ppc}x{args.ppc} ppc, shape 2, explicit MG, FP64",
+                   "nx": cfg.nx, "ny": cfg.ny, "slices": cfg.nz, "plasma_particles_per_slice": n_plasma,
+                   "beam_particles": beam.n, "mg_vcycles_per_slice": round(stats["mg_iters"] / cfg.nz, 3)},
+        "plasma_pushes_per_s": round(pushes_per_s, 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

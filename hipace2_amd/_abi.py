@@ -25,7 +25,11 @@ EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq
            "hpq_advance_plasma_particles", "hpq_poisson_dirichlet", "hpq_solve_poisson_psi_exmby_eypbx_ez_bz",
            "hpq_solve_poisson_bxby", "hpq_initialize_sxsy_with_beam", "hpq_mg_solve1", "hpq_plasma_count",
            "hpq_init_plasma", "hpq_begin_step", "hpq_solve_one_slice", "hpq_run_slices", "hpq_set_checksums",
-           "hpq_add_capture", "hpq_clear_captures", "hpq_stats"]
+           "hpq_add_capture", "hpq_clear_captures", "hpq_stats", "hpq_trace_enable", "hpq_trace_read",
+           "hpq_trace_reset"]
+
+TRACE = {"push": 0, "deposit": 1, "explicit_deposit": 2, "beam_deposit": 3, "mg_smooth": 4, "mg_coarse": 5,
+         "dst": 6}
 
 
 class Params(C.Structure):
@@ -101,6 +105,9 @@ def lib():
         L.hpq_add_capture.argtypes = [vp, ip, vp]
         L.hpq_clear_captures.argtypes = [vp]
         L.hpq_stats.argtypes = [vp, dp]
+        L.hpq_trace_enable.argtypes = [vp, ip, ip]
+        L.hpq_trace_read.argtypes = [vp, ip, dp, C.POINTER(C.c_int64)]
+        L.hpq_trace_reset.argtypes = [vp]
         _lib = L
     return _lib
 

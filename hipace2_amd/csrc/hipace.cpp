@@ -16,6 +16,25 @@
 
 namespace hpq {
 
+TraceScope::TraceScope(hpq_ctx* c, int kind) : c_(c), kind_(kind), slot_(0) {
+    Trace& t = c_->trace[kind_];
+    if (!t.on) return;
+    if (t.used == t.start.size()) {
+        hipEvent_t a, b;
+        HPQ_HIP(hipEventCreate(&a));
+        HPQ_HIP(hipEventCreate(&b));
+        t.start.push_back(a);
+        t.stop.push_back(b);
+    }
+    slot_ = t.used++;
+    HPQ_HIP(hipEventRecord(t.start[slot_], c_->stream));
+}
+
+TraceScope::~TraceScope() {
+    Trace& t = c_->trace[kind_];
+    if (t.on) (void)hipEventRecord(t.stop[slot_], c_->stream);
+}
+
 void layout_init(hpq_ctx* c) {
     Layout L;
     int n = 0;
@@ -341,6 +360,10 @@ int hpq_create(const hpq_params* p, hpq_ctx** out) {
 void hpq_destroy(hpq_ctx* c) {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
+    for (Trace& t : c->trace) {
+        for (hipEvent_t e : t.start) (void)hipEventDestroy(e);
+        for (hipEvent_t e : t.stop) (void)hipEventDestroy(e);
+    }
     poisson_free(c);
     mg_free(c);
     if (c->d_count) (void)hipFree(c->d_count);
@@ -534,6 +557,36 @@ int hpq_clear_captures(hpq_ctx* c) {
     if (!c) return HPQ_E_INVALID;
     c->captures.clear();
     return HPQ_OK;
+}
+
+int hpq_trace_enable(hpq_ctx* c, int kernel, int on) {
+    if (!c || kernel < 0 || kernel >= HPQ_TRACE_N) return HPQ_E_INVALID;
+    c->trace[kernel].on = on != 0;
+    return HPQ_OK;
+}
+
+int hpq_trace_read(hpq_ctx* c, int kernel, double* total_ms, int64_t* launches) {
+    if (!c || kernel < 0 || kernel >= HPQ_TRACE_N || !total_ms || !launches) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        HPQ_HIP(hipStreamSynchronize(c->stream));
+        Trace& t = c->trace[kernel];
+        double sum = 0.0;
+        for (size_t k = 0; k < t.used; ++k) {
+            float ms = 0.f;
+            HPQ_HIP(hipEventElapsedTime(&ms, t.start[k], t.stop[k]));
+            sum += ms;
+        }
+        *total_ms = sum;
+        *launches = int64_t(t.used);
+    });
+}
+
+int hpq_trace_reset(hpq_ctx* c) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        HPQ_HIP(hipStreamSynchronize(c->stream));
+        for (Trace& t : c->trace) t.used = 0;
+    });
 }
 
 int hpq_stats(const hpq_ctx* c, double* out4) {

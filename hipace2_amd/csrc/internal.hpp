@@ -102,6 +102,13 @@ struct Capture {
     double* dst;
 };
 
+// Event pairs around the launches of one kernel family (HPQ_TRACE_*).
+struct Trace {
+    bool on = false;
+    std::vector<hipEvent_t> start, stop;
+    size_t used = 0;
+};
+
 }  // namespace hpq
 
 struct hpq_ctx {
@@ -118,6 +125,7 @@ struct hpq_ctx {
     double* checksums = nullptr;     // caller-owned device accumulator (may be null)
     std::vector<hpq::Capture> captures;
     double stats[4] = {0, 0, 0, 0};
+    hpq::Trace trace[HPQ_TRACE_N];
     std::string err;
 };
 
@@ -150,5 +158,19 @@ void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p
 int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof);
 
 inline dim3 grid1d(long n, int block) { return dim3(unsigned((n + block - 1) / block)); }
+
+// Records HIP events around the launches in its scope when tracing of `kind` is on.
+class TraceScope {
+public:
+    TraceScope(hpq_ctx* c, int kind);
+    ~TraceScope();
+    TraceScope(const TraceScope&) = delete;
+    TraceScope& operator=(const TraceScope&) = delete;
+
+private:
+    hpq_ctx* c_;
+    int kind_;
+    size_t slot_;
+};
 
 }  // namespace hpq

@@ -394,6 +394,15 @@ void mg_free(hpq_ctx* c) {
 
 namespace {
 
+template <bool ZERO, bool RES, bool INTERP>
+void launch_gsrb4(hpq_ctx* c, const MGLevel& L, const MGLevel& C, double* phi_out, const double* rhs, long rcs,
+                  const double* acf, const double* phi_in, const double* cor_c, double* res) {
+    TraceScope ts(c, HPQ_TRACE_MG_SMOOTH);
+    const dim3 grid((L.vihi - L.vilo + 1 + TX - 1) / TX, (L.vjhi - L.vjlo + 1 + TY - 1) / TY);
+    hipLaunchKernelGGL((k_gsrb4<ZERO, RES, INTERP>), grid, dim3(256), 0, c->stream, L, C, phi_out, rhs, rcs, acf,
+                       phi_in, cor_c, res);
+}
+
 dim3 tile_grid(const MGLevel& L) {
     return dim3((L.vihi - L.vilo + 1 + TX - 1) / TX, (L.vjhi - L.vjlo + 1 + TY - 1) / TY);
 }
@@ -409,12 +418,13 @@ void vcycle(hpq_ctx* c) {
     const int cb = M.coarse_begin;
     for (int l = 0; l < cb; ++l) {
         MGLevel& L = M.lev[l];
-        if (l > 0)
-            hipLaunchKernelGGL((k_gsrb4<true, true, false>), tile_grid(L), dim3(256), 0, c->stream, L, L, L.cor, L.res,
-                               L.cs, L.acf, nullptr, nullptr, L.rescor);
+        if (l > 0) launch_gsrb4<true, true, false>(c, L, L, L.cor, L.res, L.cs, L.acf, nullptr, nullptr, L.rescor);
         launch_restrict(c, M.lev[l + 1], L, M.lev[l + 1].res, L.rescor, 2);
     }
-    hipLaunchKernelGGL(k_coarse_cycle, dim3(1), dim3(COARSE_THREADS), 0, c->stream, M.d_lev, cb, M.max_level);
+    {
+        TraceScope ts(c, HPQ_TRACE_MG_COARSE);
+        hipLaunchKernelGGL(k_coarse_cycle, dim3(1), dim3(COARSE_THREADS), 0, c->stream, M.d_lev, cb, M.max_level);
+    }
     // Up-sweep: the result of level l goes to rescor[l] (dead after the restriction), so
     // the kernel never overwrites cor[l], which neighbouring tiles read as their halo.
     // The coarse kernel leaves its top level in cor[cb].
@@ -423,15 +433,12 @@ void vcycle(hpq_ctx* c) {
         MGLevel& C = M.lev[l + 1];
         const double* corc = (l + 1 == cb) ? C.cor : C.rescor;
         if (l == 0)
-            hipLaunchKernelGGL((k_gsrb4<false, false, true>), tile_grid(L), dim3(256), 0, c->stream, L, C, M.msol,
-                               M.mrhs, L.cs, L.acf, L.cor, corc, nullptr);
+            launch_gsrb4<false, false, true>(c, L, C, M.msol, M.mrhs, L.cs, L.acf, L.cor, corc, nullptr);
         else
-            hipLaunchKernelGGL((k_gsrb4<false, false, true>), tile_grid(L), dim3(256), 0, c->stream, L, C, L.rescor,
-                               L.res, L.cs, L.acf, L.cor, corc, nullptr);
+            launch_gsrb4<false, false, true>(c, L, C, L.rescor, L.res, L.cs, L.acf, L.cor, corc, nullptr);
     }
     MGLevel& L0 = M.lev[0];
-    hipLaunchKernelGGL((k_gsrb4<false, true, false>), tile_grid(L0), dim3(256), 0, c->stream, L0, L0, L0.cor, M.mrhs,
-                       L0.cs, L0.acf, M.msol, nullptr, L0.rescor);
+    launch_gsrb4<false, true, false>(c, L0, L0, L0.cor, M.mrhs, L0.cs, L0.acf, M.msol, nullptr, L0.rescor);
     HPQ_HIP(hipGetLastError());
 }
 
@@ -449,8 +456,7 @@ int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, lon
                            M.msol, M.mrhs);
     }
     for (int l = 1; l <= M.max_level; ++l) launch_restrict(c, M.lev[l], M.lev[l - 1], M.lev[l].acf, M.lev[l - 1].acf, 1);
-    hipLaunchKernelGGL((k_gsrb4<false, true, false>), tile_grid(L0), dim3(256), 0, c->stream, L0, L0, L0.cor, M.mrhs,
-                       L0.cs, L0.acf, M.msol, nullptr, L0.rescor);
+    launch_gsrb4<false, true, false>(c, L0, L0, L0.cor, M.mrhs, L0.cs, L0.acf, M.msol, nullptr, L0.rescor);
     HPQ_HIP(hipMemsetAsync(M.d_norm, 0, 4 * sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_absmax, dim3(64), dim3(256), 0, c->stream, L0, L0.rescor, true, M.d_norm);
     hipLaunchKernelGGL(k_absmax, dim3(64), dim3(256), 0, c->stream, L0, M.mrhs, true, M.d_norm + 1);

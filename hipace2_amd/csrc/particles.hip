@@ -334,6 +334,7 @@ void deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32
     const int zero = 0;
     (void)zero;
     auto cm = [&](int k) { return comps[k] >= 0 ? comps[k] : -1; };
+    TraceScope ts(c, HPQ_TRACE_DEPOSIT);
     dispatch_order(c->p.depos_order, [&](auto O) {
         hipLaunchKernelGGL((k_deposit_plasma<decltype(O)::value>), grid1d(p->n, 256), dim3(256), 0, c->stream, g, *p,
                            store, cm(0), cm(1), cm(2), cm(3), cm(4), cm(5), charge_invvol, cmr, c->p.max_qsa,
@@ -352,6 +353,7 @@ void explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store) {
     const double qmr = c->p.plasma_charge / c->p.plasma_mass;
     const int T = c->p.derivative_type;
     if (T < 0 || T > 2) throw HpqError(HPQ_E_UNSUPPORTED, "depos_derivative_type must be 0..2");
+    TraceScope ts(c, HPQ_TRACE_EXPLICIT_DEPOSIT);
     dispatch_order(c->p.depos_order, [&](auto O) {
         constexpr int o = decltype(O)::value;
         auto launch = [&](auto kern) {
@@ -370,6 +372,7 @@ void deposit_beam(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const
     const long first = b->off[slice];
     const long count = b->off[slice + 1] - first;
     if (count <= 0) return;
+    TraceScope ts(c, HPQ_TRACE_BEAM_DEPOSIT);
     dispatch_order(c->p.depos_order, [&](auto O) {
         hipLaunchKernelGGL((k_deposit_beam<decltype(O)::value>), grid1d(count, 256), dim3(256), 0, c->stream, c->g, *b,
                            first, count, store, comps[0], comps[1], comps[2]);
@@ -382,6 +385,7 @@ void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool t
     const Layout& L = c->L;
     const double qmc = c->p.plasma_charge / (c->p.plasma_mass * 1.0);
     BC bc{c->p.particle_bc, c->p.plo[0], c->p.plo[1], c->p.phi[0], c->p.phi[1]};
+    TraceScope ts(c, HPQ_TRACE_PUSH);
     dispatch_order(c->p.depos_order, [&](auto O) {
         hipLaunchKernelGGL((k_push_plasma<decltype(O)::value>), grid1d(p->n, 256), dim3(256), 0, c->stream, c->g, *p,
                            store, L.Psi, L.Ez, L.Bx, L.By, L.Bz, qmc, bc, int(temp_slice));

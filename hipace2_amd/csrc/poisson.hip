@@ -124,6 +124,7 @@ __global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ in
 }
 
 static void dst_rows(hpq_ctx* c, const DstPlan& P, const Rows& in, const Rows& out, int nrows, const double* scale) {
+    TraceScope ts(c, HPQ_TRACE_DST);
     if (P.fft) {
         const size_t lds = size_t(P.M) * sizeof(double2);
         hipLaunchKernelGGL(k_dst_rows_fft, dim3((nrows + 1) / 2), dim3(256), lds, c->stream, in, out, nrows, P.n, P.M,

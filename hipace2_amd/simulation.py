@@ -150,6 +150,21 @@ class Simulation:
         _abi.check(self.ctx, self.L.hpq_stats(self.ctx, out))
         return dict(n_qsa=out[0], mg_iters=out[1], pc_iters=out[2], pushes=out[3])
 
+    def trace(self, kinds, on: bool = True):
+        """Enable HIP-event timing of the named kernel families (_abi.TRACE)."""
+        for k in kinds:
+            _abi.check(self.ctx, self.L.hpq_trace_enable(self.ctx, _abi.TRACE[k], int(on)))
+
+    def trace_reset(self):
+        _abi.check(self.ctx, self.L.hpq_trace_reset(self.ctx))
+
+    def trace_read(self, kind: str):
+        """(total kernel milliseconds, launches) since the last reset."""
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        _abi.check(self.ctx, self.L.hpq_trace_read(self.ctx, _abi.TRACE[kind], C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def plasma_host(self) -> Dict[str, np.ndarray]:
         torch.cuda.synchronize(self.device)
         return {k: v[: self.np].cpu().numpy() for k, v in self.plasma.items()}

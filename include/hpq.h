@@ -174,6 +174,24 @@ int hpq_set_checksums(hpq_ctx *ctx, double *checksums);
 int hpq_add_capture(hpq_ctx *ctx, int comp, double *dst);
 int hpq_clear_captures(hpq_ctx *ctx);
 
+/* Kernel timing (the reference's HIPACE_PROFILE regions, src/utils/HipaceProfilerWrapper.H):
+ * when enabled for a kernel family, every launch of it is bracketed by HIP events on the
+ * context's stream; hpq_trace_read synchronises the stream and returns the summed kernel
+ * time (ms) and the number of launches since the last reset. */
+enum {
+    HPQ_TRACE_PUSH = 0,             /* AdvancePlasmaParticles kernel */
+    HPQ_TRACE_DEPOSIT = 1,          /* DepositCurrent kernel */
+    HPQ_TRACE_EXPLICIT_DEPOSIT = 2, /* ExplicitDeposition kernel */
+    HPQ_TRACE_BEAM_DEPOSIT = 3,     /* DepositCurrentSlice kernel */
+    HPQ_TRACE_MG_SMOOTH = 4,        /* four red-black sweeps (+ residual), tiled levels */
+    HPQ_TRACE_MG_COARSE = 5,        /* coarse levels in one workgroup */
+    HPQ_TRACE_DST = 6,              /* batched row DST-I */
+    HPQ_TRACE_N = 7
+};
+int hpq_trace_enable(hpq_ctx *ctx, int kernel, int on);
+int hpq_trace_read(hpq_ctx *ctx, int kernel, double *total_ms, int64_t *launches);
+int hpq_trace_reset(hpq_ctx *ctx);
+
 /* Counters of the last hpq_begin_step..hpq_run_slices sequence:
  * out[0] QSA removals, out[1] MG V-cycles, out[2] PC iterations, out[3] plasma pushes
  * (non-temporary, as src/particles/pusher/PlasmaParticleAdvance.cpp:70-73 counts). */
