@@ -51,7 +51,7 @@ def main():
         deck = parse_kv(open(os.path.join(REF, "examples", example, "inputs_normalized")).read())
         golden = json.load(open(os.path.join(REF, "tests", "checksum", "benchmarks_json", test + ".json")))
         fixture[test] = {"deck": deck, "overrides": overrides, "rtol": rtol,
-                         "fields": golden["lev=0"]}
+                         "fields": golden["lev=0"], "beam": golden.get("beam", {})}
     out = os.path.join(HERE, "reference_checksums.json")
     with open(out, "w") as f:
         json.dump(fixture, f, indent=1, sort_keys=True)
