@@ -58,6 +58,8 @@ def run_gpu(args):
     from hipace2_amd.simulation import PLASMA_REALS, Simulation
 
     cfg, beam = make_workload(args.n, args.nz, args.ppc, args.nbeam)
+    if args.bin_period is not None:
+        cfg.bin_period = args.bin_period
     sim = Simulation(cfg, beam)
     sim.init_plasma()
     pristine = {k: v.clone() for k, v in sim.plasma.items()}
@@ -69,10 +71,22 @@ def run_gpu(args):
         sim.begin_step(init_plasma=False)
         sim.run_slices()
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    sim.trace(kinds, True)
+    # Warm-up steps time every kernel family (an event pair around each launch); the
+    # timed steps keep events only around the dominant family, so the timed region is not
+    # slowed by tens of event records per slice.
+    breakdown = {}
+    if args.warmup:
+        sim.trace(kinds, True)
+        sim.trace_reset()
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        breakdown = {k: sim.trace_read(k) for k in kinds}
+        timed = [dominant_kind(cfg, sim.np, breakdown)]
+        sim.trace(kinds, False)
+    else:
+        timed = kinds
+    sim.trace(timed, True)
     sim.trace_reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -81,9 +95,17 @@ def run_gpu(args):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
-    traces = {k: sim.trace_read(k) for k in kinds}
+    traces = {k: sim.trace_read(k) for k in timed}
     stats = sim.stats()
-    return cfg, beam, sim, elapsed, traces, stats
+    sim.trace(timed, False)
+    return cfg, beam, sim, elapsed, traces, stats, breakdown
+
+
+def dominant_kind(cfg, n_plasma, traces):
+    """The kernel family with an algorithmic byte count that takes the most time."""
+    bounded = {k: ms for k, (ms, launches) in traces.items()
+               if launches and bytes_per_launch(k, cfg, n_plasma) is not None}
+    return max(bounded, key=bounded.get)
 
 
 def cpu_baseline(args, cfg, beam, sim):
@@ -132,13 +154,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-max-slices", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--bin-period", type=int, default=None,
+                    help="slices between tile sorts of the plasma for deposition (0: index order)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the slice loop is not sharded across GPUs "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    cfg, beam, sim, elapsed, traces, stats = run_gpu(args)
+    cfg, beam, sim, elapsed, traces, stats, breakdown = run_gpu(args)
     n_plasma = sim.np
     slices = cfg.nz * args.steps
     slices_per_s = slices / elapsed
@@ -146,19 +170,21 @@ def main():
     # a slice as one push (src/particles/pusher/PlasmaParticleAdvance.cpp:70-73)
     pushes_per_s = stats["pushes"] * args.steps / elapsed
 
-    # dominant kernel among those with an algorithmic byte count
-    rows = {}
-    for k, (ms, launches) in traces.items():
-        rows[k] = {"total_ms": ms, "launches": launches, "avg_us": 1e3 * ms / launches if launches else 0.0}
-    bounded = {k: v for k, v in rows.items() if bytes_per_launch(k, cfg, n_plasma) is not None and v["launches"]}
-    dom = max(bounded, key=lambda k: bounded[k]["total_ms"])
+    def table(tr):
+        return {k: {"total_ms": round(ms, 3), "launches": launches,
+                    "avg_us": round(1e3 * ms / launches, 2) if launches else 0.0} for k, (ms, launches) in tr.items()}
+
+    rows = table(traces)
+    dom = dominant_kind(cfg, n_plasma, traces)
     bpl = bytes_per_launch(dom, cfg, n_plasma)
     avg_s = rows[dom]["avg_us"] * 1e-6
     achieved = bpl / avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                 "bytes_per_launch": bpl, "avg_launch_us": round(rows[dom]["avg_us"], 2)}
-    print("kernel times over the timed region: " + json.dumps(rows), file=sys.stderr)
+    if breakdown:
+        print(f"kernel times over {args.warmup} warm-up step(s): " + json.dumps(table(breakdown)), file=sys.stderr)
+    print("dominant kernel over the timed region: " + json.dumps(rows), file=sys.stderr)
 
     cpu = None
     if not args.no_cpu:
@@ -180,7 +206,8 @@ def main():
         "config": {"workload": WORKLOAD if (args.n, args.nz, args.ppc) == (511, 512, 2) else
                    f"{args.n}x{args.n}x{args.nz} slice loop, {args.ppc}x{args.ppc} ppc, shape 2, explicit MG, FP64",
                    "nx": cfg.nx, "ny": cfg.ny, "slices": cfg.nz, "plasma_particles_per_slice": n_plasma,
-                   "beam_particles": beam.n, "mg_vcycles_per_slice": round(stats["mg_iters"] / cfg.nz, 3)},
+                   "beam_particles": beam.n, "mg_vcycles_per_slice": round(stats["mg_iters"] / cfg.nz, 3),
+                   "bin_period": cfg.bin_period},
         "plasma_pushes_per_s": round(pushes_per_s, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,

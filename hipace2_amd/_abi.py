@@ -26,7 +26,7 @@ EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq
            "hpq_solve_poisson_bxby", "hpq_initialize_sxsy_with_beam", "hpq_mg_solve1", "hpq_plasma_count",
            "hpq_init_plasma", "hpq_begin_step", "hpq_solve_one_slice", "hpq_run_slices", "hpq_set_checksums",
            "hpq_add_capture", "hpq_clear_captures", "hpq_stats", "hpq_trace_enable", "hpq_trace_read",
-           "hpq_trace_reset"]
+           "hpq_trace_reset", "hpq_bin_tiles"]
 
 TRACE = {"push": 0, "deposit": 1, "explicit_deposit": 2, "beam_deposit": 3, "mg_smooth": 4, "mg_coarse": 5,
          "dst": 6}
@@ -41,7 +41,8 @@ class Params(C.Structure):
                 ("mg_tol_rel", C.c_double), ("mg_tol_abs", C.c_double), ("mg_maxiter", C.c_int),
                 ("particle_bc", C.c_int), ("plo", C.c_double * 2), ("phi", C.c_double * 2),
                 ("pc_tol", C.c_double), ("pc_maxiter", C.c_int), ("pc_mix", C.c_double),
-                ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double)]
+                ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double),
+                ("bin_period", C.c_int)]
 
 
 class Plasma(C.Structure):

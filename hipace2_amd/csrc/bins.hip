@@ -1,0 +1,229 @@
+// Tile binning of plasma particles on gfx950: a stable counting sort by the tile key of
+// the reference's CPU tiled deposition,
+//   key = ((cx - lo) / T) * ntile_y + (cy - lo) / T,   invalid particles -> ntile_x * ntile_y
+// (src/particles/deposition/DepositionUtil.H:213-227, amrex::DenseBins::build), where
+// (cx, cy) is the lowest cell the particle deposits into (the get_start_cell functor,
+// PlasmaDepositCurrent.cpp:137-155) and lo = -2 is the first guard cell of the slice box.
+// The permutation lists particle indices bin by bin, in increasing index order inside a
+// bin, which is numpy.argsort(key, kind="stable").
+//
+// Passes: keys (one thread per particle); per-chunk histograms in LDS, stored bin-major
+// (bin b, chunk k at b * nchunks + k); exclusive scan of that matrix, which gives every
+// (bin, chunk) pair its first output slot; scatter, one wavefront per chunk walking its
+// particles in index order, so ties keep their order.
+#include <algorithm>
+
+#include "internal.hpp"
+#include "shape.hpp"
+
+namespace hpq {
+
+namespace {
+
+constexpr int SCAN_T = 256, SCAN_ITEMS = 16, SCAN_BLOCK = SCAN_T * SCAN_ITEMS;
+
+template <int O>
+__global__ __launch_bounds__(256) void k_bin_keys(Geom g, hpq_plasma p, int T, int ntx, int nty,
+                                                  int32_t* __restrict__ keys) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= p.n) return;
+    int key = ntx * nty;
+    if (p.valid[ip]) {
+        const double xmid = (p.x[ip] - g.xoff) * g.dxi;
+        const double ymid = (p.y[ip] - g.yoff) * g.dyi;
+        const int cx = shape<O>(xmid, 0).cell, cy = shape<O>(ymid, 0).cell;
+        // integer division truncates toward zero, as in the reference; the tile index is
+        // clamped so a particle outside the guarded box still lands in a real bin
+        const int tx = min(max((cx + g.ng) / T, 0), ntx - 1);
+        const int ty = min(max((cy + g.ng) / T, 0), nty - 1);
+        key = tx * nty + ty;
+    }
+    keys[ip] = key;
+}
+
+// per-chunk histogram, written bin-major
+__global__ __launch_bounds__(256) void k_bin_count(const int32_t* __restrict__ keys, long n, int nbins, long chunk,
+                                                   int nchunks, int32_t* __restrict__ counts) {
+    extern __shared__ int hist[];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    const long lo = long(blockIdx.x) * chunk, hi = min(n, lo + chunk);
+    for (long ip = lo + threadIdx.x; ip < hi; ip += blockDim.x) atomicAdd(&hist[keys[ip]], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) counts[long(b) * nchunks + blockIdx.x] = hist[b];
+}
+
+// one wavefront per chunk: stable ranks inside the chunk from a per-bin running count
+__global__ __launch_bounds__(64) void k_bin_scatter(const int32_t* __restrict__ keys, long n, int nbins, long chunk,
+                                                    int nchunks, const int32_t* __restrict__ first,
+                                                    int32_t* __restrict__ perm) {
+    extern __shared__ int run[];
+    for (int b = threadIdx.x; b < nbins; b += 64) run[b] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const long lo = long(blockIdx.x) * chunk, hi = min(n, lo + chunk);
+    for (long base = lo; base < hi; base += 64) {
+        const long ip = base + lane;
+        const bool act = ip < hi;
+        const int key = act ? keys[ip] : -1;
+        // peers: the active lanes holding the same key (wave-uniform loop over distinct keys)
+        unsigned long long todo = __ballot(act);
+        unsigned long long peers = 0;
+        while (todo) {
+            const int l = __ffsll((long long)todo) - 1;
+            const int k = __shfl(key, l);
+            const unsigned long long m = __ballot(act && key == k);
+            if (act && key == k) peers = m;
+            todo &= ~m;
+        }
+        const int leader = act ? __ffsll((long long)peers) - 1 : 0;
+        int start = 0;
+        if (act && lane == leader) {
+            start = run[key];
+            run[key] = start + __popcll(peers);
+        }
+        start = __shfl(start, leader);
+        __syncthreads();
+        if (act) perm[first[long(key) * nchunks + blockIdx.x] + start + __popcll(peers & lt)] = int32_t(ip);
+    }
+}
+
+// block-local exclusive scan of SCAN_BLOCK elements; block total to sums[blockIdx.x]
+__global__ __launch_bounds__(SCAN_T) void k_scan_block(int32_t* __restrict__ a, long n, int32_t* __restrict__ sums) {
+    __shared__ int wsum[SCAN_T / 64];
+    const long base = long(blockIdx.x) * SCAN_BLOCK + long(threadIdx.x) * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int tot = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        v[q] = (base + q < n) ? a[base + q] : 0;
+        tot += v[q];
+    }
+    // exclusive scan of the per-thread totals across the block
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int inc = tot;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wv; ++w) woff += wsum[w];
+    int run = woff + inc - tot;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        if (base + q < n) a[base + q] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == SCAN_T - 1 && sums) sums[blockIdx.x] = woff + inc;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_add(int32_t* __restrict__ a, long n, const int32_t* __restrict__ offs) {
+    const long base = long(blockIdx.x) * SCAN_BLOCK + long(threadIdx.x) * SCAN_ITEMS;
+    const int o = offs[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q)
+        if (base + q < n) a[base + q] += o;
+}
+
+// bin starts: offsets[b] = first slot of bin b (its chunk-0 entry), offsets[nbins] = n;
+// all zero when there are no particles (first == nullptr)
+__global__ void k_bin_offsets(const int32_t* __restrict__ first, int nbins, int nchunks, long n,
+                              int32_t* __restrict__ offsets) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nbins) offsets[b] = first ? first[long(b) * nchunks] : 0;
+    if (b == nbins) offsets[b] = int32_t(n);
+}
+
+// exclusive scan of a[0..n) in place; tmp holds the block sums of every level
+void scan_exclusive(hipStream_t s, int32_t* a, long n, int32_t* tmp) {
+    const long nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(SCAN_T), 0, s, a, n, nullptr);
+        return;
+    }
+    hipLaunchKernelGGL(k_scan_block, dim3(unsigned(nb)), dim3(SCAN_T), 0, s, a, n, tmp);
+    scan_exclusive(s, tmp, nb, tmp + nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(unsigned(nb)), dim3(SCAN_T), 0, s, a, n, tmp);
+}
+
+long scan_tmp_size(long n) {
+    long t = 0;
+    for (long nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK; nb > 1; nb = (nb + SCAN_BLOCK - 1) / SCAN_BLOCK) t += nb;
+    return t + 1;
+}
+
+template <class T>
+void grow(T*& p, long& cap, long need) {
+    if (need <= cap) return;
+    if (p) HPQ_HIP(hipFree(p));
+    p = nullptr;
+    HPQ_HIP(hipMalloc(&p, sizeof(T) * size_t(need)));
+    cap = need;
+}
+
+}  // namespace
+
+void bins_free(hpq_ctx* c) {
+    Bins& B = c->bins;
+    for (void* q : {(void*)B.keys, (void*)B.perm, (void*)B.counts, (void*)B.tmp})
+        if (q) (void)hipFree(q);
+    B = Bins{};
+}
+
+void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets) {
+    if (tile <= 0) throw HpqError(HPQ_E_INVALID, "bin_tiles: tile size must be positive");
+    if (p->n > INT32_MAX) throw HpqError(HPQ_E_INVALID, "bin_tiles: more than 2^31-1 particles");
+    const Geom& g = c->g;
+    const int ntx = (g.nx + 2 * g.ng + tile - 1) / tile, nty = (g.ny + 2 * g.ng + tile - 1) / tile;
+    const int nbins = ntx * nty + 1;
+    if (nbins > 16384) throw HpqError(HPQ_E_UNSUPPORTED, "bin_tiles: more than 16383 tiles");
+    const long n = p->n;
+    if (n == 0) {
+        if (offsets)
+            hipLaunchKernelGGL(k_bin_offsets, grid1d(nbins + 1, 256), dim3(256), 0, c->stream, nullptr, nbins, 1, 0L,
+                               offsets);
+        HPQ_HIP(hipGetLastError());
+        return;
+    }
+    // at most 4096 chunks of at least 4096 particles
+    const long chunk = std::max(4096L, (n + 4095) / 4096);
+    const int nchunks = int((n + chunk - 1) / chunk);
+    Bins& B = c->bins;
+    grow(B.keys, B.keys_cap, n);
+    const long m = long(nbins) * nchunks;
+    grow(B.counts, B.counts_cap, m);
+    grow(B.tmp, B.tmp_cap, scan_tmp_size(m));
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        hipLaunchKernelGGL((k_bin_keys<decltype(O)::value>), grid1d(n, 256), dim3(256), 0, c->stream, g, *p, tile, ntx,
+                           nty, B.keys);
+    });
+    const size_t lds = sizeof(int) * size_t(nbins);
+    hipLaunchKernelGGL(k_bin_count, dim3(nchunks), dim3(256), lds, c->stream, B.keys, n, nbins, chunk, nchunks,
+                       B.counts);
+    scan_exclusive(c->stream, B.counts, m, B.tmp);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nchunks), dim3(64), lds, c->stream, B.keys, n, nbins, chunk, nchunks,
+                       B.counts, perm);
+    if (offsets)
+        hipLaunchKernelGGL(k_bin_offsets, grid1d(nbins + 1, 256), dim3(256), 0, c->stream, B.counts, nbins, nchunks,
+                           n, offsets);
+    HPQ_HIP(hipGetLastError());
+}
+
+// the permutation the deposition kernels walk: 16x16-cell tiles, refreshed every
+// bin_period slices by the slice driver
+const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p) {
+    Bins& B = c->bins;
+    if (c->p.bin_period <= 0 || p->n <= 0) return nullptr;
+    if (B.perm_n != p->n || B.stale) {
+        grow(B.perm, B.perm_cap, p->n);
+        bin_tiles(c, p, DEPOSIT_TILE, B.perm, nullptr);
+        B.perm_n = p->n;
+        B.stale = false;
+    }
+    return B.perm;
+}
+
+}  // namespace hpq

@@ -1,0 +1,454 @@
+// Current, chi and Sx/Sy deposition on gfx950 with LDS-private windows.
+//
+// Each workgroup takes a chunk of DCHUNK particles (in tile-sorted order when the
+// deposition order from bins.hip is in use), reduces the bounding box of their stencils
+// and, when the box fits in LDS, accumulates the whole chunk there with ds_add_f64 and
+// adds the window to the slice store with one global atomic per touched cell and
+// component. A chunk whose box is too large gets a square window around the mean start
+// cell; its particles outside the window add to the store directly. The per-particle
+// contributions are the reference's expressions in its operation order; only the order
+// of the floating-point sums differs (the reference's GPU path uses global atomics).
+#include <climits>
+#include <cmath>
+
+#include "internal.hpp"
+#include "shape.hpp"
+
+namespace hpq {
+
+namespace {
+
+constexpr int DT = 256;                 // threads per workgroup
+constexpr int DPPT = 4;                 // particles per thread
+constexpr int DCHUNK = DT * DPPT;       // particles per workgroup
+constexpr int WIN_DOUBLES = 6144;       // 48 KiB of LDS per workgroup
+
+__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void lds_add(double* p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ bool in_store(const Geom& g, int i, int j) {
+    return i >= -g.ng && i <= g.nx - 1 + g.ng && j >= -g.ng && j <= g.ny - 1 + g.ng;
+}
+
+// window of cells [i0, i0 + w) x [j0, j0 + h) in LDS; w == 0: none
+struct Win {
+    int i0, j0, w, h;
+    __device__ bool holds(int ci, int cj, int S) const {
+        return w > 0 && ci >= i0 && cj >= j0 && ci + S <= i0 + w && cj + S <= j0 + h;
+    }
+    __device__ int at(int i, int j) const { return (j - j0) * w + (i - i0); }
+};
+
+// Block-wide choice of the window for stencils of S x S cells starting at (ci, cj) for
+// the particles with act set; cap is the number of cells that fit. red: 7 * DT/64 ints.
+__device__ Win choose_window(const int* ci, const int* cj, const bool* act, int S, int cap, int* red) {
+    int mni = INT_MAX, mnj = INT_MAX, mxi = INT_MIN, mxj = INT_MIN, si = 0, sj = 0, cnt = 0;
+#pragma unroll
+    for (int q = 0; q < DPPT; ++q) {
+        if (!act[q]) continue;
+        mni = min(mni, ci[q]); mxi = max(mxi, ci[q]);
+        mnj = min(mnj, cj[q]); mxj = max(mxj, cj[q]);
+        si += ci[q]; sj += cj[q]; ++cnt;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mni = min(mni, __shfl_xor(mni, o)); mxi = max(mxi, __shfl_xor(mxi, o));
+        mnj = min(mnj, __shfl_xor(mnj, o)); mxj = max(mxj, __shfl_xor(mxj, o));
+        si += __shfl_xor(si, o); sj += __shfl_xor(sj, o); cnt += __shfl_xor(cnt, o);
+    }
+    constexpr int NW = DT / 64;
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0 * NW + wv] = mni; red[1 * NW + wv] = mxi; red[2 * NW + wv] = mnj; red[3 * NW + wv] = mxj;
+        red[4 * NW + wv] = si; red[5 * NW + wv] = sj; red[6 * NW + wv] = cnt;
+    }
+    __syncthreads();
+    mni = red[0]; mxi = red[NW]; mnj = red[2 * NW]; mxj = red[3 * NW];
+    si = red[4 * NW]; sj = red[5 * NW]; cnt = red[6 * NW];
+    for (int w = 1; w < NW; ++w) {
+        mni = min(mni, red[w]); mxi = max(mxi, red[NW + w]);
+        mnj = min(mnj, red[2 * NW + w]); mxj = max(mxj, red[3 * NW + w]);
+        si += red[4 * NW + w]; sj += red[5 * NW + w]; cnt += red[6 * NW + w];
+    }
+    if (cnt == 0) return {0, 0, 0, 0};
+    const long bw = long(mxi) - mni + S, bh = long(mxj) - mnj + S;
+    if (bw * bh <= cap) return {mni, mnj, int(bw), int(bh)};
+    // square window around the mean start cell, no larger than the box
+    int side = int(sqrt(double(cap)));
+    if (side < S) return {0, 0, 0, 0};
+    const int w = int(min<long>(side, bw)), h = int(min<long>(cap / w, bh));
+    if (h < S) return {0, 0, 0, 0};
+    const int mi = int(floor(double(si) / cnt)), mj = int(floor(double(sj) / cnt));
+    const int i0 = min(max(mi + S / 2 - w / 2, mni), int(mxi + S - w));
+    const int j0 = min(max(mj + S / 2 - h / 2, mnj), int(mxj + S - h));
+    return {i0, j0, w, h};
+}
+
+// adds the window to the store: quantity k (slot[k] >= 0) goes to component comp[k];
+// cells holding zero are skipped
+template <int NK>
+__device__ void flush_window(const Geom& g, const Win& W, const double* win, const int (&slot)[NK],
+                             const int (&comp)[NK], double* store) {
+    const int cells = W.w * W.h;
+    for (int t = threadIdx.x; t < cells; t += DT) {
+        const int j = W.j0 + t / W.w, i = W.i0 + t % W.w;
+        if (!in_store(g, i, j)) continue;
+        const long k = g.idx(i, j);
+#pragma unroll
+        for (int q = 0; q < NK; ++q) {
+            if (slot[q] < 0) continue;
+            const double v = win[slot[q] * cells + t];
+            if (v != 0.0) atomic_add(store + long(comp[q]) * g.plane + k, v);
+        }
+    }
+}
+
+// window slots of the quantities whose component is not -1, in order
+template <int NK>
+__device__ __forceinline__ int assign_slots(const int (&comp)[NK], int (&slot)[NK]) {
+    int ns = 0;
+#pragma unroll
+    for (int q = 0; q < NK; ++q) {
+        slot[q] = comp[q] >= 0 ? ns : -1;
+        ns += comp[q] >= 0 ? 1 : 0;
+    }
+    return ns;
+}
+
+// ::DepositCurrent deposit functor, src/particles/deposition/PlasmaDepositCurrent.cpp:182-251.
+// comp order: jx, jy, jz, rho, chi, rhomjz (-1: skip; jx and jy go together).
+template <int O>
+__global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, const int32_t* __restrict__ order,
+                                                       double* __restrict__ store, int cjx, int cjy, int cjz, int crho,
+                                                       int cchi, int crm, double charge_invvol,
+                                                       double charge_mu0_mass_ratio, double max_qsa,
+                                                       int* __restrict__ qsa_count) {
+    constexpr int S = O + 1;
+    __shared__ double win[WIN_DOUBLES];
+    __shared__ int red[7 * DT / 64];
+    const double clight = 1.0, clightinv = 1.0 / 1.0;
+    const int comp[6] = {cjx, cjy, cjz, crho, cchi, crm};
+    int slot[6];
+    const int ns = assign_slots(comp, slot);
+
+    double q_invvol[DPPT], vx_c[DPPT], vy_c[DPPT], gamma_psi[DPPT], psi_inv[DPPT], xmid[DPPT], ymid[DPPT];
+    int ci[DPPT], cj[DPPT];
+    bool act[DPPT];
+    const long base = long(blockIdx.x) * DCHUNK;
+#pragma unroll
+    for (int q = 0; q < DPPT; ++q) {
+        const long t = base + q * DT + threadIdx.x;
+        act[q] = false;
+        if (t >= p.n) continue;
+        const long ip = order ? long(order[t]) : t;
+        if (!p.valid[ip]) continue;
+        psi_inv[q] = 1.0 / p.psi[ip];
+        const double xp = p.x[ip], yp = p.y[ip];
+        vx_c[q] = p.ux[ip] * psi_inv[q];
+        vy_c[q] = p.uy[ip] * psi_inv[q];
+        q_invvol[q] = charge_invvol * p.w[ip];
+        xmid[q] = (xp - g.xoff) * g.dxi;
+        ymid[q] = (yp - g.yoff) * g.dyi;
+        const double Aabssqp = 0.0;
+        gamma_psi[q] = 0.5 * ((1.0 + 0.5 * Aabssqp) * psi_inv[q] * psi_inv[q] +
+                              vx_c[q] * vx_c[q] * clightinv * clightinv + vy_c[q] * vy_c[q] * clightinv * clightinv +
+                              1.0);
+        if (gamma_psi[q] < 0.0 || gamma_psi[q] > max_qsa || psi_inv[q] < 0.0) {
+            atomicAdd(qsa_count, 1);
+            p.w[ip] = 0.0;
+            p.valid[ip] = 0;
+            continue;
+        }
+        act[q] = true;
+        ci[q] = shape<O>(xmid[q], 0).cell;
+        cj[q] = shape<O>(ymid[q], 0).cell;
+    }
+    const Win W = choose_window(ci, cj, act, S, ns > 0 ? WIN_DOUBLES / ns : 0, red);
+    const int cells = W.w * W.h;
+    for (int t = threadIdx.x; t < cells * ns; t += DT) win[t] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < DPPT; ++q) {
+        if (!act[q]) continue;
+        Sf sx[S], sy[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            sx[k] = shape<O>(xmid[q], k);
+            sy[k] = shape<O>(ymid[q], k);
+        }
+        const bool local = W.holds(ci[q], cj[q], S);
+#pragma unroll
+        for (int iy = 0; iy < S; ++iy) {
+#pragma unroll
+            for (int ix = 0; ix < S; ++ix) {
+                const int i = sx[ix].cell, j = sy[iy].cell;
+                if (!in_store(g, i, j)) continue;
+                const double cd = q_invvol[q] * sx[ix].s * sy[iy].s;
+                const double v[6] = {cd * vx_c[q], cd * vy_c[q], cd * (gamma_psi[q] - 1.0) * clight,
+                                     cd * gamma_psi[q], cd * charge_mu0_mass_ratio * psi_inv[q], cd};
+                if (local) {
+                    const int k = W.at(i, j);
+#pragma unroll
+                    for (int a = 0; a < 6; ++a)
+                        if (slot[a] >= 0) lds_add(&win[slot[a] * cells + k], v[a]);
+                } else {
+                    const long k = g.idx(i, j);
+#pragma unroll
+                    for (int a = 0; a < 6; ++a)
+                        if (slot[a] >= 0) atomic_add(store + long(comp[a]) * g.plane + k, v[a]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    flush_window(g, W, win, slot, comp, store);
+}
+
+// ::ExplicitDeposition deposit functor, src/particles/deposition/ExplicitDeposition.cpp:166-262
+template <int O, int T>
+__global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, const int32_t* __restrict__ order,
+                                                         double* __restrict__ store, int cBz, int cEz, int cExmBy,
+                                                         int cEypBx, int cSy, int cSx, double charge_invvol_mu0,
+                                                         double charge_mass_ratio) {
+    constexpr int S = dshape_width<T, O>();
+    __shared__ double win[WIN_DOUBLES];
+    __shared__ int red[7 * DT / 64];
+    const double clight = 1.0, clight_inv = 1.0 / 1.0;
+    double xmid[DPPT], ymid[DPPT], vx[DPPT], vy[DPPT], psi_inv[DPPT], cdm[DPPT];
+    int ci[DPPT], cj[DPPT];
+    bool act[DPPT];
+    const long base = long(blockIdx.x) * DCHUNK;
+#pragma unroll
+    for (int q = 0; q < DPPT; ++q) {
+        const long t = base + q * DT + threadIdx.x;
+        act[q] = false;
+        if (t >= p.n) continue;
+        const long ip = order ? long(order[t]) : t;
+        if (!p.valid[ip]) continue;
+        act[q] = true;
+        psi_inv[q] = 1.0 / p.psi[ip];
+        const double xp = p.x[ip], yp = p.y[ip];
+        vx[q] = p.ux[ip] * psi_inv[q] * clight_inv;
+        vy[q] = p.uy[ip] * psi_inv[q] * clight_inv;
+        cdm[q] = charge_invvol_mu0 * p.w[ip];
+        xmid[q] = (xp - g.xoff) * g.dxi;
+        ymid[q] = (yp - g.yoff) * g.dyi;
+        ci[q] = dshape<T, O>(xmid[q], 0).cell;
+        cj[q] = dshape<T, O>(ymid[q], 0).cell;
+    }
+    const Win W = choose_window(ci, cj, act, S, WIN_DOUBLES / 2, red);
+    const int cells = W.w * W.h;
+    for (int t = threadIdx.x; t < cells * 2; t += DT) win[t] = 0.0;
+    __syncthreads();
+    const double* Bz = store + long(cBz) * g.plane;
+    const double* Ez = store + long(cEz) * g.plane;
+    const double* ExmBy = store + long(cExmBy) * g.plane;
+    const double* EypBx = store + long(cEypBx) * g.plane;
+    double* Sy = store + long(cSy) * g.plane;
+    double* Sx = store + long(cSx) * g.plane;
+#pragma unroll
+    for (int q = 0; q < DPPT; ++q) {
+        if (!act[q]) continue;
+        const double q_mass_ratio = charge_mass_ratio;
+        const double charge_density_mu0 = cdm[q];
+        const double Aabssqp = 0.0;
+        const double gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * psi_inv[q] * psi_inv[q] + vx[q] * vx[q] +
+                                        vy[q] * vy[q] + 1.0);
+        Dsf sx[S], sy[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            sx[k] = dshape<T, O>(xmid[q], k);
+            sy[k] = dshape<T, O>(ymid[q], k);
+        }
+        const bool local = W.holds(ci[q], cj[q], S);
+        const double AabssqDxp = 0.0, AabssqDyp = 0.0;
+        const double pvx = vx[q], pvy = vy[q], pinv = psi_inv[q];
+#pragma unroll
+        for (int iy = 0; iy < S; ++iy) {
+#pragma unroll
+            for (int ix = 0; ix < S; ++ix) {
+                if (T == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) continue;
+                const Dsf X = sx[ix], Y = sy[iy];
+                if (!in_store(g, X.cell, Y.cell)) continue;
+                const long k = g.idx(X.cell, Y.cell);
+                const double Bz_v = Bz[k], Ez_v = Ez[k], ExmBy_v = ExmBy[k], EypBx_v = EypBx[k];
+                const double sy_add = charge_density_mu0 * (
+                    -X.s * Y.s * (
+                        -Bz_v * pvx
+                        + (Ez_v * pvy
+                        + ExmBy_v * (-pvx * pvy)
+                        + EypBx_v * (gamma_psi - pvy * pvy)) * clight_inv
+                        - 0.25 * AabssqDyp * q_mass_ratio * pinv
+                    ) * q_mass_ratio * pinv
+                    + (-X.ds * Y.s * g.dxi * (-pvx * pvy)
+                       - X.s * Y.ds * g.dyi * (gamma_psi - pvy * pvy - 1.0)) * clight);
+                const double sx_add = charge_density_mu0 * (
+                    +X.s * Y.s * (
+                        +Bz_v * pvy
+                        + (Ez_v * pvx
+                        + ExmBy_v * (gamma_psi - pvx * pvx)
+                        + EypBx_v * (-pvx * pvy)) * clight_inv
+                        - 0.25 * AabssqDxp * q_mass_ratio * pinv
+                    ) * q_mass_ratio * pinv
+                    + (+X.ds * Y.s * g.dxi * (gamma_psi - pvx * pvx - 1.0)
+                       + X.s * Y.ds * g.dyi * (-pvx * pvy)) * clight);
+                if (local) {
+                    const int kw = W.at(X.cell, Y.cell);
+                    lds_add(&win[kw], sy_add);
+                    lds_add(&win[cells + kw], sx_add);
+                } else {
+                    atomic_add(Sy + k, sy_add);
+                    atomic_add(Sx + k, sx_add);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int comp[2] = {cSy, cSx};
+    const int slot[2] = {0, 1};
+    flush_window(g, W, win, slot, comp, store);
+}
+
+// ::DepositCurrentSlice deposit functor, src/particles/deposition/BeamDepositCurrent.cpp:133-193
+template <int O>
+__global__ __launch_bounds__(DT) void k_deposit_beam(Geom g, hpq_beam b, long first, long count,
+                                                     double* __restrict__ store, int cjx, int cjy, int cjz) {
+    constexpr int S = O + 1;
+    __shared__ double win[WIN_DOUBLES];
+    __shared__ int red[7 * DT / 64];
+    const double clightsq = 1.0 / (1.0 * 1.0);
+    const int comp[3] = {cjx, cjy, cjz};
+    int slot[3];
+    const int ns = assign_slots(comp, slot);
+    double wqx[DPPT], wqy[DPPT], wqz[DPPT], xmid[DPPT], ymid[DPPT];
+    int ci[DPPT], cj[DPPT];
+    bool act[DPPT];
+    const long base = long(blockIdx.x) * DCHUNK;
+#pragma unroll
+    for (int q = 0; q < DPPT; ++q) {
+        const long t = base + q * DT + threadIdx.x;
+        act[q] = false;
+        if (t >= count) continue;
+        const long ip = first + t;
+        if (!b.valid[ip]) continue;
+        act[q] = true;
+        const double ux = b.ux[ip], uy = b.uy[ip], uz = b.uz[ip];
+        const double gaminv = 1.0 / sqrt(1.0 + ux * ux * clightsq + uy * uy * clightsq + uz * uz * clightsq);
+        const double wq = b.charge * b.w[ip] * 1.0;
+        wqx[q] = wq * (ux * gaminv);
+        wqy[q] = wq * (uy * gaminv);
+        wqz[q] = wq * (uz * gaminv);
+        xmid[q] = (b.x[ip] - g.xoff) * g.dxi;
+        ymid[q] = (b.y[ip] - g.yoff) * g.dyi;
+        ci[q] = shape<O>(xmid[q], 0).cell;
+        cj[q] = shape<O>(ymid[q], 0).cell;
+    }
+    const Win W = choose_window(ci, cj, act, S, ns > 0 ? WIN_DOUBLES / ns : 0, red);
+    const int cells = W.w * W.h;
+    for (int t = threadIdx.x; t < cells * ns; t += DT) win[t] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < DPPT; ++q) {
+        if (!act[q]) continue;
+        Sf sx[S], sy[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            sx[k] = shape<O>(xmid[q], k);
+            sy[k] = shape<O>(ymid[q], k);
+        }
+        const bool local = W.holds(ci[q], cj[q], S);
+#pragma unroll
+        for (int iy = 0; iy < S; ++iy) {
+#pragma unroll
+            for (int ix = 0; ix < S; ++ix) {
+                const int i = sx[ix].cell, j = sy[iy].cell;
+                if (!in_store(g, i, j)) continue;
+                const double s = sx[ix].s * sy[iy].s;
+                const double v[3] = {s * wqx[q], s * wqy[q], s * wqz[q]};
+                if (local) {
+                    const int k = W.at(i, j);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        if (slot[a] >= 0) lds_add(&win[slot[a] * cells + k], v[a]);
+                } else {
+                    const long k = g.idx(i, j);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        if (slot[a] >= 0) atomic_add(store + long(comp[a]) * g.plane + k, v[a]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    flush_window(g, W, win, slot, comp, store);
+}
+
+}  // namespace
+
+void deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32_t* comps, bool ions) {
+    if (p->n <= 0) return;
+    const Geom& g = c->g;
+    const double charge = ions ? -c->p.plasma_charge : c->p.plasma_charge;
+    const double mu0 = 1.0;
+    const double invvol = g.dx * g.dy * g.dxi * g.dyi;
+    const double charge_invvol = charge * invvol;
+    const double cmr = charge * mu0 / c->p.plasma_mass;
+    auto cm = [&](int k) { return comps[k] >= 0 ? comps[k] : -1; };
+    if ((cm(0) >= 0) != (cm(1) >= 0)) throw HpqError(HPQ_E_INVALID, "deposit_current: jx and jy go together");
+    // the first deposition of every slice counts towards the re-sort period
+    if (!ions && c->p.bin_period > 0 && ++c->bins.calls >= c->p.bin_period) {
+        c->bins.calls = 0;
+        c->bins.stale = true;
+    }
+    const int32_t* order = deposit_order(c, p);
+    TraceScope ts(c, HPQ_TRACE_DEPOSIT);
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        hipLaunchKernelGGL((k_deposit_plasma<decltype(O)::value>), grid1d(p->n, DCHUNK), dim3(DT), 0, c->stream, g,
+                           *p, order, store, cm(0), cm(1), cm(2), cm(3), cm(4), cm(5), charge_invvol, cmr,
+                           c->p.max_qsa, c->d_count);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+void explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store) {
+    if (p->n <= 0) return;
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    const double mu0 = 1.0;
+    const double invvol = g.dx * g.dy * g.dxi * g.dyi;
+    const double qimu0 = c->p.plasma_charge * invvol * mu0;
+    const double qmr = c->p.plasma_charge / c->p.plasma_mass;
+    const int T = c->p.derivative_type;
+    if (T < 0 || T > 2) throw HpqError(HPQ_E_UNSUPPORTED, "depos_derivative_type must be 0..2");
+    const int32_t* order = deposit_order(c, p);
+    TraceScope ts(c, HPQ_TRACE_EXPLICIT_DEPOSIT);
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        constexpr int o = decltype(O)::value;
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid1d(p->n, DCHUNK), dim3(DT), 0, c->stream, g, *p, order, store, L.Bz, L.Ez,
+                               L.ExmBy, L.EypBx, L.Sy, L.Sx, qimu0, qmr);
+        };
+        if (T == 0) launch(k_explicit_deposit<o, 0>);
+        else if (T == 1) launch(k_explicit_deposit<o, 1>);
+        else launch(k_explicit_deposit<o, 2>);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+void deposit_beam(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const int32_t* comps) {
+    if (!b || slice < 0 || slice >= c->g.nz || !b->off) return;
+    const long first = b->off[slice];
+    const long count = b->off[slice + 1] - first;
+    if (count <= 0) return;
+    if ((comps[0] >= 0) != (comps[1] >= 0))
+        throw HpqError(HPQ_E_INVALID, "deposit_current_slice: jx and jy go together");
+    TraceScope ts(c, HPQ_TRACE_BEAM_DEPOSIT);
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        hipLaunchKernelGGL((k_deposit_beam<decltype(O)::value>), grid1d(count, DCHUNK), dim3(DT), 0, c->stream, c->g,
+                           *b, first, count, store, comps[0], comps[1], comps[2]);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+}  // namespace hpq

@@ -366,6 +366,7 @@ void hpq_destroy(hpq_ctx* c) {
     }
     poisson_free(c);
     mg_free(c);
+    bins_free(c);
     if (c->d_count) (void)hipFree(c->d_count);
     if (c->h_count) (void)hipHostFree(c->h_count);
     if (c->d_red) (void)hipFree(c->d_red);
@@ -505,6 +506,16 @@ int hpq_init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma
         check_plasma(p);
         if (prof->ppc[0] < 1 || prof->ppc[1] < 1) throw HpqError(HPQ_E_INVALID, "ppc must be >= 1");
         init_plasma(c, prof, p);
+        c->bins.stale = true;
+    });
+}
+
+int hpq_bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile_size, int32_t* perm, int32_t* offsets) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_plasma(p);
+        if (!perm && p->n > 0) throw HpqError(HPQ_E_INVALID, "perm is null");
+        bin_tiles(c, p, tile_size, perm, offsets);
     });
 }
 
@@ -516,6 +527,7 @@ int hpq_begin_step(hpq_ctx* c, const hpq_plasma* p, double* store) {
         HPQ_HIP(hipMemsetAsync(store, 0, sizeof(double) * c->L.ncomp * c->g.plane, c->stream));
         HPQ_HIP(hipMemsetAsync(c->d_count, 0, sizeof(int) * 4, c->stream));
         for (double& s : c->stats) s = 0;
+        c->bins.stale = true;
         if (c->L.ions_rhomjz >= 0) DepositCurrent(c, p, store, -1, -1, -1, -1, -1, c->L.ions_rhomjz, true);
     });
 }

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hpq.h"
@@ -84,22 +85,58 @@ struct MGLevel {
     double *acf = nullptr, *res = nullptr, *cor = nullptr, *rescor = nullptr;
 };
 
+// Convergence state of one solve, kept on the device so V-cycles can be enqueued without
+// a host round trip per cycle; max norms are accumulated as the bit patterns of
+// non-negative doubles (their integer order is the numeric order).
+struct MGCtl {
+    int done = 0;
+    int iters = 0;
+    int status = 0;
+    int maxiter = 0;
+    double max_norm = 0;
+    double res_target = 0;
+    unsigned long long norm_bits = 0;   // max |residual| of the current V-cycle
+    unsigned long long res0_bits = 0;   // max |residual| of the initial guess
+    unsigned long long rhs0_bits = 0;   // max |rhs|
+};
+
 struct MultiGrid {
     bool cc = true;
     int max_level = 0;
     int coarse_begin = 0;   // first level handled by the single-workgroup coarse kernel
+    bool coarse_in_lds = false;
+    size_t coarse_lds_bytes = 0;
+    int last_iters = 1;     // V-cycles of the previous solve: first batch size of the next
     double dx = 0, dy = 0;
     std::vector<MGLevel> lev;
-    MGLevel* d_lev = nullptr;        // device copy of lev (for the coarse kernel)
-    double* msol = nullptr;          // level-0 solution (2 comps)
+    alignas(16) unsigned char coarse_args[2048];   // kernel argument block of the coarse kernel
+    double* coarse_work = nullptr;   // coarse-level arrays when they do not fit in LDS
+    double* msol = nullptr;          // level-0 iterate after odd V-cycles / loaded guess (2 comps)
     double* mrhs = nullptr;          // level-0 right-hand side (2 comps)
-    unsigned long long* d_norm = nullptr;  // 2 max-norm accumulators (bit patterns)
-    unsigned long long* h_norm = nullptr;  // pinned host mirror
+    MGCtl* d_ctl = nullptr;
+    MGCtl* h_ctl = nullptr;          // pinned host mirror
 };
 
 struct Capture {
     int comp;
     double* dst;
+};
+
+// Tile sort of the plasma for deposition (bins.hip): scratch of the counting sort and the
+// permutation the deposition kernels walk.
+constexpr int DEPOSIT_TILE = 16;
+struct Bins {
+    int32_t* keys = nullptr;
+    long keys_cap = 0;
+    int32_t* counts = nullptr;   // (bin, chunk) counts, then their exclusive scan
+    long counts_cap = 0;
+    int32_t* tmp = nullptr;      // block sums of the scan
+    long tmp_cap = 0;
+    int32_t* perm = nullptr;     // deposition order
+    long perm_cap = 0;
+    long perm_n = -1;
+    bool stale = true;           // re-sort before the next deposition
+    int calls = 0;               // plasma depositions since the last re-sort
 };
 
 // Event pairs around the launches of one kernel family (HPQ_TRACE_*).
@@ -118,6 +155,7 @@ struct hpq_ctx {
     hipStream_t stream = nullptr;
     hpq::Poisson ps;
     hpq::MultiGrid mg;
+    hpq::Bins bins;
     int* d_count = nullptr;          // QSA counter
     int* h_count = nullptr;          // pinned
     double* d_red = nullptr;         // reduction scratch (8 doubles)
@@ -156,8 +194,23 @@ void diag_accumulate(hpq_ctx* c, const double* store, int islice);
 double rel_b_error(hpq_ctx* c, const double* store, int bx, int by, int bxi, int byi);
 void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p);
 int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof);
+void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets);
+const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p);
+void bins_free(hpq_ctx* c);
 
 inline dim3 grid1d(long n, int block) { return dim3(unsigned((n + block - 1) / block)); }
+
+// calls f(std::integral_constant<int, O>) for the run-time shape order O
+template <class F>
+void dispatch_order(int order, F&& f) {
+    switch (order) {
+        case 0: f(std::integral_constant<int, 0>{}); break;
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        default: throw HpqError(HPQ_E_UNSUPPORTED, "depos_order_xy must be 0..3");
+    }
+}
 
 // Records HIP events around the launches in its scope when tracing of `kind` is on.
 class TraceScope {

@@ -1,11 +1,9 @@
-// Particle kernels on gfx950: plasma current/chi deposition, explicit Sx/Sy deposition,
-// beam current deposition, field gather + leapfrog push, plasma injection.
+// Particle kernels on gfx950: field gather + leapfrog push of the plasma, plasma injection.
+// (Deposition is in deposit.hip.)
 //
 // One thread per particle over coalesced SoA arrays; stencils unrolled at compile time
-// (shape order and derivative type are template parameters); deposits go to the slice
-// store with hardware FP64 atomics (global_atomic_add_f64, -munsafe-fp-atomics on
-// coarse-grained device memory). Every expression keeps the reference's operation order,
-// so FP64 results equal the reference CPU build's up to the order of the atomic sums.
+// (shape order is a template parameter). Every expression keeps the reference's operation
+// order, so FP64 results equal the reference CPU build's bit for bit.
 #include <cmath>
 
 #include "internal.hpp"
@@ -15,168 +13,8 @@ namespace hpq {
 
 namespace {
 
-__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
-
 __device__ __forceinline__ bool in_store(const Geom& g, int i, int j) {
     return i >= -g.ng && i <= g.nx - 1 + g.ng && j >= -g.ng && j <= g.ny - 1 + g.ng;
-}
-
-// ::DepositCurrent deposit functor, src/particles/deposition/PlasmaDepositCurrent.cpp:182-251
-template <int O>
-__global__ __launch_bounds__(256) void k_deposit_plasma(Geom g, hpq_plasma p, double* __restrict__ store, int cjx,
-                                                        int cjy, int cjz, int crho, int cchi, int crm,
-                                                        double charge_invvol, double charge_mu0_mass_ratio,
-                                                        double max_qsa, int* __restrict__ qsa_count) {
-    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (ip >= p.n || !p.valid[ip]) return;
-    const double clight = 1.0, clightinv = 1.0 / 1.0;
-    const double psi_inv = 1.0 / p.psi[ip];
-    const double xp = p.x[ip], yp = p.y[ip];
-    const double vx_c = p.ux[ip] * psi_inv;
-    const double vy_c = p.uy[ip] * psi_inv;
-    const double q_invvol = charge_invvol * p.w[ip];
-    const double xmid = (xp - g.xoff) * g.dxi;
-    const double ymid = (yp - g.yoff) * g.dyi;
-    const double Aabssqp = 0.0;
-    const double gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * psi_inv * psi_inv + vx_c * vx_c * clightinv * clightinv +
-                                    vy_c * vy_c * clightinv * clightinv + 1.0);
-    if (gamma_psi < 0.0 || gamma_psi > max_qsa || psi_inv < 0.0) {
-        atomicAdd(qsa_count, 1);
-        p.w[ip] = 0.0;
-        p.valid[ip] = 0;
-        return;
-    }
-    Sf sx[O + 1], sy[O + 1];
-#pragma unroll
-    for (int k = 0; k <= O; ++k) {
-        sx[k] = shape<O>(xmid, k);
-        sy[k] = shape<O>(ymid, k);
-    }
-    double* pjx = store + long(cjx) * g.plane;
-    double* pjy = store + long(cjy) * g.plane;
-    double* pjz = store + long(cjz) * g.plane;
-    double* prho = store + long(crho) * g.plane;
-    double* pchi = store + long(cchi) * g.plane;
-    double* prm = store + long(crm) * g.plane;
-#pragma unroll
-    for (int iy = 0; iy <= O; ++iy) {
-#pragma unroll
-        for (int ix = 0; ix <= O; ++ix) {
-            if (!in_store(g, sx[ix].cell, sy[iy].cell)) continue;
-            const long k = g.idx(sx[ix].cell, sy[iy].cell);
-            const double cd = q_invvol * sx[ix].s * sy[iy].s;
-            if (cjx >= 0) {
-                atomic_add(pjx + k, cd * vx_c);
-                atomic_add(pjy + k, cd * vy_c);
-            }
-            if (cjz >= 0) atomic_add(pjz + k, cd * (gamma_psi - 1.0) * clight);
-            if (crho >= 0) atomic_add(prho + k, cd * gamma_psi);
-            if (cchi >= 0) atomic_add(pchi + k, cd * charge_mu0_mass_ratio * psi_inv);
-            if (crm >= 0) atomic_add(prm + k, cd);
-        }
-    }
-}
-
-// ::ExplicitDeposition deposit functor, src/particles/deposition/ExplicitDeposition.cpp:166-262
-template <int O, int T>
-__global__ __launch_bounds__(256) void k_explicit_deposit(Geom g, hpq_plasma p, double* __restrict__ store, int cBz,
-                                                          int cEz, int cExmBy, int cEypBx, int cSy, int cSx,
-                                                          double charge_invvol_mu0, double charge_mass_ratio) {
-    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (ip >= p.n || !p.valid[ip]) return;
-    const double clight = 1.0, clight_inv = 1.0 / 1.0;
-    const double psi_inv = 1.0 / p.psi[ip];
-    const double xp = p.x[ip], yp = p.y[ip];
-    const double vx = p.ux[ip] * psi_inv * clight_inv;
-    const double vy = p.uy[ip] * psi_inv * clight_inv;
-    const double q_mass_ratio = charge_mass_ratio;
-    const double charge_density_mu0 = charge_invvol_mu0 * p.w[ip];
-    const double xmid = (xp - g.xoff) * g.dxi;
-    const double ymid = (yp - g.yoff) * g.dyi;
-    const double Aabssqp = 0.0;
-    const double gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * psi_inv * psi_inv + vx * vx + vy * vy + 1.0);
-    constexpr int W = dshape_width<T, O>();
-    Dsf sx[W], sy[W];
-#pragma unroll
-    for (int k = 0; k < W; ++k) {
-        sx[k] = dshape<T, O>(xmid, k);
-        sy[k] = dshape<T, O>(ymid, k);
-    }
-    const double* Bz = store + long(cBz) * g.plane;
-    const double* Ez = store + long(cEz) * g.plane;
-    const double* ExmBy = store + long(cExmBy) * g.plane;
-    const double* EypBx = store + long(cEypBx) * g.plane;
-    double* Sy = store + long(cSy) * g.plane;
-    double* Sx = store + long(cSx) * g.plane;
-    const double AabssqDxp = 0.0, AabssqDyp = 0.0;
-#pragma unroll
-    for (int iy = 0; iy < W; ++iy) {
-#pragma unroll
-        for (int ix = 0; ix < W; ++ix) {
-            if (T == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) continue;
-            const Dsf X = sx[ix], Y = sy[iy];
-            if (!in_store(g, X.cell, Y.cell)) continue;
-            const long k = g.idx(X.cell, Y.cell);
-            const double Bz_v = Bz[k], Ez_v = Ez[k], ExmBy_v = ExmBy[k], EypBx_v = EypBx[k];
-            atomic_add(Sy + k, charge_density_mu0 * (
-                -X.s * Y.s * (
-                    -Bz_v * vx
-                    + (Ez_v * vy
-                    + ExmBy_v * (-vx * vy)
-                    + EypBx_v * (gamma_psi - vy * vy)) * clight_inv
-                    - 0.25 * AabssqDyp * q_mass_ratio * psi_inv
-                ) * q_mass_ratio * psi_inv
-                + (-X.ds * Y.s * g.dxi * (-vx * vy)
-                   - X.s * Y.ds * g.dyi * (gamma_psi - vy * vy - 1.0)) * clight));
-            atomic_add(Sx + k, charge_density_mu0 * (
-                +X.s * Y.s * (
-                    +Bz_v * vy
-                    + (Ez_v * vx
-                    + ExmBy_v * (gamma_psi - vx * vx)
-                    + EypBx_v * (-vx * vy)) * clight_inv
-                    - 0.25 * AabssqDxp * q_mass_ratio * psi_inv
-                ) * q_mass_ratio * psi_inv
-                + (+X.ds * Y.s * g.dxi * (gamma_psi - vx * vx - 1.0)
-                   + X.s * Y.ds * g.dyi * (-vx * vy)) * clight));
-        }
-    }
-}
-
-// ::DepositCurrentSlice deposit functor, src/particles/deposition/BeamDepositCurrent.cpp:133-193
-template <int O>
-__global__ __launch_bounds__(256) void k_deposit_beam(Geom g, hpq_beam b, long first, long count,
-                                                      double* __restrict__ store, int cjx, int cjy, int cjz) {
-    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const long ip = first + t;
-    if (!b.valid[ip]) return;
-    const double clightsq = 1.0 / (1.0 * 1.0);
-    const double ux = b.ux[ip], uy = b.uy[ip], uz = b.uz[ip];
-    const double gaminv = 1.0 / sqrt(1.0 + ux * ux * clightsq + uy * uy * clightsq + uz * uz * clightsq);
-    const double wq = b.charge * b.w[ip] * 1.0;
-    const double wqx = wq * (ux * gaminv), wqy = wq * (uy * gaminv), wqz = wq * (uz * gaminv);
-    const double xmid = (b.x[ip] - g.xoff) * g.dxi;
-    const double ymid = (b.y[ip] - g.yoff) * g.dyi;
-    Sf sx[O + 1], sy[O + 1];
-#pragma unroll
-    for (int k = 0; k <= O; ++k) {
-        sx[k] = shape<O>(xmid, k);
-        sy[k] = shape<O>(ymid, k);
-    }
-#pragma unroll
-    for (int iy = 0; iy <= O; ++iy) {
-#pragma unroll
-        for (int ix = 0; ix <= O; ++ix) {
-            if (!in_store(g, sx[ix].cell, sy[iy].cell)) continue;
-            const long k = g.idx(sx[ix].cell, sy[iy].cell);
-            const double s = sx[ix].s * sy[iy].s;
-            if (cjx >= 0) {
-                atomic_add(store + long(cjx) * g.plane + k, s * wqx);
-                atomic_add(store + long(cjy) * g.plane + k, s * wqy);
-            }
-            if (cjz >= 0) atomic_add(store + long(cjz) * g.plane + k, s * wqz);
-        }
-    }
 }
 
 // Forward-mode dual number for the second-order Taylor step (src/utils/DualNumbers.H:13-43)
@@ -310,75 +148,7 @@ __global__ __launch_bounds__(256) void k_push_plasma(Geom g, hpq_plasma p, const
     p.psi[ip] = psi;
 }
 
-template <class F>
-void dispatch_order(int order, F&& f) {
-    switch (order) {
-        case 0: f(std::integral_constant<int, 0>{}); break;
-        case 1: f(std::integral_constant<int, 1>{}); break;
-        case 2: f(std::integral_constant<int, 2>{}); break;
-        case 3: f(std::integral_constant<int, 3>{}); break;
-        default: throw HpqError(HPQ_E_UNSUPPORTED, "depos_order_xy must be 0..3");
-    }
-}
-
 }  // namespace
-
-void deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32_t* comps, bool ions) {
-    if (p->n <= 0) return;
-    const Geom& g = c->g;
-    const double charge = ions ? -c->p.plasma_charge : c->p.plasma_charge;
-    const double mu0 = 1.0;
-    const double invvol = g.dx * g.dy * g.dxi * g.dyi;
-    const double charge_invvol = charge * invvol;
-    const double cmr = charge * mu0 / c->p.plasma_mass;
-    const int zero = 0;
-    (void)zero;
-    auto cm = [&](int k) { return comps[k] >= 0 ? comps[k] : -1; };
-    TraceScope ts(c, HPQ_TRACE_DEPOSIT);
-    dispatch_order(c->p.depos_order, [&](auto O) {
-        hipLaunchKernelGGL((k_deposit_plasma<decltype(O)::value>), grid1d(p->n, 256), dim3(256), 0, c->stream, g, *p,
-                           store, cm(0), cm(1), cm(2), cm(3), cm(4), cm(5), charge_invvol, cmr, c->p.max_qsa,
-                           c->d_count);
-    });
-    HPQ_HIP(hipGetLastError());
-}
-
-void explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store) {
-    if (p->n <= 0) return;
-    const Geom& g = c->g;
-    const Layout& L = c->L;
-    const double mu0 = 1.0;
-    const double invvol = g.dx * g.dy * g.dxi * g.dyi;
-    const double qimu0 = c->p.plasma_charge * invvol * mu0;
-    const double qmr = c->p.plasma_charge / c->p.plasma_mass;
-    const int T = c->p.derivative_type;
-    if (T < 0 || T > 2) throw HpqError(HPQ_E_UNSUPPORTED, "depos_derivative_type must be 0..2");
-    TraceScope ts(c, HPQ_TRACE_EXPLICIT_DEPOSIT);
-    dispatch_order(c->p.depos_order, [&](auto O) {
-        constexpr int o = decltype(O)::value;
-        auto launch = [&](auto kern) {
-            hipLaunchKernelGGL(kern, grid1d(p->n, 256), dim3(256), 0, c->stream, g, *p, store, L.Bz, L.Ez, L.ExmBy,
-                               L.EypBx, L.Sy, L.Sx, qimu0, qmr);
-        };
-        if (T == 0) launch(k_explicit_deposit<o, 0>);
-        else if (T == 1) launch(k_explicit_deposit<o, 1>);
-        else launch(k_explicit_deposit<o, 2>);
-    });
-    HPQ_HIP(hipGetLastError());
-}
-
-void deposit_beam(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const int32_t* comps) {
-    if (!b || slice < 0 || slice >= c->g.nz || !b->off) return;
-    const long first = b->off[slice];
-    const long count = b->off[slice + 1] - first;
-    if (count <= 0) return;
-    TraceScope ts(c, HPQ_TRACE_BEAM_DEPOSIT);
-    dispatch_order(c->p.depos_order, [&](auto O) {
-        hipLaunchKernelGGL((k_deposit_beam<decltype(O)::value>), grid1d(count, 256), dim3(256), 0, c->stream, c->g, *b,
-                           first, count, store, comps[0], comps[1], comps[2]);
-    });
-    HPQ_HIP(hipGetLastError());
-}
 
 void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool temp_slice) {
     if (p->n <= 0) return;

@@ -65,6 +65,9 @@ class SliceConfig:
     plasma_mass: float = 1.0
     max_qsa: float = 35.0
     neutralize: bool = True
+    # plasma deposition walks the particles sorted by 16x16-cell tiles, re-sorted every
+    # bin_period slices (0: index order); only the order of floating-point sums depends on it
+    bin_period: int = 4
 
     @property
     def dx(self) -> float:

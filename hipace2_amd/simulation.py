@@ -36,6 +36,7 @@ def make_params(cfg: SliceConfig) -> _abi.Params:
     p.plo[0], p.plo[1], p.phi[0], p.phi[1] = plx, ply, phx, phy
     p.pc_tol, p.pc_maxiter, p.pc_mix = cfg.pc_tol, cfg.pc_maxiter, cfg.pc_mix
     p.plasma_charge, p.plasma_mass, p.max_qsa = cfg.plasma_charge, cfg.plasma_mass, cfg.max_qsa
+    p.bin_period = cfg.bin_period
     return p
 
 
@@ -216,6 +217,17 @@ class Simulation:
                                   C.c_void_p(base + 8 * plane * self.layout["chi"]), plane, tol_rel, tol_abs,
                                   maxiter, C.byref(it))
         return rc, it.value
+
+    def bin_tiles(self, tile: int):
+        """hpq_bin_tiles on the current plasma: (perm, offsets) as int32 device tensors."""
+        n = self.np
+        ntx = (self.cfg.nx + 4 + tile - 1) // tile
+        nty = (self.cfg.ny + 4 + tile - 1) // tile
+        perm = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
+        offsets = torch.empty(ntx * nty + 2, dtype=torch.int32, device=self.device)
+        _abi.check(self.ctx, self.L.hpq_bin_tiles(self.ctx, C.byref(self._pdesc), tile,
+                                                  C.c_void_p(perm.data_ptr()), C.c_void_p(offsets.data_ptr())))
+        return perm[:n], offsets
 
     def poisson_dirichlet(self, staging: torch.Tensor, lhs: torch.Tensor):
         assert staging.is_contiguous() and staging.numel() == self.cfg.nx * self.cfg.ny

@@ -56,6 +56,11 @@ typedef struct hpq_params {
     double pc_mix;                       /* hipace.predcorr_B_mixing_factor */
     double plasma_charge, plasma_mass;   /* species charge and mass */
     double max_qsa;                      /* plasma.max_qsa_weighting_factor */
+    int bin_period;                      /* plasma deposition walks the particles sorted by
+                                            16x16-cell tiles, re-sorted every bin_period
+                                            slices (plasmas.reorder_period plays this role
+                                            in the reference); 0: index order. Changes only
+                                            the order of floating-point sums. */
 } hpq_params;
 
 /* Plasma species, SoA device arrays in PlasmaIdx order
@@ -152,6 +157,15 @@ int64_t hpq_plasma_count(const hpq_ctx *ctx, const hpq_plasma_profile *prof);
 /* PlasmaParticleContainer::InitParticles (PlasmaParticleContainerInit.cpp:18-378): fills
  * p (capacity p->n must equal hpq_plasma_count) with the lattice in GPU order. */
 int hpq_init_plasma(hpq_ctx *ctx, const hpq_plasma_profile *prof, const hpq_plasma *p);
+
+/* Tile binning of the plasma: the DenseBins of the CPU tiled deposition,
+ * src/particles/deposition/DepositionUtil.H:213-227 (key of get_start_cell,
+ * PlasmaDepositCurrent.cpp:137-155). perm[p->n] (device) = particle indices sorted stably
+ * by key = ((cx+2)/T)*ntile_y + (cy+2)/T, where (cx, cy) is the lowest cell of the particle's
+ * order-depos_order stencil and ntile_d = ceil((n_cell_d + 4)/T); invalid particles take
+ * key ntile_x*ntile_y. offsets (device, may be NULL) gets ntile_x*ntile_y + 2 entries: the
+ * first slot of every bin, then p->n. At most 16383 tiles. */
+int hpq_bin_tiles(hpq_ctx *ctx, const hpq_plasma *p, int tile_size, int32_t *perm, int32_t *offsets);
 
 /* Start of a time step (src/Hipace.cpp:403-471): zero the slice store and deposit the
  * neutralising ion background of p (MultiPlasma::DepositNeutralizingBackground). */

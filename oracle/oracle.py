@@ -263,3 +263,59 @@ def dshape(dtype, order, xmid, ix):
     out = np.zeros(3)
     lib().orc_dshape(dtype, order, xmid, ix, _dp(out))
     return out[0], out[1], int(out[2])
+
+
+# ---- tile binning (integer work, numpy) ----------------------------------------------
+
+def start_cell(order: int, xmid: np.ndarray) -> np.ndarray:
+    """Lowest cell of the order-`order` deposition stencil: the cell of index 0 of
+    compute_single_shape_factor (src/particles/deposition/ShapeFactors.H:122-195), i.e.
+    floor(xmid + 0.5) - 1 for order 2, floor(xmid) - 1 for order 3, floor(xmid) for order 1,
+    floor(xmid + 0.5) for order 0."""
+    xmid = np.asarray(xmid, dtype=np.float64)
+    if order == 0:
+        return np.floor(xmid + 0.5).astype(np.int64)
+    if order == 1:
+        return np.floor(xmid).astype(np.int64)
+    if order == 2:
+        return np.floor(xmid + 0.5).astype(np.int64) - 1
+    return np.floor(xmid).astype(np.int64) - 1
+
+
+def tile_keys(cfg, x, y, valid, tile: int):
+    """Bin key of every particle for the CPU tiled deposition: get_start_cell
+    (PlasmaDepositCurrent.cpp:137-155) of the particle, then
+    ((cx - lo) / T) * ntile_y + (cy - lo) / T with lo = -2 (the first guard cell) and C
+    integer division; invalid particles get ntile_x * ntile_y
+    (src/particles/deposition/DepositionUtil.H:213-227). Returns (keys, nbins)."""
+    dx = (cfg.xhi - cfg.xlo) / cfg.nx
+    dy = (cfg.yhi - cfg.ylo) / cfg.ny
+    xoff = 0.5 * (cfg.xlo + cfg.xhi - dx * (cfg.nx - 1))   # GetPosOffset, Fields.H:72-77
+    yoff = 0.5 * (cfg.ylo + cfg.yhi - dy * (cfg.ny - 1))
+    xmid = (np.asarray(x, dtype=np.float64) - xoff) * (1.0 / dx)
+    ymid = (np.asarray(y, dtype=np.float64) - yoff) * (1.0 / dy)
+    ntx = (cfg.nx + 4 + tile - 1) // tile
+    nty = (cfg.ny + 4 + tile - 1) // tile
+    cx = start_cell(cfg.depos_order, xmid) + 2
+    cy = start_cell(cfg.depos_order, ymid) + 2
+    tx = np.clip(np.trunc(cx / tile).astype(np.int64), 0, ntx - 1)
+    ty = np.clip(np.trunc(cy / tile).astype(np.int64), 0, nty - 1)
+    keys = tx * nty + ty
+    keys[np.asarray(valid) == 0] = ntx * nty
+    return keys, ntx * nty + 1
+
+
+def dense_bins(keys: np.ndarray, nbins: int):
+    """amrex::DenseBins::build as a counting sort: histogram of the keys, exclusive scan,
+    then every particle, in index order, goes to the next free slot of its bin.
+    Returns (perm, offsets[nbins + 1])."""
+    keys = np.asarray(keys, dtype=np.int64)
+    counts = np.bincount(keys, minlength=nbins)
+    offsets = np.zeros(nbins + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(counts)
+    nxt = offsets[:-1].copy()
+    perm = np.empty(keys.size, dtype=np.int64)
+    for ip, k in enumerate(keys.tolist()):
+        perm[nxt[k]] = ip
+        nxt[k] += 1
+    return perm, offsets

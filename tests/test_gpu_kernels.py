@@ -39,10 +39,17 @@ def _pair(cfg, n_particles=4000, seed=0, store_seed=1):
     return sim, orc
 
 
+# (bin_period, particles): index order; tile order with chunks wider than an LDS window
+# (a mean-centred window plus direct atomics); dense tile order (whole chunks in LDS)
+DEPOSIT_MODES = [(0, 4000), (1, 4000), (4, 60000)]
+
+
+@pytest.mark.parametrize("mode", DEPOSIT_MODES)
 @pytest.mark.parametrize("order", [0, 1, 2, 3])
-def test_deposit_current(order):
+def test_deposit_current(order, mode):
     cfg = small_config(order=order, dtype=2 if order > 0 else 1)
-    sim, orc = _pair(cfg)
+    cfg.bin_period, n = mode
+    sim, orc = _pair(cfg, n_particles=n)
     comps = ["jx", "jy", -1, -1, "chi", "rhomjz"]
     for name in ("jx", "jy", "chi", "rhomjz"):
         sim.store[sim.layout[name]].zero_()
@@ -81,9 +88,11 @@ def test_deposit_current_qsa_removal():
 
 
 @pytest.mark.parametrize("order,dtype", [(1, 2), (2, 2), (2, 1), (2, 0), (3, 2), (0, 1)])
-def test_explicit_deposition(order, dtype):
+@pytest.mark.parametrize("mode", DEPOSIT_MODES)
+def test_explicit_deposition(order, dtype, mode):
     cfg = small_config(order=order, dtype=dtype)
-    sim, orc = _pair(cfg)
+    cfg.bin_period, n = mode
+    sim, orc = _pair(cfg, n_particles=n)
     sim.explicit_deposition()
     orc.explicit_deposit()
     g, o = sim.store_host(), orc.store()
