@@ -188,8 +188,8 @@ void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t
         HPQ_HIP(hipGetLastError());
         return;
     }
-    // at most 4096 chunks of at least 4096 particles
-    const long chunk = std::max(4096L, (n + 4095) / 4096);
+    // at most 4096 chunks of at least 2048 particles (one wavefront walks a chunk)
+    const long chunk = std::max(2048L, (n + 4095) / 4096);
     const int nchunks = int((n + chunk - 1) / chunk);
     Bins& B = c->bins;
     grow(B.keys, B.keys_cap, n);

@@ -237,16 +237,28 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
         ci[q] = dshape<T, O>(xmid[q], 0).cell;
         cj[q] = dshape<T, O>(ymid[q], 0).cell;
     }
-    const Win W = choose_window(ci, cj, act, S, WIN_DOUBLES / 2, red);
+    // window planes: Sy, Sx (accumulated), then copies of Bz, Ez, ExmBy, EypBx
+    const Win W = choose_window(ci, cj, act, S, WIN_DOUBLES / 6, red);
     const int cells = W.w * W.h;
-    for (int t = threadIdx.x; t < cells * 2; t += DT) win[t] = 0.0;
-    __syncthreads();
     const double* Bz = store + long(cBz) * g.plane;
     const double* Ez = store + long(cEz) * g.plane;
     const double* ExmBy = store + long(cExmBy) * g.plane;
     const double* EypBx = store + long(cEypBx) * g.plane;
     double* Sy = store + long(cSy) * g.plane;
     double* Sx = store + long(cSx) * g.plane;
+    for (int t = threadIdx.x; t < cells; t += DT) {
+        const int j = W.j0 + t / W.w, i = W.i0 + t - (t / W.w) * W.w;
+        win[t] = 0.0;
+        win[cells + t] = 0.0;
+        if (in_store(g, i, j)) {
+            const long k = g.idx(i, j);
+            win[2 * cells + t] = Bz[k];
+            win[3 * cells + t] = Ez[k];
+            win[4 * cells + t] = ExmBy[k];
+            win[5 * cells + t] = EypBx[k];
+        }
+    }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < DPPT; ++q) {
         if (!act[q]) continue;
@@ -272,7 +284,14 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
                 const Dsf X = sx[ix], Y = sy[iy];
                 if (!in_store(g, X.cell, Y.cell)) continue;
                 const long k = g.idx(X.cell, Y.cell);
-                const double Bz_v = Bz[k], Ez_v = Ez[k], ExmBy_v = ExmBy[k], EypBx_v = EypBx[k];
+                const int kw = local ? W.at(X.cell, Y.cell) : 0;
+                double Bz_v, Ez_v, ExmBy_v, EypBx_v;
+                if (local) {
+                    Bz_v = win[2 * cells + kw]; Ez_v = win[3 * cells + kw];
+                    ExmBy_v = win[4 * cells + kw]; EypBx_v = win[5 * cells + kw];
+                } else {
+                    Bz_v = Bz[k]; Ez_v = Ez[k]; ExmBy_v = ExmBy[k]; EypBx_v = EypBx[k];
+                }
                 const double sy_add = charge_density_mu0 * (
                     -X.s * Y.s * (
                         -Bz_v * pvx
@@ -294,7 +313,6 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
                     + (+X.ds * Y.s * g.dxi * (gamma_psi - pvx * pvx - 1.0)
                        + X.s * Y.ds * g.dyi * (-pvx * pvy)) * clight);
                 if (local) {
-                    const int kw = W.at(X.cell, Y.cell);
                     lds_add(&win[kw], sy_add);
                     lds_add(&win[cells + kw], sx_add);
                 } else {

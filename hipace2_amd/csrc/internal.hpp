@@ -87,17 +87,21 @@ struct MGLevel {
 
 // Convergence state of one solve, kept on the device so V-cycles can be enqueued without
 // a host round trip per cycle; max norms are accumulated as the bit patterns of
-// non-negative doubles (their integer order is the numeric order).
+// non-negative doubles (their integer order is the numeric order). Two of them alternate
+// between solves: the kernel that ends a solve clears the other one for the next solve.
 struct MGCtl {
     int done = 0;
     int iters = 0;
     int status = 0;
-    int maxiter = 0;
+    int pad = 0;
     double max_norm = 0;
     double res_target = 0;
-    unsigned long long norm_bits = 0;   // max |residual| of the current V-cycle
-    unsigned long long res0_bits = 0;   // max |residual| of the initial guess
-    unsigned long long rhs0_bits = 0;   // max |rhs|
+};
+
+// What the device publishes to the host after every V-cycle, in one 64-bit word of pinned
+// coherent host memory: seq (bits 0-31), iters (32-55), status (56-62), done (63).
+struct MGMirror {
+    unsigned long long word;
 };
 
 struct MultiGrid {
@@ -111,10 +115,15 @@ struct MultiGrid {
     std::vector<MGLevel> lev;
     alignas(16) unsigned char coarse_args[2048];   // kernel argument block of the coarse kernel
     double* coarse_work = nullptr;   // coarse-level arrays when they do not fit in LDS
-    double* msol = nullptr;          // level-0 iterate after odd V-cycles / loaded guess (2 comps)
-    double* mrhs = nullptr;          // level-0 right-hand side (2 comps)
-    MGCtl* d_ctl = nullptr;
-    MGCtl* h_ctl = nullptr;          // pinned host mirror
+    double* msol = nullptr;          // level-0 initial guess / odd iterates (copy path, 2 comps)
+    double* mrhs = nullptr;          // level-0 right-hand side (copy path, 2 comps)
+    MGCtl* d_ctl = nullptr;          // two control blocks, used alternately
+    double* d_norms = nullptr;       // max |residual|, max |rhs| of every level-0 workgroup
+    int ctl_pick = 0;
+    bool ctl_dirty = false;          // a solve ended early: clear both before the next one
+    MGMirror* h_mirror = nullptr;    // pinned coherent host memory
+    MGMirror* d_mirror = nullptr;    // its device address
+    int seq = 0;
 };
 
 struct Capture {

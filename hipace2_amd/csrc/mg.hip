@@ -18,10 +18,16 @@
 // the values a global sweep gives them) and writes back the interior; the residual, its
 // restriction to the next level and its max norm come out of the same launch.
 // Prolongation is fused into the load of the up-sweep. On level 0 the up-sweep and the
-// final smoothing of the cycle are one launch of eight sweeps. Levels from the first one
-// of at most 33x33 points down run as one workgroup that does the coarse part of the
-// V-cycle with barriers between sweeps, its arrays in LDS.
+// final smoothing of the cycle are one launch of eight sweeps; its last workgroup to
+// finish evaluates convergence and publishes it to pinned host memory, which the host
+// polls instead of synchronising the stream. Levels from the first one of at most 33x33
+// points down run as one workgroup that does the coarse part of the V-cycle with barriers
+// between sweeps, its arrays in LDS; the levels of at most 64 points (and the bottom
+// solve) run in its first wavefront alone, without workgroup barriers. For odd n the
+// level-0 arrays are the caller's planes themselves (the node-centred level-0 box with
+// its ghost ring has the slice store's guarded shape).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -111,122 +117,266 @@ __device__ __forceinline__ double restrict_from(FA f, int i, int j) {
                          f(2 * i - 1, 2 * j + 1) + 2. * f(2 * i, 2 * j + 1) + f(2 * i + 1, 2 * j + 1));
 }
 
+
+__device__ void clear_ctl(MGCtl* c) {
+    c->done = 0; c->iters = 0; c->status = 0;
+    c->max_norm = 0; c->res_target = 0;
+}
+
+// one relaxed 64-bit store: the host sees the whole word or none of it
+__device__ void publish(const MGCtl* c, MGMirror* m, int seq) {
+    const unsigned long long w = (unsigned long long)(unsigned)seq | ((unsigned long long)(c->iters & 0xffffff) << 32) |
+                                 ((unsigned long long)(c->status & 0x7f) << 56) |
+                                 ((unsigned long long)(c->done ? 1 : 0) << 63);
+    __hip_atomic_store(&m->word, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// end of one V-cycle (solve_doit :1371-1417), run by the last workgroup of the level-0
+// launch with the max norm of the cycle's residual
+__device__ void end_cycle(MGCtl* ctl, MGCtl* other, MGMirror* mirror, int seq, int maxiter, double norminf) {
+    ctl->iters += 1;
+    if (norminf <= ctl->res_target) {
+        ctl->done = 1;
+    } else if (norminf > 1.e20 * ctl->max_norm) {
+        ctl->done = 1;
+        ctl->status = HPQ_E_MG_DIVERGED;
+    } else if (ctl->iters >= maxiter) {
+        ctl->done = 1;
+        ctl->status = HPQ_E_MG_NOT_CONVERGED;
+    }
+    if (ctl->done) clear_ctl(other);
+    publish(ctl, mirror, seq);
+}
+
+// convergence target after the initial residual (solve_doit :1330-1366)
+__device__ void end_initial(MGCtl* ctl, MGCtl* other, double tol_rel, double tol_abs, double resnorm0,
+                            double rhsnorm0) {
+    const double max_norm = (rhsnorm0 >= resnorm0) ? rhsnorm0 : resnorm0;
+    ctl->max_norm = max_norm;
+    ctl->res_target = fmax(tol_abs, fmax(tol_rel, 1.e-16) * max_norm);
+    if (resnorm0 <= ctl->res_target) {
+        ctl->done = 1;
+        clear_ctl(other);
+    }
+}
+
+// Arguments of the convergence bookkeeping after a level-0 launch.
+struct Check {
+    MGCtl* ctl;
+    MGCtl* other;
+    MGMirror* mirror;
+    const double* norms;   // max |residual|, max |rhs| of every workgroup of the launch
+    int nblk, seq, maxiter;
+    double tol_rel, tol_abs;
+};
+
+// max over the workgroups' entries norms[2 b + k], one workgroup of NT threads
+__device__ double max_norms(const double* norms, int nblk, int k, double* red) {
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += NT) m = fmax(m, norms[2 * b + k]);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    double mm = red[0];
+    for (int w = 1; w < NT / 64; ++w) mm = fmax(mm, red[w]);
+    return mm;
+}
+
+// end of one V-cycle; a cycle enqueued after convergence only reports that it drained
+__global__ __launch_bounds__(NT) void k_check(Check chk) {
+    __shared__ double red[NT / 64];
+    if (chk.ctl->done) {
+        if (threadIdx.x == 0) publish(chk.ctl, chk.mirror, chk.seq);
+        return;
+    }
+    const double norminf = max_norms(chk.norms, chk.nblk, 0, red);
+    if (threadIdx.x == 0) end_cycle(chk.ctl, chk.other, chk.mirror, chk.seq, chk.maxiter, norminf);
+}
+
+__global__ __launch_bounds__(NT) void k_target(Check chk) {
+    __shared__ double red[NT / 64];
+    const double resnorm0 = max_norms(chk.norms, chk.nblk, 0, red);
+    const double rhsnorm0 = max_norms(chk.norms, chk.nblk, 1, red);
+    if (threadIdx.x == 0) end_initial(chk.ctl, chk.other, chk.tol_rel, chk.tol_abs, resnorm0, rhsnorm0);
+}
+
 // NSW red-black sweeps (colours 0,1,2,3,0,1,...) of one level, LDS-tiled, optionally
-// followed by the residual, its restriction to the coarse points this tile owns (res_c)
-// and the max norm of the residual over the tile (atomicMax into *norm).
+// followed by the residual and its restriction to the coarse points this tile owns (res_c).
 // Start: zero (ZERO), phi_in + I(cor_c) (INTERP) or phi_in. Result -> phi_out.
-// PP (ping-pong): on odd V-cycles (ctl->iters odd) phi_in and phi_out swap roles, so a
-// level-0 cycle never sweeps in place while neighbouring tiles read their halo.
-// Each thread keeps rhs, acf and 1/c0 of the points it sweeps in registers.
-template <int NSW, bool CC, bool ZERO, bool RES, bool INTERP, bool PP>
+// NORMS: the workgroup's max |residual| over its tile and (RHSMAX) max |rhs| over its
+// points go to norms[2 b], norms[2 b + 1] for the convergence check that follows.
+// Every launch returns at once when the control block says the solve is finished (the
+// data loads are issued before that flag is tested, so its latency is hidden).
+//
+// Every thread owns up to KP swept points of each parity of (i + j) and keeps their rhs,
+// acf and 1/c0 in registers. A sweep of one colour touches one parity class only; the
+// thread reads all neighbours of its points first, then updates, then writes (points of
+// one colour never neighbour each other, so this is the order a global sweep has).
+template <int NSW, bool CC, bool ZERO, bool RES, bool INTERP, bool RHSMAX, bool NORMS>
 __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __restrict__ phi_out,
                                                const double* __restrict__ rhs, long rcs,
-                                               const double* __restrict__ acf, double* __restrict__ phi_in,
+                                               const double* __restrict__ acf, const double* __restrict__ phi_in,
                                                const double* __restrict__ cor_c, double* __restrict__ res_c,
-                                               unsigned long long* __restrict__ norm, const MGCtl* __restrict__ ctl) {
+                                               const MGCtl* __restrict__ gate, double* __restrict__ norms) {
+    static_assert(NSW % 2 == 0, "sweeps come in red-black pairs");
     constexpr int HALO = NSW + 2;
     constexpr int LX = TX + 2 * HALO, LY = TY + 2 * HALO;
     constexpr int SW = LX - 2, SH = LY - 2;        // swept region (inside the frozen ring)
-    constexpr int KPT = (SW * SH + NT - 1) / NT;   // swept points per thread
-    if (ctl && ctl->done) return;
-    if (PP && (ctl->iters & 1)) {
-        double* t = phi_out;
-        phi_out = phi_in;
-        phi_in = t;
-    }
+    constexpr int HW = SW / 2;                     // points of one parity per swept row
+    constexpr int NPAR = HW * SH;                  // points of one parity
+    constexpr int KP = (NPAR + NT - 1) / NT;       // points of one parity per thread
+    constexpr int NLD = (LX * LY + NT - 1) / NT;   // start values loaded per thread
     __shared__ double s[2][LY][LX];
     __shared__ double r[2][RH][RW];
-    __shared__ double red[NT / 64];
+    __shared__ double red[2][NT / 64];
+    static_assert(!NORMS || RES, "the norms come from the residual");
+    const int gate_done = gate ? gate->done : 0;
     const int ti0 = L.vilo + int(blockIdx.x) * TX - HALO;
     const int tj0 = L.vjlo + int(blockIdx.y) * TY - HALO;
+    const int base = (ti0 + tj0) & 1;
     const double facx = 1. / (L.dx * L.dx), facy = 1. / (L.dy * L.dy);
-    // own swept points: rhs, acf and 1/c0 into registers
-    double r0[KPT], r1[KPT], ac[KPT], ci[KPT];
-    int pl[KPT];   // lx | ly << 8, or -1 when outside the valid box or the swept region
+    // owned points: [parity][q]; pl = lx | ly << 8, or -1 outside the valid box / region
+    double r0[2][KP], r1[2][KP], ac[2][KP], ci[2][KP];
+    int pl[2][KP];
 #pragma unroll
-    for (int q = 0; q < KPT; ++q) {
-        const int e = threadIdx.x + q * NT;
-        pl[q] = -1;
-        r0[q] = r1[q] = ac[q] = ci[q] = 0.0;
-        if (e < SW * SH) {
-            const int ly = 1 + e / SW, lx = 1 + e - (ly - 1) * SW;
-            const int i = ti0 + lx, j = tj0 + ly;
-            if (in_valid(L, i, j)) {
-                const long k = at(L, i, j);
-                pl[q] = lx | (ly << 8);
-                r0[q] = rhs[k];
-                r1[q] = rhs[k + rcs];
-                ac[q] = acf[k];
-                ci[q] = c0_inverse<CC>(L, i, j, ac[q], facx, facy);
+    for (int par = 0; par < 2; ++par) {
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            const int e = threadIdx.x + q * NT;
+            pl[par][q] = -1;
+            r0[par][q] = r1[par][q] = ac[par][q] = 0.0;
+            if (e < NPAR) {
+                const int ly = 1 + e / HW;
+                const int lx = 1 + 2 * (e - (ly - 1) * HW) + ((par + base + ly + 1) & 1);
+                const int i = ti0 + lx, j = tj0 + ly;
+                if (in_valid(L, i, j)) {
+                    const long k = at(L, i, j);
+                    pl[par][q] = lx | (ly << 8);
+                    r0[par][q] = rhs[k];
+                    r1[par][q] = rhs[k + rcs];
+                    ac[par][q] = acf[k];
+                }
             }
         }
     }
-    for (int e = threadIdx.x; e < LX * LY; e += NT) {
+    // start values: all loads first, then the LDS stores
+    double v0[NLD], v1[NLD];
+#pragma unroll
+    for (int it = 0; it < NLD; ++it) {
+        const int e = threadIdx.x + it * NT;
         const int ly = e / LX, lx = e - ly * LX;
         const int i = ti0 + lx, j = tj0 + ly;
-        double v0 = 0.0, v1 = 0.0;
-        if (!ZERO && in_valid(L, i, j)) {
+        v0[it] = v1[it] = 0.0;
+        if (!ZERO && e < LX * LY && in_valid(L, i, j)) {
             const long k = at(L, i, j);
             if (INTERP) {
                 auto c0 = [&](int a, int b) { return cor_c[at(C, a, b)]; };
                 auto c1 = [&](int a, int b) { return cor_c[at(C, a, b) + C.cs]; };
-                v0 = interp_add<CC>(i, j, phi_in[k], c0);
-                v1 = interp_add<CC>(i, j, phi_in[k + L.cs], c1);
+                v0[it] = interp_add<CC>(i, j, phi_in[k], c0);
+                v1[it] = interp_add<CC>(i, j, phi_in[k + L.cs], c1);
             } else {
-                v0 = phi_in[k];
-                v1 = phi_in[k + L.cs];
+                v0[it] = phi_in[k];
+                v1[it] = phi_in[k + L.cs];
             }
         }
-        s[0][ly][lx] = v0;
-        s[1][ly][lx] = v1;
+    }
+    if (gate_done) return;
+#pragma unroll
+    for (int it = 0; it < NLD; ++it) {
+        const int e = threadIdx.x + it * NT;
+        if (e < LX * LY) {
+            const int ly = e / LX, lx = e - ly * LX;
+            s[0][ly][lx] = v0[it];
+            s[1][ly][lx] = v1[it];
+        }
+    }
+    double rmax = 0.0;
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            ci[par][q] = 0.0;
+            if (pl[par][q] < 0) continue;
+            const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
+            ci[par][q] = c0_inverse<CC>(L, ti0 + lx, tj0 + ly, ac[par][q], facx, facy);
+            if (RHSMAX) rmax = fmax(rmax, fmax(fabs(r0[par][q]), fabs(r1[par][q])));
+        }
     }
     __syncthreads();
-    for (int color = 0; color < NSW; ++color) {
+    // one colour: all reads, then all updates, then all writes
+    auto sweep = [&](auto PARC) {
+        constexpr int par = decltype(PARC)::value;
+        double w0[KP], e0[KP], s0[KP], n0[KP], w1[KP], e1[KP], s1[KP], n1[KP];
 #pragma unroll
-        for (int q = 0; q < KPT; ++q) {
-            if (pl[q] < 0) continue;
-            const int lx = pl[q] & 255, ly = pl[q] >> 8;
+        for (int q = 0; q < KP; ++q) {
+            if (pl[par][q] < 0) continue;
+            const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
+            w0[q] = s[0][ly][lx - 1]; e0[q] = s[0][ly][lx + 1]; s0[q] = s[0][ly - 1][lx]; n0[q] = s[0][ly + 1][lx];
+            w1[q] = s[1][ly][lx - 1]; e1[q] = s[1][ly][lx + 1]; s1[q] = s[1][ly - 1][lx]; n1[q] = s[1][ly + 1][lx];
+        }
+        double u0[KP], u1[KP];
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            if (pl[par][q] < 0) continue;
+            const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
             const int i = ti0 + lx, j = tj0 + ly;
-            if (((i + j + color) & 1) != 0) continue;
-            auto P0 = [&](int di, int dj) { return s[0][ly + dj][lx + di]; };
-            auto P1 = [&](int di, int dj) { return s[1][ly + dj][lx + di]; };
-            const double n0 = gs_point<CC>(L, i, j, P0, r0[q], ci[q], facx, facy);
-            const double n1 = gs_point<CC>(L, i, j, P1, r1[q], ci[q], facx, facy);
-            s[0][ly][lx] = n0;
-            s[1][ly][lx] = n1;
+            auto P0 = [&](int di, int dj) { return di < 0 ? w0[q] : di > 0 ? e0[q] : dj < 0 ? s0[q] : n0[q]; };
+            auto P1 = [&](int di, int dj) { return di < 0 ? w1[q] : di > 0 ? e1[q] : dj < 0 ? s1[q] : n1[q]; };
+            u0[q] = gs_point<CC>(L, i, j, P0, r0[par][q], ci[par][q], facx, facy);
+            u1[q] = gs_point<CC>(L, i, j, P1, r1[par][q], ci[par][q], facx, facy);
+        }
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            if (pl[par][q] < 0) continue;
+            const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
+            s[0][ly][lx] = u0[q];
+            s[1][ly][lx] = u1[q];
         }
         __syncthreads();
+    };
+    // colour c updates the points with (i + j + c) even, i.e. parity c & 1
+    for (int c2 = 0; c2 < NSW; c2 += 2) {
+        sweep(std::integral_constant<int, 0>{});
+        sweep(std::integral_constant<int, 1>{});
     }
     double m = 0.0;
 #pragma unroll
-    for (int q = 0; q < KPT; ++q) {
-        const int e = threadIdx.x + q * NT;
-        if (e >= SW * SH) continue;
-        const int ly = 1 + e / SW, lx = 1 + e - (ly - 1) * SW;
-        const bool out = lx >= HALO && lx < HALO + TX && ly >= HALO && ly < HALO + TY;
-        if (RES) {
-            const bool ring = lx >= HALO - 1 && lx <= HALO + TX && ly >= HALO - 1 && ly <= HALO + TY;
-            if (ring) {
-                double v0 = 0.0, v1 = 0.0;
-                if (pl[q] >= 0) {
+    for (int par = 0; par < 2; ++par) {
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            if (pl[par][q] < 0) continue;
+            const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
+            const bool out = lx >= HALO && lx < HALO + TX && ly >= HALO && ly < HALO + TY;
+            if (RES) {
+                const bool ring = lx >= HALO - 1 && lx <= HALO + TX && ly >= HALO - 1 && ly <= HALO + TY;
+                if (ring) {
                     const int i = ti0 + lx, j = tj0 + ly;
                     auto P0 = [&](int di, int dj) { return s[0][ly + dj][lx + di]; };
                     auto P1 = [&](int di, int dj) { return s[1][ly + dj][lx + di]; };
-                    v0 = residual1<CC>(L, i, j, P0, r0[q], ac[q], facx, facy);
-                    v1 = residual1<CC>(L, i, j, P1, r1[q], ac[q], facx, facy);
-                    if (out) m = fmax(m, fmax(fabs(v0), fabs(v1)));
+                    const double g0 = residual1<CC>(L, i, j, P0, r0[par][q], ac[par][q], facx, facy);
+                    const double g1 = residual1<CC>(L, i, j, P1, r1[par][q], ac[par][q], facx, facy);
+                    if (out) m = fmax(m, fmax(fabs(g0), fabs(g1)));
+                    r[0][ly - HALO + 1][lx - HALO + 1] = g0;
+                    r[1][ly - HALO + 1][lx - HALO + 1] = g1;
                 }
-                r[0][ly - HALO + 1][lx - HALO + 1] = v0;
-                r[1][ly - HALO + 1][lx - HALO + 1] = v1;
             }
-        }
-        if (out && pl[q] >= 0) {
-            const long k = at(L, ti0 + lx, tj0 + ly);
-            phi_out[k] = s[0][ly][lx];
-            phi_out[k + L.cs] = s[1][ly][lx];
+            if (out) {
+                const long k = at(L, ti0 + lx, tj0 + ly);
+                phi_out[k] = s[0][ly][lx];
+                phi_out[k + L.cs] = s[1][ly][lx];
+            }
         }
     }
     if (RES) {
+        // ring points outside the valid box hold zero residual
+        for (int e = threadIdx.x; e < RW * RH; e += NT) {
+            const int ry = e / RW, rx = e - ry * RW;
+            if (!in_valid(L, ti0 + HALO - 1 + rx, tj0 + HALO - 1 + ry)) {
+                r[0][ry][rx] = 0.0;
+                r[1][ry][rx] = 0.0;
+            }
+        }
         __syncthreads();
         // coarse points whose centre fine point 2*ic lies in this tile
         const int ox0 = ti0 + HALO, oy0 = tj0 + HALO;
@@ -241,15 +391,26 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
             res_c[at(C, ic, jc)] = restrict_from<CC>(f0, ic, jc);
             res_c[at(C, ic, jc) + C.cs] = restrict_from<CC>(f1, ic, jc);
         }
-        if (norm) {
-            for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double mm = red[0];
-                for (int w = 1; w < NT / 64; ++w) mm = fmax(mm, red[w]);
-                atomicMax(norm, (unsigned long long)__double_as_longlong(mm));
+    }
+    if (NORMS) {
+        for (int o = 32; o > 0; o >>= 1) {
+            m = fmax(m, __shfl_xor(m, o));
+            if (RHSMAX) rmax = fmax(rmax, __shfl_xor(rmax, o));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            red[0][threadIdx.x >> 6] = m;
+            red[1][threadIdx.x >> 6] = rmax;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mm = red[0][0], rr = red[1][0];
+            for (int w = 1; w < NT / 64; ++w) {
+                mm = fmax(mm, red[0][w]);
+                rr = fmax(rr, red[1][w]);
             }
+            const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
+            norms[2 * b] = mm;
+            norms[2 * b + 1] = rr;
         }
     }
 }
@@ -282,6 +443,7 @@ struct CLev {
 };
 struct CArgs {
     int nlev;           // levels cb .. max_level
+    int nblock;         // levels [0, nblock) run on the whole workgroup, the rest on wave 0
     int bottom_sweeps;
     int work_doubles;
     CLev lv[COARSE_MAX_LEVELS];
@@ -293,27 +455,50 @@ struct CArgs {
 
 __device__ __forceinline__ int lat(const CLev& L, int i, int j) { return (j - L.jlo + 1) * L.ldx + (i - L.ilo + 1); }
 
-// thread (tx, ty) visits the valid points (vilo + tx + CTX a, vjlo + ty + CTY b)
-template <class F>
-__device__ __forceinline__ void for_valid(const CLev& L, F f) {
-    const int tx = threadIdx.x % CTX, ty = threadIdx.x / CTX;
-    for (int j = L.vjlo + ty; j <= L.vjhi; j += CTY)
-        for (int i = L.vilo + tx; i <= L.vihi; i += CTX) f(i, j);
-}
-// the valid points with (i + j + color) even, two per CTX columns
-template <class F>
-__device__ __forceinline__ void for_color(const CLev& L, int color, F f) {
-    const int tx = threadIdx.x % CTX, ty = threadIdx.x / CTX;
-    for (int j = L.vjlo + ty; j <= L.vjhi; j += CTY) {
-        const int i0 = L.vilo + ((L.vilo + j + color) & 1) + 2 * tx;
-        for (int i = i0; i <= L.vihi; i += 2 * CTX) f(i, j);
+// Thread layouts of the coarse kernel. Block: thread (tx, ty) of CTX x CTY visits the
+// valid points (vilo + tx + CTX a, vjlo + ty + CTY b); a colour sweep gives each thread
+// two columns of CTX. Wave: the 64 lanes of wavefront 0 walk the valid points in order;
+// LDS traffic of one wavefront is ordered, so a wait for the LDS counter (plus a compiler
+// barrier) separates phases.
+struct BlockMode {
+    template <class F>
+    static __device__ __forceinline__ void valid(const CLev& L, F f) {
+        const int tx = threadIdx.x % CTX, ty = threadIdx.x / CTX;
+        for (int j = L.vjlo + ty; j <= L.vjhi; j += CTY)
+            for (int i = L.vilo + tx; i <= L.vihi; i += CTX) f(i, j);
     }
-}
+    template <class F>
+    static __device__ __forceinline__ void color(const CLev& L, int c, F f) {
+        const int tx = threadIdx.x % CTX, ty = threadIdx.x / CTX;
+        for (int j = L.vjlo + ty; j <= L.vjhi; j += CTY) {
+            const int i0 = L.vilo + ((L.vilo + j + c) & 1) + 2 * tx;
+            for (int i = i0; i <= L.vihi; i += 2 * CTX) f(i, j);
+        }
+    }
+    static __device__ __forceinline__ void sync() { __syncthreads(); }
+};
+struct WaveMode {
+    template <class F>
+    static __device__ __forceinline__ void valid(const CLev& L, F f) {
+        const int nvx = L.vihi - L.vilo + 1, n = nvx * (L.vjhi - L.vjlo + 1);
+        for (int e = threadIdx.x; e < n; e += 64) f(L.vilo + e % nvx, L.vjlo + e / nvx);
+    }
+    template <class F>
+    static __device__ __forceinline__ void color(const CLev& L, int c, F f) {
+        valid(L, [&](int i, int j) {
+            if (((i + j + c) & 1) == 0) f(i, j);
+        });
+    }
+    static __device__ __forceinline__ void sync() {
+        __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0)
+        __builtin_amdgcn_wave_barrier();
+    }
+};
 
-template <bool CC>
+template <class Mode, bool CC>
 __device__ __forceinline__ void c_sweep(double* w, const CLev& L, int color) {
     const int n1 = L.cs;
-    for_color(L, color, [&](int i, int j) {
+    Mode::color(L, color, [&](int i, int j) {
         const int k = lat(L, i, j);
         const double* p0 = w + L.cor;
         const double* p1 = w + L.cor + n1;
@@ -325,7 +510,45 @@ __device__ __forceinline__ void c_sweep(double* w, const CLev& L, int color) {
         w[L.cor + k] = v0;
         w[L.cor + n1 + k] = v1;
     });
-    __syncthreads();
+    Mode::sync();
+}
+
+// down-sweep step of level L: four sweeps from zero, residual, restriction into C.res
+template <class Mode, bool CC>
+__device__ void c_down(double* w, const CLev& L, const CLev& C) {
+    for (int color = 0; color < 4; ++color) c_sweep<Mode, CC>(w, L, color);
+    Mode::valid(L, [&](int i, int j) {
+        const int k = lat(L, i, j);
+        const double a = w[L.acf + k];
+        for (int n = 0; n < 2; ++n) {
+            const double* p = w + L.cor + n * L.cs;
+            auto P = [&](int di, int dj) { return p[k + dj * L.ldx + di]; };
+            w[L.rescor + n * L.cs + k] = residual1<CC>(L, i, j, P, w[L.res + n * L.cs + k], a, L.facx, L.facy);
+        }
+    });
+    Mode::sync();
+    Mode::valid(C, [&](int i, int j) {
+        for (int n = 0; n < 2; ++n) {
+            auto fa = [&](int a, int b) { return w[L.rescor + n * L.cs + lat(L, a, b)]; };
+            w[C.res + n * C.cs + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
+        }
+    });
+    Mode::sync();
+}
+
+// up-sweep step of level L: cor += I(C.cor) in place (each point reads only itself on this
+// level), then four sweeps
+template <class Mode, bool CC>
+__device__ void c_up(double* w, const CLev& L, const CLev& C) {
+    Mode::valid(L, [&](int i, int j) {
+        const int k = lat(L, i, j);
+        for (int n = 0; n < 2; ++n) {
+            auto c = [&](int a, int b) { return w[C.cor + n * C.cs + lat(C, a, b)]; };
+            w[L.cor + n * L.cs + k] = interp_add<CC>(i, j, w[L.cor + n * L.cs + k], c);
+        }
+    });
+    Mode::sync();
+    for (int color = 0; color < 4; ++color) c_sweep<Mode, CC>(w, L, color);
 }
 
 // Levels [cb, maxl] of one V-cycle: down-sweep with residual restriction, bottom solve
@@ -335,166 +558,94 @@ __device__ __forceinline__ void c_sweep(double* w, const CLev& L, int color) {
 template <bool CC, bool IN_LDS>
 __global__ __launch_bounds__(COARSE_THREADS) void k_coarse(const CArgs A, double* __restrict__ gwork,
                                                            const MGCtl* __restrict__ ctl) {
-    if (ctl && ctl->done) return;
+    constexpr int NB = 12;   // loads in flight per thread
     extern __shared__ double sm[];
     double* w = IN_LDS ? sm : gwork;
     const int nl = A.nlev;
-    {
-        const CLev& T = A.lv[0];
-        for (int l = 0; l < nl; ++l) {
-            const CLev& L = A.lv[l];
-            for (int t = threadIdx.x; t < 2 * L.cs; t += COARSE_THREADS) w[L.cor + t] = 0.0;
+    const CLev& T = A.lv[0];
+    const int done = ctl ? ctl->done : 0;
+    // acf, res (2 comps) of level cb: up to NB loads per thread in flight at once
+    const int nload = 3 * T.cs;
+    for (int b0 = 0; b0 < nload; b0 += NB * COARSE_THREADS) {
+        double v[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int t = b0 + q * COARSE_THREADS + threadIdx.x;
+            v[q] = 0.0;
+            if (t < nload) v[q] = t < T.cs ? A.acf[t] : t < 2 * T.cs ? A.res[t - T.cs] : A.res[t - 2 * T.cs + A.gcs];
         }
-        for (int t = threadIdx.x; t < T.cs; t += COARSE_THREADS) {
-            w[T.acf + t] = A.acf[t];
-            w[T.res + t] = A.res[t];
-            w[T.res + T.cs + t] = A.res[t + A.gcs];
+        if (done) return;
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int t = b0 + q * COARSE_THREADS + threadIdx.x;
+            if (t < nload) w[t < T.cs ? T.acf + t : T.res + (t - T.cs)] = v[q];
         }
-        __syncthreads();
     }
+    for (int l = 0; l < nl; ++l) {
+        const CLev& L = A.lv[l];
+        for (int t = threadIdx.x; t < 2 * L.cs; t += COARSE_THREADS) w[L.cor + t] = 0.0;
+    }
+    __syncthreads();
     // average_down_acoef (:1641-1661) and the diagonal inverses
     for (int l = 0; l < nl; ++l) {
         const CLev& L = A.lv[l];
-        for_valid(L, [&](int i, int j) {
+        BlockMode::valid(L, [&](int i, int j) {
             const int k = lat(L, i, j);
             w[L.c0i + k] = c0_inverse<CC>(L, i, j, w[L.acf + k], L.facx, L.facy);
         });
         if (l + 1 < nl) {
             const CLev& C = A.lv[l + 1];
-            for_valid(C, [&](int i, int j) {
+            BlockMode::valid(C, [&](int i, int j) {
                 auto fa = [&](int a, int b) { return w[L.acf + lat(L, a, b)]; };
                 w[C.acf + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
             });
         }
         __syncthreads();
     }
-    // down-sweep
-    for (int l = 0; l + 1 < nl; ++l) {
-        const CLev& L = A.lv[l];
-        const CLev& C = A.lv[l + 1];
-        for (int color = 0; color < 4; ++color) c_sweep<CC>(w, L, color);
-        for_valid(L, [&](int i, int j) {
-            const int k = lat(L, i, j);
-            const double a = w[L.acf + k];
-            for (int n = 0; n < 2; ++n) {
-                const double* p = w + L.cor + n * L.cs;
-                auto P = [&](int di, int dj) { return p[k + dj * L.ldx + di]; };
-                w[L.rescor + n * L.cs + k] = residual1<CC>(L, i, j, P, w[L.res + n * L.cs + k], a, L.facx, L.facy);
-            }
-        });
-        __syncthreads();
-        for_valid(C, [&](int i, int j) {
-            for (int n = 0; n < 2; ++n) {
-                auto fa = [&](int a, int b) { return w[L.rescor + n * L.cs + lat(L, a, b)]; };
-                w[C.res + n * C.cs + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
-            }
-        });
-        __syncthreads();
+    const int nb = A.nblock;
+    for (int l = 0; l < nb && l + 1 < nl; ++l) c_down<BlockMode, CC>(w, A.lv[l], A.lv[l + 1]);
+    if (threadIdx.x < 64) {
+        for (int l = nb; l + 1 < nl; ++l) c_down<WaveMode, CC>(w, A.lv[l], A.lv[l + 1]);
+        // bottom solve from zero (the bottom level is always a wavefront level)
+        for (int sw = 0; sw < A.bottom_sweeps; ++sw) c_sweep<WaveMode, CC>(w, A.lv[nl - 1], sw);
+        for (int l = nl - 2; l >= nb; --l) c_up<WaveMode, CC>(w, A.lv[l], A.lv[l + 1]);
     }
-    // bottom solve from zero
-    for (int sw = 0; sw < A.bottom_sweeps; ++sw) c_sweep<CC>(w, A.lv[nl - 1], sw);
-    // up-sweep: cor += I(cor_coarse) in place (each point reads only itself on this level),
-    // then four sweeps
-    for (int l = nl - 2; l >= 0; --l) {
-        const CLev& L = A.lv[l];
-        const CLev& C = A.lv[l + 1];
-        for_valid(L, [&](int i, int j) {
-            const int k = lat(L, i, j);
-            for (int n = 0; n < 2; ++n) {
-                auto c = [&](int a, int b) { return w[C.cor + n * C.cs + lat(C, a, b)]; };
-                w[L.cor + n * L.cs + k] = interp_add<CC>(i, j, w[L.cor + n * L.cs + k], c);
-            }
-        });
-        __syncthreads();
-        for (int color = 0; color < 4; ++color) c_sweep<CC>(w, L, color);
-    }
-    {
-        const CLev& T = A.lv[0];
-        for (int t = threadIdx.x; t < T.cs; t += COARSE_THREADS) {
-            A.cor[t] = w[T.cor + t];
-            A.cor[t + A.gcs] = w[T.cor + T.cs + t];
-        }
+    __syncthreads();
+    for (int l = min(nb, nl - 1) - 1; l >= 0; --l) c_up<BlockMode, CC>(w, A.lv[l], A.lv[l + 1]);
+    for (int t = threadIdx.x; t < T.cs; t += COARSE_THREADS) {
+        A.cor[t] = w[T.cor + t];
+        A.cor[t + A.gcs] = w[T.cor + T.cs + t];
     }
 }
 
-// level-0 load: acf over the whole box, sol and rhs (2 comps) over the whole box; max |rhs|
-// over the valid box into ctl->rhs0_bits
+// level-0 load (copy path): acf, sol and rhs (2 comps) over the whole box
 __global__ void k_mg_load(MGLevel L, int sh, Geom g, const double* __restrict__ sol, const double* __restrict__ rhs,
-                          const double* __restrict__ acf, long cstride, double* __restrict__ msol,
-                          double* __restrict__ mrhs, MGCtl* __restrict__ ctl) {
-    __shared__ double red[4];
+                          const double* __restrict__ acf, long cstride, double* __restrict__ acf0,
+                          double* __restrict__ msol, double* __restrict__ mrhs) {
     const int nbx = L.ihi - L.ilo + 1, nby = L.jhi - L.jlo + 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double m = 0.0;
-    if (t < nbx * nby) {
-        const int j = L.jlo + t / nbx, i = L.ilo + t % nbx;
-        const long gk = g.idx(i - sh, j - sh);
-        const long k = at(L, i, j);
-        L.acf[k] = acf[gk];
-        msol[k] = sol[gk];
-        msol[k + L.cs] = sol[gk + cstride];
-        const double a = rhs[gk], b = rhs[gk + cstride];
-        mrhs[k] = a;
-        mrhs[k + L.cs] = b;
-        if (in_valid(L, i, j)) m = fmax(fabs(a), fabs(b));
-    }
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double mm = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-        atomicMax(&ctl->rhs0_bits, (unsigned long long)__double_as_longlong(mm));
-    }
+    if (t >= nbx * nby) return;
+    const int j = L.jlo + t / nbx, i = L.ilo + t % nbx;
+    const long gk = g.idx(i - sh, j - sh);
+    const long k = at(L, i, j);
+    acf0[k] = acf[gk];
+    msol[k] = sol[gk];
+    msol[k + L.cs] = sol[gk + cstride];
+    mrhs[k] = rhs[gk];
+    mrhs[k + L.cs] = rhs[gk + cstride];
 }
 
-// sol (valid points) = the current level-0 iterate: cor after an even number of
-// V-cycles, alt after an odd number (the level-0 cycle kernel alternates between them)
+// sol (valid points) = src, the level-0 iterate
 __global__ void k_mg_store(MGLevel L, int sh, Geom g, double* __restrict__ sol, long cstride,
-                           const double* __restrict__ alt, const MGCtl* __restrict__ ctl) {
+                           const double* __restrict__ src) {
     const int nvx = L.vihi - L.vilo + 1, nvy = L.vjhi - L.vjlo + 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nvx * nvy) return;
-    const double* src = (ctl->iters & 1) ? alt : L.cor;
     const int j = L.vjlo + t / nvx, i = L.vilo + t % nvx;
     const long gk = g.idx(i - sh, j - sh);
     const long k = at(L, i, j);
     sol[gk] = src[k];
     sol[gk + cstride] = src[k + L.cs];
-}
-
-__global__ void k_ctl_reset(MGCtl* ctl, int maxiter) {
-    MGCtl c;
-    c.maxiter = maxiter;
-    *ctl = c;
-}
-
-__device__ __forceinline__ double bits2d(unsigned long long b) { return __longlong_as_double((long long)b); }
-
-// convergence target after the initial residual (solve_doit :1330-1366)
-__global__ void k_target(MGCtl* ctl, double tol_rel, double tol_abs) {
-    const double resnorm0 = bits2d(ctl->res0_bits), rhsnorm0 = bits2d(ctl->rhs0_bits);
-    const double max_norm = (rhsnorm0 >= resnorm0) ? rhsnorm0 : resnorm0;
-    ctl->max_norm = max_norm;
-    ctl->res_target = fmax(tol_abs, fmax(tol_rel, 1.e-16) * max_norm);
-    ctl->norm_bits = 0;
-    if (resnorm0 <= ctl->res_target) ctl->done = 1;
-}
-
-// end of one V-cycle (solve_doit :1371-1417)
-__global__ void k_check(MGCtl* ctl) {
-    if (ctl->done) return;
-    ctl->iters += 1;
-    const double norminf = bits2d(ctl->norm_bits);
-    ctl->norm_bits = 0;
-    if (norminf <= ctl->res_target) {
-        ctl->done = 1;
-    } else if (norminf > 1.e20 * ctl->max_norm) {
-        ctl->done = 1;
-        ctl->status = HPQ_E_MG_DIVERGED;
-    } else if (ctl->iters >= ctl->maxiter) {
-        ctl->done = 1;
-        ctl->status = HPQ_E_MG_NOT_CONVERGED;
-    }
 }
 
 bool coarsenable(bool cc, int lo, int hi, int min_width) {
@@ -527,6 +678,14 @@ void coarse_args(MultiGrid& M) {
         L.facy = 1. / (G.dy * G.dy);
     }
     A.work_doubles = off;
+    // levels of more than 64 valid points run on the whole workgroup; the bottom level
+    // always on one wavefront
+    A.nblock = 0;
+    while (A.nblock < A.nlev - 1) {
+        const CLev& L = A.lv[A.nblock];
+        if ((L.vihi - L.vilo + 1) * (L.vjhi - L.vjlo + 1) <= 64) break;
+        ++A.nblock;
+    }
     const MGLevel& B = M.lev[M.max_level];
     const int len = std::max(B.ihi - B.ilo + 1, B.jhi - B.jlo + 1);
     A.bottom_sweeps = std::max(16, (len + 1) / 2 * 2);
@@ -605,8 +764,16 @@ void mg_init(hpq_ctx* c) {
     HPQ_HIP(hipMalloc(&M.mrhs, b0));
     HPQ_HIP(hipMemset(M.msol, 0, b0));
     HPQ_HIP(hipMemset(M.mrhs, 0, b0));
-    HPQ_HIP(hipMalloc(&M.d_ctl, sizeof(MGCtl)));
-    HPQ_HIP(hipHostMalloc(&M.h_ctl, sizeof(MGCtl), hipHostMallocDefault));
+    HPQ_HIP(hipMalloc(&M.d_ctl, 2 * sizeof(MGCtl)));
+    HPQ_HIP(hipMemset(M.d_ctl, 0, 2 * sizeof(MGCtl)));
+    {
+        const MGLevel& L0 = M.lev[0];
+        const long nblk = long((L0.vihi - L0.vilo + 1 + TX - 1) / TX) * ((L0.vjhi - L0.vjlo + 1 + TY - 1) / TY);
+        HPQ_HIP(hipMalloc(&M.d_norms, sizeof(double) * 2 * nblk));
+    }
+    HPQ_HIP(hipHostMalloc(&M.h_mirror, sizeof(MGMirror), hipHostMallocMapped | hipHostMallocCoherent));
+    M.h_mirror->word = 0;
+    HPQ_HIP(hipHostGetDevicePointer((void**)&M.d_mirror, M.h_mirror, 0));
     if (M.coarse_in_lds) {
         const void* f = M.cc ? (const void*)k_coarse<true, true> : (const void*)k_coarse<false, true>;
         HPQ_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(M.coarse_lds_bytes)));
@@ -622,24 +789,39 @@ void mg_free(hpq_ctx* c) {
     if (M.msol) (void)hipFree(M.msol);
     if (M.mrhs) (void)hipFree(M.mrhs);
     if (M.d_ctl) (void)hipFree(M.d_ctl);
-    if (M.h_ctl) (void)hipHostFree(M.h_ctl);
+    if (M.d_norms) (void)hipFree(M.d_norms);
+    if (M.h_mirror) (void)hipHostFree(M.h_mirror);
     M = MultiGrid{};
 }
 
 namespace {
 
-template <int NSW, bool ZERO, bool RES, bool INTERP, bool PP>
+int level0_blocks(const MGLevel& L) {
+    return ((L.vihi - L.vilo + 1 + TX - 1) / TX) * ((L.vjhi - L.vjlo + 1 + TY - 1) / TY);
+}
+
+// the caller's view of one solve: level-0 rhs, acf and the two iterate buffers
+struct Solve {
+    const double* rhs0;   // comp stride rcs
+    long rcs;
+    MGLevel L0;           // level 0 with acf pointing at the level-0 acf
+    double* buf[2];       // iterate after an even / odd number of V-cycles
+    MGCtl* ctl;
+    MGCtl* other;
+};
+
+template <int NSW, bool ZERO, bool RES, bool INTERP, bool RHSMAX = false, bool NORMS = false>
 void smooth(hpq_ctx* c, const MGLevel& L, const MGLevel& C, double* phi_out, const double* rhs, long rcs,
-            const double* acf, double* phi_in, const double* cor_c, double* res_c, unsigned long long* norm,
-            const MGCtl* ctl) {
+            const double* acf, const double* phi_in, const double* cor_c, double* res_c, const MGCtl* gate,
+            double* norms = nullptr) {
     TraceScope ts(c, HPQ_TRACE_MG_SMOOTH);
     const dim3 grid((L.vihi - L.vilo + 1 + TX - 1) / TX, (L.vjhi - L.vjlo + 1 + TY - 1) / TY);
     if (L.cc)
-        hipLaunchKernelGGL((k_smooth<NSW, true, ZERO, RES, INTERP, PP>), grid, dim3(NT), 0, c->stream, L, C, phi_out,
-                           rhs, rcs, acf, phi_in, cor_c, res_c, norm, ctl);
+        hipLaunchKernelGGL((k_smooth<NSW, true, ZERO, RES, INTERP, RHSMAX, NORMS>), grid, dim3(NT), 0, c->stream, L,
+                           C, phi_out, rhs, rcs, acf, phi_in, cor_c, res_c, gate, norms);
     else
-        hipLaunchKernelGGL((k_smooth<NSW, false, ZERO, RES, INTERP, PP>), grid, dim3(NT), 0, c->stream, L, C,
-                           phi_out, rhs, rcs, acf, phi_in, cor_c, res_c, norm, ctl);
+        hipLaunchKernelGGL((k_smooth<NSW, false, ZERO, RES, INTERP, RHSMAX, NORMS>), grid, dim3(NT), 0, c->stream,
+                           L, C, phi_out, rhs, rcs, acf, phi_in, cor_c, res_c, gate, norms);
 }
 
 void launch_restrict(hpq_ctx* c, const MGLevel& C, const MGLevel& F, double* crse, const double* fine, int ncomp) {
@@ -652,18 +834,19 @@ void launch_restrict(hpq_ctx* c, const MGLevel& C, const MGLevel& F, double* crs
                            F.cs);
 }
 
-// MultiGrid::vcycle (:1430-1512). Level 0's residual and its restriction to level 1 come
-// from the end of the previous cycle (or the initial residual); every kernel returns at
-// once when the device-side control block says the solve is finished.
-void vcycle(hpq_ctx* c) {
+// MultiGrid::vcycle (:1430-1512), V-cycle number m of the solve. Level 0's residual and
+// its restriction to level 1 come from the end of the previous cycle (or the initial
+// residual); every kernel returns at once when the control block says the solve is
+// finished. Cycle m runs only if cycles 0..m-1 ran, so the host knows that it starts from
+// buf[m & 1] and ends in buf[(m + 1) & 1].
+void vcycle(hpq_ctx* c, const Solve& S, int m, int maxiter) {
     MultiGrid& M = c->mg;
-    MGCtl* ctl = M.d_ctl;
+    MGCtl* ctl = S.ctl;
     const int cb = M.coarse_begin;
     for (int l = 1; l < cb; ++l) {
         MGLevel& L = M.lev[l];
         MGLevel& C = M.lev[l + 1];
-        smooth<4, true, true, false, false>(c, L, C, L.cor, L.res, L.cs, L.acf, nullptr, nullptr, C.res, nullptr,
-                                            ctl);
+        smooth<4, true, true, false>(c, L, C, L.cor, L.res, L.cs, L.acf, nullptr, nullptr, C.res, ctl);
     }
     {
         TraceScope ts(c, HPQ_TRACE_MG_COARSE);
@@ -691,18 +874,43 @@ void vcycle(hpq_ctx* c) {
         MGLevel& L = M.lev[l];
         MGLevel& C = M.lev[l + 1];
         const double* corc = (l + 1 == cb) ? C.cor : C.rescor;
-        smooth<4, false, false, true, false>(c, L, C, L.rescor, L.res, L.cs, L.acf, L.cor, corc, nullptr, nullptr,
-                                             ctl);
+        smooth<4, false, false, true>(c, L, C, L.rescor, L.res, L.cs, L.acf, L.cor, corc, nullptr, ctl);
     }
     // Level 0: the up-sweep's four sweeps and the cycle's final four sweeps + residual are
     // eight consecutive sweeps from x + I(cor_1), where x is the current iterate; the
-    // intermediate after four sweeps is not needed. x alternates between cor[0] and msol.
-    MGLevel& L0 = M.lev[0];
+    // intermediate after four sweeps is not needed.
     MGLevel& L1 = M.lev[1];
     const double* corc = (cb == 1) ? L1.cor : L1.rescor;
-    smooth<8, false, true, true, true>(c, L0, L1, M.msol, M.mrhs, L0.cs, L0.acf, L0.cor, corc, L1.res,
-                                       &ctl->norm_bits, ctl);
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(1), 0, c->stream, ctl);
+    smooth<8, false, true, true, false, true>(c, S.L0, L1, S.buf[(m + 1) & 1], S.rhs0, S.rcs, S.L0.acf, S.buf[m & 1],
+                                              corc, L1.res, ctl, M.d_norms);
+    const Check chk{ctl, S.other, M.d_mirror, M.d_norms, level0_blocks(S.L0), ++M.seq, maxiter, 0.0, 0.0};
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(NT), 0, c->stream, chk);
+}
+
+struct Published {
+    int seq, iters, status;
+    bool done;
+};
+
+Published decode(unsigned long long w) {
+    return {int(unsigned(w & 0xffffffffull)), int((w >> 32) & 0xffffff), int((w >> 56) & 0x7f), (w >> 63) != 0};
+}
+
+// waits until the device has published sequence number seq
+Published wait_mirror(hpq_ctx* c, int seq) {
+    MultiGrid& M = c->mg;
+    unsigned long long* word = &M.h_mirror->word;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (long spin = 0;; ++spin) {
+        const Published p = decode(__atomic_load_n(word, __ATOMIC_ACQUIRE));
+        if (p.seq == seq) return p;
+        if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30)) break;
+    }
+    // the device has not reported: let the stream drain, then look once more
+    HPQ_HIP(hipStreamSynchronize(c->stream));
+    const Published p = decode(__atomic_load_n(word, __ATOMIC_ACQUIRE));
+    if (p.seq != seq) throw HpqError(HPQ_E_HIP, "hpmg: the device did not report convergence");
+    return p;
 }
 
 }  // namespace
@@ -711,45 +919,82 @@ void vcycle(hpq_ctx* c) {
 int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, long cstride, double tol_rel,
               double tol_abs, int maxiter, int* iters) {
     MultiGrid& M = c->mg;
-    MGLevel& L0 = M.lev[0];
     MGLevel& L1 = M.lev[1];
-    MGCtl* ctl = M.d_ctl;
-    const int sh = M.cc ? 0 : 1;
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, c->stream, ctl, maxiter);
-    {
-        const int n = (L0.ihi - L0.ilo + 1) * (L0.jhi - L0.jlo + 1);
-        hipLaunchKernelGGL(k_mg_load, grid1d(n, 256), dim3(256), 0, c->stream, L0, sh, c->g, sol, rhs, acf, cstride,
-                           M.msol, M.mrhs, ctl);
+    if (M.ctl_dirty) {
+        HPQ_HIP(hipMemsetAsync(M.d_ctl, 0, 2 * sizeof(MGCtl), c->stream));
+        M.ctl_dirty = false;
     }
+    Solve S;
+    S.ctl = M.d_ctl + M.ctl_pick;
+    S.other = M.d_ctl + (M.ctl_pick ^ 1);
+    S.L0 = M.lev[0];
+    const int sh = M.cc ? 0 : 1;
+    // node-centred level 0 (odd n) has the guarded plane's shape: work on the caller's
+    // planes directly when the two components are one plane apart
+    const bool alias = !M.cc && cstride == S.L0.cs;
+    if (alias) {
+        S.rhs0 = rhs;
+        S.rcs = cstride;
+        S.L0.acf = const_cast<double*>(acf);
+        S.buf[0] = S.L0.cor;
+        S.buf[1] = sol;
+    } else {
+        const int n = (S.L0.ihi - S.L0.ilo + 1) * (S.L0.jhi - S.L0.jlo + 1);
+        hipLaunchKernelGGL(k_mg_load, grid1d(n, 256), dim3(256), 0, c->stream, S.L0, sh, c->g, sol, rhs, acf, cstride,
+                           M.lev[0].acf, M.msol, M.mrhs);
+        S.rhs0 = M.mrhs;
+        S.rcs = S.L0.cs;
+        S.buf[0] = S.L0.cor;
+        S.buf[1] = M.msol;
+    }
+    M.ctl_dirty = true;   // until the solve ends normally
     // average_down_acoef (:1641-1661): the tiled levels here, the coarse ones inside k_coarse
     for (int l = 1; l <= M.coarse_begin; ++l)
-        launch_restrict(c, M.lev[l], M.lev[l - 1], M.lev[l].acf, M.lev[l - 1].acf, 1);
-    // initial smoothing from the loaded guess (msol) into cor[0], which is the iterate
-    // after zero V-cycles
-    smooth<4, false, true, false, false>(c, L0, L1, L0.cor, M.mrhs, L0.cs, L0.acf, M.msol, nullptr, L1.res,
-                                         &ctl->res0_bits, nullptr);
-    hipLaunchKernelGGL(k_target, dim3(1), dim3(1), 0, c->stream, ctl, tol_rel, tol_abs);
+        launch_restrict(c, M.lev[l], l == 1 ? S.L0 : M.lev[l - 1], M.lev[l].acf, l == 1 ? S.L0.acf : M.lev[l - 1].acf,
+                        1);
+    // initial smoothing from the guess (buf[1]) into buf[0], the iterate after zero cycles;
+    // its last workgroup sets the convergence target
+    smooth<4, false, true, false, true, true>(c, S.L0, L1, S.buf[0], S.rhs0, S.rcs, S.L0.acf, S.buf[1], nullptr, L1.res,
+                                              nullptr, M.d_norms);
+    const Check chk0{S.ctl, S.other, M.d_mirror, M.d_norms, level0_blocks(S.L0), 0, maxiter, tol_rel, tol_abs};
+    hipLaunchKernelGGL(k_target, dim3(1), dim3(NT), 0, c->stream, chk0);
     HPQ_HIP(hipGetLastError());
     int batch = std::max(1, std::min(M.last_iters, maxiter));
     int issued = 0;
+    Published pub{};
     while (true) {
-        for (int b = 0; b < batch && issued < maxiter; ++b, ++issued) vcycle(c);
+        for (int b = 0; b < batch && issued < maxiter; ++b, ++issued) vcycle(c, S, issued, maxiter);
         HPQ_HIP(hipGetLastError());
-        HPQ_HIP(hipMemcpyAsync(M.h_ctl, ctl, sizeof(MGCtl), hipMemcpyDeviceToHost, c->stream));
-        HPQ_HIP(hipStreamSynchronize(c->stream));
-        if (M.h_ctl->done || issued >= maxiter) break;
+        if (issued == 0) break;   // maxiter == 0
+
+        pub = wait_mirror(c, M.seq);
+        if (pub.done || issued >= maxiter) break;
         batch = 1;
     }
-    const int it = M.h_ctl->iters;
-    const int status = M.h_ctl->status;
+    int it = 0, status = 0;
+    if (issued) {
+        it = pub.iters;
+        status = pub.status;
+    } else {
+        // no V-cycle allowed: converged only if the initial residual met the target
+        MGCtl h;
+        HPQ_HIP(hipMemcpyAsync(&h, S.ctl, sizeof(MGCtl), hipMemcpyDeviceToHost, c->stream));
+        HPQ_HIP(hipStreamSynchronize(c->stream));
+        status = h.done ? HPQ_OK : HPQ_E_MG_NOT_CONVERGED;
+    }
     M.last_iters = std::max(1, it);
     if (iters) *iters = it;
-    {
-        const int n = (L0.vihi - L0.vilo + 1) * (L0.vjhi - L0.vjlo + 1);
-        hipLaunchKernelGGL(k_mg_store, grid1d(n, 256), dim3(256), 0, c->stream, L0, sh, c->g, sol, cstride, M.msol,
-                           ctl);
+    // the result is buf[it & 1]; in the aliased layout buf[1] is sol itself
+    if (!(alias && (it & 1))) {
+        const int n = (S.L0.vihi - S.L0.vilo + 1) * (S.L0.vjhi - S.L0.vjlo + 1);
+        hipLaunchKernelGGL(k_mg_store, grid1d(n, 256), dim3(256), 0, c->stream, S.L0, sh, c->g, sol, cstride,
+                           S.buf[it & 1]);
     }
     HPQ_HIP(hipGetLastError());
+    if (issued) {
+        M.ctl_pick ^= 1;
+        M.ctl_dirty = false;
+    }
     return status;
 }
 

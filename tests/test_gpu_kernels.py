@@ -155,8 +155,16 @@ def test_poisson_bxby_predictor_corrector():
         assert e <= TOL, (name, e)
 
 
+# (tol_rel, maxiter): converged after odd and even numbers of V-cycles, converged before
+# the first V-cycle (tolerance above the initial residual), stopped by maxiter (odd, even, 0)
+MG_RUNS = [(1e-4, 200), (1e-7, 200), (1e-10, 200), (1.0, 200), (1e-300, 1), (1e-300, 2), (1e-300, 0)]
+ORACLE_RC = {0: 0, 1: 3, 2: 4}
+
+
 @pytest.mark.parametrize("n", [63, 64, 127, 32])
 def test_mg_solve1_bit_exact(n):
+    """Consecutive solves on one context (each starts from the previous solution) against
+    the oracle: same return code, same V-cycle count, identical planes."""
     cfg = small_config(n=n)
     sim, orc = _pair(cfg, n_particles=0)
     st = orc.store()
@@ -164,14 +172,18 @@ def test_mg_solve1_bit_exact(n):
     st[orc.layout["chi"]] = rng.uniform(0.0, 2.0, st[0].shape)
     orc.set_store(st)
     sim.store.copy_(torch.as_tensor(st))
-    rc_g, it_g = sim.mg_solve1(tol_rel=1e-10)
-    rc_o, it_o = orc.mg_solve1(tol_rel=1e-10)
-    assert rc_g == 0 and rc_o == 0
-    assert it_g == it_o
-    g, o = sim.store_host(), orc.store()
-    for name in ("Bx", "By"):
-        assert np.array_equal(g[sim.layout[name]], o[orc.layout[name]]), (name, rel_err(g[sim.layout[name]],
-                                                                                        o[orc.layout[name]]))
+    iters = set()
+    for tol, maxiter in MG_RUNS:
+        rc_g, it_g = sim.mg_solve1(tol_rel=tol, maxiter=maxiter)
+        rc_o, it_o = orc.mg_solve1(tol_rel=tol, maxiter=maxiter)
+        assert rc_g == ORACLE_RC[rc_o], (tol, maxiter, rc_g, rc_o)
+        assert it_g == it_o, (tol, maxiter, it_g, it_o)
+        iters.add(it_o)
+        g, o = sim.store_host(), orc.store()
+        for name in ("Bx", "By"):
+            assert np.array_equal(g[sim.layout[name]], o[orc.layout[name]]), (
+                tol, maxiter, name, rel_err(g[sim.layout[name]], o[orc.layout[name]]))
+    assert 0 in iters and any(i % 2 == 1 for i in iters) and any(i % 2 == 0 and i > 0 for i in iters), iters
 
 
 def test_mg_nonconvergence_reported():
