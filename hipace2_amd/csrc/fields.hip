@@ -4,7 +4,10 @@
 // the relative B-field error of the predictor-corrector loop
 // (Fields::ComputeRelBFieldError, src/fields/Fields.cpp:1229-1286) and the per-slice
 // field diagnostics (sum |F| over valid cells, tests/checksum/backend/openpmd_backend.py).
-// Up to 8 planes are handled per launch so a slice-store update is one kernel.
+// Up to 16 planes are handled per launch so a slice-store update is one kernel; the end of
+// a slice (diagnostics, ShiftSlices and the next slice's InitializeSlices) is one pass.
+#include <vector>
+
 #include "internal.hpp"
 
 namespace hpq {
@@ -13,8 +16,8 @@ namespace {
 
 struct PlaneSet {
     int n;
-    int dst[8];
-    int src[8];
+    int dst[16];
+    int src[16];
     double v;
 };
 
@@ -24,18 +27,87 @@ __global__ void k_fill(double* __restrict__ store, long plane, PlaneSet s) {
     for (int k = 0; k < s.n; ++k) store[long(s.dst[k]) * plane + t] = s.v;
 }
 
+// copies applied in list order at every cell (a later copy may read an earlier one's dst)
 __global__ void k_copy(double* __restrict__ store, long plane, PlaneSet s) {
     const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= plane) return;
-    double v[8];
-    for (int k = 0; k < s.n; ++k) v[k] = store[long(s.src[k]) * plane + t];
-    for (int k = 0; k < s.n; ++k) store[long(s.dst[k]) * plane + t] = v[k];
+    for (int k = 0; k < s.n; ++k) store[long(s.dst[k]) * plane + t] = store[long(s.src[k]) * plane + t];
 }
 
-__global__ void k_add(double* __restrict__ store, long plane, int dst, int src) {
+// dst[k] += src[k]
+__global__ void k_add(double* __restrict__ store, long plane, PlaneSet s) {
     const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= plane) return;
-    store[long(dst) * plane + t] += store[long(src) * plane + t];
+    for (int k = 0; k < s.n; ++k) store[long(s.dst[k]) * plane + t] += store[long(s.src[k]) * plane + t];
+}
+
+// End of a slice in one pass over the guarded plane: sum |F| of the diagnostic
+// components over valid cells (per-workgroup partial sums, accumulated over slices in
+// partial[block][k], no atomics), then the copies of ShiftSlices in order, then the zeroing
+// of InitializeSlices for the next slice. Per cell the reads come before the writes.
+constexpr int SE_T = 256, SE_DIAG = 24;
+struct SliceEnd {
+    int ndiag;
+    int diag[SE_DIAG];
+    PlaneSet shift;   // dst <- src, in order
+    PlaneSet zero;    // dst <- 0
+};
+
+template <bool DIAG, bool SHIFT>
+__global__ __launch_bounds__(SE_T) void k_slice_end(Geom g, double* __restrict__ store, SliceEnd e,
+                                                    double* __restrict__ partial) {
+    __shared__ double red[SE_T / 64][SE_DIAG];
+    const int plane = int(g.plane);
+    const int t = blockIdx.x * SE_T + threadIdx.x;
+    double acc[SE_DIAG];
+#pragma unroll
+    for (int k = 0; k < SE_DIAG; ++k) acc[k] = 0.0;
+    if (DIAG && t < plane) {
+        const int jj = t / g.ldx, ii = t - jj * g.ldx;
+        if (ii >= g.ng && ii < g.ng + g.nx && jj >= g.ng && jj < g.ng + g.ny) {
+#pragma unroll
+            for (int k = 0; k < SE_DIAG; ++k)
+                if (k < e.ndiag) acc[k] = fabs(store[long(e.diag[k]) * plane + t]);
+        }
+    }
+    if (SHIFT && t < plane) {
+        for (int k = 0; k < e.shift.n; ++k)
+            store[long(e.shift.dst[k]) * plane + t] = store[long(e.shift.src[k]) * plane + t];
+        for (int k = 0; k < e.zero.n; ++k) store[long(e.zero.dst[k]) * plane + t] = 0.0;
+    }
+    if (DIAG) {
+#pragma unroll
+        for (int k = 0; k < SE_DIAG; ++k) {
+            if (k >= e.ndiag) break;
+            double v = acc[k];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < e.ndiag) {
+            double v = red[0][threadIdx.x];
+            for (int w = 1; w < SE_T / 64; ++w) v += red[w][threadIdx.x];
+            partial[long(blockIdx.x) * SE_DIAG + threadIdx.x] += v;
+        }
+    }
+}
+
+// checksums[k] += sum over workgroups of partial[b][k]; partial cleared
+__global__ __launch_bounds__(256) void k_diag_finalize(double* __restrict__ partial, int nblk, int ndiag,
+                                                       const int* __restrict__ comp_of, double* __restrict__ sums) {
+    __shared__ double red[256 / 64];
+    for (int k = 0; k < ndiag; ++k) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += 256) {
+            v += partial[long(b) * SE_DIAG + k];
+            partial[long(b) * SE_DIAG + k] = 0.0;
+        }
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) sums[comp_of[k]] += red[0] + red[1] + red[2] + red[3];
+    }
 }
 
 __global__ void k_lincomb(double* __restrict__ store, long plane, int dst, double a, int sa, double b, int sb) {
@@ -58,27 +130,6 @@ __global__ void k_sxsy(Geom g, const double* __restrict__ jzb, const double* __r
     const double dz_jyb = (pjy[k] - njy[k]) / (2.0 * g.dz);
     Sy[k] = mu0 * (-dy_jzb + dz_jyb);
     Sx[k] = -mu0 * (-dx_jzb + dz_jxb);
-}
-
-// sum |F| over valid cells of every component: one block per (component, 8 rows)
-__global__ void k_diag(Geom g, const double* __restrict__ store, double* __restrict__ sums) {
-    __shared__ double red[256];
-    const int comp = blockIdx.y;
-    const double* P = store + long(comp) * g.plane;
-    double acc = 0.0;
-    const int j0 = blockIdx.x * 8;
-    for (int jj = 0; jj < 8; ++jj) {
-        const int j = j0 + jj;
-        if (j >= g.ny) break;
-        for (int i = threadIdx.x; i < g.nx; i += blockDim.x) acc += fabs(P[g.idx(i, j)]);
-    }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(sums + comp, red[0]);
 }
 
 __global__ void k_capture(Geom g, const double* __restrict__ P, double* __restrict__ dst) {
@@ -140,8 +191,16 @@ void plane_copy(hpq_ctx* c, double* store, const int* dst, const int* src, int n
     HPQ_HIP(hipGetLastError());
 }
 
-void plane_add(hpq_ctx* c, double* store, int dst, int src) {
-    hipLaunchKernelGGL(k_add, grid1d(c->g.plane, 256), dim3(256), 0, c->stream, store, c->g.plane, dst, src);
+void plane_add(hpq_ctx* c, double* store, const int* dst, const int* src, int ncomps) {
+    PlaneSet s{};
+    for (int k = 0; k < ncomps; ++k)
+        if (dst[k] >= 0) {
+            s.dst[s.n] = dst[k];
+            s.src[s.n] = src[k];
+            ++s.n;
+        }
+    if (s.n == 0) return;
+    hipLaunchKernelGGL(k_add, grid1d(c->g.plane, 256), dim3(256), 0, c->stream, store, c->g.plane, s);
     HPQ_HIP(hipGetLastError());
 }
 
@@ -161,17 +220,76 @@ void sxsy_from_beam(hpq_ctx* c, double* store) {
     HPQ_HIP(hipGetLastError());
 }
 
-void diag_accumulate(hpq_ctx* c, const double* store, int islice) {
-    const Geom& g = c->g;
-    if (c->checksums) {
-        hipLaunchKernelGGL(k_diag, dim3((g.ny + 7) / 8, c->L.ncomp), dim3(256), 0, c->stream, g, store, c->checksums);
-        HPQ_HIP(hipGetLastError());
+// Components summed by the field diagnostics: every plane of the layout except the
+// scratch planes of the slice shift and of the predictor-corrector loop.
+void diag_comps(const hpq_ctx* c, std::vector<int>& out) {
+    const Layout& L = c->L;
+    const int skip[] = {L.next_jx_beam, L.next_jy_beam, L.prev_jx_beam, L.prev_jy_beam, L.ions_rhomjz, L.next_jx,
+                        L.next_jy,      L.prev_Bx,      L.prev_By,      L.prev_jx,      L.prev_jy,     L.pciter_Bx,
+                        L.pciter_By,    L.pcprev_Bx,    L.pcprev_By};
+    out.clear();
+    for (int k = 0; k < L.ncomp; ++k) {
+        bool s = false;
+        for (int q : skip) s = s || q == k;
+        if (!s) out.push_back(k);
     }
+}
+
+void captures(hpq_ctx* c, const double* store, int islice) {
+    const Geom& g = c->g;
     for (const Capture& cap : c->captures) {
         hipLaunchKernelGGL(k_capture, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g,
                            store + long(cap.comp) * g.plane, cap.dst + long(islice) * g.nx * g.ny);
         HPQ_HIP(hipGetLastError());
     }
+}
+
+void slice_end(hpq_ctx* c, double* store, const int* shift_dst, const int* shift_src, int nshift, const int* zero,
+               int nzero) {
+    const Geom& g = c->g;
+    const bool diag = c->checksums != nullptr;
+    const bool shift = nshift > 0 || nzero > 0;
+    if (!diag && !shift) return;
+    SliceEnd e{};
+    std::vector<int> dc;
+    diag_comps(c, dc);
+    if (int(dc.size()) > SE_DIAG) throw HpqError(HPQ_E_UNSUPPORTED, "slice_end: too many diagnostic components");
+    e.ndiag = int(dc.size());
+    for (int k = 0; k < e.ndiag; ++k) e.diag[k] = dc[k];
+    for (int k = 0; k < nshift; ++k) {
+        e.shift.dst[e.shift.n] = shift_dst[k];
+        e.shift.src[e.shift.n] = shift_src[k];
+        ++e.shift.n;
+    }
+    for (int k = 0; k < nzero; ++k)
+        if (zero[k] >= 0) e.zero.dst[e.zero.n++] = zero[k];
+    const long nblk = (g.plane + SE_T - 1) / SE_T;
+    if (diag && !c->diag_partial) {
+        HPQ_HIP(hipMalloc(&c->diag_partial, sizeof(double) * SE_DIAG * nblk));
+        HPQ_HIP(hipMemset(c->diag_partial, 0, sizeof(double) * SE_DIAG * nblk));
+        HPQ_HIP(hipMalloc(&c->diag_comp_of, sizeof(int) * SE_DIAG));
+        HPQ_HIP(hipMemcpy(c->diag_comp_of, dc.data(), sizeof(int) * dc.size(), hipMemcpyHostToDevice));
+    }
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(SE_T), 0, c->stream, g, store, e, c->diag_partial);
+    };
+    if (diag && shift) launch(k_slice_end<true, true>);
+    else if (diag) launch(k_slice_end<true, false>);
+    else launch(k_slice_end<false, true>);
+    HPQ_HIP(hipGetLastError());
+    c->diag_pending = c->diag_pending || diag;
+}
+
+// adds the pending per-workgroup diagnostic sums into the caller's checksums
+void diag_finalize(hpq_ctx* c) {
+    if (!c->diag_pending || !c->checksums || !c->diag_partial) return;
+    std::vector<int> dc;
+    diag_comps(c, dc);
+    const long nblk = (c->g.plane + SE_T - 1) / SE_T;
+    hipLaunchKernelGGL(k_diag_finalize, dim3(1), dim3(256), 0, c->stream, c->diag_partial, int(nblk), int(dc.size()),
+                       c->diag_comp_of, c->checksums);
+    HPQ_HIP(hipGetLastError());
+    c->diag_pending = false;
 }
 
 double rel_b_error(hpq_ctx* c, const double* store, int bx, int by, int bxi, int byi) {

@@ -68,41 +68,62 @@ class Fields {
 public:
     Fields(hpq_ctx* c, double* store) : c_(c), s_(store), L_(c->L) {}
 
-    // Fields::InitializeSlices, :536-586 (level 0)
-    void InitializeSlices() {
+    // Fields::InitializeSlices, :536-586 (level 0): the planes zeroed before a slice
+    int InitZeros(int* z) const {
         if (c_->p.explicit_solver) {
-            const int z[] = {L_.chi, L_.Sy, L_.Sx, L_.ExmBy, L_.EypBx, L_.jz_beam, L_.rhomjz, L_.next_jx_beam};
-            plane_fill(c_, s_, z, 8, 0.0);
-            const int z2[] = {L_.next_jy_beam, L_.rho};
-            plane_fill(c_, s_, z2, 2, 0.0);
-        } else {
-            const int z[] = {L_.ExmBy, L_.EypBx, L_.jx, L_.jy, L_.jz, L_.rhomjz, L_.rho};
-            plane_fill(c_, s_, z, 7, 0.0);
+            const int v[] = {L_.chi, L_.Sy, L_.Sx, L_.ExmBy, L_.EypBx, L_.jz_beam, L_.rhomjz, L_.next_jx_beam,
+                             L_.next_jy_beam, L_.rho};
+            for (int k = 0; k < 10; ++k) z[k] = v[k];
+            return 10;
         }
+        const int v[] = {L_.ExmBy, L_.EypBx, L_.jx, L_.jy, L_.jz, L_.rhomjz, L_.rho};
+        for (int k = 0; k < 7; ++k) z[k] = v[k];
+        return 7;
     }
-    // Fields::ShiftSlices, :589-604
-    void ShiftSlices() {
+    void InitializeSlices() {
+        int z[16];
+        const int n = InitZeros(z);
+        plane_fill(c_, s_, z, n, 0.0);
+    }
+    // Fields::ShiftSlices, :589-604: copies in the order they apply at every cell
+    int ShiftCopies(int* d, int* s) const {
         if (c_->p.explicit_solver) {
-            const int d1[] = {L_.prev_jx_beam, L_.prev_jy_beam};
-            const int s1[] = {L_.jx_beam, L_.jy_beam};
-            plane_copy(c_, s_, d1, s1, 2);
-            const int d2[] = {L_.jx_beam, L_.jy_beam, L_.jx, L_.jy};
-            const int s2[] = {L_.next_jx_beam, L_.next_jy_beam, L_.next_jx_beam, L_.next_jy_beam};
-            plane_copy(c_, s_, d2, s2, 4);
-        } else {
-            const int d1[] = {L_.pcprev_Bx, L_.pcprev_By};
-            const int s1[] = {L_.prev_Bx, L_.prev_By};
-            plane_copy(c_, s_, d1, s1, 2);
-            const int d2[] = {L_.prev_Bx, L_.prev_By, L_.prev_jx, L_.prev_jy};
-            const int s2[] = {L_.Bx, L_.By, L_.jx, L_.jy};
-            plane_copy(c_, s_, d2, s2, 4);
+            const int dv[] = {L_.prev_jx_beam, L_.prev_jy_beam, L_.jx_beam, L_.jy_beam, L_.jx, L_.jy};
+            const int sv[] = {L_.jx_beam, L_.jy_beam, L_.next_jx_beam, L_.next_jy_beam, L_.next_jx_beam,
+                              L_.next_jy_beam};
+            for (int k = 0; k < 6; ++k) { d[k] = dv[k]; s[k] = sv[k]; }
+            return 6;
         }
+        const int dv[] = {L_.pcprev_Bx, L_.pcprev_By, L_.prev_Bx, L_.prev_By, L_.prev_jx, L_.prev_jy};
+        const int sv[] = {L_.prev_Bx, L_.prev_By, L_.Bx, L_.By, L_.jx, L_.jy};
+        for (int k = 0; k < 6; ++k) { d[k] = dv[k]; s[k] = sv[k]; }
+        return 6;
+    }
+    void ShiftSlices() {
+        int d[8], s[8];
+        const int n = ShiftCopies(d, s);
+        plane_copy(c_, s_, d, s, n);
     }
     // Fields::AddRhoIons, :607-615
     void AddRhoIons() {
         if (L_.ions_rhomjz < 0) return;
-        plane_add(c_, s_, L_.rhomjz, L_.ions_rhomjz);
-        if (L_.rho >= 0) plane_add(c_, s_, L_.rho, L_.ions_rhomjz);
+        const int d[2] = {L_.rhomjz, L_.rho};
+        const int s[2] = {L_.ions_rhomjz, L_.ions_rhomjz};
+        plane_add(c_, s_, d, s, 2);
+    }
+    // The field diagnostics of this slice (sum |F|, FillDiagnostics), and, when another
+    // slice follows in the same pass, ShiftSlices plus that slice's InitializeSlices: one
+    // pass over the store. The plasma push that comes next reads none of these planes.
+    void EndSlice(int islice, bool chain) {
+        captures(c_, s_, islice);
+        if (chain) {
+            int d[8], s[8], z[16];
+            const int ns = ShiftCopies(d, s);
+            const int nz = InitZeros(z);
+            slice_end(c_, s_, d, s, ns, z, nz);
+        } else {
+            slice_end(c_, s_, nullptr, nullptr, 0, nullptr, 0);
+        }
     }
     // Fields::SolvePoissonPsiExmByEypBxEzBz, :841-957
     void SolvePoissonPsiExmByEypBxEzBz() {
@@ -188,10 +209,12 @@ public:
     Hipace(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store)
         : c_(c), p_(p), b_(b), s_(store), fields_(c, store), L_(c->L) {}
 
-    // Hipace::SolveOneSlice, :557-728 (level 0; no laser, SALAME, ionisation, collisions)
-    void SolveOneSlice(int islice) {
+    // Hipace::SolveOneSlice, :557-728 (level 0; no laser, SALAME, ionisation, collisions).
+    // initialized: the previous slice of the same pass has already run InitializeSlices for
+    // this one; chain: another slice follows in this pass.
+    void SolveOneSlice(int islice, bool initialized = false, bool chain = false) {
         const hpq_params& P = c_->p;
-        fields_.InitializeSlices();
+        if (!initialized) fields_.InitializeSlices();
         if (P.explicit_solver) {
             DepositCurrent(c_, p_, s_, L_.jx, L_.jy, -1, L_.rho, L_.chi, L_.rhomjz);
             DepositCurrentSlice(c_, b_, islice, s_, -1, -1, L_.jz_beam);
@@ -210,10 +233,10 @@ public:
         } else {
             PredictorCorrectorLoopToSolveBxBy(islice);
         }
-        diag_accumulate(c_, s_, islice);
+        fields_.EndSlice(islice, chain);
         advance_plasma(c_, p_, s_, false);
         c_->stats[3] += double(p_->n);
-        fields_.ShiftSlices();
+        if (!chain) fields_.ShiftSlices();
     }
 
     // Hipace::InitializeSxSyWithBeam, :745-790
@@ -367,6 +390,8 @@ void hpq_destroy(hpq_ctx* c) {
     poisson_free(c);
     mg_free(c);
     bins_free(c);
+    if (c->diag_partial) (void)hipFree(c->diag_partial);
+    if (c->diag_comp_of) (void)hipFree(c->diag_comp_of);
     if (c->d_count) (void)hipFree(c->d_count);
     if (c->h_count) (void)hipHostFree(c->h_count);
     if (c->d_red) (void)hipFree(c->d_red);
@@ -539,6 +564,7 @@ int hpq_solve_one_slice(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, doub
         check_plasma(p);
         if (islice < 0 || islice >= c->g.nz) throw HpqError(HPQ_E_INVALID, "slice index out of range");
         Hipace(c, p, b, store).SolveOneSlice(islice);
+        diag_finalize(c);
     });
 }
 
@@ -549,14 +575,17 @@ int hpq_run_slices(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* s
         check_plasma(p);
         if (first >= c->g.nz || last < 0 || first < last) throw HpqError(HPQ_E_INVALID, "bad slice range");
         Hipace h(c, p, b, store);
-        for (int isl = first; isl >= last; --isl) h.SolveOneSlice(isl);
+        for (int isl = first; isl >= last; --isl) h.SolveOneSlice(isl, isl != first, isl != last);
+        diag_finalize(c);
     });
 }
 
 int hpq_set_checksums(hpq_ctx* c, double* checksums) {
     if (!c) return HPQ_E_INVALID;
-    c->checksums = checksums;
-    return HPQ_OK;
+    return guarded(c, [&] {
+        diag_finalize(c);
+        c->checksums = checksums;
+    });
 }
 
 int hpq_add_capture(hpq_ctx* c, int comp, double* dst) {

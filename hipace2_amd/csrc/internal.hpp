@@ -170,6 +170,9 @@ struct hpq_ctx {
     double* d_red = nullptr;         // reduction scratch (8 doubles)
     double* h_red = nullptr;         // pinned
     double* checksums = nullptr;     // caller-owned device accumulator (may be null)
+    double* diag_partial = nullptr;  // per-workgroup diagnostic sums not yet in checksums
+    int* diag_comp_of = nullptr;
+    bool diag_pending = false;
     std::vector<hpq::Capture> captures;
     double stats[4] = {0, 0, 0, 0};
     hpq::Trace trace[HPQ_TRACE_N];
@@ -196,10 +199,13 @@ void deposit_beam(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const
 void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool temp_slice);
 void plane_fill(hpq_ctx* c, double* store, const int* comps, int ncomps, double v);
 void plane_copy(hpq_ctx* c, double* store, const int* dst, const int* src, int ncomps);
-void plane_add(hpq_ctx* c, double* store, int dst, int src);
+void plane_add(hpq_ctx* c, double* store, const int* dst, const int* src, int ncomps);
 void plane_lincomb(hpq_ctx* c, double* store, int dst, double a, int sa, double b, int sb);
 void sxsy_from_beam(hpq_ctx* c, double* store);
-void diag_accumulate(hpq_ctx* c, const double* store, int islice);
+void captures(hpq_ctx* c, const double* store, int islice);
+void slice_end(hpq_ctx* c, double* store, const int* shift_dst, const int* shift_src, int nshift, const int* zero,
+               int nzero);
+void diag_finalize(hpq_ctx* c);
 double rel_b_error(hpq_ctx* c, const double* store, int bx, int by, int bxi, int byi);
 void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p);
 int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof);

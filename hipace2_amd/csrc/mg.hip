@@ -43,7 +43,6 @@ constexpr int RW = TX + 2, RH = TY + 2;        // residual region: tile + 1 ring
 constexpr int COARSE_MAX_POINTS = 33 * 33;
 constexpr int CTX = 16, CTY = 32;              // coarse workgroup: CTX x CTY threads
 constexpr int COARSE_THREADS = CTX * CTY;
-constexpr int COARSE_ARRAYS = 8;               // acf, 1/c0, res (2), cor (2), rescor (2)
 constexpr size_t COARSE_LDS_MAX = 150 * 1024;
 constexpr int COARSE_MAX_LEVELS = 16;
 
@@ -260,8 +259,14 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
             }
         }
     }
-    // start values: all loads first, then the LDS stores
-    double v0[NLD], v1[NLD];
+    // start values: phi_in (and the coarse correction under the tile) into registers with
+    // all loads in flight at once, then into LDS; the prolongation reads the coarse tile
+    // from LDS
+    constexpr int CLX = INTERP ? LX / 2 + 2 : 1, CLY = INTERP ? LY / 2 + 2 : 1;
+    constexpr int NCL = INTERP ? (CLX * CLY + NT - 1) / NT : 1;
+    __shared__ double cc[2][CLY][CLX];
+    const int ic0 = cdiv2(ti0), jc0 = cdiv2(tj0);
+    double v0[NLD], v1[NLD], cv0[NCL], cv1[NCL];
 #pragma unroll
     for (int it = 0; it < NLD; ++it) {
         const int e = threadIdx.x + it * NT;
@@ -270,25 +275,51 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
         v0[it] = v1[it] = 0.0;
         if (!ZERO && e < LX * LY && in_valid(L, i, j)) {
             const long k = at(L, i, j);
-            if (INTERP) {
-                auto c0 = [&](int a, int b) { return cor_c[at(C, a, b)]; };
-                auto c1 = [&](int a, int b) { return cor_c[at(C, a, b) + C.cs]; };
-                v0[it] = interp_add<CC>(i, j, phi_in[k], c0);
-                v1[it] = interp_add<CC>(i, j, phi_in[k + L.cs], c1);
-            } else {
-                v0[it] = phi_in[k];
-                v1[it] = phi_in[k + L.cs];
+            v0[it] = phi_in[k];
+            v1[it] = phi_in[k + L.cs];
+        }
+    }
+    if (INTERP) {
+#pragma unroll
+        for (int it = 0; it < NCL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            const int cy = e / CLX, cx = e - cy * CLX;
+            const int ic = ic0 + cx, jc = jc0 + cy;
+            cv0[it] = cv1[it] = 0.0;
+            if (e < CLX * CLY && ic >= C.ilo - 1 && ic <= C.ihi + 1 && jc >= C.jlo - 1 && jc <= C.jhi + 1) {
+                const long k = at(C, ic, jc);
+                cv0[it] = cor_c[k];
+                cv1[it] = cor_c[k + C.cs];
             }
         }
     }
     if (gate_done) return;
+    if (INTERP) {
+#pragma unroll
+        for (int it = 0; it < NCL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            if (e < CLX * CLY) {
+                const int cy = e / CLX, cx = e - cy * CLX;
+                cc[0][cy][cx] = cv0[it];
+                cc[1][cy][cx] = cv1[it];
+            }
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int it = 0; it < NLD; ++it) {
         const int e = threadIdx.x + it * NT;
         if (e < LX * LY) {
             const int ly = e / LX, lx = e - ly * LX;
-            s[0][ly][lx] = v0[it];
-            s[1][ly][lx] = v1[it];
+            double a0 = v0[it], a1 = v1[it];
+            if (INTERP && in_valid(L, ti0 + lx, tj0 + ly)) {
+                auto c0 = [&](int a, int b) { return cc[0][b - jc0][a - ic0]; };
+                auto c1 = [&](int a, int b) { return cc[1][b - jc0][a - ic0]; };
+                a0 = interp_add<CC>(ti0 + lx, tj0 + ly, a0, c0);
+                a1 = interp_add<CC>(ti0 + lx, tj0 + ly, a1, c1);
+            }
+            s[0][ly][lx] = a0;
+            s[1][ly][lx] = a1;
         }
     }
     double rmax = 0.0;
@@ -435,7 +466,9 @@ __global__ void k_restrict(MGLevel C, MGLevel F, double* __restrict__ crse, cons
 // ---------------------------------------------------------------------------------
 // Geometry of one coarse level and the offsets of its arrays in the work array (LDS, or
 // global memory when the levels do not fit). Every array covers the level box plus one
-// ghost ring, x fastest, as MGLevel's arrays do.
+// ghost ring, x fastest, as MGLevel's arrays do. The two components are independent
+// systems with the same acf, so each has a workgroup of its own: the work array holds one
+// component (a second one follows it when the arrays live in global memory).
 struct CLev {
     int ilo, ihi, jlo, jhi, vilo, vihi, vjlo, vjhi, ldx, cs;
     int acf, c0i, res, cor, rescor;
@@ -497,18 +530,11 @@ struct WaveMode {
 
 template <class Mode, bool CC>
 __device__ __forceinline__ void c_sweep(double* w, const CLev& L, int color) {
-    const int n1 = L.cs;
     Mode::color(L, color, [&](int i, int j) {
         const int k = lat(L, i, j);
-        const double* p0 = w + L.cor;
-        const double* p1 = w + L.cor + n1;
-        auto P0 = [&](int a, int b) { return p0[k + b * L.ldx + a]; };
-        auto P1 = [&](int a, int b) { return p1[k + b * L.ldx + a]; };
-        const double ci = w[L.c0i + k];
-        const double v0 = gs_point<CC>(L, i, j, P0, w[L.res + k], ci, L.facx, L.facy);
-        const double v1 = gs_point<CC>(L, i, j, P1, w[L.res + n1 + k], ci, L.facx, L.facy);
-        w[L.cor + k] = v0;
-        w[L.cor + n1 + k] = v1;
+        const double* p = w + L.cor;
+        auto P = [&](int a, int b) { return p[k + b * L.ldx + a]; };
+        w[L.cor + k] = gs_point<CC>(L, i, j, P, w[L.res + k], w[L.c0i + k], L.facx, L.facy);
     });
     Mode::sync();
 }
@@ -519,19 +545,14 @@ __device__ void c_down(double* w, const CLev& L, const CLev& C) {
     for (int color = 0; color < 4; ++color) c_sweep<Mode, CC>(w, L, color);
     Mode::valid(L, [&](int i, int j) {
         const int k = lat(L, i, j);
-        const double a = w[L.acf + k];
-        for (int n = 0; n < 2; ++n) {
-            const double* p = w + L.cor + n * L.cs;
-            auto P = [&](int di, int dj) { return p[k + dj * L.ldx + di]; };
-            w[L.rescor + n * L.cs + k] = residual1<CC>(L, i, j, P, w[L.res + n * L.cs + k], a, L.facx, L.facy);
-        }
+        const double* p = w + L.cor;
+        auto P = [&](int di, int dj) { return p[k + dj * L.ldx + di]; };
+        w[L.rescor + k] = residual1<CC>(L, i, j, P, w[L.res + k], w[L.acf + k], L.facx, L.facy);
     });
     Mode::sync();
     Mode::valid(C, [&](int i, int j) {
-        for (int n = 0; n < 2; ++n) {
-            auto fa = [&](int a, int b) { return w[L.rescor + n * L.cs + lat(L, a, b)]; };
-            w[C.res + n * C.cs + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
-        }
+        auto fa = [&](int a, int b) { return w[L.rescor + lat(L, a, b)]; };
+        w[C.res + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
     });
     Mode::sync();
 }
@@ -542,37 +563,37 @@ template <class Mode, bool CC>
 __device__ void c_up(double* w, const CLev& L, const CLev& C) {
     Mode::valid(L, [&](int i, int j) {
         const int k = lat(L, i, j);
-        for (int n = 0; n < 2; ++n) {
-            auto c = [&](int a, int b) { return w[C.cor + n * C.cs + lat(C, a, b)]; };
-            w[L.cor + n * L.cs + k] = interp_add<CC>(i, j, w[L.cor + n * L.cs + k], c);
-        }
+        auto c = [&](int a, int b) { return w[C.cor + lat(C, a, b)]; };
+        w[L.cor + k] = interp_add<CC>(i, j, w[L.cor + k], c);
     });
     Mode::sync();
     for (int color = 0; color < 4; ++color) c_sweep<Mode, CC>(w, L, color);
 }
 
-// Levels [cb, maxl] of one V-cycle: down-sweep with residual restriction, bottom solve
-// (CPU branch of MultiGrid::bottomsolve, :1583-1593), up-sweep. acf and res of level cb
-// are loaded; acf of the coarser levels is averaged down here; cor of level cb is written
-// back for the finer up-sweep.
+// Levels [cb, maxl] of one V-cycle for component blockIdx.x: down-sweep with residual
+// restriction, bottom solve (CPU branch of MultiGrid::bottomsolve, :1583-1593), up-sweep.
+// acf and res of level cb are loaded; acf of the coarser levels is averaged down here;
+// cor of level cb is written back for the finer up-sweep.
 template <bool CC, bool IN_LDS>
 __global__ __launch_bounds__(COARSE_THREADS) void k_coarse(const CArgs A, double* __restrict__ gwork,
                                                            const MGCtl* __restrict__ ctl) {
-    constexpr int NB = 12;   // loads in flight per thread
+    constexpr int NB = 8;   // loads in flight per thread
     extern __shared__ double sm[];
-    double* w = IN_LDS ? sm : gwork;
+    const int comp = blockIdx.x;
+    double* w = IN_LDS ? sm : gwork + long(comp) * A.work_doubles;
     const int nl = A.nlev;
     const CLev& T = A.lv[0];
     const int done = ctl ? ctl->done : 0;
-    // acf, res (2 comps) of level cb: up to NB loads per thread in flight at once
-    const int nload = 3 * T.cs;
+    const double* gres = A.res + long(comp) * A.gcs;
+    // acf, res of level cb: up to NB loads per thread in flight at once
+    const int nload = 2 * T.cs;
     for (int b0 = 0; b0 < nload; b0 += NB * COARSE_THREADS) {
         double v[NB];
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             const int t = b0 + q * COARSE_THREADS + threadIdx.x;
             v[q] = 0.0;
-            if (t < nload) v[q] = t < T.cs ? A.acf[t] : t < 2 * T.cs ? A.res[t - T.cs] : A.res[t - 2 * T.cs + A.gcs];
+            if (t < nload) v[q] = t < T.cs ? A.acf[t] : gres[t - T.cs];
         }
         if (done) return;
 #pragma unroll
@@ -583,7 +604,7 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse(const CArgs A, double
     }
     for (int l = 0; l < nl; ++l) {
         const CLev& L = A.lv[l];
-        for (int t = threadIdx.x; t < 2 * L.cs; t += COARSE_THREADS) w[L.cor + t] = 0.0;
+        for (int t = threadIdx.x; t < L.cs; t += COARSE_THREADS) w[L.cor + t] = 0.0;
     }
     __syncthreads();
     // average_down_acoef (:1641-1661) and the diagonal inverses
@@ -612,10 +633,8 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse(const CArgs A, double
     }
     __syncthreads();
     for (int l = min(nb, nl - 1) - 1; l >= 0; --l) c_up<BlockMode, CC>(w, A.lv[l], A.lv[l + 1]);
-    for (int t = threadIdx.x; t < T.cs; t += COARSE_THREADS) {
-        A.cor[t] = w[T.cor + t];
-        A.cor[t + A.gcs] = w[T.cor + T.cs + t];
-    }
+    double* gcor = A.cor + long(comp) * A.gcs;
+    for (int t = threadIdx.x; t < T.cs; t += COARSE_THREADS) gcor[t] = w[T.cor + t];
 }
 
 // level-0 load (copy path): acf, sol and rhs (2 comps) over the whole box
@@ -671,9 +690,9 @@ void coarse_args(MultiGrid& M) {
         L.cs = int(G.cs);
         L.acf = off; off += L.cs;
         L.c0i = off; off += L.cs;
-        L.res = off; off += 2 * L.cs;
-        L.cor = off; off += 2 * L.cs;
-        L.rescor = off; off += 2 * L.cs;
+        L.res = off; off += L.cs;
+        L.cor = off; off += L.cs;
+        L.rescor = off; off += L.cs;
         L.facx = 1. / (G.dx * G.dx);
         L.facy = 1. / (G.dy * G.dy);
     }
@@ -758,7 +777,7 @@ void mg_init(hpq_ctx* c) {
     const CArgs& A = *reinterpret_cast<const CArgs*>(M.coarse_args);
     M.coarse_lds_bytes = sizeof(double) * size_t(A.work_doubles);
     M.coarse_in_lds = M.coarse_lds_bytes <= COARSE_LDS_MAX;
-    if (!M.coarse_in_lds) HPQ_HIP(hipMalloc(&M.coarse_work, M.coarse_lds_bytes));
+    if (!M.coarse_in_lds) HPQ_HIP(hipMalloc(&M.coarse_work, 2 * M.coarse_lds_bytes));
     const size_t b0 = sizeof(double) * 2 * M.lev[0].cs;
     HPQ_HIP(hipMalloc(&M.msol, b0));
     HPQ_HIP(hipMalloc(&M.mrhs, b0));
@@ -853,17 +872,17 @@ void vcycle(hpq_ctx* c, const Solve& S, int m, int maxiter) {
         const CArgs& A = *reinterpret_cast<const CArgs*>(M.coarse_args);
         if (M.coarse_in_lds) {
             if (M.cc)
-                hipLaunchKernelGGL((k_coarse<true, true>), dim3(1), dim3(COARSE_THREADS), M.coarse_lds_bytes,
+                hipLaunchKernelGGL((k_coarse<true, true>), dim3(2), dim3(COARSE_THREADS), M.coarse_lds_bytes,
                                    c->stream, A, nullptr, ctl);
             else
-                hipLaunchKernelGGL((k_coarse<false, true>), dim3(1), dim3(COARSE_THREADS), M.coarse_lds_bytes,
+                hipLaunchKernelGGL((k_coarse<false, true>), dim3(2), dim3(COARSE_THREADS), M.coarse_lds_bytes,
                                    c->stream, A, nullptr, ctl);
         } else {
             if (M.cc)
-                hipLaunchKernelGGL((k_coarse<true, false>), dim3(1), dim3(COARSE_THREADS), 0, c->stream, A,
+                hipLaunchKernelGGL((k_coarse<true, false>), dim3(2), dim3(COARSE_THREADS), 0, c->stream, A,
                                    M.coarse_work, ctl);
             else
-                hipLaunchKernelGGL((k_coarse<false, false>), dim3(1), dim3(COARSE_THREADS), 0, c->stream, A,
+                hipLaunchKernelGGL((k_coarse<false, false>), dim3(2), dim3(COARSE_THREADS), 0, c->stream, A,
                                    M.coarse_work, ctl);
         }
     }

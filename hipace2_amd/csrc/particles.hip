@@ -90,20 +90,46 @@ __global__ __launch_bounds__(256) void k_push_plasma(Geom g, hpq_plasma p, const
             sx[k] = dshape<1, O>(xm, k);
             sy[k] = dshape<1, O>(ym, k);
         }
+        auto accumulate = [&](const Dsf& X, const Dsf& Y, double ps, double ez, double bx, double by, double bz) {
+            ExmByp += (X.ds * Y.s) * ps * g.dxi;
+            EypBxp += (X.s * Y.ds) * ps * g.dyi;
+            Ezp += (X.s * Y.s) * ez;
+            Bxp += (X.s * Y.s) * bx;
+            Byp += (X.s * Y.s) * by;
+            Bzp += (X.s * Y.s) * bz;
+        };
+        if (in_store(g, sx[0].cell, sy[0].cell) && in_store(g, sx[W - 1].cell, sy[W - 1].cell)) {
+            // the whole stencil lies in the guarded plane: every load issued up front
+            const long k0 = g.idx(sx[0].cell, sy[0].cell);
+            double ps[W][W], ez[W][W], bx[W][W], by[W][W], bz[W][W];
 #pragma unroll
-        for (int iy = 0; iy < W; ++iy) {
+            for (int iy = 0; iy < W; ++iy) {
 #pragma unroll
-            for (int ix = 0; ix < W; ++ix) {
-                const Dsf X = sx[ix], Y = sy[iy];
-                if (!in_store(g, X.cell, Y.cell)) continue;
-                const long k = g.idx(X.cell, Y.cell);
-                const double ps = Psi[k];
-                ExmByp += (X.ds * Y.s) * ps * g.dxi;
-                EypBxp += (X.s * Y.ds) * ps * g.dyi;
-                Ezp += (X.s * Y.s) * Ez[k];
-                Bxp += (X.s * Y.s) * Bx[k];
-                Byp += (X.s * Y.s) * By[k];
-                Bzp += (X.s * Y.s) * Bz[k];
+                for (int ix = 0; ix < W; ++ix) {
+                    const long k = k0 + long(iy) * g.ldx + ix;
+                    ps[iy][ix] = Psi[k];
+                    ez[iy][ix] = Ez[k];
+                    bx[iy][ix] = Bx[k];
+                    by[iy][ix] = By[k];
+                    bz[iy][ix] = Bz[k];
+                }
+            }
+#pragma unroll
+            for (int iy = 0; iy < W; ++iy) {
+#pragma unroll
+                for (int ix = 0; ix < W; ++ix)
+                    accumulate(sx[ix], sy[iy], ps[iy][ix], ez[iy][ix], bx[iy][ix], by[iy][ix], bz[iy][ix]);
+            }
+        } else {
+#pragma unroll
+            for (int iy = 0; iy < W; ++iy) {
+#pragma unroll
+                for (int ix = 0; ix < W; ++ix) {
+                    const Dsf X = sx[ix], Y = sy[iy];
+                    if (!in_store(g, X.cell, Y.cell)) continue;
+                    const long k = g.idx(X.cell, Y.cell);
+                    accumulate(X, Y, Psi[k], Ez[k], Bx[k], By[k], Bz[k]);
+                }
             }
         }
     }
