@@ -7,7 +7,9 @@
 // component. A chunk whose box is too large gets a square window around the mean start
 // cell; its particles outside the window add to the store directly. The per-particle
 // contributions are the reference's expressions in its operation order; only the order
-// of the floating-point sums differs (the reference's GPU path uses global atomics).
+// of the floating-point sums differs (the reference's GPU path uses global atomics), and in
+// the explicit Sx/Sy deposition the per-cell terms are regrouped around per-particle
+// coefficients (a few ulp per contribution).
 #include <climits>
 #include <cmath>
 
@@ -21,7 +23,13 @@ namespace {
 constexpr int DT = 256;                 // threads per workgroup
 constexpr int DPPT = 4;                 // particles per thread
 constexpr int DCHUNK = DT * DPPT;       // particles per workgroup
-constexpr int WIN_DOUBLES = 6144;       // 48 KiB of LDS per workgroup
+// LDS window sizes (doubles). A chunk of 1024 tile-sorted particles at 2 x 2 per cell
+// covers one 16 x 16-cell tile: 19 x 19 cells for a 3-point stencil, 21 x 21 for the
+// 5-point explicit one; the windows leave room for particles that drifted since the sort
+// while keeping four workgroups per CU resident.
+constexpr int WIN_DEPOSIT = 3072;       // 4 planes x 768 cells, 24 KiB
+constexpr int WIN_EXPLICIT = 4608;      // 6 planes x 768 cells, 36 KiB
+constexpr int WIN_BEAM = 3072;          // 3 planes x 1024 cells, 24 KiB
 
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void lds_add(double* p, double v) {
@@ -125,7 +133,7 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
                                                        double charge_mu0_mass_ratio, double max_qsa,
                                                        int* __restrict__ qsa_count) {
     constexpr int S = O + 1;
-    __shared__ double win[WIN_DOUBLES];
+    __shared__ double win[WIN_DEPOSIT];
     __shared__ int red[7 * DT / 64];
     const double clight = 1.0, clightinv = 1.0 / 1.0;
     const int comp[6] = {cjx, cjy, cjz, crho, cchi, crm};
@@ -164,7 +172,7 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
         ci[q] = shape<O>(xmid[q], 0).cell;
         cj[q] = shape<O>(ymid[q], 0).cell;
     }
-    const Win W = choose_window(ci, cj, act, S, ns > 0 ? WIN_DOUBLES / ns : 0, red);
+    const Win W = choose_window(ci, cj, act, S, ns > 0 ? WIN_DEPOSIT / ns : 0, red);
     const int cells = W.w * W.h;
     for (int t = threadIdx.x; t < cells * ns; t += DT) win[t] = 0.0;
     __syncthreads();
@@ -212,10 +220,10 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
                                                          int cEypBx, int cSy, int cSx, double charge_invvol_mu0,
                                                          double charge_mass_ratio) {
     constexpr int S = dshape_width<T, O>();
-    __shared__ double win[WIN_DOUBLES];
+    __shared__ double win[WIN_EXPLICIT];
     __shared__ int red[7 * DT / 64];
     const double clight = 1.0, clight_inv = 1.0 / 1.0;
-    double xmid[DPPT], ymid[DPPT], vx[DPPT], vy[DPPT], psi_inv[DPPT], cdm[DPPT];
+    double xmid[DPPT], ymid[DPPT], vx[DPPT], vy[DPPT], psi_inv[DPPT], cdm_[DPPT];
     int ci[DPPT], cj[DPPT];
     bool act[DPPT];
     const long base = long(blockIdx.x) * DCHUNK;
@@ -231,14 +239,14 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
         const double xp = p.x[ip], yp = p.y[ip];
         vx[q] = p.ux[ip] * psi_inv[q] * clight_inv;
         vy[q] = p.uy[ip] * psi_inv[q] * clight_inv;
-        cdm[q] = charge_invvol_mu0 * p.w[ip];
+        cdm_[q] = charge_invvol_mu0 * p.w[ip];
         xmid[q] = (xp - g.xoff) * g.dxi;
         ymid[q] = (yp - g.yoff) * g.dyi;
         ci[q] = dshape<T, O>(xmid[q], 0).cell;
         cj[q] = dshape<T, O>(ymid[q], 0).cell;
     }
     // window planes: Sy, Sx (accumulated), then copies of Bz, Ez, ExmBy, EypBx
-    const Win W = choose_window(ci, cj, act, S, WIN_DOUBLES / 6, red);
+    const Win W = choose_window(ci, cj, act, S, WIN_EXPLICIT / 6, red);
     const int cells = W.w * W.h;
     const double* Bz = store + long(cBz) * g.plane;
     const double* Ez = store + long(cEz) * g.plane;
@@ -262,11 +270,22 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
 #pragma unroll
     for (int q = 0; q < DPPT; ++q) {
         if (!act[q]) continue;
+        // Per particle, the reference's Sy/Sx contribution of a cell
+        //   Sy += cdm (-XsYs A q/psi + (-XdsYs dxi (-vx vy) - XsYds dyi (gamma/psi - vy^2 - 1)) c)
+        //   Sx += cdm ( XsYs B q/psi + ( XdsYs dxi (gamma/psi - vx^2 - 1) + XsYds dyi (-vx vy)) c)
+        // with A = -Bz vx + (Ez vy - ExmBy vx vy + EypBx (gamma/psi - vy^2)) / c and
+        // B = Bz vy + (Ez vx + ExmBy (gamma/psi - vx^2) - EypBx vx vy) / c (no laser term), is
+        // regrouped into per-particle coefficients times three shape-factor products.
         const double q_mass_ratio = charge_mass_ratio;
-        const double charge_density_mu0 = cdm[q];
+        const double cdm = cdm_[q];
+        const double pvx = vx[q], pvy = vy[q], pinv = psi_inv[q];
         const double Aabssqp = 0.0;
-        const double gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * psi_inv[q] * psi_inv[q] + vx[q] * vx[q] +
-                                        vy[q] * vy[q] + 1.0);
+        const double gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * pinv * pinv + pvx * pvx + pvy * pvy + 1.0);
+        const double vxvy = pvx * pvy;
+        const double gpx = gamma_psi - pvx * pvx, gpy = gamma_psi - pvy * pvy;
+        const double qp = q_mass_ratio * pinv;
+        const double ay = -cdm * qp, by = cdm * clight * g.dxi * vxvy, cy = -cdm * clight * g.dyi * (gpy - 1.0);
+        const double ax = cdm * qp, bx = cdm * clight * g.dxi * (gpx - 1.0), cx = -cdm * clight * g.dyi * vxvy;
         Dsf sx[S], sy[S];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
@@ -274,8 +293,6 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
             sy[k] = dshape<T, O>(ymid[q], k);
         }
         const bool local = W.holds(ci[q], cj[q], S);
-        const double AabssqDxp = 0.0, AabssqDyp = 0.0;
-        const double pvx = vx[q], pvy = vy[q], pinv = psi_inv[q];
 #pragma unroll
         for (int iy = 0; iy < S; ++iy) {
 #pragma unroll
@@ -292,26 +309,11 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
                 } else {
                     Bz_v = Bz[k]; Ez_v = Ez[k]; ExmBy_v = ExmBy[k]; EypBx_v = EypBx[k];
                 }
-                const double sy_add = charge_density_mu0 * (
-                    -X.s * Y.s * (
-                        -Bz_v * pvx
-                        + (Ez_v * pvy
-                        + ExmBy_v * (-pvx * pvy)
-                        + EypBx_v * (gamma_psi - pvy * pvy)) * clight_inv
-                        - 0.25 * AabssqDyp * q_mass_ratio * pinv
-                    ) * q_mass_ratio * pinv
-                    + (-X.ds * Y.s * g.dxi * (-pvx * pvy)
-                       - X.s * Y.ds * g.dyi * (gamma_psi - pvy * pvy - 1.0)) * clight);
-                const double sx_add = charge_density_mu0 * (
-                    +X.s * Y.s * (
-                        +Bz_v * pvy
-                        + (Ez_v * pvx
-                        + ExmBy_v * (gamma_psi - pvx * pvx)
-                        + EypBx_v * (-pvx * pvy)) * clight_inv
-                        - 0.25 * AabssqDxp * q_mass_ratio * pinv
-                    ) * q_mass_ratio * pinv
-                    + (+X.ds * Y.s * g.dxi * (gamma_psi - pvx * pvx - 1.0)
-                       + X.s * Y.ds * g.dyi * (-pvx * pvy)) * clight);
+                const double A = fma(-Bz_v, pvx, fma(EypBx_v, gpy, fma(-ExmBy_v, vxvy, Ez_v * pvy)) * clight_inv);
+                const double B = fma(Bz_v, pvy, fma(-EypBx_v, vxvy, fma(ExmBy_v, gpx, Ez_v * pvx)) * clight_inv);
+                const double ss = X.s * Y.s, ds = X.ds * Y.s, sd = X.s * Y.ds;
+                const double sy_add = fma(ay * A, ss, fma(by, ds, cy * sd));
+                const double sx_add = fma(ax * B, ss, fma(bx, ds, cx * sd));
                 if (local) {
                     lds_add(&win[kw], sy_add);
                     lds_add(&win[cells + kw], sx_add);
@@ -333,7 +335,7 @@ template <int O>
 __global__ __launch_bounds__(DT) void k_deposit_beam(Geom g, hpq_beam b, long first, long count,
                                                      double* __restrict__ store, int cjx, int cjy, int cjz) {
     constexpr int S = O + 1;
-    __shared__ double win[WIN_DOUBLES];
+    __shared__ double win[WIN_BEAM];
     __shared__ int red[7 * DT / 64];
     const double clightsq = 1.0 / (1.0 * 1.0);
     const int comp[3] = {cjx, cjy, cjz};
@@ -362,7 +364,7 @@ __global__ __launch_bounds__(DT) void k_deposit_beam(Geom g, hpq_beam b, long fi
         ci[q] = shape<O>(xmid[q], 0).cell;
         cj[q] = shape<O>(ymid[q], 0).cell;
     }
-    const Win W = choose_window(ci, cj, act, S, ns > 0 ? WIN_DOUBLES / ns : 0, red);
+    const Win W = choose_window(ci, cj, act, S, ns > 0 ? WIN_BEAM / ns : 0, red);
     const int cells = W.w * W.h;
     for (int t = threadIdx.x; t < cells * ns; t += DT) win[t] = 0.0;
     __syncthreads();

@@ -71,9 +71,12 @@ __global__ __launch_bounds__(SE_T) void k_slice_end(Geom g, double* __restrict__
         }
     }
     if (SHIFT && t < plane) {
-        for (int k = 0; k < e.shift.n; ++k)
-            store[long(e.shift.dst[k]) * plane + t] = store[long(e.shift.src[k]) * plane + t];
-        for (int k = 0; k < e.zero.n; ++k) store[long(e.zero.dst[k]) * plane + t] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k < e.shift.n) store[long(e.shift.dst[k]) * plane + t] = store[long(e.shift.src[k]) * plane + t];
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k < e.zero.n) store[long(e.zero.dst[k]) * plane + t] = 0.0;
     }
     if (DIAG) {
 #pragma unroll

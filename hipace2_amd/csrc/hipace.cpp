@@ -346,6 +346,8 @@ int hpq_create(const hpq_params* p, hpq_ctx** out) {
         if (p->nx < 4 || p->ny < 4 || p->nz < 1) throw HpqError(HPQ_E_INVALID, "grid too small");
         if (!(p->xhi > p->xlo && p->yhi > p->ylo && p->zhi > p->zlo)) throw HpqError(HPQ_E_INVALID, "empty domain");
         if (p->depos_order < 0 || p->depos_order > 3) throw HpqError(HPQ_E_UNSUPPORTED, "depos_order_xy must be 0..3");
+        if (p->precision != HPQ_PREC_FP64 && p->precision != HPQ_PREC_MIXED)
+            throw HpqError(HPQ_E_INVALID, "precision must be HPQ_PREC_FP64 or HPQ_PREC_MIXED");
         if (p->derivative_type < 0 || p->derivative_type > 2)
             throw HpqError(HPQ_E_UNSUPPORTED, "depos_derivative_type must be 0..2");
         if (p->depos_order == 0 && p->derivative_type == 0)

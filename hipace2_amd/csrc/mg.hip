@@ -205,6 +205,8 @@ __global__ __launch_bounds__(NT) void k_target(Check chk) {
 // Start: zero (ZERO), phi_in + I(cor_c) (INTERP) or phi_in. Result -> phi_out.
 // NORMS: the workgroup's max |residual| over its tile and (RHSMAX) max |rhs| over its
 // points go to norms[2 b], norms[2 b + 1] for the convergence check that follows.
+// ACF: acf is restricted to the coarse points the tile owns as well (average_down_acoef,
+// :1641-1661), so the first pass over each level prepares the next level's coefficient.
 // Every launch returns at once when the control block says the solve is finished (the
 // data loads are issued before that flag is tested, so its latency is hidden).
 //
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(NT) void k_target(Check chk) {
 // acf and 1/c0 in registers. A sweep of one colour touches one parity class only; the
 // thread reads all neighbours of its points first, then updates, then writes (points of
 // one colour never neighbour each other, so this is the order a global sweep has).
-template <int NSW, bool CC, bool ZERO, bool RES, bool INTERP, bool RHSMAX, bool NORMS>
+template <int NSW, bool CC, bool ZERO, bool RES, bool INTERP, bool RHSMAX, bool NORMS, bool ACF>
 __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __restrict__ phi_out,
                                                const double* __restrict__ rhs, long rcs,
                                                const double* __restrict__ acf, const double* __restrict__ phi_in,
@@ -229,14 +231,16 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
     __shared__ double s[2][LY][LX];
     __shared__ double r[2][RH][RW];
     __shared__ double red[2][NT / 64];
+    __shared__ double ra[ACF ? RH : 1][ACF ? RW : 1];
     static_assert(!NORMS || RES, "the norms come from the residual");
+    static_assert(!ACF || RES, "acf is restricted with the residual");
     const int gate_done = gate ? gate->done : 0;
     const int ti0 = L.vilo + int(blockIdx.x) * TX - HALO;
     const int tj0 = L.vjlo + int(blockIdx.y) * TY - HALO;
     const int base = (ti0 + tj0) & 1;
     const double facx = 1. / (L.dx * L.dx), facy = 1. / (L.dy * L.dy);
     // owned points: [parity][q]; pl = lx | ly << 8, or -1 outside the valid box / region
-    double r0[2][KP], r1[2][KP], ac[2][KP], ci[2][KP];
+    double r0[2][KP], r1[2][KP], ci[2][KP];
     int pl[2][KP];
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
@@ -244,7 +248,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
         for (int q = 0; q < KP; ++q) {
             const int e = threadIdx.x + q * NT;
             pl[par][q] = -1;
-            r0[par][q] = r1[par][q] = ac[par][q] = 0.0;
+            r0[par][q] = r1[par][q] = ci[par][q] = 0.0;
             if (e < NPAR) {
                 const int ly = 1 + e / HW;
                 const int lx = 1 + 2 * (e - (ly - 1) * HW) + ((par + base + ly + 1) & 1);
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                     pl[par][q] = lx | (ly << 8);
                     r0[par][q] = rhs[k];
                     r1[par][q] = rhs[k + rcs];
-                    ac[par][q] = acf[k];
+                    ci[par][q] = acf[k];   // acf for now, 1/c0 below
                 }
             }
         }
@@ -327,42 +331,48 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
     for (int par = 0; par < 2; ++par) {
 #pragma unroll
         for (int q = 0; q < KP; ++q) {
-            ci[par][q] = 0.0;
             if (pl[par][q] < 0) continue;
             const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
-            ci[par][q] = c0_inverse<CC>(L, ti0 + lx, tj0 + ly, ac[par][q], facx, facy);
+            ci[par][q] = c0_inverse<CC>(L, ti0 + lx, tj0 + ly, ci[par][q], facx, facy);
             if (RHSMAX) rmax = fmax(rmax, fmax(fabs(r0[par][q]), fabs(r1[par][q])));
         }
     }
     __syncthreads();
-    // one colour: all reads, then all updates, then all writes
+    // one colour: reads, then updates, then writes, in batches of up to KB points
+    constexpr int KB = KP > 2 ? (KP + 1) / 2 : KP;
     auto sweep = [&](auto PARC) {
         constexpr int par = decltype(PARC)::value;
-        double w0[KP], e0[KP], s0[KP], n0[KP], w1[KP], e1[KP], s1[KP], n1[KP];
 #pragma unroll
-        for (int q = 0; q < KP; ++q) {
-            if (pl[par][q] < 0) continue;
-            const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
-            w0[q] = s[0][ly][lx - 1]; e0[q] = s[0][ly][lx + 1]; s0[q] = s[0][ly - 1][lx]; n0[q] = s[0][ly + 1][lx];
-            w1[q] = s[1][ly][lx - 1]; e1[q] = s[1][ly][lx + 1]; s1[q] = s[1][ly - 1][lx]; n1[q] = s[1][ly + 1][lx];
-        }
-        double u0[KP], u1[KP];
+        for (int q0 = 0; q0 < KP; q0 += KB) {
+            double w0[KB], e0[KB], s0[KB], n0[KB], w1[KB], e1[KB], s1[KB], n1[KB];
 #pragma unroll
-        for (int q = 0; q < KP; ++q) {
-            if (pl[par][q] < 0) continue;
-            const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
-            const int i = ti0 + lx, j = tj0 + ly;
-            auto P0 = [&](int di, int dj) { return di < 0 ? w0[q] : di > 0 ? e0[q] : dj < 0 ? s0[q] : n0[q]; };
-            auto P1 = [&](int di, int dj) { return di < 0 ? w1[q] : di > 0 ? e1[q] : dj < 0 ? s1[q] : n1[q]; };
-            u0[q] = gs_point<CC>(L, i, j, P0, r0[par][q], ci[par][q], facx, facy);
-            u1[q] = gs_point<CC>(L, i, j, P1, r1[par][q], ci[par][q], facx, facy);
-        }
+            for (int b = 0; b < KB; ++b) {
+                const int q = q0 + b;
+                if (q >= KP || pl[par][q] < 0) continue;
+                const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
+                w0[b] = s[0][ly][lx - 1]; e0[b] = s[0][ly][lx + 1]; s0[b] = s[0][ly - 1][lx]; n0[b] = s[0][ly + 1][lx];
+                w1[b] = s[1][ly][lx - 1]; e1[b] = s[1][ly][lx + 1]; s1[b] = s[1][ly - 1][lx]; n1[b] = s[1][ly + 1][lx];
+            }
+            double u0[KB], u1[KB];
 #pragma unroll
-        for (int q = 0; q < KP; ++q) {
-            if (pl[par][q] < 0) continue;
-            const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
-            s[0][ly][lx] = u0[q];
-            s[1][ly][lx] = u1[q];
+            for (int b = 0; b < KB; ++b) {
+                const int q = q0 + b;
+                if (q >= KP || pl[par][q] < 0) continue;
+                const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
+                const int i = ti0 + lx, j = tj0 + ly;
+                auto P0 = [&](int di, int dj) { return di < 0 ? w0[b] : di > 0 ? e0[b] : dj < 0 ? s0[b] : n0[b]; };
+                auto P1 = [&](int di, int dj) { return di < 0 ? w1[b] : di > 0 ? e1[b] : dj < 0 ? s1[b] : n1[b]; };
+                u0[b] = gs_point<CC>(L, i, j, P0, r0[par][q], ci[par][q], facx, facy);
+                u1[b] = gs_point<CC>(L, i, j, P1, r1[par][q], ci[par][q], facx, facy);
+            }
+#pragma unroll
+            for (int b = 0; b < KB; ++b) {
+                const int q = q0 + b;
+                if (q >= KP || pl[par][q] < 0) continue;
+                const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
+                s[0][ly][lx] = u0[b];
+                s[1][ly][lx] = u1[b];
+            }
         }
         __syncthreads();
     };
@@ -383,13 +393,15 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                 const bool ring = lx >= HALO - 1 && lx <= HALO + TX && ly >= HALO - 1 && ly <= HALO + TY;
                 if (ring) {
                     const int i = ti0 + lx, j = tj0 + ly;
+                    const double a = acf[at(L, i, j)];
                     auto P0 = [&](int di, int dj) { return s[0][ly + dj][lx + di]; };
                     auto P1 = [&](int di, int dj) { return s[1][ly + dj][lx + di]; };
-                    const double g0 = residual1<CC>(L, i, j, P0, r0[par][q], ac[par][q], facx, facy);
-                    const double g1 = residual1<CC>(L, i, j, P1, r1[par][q], ac[par][q], facx, facy);
+                    const double g0 = residual1<CC>(L, i, j, P0, r0[par][q], a, facx, facy);
+                    const double g1 = residual1<CC>(L, i, j, P1, r1[par][q], a, facx, facy);
                     if (out) m = fmax(m, fmax(fabs(g0), fabs(g1)));
                     r[0][ly - HALO + 1][lx - HALO + 1] = g0;
                     r[1][ly - HALO + 1][lx - HALO + 1] = g1;
+                    if (ACF) ra[ly - HALO + 1][lx - HALO + 1] = a;
                 }
             }
             if (out) {
@@ -406,6 +418,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
             if (!in_valid(L, ti0 + HALO - 1 + rx, tj0 + HALO - 1 + ry)) {
                 r[0][ry][rx] = 0.0;
                 r[1][ry][rx] = 0.0;
+                if (ACF) ra[ry][rx] = 0.0;
             }
         }
         __syncthreads();
@@ -421,6 +434,10 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
             auto f1 = [&](int fi, int fj) { return r[1][fj - oy0 + 1][fi - ox0 + 1]; };
             res_c[at(C, ic, jc)] = restrict_from<CC>(f0, ic, jc);
             res_c[at(C, ic, jc) + C.cs] = restrict_from<CC>(f1, ic, jc);
+            if (ACF) {
+                auto fa = [&](int fi, int fj) { return ra[fj - oy0 + 1][fi - ox0 + 1]; };
+                C.acf[at(C, ic, jc)] = restrict_from<CC>(fa, ic, jc);
+            }
         }
     }
     if (NORMS) {
@@ -443,21 +460,6 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
             norms[2 * b] = mm;
             norms[2 * b + 1] = rr;
         }
-    }
-}
-
-// crse (valid points) = R(fine), ncomp components
-template <bool CC>
-__global__ void k_restrict(MGLevel C, MGLevel F, double* __restrict__ crse, const double* __restrict__ fine,
-                           int ncomp, long ccs, long fcs) {
-    const int nvx = C.vihi - C.vilo + 1, nvy = C.vjhi - C.vjlo + 1;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nvx * nvy) return;
-    const int j = C.vjlo + t / nvx, i = C.vilo + t % nvx;
-    for (int n = 0; n < ncomp; ++n) {
-        const double* f = fine + n * fcs;
-        auto fa = [&](int a, int b) { return f[at(F, a, b)]; };
-        crse[n * ccs + at(C, i, j)] = restrict_from<CC>(fa, i, j);
     }
 }
 
@@ -829,28 +831,18 @@ struct Solve {
     MGCtl* other;
 };
 
-template <int NSW, bool ZERO, bool RES, bool INTERP, bool RHSMAX = false, bool NORMS = false>
+template <int NSW, bool ZERO, bool RES, bool INTERP, bool RHSMAX = false, bool NORMS = false, bool ACF = false>
 void smooth(hpq_ctx* c, const MGLevel& L, const MGLevel& C, double* phi_out, const double* rhs, long rcs,
             const double* acf, const double* phi_in, const double* cor_c, double* res_c, const MGCtl* gate,
             double* norms = nullptr) {
     TraceScope ts(c, HPQ_TRACE_MG_SMOOTH);
     const dim3 grid((L.vihi - L.vilo + 1 + TX - 1) / TX, (L.vjhi - L.vjlo + 1 + TY - 1) / TY);
     if (L.cc)
-        hipLaunchKernelGGL((k_smooth<NSW, true, ZERO, RES, INTERP, RHSMAX, NORMS>), grid, dim3(NT), 0, c->stream, L,
+        hipLaunchKernelGGL((k_smooth<NSW, true, ZERO, RES, INTERP, RHSMAX, NORMS, ACF>), grid, dim3(NT), 0, c->stream, L,
                            C, phi_out, rhs, rcs, acf, phi_in, cor_c, res_c, gate, norms);
     else
-        hipLaunchKernelGGL((k_smooth<NSW, false, ZERO, RES, INTERP, RHSMAX, NORMS>), grid, dim3(NT), 0, c->stream,
+        hipLaunchKernelGGL((k_smooth<NSW, false, ZERO, RES, INTERP, RHSMAX, NORMS, ACF>), grid, dim3(NT), 0, c->stream,
                            L, C, phi_out, rhs, rcs, acf, phi_in, cor_c, res_c, gate, norms);
-}
-
-void launch_restrict(hpq_ctx* c, const MGLevel& C, const MGLevel& F, double* crse, const double* fine, int ncomp) {
-    const int n = (C.vihi - C.vilo + 1) * (C.vjhi - C.vjlo + 1);
-    if (C.cc)
-        hipLaunchKernelGGL(k_restrict<true>, grid1d(n, 256), dim3(256), 0, c->stream, C, F, crse, fine, ncomp, C.cs,
-                           F.cs);
-    else
-        hipLaunchKernelGGL(k_restrict<false>, grid1d(n, 256), dim3(256), 0, c->stream, C, F, crse, fine, ncomp, C.cs,
-                           F.cs);
 }
 
 // MultiGrid::vcycle (:1430-1512), V-cycle number m of the solve. Level 0's residual and
@@ -865,7 +857,11 @@ void vcycle(hpq_ctx* c, const Solve& S, int m, int maxiter) {
     for (int l = 1; l < cb; ++l) {
         MGLevel& L = M.lev[l];
         MGLevel& C = M.lev[l + 1];
-        smooth<4, true, true, false>(c, L, C, L.cor, L.res, L.cs, L.acf, nullptr, nullptr, C.res, ctl);
+        if (m == 0)
+            smooth<4, true, true, false, false, false, true>(c, L, C, L.cor, L.res, L.cs, L.acf, nullptr, nullptr,
+                                                             C.res, ctl);
+        else
+            smooth<4, true, true, false>(c, L, C, L.cor, L.res, L.cs, L.acf, nullptr, nullptr, C.res, ctl);
     }
     {
         TraceScope ts(c, HPQ_TRACE_MG_COARSE);
@@ -967,14 +963,12 @@ int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, lon
         S.buf[1] = M.msol;
     }
     M.ctl_dirty = true;   // until the solve ends normally
-    // average_down_acoef (:1641-1661): the tiled levels here, the coarse ones inside k_coarse
-    for (int l = 1; l <= M.coarse_begin; ++l)
-        launch_restrict(c, M.lev[l], l == 1 ? S.L0 : M.lev[l - 1], M.lev[l].acf, l == 1 ? S.L0.acf : M.lev[l - 1].acf,
-                        1);
+    // average_down_acoef (:1641-1661): level 1 in the initial smoothing, levels 2..cb in the
+    // first V-cycle's down-sweep, the coarse ones inside k_coarse
     // initial smoothing from the guess (buf[1]) into buf[0], the iterate after zero cycles;
     // its last workgroup sets the convergence target
-    smooth<4, false, true, false, true, true>(c, S.L0, L1, S.buf[0], S.rhs0, S.rcs, S.L0.acf, S.buf[1], nullptr, L1.res,
-                                              nullptr, M.d_norms);
+    smooth<4, false, true, false, true, true, true>(c, S.L0, L1, S.buf[0], S.rhs0, S.rcs, S.L0.acf, S.buf[1], nullptr,
+                                                    L1.res, nullptr, M.d_norms);
     const Check chk0{S.ctl, S.other, M.d_mirror, M.d_norms, level0_blocks(S.L0), 0, maxiter, tol_rel, tol_abs};
     hipLaunchKernelGGL(k_target, dim3(1), dim3(NT), 0, c->stream, chk0);
     HPQ_HIP(hipGetLastError());

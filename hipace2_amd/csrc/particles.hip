@@ -3,7 +3,9 @@
 //
 // One thread per particle over coalesced SoA arrays; stencils unrolled at compile time
 // (shape order is a template parameter). Every expression keeps the reference's operation
-// order, so FP64 results equal the reference CPU build's bit for bit.
+// order, so FP64 results equal the reference CPU build's bit for bit. The arithmetic type R
+// of the gather and of the momentum sub-steps is double, or float on the mixed-precision
+// path, where the FP64 particle state accumulates the FP32 increments.
 #include <cmath>
 
 #include "internal.hpp"
@@ -18,21 +20,26 @@ __device__ __forceinline__ bool in_store(const Geom& g, int i, int j) {
 }
 
 // Forward-mode dual number for the second-order Taylor step (src/utils/DualNumbers.H:13-43)
+template <class R>
 struct Dual {
-    double v, e;
-    __device__ Dual(double a) : v(a), e(0) {}
-    __device__ Dual(double a, double b) : v(a), e(b) {}
+    R v, e;
+    __device__ Dual(R a) : v(a), e(0) {}
+    __device__ Dual(R a, R b) : v(a), e(b) {}
 };
-__device__ __forceinline__ Dual operator+(const Dual& a, const Dual& b) { return {a.v + b.v, a.e + b.e}; }
-__device__ __forceinline__ Dual operator-(const Dual& a, const Dual& b) { return {a.v - b.v, a.e - b.e}; }
-__device__ __forceinline__ Dual operator*(const Dual& a, const Dual& b) { return {a.v * b.v, a.e * b.v + a.v * b.e}; }
+template <class R>
+__device__ __forceinline__ Dual<R> operator+(const Dual<R>& a, const Dual<R>& b) { return {a.v + b.v, a.e + b.e}; }
+template <class R>
+__device__ __forceinline__ Dual<R> operator-(const Dual<R>& a, const Dual<R>& b) { return {a.v - b.v, a.e - b.e}; }
+template <class R>
+__device__ __forceinline__ Dual<R> operator*(const Dual<R>& a, const Dual<R>& b) {
+    return {a.v * b.v, a.e * b.v + a.v * b.e};
+}
 
 // PlasmaMomentumPush, src/particles/pusher/PushPlasmaParticles.H:37-75 (no laser terms)
-template <class T>
-__device__ __forceinline__ void momentum_rhs(const T& ux, const T& uy, const T& psi_inv, double ExmBy, double EypBx,
-                                             double Ez, double Bx_c, double By_c, double Bz, double ci, double qmc,
-                                             T& dux, T& duy, T& dpsi) {
-    const double A = 0.0, ADx = 0.0, ADy = 0.0;
+template <class T, class R>
+__device__ __forceinline__ void momentum_rhs(const T& ux, const T& uy, const T& psi_inv, R ExmBy, R EypBx, R Ez,
+                                             R Bx_c, R By_c, R Bz, R ci, R qmc, T& dux, T& duy, T& dpsi) {
+    const R A = 0.0, ADx = 0.0, ADy = 0.0;
     const T gamma_psi =
         T(0.5) * psi_inv * psi_inv * (T(1.0) + T(A) + ux * ux * T(ci * ci) + uy * uy * T(ci * ci)) + T(0.5);
     dux = (T(qmc) * (gamma_psi * T(ExmBy) + T(By_c) + (uy * T(Bz)) * psi_inv) - T(ADx) * psi_inv);
@@ -66,7 +73,7 @@ __device__ __forceinline__ bool enforce_bc(const BC& bc, double& x, double& y, d
 }
 
 // AdvancePlasmaParticles, src/particles/pusher/PlasmaParticleAdvance.cpp:84-240 (leapfrog)
-template <int O>
+template <int O, class R>
 __global__ __launch_bounds__(256) void k_push_plasma(Geom g, hpq_plasma p, const double* __restrict__ store, int cPsi,
                                                      int cEz, int cBx, int cBy, int cBz, double qmc, BC bc,
                                                      int temp_slice) {
@@ -75,7 +82,7 @@ __global__ __launch_bounds__(256) void k_push_plasma(Geom g, hpq_plasma p, const
     const double clight = 1.0, clight_inv = 1.0 / 1.0;
     const double dz = g.dz / 1;
     double xp = p.x_prev[ip], yp = p.y_prev[ip];
-    double ExmByp = 0, EypBxp = 0, Ezp = 0, Bxp = 0, Byp = 0, Bzp = 0;
+    R ExmByp = 0, EypBxp = 0, Ezp = 0, Bxp = 0, Byp = 0, Bzp = 0;
     {
         const double xm = (xp - g.xoff) * g.dxi, ym = (yp - g.yoff) * g.dyi;
         const double* Psi = store + long(cPsi) * g.plane;
@@ -84,75 +91,87 @@ __global__ __launch_bounds__(256) void k_push_plasma(Geom g, hpq_plasma p, const
         const double* By = store + long(cBy) * g.plane;
         const double* Bz = store + long(cBz) * g.plane;
         constexpr int W = O + 2;
-        Dsf sx[W], sy[W];
+        const R dxi = R(g.dxi), dyi = R(g.dyi);
+        DsfR<R> sx[W], sy[W];
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            sx[k] = dshape<1, O>(xm, k);
-            sy[k] = dshape<1, O>(ym, k);
+            sx[k] = dshape<1, O, R>(R(xm), k);
+            sy[k] = dshape<1, O, R>(R(ym), k);
         }
-        auto accumulate = [&](const Dsf& X, const Dsf& Y, double ps, double ez, double bx, double by, double bz) {
-            ExmByp += (X.ds * Y.s) * ps * g.dxi;
-            EypBxp += (X.s * Y.ds) * ps * g.dyi;
+        auto accumulate = [&](const DsfR<R>& X, const DsfR<R>& Y, R ps, R ez, R bx, R by, R bz) {
+            ExmByp += (X.ds * Y.s) * ps * dxi;
+            EypBxp += (X.s * Y.ds) * ps * dyi;
             Ezp += (X.s * Y.s) * ez;
             Bxp += (X.s * Y.s) * bx;
             Byp += (X.s * Y.s) * by;
             Bzp += (X.s * Y.s) * bz;
         };
         if (in_store(g, sx[0].cell, sy[0].cell) && in_store(g, sx[W - 1].cell, sy[W - 1].cell)) {
-            // the whole stencil lies in the guarded plane: every load issued up front
+            // The whole stencil lies in the guarded plane: one field at a time, its W x W
+            // loads issued together. Every accumulator still sums its cells in (iy, ix)
+            // order, as the reference's cell loop does.
             const long k0 = g.idx(sx[0].cell, sy[0].cell);
-            double ps[W][W], ez[W][W], bx[W][W], by[W][W], bz[W][W];
+            auto tile = [&](const double* F, R (&v)[W][W]) {
 #pragma unroll
-            for (int iy = 0; iy < W; ++iy) {
+                for (int iy = 0; iy < W; ++iy)
+#pragma unroll
+                    for (int ix = 0; ix < W; ++ix) v[iy][ix] = R(F[k0 + long(iy) * g.ldx + ix]);
+            };
+            R v[W][W];
+            tile(Psi, v);
+#pragma unroll
+            for (int iy = 0; iy < W; ++iy)
 #pragma unroll
                 for (int ix = 0; ix < W; ++ix) {
-                    const long k = k0 + long(iy) * g.ldx + ix;
-                    ps[iy][ix] = Psi[k];
-                    ez[iy][ix] = Ez[k];
-                    bx[iy][ix] = Bx[k];
-                    by[iy][ix] = By[k];
-                    bz[iy][ix] = Bz[k];
+                    ExmByp += (sx[ix].ds * sy[iy].s) * v[iy][ix] * dxi;
+                    EypBxp += (sx[ix].s * sy[iy].ds) * v[iy][ix] * dyi;
                 }
-            }
+            auto plain = [&](const double* F, R& acc) {
+                tile(F, v);
 #pragma unroll
-            for (int iy = 0; iy < W; ++iy) {
+                for (int iy = 0; iy < W; ++iy)
 #pragma unroll
-                for (int ix = 0; ix < W; ++ix)
-                    accumulate(sx[ix], sy[iy], ps[iy][ix], ez[iy][ix], bx[iy][ix], by[iy][ix], bz[iy][ix]);
-            }
+                    for (int ix = 0; ix < W; ++ix) acc += (sx[ix].s * sy[iy].s) * v[iy][ix];
+            };
+            plain(Ez, Ezp);
+            plain(Bx, Bxp);
+            plain(By, Byp);
+            plain(Bz, Bzp);
         } else {
 #pragma unroll
             for (int iy = 0; iy < W; ++iy) {
 #pragma unroll
                 for (int ix = 0; ix < W; ++ix) {
-                    const Dsf X = sx[ix], Y = sy[iy];
+                    const DsfR<R> X = sx[ix], Y = sy[iy];
                     if (!in_store(g, X.cell, Y.cell)) continue;
                     const long k = g.idx(X.cell, Y.cell);
-                    accumulate(X, Y, Psi[k], Ez[k], Bx[k], By[k], Bz[k]);
+                    accumulate(X, Y, R(Psi[k]), R(Ez[k]), R(Bx[k]), R(By[k]), R(Bz[k]));
                 }
             }
         }
     }
-    Bxp *= clight;
-    Byp *= clight;
+    Bxp *= R(clight);
+    Byp *= R(clight);
     constexpr int nsub = 4;
     const double sdz = dz / nsub;
+    // the (FP64) state; the increments are computed in R
     double ux = p.ux_half_step[ip], uy = p.uy_half_step[ip], psi = p.psi_half_step[ip];
+    const R sdr = R(sdz), cir = R(clight_inv), qmr = R(qmc);
     auto substep = [&]() {
-        const double psi_inv = 1.0 / psi;
-        double dux, duy, dpsi;
-        momentum_rhs<double>(ux, uy, psi_inv, ExmByp, EypBxp, Ezp, Bxp, Byp, Bzp, clight_inv, qmc, dux, duy, dpsi);
-        const Dual uxd{ux, dux}, uyd{uy, duy}, pid{psi_inv, -psi_inv * psi_inv * dpsi};
-        Dual ddux(0.0), dduy(0.0), ddpsi(0.0);
-        momentum_rhs<Dual>(uxd, uyd, pid, ExmByp, EypBxp, Ezp, Bxp, Byp, Bzp, clight_inv, qmc, ddux, dduy, ddpsi);
-        ux += sdz * dux + 0.5 * sdz * sdz * ddux.e;
-        uy += sdz * duy + 0.5 * sdz * sdz * dduy.e;
-        psi += sdz * dpsi + 0.5 * sdz * sdz * ddpsi.e;
+        const R psi_inv = R(1.0) / R(psi);
+        R dux, duy, dpsi;
+        momentum_rhs<R, R>(R(ux), R(uy), psi_inv, ExmByp, EypBxp, Ezp, Bxp, Byp, Bzp, cir, qmr, dux, duy, dpsi);
+        const Dual<R> uxd{R(ux), dux}, uyd{R(uy), duy}, pid{psi_inv, -psi_inv * psi_inv * dpsi};
+        Dual<R> ddux(R(0.0)), dduy(R(0.0)), ddpsi(R(0.0));
+        momentum_rhs<Dual<R>, R>(uxd, uyd, pid, ExmByp, EypBxp, Ezp, Bxp, Byp, Bzp, cir, qmr, ddux, dduy, ddpsi);
+        ux += double(sdr * dux + R(0.5) * sdr * sdr * ddux.e);
+        uy += double(sdr * duy + R(0.5) * sdr * sdr * dduy.e);
+        psi += double(sdr * dpsi + R(0.5) * sdr * sdr * ddpsi.e);
     };
 #pragma unroll 1
     for (int s = 0; s < nsub; ++s) substep();
-    xp += dz * clight_inv * (ux * (1.0 / psi));
-    yp += dz * clight_inv * (uy * (1.0 / psi));
+    xp += double(R(dz) * cir * (R(ux) * (R(1.0) / R(psi))));
+    yp += double(R(dz) * cir * (R(uy) * (R(1.0) / R(psi))));
     if (enforce_bc(bc, xp, yp, ux, uy)) {
         p.w[ip] = 0.0;
         p.valid[ip] = 0;
@@ -183,7 +202,9 @@ void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool t
     BC bc{c->p.particle_bc, c->p.plo[0], c->p.plo[1], c->p.phi[0], c->p.phi[1]};
     TraceScope ts(c, HPQ_TRACE_PUSH);
     dispatch_order(c->p.depos_order, [&](auto O) {
-        hipLaunchKernelGGL((k_push_plasma<decltype(O)::value>), grid1d(p->n, 256), dim3(256), 0, c->stream, c->g, *p,
+        auto kern = c->p.precision == HPQ_PREC_MIXED ? k_push_plasma<decltype(O)::value, float>
+                                                     : k_push_plasma<decltype(O)::value, double>;
+        hipLaunchKernelGGL(kern, grid1d(p->n, 256), dim3(256), 0, c->stream, c->g, *p,
                            store, L.Psi, L.Ez, L.Bx, L.By, L.Bz, qmc, bc, int(temp_slice));
     });
     HPQ_HIP(hipGetLastError());

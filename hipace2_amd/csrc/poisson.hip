@@ -30,49 +30,124 @@ __device__ __forceinline__ double* row_ptr(const Rows& R, int r) {
     return R.base[f] + long(j) * R.stride;
 }
 
-__global__ __launch_bounds__(256) void k_dst_rows_fft(Rows in, Rows out, int nrows, int n, int M, int logM,
-                                                      const double2* __restrict__ tw,
-                                                      const double* __restrict__ scale) {
-    extern __shared__ double2 a[];
-    const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
-    const bool has1 = r1 < nrows;
-    const double* p0 = row_ptr(in, r0);
-    const double* p1 = has1 ? row_ptr(in, r1) : p0;
-    const int N = M >> 1;
-    for (int m = threadIdx.x; m < M; m += blockDim.x) {
-        double va = 0.0, vb = 0.0;
-        if (m >= 1 && m <= n) {
-            va = p0[m - 1];
-            vb = has1 ? p1[m - 1] : 0.0;
-        } else if (m > N) {
-            const int q = M - m;
-            va = -p0[q - 1];
-            vb = has1 ? -p1[q - 1] : 0.0;
+constexpr int DST_T = 256;
+
+// complex a * w
+__device__ __forceinline__ double2 cmul(double2 x, double2 w) {
+    return make_double2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+}
+
+// In-place radix-2 DIT FFT of the bit-reversed sequence a[0..M): the stages are taken in
+// pairs (a 4-point butterfly per thread and one barrier per pair); the arithmetic is that
+// of the radix-2 stages. tw[k] = exp(-2 pi i k / M).
+__device__ void fft_lds(double2* a, int M, int logM, const double2* __restrict__ tw) {
+    int s = 1;
+    for (; s + 1 <= logM; s += 2) {
+        const int h = 1 << (s - 1);
+        const int st1 = M >> s, st2 = M >> (s + 1);
+        for (int b = threadIdx.x; b < (M >> 2); b += DST_T) {
+            const int k = b & (h - 1);
+            const int i0 = ((b >> (s - 1)) << (s + 1)) + k;
+            const int i1 = i0 + h, i2 = i0 + 2 * h, i3 = i0 + 3 * h;
+            const double2 w1 = tw[k * st1], w2 = tw[k * st2], w3 = tw[(k + h) * st2];
+            const double2 a0 = a[i0], a1 = a[i1], a2 = a[i2], a3 = a[i3];
+            const double2 v1 = cmul(a1, w1), v3 = cmul(a3, w1);
+            const double2 x0 = make_double2(a0.x + v1.x, a0.y + v1.y), x1 = make_double2(a0.x - v1.x, a0.y - v1.y);
+            const double2 x2 = make_double2(a2.x + v3.x, a2.y + v3.y), x3 = make_double2(a2.x - v3.x, a2.y - v3.y);
+            const double2 u2 = cmul(x2, w2), u3 = cmul(x3, w3);
+            a[i0] = make_double2(x0.x + u2.x, x0.y + u2.y);
+            a[i2] = make_double2(x0.x - u2.x, x0.y - u2.y);
+            a[i1] = make_double2(x1.x + u3.x, x1.y + u3.y);
+            a[i3] = make_double2(x1.x - u3.x, x1.y - u3.y);
         }
-        a[__brev(unsigned(m)) >> (32 - logM)] = make_double2(va, vb);
+        __syncthreads();
     }
-    __syncthreads();
-    for (int s = 1; s <= logM; ++s) {
+    if (s == logM) {   // odd number of stages: the last one alone
         const int half = 1 << (s - 1);
         const int step = M >> s;
-        for (int b = threadIdx.x; b < (M >> 1); b += blockDim.x) {
+        for (int b = threadIdx.x; b < (M >> 1); b += DST_T) {
             const int k = b & (half - 1);
             const int i0 = ((b >> (s - 1)) << s) + k;
             const int i1 = i0 + half;
-            const double2 w = tw[k * step];
             const double2 u = a[i0];
-            const double2 x = a[i1];
-            const double2 v = make_double2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+            const double2 v = cmul(a[i1], tw[k * step]);
             a[i0] = make_double2(u.x + v.x, u.y + v.y);
             a[i1] = make_double2(u.x - v.x, u.y - v.y);
         }
         __syncthreads();
     }
-    double* q0 = row_ptr(out, r0);
-    double* q1 = has1 ? row_ptr(out, r1) : nullptr;
+}
+
+// Two rows per workgroup: a + i b odd-extended to length M, bit-reversed into LDS.
+// v(m) gives the pair of values at m in [1, n].
+template <class V>
+__device__ __forceinline__ void load_odd_extension(double2* a, int n, int M, int logM, V v) {
+    const int N = M >> 1;
+    for (int m = threadIdx.x; m < M; m += DST_T) {
+        double2 x = make_double2(0.0, 0.0);
+        if (m >= 1 && m <= n) {
+            x = v(m);
+        } else if (m > N) {
+            const double2 y = v(M - m);
+            x = make_double2(-y.x, -y.y);
+        }
+        a[__brev(unsigned(m)) >> (32 - logM)] = x;
+    }
+    __syncthreads();
+}
+
+// Rows r0, r0 + 1: RODFT00 of each (with F = FFT of the packed odd extension,
+// RODFT00(a)_{k-1} = -Im F_k and RODFT00(b)_{k-1} = Re F_k), optionally times the
+// eigenvalue rows (scale != null). SYM: the result is transformed once more in LDS (the
+// inverse pass of the same direction), so the middle of the 2-D solve is one launch.
+template <bool SYM>
+__global__ __launch_bounds__(DST_T) void k_dst_rows_fft(Rows in, Rows out, int nrows, int n, int M, int logM,
+                                                        const double2* __restrict__ tw,
+                                                        const double* __restrict__ scale) {
+    extern __shared__ double2 a[];
+    const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
+    const bool has1 = r1 < nrows;
+    const double* p0 = row_ptr(in, r0);
+    const double* p1 = has1 ? row_ptr(in, r1) : p0;
+    load_odd_extension(a, n, M, logM, [&](int m) { return make_double2(p0[m - 1], has1 ? p1[m - 1] : 0.0); });
+    fft_lds(a, M, logM, tw);
     const double* s0 = scale ? scale + long(r0 % in.rows_per_field) * n : nullptr;
     const double* s1 = (scale && has1) ? scale + long(r1 % in.rows_per_field) * n : nullptr;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    if (SYM) {
+        // RODFT00 of the scaled rows: pack them again; F_m is read for m in [1, n] only,
+        // so the new sequence goes through registers (each thread M / DST_T entries)
+        constexpr int MAXQ = 4096 / DST_T;
+        double2 nv[MAXQ];
+        const int N = M >> 1;
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) {
+            const int m = threadIdx.x + q * DST_T;
+            nv[q] = make_double2(0.0, 0.0);
+            if (m < M) {
+                const int src = (m >= 1 && m <= n) ? m : (m > N ? M - m : 0);
+                if (src) {
+                    const double2 F = a[src];
+                    double o0 = -F.y, o1 = F.x;
+                    if (s0) o0 *= s0[src - 1];
+                    if (s1) o1 *= s1[src - 1];
+                    if (!has1) o1 = 0.0;
+                    nv[q] = m <= n ? make_double2(o0, o1) : make_double2(-o0, -o1);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) {
+            const int m = threadIdx.x + q * DST_T;
+            if (m < M) a[__brev(unsigned(m)) >> (32 - logM)] = nv[q];
+        }
+        __syncthreads();
+        fft_lds(a, M, logM, tw);
+        s0 = s1 = nullptr;
+    }
+    double* q0 = row_ptr(out, r0);
+    double* q1 = has1 ? row_ptr(out, r1) : nullptr;
+    for (int k = threadIdx.x; k < n; k += DST_T) {
         const double2 F = a[k + 1];
         double o0 = -F.y, o1 = F.x;
         if (s0) o0 *= s0[k];
@@ -123,15 +198,28 @@ __global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ in
     }
 }
 
-static void dst_rows(hpq_ctx* c, const DstPlan& P, const Rows& in, const Rows& out, int nrows, const double* scale) {
+// RODFT00 of nrows rows (times the eigenvalue rows when scale != null); sym: followed by
+// the inverse transform of the same rows (fused in LDS on the FFT path)
+static void dst_rows(hpq_ctx* c, const DstPlan& P, const Rows& in, const Rows& out, int nrows, const double* scale,
+                     bool sym = false) {
     TraceScope ts(c, HPQ_TRACE_DST);
     if (P.fft) {
         const size_t lds = size_t(P.M) * sizeof(double2);
-        hipLaunchKernelGGL(k_dst_rows_fft, dim3((nrows + 1) / 2), dim3(256), lds, c->stream, in, out, nrows, P.n, P.M,
-                           P.logM, P.tw, scale);
+        if (sym)
+            hipLaunchKernelGGL(k_dst_rows_fft<true>, dim3((nrows + 1) / 2), dim3(DST_T), lds, c->stream, in, out,
+                               nrows, P.n, P.M, P.logM, P.tw, scale);
+        else
+            hipLaunchKernelGGL(k_dst_rows_fft<false>, dim3((nrows + 1) / 2), dim3(DST_T), lds, c->stream, in, out,
+                               nrows, P.n, P.M, P.logM, P.tw, scale);
     } else {
         const size_t lds = size_t(P.n) * sizeof(double);
         hipLaunchKernelGGL(k_dst_rows_direct, dim3(nrows), dim3(256), lds, c->stream, in, out, P.n, P.sintab, scale);
+        if (sym) {
+            HPQ_HIP(hipGetLastError());
+            // the direct path writes in place row by row: transform out again into in
+            hipLaunchKernelGGL(k_dst_rows_direct, dim3(nrows), dim3(256), lds, c->stream, out, in, P.n, P.sintab,
+                               nullptr);
+        }
     }
     HPQ_HIP(hipGetLastError());
 }
@@ -217,9 +305,12 @@ void poisson_solve_staged(hpq_ctx* c, int nf, double* const* lhs_planes) {
     dst_rows(c, S.px, rows(S.bufA, nx, ny), rows(S.bufB, nx, ny), nf * ny, nullptr);
     hipLaunchKernelGGL(k_transpose, dim3((nx + 31) / 32, (ny + 31) / 32, nf), dim3(32, 8), 0, c->stream, S.bufB,
                        S.bufA, ny, nx);
-    // y pass, eigenvalue scaling, inverse y pass on rows i
-    dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufB, ny, nx), nf * nx, S.eigT);
-    dst_rows(c, S.py, rows(S.bufB, ny, nx), rows(S.bufA, ny, nx), nf * nx, nullptr);
+    // y pass, eigenvalue scaling, inverse y pass on rows i (one launch on the FFT path;
+    // the direct path leaves its result in bufA as well)
+    if (S.py.fft)
+        dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufA, ny, nx), nf * nx, S.eigT, true);
+    else
+        dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufB, ny, nx), nf * nx, S.eigT, true);
     hipLaunchKernelGGL(k_transpose, dim3((ny + 31) / 32, (nx + 31) / 32, nf), dim3(32, 8), 0, c->stream, S.bufA,
                        S.bufB, nx, ny);
     // inverse x pass straight into the destination planes

@@ -6,165 +6,171 @@
 // derivative, type 1 (nodal) the derivative of the order-(O+1) spline on a one cell
 // wider window, type 2 (centred) half the difference of the two neighbouring values.
 // `ds` is returned negated, as the reference returns {s, -sdx, cell}.
-// Order and type are template parameters so every stencil loop unrolls.
+// Order and type are template parameters so every stencil loop unrolls; the arithmetic
+// type R is double for the FP64 path and float for the mixed-precision path.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace hpq {
 
-struct Sf { double s; int cell; };
-struct Dsf { double s; double ds; int cell; };
+template <class R>
+struct SfR { R s; int cell; };
+template <class R>
+struct DsfR { R s; R ds; int cell; };
+using Sf = SfR<double>;
+using Dsf = DsfR<double>;
 
-template <int O>
-__host__ __device__ __forceinline__ Sf shape(double xmid, int ix) {
+template <int O, class R = double>
+__host__ __device__ __forceinline__ SfR<R> shape(R xmid, int ix) {
     if constexpr (O == 0) {
-        return {1.0, int(floor(xmid + 0.5))};
+        return {R(1.0), int(floor(xmid + R(0.5)))};
     } else if constexpr (O == 1) {
-        const double xf = floor(xmid);
-        const double t = xmid - xf;
-        return ix == 0 ? Sf{1.0 - t, int(xf)} : Sf{t, int(xf) + 1};
+        const R xf = floor(xmid);
+        const R t = xmid - xf;
+        return ix == 0 ? SfR<R>{R(1.0) - t, int(xf)} : SfR<R>{t, int(xf) + 1};
     } else if constexpr (O == 2) {
-        const double xf = floor(xmid + 0.5);
-        const double t = xmid - xf;
-        if (ix == 0) return {0.5 * (0.5 - t) * (0.5 - t), int(xf) - 1};
-        if (ix == 1) return {0.75 - t * t, int(xf)};
-        return {0.5 * (0.5 + t) * (0.5 + t), int(xf) + 1};
+        const R xf = floor(xmid + R(0.5));
+        const R t = xmid - xf;
+        if (ix == 0) return {R(0.5) * (R(0.5) - t) * (R(0.5) - t), int(xf) - 1};
+        if (ix == 1) return {R(0.75) - t * t, int(xf)};
+        return {R(0.5) * (R(0.5) + t) * (R(0.5) + t), int(xf) + 1};
     } else {
-        const double xf = floor(xmid);
-        const double t = xmid - xf;
-        if (ix == 0) return {1.0 / 6.0 * (1.0 - t) * (1.0 - t) * (1.0 - t), int(xf) - 1};
-        if (ix == 1) return {2.0 / 3.0 - t * t * (1.0 - t / 2.0), int(xf)};
-        if (ix == 2) return {2.0 / 3.0 - (1.0 - t) * (1.0 - t) * (1.0 - 0.5 * (1.0 - t)), int(xf) + 1};
-        return {1.0 / 6.0 * t * t * t, int(xf) + 2};
+        const R xf = floor(xmid);
+        const R t = xmid - xf;
+        if (ix == 0) return {R(1.0) / R(6.0) * (R(1.0) - t) * (R(1.0) - t) * (R(1.0) - t), int(xf) - 1};
+        if (ix == 1) return {R(2.0) / R(3.0) - t * t * (R(1.0) - t / R(2.0)), int(xf)};
+        if (ix == 2) return {R(2.0) / R(3.0) - (R(1.0) - t) * (R(1.0) - t) * (R(1.0) - R(0.5) * (R(1.0) - t)), int(xf) + 1};
+        return {R(1.0) / R(6.0) * t * t * t, int(xf) + 2};
     }
 }
 
-__host__ __device__ __forceinline__ Dsf dmk(double s, double sdx, int cell) { return {s, -sdx, cell}; }
+template <class R>
+__host__ __device__ __forceinline__ DsfR<R> dmk(R s, R sdx, int cell) { return {s, -sdx, cell}; }
 
 // type 0: analytic derivative of the order-O spline, O+1 cells
-template <int O>
-__host__ __device__ __forceinline__ Dsf dshape_analytic(double xmid, int ix) {
+template <int O, class R = double>
+__host__ __device__ __forceinline__ DsfR<R> dshape_analytic(R xmid, int ix) {
     if constexpr (O == 0) {
-        return dmk(1.0, 0.0, int(floor(xmid + 0.5)));
+        return dmk<R>(R(1.0), R(0.0), int(floor(xmid + R(0.5))));
     } else if constexpr (O == 1) {
-        const double xf = floor(xmid);
-        const double t = xmid - xf;
-        return ix == 0 ? dmk(1 - t, -1.0, int(xf)) : dmk(t, 1.0, int(xf) + 1);
+        const R xf = floor(xmid);
+        const R t = xmid - xf;
+        return ix == 0 ? dmk<R>(1 - t, -R(1.0), int(xf)) : dmk<R>(t, R(1.0), int(xf) + 1);
     } else if constexpr (O == 2) {
-        const double xf = floor(xmid + 0.5);
-        const double t = xmid + 0.5 - xf;
-        const double t2 = t * t;
+        const R xf = floor(xmid + R(0.5));
+        const R t = xmid + R(0.5) - xf;
+        const R t2 = t * t;
         const int c = int(xf) - 1 + ix;
-        if (ix == 0) return dmk(0.5 * t2 - 1.0 * t + 0.5, t - 1.0, c);
-        if (ix == 1) return dmk(-t2 + 1.0 * t + 0.5, 1.0 - 2 * t, c);
-        return dmk(0.5 * t2, t, c);
+        if (ix == 0) return dmk<R>(R(0.5) * t2 - R(1.0) * t + R(0.5), t - R(1.0), c);
+        if (ix == 1) return dmk<R>(-t2 + R(1.0) * t + R(0.5), R(1.0) - 2 * t, c);
+        return dmk<R>(R(0.5) * t2, t, c);
     } else {
-        const double xf = floor(xmid);
-        const double t = xmid - xf;
-        const double t2 = t * t, t3 = t2 * t;
+        const R xf = floor(xmid);
+        const R t = xmid - xf;
+        const R t2 = t * t, t3 = t2 * t;
         const int c = int(xf) - 1 + ix;
-        if (ix == 0) return dmk(-1.0 / 6.0 * t3 + 0.5 * t2 - 0.5 * t + 1.0 / 6.0, -0.5 * t2 + t - 0.5, c);
-        if (ix == 1) return dmk(0.5 * t3 - t2 + 2.0 / 3.0, 1.5 * t2 - 2 * t, c);
-        if (ix == 2) return dmk(-0.5 * t3 + 0.5 * t2 + 0.5 * t + 1.0 / 6.0, -1.5 * t2 + t + 0.5, c);
-        return dmk(1.0 / 6.0 * t3, 0.5 * t2, c);
+        if (ix == 0) return dmk<R>(-R(1.0) / R(6.0) * t3 + R(0.5) * t2 - R(0.5) * t + R(1.0) / R(6.0), -R(0.5) * t2 + t - R(0.5), c);
+        if (ix == 1) return dmk<R>(R(0.5) * t3 - t2 + R(2.0) / R(3.0), R(1.5) * t2 - 2 * t, c);
+        if (ix == 2) return dmk<R>(-R(0.5) * t3 + R(0.5) * t2 + R(0.5) * t + R(1.0) / R(6.0), -R(1.5) * t2 + t + R(0.5), c);
+        return dmk<R>(R(1.0) / R(6.0) * t3, R(0.5) * t2, c);
     }
 }
 
 // type 1: nodal (value of B_O, derivative of B_{O+1}), O+2 cells
-template <int O>
-__host__ __device__ __forceinline__ Dsf dshape_nodal(double xmid, int ix) {
+template <int O, class R = double>
+__host__ __device__ __forceinline__ DsfR<R> dshape_nodal(R xmid, int ix) {
     if constexpr (O == 0) {
-        const double xf = floor(xmid);
-        const double t = xmid - xf;
-        return ix == 0 ? dmk((t < 0.5) ? 1 : 0, -1, int(xf)) : dmk((t < 0.5) ? 0 : 1, 1, int(xf) + 1);
+        const R xf = floor(xmid);
+        const R t = xmid - xf;
+        return ix == 0 ? dmk<R>((t < R(0.5)) ? 1 : 0, -1, int(xf)) : dmk<R>((t < R(0.5)) ? 0 : 1, 1, int(xf) + 1);
     } else if constexpr (O == 1) {
-        const double xf = floor(xmid + 0.5);
-        const double t = xmid + 0.5 - xf;
+        const R xf = floor(xmid + R(0.5));
+        const R t = xmid + R(0.5) - xf;
         const int c = int(xf) - 1 + ix;
-        if (ix == 0) return dmk((t < 0.5) ? 0.5 - t : 0, t - 1, c);
-        if (ix == 1) return dmk((t < 0.5) ? t + 0.5 : 1.5 - t, 1 - 2 * t, c);
-        return dmk((t < 0.5) ? 0 : t - 0.5, t, c);
+        if (ix == 0) return dmk<R>((t < R(0.5)) ? R(0.5) - t : 0, t - 1, c);
+        if (ix == 1) return dmk<R>((t < R(0.5)) ? t + R(0.5) : R(1.5) - t, 1 - 2 * t, c);
+        return dmk<R>((t < R(0.5)) ? 0 : t - R(0.5), t, c);
     } else if constexpr (O == 2) {
-        const double xf = floor(xmid);
-        const double t = xmid - xf;
-        const double t2 = t * t;
+        const R xf = floor(xmid);
+        const R t = xmid - xf;
+        const R t2 = t * t;
         const int c = int(xf) - 1 + ix;
-        if (ix == 0) return dmk((t < 0.5) ? 0.5 * t2 - 0.5 * t + 1.0 / 8.0 : 0, -0.5 * t2 + t - 0.5, c);
-        if (ix == 1) return dmk((t < 0.5) ? 3.0 / 4.0 - t2 : 0.5 * t2 - 1.5 * t + 9.0 / 8.0, 1.5 * t2 - 2 * t, c);
+        if (ix == 0) return dmk<R>((t < R(0.5)) ? R(0.5) * t2 - R(0.5) * t + R(1.0) / R(8.0) : 0, -R(0.5) * t2 + t - R(0.5), c);
+        if (ix == 1) return dmk<R>((t < R(0.5)) ? R(3.0) / R(4.0) - t2 : R(0.5) * t2 - R(1.5) * t + R(9.0) / R(8.0), R(1.5) * t2 - 2 * t, c);
         if (ix == 2)
-            return dmk((t < 0.5) ? 0.5 * t2 + 0.5 * t + 1.0 / 8.0 : -t2 + 2 * t - 1.0 / 4.0, -1.5 * t2 + t + 0.5, c);
-        return dmk((t < 0.5) ? 0 : 0.5 * t2 - 0.5 * t + 1.0 / 8.0, 0.5 * t2, c);
+            return dmk<R>((t < R(0.5)) ? R(0.5) * t2 + R(0.5) * t + R(1.0) / R(8.0) : -t2 + 2 * t - R(1.0) / R(4.0), -R(1.5) * t2 + t + R(0.5), c);
+        return dmk<R>((t < R(0.5)) ? 0 : R(0.5) * t2 - R(0.5) * t + R(1.0) / R(8.0), R(0.5) * t2, c);
     } else {
-        const double xf = floor(xmid + 0.5);
-        const double t = xmid + 0.5 - xf;
-        const double t2 = t * t, t3 = t2 * t;
+        const R xf = floor(xmid + R(0.5));
+        const R t = xmid + R(0.5) - xf;
+        const R t2 = t * t, t3 = t2 * t;
         const int c = int(xf) - 2 + ix;
-        const bool lo = t < 0.5;
+        const bool lo = t < R(0.5);
         if (ix == 0)
-            return dmk(lo ? -1.0 / 6.0 * t3 + 0.25 * t2 - 1.0 / 8.0 * t + 1.0 / 48.0 : 0,
-                       1.0 / 6.0 * t3 - 0.5 * t2 + 0.5 * t - 1.0 / 6.0, c);
+            return dmk<R>(lo ? -R(1.0) / R(6.0) * t3 + R(0.25) * t2 - R(1.0) / R(8.0) * t + R(1.0) / R(48.0) : 0,
+                       R(1.0) / R(6.0) * t3 - R(0.5) * t2 + R(0.5) * t - R(1.0) / R(6.0), c);
         if (ix == 1)
-            return dmk(lo ? 0.5 * t3 - 0.25 * t2 - 5.0 / 8.0 * t + 23.0 / 48.0
-                          : -1.0 / 6.0 * t3 + 0.75 * t2 - 9.0 / 8.0 * t + 9.0 / 16.0,
-                       -2.0 / 3.0 * t3 + 1.5 * t2 - 0.5 * t - 0.5, c);
+            return dmk<R>(lo ? R(0.5) * t3 - R(0.25) * t2 - R(5.0) / R(8.0) * t + R(23.0) / R(48.0)
+                          : -R(1.0) / R(6.0) * t3 + R(0.75) * t2 - R(9.0) / R(8.0) * t + R(9.0) / R(16.0),
+                       -R(2.0) / R(3.0) * t3 + R(1.5) * t2 - R(0.5) * t - R(0.5), c);
         if (ix == 2)
-            return dmk(lo ? -0.5 * t3 - 0.25 * t2 + 5.0 / 8.0 * t + 23.0 / 48.0
-                          : 0.5 * t3 - 7.0 / 4.0 * t2 + 11.0 / 8.0 * t + 17.0 / 48.0,
-                       t3 - 1.5 * t2 - 0.5 * t + 0.5, c);
+            return dmk<R>(lo ? -R(0.5) * t3 - R(0.25) * t2 + R(5.0) / R(8.0) * t + R(23.0) / R(48.0)
+                          : R(0.5) * t3 - R(7.0) / R(4.0) * t2 + R(11.0) / R(8.0) * t + R(17.0) / R(48.0),
+                       t3 - R(1.5) * t2 - R(0.5) * t + R(0.5), c);
         if (ix == 3)
-            return dmk(lo ? 1.0 / 6.0 * t3 + 0.25 * t2 + 1.0 / 8.0 * t + 1.0 / 48.0
-                          : -0.5 * t3 + 5.0 / 4.0 * t2 - 3.0 / 8.0 * t + 5.0 / 48.0,
-                       -2.0 / 3.0 * t3 + 0.5 * t2 + 0.5 * t + 1.0 / 6.0, c);
-        return dmk(lo ? 0 : 1.0 / 6.0 * t3 - 0.25 * t2 + 1.0 / 8.0 * t - 1.0 / 48.0, 1.0 / 6.0 * t3, c);
+            return dmk<R>(lo ? R(1.0) / R(6.0) * t3 + R(0.25) * t2 + R(1.0) / R(8.0) * t + R(1.0) / R(48.0)
+                          : -R(0.5) * t3 + R(5.0) / R(4.0) * t2 - R(3.0) / R(8.0) * t + R(5.0) / R(48.0),
+                       -R(2.0) / R(3.0) * t3 + R(0.5) * t2 + R(0.5) * t + R(1.0) / R(6.0), c);
+        return dmk<R>(lo ? 0 : R(1.0) / R(6.0) * t3 - R(0.25) * t2 + R(1.0) / R(8.0) * t - R(1.0) / R(48.0), R(1.0) / R(6.0) * t3, c);
     }
 }
 
 // type 2: centred derivative, O+3 cells
-template <int O>
-__host__ __device__ __forceinline__ Dsf dshape_centred(double xmid, int ix) {
+template <int O, class R = double>
+__host__ __device__ __forceinline__ DsfR<R> dshape_centred(R xmid, int ix) {
     if constexpr (O == 0) {
-        const double xf = floor(xmid + 0.5);
+        const R xf = floor(xmid + R(0.5));
         const int c = int(xf) - 1 + ix;
-        if (ix == 0) return dmk(0, -0.5, c);
-        if (ix == 1) return dmk(1, 0, c);
-        return dmk(0, 0.5, c);
+        if (ix == 0) return dmk<R>(0, -R(0.5), c);
+        if (ix == 1) return dmk<R>(1, 0, c);
+        return dmk<R>(0, R(0.5), c);
     } else if constexpr (O == 1) {
-        const double xf = floor(xmid);
-        const double t = xmid - xf;
+        const R xf = floor(xmid);
+        const R t = xmid - xf;
         const int c = int(xf) - 1 + ix;
-        if (ix == 0) return dmk(0, 0.5 * t - 0.5, c);
-        if (ix == 1) return dmk(1 - t, -0.5 * t, c);
-        if (ix == 2) return dmk(t, 0.5 - 0.5 * t, c);
-        return dmk(0, 0.5 * t, c);
+        if (ix == 0) return dmk<R>(0, R(0.5) * t - R(0.5), c);
+        if (ix == 1) return dmk<R>(1 - t, -R(0.5) * t, c);
+        if (ix == 2) return dmk<R>(t, R(0.5) - R(0.5) * t, c);
+        return dmk<R>(0, R(0.5) * t, c);
     } else if constexpr (O == 2) {
-        const double xf = floor(xmid + 0.5);
-        const double t = xmid + 0.5 - xf;
-        const double t2 = t * t;
+        const R xf = floor(xmid + R(0.5));
+        const R t = xmid + R(0.5) - xf;
+        const R t2 = t * t;
         const int c = int(xf) - 2 + ix;
-        if (ix == 0) return dmk(0, -0.25 * t2 + 0.5 * t - 0.25, c);
-        if (ix == 1) return dmk(0.5 * t2 - t + 0.5, 0.5 * t2 - 0.5 * t - 0.25, c);
-        if (ix == 2) return dmk(-t2 + t + 0.5, 0.25 - 0.5 * t, c);
-        if (ix == 3) return dmk(0.5 * t2, -0.5 * t2 + 0.5 * t + 0.25, c);
-        return dmk(0, 0.25 * t2, c);
+        if (ix == 0) return dmk<R>(0, -R(0.25) * t2 + R(0.5) * t - R(0.25), c);
+        if (ix == 1) return dmk<R>(R(0.5) * t2 - t + R(0.5), R(0.5) * t2 - R(0.5) * t - R(0.25), c);
+        if (ix == 2) return dmk<R>(-t2 + t + R(0.5), R(0.25) - R(0.5) * t, c);
+        if (ix == 3) return dmk<R>(R(0.5) * t2, -R(0.5) * t2 + R(0.5) * t + R(0.25), c);
+        return dmk<R>(0, R(0.25) * t2, c);
     } else {
-        const double xf = floor(xmid);
-        const double t = xmid - xf;
-        const double t2 = t * t, t3 = t2 * t;
+        const R xf = floor(xmid);
+        const R t = xmid - xf;
+        const R t2 = t * t, t3 = t2 * t;
         const int c = int(xf) - 2 + ix;
-        if (ix == 0) return dmk(0, 1.0 / 12.0 * t3 - 0.25 * t2 + 0.25 * t - 1.0 / 12.0, c);
-        if (ix == 1) return dmk(-1.0 / 6.0 * t3 + 0.5 * t2 - 0.5 * t + 1.0 / 6.0, -0.25 * t3 + 0.5 * t2 - 1.0 / 3.0, c);
-        if (ix == 2) return dmk(0.5 * t3 - t2 + 2.0 / 3.0, 1.0 / 6.0 * t3 - 0.5 * t, c);
-        if (ix == 3) return dmk(-0.5 * t3 + 0.5 * t2 + 0.5 * t + 1.0 / 6.0, 1.0 / 6.0 * t3 - 0.5 * t2 + 1.0 / 3.0, c);
-        if (ix == 4) return dmk(1.0 / 6.0 * t3, -0.25 * t3 + 0.25 * t2 + 0.25 * t + 1.0 / 12.0, c);
-        return dmk(0, 1.0 / 12.0 * t3, c);
+        if (ix == 0) return dmk<R>(0, R(1.0) / R(12.0) * t3 - R(0.25) * t2 + R(0.25) * t - R(1.0) / R(12.0), c);
+        if (ix == 1) return dmk<R>(-R(1.0) / R(6.0) * t3 + R(0.5) * t2 - R(0.5) * t + R(1.0) / R(6.0), -R(0.25) * t3 + R(0.5) * t2 - R(1.0) / R(3.0), c);
+        if (ix == 2) return dmk<R>(R(0.5) * t3 - t2 + R(2.0) / R(3.0), R(1.0) / R(6.0) * t3 - R(0.5) * t, c);
+        if (ix == 3) return dmk<R>(-R(0.5) * t3 + R(0.5) * t2 + R(0.5) * t + R(1.0) / R(6.0), R(1.0) / R(6.0) * t3 - R(0.5) * t2 + R(1.0) / R(3.0), c);
+        if (ix == 4) return dmk<R>(R(1.0) / R(6.0) * t3, -R(0.25) * t3 + R(0.25) * t2 + R(0.25) * t + R(1.0) / R(12.0), c);
+        return dmk<R>(0, R(1.0) / R(12.0) * t3, c);
     }
 }
 
-template <int T, int O>
-__host__ __device__ __forceinline__ Dsf dshape(double xmid, int ix) {
-    if constexpr (T == 0) return dshape_analytic<O>(xmid, ix);
-    else if constexpr (T == 1) return dshape_nodal<O>(xmid, ix);
-    else return dshape_centred<O>(xmid, ix);
+template <int T, int O, class R = double>
+__host__ __device__ __forceinline__ DsfR<R> dshape(R xmid, int ix) {
+    if constexpr (T == 0) return dshape_analytic<O, R>(xmid, ix);
+    else if constexpr (T == 1) return dshape_nodal<O, R>(xmid, ix);
+    else return dshape_centred<O, R>(xmid, ix);
 }
 
 // number of cells of the derivative stencil

@@ -68,6 +68,9 @@ class SliceConfig:
     # plasma deposition walks the particles sorted by 16x16-cell tiles, re-sorted every
     # bin_period slices (0: index order); only the order of floating-point sums depends on it
     bin_period: int = 4
+    # "fp64": the drop-in path; "mixed": FP32 gather, push increments and deposition with
+    # compensated LDS sums (FP64 store, particle state and multigrid)
+    precision: str = "fp64"
 
     @property
     def dx(self) -> float:

@@ -37,6 +37,7 @@ def make_params(cfg: SliceConfig) -> _abi.Params:
     p.pc_tol, p.pc_maxiter, p.pc_mix = cfg.pc_tol, cfg.pc_maxiter, cfg.pc_mix
     p.plasma_charge, p.plasma_mass, p.max_qsa = cfg.plasma_charge, cfg.plasma_mass, cfg.max_qsa
     p.bin_period = cfg.bin_period
+    p.precision = _abi.PRECISION[cfg.precision]
     return p
 
 

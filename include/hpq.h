@@ -36,6 +36,11 @@ enum {
 
 enum { HPQ_BC_REFLECTING = 0, HPQ_BC_PERIODIC = 1, HPQ_BC_ABSORBING = 2 };
 
+/* Arithmetic of the particle kernels: FP64 throughout (the drop-in path), or FP32 gather,
+ * push increments and deposition (compensated LDS sums, FP64 slice store and particle state,
+ * FP64 multigrid). */
+enum { HPQ_PREC_FP64 = 0, HPQ_PREC_MIXED = 1 };
+
 /* Run parameters: the reference input keys this path honours
  * (src/Hipace.cpp:97-213, src/Hipace.H:208-252). Units are normalised (c=ep0=mu0=1). */
 typedef struct hpq_params {
@@ -61,6 +66,7 @@ typedef struct hpq_params {
                                             slices (plasmas.reorder_period plays this role
                                             in the reference); 0: index order. Changes only
                                             the order of floating-point sums. */
+    int precision;                       /* HPQ_PREC_FP64 or HPQ_PREC_MIXED */
 } hpq_params;
 
 /* Plasma species, SoA device arrays in PlasmaIdx order
