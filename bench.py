@@ -5,11 +5,17 @@ Workload (one step = one pass of the slice loop over all ζ-slices, inputs resid
   n0 = 1 at 2 x 2 particles per cell (1 044 484 plasma particles per slice), shape order 2,
   explicit Bx/By solver with multigrid; Gaussian electron drive beam with peak density
   0.01 n0, sigma_r = 0.3, sigma_z = 1.41, 4e6 macro-particles (seeded Philox), gamma = 1000.
-  Arithmetic in FP64 (the plain path).
 
-Prints one JSON line: slices/s (value), plasma particle pushes/s, the roofline of the
-dominant kernel (HIP-event timed inside the timed region) and the oracle timed on the host
-cores on a bounded sample of the same workload.
+Two arithmetic paths are timed on the same workload: the FP64 plain path (the drop-in)
+and the mixed path (FP32 gather, push increments and deposition with compensated FP32
+window sums; DST-I as split-BF16 sine-matrix GEMMs on the matrix cores; FP64 multigrid
+and particle state). One more untimed step of each records
+Ez, Ex-By, jx and rho-jz on every slice, and the mixed path's norm-wise relative error
+against the FP64 path is reported.
+
+Prints one JSON line: slices/s of the mixed path (value) beside the FP64 path's, plasma
+particle pushes/s, the roofline of the dominant kernel (HIP-event timed inside the timed
+region) and the oracle timed on the host cores on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -25,7 +31,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-WORKLOAD = "511x511x512 blowout-geometry slice loop, 2x2 ppc, shape 2, explicit MG, FP64"
+WORKLOAD = "511x511x512 blowout-geometry slice loop, 2x2 ppc, shape 2, explicit MG"
+ERR_FIELDS = ("Ez", "ExmBy", "jx", "rhomjz")
+DTYPE = {"fp64": "f64",
+         "mixed": "f32 (gather, push increments, deposition) + split-bf16 x3 / f32 accumulate (DST) + f64 "
+                  "(multigrid, particle state)"}
 
 
 def make_workload(n=511, nz=512, ppc=2, nbeam=4_000_000):
@@ -53,13 +63,16 @@ def bytes_per_launch(kind, cfg, n_plasma):
     return None
 
 
-def run_gpu(args):
+def run_gpu(args, precision, capture):
     import torch
-    from hipace2_amd.simulation import PLASMA_REALS, Simulation
+    from hipace2_amd.simulation import Simulation
 
     cfg, beam = make_workload(args.n, args.nz, args.ppc, args.nbeam)
     if args.bin_period is not None:
         cfg.bin_period = args.bin_period
+    cfg.precision = precision
+    if precision == "mixed":
+        cfg.poisson_solver = args.mixed_poisson
     sim = Simulation(cfg, beam)
     sim.init_plasma()
     pristine = {k: v.clone() for k, v in sim.plasma.items()}
@@ -98,7 +111,24 @@ def run_gpu(args):
     traces = {k: sim.trace_read(k) for k in timed}
     stats = sim.stats()
     sim.trace(timed, False)
-    return cfg, beam, sim, elapsed, traces, stats, breakdown
+    caps = None
+    if capture:
+        caps = {k: sim.capture(k) for k in ERR_FIELDS}
+        step()
+        torch.cuda.synchronize()
+    print(f"{precision}: {cfg.nz * args.steps / elapsed:.1f} slices/s", file=sys.stderr, flush=True)
+    return dict(cfg=cfg, beam=beam, sim=sim, elapsed=elapsed, traces=traces, stats=stats, breakdown=breakdown,
+                caps=caps)
+
+
+def errors_vs_fp64(mixed, fp64):
+    """max |mixed - fp64| / max |fp64| over every slice of one step, per field."""
+    out = {}
+    for k in ERR_FIELDS:
+        a, b = mixed[k], fp64[k]
+        den = float(b.abs().max())
+        out[k] = float((a - b).abs().max()) / den if den > 0 else float((a - b).abs().max())
+    return out
 
 
 def dominant_kind(cfg, n_plasma, traces):
@@ -156,13 +186,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bin-period", type=int, default=None,
                     help="slices between tile sorts of the plasma for deposition (0: index order)")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
+                    help="arithmetic path(s) to time; 'both' also reports the mixed path's error")
+    ap.add_argument("--mixed-poisson", choices=["bf16x3", "fft"], default="bf16x3",
+                    help="transverse Poisson solver of the mixed path")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the slice loop is not sharded across GPUs "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    cfg, beam, sim, elapsed, traces, stats, breakdown = run_gpu(args)
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    runs = {p: run_gpu(args, p, capture=len(precs) == 2) for p in precs}
+    head = precs[-1]
+    r = runs[head]
+    cfg, beam, sim, elapsed, traces, stats, breakdown = (r[k] for k in ("cfg", "beam", "sim", "elapsed", "traces",
+                                                                        "stats", "breakdown"))
     n_plasma = sim.np
     slices = cfg.nz * args.steps
     slices_per_s = slices / elapsed
@@ -186,9 +225,25 @@ def main():
         print(f"kernel times over {args.warmup} warm-up step(s): " + json.dumps(table(breakdown)), file=sys.stderr)
     print("dominant kernel over the timed region: " + json.dumps(rows), file=sys.stderr)
 
+    extra = {}
+    if len(precs) == 2:
+        f = runs["fp64"]
+        f_slices = cfg.nz * args.steps / f["elapsed"]
+        extra["fp64"] = {"value": round(f_slices, 2), "unit": "slices/s",
+                         "ms_per_step": round(1e3 * f["elapsed"] / args.steps, 2),
+                         "plasma_pushes_per_s": round(f["stats"]["pushes"] * args.steps / f["elapsed"], 1),
+                         "mg_vcycles_per_slice": round(f["stats"]["mg_iters"] / cfg.nz, 3)}
+        extra["speedup_vs_fp64"] = round(slices_per_s / f_slices, 3)
+        extra["max_rel_err_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in
+                                        errors_vs_fp64(r["caps"], f["caps"]).items()}
+        for p in precs:
+            runs[p]["caps"] = None
+            runs[p]["sim"].clear_captures()
+
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args, cfg, beam, sim)
+        sim_cpu = runs["fp64"]["sim"] if "fp64" in runs else sim
+        cpu = cpu_baseline(args, cfg, beam, sim_cpu)
 
     line = {
         "metric": "slices/s + plasma pushes/s",
@@ -201,14 +256,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": DTYPE[head],
         "data": "synthetic (lattice plasma, seeded Philox Gaussian beam)",
         "config": {"workload": WORKLOAD if (args.n, args.nz, args.ppc) == (511, 512, 2) else
-                   f"{args.n}x{args.n}x{args.nz} slice loop, {args.ppc}x{args.ppc} ppc, shape 2, explicit MG, FP64",
+                   f"{args.n}x{args.n}x{args.nz} slice loop, {args.ppc}x{args.ppc} ppc, shape 2, explicit MG",
+                   "precision": head, "poisson_solver": cfg.poisson_solver,
                    "nx": cfg.nx, "ny": cfg.ny, "slices": cfg.nz, "plasma_particles_per_slice": n_plasma,
                    "beam_particles": beam.n, "mg_vcycles_per_slice": round(stats["mg_iters"] / cfg.nz, 3),
                    "bin_period": cfg.bin_period},
         "plasma_pushes_per_s": round(pushes_per_s, 1),
+        **extra,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -42,9 +42,10 @@ class Params(C.Structure):
                 ("particle_bc", C.c_int), ("plo", C.c_double * 2), ("phi", C.c_double * 2),
                 ("pc_tol", C.c_double), ("pc_maxiter", C.c_int), ("pc_mix", C.c_double),
                 ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double),
-                ("bin_period", C.c_int), ("precision", C.c_int)]
+                ("bin_period", C.c_int), ("precision", C.c_int), ("poisson_solver", C.c_int)]
 
 PRECISION = {"fp64": 0, "mixed": 1}
+POISSON_SOLVER = {"fft": 0, "bf16x3": 1}
 
 
 class Plasma(C.Structure):

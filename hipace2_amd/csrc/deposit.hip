@@ -93,9 +93,40 @@ __device__ Win choose_window(const int* ci, const int* cj, const bool* act, int 
     return {i0, j0, w, h};
 }
 
+// Accumulation in the window. FP64: one atomic add per contribution. FP32 (mixed
+// precision): slot s holds, for each of its `cells` cells, the running FP32 sum and beside
+// it the sum of the exact rounding errors of the additions into it (TwoSum, Knuth), so the
+// window keeps about twice the FP32 precision; both go to the FP64 store on the flush.
+template <class R>
+struct WinAcc;
+template <>
+struct WinAcc<double> {
+    static __device__ __forceinline__ void add(double* win, int cells, int s, int k, double v) {
+        lds_add(&win[s * cells + k], v);
+    }
+    static __device__ __forceinline__ double get(const double* win, int cells, int s, int k) {
+        return win[s * cells + k];
+    }
+};
+template <>
+struct WinAcc<float> {
+    static __device__ __forceinline__ void add(double* win, int cells, int s, int k, float v) {
+        float* w = reinterpret_cast<float*>(win) + 2 * s * cells;
+        const float old = __hip_atomic_fetch_add(&w[k], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const float sum = old + v;
+        const float bp = sum - old;
+        const float err = (old - (sum - bp)) + (v - bp);
+        __hip_atomic_fetch_add(&w[cells + k], err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    static __device__ __forceinline__ double get(const double* win, int cells, int s, int k) {
+        const float* w = reinterpret_cast<const float*>(win) + 2 * s * cells;
+        return double(w[k]) + double(w[cells + k]);
+    }
+};
+
 // adds the window to the store: quantity k (slot[k] >= 0) goes to component comp[k];
 // cells holding zero are skipped
-template <int NK>
+template <class R, int NK>
 __device__ void flush_window(const Geom& g, const Win& W, const double* win, const int (&slot)[NK],
                              const int (&comp)[NK], double* store) {
     const int cells = W.w * W.h;
@@ -106,7 +137,7 @@ __device__ void flush_window(const Geom& g, const Win& W, const double* win, con
 #pragma unroll
         for (int q = 0; q < NK; ++q) {
             if (slot[q] < 0) continue;
-            const double v = win[slot[q] * cells + t];
+            const double v = WinAcc<R>::get(win, cells, slot[q], t);
             if (v != 0.0) atomic_add(store + long(comp[q]) * g.plane + k, v);
         }
     }
@@ -124,9 +155,9 @@ __device__ __forceinline__ int assign_slots(const int (&comp)[NK], int (&slot)[N
     return ns;
 }
 
-// ::DepositCurrent deposit functor, src/particles/deposition/PlasmaDepositCurrent.cpp:182-251.
-// comp order: jx, jy, jz, rho, chi, rhomjz (-1: skip; jx and jy go together).
-template <int O>
+// ::DepositCurrent deposit functor, src/particles/deposition/PlasmaDepositCurrent.cpp:182-251,
+// arithmetic in R. comp order: jx, jy, jz, rho, chi, rhomjz (-1: skip; jx and jy go together).
+template <int O, class R>
 __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, const int32_t* __restrict__ order,
                                                        double* __restrict__ store, int cjx, int cjy, int cjz, int crho,
                                                        int cchi, int crm, double charge_invvol,
@@ -135,12 +166,13 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
     constexpr int S = O + 1;
     __shared__ double win[WIN_DEPOSIT];
     __shared__ int red[7 * DT / 64];
-    const double clight = 1.0, clightinv = 1.0 / 1.0;
+    const R clight = 1.0, clightinv = R(1.0) / R(1.0);
     const int comp[6] = {cjx, cjy, cjz, crho, cchi, crm};
     int slot[6];
     const int ns = assign_slots(comp, slot);
 
-    double q_invvol[DPPT], vx_c[DPPT], vy_c[DPPT], gamma_psi[DPPT], psi_inv[DPPT], xmid[DPPT], ymid[DPPT];
+    R q_invvol[DPPT], vx_c[DPPT], vy_c[DPPT], gamma_psi[DPPT], psi_inv[DPPT];
+    CellPos<R> xmid[DPPT], ymid[DPPT];
     int ci[DPPT], cj[DPPT];
     bool act[DPPT];
     const long base = long(blockIdx.x) * DCHUNK;
@@ -151,26 +183,26 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
         if (t >= p.n) continue;
         const long ip = order ? long(order[t]) : t;
         if (!p.valid[ip]) continue;
-        psi_inv[q] = 1.0 / p.psi[ip];
+        psi_inv[q] = R(1.0) / R(p.psi[ip]);
         const double xp = p.x[ip], yp = p.y[ip];
-        vx_c[q] = p.ux[ip] * psi_inv[q];
-        vy_c[q] = p.uy[ip] * psi_inv[q];
-        q_invvol[q] = charge_invvol * p.w[ip];
-        xmid[q] = (xp - g.xoff) * g.dxi;
-        ymid[q] = (yp - g.yoff) * g.dyi;
-        const double Aabssqp = 0.0;
-        gamma_psi[q] = 0.5 * ((1.0 + 0.5 * Aabssqp) * psi_inv[q] * psi_inv[q] +
-                              vx_c[q] * vx_c[q] * clightinv * clightinv + vy_c[q] * vy_c[q] * clightinv * clightinv +
-                              1.0);
-        if (gamma_psi[q] < 0.0 || gamma_psi[q] > max_qsa || psi_inv[q] < 0.0) {
+        vx_c[q] = R(p.ux[ip]) * psi_inv[q];
+        vy_c[q] = R(p.uy[ip]) * psi_inv[q];
+        q_invvol[q] = R(charge_invvol) * R(p.w[ip]);
+        xmid[q] = cell_pos<R>((xp - g.xoff) * g.dxi);
+        ymid[q] = cell_pos<R>((yp - g.yoff) * g.dyi);
+        const R Aabssqp = 0.0;
+        gamma_psi[q] = R(0.5) * ((R(1.0) + R(0.5) * Aabssqp) * psi_inv[q] * psi_inv[q] +
+                                 vx_c[q] * vx_c[q] * clightinv * clightinv +
+                                 vy_c[q] * vy_c[q] * clightinv * clightinv + R(1.0));
+        if (gamma_psi[q] < R(0.0) || gamma_psi[q] > R(max_qsa) || psi_inv[q] < R(0.0)) {
             atomicAdd(qsa_count, 1);
             p.w[ip] = 0.0;
             p.valid[ip] = 0;
             continue;
         }
         act[q] = true;
-        ci[q] = shape<O>(xmid[q], 0).cell;
-        cj[q] = shape<O>(ymid[q], 0).cell;
+        ci[q] = shape<O, R>(xmid[q], 0).cell;
+        cj[q] = shape<O, R>(ymid[q], 0).cell;
     }
     const Win W = choose_window(ci, cj, act, S, ns > 0 ? WIN_DEPOSIT / ns : 0, red);
     const int cells = W.w * W.h;
@@ -179,42 +211,44 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
 #pragma unroll
     for (int q = 0; q < DPPT; ++q) {
         if (!act[q]) continue;
-        Sf sx[S], sy[S];
+        SfR<R> sx[S], sy[S];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
-            sx[k] = shape<O>(xmid[q], k);
-            sy[k] = shape<O>(ymid[q], k);
+            sx[k] = shape<O, R>(xmid[q], k);
+            sy[k] = shape<O, R>(ymid[q], k);
         }
         const bool local = W.holds(ci[q], cj[q], S);
+        const R cmmr = R(charge_mu0_mass_ratio);
 #pragma unroll
         for (int iy = 0; iy < S; ++iy) {
 #pragma unroll
             for (int ix = 0; ix < S; ++ix) {
                 const int i = sx[ix].cell, j = sy[iy].cell;
                 if (!in_store(g, i, j)) continue;
-                const double cd = q_invvol[q] * sx[ix].s * sy[iy].s;
-                const double v[6] = {cd * vx_c[q], cd * vy_c[q], cd * (gamma_psi[q] - 1.0) * clight,
-                                     cd * gamma_psi[q], cd * charge_mu0_mass_ratio * psi_inv[q], cd};
+                const R cd = q_invvol[q] * sx[ix].s * sy[iy].s;
+                const R v[6] = {cd * vx_c[q], cd * vy_c[q], cd * (gamma_psi[q] - R(1.0)) * clight,
+                                cd * gamma_psi[q], cd * cmmr * psi_inv[q], cd};
                 if (local) {
                     const int k = W.at(i, j);
 #pragma unroll
                     for (int a = 0; a < 6; ++a)
-                        if (slot[a] >= 0) lds_add(&win[slot[a] * cells + k], v[a]);
+                        if (slot[a] >= 0) WinAcc<R>::add(win, cells, slot[a], k, v[a]);
                 } else {
                     const long k = g.idx(i, j);
 #pragma unroll
                     for (int a = 0; a < 6; ++a)
-                        if (slot[a] >= 0) atomic_add(store + long(comp[a]) * g.plane + k, v[a]);
+                        if (slot[a] >= 0) atomic_add(store + long(comp[a]) * g.plane + k, double(v[a]));
                 }
             }
         }
     }
     __syncthreads();
-    flush_window(g, W, win, slot, comp, store);
+    flush_window<R>(g, W, win, slot, comp, store);
 }
 
-// ::ExplicitDeposition deposit functor, src/particles/deposition/ExplicitDeposition.cpp:166-262
-template <int O, int T>
+// ::ExplicitDeposition deposit functor, src/particles/deposition/ExplicitDeposition.cpp:166-262,
+// arithmetic in R (the fields are read from the FP64 store and rounded to R)
+template <int O, int T, class R>
 __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, const int32_t* __restrict__ order,
                                                          double* __restrict__ store, int cBz, int cEz, int cExmBy,
                                                          int cEypBx, int cSy, int cSx, double charge_invvol_mu0,
@@ -222,8 +256,9 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
     constexpr int S = dshape_width<T, O>();
     __shared__ double win[WIN_EXPLICIT];
     __shared__ int red[7 * DT / 64];
-    const double clight = 1.0, clight_inv = 1.0 / 1.0;
-    double xmid[DPPT], ymid[DPPT], vx[DPPT], vy[DPPT], psi_inv[DPPT], cdm_[DPPT];
+    const R clight = 1.0, clight_inv = R(1.0) / R(1.0);
+    R vx[DPPT], vy[DPPT], psi_inv[DPPT], cdm_[DPPT];
+    CellPos<R> xmid[DPPT], ymid[DPPT];
     int ci[DPPT], cj[DPPT];
     bool act[DPPT];
     const long base = long(blockIdx.x) * DCHUNK;
@@ -235,15 +270,15 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
         const long ip = order ? long(order[t]) : t;
         if (!p.valid[ip]) continue;
         act[q] = true;
-        psi_inv[q] = 1.0 / p.psi[ip];
+        psi_inv[q] = R(1.0) / R(p.psi[ip]);
         const double xp = p.x[ip], yp = p.y[ip];
-        vx[q] = p.ux[ip] * psi_inv[q] * clight_inv;
-        vy[q] = p.uy[ip] * psi_inv[q] * clight_inv;
-        cdm_[q] = charge_invvol_mu0 * p.w[ip];
-        xmid[q] = (xp - g.xoff) * g.dxi;
-        ymid[q] = (yp - g.yoff) * g.dyi;
-        ci[q] = dshape<T, O>(xmid[q], 0).cell;
-        cj[q] = dshape<T, O>(ymid[q], 0).cell;
+        vx[q] = R(p.ux[ip]) * psi_inv[q] * clight_inv;
+        vy[q] = R(p.uy[ip]) * psi_inv[q] * clight_inv;
+        cdm_[q] = R(charge_invvol_mu0) * R(p.w[ip]);
+        xmid[q] = cell_pos<R>((xp - g.xoff) * g.dxi);
+        ymid[q] = cell_pos<R>((yp - g.yoff) * g.dyi);
+        ci[q] = dshape<T, O, R>(xmid[q], 0).cell;
+        cj[q] = dshape<T, O, R>(ymid[q], 0).cell;
     }
     // window planes: Sy, Sx (accumulated), then copies of Bz, Ez, ExmBy, EypBx
     const Win W = choose_window(ci, cj, act, S, WIN_EXPLICIT / 6, red);
@@ -276,21 +311,23 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
         // with A = -Bz vx + (Ez vy - ExmBy vx vy + EypBx (gamma/psi - vy^2)) / c and
         // B = Bz vy + (Ez vx + ExmBy (gamma/psi - vx^2) - EypBx vx vy) / c (no laser term), is
         // regrouped into per-particle coefficients times three shape-factor products.
-        const double q_mass_ratio = charge_mass_ratio;
-        const double cdm = cdm_[q];
-        const double pvx = vx[q], pvy = vy[q], pinv = psi_inv[q];
-        const double Aabssqp = 0.0;
-        const double gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * pinv * pinv + pvx * pvx + pvy * pvy + 1.0);
-        const double vxvy = pvx * pvy;
-        const double gpx = gamma_psi - pvx * pvx, gpy = gamma_psi - pvy * pvy;
-        const double qp = q_mass_ratio * pinv;
-        const double ay = -cdm * qp, by = cdm * clight * g.dxi * vxvy, cy = -cdm * clight * g.dyi * (gpy - 1.0);
-        const double ax = cdm * qp, bx = cdm * clight * g.dxi * (gpx - 1.0), cx = -cdm * clight * g.dyi * vxvy;
-        Dsf sx[S], sy[S];
+        const R q_mass_ratio = R(charge_mass_ratio);
+        const R cdm = cdm_[q];
+        const R pvx = vx[q], pvy = vy[q], pinv = psi_inv[q];
+        const R Aabssqp = 0.0;
+        const R gamma_psi =
+            R(0.5) * ((R(1.0) + R(0.5) * Aabssqp) * pinv * pinv + pvx * pvx + pvy * pvy + R(1.0));
+        const R vxvy = pvx * pvy;
+        const R gpx = gamma_psi - pvx * pvx, gpy = gamma_psi - pvy * pvy;
+        const R qp = q_mass_ratio * pinv;
+        const R dxi = R(g.dxi), dyi = R(g.dyi);
+        const R ay = -cdm * qp, by = cdm * clight * dxi * vxvy, cy = -cdm * clight * dyi * (gpy - R(1.0));
+        const R ax = cdm * qp, bx = cdm * clight * dxi * (gpx - R(1.0)), cx = -cdm * clight * dyi * vxvy;
+        DsfR<R> sx[S], sy[S];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
-            sx[k] = dshape<T, O>(xmid[q], k);
-            sy[k] = dshape<T, O>(ymid[q], k);
+            sx[k] = dshape<T, O, R>(xmid[q], k);
+            sy[k] = dshape<T, O, R>(ymid[q], k);
         }
         const bool local = W.holds(ci[q], cj[q], S);
 #pragma unroll
@@ -298,28 +335,28 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
 #pragma unroll
             for (int ix = 0; ix < S; ++ix) {
                 if (T == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) continue;
-                const Dsf X = sx[ix], Y = sy[iy];
+                const DsfR<R> X = sx[ix], Y = sy[iy];
                 if (!in_store(g, X.cell, Y.cell)) continue;
                 const long k = g.idx(X.cell, Y.cell);
                 const int kw = local ? W.at(X.cell, Y.cell) : 0;
-                double Bz_v, Ez_v, ExmBy_v, EypBx_v;
+                R Bz_v, Ez_v, ExmBy_v, EypBx_v;
                 if (local) {
-                    Bz_v = win[2 * cells + kw]; Ez_v = win[3 * cells + kw];
-                    ExmBy_v = win[4 * cells + kw]; EypBx_v = win[5 * cells + kw];
+                    Bz_v = R(win[2 * cells + kw]); Ez_v = R(win[3 * cells + kw]);
+                    ExmBy_v = R(win[4 * cells + kw]); EypBx_v = R(win[5 * cells + kw]);
                 } else {
-                    Bz_v = Bz[k]; Ez_v = Ez[k]; ExmBy_v = ExmBy[k]; EypBx_v = EypBx[k];
+                    Bz_v = R(Bz[k]); Ez_v = R(Ez[k]); ExmBy_v = R(ExmBy[k]); EypBx_v = R(EypBx[k]);
                 }
-                const double A = fma(-Bz_v, pvx, fma(EypBx_v, gpy, fma(-ExmBy_v, vxvy, Ez_v * pvy)) * clight_inv);
-                const double B = fma(Bz_v, pvy, fma(-EypBx_v, vxvy, fma(ExmBy_v, gpx, Ez_v * pvx)) * clight_inv);
-                const double ss = X.s * Y.s, ds = X.ds * Y.s, sd = X.s * Y.ds;
-                const double sy_add = fma(ay * A, ss, fma(by, ds, cy * sd));
-                const double sx_add = fma(ax * B, ss, fma(bx, ds, cx * sd));
+                const R A = fma(-Bz_v, pvx, fma(EypBx_v, gpy, fma(-ExmBy_v, vxvy, Ez_v * pvy)) * clight_inv);
+                const R B = fma(Bz_v, pvy, fma(-EypBx_v, vxvy, fma(ExmBy_v, gpx, Ez_v * pvx)) * clight_inv);
+                const R ss = X.s * Y.s, ds = X.ds * Y.s, sd = X.s * Y.ds;
+                const R sy_add = fma(ay * A, ss, fma(by, ds, cy * sd));
+                const R sx_add = fma(ax * B, ss, fma(bx, ds, cx * sd));
                 if (local) {
-                    lds_add(&win[kw], sy_add);
-                    lds_add(&win[cells + kw], sx_add);
+                    WinAcc<R>::add(win, cells, 0, kw, sy_add);
+                    WinAcc<R>::add(win, cells, 1, kw, sx_add);
                 } else {
-                    atomic_add(Sy + k, sy_add);
-                    atomic_add(Sx + k, sx_add);
+                    atomic_add(Sy + k, double(sy_add));
+                    atomic_add(Sx + k, double(sx_add));
                 }
             }
         }
@@ -327,7 +364,7 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
     __syncthreads();
     const int comp[2] = {cSy, cSx};
     const int slot[2] = {0, 1};
-    flush_window(g, W, win, slot, comp, store);
+    flush_window<R>(g, W, win, slot, comp, store);
 }
 
 // ::DepositCurrentSlice deposit functor, src/particles/deposition/BeamDepositCurrent.cpp:133-193
@@ -401,7 +438,7 @@ __global__ __launch_bounds__(DT) void k_deposit_beam(Geom g, hpq_beam b, long fi
         }
     }
     __syncthreads();
-    flush_window(g, W, win, slot, comp, store);
+    flush_window<double>(g, W, win, slot, comp, store);
 }
 
 }  // namespace
@@ -422,11 +459,16 @@ void deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32
         c->bins.stale = true;
     }
     const int32_t* order = deposit_order(c, p);
+    const bool mixed = c->p.precision == HPQ_PREC_MIXED;
     TraceScope ts(c, HPQ_TRACE_DEPOSIT);
     dispatch_order(c->p.depos_order, [&](auto O) {
-        hipLaunchKernelGGL((k_deposit_plasma<decltype(O)::value>), grid1d(p->n, DCHUNK), dim3(DT), 0, c->stream, g,
-                           *p, order, store, cm(0), cm(1), cm(2), cm(3), cm(4), cm(5), charge_invvol, cmr,
-                           c->p.max_qsa, c->d_count);
+        constexpr int o = decltype(O)::value;
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid1d(p->n, DCHUNK), dim3(DT), 0, c->stream, g, *p, order, store, cm(0), cm(1),
+                               cm(2), cm(3), cm(4), cm(5), charge_invvol, cmr, c->p.max_qsa, c->d_count);
+        };
+        if (mixed) launch(k_deposit_plasma<o, float>);
+        else launch(k_deposit_plasma<o, double>);
     });
     HPQ_HIP(hipGetLastError());
 }
@@ -449,9 +491,15 @@ void explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store) {
             hipLaunchKernelGGL(kern, grid1d(p->n, DCHUNK), dim3(DT), 0, c->stream, g, *p, order, store, L.Bz, L.Ez,
                                L.ExmBy, L.EypBx, L.Sy, L.Sx, qimu0, qmr);
         };
-        if (T == 0) launch(k_explicit_deposit<o, 0>);
-        else if (T == 1) launch(k_explicit_deposit<o, 1>);
-        else launch(k_explicit_deposit<o, 2>);
+        if (c->p.precision == HPQ_PREC_MIXED) {
+            if (T == 0) launch(k_explicit_deposit<o, 0, float>);
+            else if (T == 1) launch(k_explicit_deposit<o, 1, float>);
+            else launch(k_explicit_deposit<o, 2, float>);
+        } else {
+            if (T == 0) launch(k_explicit_deposit<o, 0, double>);
+            else if (T == 1) launch(k_explicit_deposit<o, 1, double>);
+            else launch(k_explicit_deposit<o, 2, double>);
+        }
     });
     HPQ_HIP(hipGetLastError());
 }

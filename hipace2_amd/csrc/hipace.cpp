@@ -348,6 +348,8 @@ int hpq_create(const hpq_params* p, hpq_ctx** out) {
         if (p->depos_order < 0 || p->depos_order > 3) throw HpqError(HPQ_E_UNSUPPORTED, "depos_order_xy must be 0..3");
         if (p->precision != HPQ_PREC_FP64 && p->precision != HPQ_PREC_MIXED)
             throw HpqError(HPQ_E_INVALID, "precision must be HPQ_PREC_FP64 or HPQ_PREC_MIXED");
+        if (p->poisson_solver != HPQ_POISSON_FFT && p->poisson_solver != HPQ_POISSON_BF16X3)
+            throw HpqError(HPQ_E_INVALID, "poisson_solver must be HPQ_POISSON_FFT or HPQ_POISSON_BF16X3");
         if (p->derivative_type < 0 || p->derivative_type > 2)
             throw HpqError(HPQ_E_UNSUPPORTED, "depos_derivative_type must be 0..2");
         if (p->depos_order == 0 && p->derivative_type == 0)

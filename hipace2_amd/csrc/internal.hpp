@@ -72,6 +72,11 @@ struct Poisson {
     double* eigT = nullptr;   // [i][j] eigenvalue matrix with the RODFT00 normalisation
     double* bufA = nullptr;   // 3 * nx * ny
     double* bufB = nullptr;
+    // sine-matrix GEMM path (HPQ_POISSON_BF16X3): sizes padded to 64, S split into BF16
+    // hi/lo planes [j][k], FP32 intermediates 3 * npx * npy
+    int npx = 0, npy = 0;
+    __bf16 *sx_hi = nullptr, *sx_lo = nullptr, *sy_hi = nullptr, *sy_lo = nullptr;
+    float *gA = nullptr, *gB = nullptr;
 };
 
 // One multigrid level; arrays cover the level box plus one ghost ring.
@@ -186,6 +191,9 @@ void layout_init(hpq_ctx* c);
 void poisson_init(hpq_ctx* c);
 void poisson_free(hpq_ctx* c);
 void poisson_solve_staged(hpq_ctx* c, int nfields, double* const* lhs_planes);
+void dst_mfma_init(hpq_ctx* c);
+void dst_mfma_free(hpq_ctx* c);
+void poisson_solve_mfma(hpq_ctx* c, int nfields, double* const* lhs_planes);
 void build_rhs_psi_ez_bz(hpq_ctx* c, const double* store);
 void build_rhs_bxby(hpq_ctx* c, const double* store);
 void exmby_eypbx(hpq_ctx* c, double* store);

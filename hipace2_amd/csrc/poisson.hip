@@ -278,6 +278,7 @@ void poisson_init(hpq_ctx* c) {
     HPQ_HIP(hipMemcpy(S.eigT, ev.data(), sizeof(double) * ev.size(), hipMemcpyHostToDevice));
     HPQ_HIP(hipMalloc(&S.bufA, sizeof(double) * 3 * size_t(nx) * ny));
     HPQ_HIP(hipMalloc(&S.bufB, sizeof(double) * 3 * size_t(nx) * ny));
+    if (c->p.poisson_solver == HPQ_POISSON_BF16X3) dst_mfma_init(c);
 }
 
 void poisson_free(hpq_ctx* c) {
@@ -285,12 +286,17 @@ void poisson_free(hpq_ctx* c) {
     for (void* p : {(void*)S.px.tw, (void*)S.px.sintab, (void*)S.py.tw, (void*)S.py.sintab, (void*)S.eigT,
                     (void*)S.bufA, (void*)S.bufB})
         if (p) (void)hipFree(p);
+    dst_mfma_free(c);
     S = Poisson{};
 }
 
 // Solves nfields staged right-hand sides held in ps.bufA as [f][j][i] and writes the
 // valid region of field f into lhs_planes[f] (guarded planes).
 void poisson_solve_staged(hpq_ctx* c, int nf, double* const* lhs_planes) {
+    if (c->p.poisson_solver == HPQ_POISSON_BF16X3) {
+        poisson_solve_mfma(c, nf, lhs_planes);
+        return;
+    }
     Poisson& S = c->ps;
     const int nx = S.nx, ny = S.ny;
     const long fs = long(nx) * ny;

@@ -11,6 +11,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace hpq {
 
 template <class R>
@@ -171,6 +173,38 @@ __host__ __device__ __forceinline__ DsfR<R> dshape(R xmid, int ix) {
     if constexpr (T == 0) return dshape_analytic<O, R>(xmid, ix);
     else if constexpr (T == 1) return dshape_nodal<O, R>(xmid, ix);
     else return dshape_centred<O, R>(xmid, ix);
+}
+
+// A position in cell units as an integer base plus a remainder in R. FP64 keeps the whole
+// position in the remainder (base 0, the arithmetic of the plain overloads). FP32 keeps
+// the remainder in [0, 1], so the spline argument has FP32's full relative precision
+// instead of losing the bits of the cell index; since the splines shift with their
+// argument, cell = base + cell of the remainder.
+template <class R>
+struct CellPos { R r; int base; };
+
+template <class R>
+__host__ __device__ __forceinline__ CellPos<R> cell_pos(double xm) {
+    if constexpr (std::is_same_v<R, double>) {
+        return {xm, 0};
+    } else {
+        const double b = floor(xm);
+        return {R(xm - b), int(b)};
+    }
+}
+
+template <int O, class R>
+__host__ __device__ __forceinline__ SfR<R> shape(CellPos<R> x, int ix) {
+    SfR<R> s = shape<O, R>(x.r, ix);
+    s.cell += x.base;
+    return s;
+}
+
+template <int T, int O, class R>
+__host__ __device__ __forceinline__ DsfR<R> dshape(CellPos<R> x, int ix) {
+    DsfR<R> s = dshape<T, O, R>(x.r, ix);
+    s.cell += x.base;
+    return s;
 }
 
 // number of cells of the derivative stencil

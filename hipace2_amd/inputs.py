@@ -71,6 +71,9 @@ class SliceConfig:
     # "fp64": the drop-in path; "mixed": FP32 gather, push increments and deposition with
     # compensated LDS sums (FP64 store, particle state and multigrid)
     precision: str = "fp64"
+    # transverse Poisson solves: "fft" (DST-I by FFT, FP64) or "bf16x3" (DST-I as sine-matrix
+    # GEMMs on the matrix cores, FP32 operands split into two BF16 terms each)
+    poisson_solver: str = "fft"
 
     @property
     def dx(self) -> float:

@@ -38,6 +38,7 @@ def make_params(cfg: SliceConfig) -> _abi.Params:
     p.plasma_charge, p.plasma_mass, p.max_qsa = cfg.plasma_charge, cfg.plasma_mass, cfg.max_qsa
     p.bin_period = cfg.bin_period
     p.precision = _abi.PRECISION[cfg.precision]
+    p.poisson_solver = _abi.POISSON_SOLVER[cfg.poisson_solver]
     return p
 
 
@@ -119,6 +120,11 @@ class Simulation:
         self._captures[name] = buf
         _abi.check(self.ctx, self.L.hpq_add_capture(self.ctx, self.layout[name], C.c_void_p(buf.data_ptr())))
         return buf
+
+    def clear_captures(self):
+        """Stop recording every captured component (hpq_clear_captures)."""
+        _abi.check(self.ctx, self.L.hpq_clear_captures(self.ctx))
+        self._captures.clear()
 
     # ---- slice loop -------------------------------------------------------------------
     def begin_step(self, init_plasma: bool = True):

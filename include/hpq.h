@@ -41,6 +41,12 @@ enum { HPQ_BC_REFLECTING = 0, HPQ_BC_PERIODIC = 1, HPQ_BC_ABSORBING = 2 };
  * FP64 multigrid). */
 enum { HPQ_PREC_FP64 = 0, HPQ_PREC_MIXED = 1 };
 
+/* Transverse Dirichlet Poisson solver (fields.poisson_solver, src/fields/Fields.cpp:179-208):
+ * DST-I by FFT in FP64, or DST-I as dense sine-matrix GEMMs on the matrix cores with each
+ * FP32 operand split into two BF16 terms (hi + lo; three products hi*hi + hi*lo + lo*hi,
+ * FP32 accumulation), about 1e-5 relative. */
+enum { HPQ_POISSON_FFT = 0, HPQ_POISSON_BF16X3 = 1 };
+
 /* Run parameters: the reference input keys this path honours
  * (src/Hipace.cpp:97-213, src/Hipace.H:208-252). Units are normalised (c=ep0=mu0=1). */
 typedef struct hpq_params {
@@ -67,6 +73,7 @@ typedef struct hpq_params {
                                             in the reference); 0: index order. Changes only
                                             the order of floating-point sums. */
     int precision;                       /* HPQ_PREC_FP64 or HPQ_PREC_MIXED */
+    int poisson_solver;                  /* HPQ_POISSON_FFT or HPQ_POISSON_BF16X3 */
 } hpq_params;
 
 /* Plasma species, SoA device arrays in PlasmaIdx order

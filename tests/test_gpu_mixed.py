@@ -1,0 +1,134 @@
+"""Mixed-precision path (precision="mixed"): FP32 gather, push increments and deposition
+with compensated FP32 window sums, FP64 field solves and slice-to-slice particle state.
+
+Each operation is compared with the FP64 oracle on the same seeded inputs. The bounds
+are error budgets, not parity: norm-wise relative error (max |a - b| / max |b|) of
+  - deposition: per-particle FP32 quantities (a few roundings, ~5e-7 each) summed in
+    windows whose compensation keeps the sums themselves exact to ~1e-14: 2e-6;
+  - explicit deposition: fields rounded to FP32 and the Sx/Sy coefficients: 1e-5;
+  - push: FP32 increments added to FP64 state: 1e-6 of the state's norm;
+  - the whole explicit slice loop on a linear wake: Ez, Ex-By, jx and rho-jz (the
+    explicit solver's jz carrier), Bx and By within 1e-4.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from helpers import PLASMA_REALS, rel_err, small_config  # noqa: E402
+from test_gpu_kernels import DEPOSIT_MODES, _pair  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed(cfg):
+    cfg.precision = "mixed"
+    return cfg
+
+
+@pytest.mark.parametrize("mode", DEPOSIT_MODES)
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+def test_mixed_deposit_current(order, mode):
+    cfg = _mixed(small_config(order=order, dtype=2 if order > 0 else 1))
+    cfg.bin_period, n = mode
+    sim, orc = _pair(cfg, n_particles=n)
+    comps = [k if sim.layout[k] >= 0 else -1 for k in ("jx", "jy", "jz", "rho", "chi", "rhomjz")]
+    names = [k for k in comps if k != -1]
+    for name in names:
+        sim.store[sim.layout[name]].zero_()
+    st = orc.store()
+    for name in names:
+        st[orc.layout[name]] = 0
+    orc.set_store(st)
+    assert sim.deposit_current(comps) == orc.deposit_plasma(comps)
+    g, o = sim.store_host(), orc.store()
+    for name in names:
+        e = rel_err(g[sim.layout[name]], o[orc.layout[name]])
+        assert e <= 2e-6, (name, e)
+        assert e > 0.0 or order == 0, (name, "FP32 path expected to round")
+    pg, po = sim.plasma_host(), orc.plasma()
+    assert np.array_equal(pg["valid"], po["valid"])
+
+
+@pytest.mark.parametrize("order,dtype", [(1, 2), (2, 2), (2, 1), (2, 0), (3, 2), (0, 1)])
+@pytest.mark.parametrize("mode", DEPOSIT_MODES)
+def test_mixed_explicit_deposition(order, dtype, mode):
+    cfg = _mixed(small_config(order=order, dtype=dtype))
+    cfg.bin_period, n = mode
+    sim, orc = _pair(cfg, n_particles=n)
+    sim.explicit_deposition()
+    orc.explicit_deposit()
+    g, o = sim.store_host(), orc.store()
+    for name in ("Sx", "Sy"):
+        e = rel_err(g[sim.layout[name]], o[orc.layout[name]])
+        assert e <= 1e-5, (name, e)
+
+
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+def test_mixed_push(order):
+    cfg = _mixed(small_config(order=order, dtype=2 if order > 0 else 1))
+    for temp in (False, True):
+        sim, orc = _pair(cfg, seed=5)
+        sim.advance_plasma_particles(temp_slice=temp)
+        orc.push_plasma(temp_slice=temp)
+        pg, po = sim.plasma_host(), orc.plasma()
+        for k in PLASMA_REALS:
+            assert rel_err(pg[k], po[k]) <= 1e-6, (k, temp, rel_err(pg[k], po[k]))
+        assert np.array_equal(pg["valid"], po["valid"])
+
+
+@pytest.mark.parametrize("n,order,peak", [(63, 2, 1.0), (64, 1, 0.5)])
+def test_mixed_slices_vs_fp64_oracle(n, order, peak):
+    from hipace2_amd.inputs import BeamSpec, make_beam
+    from hipace2_amd.simulation import Simulation
+    from oracle.oracle import OracleSim
+    cfg = _mixed(small_config(n=n, nz=48, order=order, explicit=True))
+    spec = BeamSpec(injection_type="fixed_weight", profile="gaussian", density=peak,
+                    position_mean=(0.0, 0.0, 0.0), position_std=(0.3, 0.3, 1.41),
+                    num_particles=20000, u_mean=(0.0, 0.0, 1000.0), seed=0)
+    beam = make_beam(cfg, spec)
+    sim = Simulation(cfg, beam)
+    orc = OracleSim(cfg, beam)
+    names = [k for k in ("Ez", "ExmBy", "jx", "rhomjz", "Bx", "By") if sim.layout[k] >= 0]
+    cg = {k: sim.capture(k) for k in names}
+    co = {k: orc.capture(k) for k in names}
+    sim.run_step()
+    orc.run_step()
+    torch.cuda.synchronize()
+    errs = {k: rel_err(cg[k].cpu().numpy(), co[k]) for k in names}
+    print("mixed vs FP64 oracle:", errs)
+    for k, e in errs.items():
+        assert e <= 1e-4, (k, e)
+
+
+# Split-BF16 sine-matrix DST (poisson_solver="bf16x3"): operands carry ~16 significant bits
+# (hi + lo), products are accumulated in FP32 over K <= 512 terms; four products per
+# solve. Bound: 5e-5 of the field's norm against the FP64 FFT solve of the oracle.
+BF16X3_TOL = 5e-5
+
+
+@pytest.mark.parametrize("n", [63, 64, 48, 255, 511])
+def test_poisson_bf16x3(n):
+    cfg = small_config(n=n)
+    cfg.poisson_solver = "bf16x3"
+    sim, orc = _pair(cfg, n_particles=0)
+    sim.solve_poisson_psi_exmby_eypbx_ez_bz()
+    orc.solve_poisson_psi_ez_bz()
+    g, o = sim.store_host(), orc.store()
+    errs = {name: rel_err(g[sim.layout[name]], o[orc.layout[name]]) for name in ("Psi", "Ez", "Bz", "ExmBy", "EypBx")}
+    print("bf16x3 vs FP64 FFT:", errs)
+    for name, e in errs.items():
+        assert e <= BF16X3_TOL, (name, e)
+
+
+def test_poisson_bf16x3_rectangular_and_pc():
+    cfg = small_config(n=63, explicit=False)
+    cfg.ny = 100
+    cfg.poisson_solver = "bf16x3"
+    sim, orc = _pair(cfg, n_particles=0)
+    sim.solve_poisson_bxby()
+    orc.solve_poisson_bxby()
+    g, o = sim.store_host(), orc.store()
+    for name in ("pciter_Bx", "pciter_By"):
+        e = rel_err(g[sim.layout[name]], o[orc.layout[name]])
+        assert e <= BF16X3_TOL, (name, e)
