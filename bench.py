@@ -7,8 +7,8 @@ Workload (one step = one pass of the slice loop over all ζ-slices, inputs resid
   0.01 n0, sigma_r = 0.3, sigma_z = 1.41, 4e6 macro-particles (seeded Philox), gamma = 1000.
 
 Two arithmetic paths are timed on the same workload: the FP64 plain path (the drop-in)
-and the mixed path (FP32 gather, push increments and deposition with compensated FP32
-window sums; DST-I as split-BF16 sine-matrix GEMMs on the matrix cores; FP64 multigrid
+and the mixed path (FP32 gather, push increments and deposition, the FP32 contributions
+summed in FP64 LDS words; DST-I as split-BF16 sine-matrix GEMMs on the matrix cores; FP64 multigrid
 and particle state). One more untimed step of each records
 Ez, Ex-By, jx and rho-jz on every slice, and the mixed path's norm-wise relative error
 against the FP64 path is reported.

@@ -1,8 +1,9 @@
 // Dirichlet Poisson solve with DST-I as dense sine-matrix products on the gfx950 matrix
 // cores (poisson_solver = HPQ_POISSON_BF16X3).
 //
-// The solve is F = Sy (L o (Sy X Sx)) Sx with S_jk = sin(pi (j+1)(k+1) / (n+1)) and L the
-// normalised eigenvalue matrix (the same operator as FFTPoissonSolverDirichletDirect,
+// The solve is F = Sy (L o (Sy X Sx)) Sx with S_jk = 2 sin(pi (j+1)(k+1) / (n+1)) (the RODFT00
+// normalisation) and L the normalised eigenvalue matrix (the same operator as
+// FFTPoissonSolverDirichletDirect,
 // src/fields/fft_poisson_solver/FFTPoissonSolverDirichletDirect.cpp:60-139). S is
 // symmetric, so one kernel shape serves all four products: out = (in * S)^T, i.e. every
 // row of `in` is transformed and the result is written transposed; two applications give
@@ -187,7 +188,7 @@ float bf16_value(uint16_t b) {
 
 int pad64(int n) { return (n + GT - 1) / GT * GT; }
 
-// S_jk = sin(pi (j+1)(k+1) / (n+1)) split into hi and lo BF16 planes, zero outside n x n
+// S_jk = 2 sin(pi (j+1)(k+1) / (n+1)) split into hi and lo BF16 planes, zero outside n x n
 void sine_split(int n, int np, __bf16** hi, __bf16** lo) {
     std::vector<uint16_t> h(size_t(np) * np, 0), l(size_t(np) * np, 0);
     const long N = n + 1;
@@ -197,7 +198,7 @@ void sine_split(int n, int np, __bf16** hi, __bf16** lo) {
             double sgn = 1.0;
             if (m >= N) { m -= N; sgn = -1.0; }
             if (2 * m > N) m = N - m;
-            const double s = sgn * std::sin(M_PI * double(m) / double(N));
+            const double s = 2.0 * sgn * std::sin(M_PI * double(m) / double(N));
             const float sf = float(s);
             const uint16_t bh = bf16_bits(sf);
             h[size_t(j) * np + k] = bh;

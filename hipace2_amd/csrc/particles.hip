@@ -95,8 +95,8 @@ __global__ __launch_bounds__(256) void k_push_plasma(Geom g, hpq_plasma p, const
         DsfR<R> sx[W], sy[W];
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            sx[k] = dshape<1, O, R>(cell_pos<R>(xm), k);
-            sy[k] = dshape<1, O, R>(cell_pos<R>(ym), k);
+            sx[k] = dshape<1, O, R>(xm, k);
+            sy[k] = dshape<1, O, R>(ym, k);
         }
         auto accumulate = [&](const DsfR<R>& X, const DsfR<R>& Y, R ps, R ez, R bx, R by, R bz) {
             ExmByp += (X.ds * Y.s) * ps * dxi;

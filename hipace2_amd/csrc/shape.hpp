@@ -7,11 +7,12 @@
 // wider window, type 2 (centred) half the difference of the two neighbouring values.
 // `ds` is returned negated, as the reference returns {s, -sdx, cell}.
 // Order and type are template parameters so every stencil loop unrolls; the arithmetic
-// type R is double for the FP64 path and float for the mixed-precision path.
+// type R of the polynomials is double for the FP64 path and float for the mixed-precision
+// path. The position xmid stays FP64 in both: the cell index (floor), the offset inside the
+// cell and the choice between polynomial pieces are taken in FP64, so both paths pick the
+// same cells and pieces, and only the polynomial values are rounded to R.
 #pragma once
 #include <hip/hip_runtime.h>
-
-#include <type_traits>
 
 namespace hpq {
 
@@ -23,22 +24,22 @@ using Sf = SfR<double>;
 using Dsf = DsfR<double>;
 
 template <int O, class R = double>
-__host__ __device__ __forceinline__ SfR<R> shape(R xmid, int ix) {
+__host__ __device__ __forceinline__ SfR<R> shape(double xmid, int ix) {
     if constexpr (O == 0) {
-        return {R(1.0), int(floor(xmid + R(0.5)))};
+        return {R(1.0), int(floor(xmid + 0.5))};
     } else if constexpr (O == 1) {
-        const R xf = floor(xmid);
-        const R t = xmid - xf;
+        const double xf = floor(xmid);
+        const R t = R(xmid - xf);
         return ix == 0 ? SfR<R>{R(1.0) - t, int(xf)} : SfR<R>{t, int(xf) + 1};
     } else if constexpr (O == 2) {
-        const R xf = floor(xmid + R(0.5));
-        const R t = xmid - xf;
+        const double xf = floor(xmid + 0.5);
+        const R t = R(xmid - xf);
         if (ix == 0) return {R(0.5) * (R(0.5) - t) * (R(0.5) - t), int(xf) - 1};
         if (ix == 1) return {R(0.75) - t * t, int(xf)};
         return {R(0.5) * (R(0.5) + t) * (R(0.5) + t), int(xf) + 1};
     } else {
-        const R xf = floor(xmid);
-        const R t = xmid - xf;
+        const double xf = floor(xmid);
+        const R t = R(xmid - xf);
         if (ix == 0) return {R(1.0) / R(6.0) * (R(1.0) - t) * (R(1.0) - t) * (R(1.0) - t), int(xf) - 1};
         if (ix == 1) return {R(2.0) / R(3.0) - t * t * (R(1.0) - t / R(2.0)), int(xf)};
         if (ix == 2) return {R(2.0) / R(3.0) - (R(1.0) - t) * (R(1.0) - t) * (R(1.0) - R(0.5) * (R(1.0) - t)), int(xf) + 1};
@@ -51,24 +52,24 @@ __host__ __device__ __forceinline__ DsfR<R> dmk(R s, R sdx, int cell) { return {
 
 // type 0: analytic derivative of the order-O spline, O+1 cells
 template <int O, class R = double>
-__host__ __device__ __forceinline__ DsfR<R> dshape_analytic(R xmid, int ix) {
+__host__ __device__ __forceinline__ DsfR<R> dshape_analytic(double xmid, int ix) {
     if constexpr (O == 0) {
-        return dmk<R>(R(1.0), R(0.0), int(floor(xmid + R(0.5))));
+        return dmk<R>(R(1.0), R(0.0), int(floor(xmid + 0.5)));
     } else if constexpr (O == 1) {
-        const R xf = floor(xmid);
-        const R t = xmid - xf;
+        const double xf = floor(xmid);
+        const R t = R(xmid - xf);
         return ix == 0 ? dmk<R>(1 - t, -R(1.0), int(xf)) : dmk<R>(t, R(1.0), int(xf) + 1);
     } else if constexpr (O == 2) {
-        const R xf = floor(xmid + R(0.5));
-        const R t = xmid + R(0.5) - xf;
+        const double xf = floor(xmid + 0.5);
+        const R t = R(xmid + 0.5 - xf);
         const R t2 = t * t;
         const int c = int(xf) - 1 + ix;
         if (ix == 0) return dmk<R>(R(0.5) * t2 - R(1.0) * t + R(0.5), t - R(1.0), c);
         if (ix == 1) return dmk<R>(-t2 + R(1.0) * t + R(0.5), R(1.0) - 2 * t, c);
         return dmk<R>(R(0.5) * t2, t, c);
     } else {
-        const R xf = floor(xmid);
-        const R t = xmid - xf;
+        const double xf = floor(xmid);
+        const R t = R(xmid - xf);
         const R t2 = t * t, t3 = t2 * t;
         const int c = int(xf) - 1 + ix;
         if (ix == 0) return dmk<R>(-R(1.0) / R(6.0) * t3 + R(0.5) * t2 - R(0.5) * t + R(1.0) / R(6.0), -R(0.5) * t2 + t - R(0.5), c);
@@ -80,34 +81,39 @@ __host__ __device__ __forceinline__ DsfR<R> dshape_analytic(R xmid, int ix) {
 
 // type 1: nodal (value of B_O, derivative of B_{O+1}), O+2 cells
 template <int O, class R = double>
-__host__ __device__ __forceinline__ DsfR<R> dshape_nodal(R xmid, int ix) {
+__host__ __device__ __forceinline__ DsfR<R> dshape_nodal(double xmid, int ix) {
     if constexpr (O == 0) {
-        const R xf = floor(xmid);
-        const R t = xmid - xf;
-        return ix == 0 ? dmk<R>((t < R(0.5)) ? 1 : 0, -1, int(xf)) : dmk<R>((t < R(0.5)) ? 0 : 1, 1, int(xf) + 1);
+        const double xf = floor(xmid);
+        const bool lo = xmid - xf < 0.5;
+        return ix == 0 ? dmk<R>(lo ? 1 : 0, -1, int(xf)) : dmk<R>(lo ? 0 : 1, 1, int(xf) + 1);
     } else if constexpr (O == 1) {
-        const R xf = floor(xmid + R(0.5));
-        const R t = xmid + R(0.5) - xf;
+        const double xf = floor(xmid + 0.5);
+        const double td = xmid + 0.5 - xf;
+        const bool lo = td < 0.5;
+        const R t = R(td);
         const int c = int(xf) - 1 + ix;
-        if (ix == 0) return dmk<R>((t < R(0.5)) ? R(0.5) - t : 0, t - 1, c);
-        if (ix == 1) return dmk<R>((t < R(0.5)) ? t + R(0.5) : R(1.5) - t, 1 - 2 * t, c);
-        return dmk<R>((t < R(0.5)) ? 0 : t - R(0.5), t, c);
+        if (ix == 0) return dmk<R>(lo ? R(0.5) - t : 0, t - 1, c);
+        if (ix == 1) return dmk<R>(lo ? t + R(0.5) : R(1.5) - t, 1 - 2 * t, c);
+        return dmk<R>(lo ? 0 : t - R(0.5), t, c);
     } else if constexpr (O == 2) {
-        const R xf = floor(xmid);
-        const R t = xmid - xf;
+        const double xf = floor(xmid);
+        const double td = xmid - xf;
+        const bool lo = td < 0.5;
+        const R t = R(td);
         const R t2 = t * t;
         const int c = int(xf) - 1 + ix;
-        if (ix == 0) return dmk<R>((t < R(0.5)) ? R(0.5) * t2 - R(0.5) * t + R(1.0) / R(8.0) : 0, -R(0.5) * t2 + t - R(0.5), c);
-        if (ix == 1) return dmk<R>((t < R(0.5)) ? R(3.0) / R(4.0) - t2 : R(0.5) * t2 - R(1.5) * t + R(9.0) / R(8.0), R(1.5) * t2 - 2 * t, c);
+        if (ix == 0) return dmk<R>(lo ? R(0.5) * t2 - R(0.5) * t + R(1.0) / R(8.0) : 0, -R(0.5) * t2 + t - R(0.5), c);
+        if (ix == 1) return dmk<R>(lo ? R(3.0) / R(4.0) - t2 : R(0.5) * t2 - R(1.5) * t + R(9.0) / R(8.0), R(1.5) * t2 - 2 * t, c);
         if (ix == 2)
-            return dmk<R>((t < R(0.5)) ? R(0.5) * t2 + R(0.5) * t + R(1.0) / R(8.0) : -t2 + 2 * t - R(1.0) / R(4.0), -R(1.5) * t2 + t + R(0.5), c);
-        return dmk<R>((t < R(0.5)) ? 0 : R(0.5) * t2 - R(0.5) * t + R(1.0) / R(8.0), R(0.5) * t2, c);
+            return dmk<R>(lo ? R(0.5) * t2 + R(0.5) * t + R(1.0) / R(8.0) : -t2 + 2 * t - R(1.0) / R(4.0), -R(1.5) * t2 + t + R(0.5), c);
+        return dmk<R>(lo ? 0 : R(0.5) * t2 - R(0.5) * t + R(1.0) / R(8.0), R(0.5) * t2, c);
     } else {
-        const R xf = floor(xmid + R(0.5));
-        const R t = xmid + R(0.5) - xf;
+        const double xf = floor(xmid + 0.5);
+        const double td = xmid + 0.5 - xf;
+        const R t = R(td);
         const R t2 = t * t, t3 = t2 * t;
         const int c = int(xf) - 2 + ix;
-        const bool lo = t < R(0.5);
+        const bool lo = td < 0.5;
         if (ix == 0)
             return dmk<R>(lo ? -R(1.0) / R(6.0) * t3 + R(0.25) * t2 - R(1.0) / R(8.0) * t + R(1.0) / R(48.0) : 0,
                        R(1.0) / R(6.0) * t3 - R(0.5) * t2 + R(0.5) * t - R(1.0) / R(6.0), c);
@@ -129,24 +135,24 @@ __host__ __device__ __forceinline__ DsfR<R> dshape_nodal(R xmid, int ix) {
 
 // type 2: centred derivative, O+3 cells
 template <int O, class R = double>
-__host__ __device__ __forceinline__ DsfR<R> dshape_centred(R xmid, int ix) {
+__host__ __device__ __forceinline__ DsfR<R> dshape_centred(double xmid, int ix) {
     if constexpr (O == 0) {
-        const R xf = floor(xmid + R(0.5));
+        const double xf = floor(xmid + 0.5);
         const int c = int(xf) - 1 + ix;
         if (ix == 0) return dmk<R>(0, -R(0.5), c);
         if (ix == 1) return dmk<R>(1, 0, c);
         return dmk<R>(0, R(0.5), c);
     } else if constexpr (O == 1) {
-        const R xf = floor(xmid);
-        const R t = xmid - xf;
+        const double xf = floor(xmid);
+        const R t = R(xmid - xf);
         const int c = int(xf) - 1 + ix;
         if (ix == 0) return dmk<R>(0, R(0.5) * t - R(0.5), c);
         if (ix == 1) return dmk<R>(1 - t, -R(0.5) * t, c);
         if (ix == 2) return dmk<R>(t, R(0.5) - R(0.5) * t, c);
         return dmk<R>(0, R(0.5) * t, c);
     } else if constexpr (O == 2) {
-        const R xf = floor(xmid + R(0.5));
-        const R t = xmid + R(0.5) - xf;
+        const double xf = floor(xmid + 0.5);
+        const R t = R(xmid + 0.5 - xf);
         const R t2 = t * t;
         const int c = int(xf) - 2 + ix;
         if (ix == 0) return dmk<R>(0, -R(0.25) * t2 + R(0.5) * t - R(0.25), c);
@@ -155,8 +161,8 @@ __host__ __device__ __forceinline__ DsfR<R> dshape_centred(R xmid, int ix) {
         if (ix == 3) return dmk<R>(R(0.5) * t2, -R(0.5) * t2 + R(0.5) * t + R(0.25), c);
         return dmk<R>(0, R(0.25) * t2, c);
     } else {
-        const R xf = floor(xmid);
-        const R t = xmid - xf;
+        const double xf = floor(xmid);
+        const R t = R(xmid - xf);
         const R t2 = t * t, t3 = t2 * t;
         const int c = int(xf) - 2 + ix;
         if (ix == 0) return dmk<R>(0, R(1.0) / R(12.0) * t3 - R(0.25) * t2 + R(0.25) * t - R(1.0) / R(12.0), c);
@@ -169,42 +175,10 @@ __host__ __device__ __forceinline__ DsfR<R> dshape_centred(R xmid, int ix) {
 }
 
 template <int T, int O, class R = double>
-__host__ __device__ __forceinline__ DsfR<R> dshape(R xmid, int ix) {
+__host__ __device__ __forceinline__ DsfR<R> dshape(double xmid, int ix) {
     if constexpr (T == 0) return dshape_analytic<O, R>(xmid, ix);
     else if constexpr (T == 1) return dshape_nodal<O, R>(xmid, ix);
     else return dshape_centred<O, R>(xmid, ix);
-}
-
-// A position in cell units as an integer base plus a remainder in R. FP64 keeps the whole
-// position in the remainder (base 0, the arithmetic of the plain overloads). FP32 keeps
-// the remainder in [0, 1], so the spline argument has FP32's full relative precision
-// instead of losing the bits of the cell index; since the splines shift with their
-// argument, cell = base + cell of the remainder.
-template <class R>
-struct CellPos { R r; int base; };
-
-template <class R>
-__host__ __device__ __forceinline__ CellPos<R> cell_pos(double xm) {
-    if constexpr (std::is_same_v<R, double>) {
-        return {xm, 0};
-    } else {
-        const double b = floor(xm);
-        return {R(xm - b), int(b)};
-    }
-}
-
-template <int O, class R>
-__host__ __device__ __forceinline__ SfR<R> shape(CellPos<R> x, int ix) {
-    SfR<R> s = shape<O, R>(x.r, ix);
-    s.cell += x.base;
-    return s;
-}
-
-template <int T, int O, class R>
-__host__ __device__ __forceinline__ DsfR<R> dshape(CellPos<R> x, int ix) {
-    DsfR<R> s = dshape<T, O, R>(x.r, ix);
-    s.cell += x.base;
-    return s;
 }
 
 // number of cells of the derivative stencil

@@ -69,7 +69,7 @@ class SliceConfig:
     # bin_period slices (0: index order); only the order of floating-point sums depends on it
     bin_period: int = 4
     # "fp64": the drop-in path; "mixed": FP32 gather, push increments and deposition with
-    # compensated LDS sums (FP64 store, particle state and multigrid)
+    # FP32 contributions summed in FP64 LDS words (FP64 store, particle state and multigrid)
     precision: str = "fp64"
     # transverse Poisson solves: "fft" (DST-I by FFT, FP64) or "bf16x3" (DST-I as sine-matrix
     # GEMMs on the matrix cores, FP32 operands split into two BF16 terms each)

@@ -37,7 +37,7 @@ enum {
 enum { HPQ_BC_REFLECTING = 0, HPQ_BC_PERIODIC = 1, HPQ_BC_ABSORBING = 2 };
 
 /* Arithmetic of the particle kernels: FP64 throughout (the drop-in path), or FP32 gather,
- * push increments and deposition (compensated LDS sums, FP64 slice store and particle state,
+ * push increments and deposition (FP32 contributions summed in FP64, FP64 slice store and particle state,
  * FP64 multigrid). */
 enum { HPQ_PREC_FP64 = 0, HPQ_PREC_MIXED = 1 };
 

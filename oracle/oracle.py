@@ -253,6 +253,56 @@ def poisson(cfg, staging: np.ndarray) -> np.ndarray:
     return lhs
 
 
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """FP32 -> BF16 with round-to-nearest-even, returned as FP32 values (finite inputs)."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def split_bf16(v: np.ndarray):
+    """v (rounded to FP32) as hi + lo, hi = bf16(v), lo = bf16(v - hi)."""
+    v = np.asarray(v, dtype=np.float32)
+    hi = bf16_round(v)
+    return hi, bf16_round(v - hi)
+
+
+def _gemm_t_bf16x3(a: np.ndarray, s_hi: np.ndarray, s_lo: np.ndarray) -> np.ndarray:
+    """(a S)^T with a split into BF16 hi/lo: hi*hi + hi*lo + lo*hi, sums exact, rounded to FP32."""
+    ah, al = split_bf16(a)
+    ah, al = ah.astype(np.float64), al.astype(np.float64)
+    d = ah @ s_hi + ah @ s_lo + al @ s_hi
+    return d.astype(np.float32).T
+
+
+def poisson_bf16x3(cfg, staging: np.ndarray) -> np.ndarray:
+    """The split-BF16 sine-matrix Dirichlet solve (poisson_solver="bf16x3") restated in numpy:
+    F = Sy (L o (Sy X Sx)) Sx with S_jk = 2 sin(pi (j+1)(k+1)/(n+1)) (RODFT00,
+    FFTPoissonSolverDirichletDirect.cpp:87-139) and L the eigenvalue matrix with the
+    normalisation 1/(4 (nx+1)(ny+1)) (:60-82), evaluated as four products
+    out = (in S)^T whose operands are split into two BF16 terms each. Products are summed
+    exactly here and rounded to FP32 per product; returns the valid ny x nx solution."""
+    nx, ny = cfg.nx, cfg.ny
+    X = np.asarray(staging, dtype=np.float64).reshape(ny, nx)
+    def sine(n):
+        k = np.arange(1, n + 1)
+        return 2.0 * np.sin(np.pi * np.outer(k, k) / (n + 1))
+    sx, sy = sine(nx), sine(ny)
+    # S is split from its FP64 value: hi = bf16(S), lo = bf16(S - hi)
+    sxh = bf16_round(sx).astype(np.float64)
+    sxl = bf16_round(sx - sxh).astype(np.float64)
+    syh = bf16_round(sy).astype(np.float64)
+    syl = bf16_round(sy - syh).astype(np.float64)
+    dx, dy = cfg.dx, cfg.dy
+    fx = np.sin(np.arange(1, nx + 1) * np.pi / (2.0 * (nx + 1))) ** 2
+    fy = np.sin(np.arange(1, ny + 1) * np.pi / (2.0 * (ny + 1))) ** 2
+    ev = (0.5 / (2 * ((nx + 1) * (ny + 1)))) / (-4.0 * (fx[:, None] / dx ** 2 + fy[None, :] / dy ** 2))
+    p1 = _gemm_t_bf16x3(X, sxh, sxl)
+    p2 = (_gemm_t_bf16x3(p1, syh, syl).astype(np.float64) * ev.T).astype(np.float32)
+    p3 = _gemm_t_bf16x3(p2, sxh, sxl)
+    return _gemm_t_bf16x3(p3, syh, syl).astype(np.float64)
+
+
 def shape(order, xmid, ix):
     out = np.zeros(2)
     lib().orc_shape(order, xmid, ix, _dp(out))

@@ -53,6 +53,32 @@ def test_poisson_eigenmode():
     assert rel_err(lhs[2:-2, 2:-2], mode / lam) < 1e-13
 
 
+@pytest.mark.parametrize("nx,ny", [(63, 63), (64, 64), (48, 48), (63, 101)])
+def test_poisson_bf16x3_error_budget(nx, ny):
+    """The split-BF16 restatement against the FP64 solve: within 5e-5 of the norm, and
+    not exact (the BF16 rounding is really there)."""
+    cfg = small_config(n=nx)
+    cfg.ny = ny
+    S = np.random.default_rng(5).normal(size=(ny, nx))
+    want = orc.poisson(cfg, S)[2:-2, 2:-2]
+    got = orc.poisson_bf16x3(cfg, S)
+    e = rel_err(got, want)
+    assert 1e-7 < e < 5e-5, e
+    # the residual of the 5-point Laplacian stays at the same level
+    assert rel_err(_lap5(got, cfg.dx, cfg.dy), S) < 1e-3
+
+
+def test_bf16_round_to_nearest_even():
+    # 1 + 2^-8 is halfway between two BF16 values: ties go to the even mantissa (1.0);
+    # 1 + 3 * 2^-8 goes up to 1 + 2^-6
+    v = np.array([1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, -2.5, 3.0e-3], dtype=np.float32)
+    r = orc.bf16_round(v)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1 + 2 ** -6 and r[3] == -2.5
+    assert abs(r[4] - 3.0e-3) <= 2 ** -8 * 3.0e-3  # half a unit in the 8th significant bit
+    hi, lo = orc.split_bf16(np.float32(math.pi))
+    assert abs(float(hi) + float(lo) - float(np.float32(math.pi))) < 2 ** -16 * math.pi
+
+
 # ----------------------------------------------------------------------------- shape factors
 
 def bspline(o, u):
