@@ -33,6 +33,10 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOAD = "511x511x512 blowout-geometry slice loop, 2x2 ppc, shape 2, explicit MG"
 ERR_FIELDS = ("Ez", "ExmBy", "jx", "rhomjz")
+COPY_BYTES = 256 << 20
+# HBM bytes per launch of the dominant kernels, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs
+# scaled by the same counters on a 256 MiB device copy (see the summary beside it)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_511x511x512.json")
 DTYPE = {"fp64": "f64",
          "mixed": "f32 (gather, push increments, deposition) + split-bf16 x3 / f32 accumulate (DST) + f64 "
                   "(multigrid, particle state)"}
@@ -121,6 +125,17 @@ def run_gpu(args, precision, capture):
                 caps=caps)
 
 
+def measured_traffic(kind, precision, args):
+    """HBM bytes per launch of kernel family `kind` on this workload, from the committed
+    counter summary, or None when it does not cover this kernel and workload."""
+    if (args.n, args.nz, args.ppc) != (511, 512, 2) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    e = t.get("kernels", {}).get(f"{kind}/{precision}")
+    return None if e is None else int(e["hbm_bytes_per_launch"])
+
+
 def errors_vs_fp64(mixed, fp64):
     """max |mixed - fp64| / max |fp64| over every slice of one step, per field."""
     out = {}
@@ -190,12 +205,21 @@ def main():
                     help="arithmetic path(s) to time; 'both' also reports the mixed path's error")
     ap.add_argument("--mixed-poisson", choices=["bf16x3", "fft"], default="bf16x3",
                     help="transverse Poisson solver of the mixed path")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 256 MiB device to device, the reference dispatch for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the slice loop is not sharded across GPUs "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     runs = {p: run_gpu(args, p, capture=len(precs) == 2) for p in precs}
     head = precs[-1]
@@ -219,8 +243,8 @@ def main():
     avg_s = rows[dom]["avg_us"] * 1e-6
     achieved = bpl / avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "bytes_per_launch": bpl, "avg_launch_us": round(rows[dom]["avg_us"], 2)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, head, args),
+                "kernel": dom, "bytes_per_launch": bpl, "avg_launch_us": round(rows[dom]["avg_us"], 2)}
     if breakdown:
         print(f"kernel times over {args.warmup} warm-up step(s): " + json.dumps(table(breakdown)), file=sys.stderr)
     print("dominant kernel over the timed region: " + json.dumps(rows), file=sys.stderr)

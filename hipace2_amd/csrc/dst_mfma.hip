@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "internal.hpp"
@@ -38,17 +39,16 @@ template <class T>
 struct Ptr3 { T* p[3]; };
 
 struct GemmArgs {
-    long lda;          // row pitch of `in`
-    int m_valid;       // rows of `in` that hold data (the rest read as zero)
-    int k_valid;       // columns of `in` that hold data
+    long lda;          // row pitch of `in` (a multiple of 4)
     int kpad;          // K extent (a multiple of GK)
     const __bf16* s_hi;  // S split, [j][k] with pitch kpad (S is symmetric)
     const __bf16* s_lo;
     long ldo;          // row pitch of `out`
-    int out_rows;      // rows of `out` written (columns j of the product)
-    int out_cols;      // columns of `out` written (rows i of `in`)
-    const double* eig; // optional scale: out(r, c) *= eig[c * eig_ld + r]
-    long eig_ld;
+    int out_rows;      // FP64 output: rows of `out` written (columns j of the product)
+    int out_cols;      // FP64 output: columns of `out` written (rows i of `in`)
+    const double* eig; // optional scale: out(r, c) *= eig[c * eig_ld + r] for r < eig_rows,
+    long eig_ld;       // c < eig_cols, and 0 outside (keeps the padding of `out` zero)
+    int eig_rows, eig_cols;
 };
 
 __device__ __forceinline__ void split(float v, __bf16& hi, __bf16& lo) {
@@ -56,46 +56,50 @@ __device__ __forceinline__ void split(float v, __bf16& hi, __bf16& lo) {
     lo = __bf16(v - float(hi));
 }
 
-template <class TIN, class TOUT, bool SCALE>
-__global__ __launch_bounds__(GEMM_T) void k_dst_gemm(Ptr3<const TIN> in, Ptr3<TOUT> out, GemmArgs a) {
-    __shared__ __attribute__((aligned(16))) __bf16 ah[GT * PK], al[GT * PK], bh[GT * PK], bl[GT * PK];
+// One workgroup computes a 64 x 64 tile of (in * S) over the whole K extent, K chunks of 64
+// double-buffered in LDS (one barrier per chunk). Inputs are FP32, padded to multiples of
+// 64 with zeros outside the data, rows 16-byte aligned: vector loads without bounds.
+template <class TOUT, bool SCALE>
+__global__ __launch_bounds__(GEMM_T) void k_dst_gemm(Ptr3<const float> in, Ptr3<TOUT> out, GemmArgs a) {
+    constexpr int STAGE = 4 * GT * PK;   // ah, al, bh, bl of one K chunk
+    __shared__ __attribute__((aligned(16))) __bf16 lds[2 * STAGE];
     const int f = blockIdx.z;
-    const TIN* A = in.p[f];
+    const float* A = in.p[f];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w >> 1, wn = w & 1;
     const int m0 = blockIdx.y * GT;   // rows of `in` (output columns)
     const int n0 = blockIdx.x * GT;   // columns of S (output rows)
 
-    // staging: `in` as 4 consecutive k per thread over 4 row groups; S as 8 consecutive k
+    // staging: `in` as 4 consecutive k per thread over 4 row groups; S as 8 consecutive k.
+    // Chunk ck+1 is loaded into registers while chunk ck is multiplied, then split into the
+    // other LDS buffer.
     const int ar = t >> 4, ak = (t & 15) * 4;
     const int br = t >> 3, bk = (t & 7) * 8;
-    float av[4][4];
-    bf16x8 bvh[2], bvl[2];
-    auto load = [&](int kc) {
+    struct Regs {
+        f32x4 av[4];
+        bf16x8 bvh[2], bvl[2];
+    };
+    Regs R;
+    auto load = [&](int kc, Regs& R) {
 #pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const int r = m0 + ar + 16 * it;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int k = kc + ak + e;
-                av[it][e] = (r < a.m_valid && k < a.k_valid) ? float(A[long(r) * a.lda + k]) : 0.0f;
-            }
-        }
+        for (int it = 0; it < 4; ++it)
+            R.av[it] = *reinterpret_cast<const f32x4*>(A + long(m0 + ar + 16 * it) * a.lda + kc + ak);
 #pragma unroll
         for (int it = 0; it < 2; ++it) {
             const long o = long(n0 + br + 32 * it) * a.kpad + kc + bk;
-            bvh[it] = *reinterpret_cast<const bf16x8*>(a.s_hi + o);
-            bvl[it] = *reinterpret_cast<const bf16x8*>(a.s_lo + o);
+            R.bvh[it] = *reinterpret_cast<const bf16x8*>(a.s_hi + o);
+            R.bvl[it] = *reinterpret_cast<const bf16x8*>(a.s_lo + o);
         }
     };
-    auto stage = [&]() {
+    auto stage = [&](const Regs& R, __bf16* st) {
+        __bf16 *ah = st, *al = st + GT * PK, *bh = st + 2 * GT * PK, *bl = st + 3 * GT * PK;
 #pragma unroll
         for (int it = 0; it < 4; ++it) {
             bf16x4 h, l;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 __bf16 x, y;
-                split(av[it][e], x, y);
+                split(R.av[it][e], x, y);
                 h[e] = x;
                 l[e] = y;
             }
@@ -106,8 +110,8 @@ __global__ __launch_bounds__(GEMM_T) void k_dst_gemm(Ptr3<const TIN> in, Ptr3<TO
 #pragma unroll
         for (int it = 0; it < 2; ++it) {
             const int o = (br + 32 * it) * PK + bk;
-            *reinterpret_cast<bf16x8*>(bh + o) = bvh[it];
-            *reinterpret_cast<bf16x8*>(bl + o) = bvl[it];
+            *reinterpret_cast<bf16x8*>(bh + o) = R.bvh[it];
+            *reinterpret_cast<bf16x8*>(bl + o) = R.bvl[it];
         }
     };
 
@@ -119,12 +123,8 @@ __global__ __launch_bounds__(GEMM_T) void k_dst_gemm(Ptr3<const TIN> in, Ptr3<TO
 
     // fragment of v_mfma_f32_16x16x32_bf16: lane l holds row l%16, k = 8*(l/16) .. +7
     const int fr = lane & 15, fk = 8 * (lane >> 4);
-    load(0);
-    for (int kc = 0; kc < a.kpad; kc += GK) {
-        __syncthreads();
-        stage();
-        __syncthreads();
-        if (kc + GK < a.kpad) load(kc + GK);
+    auto multiply = [&](const __bf16* st) {
+        const __bf16 *ah = st, *al = st + GT * PK, *bh = st + 2 * GT * PK, *bl = st + 3 * GT * PK;
 #pragma unroll
         for (int kk = 0; kk < GK; kk += 32) {
             bf16x8 xh[2], xl[2], yh[2], yl[2];
@@ -143,12 +143,30 @@ __global__ __launch_bounds__(GEMM_T) void k_dst_gemm(Ptr3<const TIN> in, Ptr3<TO
 #pragma unroll
             for (int x = 0; x < 2; ++x)
 #pragma unroll
-                for (int y = 0; y < 2; ++y) {
+                for (int y = 0; y < 2; ++y)
                     acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[x], yh[y], acc[x][y], 0, 0, 0);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
                     acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[x], yl[y], acc[x][y], 0, 0, 0);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
                     acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[x], yh[y], acc[x][y], 0, 0, 0);
-                }
         }
+    };
+    // the other LDS buffer was last read in chunk ck-1, before the barrier that ended it
+    const int nchunk = a.kpad / GK;
+    load(0, R);
+    stage(R, lds);
+    __syncthreads();
+    for (int ck = 0; ck < nchunk; ++ck) {
+        if (ck + 1 < nchunk) load((ck + 1) * GK, R);
+        multiply(lds + (ck & 1) * STAGE);
+        if (ck + 1 < nchunk) stage(R, lds + ((ck + 1) & 1) * STAGE);
+        __syncthreads();
     }
 
     // D fragment: lane l holds rows 4*(l/16) + r (r = 0..3) of column l%16; stored transposed,
@@ -160,14 +178,23 @@ __global__ __launch_bounds__(GEMM_T) void k_dst_gemm(Ptr3<const TIN> in, Ptr3<TO
         for (int y = 0; y < 2; ++y) {
             const int orow = n0 + wn * 32 + y * 16 + fr;
             const int ocol = m0 + wm * 32 + x * 16 + 4 * (lane >> 4);
-            if (orow >= a.out_rows) continue;
+            f32x4 v = acc[x][y];
+            if constexpr (SCALE) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int c = ocol + r;
-                if (c >= a.out_cols) continue;
-                float v = acc[x][y][r];
-                if constexpr (SCALE) v = float(double(v) * a.eig[long(c) * a.eig_ld + orow]);
-                O[long(orow) * a.ldo + c] = TOUT(v);
+                for (int r = 0; r < 4; ++r) {
+                    const int c = ocol + r;
+                    v[r] = (orow < a.eig_rows && c < a.eig_cols) ? float(double(v[r]) * a.eig[long(c) * a.eig_ld + orow])
+                                                                 : 0.0f;
+                }
+            }
+            if constexpr (std::is_same_v<TOUT, float>) {
+                // padded FP32 intermediate: the whole tile is inside the buffer
+                *reinterpret_cast<f32x4*>(O + long(orow) * a.ldo + ocol) = v;
+            } else {
+                if (orow >= a.out_rows) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (ocol + r < a.out_cols) O[long(orow) * a.ldo + ocol + r] = TOUT(v[r]);
             }
         }
 }
@@ -210,10 +237,10 @@ void sine_split(int n, int np, __bf16** hi, __bf16** lo) {
     HPQ_HIP(hipMemcpy(*lo, l.data(), sizeof(uint16_t) * l.size(), hipMemcpyHostToDevice));
 }
 
-template <class TIN, class TOUT, bool SCALE>
-void gemm(hpq_ctx* c, int nf, Ptr3<const TIN> in, Ptr3<TOUT> out, const GemmArgs& a, int mpad) {
+template <class TOUT, bool SCALE>
+void gemm(hpq_ctx* c, int nf, Ptr3<const float> in, Ptr3<TOUT> out, const GemmArgs& a, int mpad) {
     const dim3 grid(a.kpad / GT, mpad / GT, nf);
-    hipLaunchKernelGGL((k_dst_gemm<TIN, TOUT, SCALE>), grid, dim3(GEMM_T), 0, c->stream, in, out, a);
+    hipLaunchKernelGGL((k_dst_gemm<TOUT, SCALE>), grid, dim3(GEMM_T), 0, c->stream, in, out, a);
     HPQ_HIP(hipGetLastError());
 }
 
@@ -228,18 +255,22 @@ void dst_mfma_init(hpq_ctx* c) {
     const size_t fs = size_t(S.npx) * S.npy;
     HPQ_HIP(hipMalloc(&S.gA, sizeof(float) * 3 * fs));
     HPQ_HIP(hipMalloc(&S.gB, sizeof(float) * 3 * fs));
+    HPQ_HIP(hipMalloc(&S.gS, sizeof(float) * 3 * fs));
+    // the padding of the staged right-hand sides is never written: zero once
+    HPQ_HIP(hipMemset(S.gS, 0, sizeof(float) * 3 * fs));
 }
 
 void dst_mfma_free(hpq_ctx* c) {
     Poisson& S = c->ps;
-    for (void* p : {(void*)S.sx_hi, (void*)S.sx_lo, (void*)S.sy_hi, (void*)S.sy_lo, (void*)S.gA, (void*)S.gB})
+    for (void* p : {(void*)S.sx_hi, (void*)S.sx_lo, (void*)S.sy_hi, (void*)S.sy_lo, (void*)S.gA, (void*)S.gB,
+                    (void*)S.gS})
         if (p) (void)hipFree(p);
     S.sx_hi = S.sx_lo = S.sy_hi = S.sy_lo = nullptr;
-    S.gA = S.gB = nullptr;
+    S.gA = S.gB = S.gS = nullptr;
 }
 
-// The staged right-hand sides ps.bufA ([f][j][i], pitch nx) through four products into the
-// valid region of lhs_planes[f].
+// The staged right-hand sides ps.gS ([f][npy][npx], FP32, zero padding) through four
+// products into the valid region of lhs_planes[f].
 void poisson_solve_mfma(hpq_ctx* c, int nf, double* const* lhs_planes) {
     Poisson& S = c->ps;
     const int nx = S.nx, ny = S.ny, npx = S.npx, npy = S.npy;
@@ -250,35 +281,33 @@ void poisson_solve_mfma(hpq_ctx* c, int nf, double* const* lhs_planes) {
         for (int f = 0; f < 3; ++f) p.p[f] = b + f * fs;
         return p;
     };
-    auto cfbuf = [&](float* b) {
+    auto cfbuf = [&](const float* b) {
         Ptr3<const float> p;
         for (int f = 0; f < 3; ++f) p.p[f] = b + f * fs;
         return p;
     };
     // 1: rows j of X times Sx -> P1 [i][j]
     {
-        Ptr3<const double> in;
-        for (int f = 0; f < 3; ++f) in.p[f] = S.bufA + long(f) * nx * ny;
-        GemmArgs a{nx, ny, nx, npx, S.sx_hi, S.sx_lo, npy, npx, npy, nullptr, 0};
-        gemm<double, float, false>(c, nf, in, fbuf(S.gA), a, npy);
+        GemmArgs a{npx, npx, S.sx_hi, S.sx_lo, npy, 0, 0, nullptr, 0, 0, 0};
+        gemm<float, false>(c, nf, cfbuf(S.gS), fbuf(S.gA), a, npy);
     }
-    // 2: rows i of P1 times Sy -> P2 [j][i] = Sy X Sx, scaled by the eigenvalues (ev[i][j]);
-    // only the valid n x n block is written, so product 3 reads P2 with bounds
+    // 2: rows i of P1 times Sy -> P2 [j][i] = Sy X Sx, scaled by the eigenvalues (ev[i][j]),
+    // zero in the padding
     {
-        GemmArgs a{npy, npx, npy, npy, S.sy_hi, S.sy_lo, npx, ny, nx, S.eigT, ny};
-        gemm<float, float, true>(c, nf, cfbuf(S.gA), fbuf(S.gB), a, npx);
+        GemmArgs a{npy, npy, S.sy_hi, S.sy_lo, npx, 0, 0, S.eigT, ny, ny, nx};
+        gemm<float, true>(c, nf, cfbuf(S.gA), fbuf(S.gB), a, npx);
     }
     // 3: rows j of P2 times Sx -> P3 [i][j]
     {
-        GemmArgs a{npx, ny, nx, npx, S.sx_hi, S.sx_lo, npy, npx, npy, nullptr, 0};
-        gemm<float, float, false>(c, nf, cfbuf(S.gB), fbuf(S.gA), a, npy);
+        GemmArgs a{npx, npx, S.sx_hi, S.sx_lo, npy, 0, 0, nullptr, 0, 0, 0};
+        gemm<float, false>(c, nf, cfbuf(S.gB), fbuf(S.gA), a, npy);
     }
     // 4: rows i of P3 times Sy -> F [j][i] into the guarded planes
     {
         Ptr3<double> out;
         for (int f = 0; f < 3; ++f) out.p[f] = lhs_planes[f < nf ? f : 0] + c->g.idx(0, 0);
-        GemmArgs a{npy, npx, npy, npy, S.sy_hi, S.sy_lo, c->g.ldx, ny, nx, nullptr, 0};
-        gemm<float, double, false>(c, nf, cfbuf(S.gA), out, a, npx);
+        GemmArgs a{npy, npy, S.sy_hi, S.sy_lo, c->g.ldx, ny, nx, nullptr, 0, 0, 0};
+        gemm<double, false>(c, nf, cfbuf(S.gA), out, a, npx);
     }
 }
 

@@ -476,8 +476,7 @@ int hpq_poisson_dirichlet(hpq_ctx* c, const double* staging, double* lhs) {
     if (!c) return HPQ_E_INVALID;
     return guarded(c, [&] {
         if (!staging || !lhs) throw HpqError(HPQ_E_INVALID, "null pointer");
-        HPQ_HIP(hipMemcpyAsync(c->ps.bufA, staging, sizeof(double) * c->g.nx * c->g.ny, hipMemcpyDeviceToDevice,
-                               c->stream));
+        stage_rhs(c, staging);
         double* planes[3] = {lhs, lhs, lhs};
         poisson_solve_staged(c, 1, planes);
     });

@@ -77,6 +77,7 @@ struct Poisson {
     int npx = 0, npy = 0;
     __bf16 *sx_hi = nullptr, *sx_lo = nullptr, *sy_hi = nullptr, *sy_lo = nullptr;
     float *gA = nullptr, *gB = nullptr;
+    float* gS = nullptr;      // staged right-hand sides [f][npy][npx], zero padding
 };
 
 // One multigrid level; arrays cover the level box plus one ghost ring.
@@ -194,6 +195,7 @@ void poisson_solve_staged(hpq_ctx* c, int nfields, double* const* lhs_planes);
 void dst_mfma_init(hpq_ctx* c);
 void dst_mfma_free(hpq_ctx* c);
 void poisson_solve_mfma(hpq_ctx* c, int nfields, double* const* lhs_planes);
+void stage_rhs(hpq_ctx* c, const double* src);
 void build_rhs_psi_ez_bz(hpq_ctx* c, const double* store);
 void build_rhs_bxby(hpq_ctx* c, const double* store);
 void exmby_eypbx(hpq_ctx* c, double* store);
@@ -222,6 +224,18 @@ const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p);
 void bins_free(hpq_ctx* c);
 
 inline dim3 grid1d(long n, int block) { return dim3(unsigned((n + block - 1) / block)); }
+
+// The 8 XCDs of an MI355X receive the workgroups of a 1-D grid round robin (workgroup b on
+// XCD b % 8), and each XCD has its own L2. Maps the physical workgroup index onto a logical
+// one such that every XCD works on one contiguous range of logical workgroups (XCD x < r
+// takes q + 1 of them, the others q, with n = 8 q + r), so neighbouring particles and the
+// field rows they read stay in one L2. A bijection of [0, n).
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    const int q = n / NUM_XCD, r = n % NUM_XCD;
+    const int x = b % NUM_XCD, i = b / NUM_XCD;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
 
 // calls f(std::integral_constant<int, O>) for the run-time shape order O
 template <class F>

@@ -14,6 +14,8 @@
 // Other n: direct sum with an exactly reduced sine table, one row per workgroup.
 #include <cmath>
 
+#include <type_traits>
+
 #include "internal.hpp"
 
 namespace hpq {
@@ -327,56 +329,93 @@ void poisson_solve_staged(hpq_ctx* c, int nf, double* const* lhs_planes) {
     dst_rows(c, S.px, rows(S.bufB, nx, ny), out, nf * ny, nullptr);
 }
 
+// where the staged right-hand sides go: FP64 [f][j][i] for the FFT path, FP32 padded rows
+// for the sine-matrix path
+template <class F>
+void with_staging(hpq_ctx* c, F&& f) {
+    const Poisson& S = c->ps;
+    if (c->p.poisson_solver == HPQ_POISSON_BF16X3)
+        f(S.gS, long(S.npx), long(S.npx) * S.npy);
+    else
+        f(S.bufA, long(S.nx), long(S.nx) * S.ny);
+}
+
+// copies a caller's nx x ny FP64 right-hand side into the staging of field 0
+template <class T>
+__global__ void k_stage_rhs(Geom g, const double* __restrict__ src, T* __restrict__ st, long ld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= g.nx) return;
+    st[long(j) * ld + i] = T(src[long(j) * g.nx + i]);
+}
+
+void stage_rhs(hpq_ctx* c, const double* src) {
+    const Geom& g = c->g;
+    with_staging(c, [&](auto* st, long ld, long) {
+        using T = std::remove_pointer_t<decltype(st)>;
+        hipLaunchKernelGGL(k_stage_rhs<T>, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g, src, st, ld);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
 // Right-hand sides of Fields::SolvePoissonPsiExmByEypBxEzBz (src/fields/Fields.cpp:886-917):
 // Psi: -rhomjz/ep0; Ez: (d_x jx + d_y jy)/(ep0 c); Bz: mu0 (d_y jx - d_x jy), centred
 // differences (derivative<>, :223-277) that read the guard cells of jx, jy.
+// st: [f][j][i] with row pitch ld and field stride fs, FP64 (FFT path) or FP32 (bf16x3 path)
+template <class T>
 __global__ void k_rhs_psi_ez_bz(Geom g, const double* __restrict__ rhomjz, const double* __restrict__ jx,
-                                const double* __restrict__ jy, double* __restrict__ st) {
+                                const double* __restrict__ jy, T* __restrict__ st, long ld, long fs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= g.nx) return;
     const double ep0 = 1.0, mu0 = 1.0, cl = 1.0;
     const double dxih = 0.5 * g.dxi, dyih = 0.5 * g.dyi;
-    const long fs = long(g.nx) * g.ny;
-    const long o = long(j) * g.nx + i;
-    st[o] = (-1.0 / ep0) * rhomjz[g.idx(i, j)];
+    const long o = long(j) * ld + i;
+    st[o] = T((-1.0 / ep0) * rhomjz[g.idx(i, j)]);
     const double fe = 1.0 / (ep0 * cl);
-    st[fs + o] = fe * ((jx[g.idx(i + 1, j)] - jx[g.idx(i - 1, j)]) * dxih) +
-                 fe * ((jy[g.idx(i, j + 1)] - jy[g.idx(i, j - 1)]) * dyih);
-    st[2 * fs + o] = mu0 * ((jx[g.idx(i, j + 1)] - jx[g.idx(i, j - 1)]) * dyih) +
-                     (-mu0) * ((jy[g.idx(i + 1, j)] - jy[g.idx(i - 1, j)]) * dxih);
+    st[fs + o] = T(fe * ((jx[g.idx(i + 1, j)] - jx[g.idx(i - 1, j)]) * dxih) +
+                   fe * ((jy[g.idx(i, j + 1)] - jy[g.idx(i, j - 1)]) * dyih));
+    st[2 * fs + o] = T(mu0 * ((jx[g.idx(i, j + 1)] - jx[g.idx(i, j - 1)]) * dyih) +
+                       (-mu0) * ((jy[g.idx(i + 1, j)] - jy[g.idx(i - 1, j)]) * dxih));
 }
 
 void build_rhs_psi_ez_bz(hpq_ctx* c, const double* store) {
     const Geom& g = c->g;
     const Layout& L = c->L;
-    hipLaunchKernelGGL(k_rhs_psi_ez_bz, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g,
-                       store + L.rhomjz * g.plane, store + L.jx * g.plane, store + L.jy * g.plane, c->ps.bufA);
+    with_staging(c, [&](auto* st, long ld, long fs) {
+        using T = std::remove_pointer_t<decltype(st)>;
+        hipLaunchKernelGGL(k_rhs_psi_ez_bz<T>, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g,
+                           store + L.rhomjz * g.plane, store + L.jx * g.plane, store + L.jy * g.plane, st, ld, fs);
+    });
     HPQ_HIP(hipGetLastError());
 }
 
 // Right-hand sides of Fields::SolvePoissonBxBy (src/fields/Fields.cpp:1044-1068).
+template <class T>
 __global__ void k_rhs_bxby(Geom g, const double* __restrict__ jz, const double* __restrict__ pjx,
                            const double* __restrict__ pjy, const double* __restrict__ njx,
-                           const double* __restrict__ njy, double* __restrict__ st) {
+                           const double* __restrict__ njy, T* __restrict__ st, long ld, long fs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= g.nx) return;
     const double mu0 = 1.0;
     const double dxih = 0.5 * g.dxi, dyih = 0.5 * g.dyi, dzih = 0.5 * g.dzi;
-    const long fs = long(g.nx) * g.ny;
-    const long o = long(j) * g.nx + i;
+    const long o = long(j) * ld + i;
     const long k = g.idx(i, j);
-    st[o] = -mu0 * ((jz[g.idx(i, j + 1)] - jz[g.idx(i, j - 1)]) * dyih) + mu0 * ((pjy[k] - njy[k]) * dzih);
-    st[fs + o] = mu0 * ((jz[g.idx(i + 1, j)] - jz[g.idx(i - 1, j)]) * dxih) + (-mu0) * ((pjx[k] - njx[k]) * dzih);
+    st[o] = T(-mu0 * ((jz[g.idx(i, j + 1)] - jz[g.idx(i, j - 1)]) * dyih) + mu0 * ((pjy[k] - njy[k]) * dzih));
+    st[fs + o] =
+        T(mu0 * ((jz[g.idx(i + 1, j)] - jz[g.idx(i - 1, j)]) * dxih) + (-mu0) * ((pjx[k] - njx[k]) * dzih));
 }
 
 void build_rhs_bxby(hpq_ctx* c, const double* store) {
     const Geom& g = c->g;
     const Layout& L = c->L;
-    hipLaunchKernelGGL(k_rhs_bxby, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g, store + L.jz * g.plane,
-                       store + L.prev_jx * g.plane, store + L.prev_jy * g.plane, store + L.next_jx * g.plane,
-                       store + L.next_jy * g.plane, c->ps.bufA);
+    with_staging(c, [&](auto* st, long ld, long fs) {
+        using T = std::remove_pointer_t<decltype(st)>;
+        hipLaunchKernelGGL(k_rhs_bxby<T>, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g,
+                           store + L.jz * g.plane, store + L.prev_jx * g.plane, store + L.prev_jy * g.plane,
+                           store + L.next_jx * g.plane, store + L.next_jy * g.plane, st, ld, fs);
+    });
     HPQ_HIP(hipGetLastError());
 }
 
