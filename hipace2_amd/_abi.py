@@ -26,7 +26,8 @@ EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq
            "hpq_solve_poisson_bxby", "hpq_initialize_sxsy_with_beam", "hpq_mg_solve1", "hpq_plasma_count",
            "hpq_init_plasma", "hpq_begin_step", "hpq_solve_one_slice", "hpq_run_slices", "hpq_set_checksums",
            "hpq_add_capture", "hpq_clear_captures", "hpq_stats", "hpq_trace_enable", "hpq_trace_read",
-           "hpq_trace_reset", "hpq_bin_tiles"]
+           "hpq_trace_reset", "hpq_bin_tiles", "hpq_deposit_current_vjp", "hpq_explicit_deposition_vjp",
+           "hpq_advance_plasma_particles_vjp"]
 
 TRACE = {"push": 0, "deposit": 1, "explicit_deposit": 2, "beam_deposit": 3, "mg_smooth": 4, "mg_coarse": 5,
          "dst": 6}
@@ -51,6 +52,13 @@ POISSON_SOLVER = {"fft": 0, "bf16x3": 1}
 class Plasma(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("x", "y", "w", "ux", "uy", "psi", "x_prev", "y_prev", "ux_half_step",
                                            "uy_half_step", "psi_half_step", "valid")] + [("n", C.c_int64)]
+
+
+ADJ_NAMES = ["x", "y", "ux", "uy", "psi", "ux_half_step", "uy_half_step", "psi_half_step"]
+
+
+class PlasmaAdj(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ADJ_NAMES]
 
 
 class Beam(C.Structure):
@@ -112,6 +120,10 @@ def lib():
         L.hpq_trace_enable.argtypes = [vp, ip, ip]
         L.hpq_trace_read.argtypes = [vp, ip, dp, C.POINTER(C.c_int64)]
         L.hpq_trace_reset.argtypes = [vp]
+        L.hpq_deposit_current_vjp.argtypes = [vp, C.POINTER(Plasma), vp, i32p, C.POINTER(PlasmaAdj)]
+        L.hpq_explicit_deposition_vjp.argtypes = [vp, C.POINTER(Plasma), vp, vp, C.POINTER(PlasmaAdj)]
+        L.hpq_advance_plasma_particles_vjp.argtypes = [vp, C.POINTER(Plasma), vp, C.POINTER(PlasmaAdj),
+                                                       C.POINTER(PlasmaAdj), vp]
         _lib = L
     return _lib
 

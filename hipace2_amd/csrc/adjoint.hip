@@ -1,0 +1,358 @@
+// Vector-Jacobian products of the particle operations of one slice, the building blocks of
+// the adjoint slice loop (no counterpart in the reference).
+//
+// Each kernel restates one forward kernel's per-particle arithmetic on forward-mode
+// tangents (tangent.hpp): the per-particle inputs (position, momenta, psi) are seeded, the
+// forward expressions are evaluated once, and the derivative parts of the adjoint-weighted
+// outputs are the particle's share of the product. Field values enter linearly, so their
+// adjoints are scattered to the cells with the forward weights. Branches (QSA removal,
+// boundary conditions, shape-factor pieces) follow the values and contribute no derivative.
+// Plain FP64 arithmetic throughout; one thread per particle in index order; field adjoints
+// are added to the adjoint planes with FP64 atomics.
+#include <cmath>
+
+#include "internal.hpp"
+#include "push.hpp"
+#include "shape.hpp"
+#include "tangent.hpp"
+
+namespace hpq {
+
+namespace {
+
+constexpr int VT = 128;   // threads per workgroup of the adjoint kernels
+
+__device__ __forceinline__ bool in_store(const Geom& g, int i, int j) {
+    return i >= -g.ng && i <= g.nx - 1 + g.ng && j >= -g.ng && j <= g.ny - 1 + g.ng;
+}
+__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// ---- ::DepositCurrent (PlasmaDepositCurrent.cpp:182-251) ----------------------------------
+// inputs seeded: x, y, ux, uy, psi
+template <int O>
+__global__ __launch_bounds__(VT) void k_deposit_vjp(Geom g, hpq_plasma p, hpq_plasma_adj a, const double* __restrict__ adj,
+                                                    int cjx, int cjy, int cjz, int crho, int cchi, int crm,
+                                                    double charge_invvol, double charge_mu0_mass_ratio,
+                                                    double max_qsa) {
+    using T = Tan<5>;
+    constexpr int S = O + 1;
+    const long ip = long(blockIdx.x) * VT + threadIdx.x;
+    if (ip >= p.n || !p.valid[ip]) return;
+    const double clight = 1.0, clightinv = 1.0 / 1.0;
+    const T x = T::seed(p.x[ip], 0), y = T::seed(p.y[ip], 1);
+    const T ux = T::seed(p.ux[ip], 2), uy = T::seed(p.uy[ip], 3), psi = T::seed(p.psi[ip], 4);
+    const T psi_inv = 1.0 / psi;
+    const T vx = ux * psi_inv, vy = uy * psi_inv;
+    const double q_invvol = charge_invvol * p.w[ip];
+    const T xmid = (x - g.xoff) * g.dxi, ymid = (y - g.yoff) * g.dyi;
+    const double Aabssqp = 0.0;
+    const T gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * psi_inv * psi_inv + vx * vx * clightinv * clightinv +
+                               vy * vy * clightinv * clightinv + 1.0);
+    if (gamma_psi.v < 0.0 || gamma_psi.v > max_qsa || psi_inv.v < 0.0) return;   // removed by the forward
+    const int comp[6] = {cjx, cjy, cjz, crho, cchi, crm};
+    T acc(0.0);
+#pragma unroll
+    for (int iy = 0; iy < S; ++iy) {
+        const SfR<T> Y = shape<O, T, T>(ymid, iy);
+#pragma unroll
+        for (int ix = 0; ix < S; ++ix) {
+            const SfR<T> X = shape<O, T, T>(xmid, ix);
+            if (!in_store(g, X.cell, Y.cell)) continue;
+            const long k = g.idx(X.cell, Y.cell);
+            const T cd = q_invvol * X.s * Y.s;
+            const T v[6] = {cd * vx, cd * vy, cd * (gamma_psi - 1.0) * clight, cd * gamma_psi,
+                            cd * charge_mu0_mass_ratio * psi_inv, cd};
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                if (comp[q] >= 0) acc += adj[long(comp[q]) * g.plane + k] * v[q];
+        }
+    }
+    a.x[ip] += acc.d[0];
+    a.y[ip] += acc.d[1];
+    a.ux[ip] += acc.d[2];
+    a.uy[ip] += acc.d[3];
+    a.psi[ip] += acc.d[4];
+}
+
+// ---- ::ExplicitDeposition (ExplicitDeposition.cpp:166-262) --------------------------------
+// particle inputs seeded (x, y, ux, uy, psi); the Sy, Sx adjoints are read from their
+// planes, the adjoints of Bz, Ez, ExmBy, EypBx are added to theirs
+template <int O, int TD>
+__global__ __launch_bounds__(VT) void k_explicit_vjp(Geom g, hpq_plasma p, hpq_plasma_adj a, const double* __restrict__ store,
+                                                     double* __restrict__ adj, Layout L, double charge_invvol_mu0,
+                                                     double charge_mass_ratio) {
+    using T = Tan<5>;
+    constexpr int S = dshape_width<TD, O>();
+    const long ip = long(blockIdx.x) * VT + threadIdx.x;
+    if (ip >= p.n || !p.valid[ip]) return;
+    const double clight = 1.0, clight_inv = 1.0 / 1.0;
+    const T x = T::seed(p.x[ip], 0), y = T::seed(p.y[ip], 1);
+    const T ux = T::seed(p.ux[ip], 2), uy = T::seed(p.uy[ip], 3), psi = T::seed(p.psi[ip], 4);
+    const T pinv = 1.0 / psi;
+    const T pvx = ux * pinv * clight_inv, pvy = uy * pinv * clight_inv;
+    const double cdm = charge_invvol_mu0 * p.w[ip];
+    const T xmid = (x - g.xoff) * g.dxi, ymid = (y - g.yoff) * g.dyi;
+    const double Aabssqp = 0.0;
+    const T gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * pinv * pinv + pvx * pvx + pvy * pvy + 1.0);
+    const T vxvy = pvx * pvy;
+    const T gpx = gamma_psi - pvx * pvx, gpy = gamma_psi - pvy * pvy;
+    const T qp = charge_mass_ratio * pinv;
+    const T ay = -cdm * qp, by = cdm * clight * g.dxi * vxvy, cy = -cdm * clight * g.dyi * (gpy - 1.0);
+    const T ax = cdm * qp, bx = cdm * clight * g.dxi * (gpx - 1.0), cx = -cdm * clight * g.dyi * vxvy;
+    const double* Bz = store + long(L.Bz) * g.plane;
+    const double* Ez = store + long(L.Ez) * g.plane;
+    const double* ExmBy = store + long(L.ExmBy) * g.plane;
+    const double* EypBx = store + long(L.EypBx) * g.plane;
+    const double* aSy = adj + long(L.Sy) * g.plane;
+    const double* aSx = adj + long(L.Sx) * g.plane;
+    T acc(0.0);
+#pragma unroll
+    for (int iy = 0; iy < S; ++iy) {
+        const DsfR<T> Y = dshape<TD, O, T, T>(ymid, iy);
+#pragma unroll
+        for (int ix = 0; ix < S; ++ix) {
+            if (TD == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) continue;
+            const DsfR<T> X = dshape<TD, O, T, T>(xmid, ix);
+            if (!in_store(g, X.cell, Y.cell)) continue;
+            const long k = g.idx(X.cell, Y.cell);
+            const double sy_a = aSy[k], sx_a = aSx[k];
+            if (sy_a == 0.0 && sx_a == 0.0) continue;
+            const T A = -Bz[k] * pvx + (EypBx[k] * gpy - ExmBy[k] * vxvy + Ez[k] * pvy) * clight_inv;
+            const T B = Bz[k] * pvy + (-EypBx[k] * vxvy + ExmBy[k] * gpx + Ez[k] * pvx) * clight_inv;
+            const T ss = X.s * Y.s, ds = X.ds * Y.s, sd = X.s * Y.ds;
+            acc += sy_a * (ay * A * ss + by * ds + cy * sd) + sx_a * (ax * B * ss + bx * ds + cx * sd);
+            // the contributions are linear in the four fields of cell k
+            const double wy = sy_a * ay.v * ss.v, wx = sx_a * ax.v * ss.v;
+            atomic_add(adj + long(L.Bz) * g.plane + k, -wy * pvx.v + wx * pvy.v);
+            atomic_add(adj + long(L.Ez) * g.plane + k, (wy * pvy.v + wx * pvx.v) * clight_inv);
+            atomic_add(adj + long(L.ExmBy) * g.plane + k, (-wy * vxvy.v + wx * gpx.v) * clight_inv);
+            atomic_add(adj + long(L.EypBx) * g.plane + k, (wy * gpy.v - wx * vxvy.v) * clight_inv);
+        }
+    }
+    a.x[ip] += acc.d[0];
+    a.y[ip] += acc.d[1];
+    a.ux[ip] += acc.d[2];
+    a.uy[ip] += acc.d[3];
+    a.psi[ip] += acc.d[4];
+}
+
+// ---- AdvancePlasmaParticles (PlasmaParticleAdvance.cpp:84-240), not a temporary slice -----
+// The push of one particle as a function of 11 inputs: x_prev, y_prev, ux_half_step,
+// uy_half_step, psi_half_step and the six gathered field values. One launch evaluates the
+// tangents of NB of them, starting at input k0: 11 inputs in three launches keeps the
+// tangent arithmetic in registers.
+constexpr int PUSH_IN = 11;
+
+template <int N>
+struct PushOut {
+    Tan<N> x, y, uxh, uyh, psih, ux, uy, psi;
+    bool removed;
+};
+
+// EnforceBC (GetAndSetPosition.H:29-99) on tangents: folds and shifts by constants keep the
+// derivative, a reflection negates it
+template <int N>
+__device__ bool enforce_bc_tan(const BC& bc, Tan<N>& x, Tan<N>& y, Tan<N>& ux, Tan<N>& uy) {
+    if (x.v < bc.plo0 || y.v < bc.plo1 || x.v > bc.phi0 || y.v > bc.phi1) {
+        const double lx = bc.phi0 - bc.plo0, ly = bc.phi1 - bc.plo1;
+        auto shift = [](Tan<N>& t, double nv) { t.v = nv; };
+        if (bc.type == HPQ_BC_REFLECTING) {
+            double xv = fmod(x.v - bc.plo0, 2 * lx); if (xv < 0) xv += 2 * lx; xv += bc.plo0;
+            shift(x, xv);
+            if (x.v > bc.phi0) { x = 2 * bc.phi0 - x; ux = -ux; }
+            double yv = fmod(y.v - bc.plo1, 2 * ly); if (yv < 0) yv += 2 * ly; yv += bc.plo1;
+            shift(y, yv);
+            if (y.v > bc.phi1) { y = 2 * bc.phi1 - y; uy = -uy; }
+            return false;
+        } else if (bc.type == HPQ_BC_PERIODIC) {
+            double xv = fmod(x.v - bc.plo0, lx); if (xv < 0) xv += lx; xv += bc.plo0;
+            shift(x, xv);
+            double yv = fmod(y.v - bc.plo1, ly); if (yv < 0) yv += ly; yv += bc.plo1;
+            shift(y, yv);
+            return false;
+        }
+        return true;
+    }
+    return false;
+}
+
+template <int O, int N>
+__device__ PushOut<N> push_tan(const Geom& g, const hpq_plasma& p, long ip, const double* store, const Layout& L,
+                               double qmc, const BC& bc, int k0) {
+    using T = Tan<N>;
+    auto seed = [&](double v, int input) { return (input >= k0 && input < k0 + N) ? T::seed(v, input - k0) : T(v); };
+    const double clight = 1.0, clight_inv = 1.0 / 1.0;
+    const double dz = g.dz / 1;
+    T xp = seed(p.x_prev[ip], 0), yp = seed(p.y_prev[ip], 1);
+    T G[6];   // ExmBy, EypBx, Ez, Bx, By, Bz at the particle
+    {
+        const T xm = (xp - g.xoff) * g.dxi, ym = (yp - g.yoff) * g.dyi;
+        constexpr int W = O + 2;
+        DsfR<T> sx[W], sy[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            sx[k] = dshape<1, O, T, T>(xm, k);
+            sy[k] = dshape<1, O, T, T>(ym, k);
+        }
+        const double* Psi = store + long(L.Psi) * g.plane;
+        const double* Ez = store + long(L.Ez) * g.plane;
+        const double* Bx = store + long(L.Bx) * g.plane;
+        const double* By = store + long(L.By) * g.plane;
+        const double* Bz = store + long(L.Bz) * g.plane;
+#pragma unroll
+        for (int iy = 0; iy < W; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < W; ++ix) {
+                const DsfR<T>& X = sx[ix];
+                const DsfR<T>& Y = sy[iy];
+                if (!in_store(g, X.cell, Y.cell)) continue;
+                const long k = g.idx(X.cell, Y.cell);
+                G[0] += (X.ds * Y.s) * Psi[k] * g.dxi;
+                G[1] += (X.s * Y.ds) * Psi[k] * g.dyi;
+                const T ss = X.s * Y.s;
+                G[2] += ss * Ez[k];
+                G[3] += ss * Bx[k];
+                G[4] += ss * By[k];
+                G[5] += ss * Bz[k];
+            }
+    }
+    G[3] = G[3] * clight;
+    G[4] = G[4] * clight;
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        if (5 + q >= k0 && 5 + q < k0 + N) G[q].d[5 + q - k0] += 1.0;
+    constexpr int nsub = 4;
+    const double sdz = dz / nsub;
+    T ux = seed(p.ux_half_step[ip], 2), uy = seed(p.uy_half_step[ip], 3), psi = seed(p.psi_half_step[ip], 4);
+    const T cir(clight_inv), qmr(qmc);
+    auto substep = [&]() {
+        const T psi_inv = 1.0 / psi;
+        T dux, duy, dpsi;
+        momentum_rhs<T, T>(ux, uy, psi_inv, G[0], G[1], G[2], G[3], G[4], G[5], cir, qmr, dux, duy, dpsi);
+        const Dual<T> uxd{ux, dux}, uyd{uy, duy}, pid{psi_inv, -psi_inv * psi_inv * dpsi};
+        Dual<T> ddux(T(0.0)), dduy(T(0.0)), ddpsi(T(0.0));
+        momentum_rhs<Dual<T>, T>(uxd, uyd, pid, G[0], G[1], G[2], G[3], G[4], G[5], cir, qmr, ddux, dduy, ddpsi);
+        ux = ux + sdz * dux + 0.5 * sdz * sdz * ddux.e;
+        uy = uy + sdz * duy + 0.5 * sdz * sdz * dduy.e;
+        psi = psi + sdz * dpsi + 0.5 * sdz * sdz * ddpsi.e;
+    };
+    for (int s = 0; s < nsub; ++s) substep();
+    xp = xp + dz * clight_inv * (ux * (1.0 / psi));
+    yp = yp + dz * clight_inv * (uy * (1.0 / psi));
+    PushOut<N> o;
+    o.removed = enforce_bc_tan(bc, xp, yp, ux, uy);
+    o.x = xp;
+    o.y = yp;
+    o.uxh = ux;
+    o.uyh = uy;
+    o.psih = psi;
+    for (int s = 0; s < nsub / 2; ++s) substep();
+    o.ux = ux;
+    o.uy = uy;
+    o.psi = psi;
+    return o;
+}
+
+template <int O, int N>
+__global__ __launch_bounds__(VT) void k_push_vjp(Geom g, hpq_plasma p, const double* __restrict__ store, Layout L,
+                                                 double qmc, BC bc, hpq_plasma_adj out, hpq_plasma_adj in,
+                                                 double* __restrict__ adj, int k0) {
+    const long ip = long(blockIdx.x) * VT + threadIdx.x;
+    if (ip >= p.n || !p.valid[ip]) return;
+    const PushOut<N> o = push_tan<O, N>(g, p, ip, store, L, qmc, bc, k0);
+    if (o.removed) return;   // absorbed: nothing after this slice depends on the particle
+    double lam[N];
+    const double lx = out.x[ip], ly = out.y[ip], luxh = out.ux_half_step[ip], luyh = out.uy_half_step[ip],
+                 lph = out.psi_half_step[ip], lux = out.ux[ip], luy = out.uy[ip], lps = out.psi[ip];
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        lam[k] = lx * o.x.d[k] + ly * o.y.d[k] + luxh * o.uxh.d[k] + luyh * o.uyh.d[k] + lph * o.psih.d[k] +
+                 lux * o.ux.d[k] + luy * o.uy.d[k] + lps * o.psi.d[k];
+    double* dst[5] = {in.x, in.y, in.ux_half_step, in.uy_half_step, in.psi_half_step};
+    double lg[6] = {0, 0, 0, 0, 0, 0};
+    bool any_g = false;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int input = k0 + k;
+        if (input < 5) dst[input][ip] += lam[k];
+        else if (input < PUSH_IN) { lg[input - 5] = lam[k]; any_g = true; }
+    }
+    if (!any_g) return;
+    // gathered values are linear in the cell values: scatter their adjoints with the
+    // forward's weights (B*c scaled by c)
+    const double clight = 1.0;
+    const double xm = (p.x_prev[ip] - g.xoff) * g.dxi, ym = (p.y_prev[ip] - g.yoff) * g.dyi;
+    constexpr int W = O + 2;
+    Dsf sx[W], sy[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        sx[k] = dshape<1, O>(xm, k);
+        sy[k] = dshape<1, O>(ym, k);
+    }
+#pragma unroll
+    for (int iy = 0; iy < W; ++iy)
+#pragma unroll
+        for (int ix = 0; ix < W; ++ix) {
+            const Dsf X = sx[ix], Y = sy[iy];
+            if (!in_store(g, X.cell, Y.cell)) continue;
+            const long k = g.idx(X.cell, Y.cell);
+            const double ss = X.s * Y.s;
+            const double psi_a = lg[0] * (X.ds * Y.s) * g.dxi + lg[1] * (X.s * Y.ds) * g.dyi;
+            if (psi_a != 0.0) atomic_add(adj + long(L.Psi) * g.plane + k, psi_a);
+            if (lg[2] != 0.0) atomic_add(adj + long(L.Ez) * g.plane + k, lg[2] * ss);
+            if (lg[3] != 0.0) atomic_add(adj + long(L.Bx) * g.plane + k, lg[3] * clight * ss);
+            if (lg[4] != 0.0) atomic_add(adj + long(L.By) * g.plane + k, lg[4] * clight * ss);
+            if (lg[5] != 0.0) atomic_add(adj + long(L.Bz) * g.plane + k, lg[5] * ss);
+        }
+}
+
+}  // namespace
+
+void deposit_vjp(hpq_ctx* c, const hpq_plasma* p, const double* adj, const int32_t* comps, hpq_plasma_adj* a) {
+    if (p->n <= 0) return;
+    const Geom& g = c->g;
+    const double charge = c->p.plasma_charge;
+    const double invvol = g.dx * g.dy * g.dxi * g.dyi;
+    const double cmr = charge * 1.0 / c->p.plasma_mass;
+    auto cm = [&](int k) { return comps[k] >= 0 ? comps[k] : -1; };
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        hipLaunchKernelGGL((k_deposit_vjp<decltype(O)::value>), grid1d(p->n, VT), dim3(VT), 0, c->stream, g, *p, *a,
+                           adj, cm(0), cm(1), cm(2), cm(3), cm(4), cm(5), charge * invvol, cmr, c->p.max_qsa);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+void explicit_deposition_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, double* adj, hpq_plasma_adj* a) {
+    if (p->n <= 0) return;
+    const Geom& g = c->g;
+    const double invvol = g.dx * g.dy * g.dxi * g.dyi;
+    const double qimu0 = c->p.plasma_charge * invvol * 1.0;
+    const double qmr = c->p.plasma_charge / c->p.plasma_mass;
+    const int T = c->p.derivative_type;
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        constexpr int o = decltype(O)::value;
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid1d(p->n, VT), dim3(VT), 0, c->stream, g, *p, *a, store, adj, c->L, qimu0, qmr);
+        };
+        if (T == 0) launch(k_explicit_vjp<o, 0>);
+        else if (T == 1) launch(k_explicit_vjp<o, 1>);
+        else launch(k_explicit_vjp<o, 2>);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+void advance_plasma_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, const hpq_plasma_adj* out,
+                        hpq_plasma_adj* in, double* adj) {
+    if (p->n <= 0) return;
+    const double qmc = c->p.plasma_charge / (c->p.plasma_mass * 1.0);
+    BC bc{c->p.particle_bc, c->p.plo[0], c->p.plo[1], c->p.phi[0], c->p.phi[1]};
+    constexpr int NB = 4;
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        for (int k0 = 0; k0 < PUSH_IN; k0 += NB)
+            hipLaunchKernelGGL((k_push_vjp<decltype(O)::value, NB>), grid1d(p->n, VT), dim3(VT), 0, c->stream, c->g,
+                               *p, store, c->L, qmc, bc, *out, *in, adj, k0);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+}  // namespace hpq

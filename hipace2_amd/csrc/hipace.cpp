@@ -472,6 +472,52 @@ int hpq_advance_plasma_particles(hpq_ctx* c, const hpq_plasma* p, const double* 
     });
 }
 
+static void check_adj(const hpq_plasma_adj* a, bool half_step) {
+    if (!a || !a->x || !a->y || !a->ux || !a->uy || !a->psi) throw HpqError(HPQ_E_INVALID, "null adjoint array");
+    if (half_step && (!a->ux_half_step || !a->uy_half_step || !a->psi_half_step))
+        throw HpqError(HPQ_E_INVALID, "null adjoint half-step array");
+}
+
+int hpq_deposit_current_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store_adj, const int32_t* comps,
+                            hpq_plasma_adj* adj) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store_adj);
+        check_plasma(p);
+        check_adj(adj, false);
+        if (!comps) throw HpqError(HPQ_E_INVALID, "null comps");
+        for (int k = 0; k < 6; ++k)
+            if (comps[k] >= c->L.ncomp) throw HpqError(HPQ_E_INVALID, "component index out of range");
+        deposit_vjp(c, p, store_adj, comps, adj);
+    });
+}
+
+int hpq_explicit_deposition_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, double* store_adj,
+                                hpq_plasma_adj* adj) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_store(c, store_adj);
+        check_plasma(p);
+        check_adj(adj, false);
+        if (!c->p.explicit_solver) throw HpqError(HPQ_E_UNSUPPORTED, "ExplicitDeposition needs the explicit layout");
+        explicit_deposition_vjp(c, p, store, store_adj, adj);
+    });
+}
+
+int hpq_advance_plasma_particles_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store,
+                                     const hpq_plasma_adj* out_adj, hpq_plasma_adj* in_adj, double* store_adj) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_store(c, store_adj);
+        check_plasma(p);
+        check_adj(out_adj, true);
+        check_adj(in_adj, true);
+        advance_plasma_vjp(c, p, store, out_adj, in_adj, store_adj);
+    });
+}
+
 int hpq_poisson_dirichlet(hpq_ctx* c, const double* staging, double* lhs) {
     if (!c) return HPQ_E_INVALID;
     return guarded(c, [&] {

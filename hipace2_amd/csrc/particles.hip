@@ -9,6 +9,7 @@
 #include <cmath>
 
 #include "internal.hpp"
+#include "push.hpp"
 #include "shape.hpp"
 
 namespace hpq {
@@ -18,39 +19,6 @@ namespace {
 __device__ __forceinline__ bool in_store(const Geom& g, int i, int j) {
     return i >= -g.ng && i <= g.nx - 1 + g.ng && j >= -g.ng && j <= g.ny - 1 + g.ng;
 }
-
-// Forward-mode dual number for the second-order Taylor step (src/utils/DualNumbers.H:13-43)
-template <class R>
-struct Dual {
-    R v, e;
-    __device__ Dual(R a) : v(a), e(0) {}
-    __device__ Dual(R a, R b) : v(a), e(b) {}
-};
-template <class R>
-__device__ __forceinline__ Dual<R> operator+(const Dual<R>& a, const Dual<R>& b) { return {a.v + b.v, a.e + b.e}; }
-template <class R>
-__device__ __forceinline__ Dual<R> operator-(const Dual<R>& a, const Dual<R>& b) { return {a.v - b.v, a.e - b.e}; }
-template <class R>
-__device__ __forceinline__ Dual<R> operator*(const Dual<R>& a, const Dual<R>& b) {
-    return {a.v * b.v, a.e * b.v + a.v * b.e};
-}
-
-// PlasmaMomentumPush, src/particles/pusher/PushPlasmaParticles.H:37-75 (no laser terms)
-template <class T, class R>
-__device__ __forceinline__ void momentum_rhs(const T& ux, const T& uy, const T& psi_inv, R ExmBy, R EypBx, R Ez,
-                                             R Bx_c, R By_c, R Bz, R ci, R qmc, T& dux, T& duy, T& dpsi) {
-    const R A = 0.0, ADx = 0.0, ADy = 0.0;
-    const T gamma_psi =
-        T(0.5) * psi_inv * psi_inv * (T(1.0) + T(A) + ux * ux * T(ci * ci) + uy * uy * T(ci * ci)) + T(0.5);
-    dux = (T(qmc) * (gamma_psi * T(ExmBy) + T(By_c) + (uy * T(Bz)) * psi_inv) - T(ADx) * psi_inv);
-    duy = (T(qmc) * (gamma_psi * T(EypBx) - T(Bx_c) - (ux * T(Bz)) * psi_inv) - T(ADy) * psi_inv);
-    dpsi = (T(qmc) * T(ci) * ((ux * T(ExmBy) + uy * T(EypBx)) * T(ci) * psi_inv - T(Ez)));
-}
-
-struct BC {
-    int type;
-    double plo0, plo1, phi0, phi1;
-};
 
 // EnforceBC, src/particles/pusher/GetAndSetPosition.H:29-99; returns true if removed
 __device__ __forceinline__ bool enforce_bc(const BC& bc, double& x, double& y, double& ux, double& uy) {

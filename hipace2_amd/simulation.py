@@ -200,6 +200,39 @@ class Simulation:
         _abi.check(self.ctx, self.L.hpq_deposit_current_slice(self.ctx, C.byref(self._bdesc), islice,
                                                               C.c_void_p(self.store.data_ptr()), arr))
 
+    # ---- adjoint building blocks (vector-Jacobian products) ------------------------------
+    def new_plasma_adj(self) -> Dict[str, torch.Tensor]:
+        """Zeroed adjoint arrays (ADJ_NAMES) for the current plasma."""
+        return {k: torch.zeros(self.np, dtype=torch.float64, device=self.device) for k in _abi.ADJ_NAMES}
+
+    def new_store_adj(self) -> torch.Tensor:
+        """Zeroed adjoint planes in the slice-store layout."""
+        return torch.zeros_like(self.store)
+
+    @staticmethod
+    def _adj_desc(a: Dict[str, torch.Tensor]):
+        return _abi.PlasmaAdj(*[a[k].data_ptr() for k in _abi.ADJ_NAMES])
+
+    def deposit_current_vjp(self, comps, store_adj: torch.Tensor, adj: Dict[str, torch.Tensor]):
+        f = lambda c: self.layout.get(c, -1) if isinstance(c, str) else c
+        arr = (C.c_int32 * 6)(*[f(c) for c in comps])
+        d = self._adj_desc(adj)
+        _abi.check(self.ctx, self.L.hpq_deposit_current_vjp(self.ctx, C.byref(self._pdesc),
+                                                            C.c_void_p(store_adj.data_ptr()), arr, C.byref(d)))
+
+    def explicit_deposition_vjp(self, store_adj: torch.Tensor, adj: Dict[str, torch.Tensor]):
+        d = self._adj_desc(adj)
+        _abi.check(self.ctx, self.L.hpq_explicit_deposition_vjp(self.ctx, C.byref(self._pdesc),
+                                                                C.c_void_p(self.store.data_ptr()),
+                                                                C.c_void_p(store_adj.data_ptr()), C.byref(d)))
+
+    def advance_plasma_particles_vjp(self, out_adj: Dict[str, torch.Tensor], in_adj: Dict[str, torch.Tensor],
+                                     store_adj: torch.Tensor):
+        o, i = self._adj_desc(out_adj), self._adj_desc(in_adj)
+        _abi.check(self.ctx, self.L.hpq_advance_plasma_particles_vjp(
+            self.ctx, C.byref(self._pdesc), C.c_void_p(self.store.data_ptr()), C.byref(o), C.byref(i),
+            C.c_void_p(store_adj.data_ptr())))
+
     def advance_plasma_particles(self, temp_slice: bool = False):
         _abi.check(self.ctx, self.L.hpq_advance_plasma_particles(self.ctx, C.byref(self._pdesc),
                                                                  C.c_void_p(self.store.data_ptr()),

@@ -95,6 +95,12 @@ typedef struct hpq_beam {
     double charge, mass;
 } hpq_beam;
 
+/* Adjoints of plasma quantities for the vector-Jacobian products below: device arrays of
+ * n doubles (n of the matching hpq_plasma), accumulated with +=. */
+typedef struct hpq_plasma_adj {
+    double *x, *y, *ux, *uy, *psi, *ux_half_step, *uy_half_step, *psi_half_step;
+} hpq_plasma_adj;
+
 /* Plasma injection profile: n(x,y) = density * (1 + ramp_a (x^2+y^2)) inside `radius`. */
 typedef struct hpq_plasma_profile {
     int ppc[2];
@@ -139,6 +145,31 @@ int hpq_deposit_current_slice(hpq_ctx *ctx, const hpq_beam *b, int slice, double
  * (leapfrog, gather FieldGather.H:45-96, EnforceBC GetAndSetPosition.H:29-99). */
 int hpq_advance_plasma_particles(hpq_ctx *ctx, const hpq_plasma *p, const double *store,
                                  int temp_slice);
+
+/* ---- adjoint: vector-Jacobian products of the slice operations -----------------------
+ * No counterpart in the reference (its slice loop has no derivative); they make the
+ * adjoint slice loop. Adjoint planes use the slice-store layout: store_adj[c] holds the
+ * adjoint of component c. Evaluated in FP64 at the given state; branches (QSA removal,
+ * particle boundaries, shape-factor pieces) are held fixed. */
+
+/* Adjoint of hpq_deposit_current (ions = 0): adds to adj->x, y, ux, uy, psi the derivative
+ * of sum_c <store_adj[comps[c]], deposited plane c>. */
+int hpq_deposit_current_vjp(hpq_ctx *ctx, const hpq_plasma *p, const double *store_adj,
+                            const int32_t *comps, hpq_plasma_adj *adj);
+
+/* Adjoint of hpq_explicit_deposition: reads the Sy, Sx planes of store_adj; adds to its
+ * Bz, Ez, ExmBy, EypBx planes and to adj->x, y, ux, uy, psi. */
+int hpq_explicit_deposition_vjp(hpq_ctx *ctx, const hpq_plasma *p, const double *store,
+                                double *store_adj, hpq_plasma_adj *adj);
+
+/* Adjoint of hpq_advance_plasma_particles (temp_slice = 0) at the state p before the push.
+ * out_adj: adjoints of the outputs x (also x_prev), y (also y_prev), ux/uy/psi_half_step
+ * and ux, uy, psi. Adds the adjoints of the inputs x_prev, y_prev (to in_adj->x, y) and
+ * of the half-step values (to in_adj->*_half_step), and those of the Psi, Ez, Bx, By, Bz
+ * planes to store_adj. */
+int hpq_advance_plasma_particles_vjp(hpq_ctx *ctx, const hpq_plasma *p, const double *store,
+                                     const hpq_plasma_adj *out_adj, hpq_plasma_adj *in_adj,
+                                     double *store_adj);
 
 /* FFTPoissonSolverDirichlet*::SolvePoissonEquation
  * (src/fields/fft_poisson_solver/FFTPoissonSolverDirichletDirect.cpp:101-139,
