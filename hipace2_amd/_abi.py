@@ -27,7 +27,7 @@ EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq
            "hpq_init_plasma", "hpq_begin_step", "hpq_solve_one_slice", "hpq_run_slices", "hpq_set_checksums",
            "hpq_add_capture", "hpq_clear_captures", "hpq_stats", "hpq_trace_enable", "hpq_trace_read",
            "hpq_trace_reset", "hpq_bin_tiles", "hpq_deposit_current_vjp", "hpq_explicit_deposition_vjp",
-           "hpq_advance_plasma_particles_vjp"]
+           "hpq_advance_plasma_particles_vjp", "hpq_adjoint_forward", "hpq_adjoint_backward"]
 
 TRACE = {"push": 0, "deposit": 1, "explicit_deposit": 2, "beam_deposit": 3, "mg_smooth": 4, "mg_coarse": 5,
          "dst": 6}
@@ -124,6 +124,8 @@ def lib():
         L.hpq_explicit_deposition_vjp.argtypes = [vp, C.POINTER(Plasma), vp, vp, C.POINTER(PlasmaAdj)]
         L.hpq_advance_plasma_particles_vjp.argtypes = [vp, C.POINTER(Plasma), vp, C.POINTER(PlasmaAdj),
                                                        C.POINTER(PlasmaAdj), vp]
+        L.hpq_adjoint_forward.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, ip]
+        L.hpq_adjoint_backward.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, vp, vp, C.c_double]
         _lib = L
     return _lib
 

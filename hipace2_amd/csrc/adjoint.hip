@@ -356,3 +356,170 @@ void advance_plasma_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, co
 }
 
 }  // namespace hpq
+
+// ---------------------------------------------------------------------------------------
+// Linear parts of the adjoint slice step
+// ---------------------------------------------------------------------------------------
+namespace hpq {
+
+namespace {
+
+__device__ __forceinline__ bool in_valid(const Geom& g, int i, int j) { return i >= 0 && i < g.nx && j >= 0 && j < g.ny; }
+
+// Staged right-hand sides of the adjoint Poisson solves: the Psi adjoint (valid cells) plus
+// the transposed centred differences of ExmBy = -d_x Psi, EypBx = -d_y Psi (defined on the
+// valid box plus one ring, Fields.cpp:931-956), and the Ez and Bz adjoints.
+template <class T>
+__global__ void k_adj_stage(Geom g, const double* __restrict__ apsi, const double* __restrict__ aexmby,
+                            const double* __restrict__ aeypbx, const double* __restrict__ aez,
+                            const double* __restrict__ abz, T* __restrict__ st, long ld, long fs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= g.nx) return;
+    const double dxih = 0.5 * g.dxi, dyih = 0.5 * g.dyi;
+    const double v = apsi[g.idx(i, j)] - aexmby[g.idx(i - 1, j)] * dxih + aexmby[g.idx(i + 1, j)] * dxih -
+                     aeypbx[g.idx(i, j - 1)] * dyih + aeypbx[g.idx(i, j + 1)] * dyih;
+    const long o = long(j) * ld + i;
+    st[o] = T(v);
+    st[fs + o] = T(aez[g.idx(i, j)]);
+    st[2 * fs + o] = T(abz[g.idx(i, j)]);
+}
+
+// Transposes of the right-hand sides of Fields::SolvePoissonPsiExmByEypBxEzBz
+// (Fields.cpp:886-917) for every cell of the guarded planes: r_* are the adjoint solves
+// (valid cells), results are added to the rhomjz, jx, jy adjoints.
+__global__ void k_adj_rhs(Geom g, const double* __restrict__ rpsi, const double* __restrict__ rez,
+                          const double* __restrict__ rbz, double* __restrict__ arhomjz, double* __restrict__ ajx,
+                          double* __restrict__ ajy) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x) - g.ng;
+    const int j = int(blockIdx.y) - g.ng;
+    if (i > g.nx - 1 + g.ng) return;
+    const double ep0 = 1.0, mu0 = 1.0, cl = 1.0;
+    const double dxih = 0.5 * g.dxi, dyih = 0.5 * g.dyi;
+    const double fe = 1.0 / (ep0 * cl);
+    auto R = [&](const double* r, int a, int b) { return in_valid(g, a, b) ? r[g.idx(a, b)] : 0.0; };
+    const long k = g.idx(i, j);
+    if (in_valid(g, i, j)) arhomjz[k] += (-1.0 / ep0) * rpsi[k];
+    ajx[k] += fe * (R(rez, i - 1, j) - R(rez, i + 1, j)) * dxih + mu0 * (R(rbz, i, j - 1) - R(rbz, i, j + 1)) * dyih;
+    ajy[k] += fe * (R(rez, i, j - 1) - R(rez, i, j + 1)) * dyih - mu0 * (R(rbz, i - 1, j) - R(rbz, i + 1, j)) * dxih;
+}
+
+// chi adjoint of the Bx/By solve (Lap - chi) B = S: chi_adj += mu_x Bx + mu_y By
+__global__ void k_adj_chi(Geom g, const double* __restrict__ Bx, const double* __restrict__ By,
+                          const double* __restrict__ mux, const double* __restrict__ muy, double* __restrict__ achi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= g.nx) return;
+    const long k = g.idx(i, j);
+    achi[k] += mux[k] * Bx[k] + muy[k] * By[k];
+}
+
+// loss seed of Ez at this slice ([j][i] over the valid box) into the Ez adjoint
+__global__ void k_adj_seed(Geom g, const double* __restrict__ seed, double* __restrict__ aez) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= g.nx) return;
+    aez[g.idx(i, j)] += seed[long(j) * g.nx + i];
+}
+
+// Hipace::InitializeSxSyWithBeam (Hipace.cpp:745-790) transposed for the beam jz: Sy has
+// -d_y jz_b, Sx has +d_x jz_b on valid cells; the jz_b adjoint of every cell
+__global__ void k_adj_sxsy_jzb(Geom g, const double* __restrict__ asy, const double* __restrict__ asx,
+                               double* __restrict__ ajzb) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x) - g.ng;
+    const int j = int(blockIdx.y) - g.ng;
+    if (i > g.nx - 1 + g.ng) return;
+    const double mu0 = 1.0;
+    auto A = [&](const double* a, int p, int q) { return in_valid(g, p, q) ? a[g.idx(p, q)] : 0.0; };
+    // Sy(i, j) = -(jz_b(i, j+1) - jz_b(i, j-1)) / 2dy + ..., Sx(i, j) = (jz_b(i+1, j) - jz_b(i-1, j)) / 2dx + ...
+    const double v = mu0 * (A(asy, i, j + 1) - A(asy, i, j - 1)) / (2.0 * g.dy) +
+                     mu0 * (A(asx, i - 1, j) - A(asx, i + 1, j)) / (2.0 * g.dx);
+    ajzb[g.idx(i, j)] = v;
+}
+
+// ::DepositCurrentSlice jz_b (BeamDepositCurrent.cpp:133-193) transposed for the weights:
+// w_adj += charge (uz / gamma) sum_cells S jz_b_adj
+template <int O>
+__global__ void k_beam_w_vjp(Geom g, hpq_beam b, long first, long count, const double* __restrict__ ajzb,
+                             double* __restrict__ w_adj) {
+    constexpr int S = O + 1;
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const long ip = first + t;
+    if (!b.valid[ip]) return;
+    const double clightsq = 1.0 / (1.0 * 1.0);
+    const double ux = b.ux[ip], uy = b.uy[ip], uz = b.uz[ip];
+    const double gaminv = 1.0 / std::sqrt(1.0 + ux * ux * clightsq + uy * uy * clightsq + uz * uz * clightsq);
+    const double xmid = (b.x[ip] - g.xoff) * g.dxi, ymid = (b.y[ip] - g.yoff) * g.dyi;
+    double acc = 0.0;
+#pragma unroll
+    for (int iy = 0; iy < S; ++iy) {
+        const Sf Y = shape<O>(ymid, iy);
+#pragma unroll
+        for (int ix = 0; ix < S; ++ix) {
+            const Sf X = shape<O>(xmid, ix);
+            if (!in_store(g, X.cell, Y.cell)) continue;
+            acc += X.s * Y.s * ajzb[g.idx(X.cell, Y.cell)];
+        }
+    }
+    w_adj[ip] += b.charge * 1.0 * (uz * gaminv) * acc;
+}
+
+}  // namespace
+
+void adj_poisson(hpq_ctx* c, double* adj) {
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    const Poisson& S = c->ps;
+    auto pl = [&](int comp) { return adj + long(comp) * g.plane; };
+    const dim3 grid((g.nx + 127) / 128, g.ny);
+    if (c->p.poisson_solver == HPQ_POISSON_BF16X3)
+        hipLaunchKernelGGL(k_adj_stage<float>, grid, dim3(128), 0, c->stream, g, pl(L.Psi), pl(L.ExmBy), pl(L.EypBx),
+                           pl(L.Ez), pl(L.Bz), S.gS, long(S.npx), long(S.npx) * S.npy);
+    else
+        hipLaunchKernelGGL(k_adj_stage<double>, grid, dim3(128), 0, c->stream, g, pl(L.Psi), pl(L.ExmBy),
+                           pl(L.EypBx), pl(L.Ez), pl(L.Bz), S.bufA, long(S.nx), long(S.nx) * S.ny);
+    HPQ_HIP(hipGetLastError());
+    // the solves overwrite the valid cells of the Psi, Ez, Bz adjoints, which the staging
+    // has consumed
+    double* lhs[3] = {pl(L.Psi), pl(L.Ez), pl(L.Bz)};
+    poisson_solve_staged(c, 3, lhs);
+    hipLaunchKernelGGL(k_adj_rhs, dim3((g.ldx + 127) / 128, g.ldy), dim3(128), 0, c->stream, g, pl(L.Psi), pl(L.Ez),
+                       pl(L.Bz), pl(L.rhomjz), pl(L.jx), pl(L.jy));
+    HPQ_HIP(hipGetLastError());
+}
+
+void adj_chi(hpq_ctx* c, const double* store, double* adj) {
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    hipLaunchKernelGGL(k_adj_chi, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g, store + L.Bx * g.plane,
+                       store + L.By * g.plane, adj + L.Sy * g.plane, adj + L.Sx * g.plane, adj + L.chi * g.plane);
+    HPQ_HIP(hipGetLastError());
+}
+
+void adj_seed(hpq_ctx* c, const double* seed, double* adj) {
+    const Geom& g = c->g;
+    hipLaunchKernelGGL(k_adj_seed, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g, seed,
+                       adj + c->L.Ez * g.plane);
+    HPQ_HIP(hipGetLastError());
+}
+
+void adj_beam_weights(hpq_ctx* c, const hpq_beam* b, int slice, double* adj, double* w_adj) {
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    // the jz_b adjoint goes to the jz_beam plane of the adjoint store
+    hipLaunchKernelGGL(k_adj_sxsy_jzb, dim3((g.ldx + 127) / 128, g.ldy), dim3(128), 0, c->stream, g,
+                       adj + L.Sy * g.plane, adj + L.Sx * g.plane, adj + L.jz_beam * g.plane);
+    HPQ_HIP(hipGetLastError());
+    if (!b || !b->off || !w_adj) return;
+    const long first = b->off[slice];
+    const long count = b->off[slice + 1] - first;
+    if (count <= 0) return;
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        hipLaunchKernelGGL((k_beam_w_vjp<decltype(O)::value>), grid1d(count, 256), dim3(256), 0, c->stream, g, *b,
+                           first, count, adj + L.jz_beam * g.plane, w_adj);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
+}  // namespace hpq

@@ -11,6 +11,7 @@
 // (MG residual norm per V-cycle, PC-loop B-field error, QSA counter on request).
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 #include "internal.hpp"
 
@@ -212,7 +213,8 @@ public:
     // Hipace::SolveOneSlice, :557-728 (level 0; no laser, SALAME, ionisation, collisions).
     // initialized: the previous slice of the same pass has already run InitializeSlices for
     // this one; chain: another slice follows in this pass.
-    void SolveOneSlice(int islice, bool initialized = false, bool chain = false) {
+    void SolveOneSlice(int islice, bool initialized = false, bool chain = false,
+                       const std::function<void()>& before_push = {}) {
         const hpq_params& P = c_->p;
         if (!initialized) fields_.InitializeSlices();
         if (P.explicit_solver) {
@@ -234,6 +236,7 @@ public:
             PredictorCorrectorLoopToSolveBxBy(islice);
         }
         fields_.EndSlice(islice, chain);
+        if (before_push) before_push();
         advance_plasma(c_, p_, s_, false);
         c_->stats[3] += double(p_->n);
         if (!chain) fields_.ShiftSlices();
@@ -336,6 +339,210 @@ void check_comp(const hpq_ctx* c, int k) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------
+// Adjoint slice loop (no counterpart in the reference): a checkpointed forward pass, then a
+// backward march over the slices in reverse order. Each checkpoint segment of k slices is
+// recomputed from its checkpoint, saving the slice store and the plasma right before each
+// push; the adjoint slice step then runs the vector-Jacobian products of push, Bx/By solve,
+// explicit deposition, beam source, Poisson solves and plasma deposition in reverse.
+// ---------------------------------------------------------------------------------
+namespace {
+
+void plasma_buf_alloc(PlasmaBuf& b, long n) {
+    b.n = n;
+    for (int k = 0; k < PLASMA_REALS; ++k) HPQ_HIP(hipMalloc(&b.r[k], sizeof(double) * std::max(n, 1L)));
+    HPQ_HIP(hipMalloc(&b.valid, sizeof(int32_t) * std::max(n, 1L)));
+}
+
+void plasma_buf_free(PlasmaBuf& b) {
+    for (double*& r : b.r)
+        if (r) (void)hipFree(r), r = nullptr;
+    if (b.valid) (void)hipFree(b.valid), b.valid = nullptr;
+}
+
+double* const* plasma_reals(const hpq_plasma* p, double* (&a)[PLASMA_REALS]) {
+    double* v[PLASMA_REALS] = {p->x, p->y, p->w, p->ux, p->uy, p->psi, p->x_prev, p->y_prev,
+                               p->ux_half_step, p->uy_half_step, p->psi_half_step};
+    for (int k = 0; k < PLASMA_REALS; ++k) a[k] = v[k];
+    return a;
+}
+
+void plasma_save(hpq_ctx* c, const hpq_plasma* p, PlasmaBuf& b) {
+    double* a[PLASMA_REALS];
+    plasma_reals(p, a);
+    for (int k = 0; k < PLASMA_REALS; ++k)
+        HPQ_HIP(hipMemcpyAsync(b.r[k], a[k], sizeof(double) * p->n, hipMemcpyDeviceToDevice, c->stream));
+    HPQ_HIP(hipMemcpyAsync(b.valid, p->valid, sizeof(int32_t) * p->n, hipMemcpyDeviceToDevice, c->stream));
+}
+
+void plasma_restore(hpq_ctx* c, const PlasmaBuf& b, const hpq_plasma* p) {
+    double* a[PLASMA_REALS];
+    plasma_reals(p, a);
+    for (int k = 0; k < PLASMA_REALS; ++k)
+        HPQ_HIP(hipMemcpyAsync(a[k], b.r[k], sizeof(double) * p->n, hipMemcpyDeviceToDevice, c->stream));
+    HPQ_HIP(hipMemcpyAsync(p->valid, b.valid, sizeof(int32_t) * p->n, hipMemcpyDeviceToDevice, c->stream));
+}
+
+hpq_plasma plasma_view(const PlasmaBuf& b) {
+    hpq_plasma v{};
+    v.x = b.r[0]; v.y = b.r[1]; v.w = b.r[2]; v.ux = b.r[3]; v.uy = b.r[4]; v.psi = b.r[5];
+    v.x_prev = b.r[6]; v.y_prev = b.r[7]; v.ux_half_step = b.r[8]; v.uy_half_step = b.r[9];
+    v.psi_half_step = b.r[10];
+    v.valid = b.valid;
+    v.n = b.n;
+    return v;
+}
+
+hpq_plasma_adj adj_view(double* const (&a)[8]) {
+    return hpq_plasma_adj{a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]};
+}
+
+size_t store_bytes(const hpq_ctx* c) { return sizeof(double) * size_t(c->L.ncomp) * c->g.plane; }
+
+void adjoint_alloc(hpq_ctx* c, int k, long np) {
+    AdjointState& A = c->adj;
+    const int nck = (c->g.nz + k - 1) / k;
+    if (A.k == k && A.np == np && A.nck == nck) return;
+    adjoint_free(c);
+    A.k = k;
+    A.nck = nck;
+    A.np = np;
+    A.ck_store.assign(nck, nullptr);
+    A.ck_plasma.assign(nck, PlasmaBuf{});
+    A.sl_store.assign(k, nullptr);
+    A.sl_plasma.assign(k, PlasmaBuf{});
+    for (int i = 0; i < nck; ++i) {
+        HPQ_HIP(hipMalloc(&A.ck_store[i], store_bytes(c)));
+        plasma_buf_alloc(A.ck_plasma[i], np);
+    }
+    for (int i = 0; i < k; ++i) {
+        HPQ_HIP(hipMalloc(&A.sl_store[i], store_bytes(c)));
+        plasma_buf_alloc(A.sl_plasma[i], np);
+    }
+    HPQ_HIP(hipMalloc(&A.store_adj, store_bytes(c)));
+    for (auto& set : A.padj)
+        for (double*& a : set) HPQ_HIP(hipMalloc(&a, sizeof(double) * std::max(np, 1L)));
+}
+
+}  // namespace
+
+void hpq::adjoint_free(hpq_ctx* c) {
+    AdjointState& A = c->adj;
+    for (double* s : A.ck_store)
+        if (s) (void)hipFree(s);
+    for (double* s : A.sl_store)
+        if (s) (void)hipFree(s);
+    for (PlasmaBuf& b : A.ck_plasma) plasma_buf_free(b);
+    for (PlasmaBuf& b : A.sl_plasma) plasma_buf_free(b);
+    if (A.store_adj) (void)hipFree(A.store_adj);
+    for (auto& set : A.padj)
+        for (double* a : set)
+            if (a) (void)hipFree(a);
+    A = AdjointState{};
+}
+
+namespace {
+
+void adjoint_check(const hpq_ctx* c) {
+    const hpq_params& P = c->p;
+    if (!P.explicit_solver) throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: explicit Bx/By solver only");
+    if (P.nx % 2 == 0 || P.ny % 2 == 0)
+        throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: odd nx, ny (node-centred multigrid, a symmetric operator)");
+    if (P.do_beam_jx_jy) throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: do_beam_jx_jy must be 0");
+}
+
+// the forward pass with checkpoints every k slices
+void adjoint_forward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store, int k) {
+    adjoint_check(c);
+    adjoint_alloc(c, k, p->n);
+    AdjointState& A = c->adj;
+    Hipace h(c, p, b, store);
+    const int nz = c->g.nz;
+    for (int isl = nz - 1; isl >= 0; --isl) {
+        const int step = nz - 1 - isl;
+        if (step % k == 0) {
+            HPQ_HIP(hipMemcpyAsync(A.ck_store[step / k], store, store_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+            plasma_save(c, p, A.ck_plasma[step / k]);
+        }
+        h.SolveOneSlice(isl);
+    }
+    diag_finalize(c);
+    A.ready = true;
+}
+
+// the backward march; seed: d loss / d Ez per slice ([slice][j][i], valid cells) or null;
+// w_adj: beam weight adjoints, accumulated
+void adjoint_backward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store, const double* seed,
+                      double* w_adj, double mg_tol) {
+    adjoint_check(c);
+    AdjointState& A = c->adj;
+    if (!A.ready || A.np != p->n) throw HpqError(HPQ_E_INVALID, "adjoint: no matching forward pass");
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    const int nz = g.nz, k = A.k;
+    const long ps = g.plane;
+    // the recomputation must not overwrite the caller's captures or diagnostics
+    std::vector<Capture> caps;
+    caps.swap(c->captures);
+    double* checksums = c->checksums;
+    c->checksums = nullptr;
+    try {
+        for (double* a : A.padj[0]) HPQ_HIP(hipMemsetAsync(a, 0, sizeof(double) * p->n, c->stream));
+        int cur = 0;
+        for (int seg = A.nck - 1; seg >= 0; --seg) {
+            const int s_first = nz - 1 - seg * k;
+            const int s_last = std::max(0, s_first - k + 1);
+            HPQ_HIP(hipMemcpyAsync(store, A.ck_store[seg], store_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+            plasma_restore(c, A.ck_plasma[seg], p);
+            {
+                Hipace h(c, p, b, store);
+                for (int isl = s_first; isl >= s_last; --isl) {
+                    const int slot = s_first - isl;
+                    h.SolveOneSlice(isl, false, false, [&] {
+                        HPQ_HIP(hipMemcpyAsync(A.sl_store[slot], store, store_bytes(c), hipMemcpyDeviceToDevice,
+                                               c->stream));
+                        plasma_save(c, p, A.sl_plasma[slot]);
+                    });
+                }
+            }
+            for (int isl = s_last; isl <= s_first; ++isl) {
+                const int slot = s_first - isl;
+                const int nxt = 1 - cur;
+                const hpq_plasma sp = plasma_view(A.sl_plasma[slot]);
+                const double* st = A.sl_store[slot];
+                double* adj = A.store_adj;
+                HPQ_HIP(hipMemsetAsync(adj, 0, store_bytes(c), c->stream));
+                for (double* a : A.padj[nxt]) HPQ_HIP(hipMemsetAsync(a, 0, sizeof(double) * p->n, c->stream));
+                hpq_plasma_adj out = adj_view(A.padj[cur]), in = adj_view(A.padj[nxt]);
+                // push: adjoints of its inputs and of the Psi, Ez, Bx, By, Bz planes
+                advance_plasma_vjp(c, &sp, st, &out, &in, adj);
+                if (seed) adj_seed(c, seed + long(isl) * g.nx * g.ny, adj);
+                // (Lap - chi) B = S is self-adjoint for the node-centred operator:
+                // S_adj = mu with (Lap - chi) mu = B_adj, chi_adj = mu . B
+                int iters = 0;
+                const int st_mg = mg_solve1(c, adj + long(L.Sy) * ps, adj + long(L.Bx) * ps, st + long(L.chi) * ps,
+                                            ps, mg_tol, 0.0, 200, &iters);
+                if (st_mg != HPQ_OK) throw HpqError(st_mg, "adjoint: Bx/By adjoint solve did not converge");
+                adj_chi(c, st, adj);
+                explicit_deposition_vjp(c, &sp, st, adj, &in);
+                adj_beam_weights(c, b, isl, adj, w_adj);
+                adj_poisson(c, adj);
+                const int32_t comps[6] = {L.jx, L.jy, -1, -1, L.chi, L.rhomjz};
+                deposit_vjp(c, &sp, adj, comps, &in);
+                cur = nxt;
+            }
+        }
+    } catch (...) {
+        c->captures.swap(caps);
+        c->checksums = checksums;
+        throw;
+    }
+    c->captures.swap(caps);
+    c->checksums = checksums;
+}
+
+}  // namespace
+
 extern "C" {
 
 int hpq_create(const hpq_params* p, hpq_ctx** out) {
@@ -400,6 +607,7 @@ void hpq_destroy(hpq_ctx* c) {
     if (c->h_count) (void)hipHostFree(c->h_count);
     if (c->d_red) (void)hipFree(c->d_red);
     if (c->h_red) (void)hipHostFree(c->h_red);
+    adjoint_free(c);
     delete c;
 }
 
@@ -515,6 +723,27 @@ int hpq_advance_plasma_particles_vjp(hpq_ctx* c, const hpq_plasma* p, const doub
         check_adj(out_adj, true);
         check_adj(in_adj, true);
         advance_plasma_vjp(c, p, store, out_adj, in_adj, store_adj);
+    });
+}
+
+int hpq_adjoint_forward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store, int checkpoint_every) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_plasma(p);
+        if (checkpoint_every < 1) throw HpqError(HPQ_E_INVALID, "checkpoint_every must be >= 1");
+        adjoint_forward(c, p, b, store, std::min(checkpoint_every, c->g.nz));
+    });
+}
+
+int hpq_adjoint_backward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store, const double* ez_seed,
+                         double* beam_w_adj, double mg_tol) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_plasma(p);
+        if (!(mg_tol > 0.0)) throw HpqError(HPQ_E_INVALID, "mg_tol must be positive");
+        adjoint_backward(c, p, b, store, ez_seed, beam_w_adj, mg_tol);
     });
 }
 

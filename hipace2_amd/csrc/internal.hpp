@@ -154,6 +154,27 @@ struct Bins {
     int calls = 0;               // plasma depositions since the last re-sort
 };
 
+// Plasma SoA copy (11 FP64 arrays in PlasmaIdx order + valid) for checkpoints.
+constexpr int PLASMA_REALS = 11;
+struct PlasmaBuf {
+    double* r[PLASMA_REALS] = {};
+    int32_t* valid = nullptr;
+    long n = 0;
+};
+
+// State of the adjoint slice loop: checkpoints of the forward pass (slice store and plasma
+// every k slices), the per-slice saves of one recomputed segment, the adjoint planes and
+// two sets of plasma adjoints (the state entering and leaving a slice).
+struct AdjointState {
+    int k = 0, nck = 0;
+    long np = 0;
+    std::vector<double*> ck_store, sl_store;
+    std::vector<PlasmaBuf> ck_plasma, sl_plasma;
+    double* store_adj = nullptr;
+    double* padj[2][8] = {};
+    bool ready = false;
+};
+
 // Event pairs around the launches of one kernel family (HPQ_TRACE_*).
 struct Trace {
     bool on = false;
@@ -171,6 +192,7 @@ struct hpq_ctx {
     hpq::Poisson ps;
     hpq::MultiGrid mg;
     hpq::Bins bins;
+    hpq::AdjointState adj;
     int* d_count = nullptr;          // QSA counter
     int* h_count = nullptr;          // pinned
     double* d_red = nullptr;         // reduction scratch (8 doubles)
@@ -226,6 +248,11 @@ void deposit_vjp(hpq_ctx* c, const hpq_plasma* p, const double* adj, const int32
 void explicit_deposition_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, double* adj, hpq_plasma_adj* a);
 void advance_plasma_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, const hpq_plasma_adj* out,
                         hpq_plasma_adj* in, double* adj);
+void adj_poisson(hpq_ctx* c, double* adj);
+void adj_chi(hpq_ctx* c, const double* store, double* adj);
+void adj_seed(hpq_ctx* c, const double* seed, double* adj);
+void adj_beam_weights(hpq_ctx* c, const hpq_beam* b, int slice, double* adj, double* w_adj);
+void adjoint_free(hpq_ctx* c);
 
 inline dim3 grid1d(long n, int block) { return dim3(unsigned((n + block - 1) / block)); }
 

@@ -233,6 +233,20 @@ class Simulation:
             self.ctx, C.byref(self._pdesc), C.c_void_p(self.store.data_ptr()), C.byref(o), C.byref(i),
             C.c_void_p(store_adj.data_ptr())))
 
+    def adjoint_forward(self, checkpoint_every: int = 32):
+        """One step's slices with checkpoints for the backward march (after begin_step)."""
+        _abi.check(self.ctx, self.L.hpq_adjoint_forward(self.ctx, C.byref(self._pdesc), C.byref(self._bdesc),
+                                                        C.c_void_p(self.store.data_ptr()), int(checkpoint_every)))
+
+    def adjoint_backward(self, ez_seed: Optional[torch.Tensor], beam_w_adj: torch.Tensor, mg_tol: float = 1e-10):
+        """Adds d loss / d (beam weights) to beam_w_adj given d loss / d Ez ([slice][j][i])."""
+        if ez_seed is not None:
+            assert ez_seed.is_contiguous() and ez_seed.numel() == self.cfg.nz * self.cfg.ny * self.cfg.nx
+        seed = C.c_void_p(ez_seed.data_ptr()) if ez_seed is not None else None
+        _abi.check(self.ctx, self.L.hpq_adjoint_backward(self.ctx, C.byref(self._pdesc), C.byref(self._bdesc),
+                                                         C.c_void_p(self.store.data_ptr()), seed,
+                                                         C.c_void_p(beam_w_adj.data_ptr()), float(mg_tol)))
+
     def advance_plasma_particles(self, temp_slice: bool = False):
         _abi.check(self.ctx, self.L.hpq_advance_plasma_particles(self.ctx, C.byref(self._pdesc),
                                                                  C.c_void_p(self.store.data_ptr()),

@@ -171,6 +171,20 @@ int hpq_advance_plasma_particles_vjp(hpq_ctx *ctx, const hpq_plasma *p, const do
                                      const hpq_plasma_adj *out_adj, hpq_plasma_adj *in_adj,
                                      double *store_adj);
 
+/* Adjoint slice loop, step 1: after hpq_begin_step, runs the slices nz-1 .. 0 as
+ * hpq_run_slices does, keeping the slice store and the plasma every checkpoint_every slices
+ * (device memory held by the context). Explicit solver, odd nx and ny, do_beam_jx_jy = 0. */
+int hpq_adjoint_forward(hpq_ctx *ctx, const hpq_plasma *p, const hpq_beam *b, double *store,
+                        int checkpoint_every);
+
+/* Adjoint slice loop, step 2: for a loss L(Ez) with ez_seed = dL/dEz on every slice
+ * (nz*ny*nx doubles, [slice][j][i] over the valid box; null: zero), adds dL/dw of every
+ * beam macro-particle weight to beam_w_adj (b->n doubles). Marches the slices in reverse,
+ * recomputing each checkpoint segment; the plasma and the store are scratch afterwards.
+ * mg_tol: relative tolerance of the adjoint Bx/By solves. */
+int hpq_adjoint_backward(hpq_ctx *ctx, const hpq_plasma *p, const hpq_beam *b, double *store,
+                         const double *ez_seed, double *beam_w_adj, double mg_tol);
+
 /* FFTPoissonSolverDirichlet*::SolvePoissonEquation
  * (src/fields/fft_poisson_solver/FFTPoissonSolverDirichletDirect.cpp:101-139,
  * ...DirichletFast.cpp:291-328): solves Lap(lhs) = staging with zero Dirichlet values

@@ -152,3 +152,69 @@ def test_advance_plasma_particles_vjp(order):
     vjp = sum(float((got[k].cpu().numpy() * delta[k]).sum()) for k in keys)
     vjp += sum(float((sa[sim.layout[c]] * dF[c]).sum()) for c in fields)
     _close(vjp, fd)
+
+
+# ---- the adjoint slice loop --------------------------------------------------------------
+
+def _adjoint_sim(n=63, nz=40, peak=0.5, nbeam=20000, seed=0):
+    from hipace2_amd.inputs import BeamSpec, make_beam
+    from hipace2_amd.simulation import Simulation
+    cfg = small_config(n=n, nz=nz, order=2, explicit=True)
+    cfg.do_beam_jx_jy = False
+    # the gradient is that of the converged Bx/By solve: converge the forward solves too
+    cfg.mg_tol_rel = 1e-13
+    spec = BeamSpec(injection_type="fixed_weight", profile="gaussian", density=peak,
+                    position_mean=(0.0, 0.0, 0.0), position_std=(0.3, 0.3, 1.41),
+                    num_particles=nbeam, u_mean=(0.0, 0.0, 1000.0), seed=seed)
+    beam = make_beam(cfg, spec)
+    return cfg, beam, Simulation(cfg, beam)
+
+
+def _ez_of(sim, w):
+    sim.beam["w"][: w.numel()].copy_(w)
+    ez = sim.capture("Ez")
+    sim.begin_step()
+    sim.run_slices()
+    torch.cuda.synchronize()
+    out = ez.clone()
+    sim.clear_captures()
+    return out
+
+
+def _losses(cfg):
+    rng = np.random.default_rng(21)
+    c = torch.as_tensor(rng.normal(size=(cfg.nz, cfg.ny, cfg.nx))).cuda()
+    witness = slice(cfg.nz // 4, cfg.nz // 2)
+    from hipace2_amd.adjoint import on_axis_ez_variance
+    return {"linear": lambda ez: (c * ez).sum(),
+            "flatness": lambda ez: on_axis_ez_variance(ez, witness) * 1e4}
+
+
+@pytest.mark.parametrize("loss", ["linear", "flatness"])
+def test_adjoint_loop_matches_central_differences(loss):
+    from hipace2_amd.adjoint import slice_loop_ez
+    cfg, beam, sim = _adjoint_sim()
+    L = _losses(cfg)[loss]
+    w0 = sim.beam["w"][: beam.n].clone()
+    delta = w0 * torch.as_tensor(np.random.default_rng(22).normal(size=beam.n)).cuda()
+    eps = 1e-3
+    fd = (float(L(_ez_of(sim, w0 + eps * delta))) - float(L(_ez_of(sim, w0 - eps * delta)))) / (2 * eps)
+    w = w0.clone().requires_grad_(True)
+    ez = slice_loop_ez(w, sim, checkpoint_every=7)
+    L(ez).backward()
+    dot = float((w.grad * delta).sum())
+    print(loss, "adjoint", dot, "central difference", fd)
+    assert abs(dot - fd) <= 1e-4 * abs(fd), (dot, fd)
+
+
+def test_adjoint_loop_checkpoint_period_invariance():
+    from hipace2_amd.adjoint import slice_loop_ez
+    cfg, beam, sim = _adjoint_sim(nz=24)
+    L = _losses(cfg)["linear"]
+    grads = []
+    for k in (1, 5, 24):
+        w = sim.beam["w"][: beam.n].clone().requires_grad_(True)
+        L(slice_loop_ez(w, sim, checkpoint_every=k)).backward()
+        grads.append(w.grad.clone())
+    for g in grads[1:]:
+        assert float((g - grads[0]).abs().max()) <= 1e-9 * float(grads[0].abs().max())
