@@ -153,6 +153,87 @@ def dominant_kind(cfg, n_plasma, traces):
     return max(bounded, key=bounded.get)
 
 
+def run_adjoint(args):
+    """Beam-loading gradient: 511 x 511 x 512 slices, 1 x 1 ppc, drive as in the throughput
+    workload (1e5 particles), witness of 1e6 particles uniform in zeta on [-6, -4] whose
+    current profile is 64 piecewise-linear knots (seeded Uniform[0.5, 1.5]); loss = variance
+    of on-axis Ez over the witness slices; checkpoint every 32 slices. Times the plain forward
+    step, the checkpointed forward and the backward march, and checks the gradient against
+    central differences on `adjoint_fd_knots` knots. Multigrid tolerance 1e-10 in both
+    directions, so the loss is a smooth function of the knots."""
+    import numpy as np
+    import torch
+    from hipace2_amd.adjoint import BeamLoading, on_axis_ez_variance, slice_loop_ez, witness_beam
+    from hipace2_amd.inputs import BeamSpec, SliceConfig, make_beam
+    from hipace2_amd.simulation import Simulation
+
+    n, nz = args.n, args.nz
+    cfg = SliceConfig(nx=n, ny=n, nz=nz, xlo=-8.0, xhi=8.0, ylo=-8.0, yhi=8.0, zlo=-8.0, zhi=4.0,
+                      depos_order=2, derivative_type=2, explicit_solver=True, ppc=(1, 1))
+    cfg.do_beam_jx_jy = False
+    cfg.mg_tol_rel = 1e-10
+    drive = make_beam(cfg, BeamSpec(injection_type="fixed_weight", profile="gaussian", density=0.01,
+                                    position_std=(0.3, 0.3, 1.41), num_particles=100_000,
+                                    u_mean=(0.0, 0.0, 1000.0), seed=1234))
+    wit = witness_beam(cfg, 1_000_000, -6.0, -4.0, 0.2, 0.005, 1000.0, seed=4321)
+    bl = BeamLoading(cfg, drive, wit, -6.0, -4.0)
+    sim = Simulation(cfg, bl.beam)
+    knots0 = torch.as_tensor(np.random.default_rng(99).uniform(0.5, 1.5, 64), device="cuda")
+    wsl = bl.witness_slices()
+
+    def loss_of(ez):
+        return on_axis_ez_variance(ez, wsl)
+
+    def plain(knots):
+        w = bl.weights(knots)
+        sim.beam["w"][: w.numel()].copy_(w)
+        ez = sim.capture("Ez")
+        sim.begin_step()
+        sim.run_slices()
+        torch.cuda.synchronize()
+        out = float(loss_of(ez))
+        sim.clear_captures()
+        return out
+
+    plain(knots0)   # warm-up
+    t0 = time.perf_counter()
+    plain(knots0)
+    t_plain = time.perf_counter() - t0
+
+    knots = knots0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ez = slice_loop_ez(bl.weights(knots), sim, checkpoint_every=32, mg_tol=1e-9)
+    loss = loss_of(ez)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    grad = knots.grad.detach().cpu().numpy()
+
+    picks = np.linspace(0, 63, args.adjoint_fd_knots).round().astype(int) if args.adjoint_fd_knots else []
+    fd = []
+    for k in picks:
+        h = 1e-3 * float(knots0[k])
+        e = torch.zeros_like(knots0)
+        e[k] = h
+        fd.append((plain(knots0 + e) - plain(knots0 - e)) / (2 * h))
+    fd = np.array(fd)
+    g = grad[picks] if len(picks) else np.zeros(0)
+    scale = float(np.abs(fd).max()) if fd.size else 0.0
+    err = float(np.abs(g - fd).max() / scale) if scale > 0 else None
+    print(f"adjoint: plain {t_plain:.3f} s, checkpointed forward {t_fwd:.3f} s, backward {t_bwd:.3f} s, "
+          f"FD max rel err {err}", file=sys.stderr, flush=True)
+    return {"workload": f"{n}x{n}x{nz}, 1x1 ppc, drive 1e5 + witness 1e6 particles, 64 knots, loss Var(on-axis Ez) "
+                        "over the witness slices, checkpoint every 32 slices, MG tolerance 1e-10 (adjoint solves 1e-9, warm-started), FP64",
+            "loss": float(loss), "plain_forward_s": round(t_plain, 4), "checkpointed_forward_s": round(t_fwd, 4),
+            "backward_s": round(t_bwd, 4), "ratio_to_forward": round((t_fwd + t_bwd) / t_plain, 3),
+            "fd_knots": [int(k) for k in picks], "fd_max_rel_err": err,
+            "grad_norm": float(np.linalg.norm(grad))}
+
+
 def cpu_baseline(args, cfg, beam, sim):
     """The oracle on the host cores, started from the GPU state at slice s0 (inside the beam)."""
     import numpy as np
@@ -205,6 +286,9 @@ def main():
                     help="arithmetic path(s) to time; 'both' also reports the mixed path's error")
     ap.add_argument("--mixed-poisson", choices=["bf16x3", "fft"], default="bf16x3",
                     help="transverse Poisson solver of the mixed path")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the beam-loading gradient measurement")
+    ap.add_argument("--adjoint-fd-knots", type=int, default=8,
+                    help="knots checked against central differences (two forward steps each)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device to device, the reference dispatch for HBM byte counters")
     args = ap.parse_args()
@@ -269,6 +353,10 @@ def main():
         sim_cpu = runs["fp64"]["sim"] if "fp64" in runs else sim
         cpu = cpu_baseline(args, cfg, beam, sim_cpu)
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = run_adjoint(args)
+
     line = {
         "metric": "slices/s + plasma pushes/s",
         "value": round(slices_per_s, 2),
@@ -292,6 +380,7 @@ def main():
         **extra,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(line))
 

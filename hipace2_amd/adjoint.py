@@ -64,3 +64,60 @@ def on_axis_ez_variance(ez: torch.Tensor, slices: slice) -> torch.Tensor:
     ny, nx = ez.shape[1], ez.shape[2]
     v = ez[slices, ny // 2, nx // 2]
     return ((v - v.mean()) ** 2).mean()
+
+
+def witness_beam(cfg, num_particles: int, z0: float, z1: float, sigma_r: float, n_peak: float,
+                 uz: float, seed: int = 7):
+    """Equal-weight witness macro-particles, uniform in zeta on [z0, z1] and Gaussian in x, y,
+    of peak density n_peak, sorted into zeta-slices like the drive (weight as
+    make_fixed_weight_beam: density times volume over dx dy dz N)."""
+    from .inputs import BeamArrays
+    rng = np.random.Generator(np.random.Philox(seed))
+    n = int(num_particles)
+    z = rng.uniform(z0, z1, n)
+    x = rng.normal(0.0, sigma_r, n)
+    y = rng.normal(0.0, sigma_r, n)
+    total = n_peak * (sigma_r * np.sqrt(2.0 * np.pi)) ** 2 * (z1 - z0) / (cfg.dx * cfg.dy * cfg.dz)
+    w = np.full(n, total / n)
+    box = np.floor((z - cfg.zlo) / cfg.dz).astype(np.int64)
+    inside = (box >= 0) & (box < cfg.nz)
+    order = np.argsort(np.where(inside, box, cfg.nz), kind="stable")
+    order = order[inside[order]]
+    off = np.zeros(cfg.nz + 1, np.int64)
+    off[1:] = np.cumsum(np.bincount(box[inside], minlength=cfg.nz))
+    zeros = np.zeros(order.size)
+    return BeamArrays(x[order], y[order], z[order], zeros, zeros.copy(), np.full(order.size, uz), w[order],
+                      np.ones(order.size, np.int32), off)
+
+
+class BeamLoading:
+    """Drive + witness beam whose witness current profile is piecewise linear through knots:
+    the witness weights are w0 * knot_profile(zeta; knots). Maps knots to the weights of the
+    merged, slice-sorted beam (the drive's weights stay fixed)."""
+
+    def __init__(self, cfg, drive, witness, z0: float, z1: float):
+        from .inputs import merge_beams
+        self.beam = merge_beams(cfg, [drive, witness])
+        self.z0, self.z1 = z0, z1
+        idx = []
+        for s in range(cfg.nz):
+            start = int(self.beam.off[s]) + int(drive.off[s + 1] - drive.off[s])
+            idx.append(np.arange(start, start + int(witness.off[s + 1] - witness.off[s])))
+        self.witness_index = np.concatenate(idx) if idx else np.zeros(0, np.int64)
+        self.witness_z = witness.z
+        self.witness_w0 = witness.w
+        self.base_w = self.beam.w.copy()
+        self.base_w[self.witness_index] = 0.0
+        self.s0 = int(np.floor((z0 - cfg.zlo) / cfg.dz))
+        self.s1 = int(np.floor((z1 - cfg.zlo) / cfg.dz))
+
+    def weights(self, knots: torch.Tensor) -> torch.Tensor:
+        dev = knots.device
+        base = torch.as_tensor(self.base_w, device=dev)
+        zw = torch.as_tensor(self.witness_z, device=dev)
+        w0 = torch.as_tensor(self.witness_w0, device=dev)
+        idx = torch.as_tensor(self.witness_index, device=dev)
+        return base.index_put((idx,), w0 * knot_profile(zw, knots, self.z0, self.z1))
+
+    def witness_slices(self) -> slice:
+        return slice(max(self.s0, 0), self.s1 + 1)

@@ -138,9 +138,10 @@ __global__ __launch_bounds__(VT) void k_explicit_vjp(Geom g, hpq_plasma p, hpq_p
 
 // ---- AdvancePlasmaParticles (PlasmaParticleAdvance.cpp:84-240), not a temporary slice -----
 // The push of one particle as a function of 11 inputs: x_prev, y_prev, ux_half_step,
-// uy_half_step, psi_half_step and the six gathered field values. One launch evaluates the
-// tangents of NB of them, starting at input k0: 11 inputs in three launches keeps the
-// tangent arithmetic in registers.
+// uy_half_step, psi_half_step and the six gathered field values G (x_prev, y_prev also
+// enter through G; that part is taken by the gather kernel below). One launch evaluates
+// the tangents of NB of the inputs, starting at input k0: 11 inputs in three launches keeps
+// the tangent arithmetic in registers. The adjoints of G go to a scratch array.
 constexpr int PUSH_IN = 11;
 
 template <int N>
@@ -184,15 +185,15 @@ __device__ PushOut<N> push_tan(const Geom& g, const hpq_plasma& p, long ip, cons
     const double clight = 1.0, clight_inv = 1.0 / 1.0;
     const double dz = g.dz / 1;
     T xp = seed(p.x_prev[ip], 0), yp = seed(p.y_prev[ip], 1);
-    T G[6];   // ExmBy, EypBx, Ez, Bx, By, Bz at the particle
+    double Gv[6] = {0, 0, 0, 0, 0, 0};   // ExmBy, EypBx, Ez, Bx, By, Bz at the particle
     {
-        const T xm = (xp - g.xoff) * g.dxi, ym = (yp - g.yoff) * g.dyi;
+        const double xm = (p.x_prev[ip] - g.xoff) * g.dxi, ym = (p.y_prev[ip] - g.yoff) * g.dyi;
         constexpr int W = O + 2;
-        DsfR<T> sx[W], sy[W];
+        Dsf sx[W], sy[W];
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            sx[k] = dshape<1, O, T, T>(xm, k);
-            sy[k] = dshape<1, O, T, T>(ym, k);
+            sx[k] = dshape<1, O>(xm, k);
+            sy[k] = dshape<1, O>(ym, k);
         }
         const double* Psi = store + long(L.Psi) * g.plane;
         const double* Ez = store + long(L.Ez) * g.plane;
@@ -203,24 +204,24 @@ __device__ PushOut<N> push_tan(const Geom& g, const hpq_plasma& p, long ip, cons
         for (int iy = 0; iy < W; ++iy)
 #pragma unroll
             for (int ix = 0; ix < W; ++ix) {
-                const DsfR<T>& X = sx[ix];
-                const DsfR<T>& Y = sy[iy];
+                const Dsf& X = sx[ix];
+                const Dsf& Y = sy[iy];
                 if (!in_store(g, X.cell, Y.cell)) continue;
                 const long k = g.idx(X.cell, Y.cell);
-                G[0] += (X.ds * Y.s) * Psi[k] * g.dxi;
-                G[1] += (X.s * Y.ds) * Psi[k] * g.dyi;
-                const T ss = X.s * Y.s;
-                G[2] += ss * Ez[k];
-                G[3] += ss * Bx[k];
-                G[4] += ss * By[k];
-                G[5] += ss * Bz[k];
+                Gv[0] += (X.ds * Y.s) * Psi[k] * g.dxi;
+                Gv[1] += (X.s * Y.ds) * Psi[k] * g.dyi;
+                const double ss = X.s * Y.s;
+                Gv[2] += ss * Ez[k];
+                Gv[3] += ss * Bx[k];
+                Gv[4] += ss * By[k];
+                Gv[5] += ss * Bz[k];
             }
     }
-    G[3] = G[3] * clight;
-    G[4] = G[4] * clight;
+    Gv[3] *= clight;
+    Gv[4] *= clight;
+    T G[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q)
-        if (5 + q >= k0 && 5 + q < k0 + N) G[q].d[5 + q - k0] += 1.0;
+    for (int q = 0; q < 6; ++q) G[q] = seed(Gv[q], 5 + q);
     constexpr int nsub = 4;
     const double sdz = dz / nsub;
     T ux = seed(p.ux_half_step[ip], 2), uy = seed(p.uy_half_step[ip], 3), psi = seed(p.psi_half_step[ip], 4);
@@ -236,6 +237,7 @@ __device__ PushOut<N> push_tan(const Geom& g, const hpq_plasma& p, long ip, cons
         uy = uy + sdz * duy + 0.5 * sdz * sdz * dduy.e;
         psi = psi + sdz * dpsi + 0.5 * sdz * sdz * ddpsi.e;
     };
+#pragma unroll 1
     for (int s = 0; s < nsub; ++s) substep();
     xp = xp + dz * clight_inv * (ux * (1.0 / psi));
     yp = yp + dz * clight_inv * (uy * (1.0 / psi));
@@ -246,6 +248,7 @@ __device__ PushOut<N> push_tan(const Geom& g, const hpq_plasma& p, long ip, cons
     o.uxh = ux;
     o.uyh = uy;
     o.psih = psi;
+#pragma unroll 1
     for (int s = 0; s < nsub / 2; ++s) substep();
     o.ux = ux;
     o.uy = uy;
@@ -256,54 +259,81 @@ __device__ PushOut<N> push_tan(const Geom& g, const hpq_plasma& p, long ip, cons
 template <int O, int N>
 __global__ __launch_bounds__(VT) void k_push_vjp(Geom g, hpq_plasma p, const double* __restrict__ store, Layout L,
                                                  double qmc, BC bc, hpq_plasma_adj out, hpq_plasma_adj in,
-                                                 double* __restrict__ adj, int k0) {
+                                                 double* __restrict__ gadj, int k0) {
     const long ip = long(blockIdx.x) * VT + threadIdx.x;
     if (ip >= p.n || !p.valid[ip]) return;
     const PushOut<N> o = push_tan<O, N>(g, p, ip, store, L, qmc, bc, k0);
-    if (o.removed) return;   // absorbed: nothing after this slice depends on the particle
     double lam[N];
-    const double lx = out.x[ip], ly = out.y[ip], luxh = out.ux_half_step[ip], luyh = out.uy_half_step[ip],
-                 lph = out.psi_half_step[ip], lux = out.ux[ip], luy = out.uy[ip], lps = out.psi[ip];
+    if (o.removed) {   // absorbed: nothing after this slice depends on the particle
 #pragma unroll
-    for (int k = 0; k < N; ++k)
-        lam[k] = lx * o.x.d[k] + ly * o.y.d[k] + luxh * o.uxh.d[k] + luyh * o.uyh.d[k] + lph * o.psih.d[k] +
-                 lux * o.ux.d[k] + luy * o.uy.d[k] + lps * o.psi.d[k];
+        for (int k = 0; k < N; ++k) lam[k] = 0.0;
+    } else {
+        const double lx = out.x[ip], ly = out.y[ip], luxh = out.ux_half_step[ip], luyh = out.uy_half_step[ip],
+                     lph = out.psi_half_step[ip], lux = out.ux[ip], luy = out.uy[ip], lps = out.psi[ip];
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            lam[k] = lx * o.x.d[k] + ly * o.y.d[k] + luxh * o.uxh.d[k] + luyh * o.uyh.d[k] + lph * o.psih.d[k] +
+                     lux * o.ux.d[k] + luy * o.uy.d[k] + lps * o.psi.d[k];
+    }
     double* dst[5] = {in.x, in.y, in.ux_half_step, in.uy_half_step, in.psi_half_step};
-    double lg[6] = {0, 0, 0, 0, 0, 0};
-    bool any_g = false;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const int input = k0 + k;
         if (input < 5) dst[input][ip] += lam[k];
-        else if (input < PUSH_IN) { lg[input - 5] = lam[k]; any_g = true; }
+        else if (input < PUSH_IN) gadj[long(input - 5) * p.n + ip] = lam[k];
     }
-    if (!any_g) return;
-    // gathered values are linear in the cell values: scatter their adjoints with the
-    // forward's weights (B*c scaled by c)
+}
+
+// The gathered values G are linear in the cell values and depend on (x_prev, y_prev)
+// through the shape factors: scatter the G adjoints with the forward's weights and add
+// their position part to the x_prev, y_prev adjoints (B*c scaled by c).
+template <int O>
+__global__ __launch_bounds__(VT) void k_push_vjp_gather(Geom g, hpq_plasma p, const double* __restrict__ store,
+                                                        Layout L, const double* __restrict__ gadj, hpq_plasma_adj in,
+                                                        double* __restrict__ adj) {
+    using T = Tan<2>;
+    const long ip = long(blockIdx.x) * VT + threadIdx.x;
+    if (ip >= p.n || !p.valid[ip]) return;
+    double lg[6];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        lg[q] = gadj[long(q) * p.n + ip];
+        any = any || lg[q] != 0.0;
+    }
+    if (!any) return;
     const double clight = 1.0;
-    const double xm = (p.x_prev[ip] - g.xoff) * g.dxi, ym = (p.y_prev[ip] - g.yoff) * g.dyi;
+    const T xm = (T::seed(p.x_prev[ip], 0) - g.xoff) * g.dxi, ym = (T::seed(p.y_prev[ip], 1) - g.yoff) * g.dyi;
     constexpr int W = O + 2;
-    Dsf sx[W], sy[W];
+    DsfR<T> sx[W], sy[W];
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        sx[k] = dshape<1, O>(xm, k);
-        sy[k] = dshape<1, O>(ym, k);
+        sx[k] = dshape<1, O, T, T>(xm, k);
+        sy[k] = dshape<1, O, T, T>(ym, k);
     }
+    const double* F[5] = {store + long(L.Psi) * g.plane, store + long(L.Ez) * g.plane, store + long(L.Bx) * g.plane,
+                          store + long(L.By) * g.plane, store + long(L.Bz) * g.plane};
+    T acc(0.0);
 #pragma unroll
     for (int iy = 0; iy < W; ++iy)
 #pragma unroll
         for (int ix = 0; ix < W; ++ix) {
-            const Dsf X = sx[ix], Y = sy[iy];
+            const DsfR<T>& X = sx[ix];
+            const DsfR<T>& Y = sy[iy];
             if (!in_store(g, X.cell, Y.cell)) continue;
             const long k = g.idx(X.cell, Y.cell);
-            const double ss = X.s * Y.s;
-            const double psi_a = lg[0] * (X.ds * Y.s) * g.dxi + lg[1] * (X.s * Y.ds) * g.dyi;
-            if (psi_a != 0.0) atomic_add(adj + long(L.Psi) * g.plane + k, psi_a);
-            if (lg[2] != 0.0) atomic_add(adj + long(L.Ez) * g.plane + k, lg[2] * ss);
-            if (lg[3] != 0.0) atomic_add(adj + long(L.Bx) * g.plane + k, lg[3] * clight * ss);
-            if (lg[4] != 0.0) atomic_add(adj + long(L.By) * g.plane + k, lg[4] * clight * ss);
-            if (lg[5] != 0.0) atomic_add(adj + long(L.Bz) * g.plane + k, lg[5] * ss);
+            const T ss = X.s * Y.s;
+            const T wpsi = lg[0] * (X.ds * Y.s) * g.dxi + lg[1] * (X.s * Y.ds) * g.dyi;
+            const T wez = lg[2] * ss, wbx = lg[3] * clight * ss, wby = lg[4] * clight * ss, wbz = lg[5] * ss;
+            acc += wpsi * F[0][k] + wez * F[1][k] + wbx * F[2][k] + wby * F[3][k] + wbz * F[4][k];
+            if (wpsi.v != 0.0) atomic_add(adj + long(L.Psi) * g.plane + k, wpsi.v);
+            if (wez.v != 0.0) atomic_add(adj + long(L.Ez) * g.plane + k, wez.v);
+            if (wbx.v != 0.0) atomic_add(adj + long(L.Bx) * g.plane + k, wbx.v);
+            if (wby.v != 0.0) atomic_add(adj + long(L.By) * g.plane + k, wby.v);
+            if (wbz.v != 0.0) atomic_add(adj + long(L.Bz) * g.plane + k, wbz.v);
         }
+    in.x[ip] += acc.d[0];
+    in.y[ip] += acc.d[1];
 }
 
 }  // namespace
@@ -346,11 +376,21 @@ void advance_plasma_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, co
     if (p->n <= 0) return;
     const double qmc = c->p.plasma_charge / (c->p.plasma_mass * 1.0);
     BC bc{c->p.particle_bc, c->p.plo[0], c->p.plo[1], c->p.phi[0], c->p.phi[1]};
-    constexpr int NB = 4;
+    constexpr int NB = 3;
+    AdjointState& A = c->adj;
+    if (A.push_g_cap < 6 * p->n) {
+        if (A.push_g) (void)hipFree(A.push_g);
+        A.push_g = nullptr;
+        HPQ_HIP(hipMalloc(&A.push_g, sizeof(double) * 6 * p->n));
+        A.push_g_cap = 6 * p->n;
+    }
     dispatch_order(c->p.depos_order, [&](auto O) {
+        constexpr int o = decltype(O)::value;
         for (int k0 = 0; k0 < PUSH_IN; k0 += NB)
-            hipLaunchKernelGGL((k_push_vjp<decltype(O)::value, NB>), grid1d(p->n, VT), dim3(VT), 0, c->stream, c->g,
-                               *p, store, c->L, qmc, bc, *out, *in, adj, k0);
+            hipLaunchKernelGGL((k_push_vjp<o, NB>), grid1d(p->n, VT), dim3(VT), 0, c->stream, c->g, *p, store, c->L,
+                               qmc, bc, *out, *in, A.push_g, k0);
+        hipLaunchKernelGGL((k_push_vjp_gather<o>), grid1d(p->n, VT), dim3(VT), 0, c->stream, c->g, *p, store, c->L,
+                           A.push_g, *in, adj);
     });
     HPQ_HIP(hipGetLastError());
 }

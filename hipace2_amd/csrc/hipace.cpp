@@ -420,6 +420,7 @@ void adjoint_alloc(hpq_ctx* c, int k, long np) {
         plasma_buf_alloc(A.sl_plasma[i], np);
     }
     HPQ_HIP(hipMalloc(&A.store_adj, store_bytes(c)));
+    HPQ_HIP(hipMalloc(&A.mu, 2 * sizeof(double) * c->g.plane));
     for (auto& set : A.padj)
         for (double*& a : set) HPQ_HIP(hipMalloc(&a, sizeof(double) * std::max(np, 1L)));
 }
@@ -438,6 +439,8 @@ void hpq::adjoint_free(hpq_ctx* c) {
     for (auto& set : A.padj)
         for (double* a : set)
             if (a) (void)hipFree(a);
+    if (A.push_g) (void)hipFree(A.push_g);
+    if (A.mu) (void)hipFree(A.mu);
     A = AdjointState{};
 }
 
@@ -488,6 +491,7 @@ void adjoint_backward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double
     c->checksums = nullptr;
     try {
         for (double* a : A.padj[0]) HPQ_HIP(hipMemsetAsync(a, 0, sizeof(double) * p->n, c->stream));
+        HPQ_HIP(hipMemsetAsync(A.mu, 0, 2 * sizeof(double) * ps, c->stream));
         int cur = 0;
         for (int seg = A.nck - 1; seg >= 0; --seg) {
             const int s_first = nz - 1 - seg * k;
@@ -518,11 +522,17 @@ void adjoint_backward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double
                 advance_plasma_vjp(c, &sp, st, &out, &in, adj);
                 if (seed) adj_seed(c, seed + long(isl) * g.nx * g.ny, adj);
                 // (Lap - chi) B = S is self-adjoint for the node-centred operator:
-                // S_adj = mu with (Lap - chi) mu = B_adj, chi_adj = mu . B
+                // S_adj = mu with (Lap - chi) mu = B_adj, chi_adj = mu . B. The previous
+                // slice's mu starts the iteration, as the previous B does in the forward.
+                if (L.Sx != L.Sy + 1 || L.By != L.Bx + 1) throw HpqError(HPQ_E_INVALID, "adjoint: layout");
+                HPQ_HIP(hipMemcpyAsync(adj + long(L.Sy) * ps, A.mu, 2 * sizeof(double) * ps, hipMemcpyDeviceToDevice,
+                                       c->stream));
                 int iters = 0;
                 const int st_mg = mg_solve1(c, adj + long(L.Sy) * ps, adj + long(L.Bx) * ps, st + long(L.chi) * ps,
                                             ps, mg_tol, 0.0, 200, &iters);
                 if (st_mg != HPQ_OK) throw HpqError(st_mg, "adjoint: Bx/By adjoint solve did not converge");
+                HPQ_HIP(hipMemcpyAsync(A.mu, adj + long(L.Sy) * ps, 2 * sizeof(double) * ps, hipMemcpyDeviceToDevice,
+                                       c->stream));
                 adj_chi(c, st, adj);
                 explicit_deposition_vjp(c, &sp, st, adj, &in);
                 adj_beam_weights(c, b, isl, adj, w_adj);

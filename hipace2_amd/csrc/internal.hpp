@@ -172,6 +172,9 @@ struct AdjointState {
     std::vector<PlasmaBuf> ck_plasma, sl_plasma;
     double* store_adj = nullptr;
     double* padj[2][8] = {};
+    double* mu = nullptr;        // Bx/By adjoint solution of the previous slice (warm start)
+    double* push_g = nullptr;    // adjoints of the six gathered push fields per particle
+    long push_g_cap = 0;
     bool ready = false;
 };
 
