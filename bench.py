@@ -20,6 +20,7 @@ region) and the oracle timed on the host cores on a bounded sample of the same w
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -213,6 +214,20 @@ def run_adjoint(args):
     t_bwd = time.perf_counter() - t0
     grad = knots.grad.detach().cpu().numpy()
 
+    # the same gradient with the forward (and its recomputation) on the mixed path: FP32
+    # particle arithmetic and split-BF16 DST; the vector-Jacobian products stay FP64
+    cfg_m = dataclasses.replace(cfg, precision="mixed", poisson_solver="bf16x3")
+    sim_m = Simulation(cfg_m, bl.beam)
+    knots_m = knots0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss_of(slice_loop_ez(bl.weights(knots_m), sim_m, checkpoint_every=32, mg_tol=1e-9)).backward()
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    grad_m = knots_m.grad.detach().cpu().numpy()
+    mixed_err = float(np.abs(grad_m - grad).max() / np.abs(grad).max())
+    del sim_m
+
     picks = np.linspace(0, 63, args.adjoint_fd_knots).round().astype(int) if args.adjoint_fd_knots else []
     fd = []
     for k in picks:
@@ -225,12 +240,13 @@ def run_adjoint(args):
     scale = float(np.abs(fd).max()) if fd.size else 0.0
     err = float(np.abs(g - fd).max() / scale) if scale > 0 else None
     print(f"adjoint: plain {t_plain:.3f} s, checkpointed forward {t_fwd:.3f} s, backward {t_bwd:.3f} s, "
-          f"FD max rel err {err}", file=sys.stderr, flush=True)
+          f"FD max rel err {err}, mixed-forward gradient vs FP64 {mixed_err:.3e}", file=sys.stderr, flush=True)
     return {"workload": f"{n}x{n}x{nz}, 1x1 ppc, drive 1e5 + witness 1e6 particles, 64 knots, loss Var(on-axis Ez) "
                         "over the witness slices, checkpoint every 32 slices, MG tolerance 1e-10 (adjoint solves 1e-9, warm-started), FP64",
             "loss": float(loss), "plain_forward_s": round(t_plain, 4), "checkpointed_forward_s": round(t_fwd, 4),
             "backward_s": round(t_bwd, 4), "ratio_to_forward": round((t_fwd + t_bwd) / t_plain, 3),
             "fd_knots": [int(k) for k in picks], "fd_max_rel_err": err,
+            "mixed_forward_grad_rel_err_vs_fp64": mixed_err, "mixed_forward_plus_backward_s": round(t_mixed, 4),
             "grad_norm": float(np.linalg.norm(grad))}
 
 
