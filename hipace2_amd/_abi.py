@@ -27,7 +27,8 @@ EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq
            "hpq_init_plasma", "hpq_begin_step", "hpq_solve_one_slice", "hpq_run_slices", "hpq_set_checksums",
            "hpq_add_capture", "hpq_clear_captures", "hpq_stats", "hpq_trace_enable", "hpq_trace_read",
            "hpq_trace_reset", "hpq_bin_tiles", "hpq_deposit_current_vjp", "hpq_explicit_deposition_vjp",
-           "hpq_advance_plasma_particles_vjp", "hpq_adjoint_forward", "hpq_adjoint_backward"]
+           "hpq_advance_plasma_particles_vjp", "hpq_adjoint_forward", "hpq_adjoint_backward",
+           "hpq_beam_begin_step", "hpq_advance_beam_slice"]
 
 TRACE = {"push": 0, "deposit": 1, "explicit_deposit": 2, "beam_deposit": 3, "mg_smooth": 4, "mg_coarse": 5,
          "dst": 6}
@@ -55,6 +56,11 @@ class Plasma(C.Structure):
 
 
 ADJ_NAMES = ["x", "y", "ux", "uy", "psi", "ux_half_step", "uy_half_step", "psi_half_step"]
+
+
+class BeamState(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("x", "y", "z", "ux", "uy", "uz", "w", "valid", "nsubcycles")] + \
+               [("off", C.POINTER(C.c_int64)), ("n", C.c_int64), ("charge", C.c_double), ("mass", C.c_double)]
 
 
 class PlasmaAdj(C.Structure):
@@ -125,6 +131,8 @@ def lib():
         L.hpq_advance_plasma_particles_vjp.argtypes = [vp, C.POINTER(Plasma), vp, C.POINTER(PlasmaAdj),
                                                        C.POINTER(PlasmaAdj), vp]
         L.hpq_adjoint_forward.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, ip]
+        L.hpq_beam_begin_step.argtypes = [vp, C.POINTER(BeamState)]
+        L.hpq_advance_beam_slice.argtypes = [vp, C.POINTER(BeamState), ip, vp, C.c_double, ip, ip]
         L.hpq_adjoint_backward.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, vp, vp, C.c_double]
         _lib = L
     return _lib

@@ -618,6 +618,7 @@ void hpq_destroy(hpq_ctx* c) {
     if (c->d_red) (void)hipFree(c->d_red);
     if (c->h_red) (void)hipHostFree(c->h_red);
     adjoint_free(c);
+    beam_slip_free(c);
     delete c;
 }
 
@@ -733,6 +734,35 @@ int hpq_advance_plasma_particles_vjp(hpq_ctx* c, const hpq_plasma* p, const doub
         check_adj(out_adj, true);
         check_adj(in_adj, true);
         advance_plasma_vjp(c, p, store, out_adj, in_adj, store_adj);
+    });
+}
+
+static void check_beam_state(const hpq_ctx* c, const hpq_beam_state* b) {
+    if (!b) throw HpqError(HPQ_E_INVALID, "beam state is null");
+    if (b->n > 0 && (!b->x || !b->y || !b->z || !b->ux || !b->uy || !b->uz || !b->w || !b->valid || !b->nsubcycles))
+        throw HpqError(HPQ_E_INVALID, "beam state has a null array");
+    if (!b->off) throw HpqError(HPQ_E_INVALID, "beam state has no slice offsets");
+    if (b->off[0] != 0 || b->off[c->g.nz] != b->n) throw HpqError(HPQ_E_INVALID, "beam slice offsets do not cover n");
+    if (!(b->mass != 0.0)) throw HpqError(HPQ_E_INVALID, "beam mass is zero");
+}
+
+int hpq_beam_begin_step(hpq_ctx* c, hpq_beam_state* b) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_beam_state(c, b);
+        beam_begin_step(c, b);
+    });
+}
+
+int hpq_advance_beam_slice(hpq_ctx* c, hpq_beam_state* b, int slice, const double* store, double dt_step,
+                           int n_subcycles, int do_z_push) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_beam_state(c, b);
+        check_store(c, store);
+        if (slice < 0 || slice >= c->g.nz) throw HpqError(HPQ_E_INVALID, "slice index out of range");
+        if (n_subcycles < 1) throw HpqError(HPQ_E_INVALID, "n_subcycles must be >= 1");
+        advance_beam(c, b, slice, store, dt_step, n_subcycles, do_z_push != 0);
     });
 }
 

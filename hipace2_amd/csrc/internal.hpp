@@ -178,6 +178,14 @@ struct AdjointState {
     bool ready = false;
 };
 
+// Slipped beam particles between consecutive slice pushes: two index lists and their counts.
+struct BeamSlip {
+    int32_t* idx[2] = {nullptr, nullptr};
+    int32_t* cnt = nullptr;      // device, two counters
+    long cap = 0;
+    int cur = 0;                 // list holding the particles slipped into the next slice pushed
+};
+
 // Event pairs around the launches of one kernel family (HPQ_TRACE_*).
 struct Trace {
     bool on = false;
@@ -196,6 +204,7 @@ struct hpq_ctx {
     hpq::MultiGrid mg;
     hpq::Bins bins;
     hpq::AdjointState adj;
+    hpq::BeamSlip slip;
     int* d_count = nullptr;          // QSA counter
     int* h_count = nullptr;          // pinned
     double* d_red = nullptr;         // reduction scratch (8 doubles)
@@ -256,6 +265,10 @@ void adj_chi(hpq_ctx* c, const double* store, double* adj);
 void adj_seed(hpq_ctx* c, const double* seed, double* adj);
 void adj_beam_weights(hpq_ctx* c, const hpq_beam* b, int slice, double* adj, double* w_adj);
 void adjoint_free(hpq_ctx* c);
+void beam_begin_step(hpq_ctx* c, hpq_beam_state* b);
+void advance_beam(hpq_ctx* c, hpq_beam_state* b, int slice, const double* store, double dt_step, int n_subcycles,
+                  bool do_z_push);
+void beam_slip_free(hpq_ctx* c);
 
 inline dim3 grid1d(long n, int block) { return dim3(unsigned((n + block - 1) / block)); }
 

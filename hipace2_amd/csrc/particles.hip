@@ -166,6 +166,115 @@ __global__ __launch_bounds__(256) void k_push_plasma(Geom g, hpq_plasma p, const
     p.psi[ip] = psi;
 }
 
+// AdvanceBeamParticlesSlice (BeamParticleAdvance.cpp:128-336; no radiation reaction, spin
+// or external fields) for particle ip; a particle that ends below the slice with sub-cycles
+// left is appended to slip_out (shiftSlippedParticles, SliceSort.cpp:13-70).
+template <int O>
+__device__ void push_beam_particle(const Geom& g, const hpq_beam_state& b, long ip, const double* __restrict__ store,
+                                   int cPsi, int cEz, int cBx, int cBy, int cBz, double dt, int n_subcycles,
+                                   bool do_z_push, double qm, double min_z, const BC& bc, int32_t* slip_out,
+                                   int32_t* slip_out_cnt) {
+    if (!b.valid[ip]) return;
+    const double clight = 1.0, inv_c2 = 1.0 / (1.0 * 1.0);
+    double xp = b.x[ip], yp = b.y[ip], zp = b.z[ip];
+    double ux = b.ux[ip], uy = b.uy[ip], uz = b.uz[ip];
+    int i = b.nsubcycles[ip];
+    const double* Psi = store + long(cPsi) * g.plane;
+    const double* Ez = store + long(cEz) * g.plane;
+    const double* Bx = store + long(cBx) * g.plane;
+    const double* By = store + long(cBy) * g.plane;
+    const double* Bz = store + long(cBz) * g.plane;
+    constexpr int W = O + 2;
+    for (; i < n_subcycles; i++) {
+        if (zp < min_z) break;
+        const double gammap_inv = 1.0 / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) * inv_c2);
+        xp += dt * 0.5 * ux * gammap_inv;
+        yp += dt * 0.5 * uy * gammap_inv;
+        if (enforce_bc(bc, xp, yp, ux, uy)) {
+            b.w[ip] = 0.0;
+            b.valid[ip] = 0;
+            return;
+        }
+        double ExmByp = 0, EypBxp = 0, Ezp = 0, Bxp = 0, Byp = 0, Bzp = 0;
+        const double xm = (xp - g.xoff) * g.dxi, ym = (yp - g.yoff) * g.dyi;
+        Dsf sx[W], sy[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            sx[k] = dshape<1, O>(xm, k);
+            sy[k] = dshape<1, O>(ym, k);
+        }
+#pragma unroll
+        for (int iy = 0; iy < W; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < W; ++ix) {
+                const Dsf X = sx[ix], Y = sy[iy];
+                if (!in_store(g, X.cell, Y.cell)) continue;
+                const long k = g.idx(X.cell, Y.cell);
+                ExmByp += (X.ds * Y.s) * Psi[k] * g.dxi;
+                EypBxp += (X.s * Y.ds) * Psi[k] * g.dyi;
+                Ezp += (X.s * Y.s) * Ez[k];
+                Bxp += (X.s * Y.s) * Bx[k];
+                Byp += (X.s * Y.s) * By[k];
+                Bzp += (X.s * Y.s) * Bz[k];
+            }
+        const double ux_next = ux + dt * qm * (ExmByp + (clight - uz * gammap_inv) * Byp + uy * gammap_inv * Bzp);
+        const double uy_next = uy + dt * qm * (EypBxp + (uz * gammap_inv - clight) * Bxp - ux * gammap_inv * Bzp);
+        const double ux_i = (ux_next + ux) * 0.5;
+        const double uy_i = (uy_next + uy) * 0.5;
+        const double uz_i = uz + dt * 0.5 * qm * Ezp;
+        const double gamma_i_inv = 1.0 / sqrt(1.0 + (ux_i * ux_i + uy_i * uy_i + uz_i * uz_i) * inv_c2);
+        const double uz_next = uz + dt * qm * (Ezp + (ux_i * Byp - uy_i * Bxp) * gamma_i_inv);
+        const double gamma_next_inv =
+            1.0 / sqrt(1.0 + (ux_next * ux_next + uy_next * uy_next + uz_next * uz_next) * inv_c2);
+        xp += dt * 0.5 * ux_next * gamma_next_inv;
+        yp += dt * 0.5 * uy_next * gamma_next_inv;
+        if (do_z_push) zp += dt * (uz_next * gamma_next_inv - clight);
+        ux = ux_next;
+        uy = uy_next;
+        uz = uz_next;
+    }
+    if (enforce_bc(bc, xp, yp, ux, uy)) {
+        b.w[ip] = 0.0;
+        b.valid[ip] = 0;
+        return;
+    }
+    b.x[ip] = xp;
+    b.y[ip] = yp;
+    b.z[ip] = zp;
+    b.nsubcycles[ip] = i;
+    b.ux[ip] = ux;
+    b.uy[ip] = uy;
+    b.uz[ip] = uz;
+    if (zp < min_z && i < n_subcycles) {
+        const int at = atomicAdd(slip_out_cnt, 1);
+        slip_out[at] = int32_t(ip);
+    }
+}
+
+// the particles [first, first + count) one per thread, then the slipped-in list grid-strided
+template <int O>
+__global__ __launch_bounds__(256) void k_push_beam(Geom g, hpq_beam_state b, long first, long count,
+                                                   const double* __restrict__ store, int cPsi, int cEz, int cBx, int cBy,
+                                                   int cBz, double dt, int n_subcycles, int do_z_push, double qm,
+                                                   double min_z, BC bc, const int32_t* __restrict__ slip_in,
+                                                   const int32_t* __restrict__ slip_in_cnt, int32_t* slip_out,
+                                                   int32_t* slip_out_cnt) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long total = long(gridDim.x) * blockDim.x;
+    if (t < count)
+        push_beam_particle<O>(g, b, first + t, store, cPsi, cEz, cBx, cBy, cBz, dt, n_subcycles, do_z_push != 0, qm,
+                              min_z, bc, slip_out, slip_out_cnt);
+    const long n_in = *slip_in_cnt;
+    for (long j = t; j < n_in; j += total)
+        push_beam_particle<O>(g, b, long(slip_in[j]), store, cPsi, cEz, cBx, cBy, cBz, dt, n_subcycles,
+                              do_z_push != 0, qm, min_z, bc, slip_out, slip_out_cnt);
+}
+
+__global__ void k_fill_i32(int32_t* a, long n, int32_t v) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
 }  // namespace
 
 void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool temp_slice) {
@@ -181,6 +290,54 @@ void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool t
                            store, L.Psi, L.Ez, L.Bx, L.By, L.Bz, qmc, bc, int(temp_slice));
     });
     HPQ_HIP(hipGetLastError());
+}
+
+void beam_slip_free(hpq_ctx* c) {
+    BeamSlip& S = c->slip;
+    for (int32_t* p : {S.idx[0], S.idx[1], S.cnt})
+        if (p) (void)hipFree(p);
+    S = BeamSlip{};
+}
+
+void beam_begin_step(hpq_ctx* c, hpq_beam_state* b) {
+    BeamSlip& S = c->slip;
+    if (S.cap < b->n) {
+        beam_slip_free(c);
+        const long cap = std::max<long>(b->n, 1);
+        HPQ_HIP(hipMalloc(&S.idx[0], sizeof(int32_t) * cap));
+        HPQ_HIP(hipMalloc(&S.idx[1], sizeof(int32_t) * cap));
+        HPQ_HIP(hipMalloc(&S.cnt, sizeof(int32_t) * 2));
+        S.cap = cap;
+    }
+    S.cur = 0;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, S.cnt, 2L, 0);
+    if (b->n > 0)
+        hipLaunchKernelGGL(k_fill_i32, grid1d(b->n, 256), dim3(256), 0, c->stream, b->nsubcycles, long(b->n), 0);
+    HPQ_HIP(hipGetLastError());
+}
+
+void advance_beam(hpq_ctx* c, hpq_beam_state* b, int slice, const double* store, double dt_step, int n_subcycles,
+                  bool do_z_push) {
+    BeamSlip& S = c->slip;
+    if (S.cap < b->n || !S.cnt) throw HpqError(HPQ_E_INVALID, "hpq_beam_begin_step must precede the beam pushes");
+    const Layout& L = c->L;
+    const Geom& g = c->g;
+    const long first = b->off[slice];
+    const long count = b->off[slice + 1] - first;
+    const int in = S.cur, out = 1 - S.cur;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, S.cnt + out, 1L, 0);
+    const double qm = b->charge / b->mass;
+    const double dt = dt_step / n_subcycles;
+    const double min_z = g.zlo + slice * g.dz;
+    BC bc{c->p.particle_bc, c->p.plo[0], c->p.plo[1], c->p.phi[0], c->p.phi[1]};
+    TraceScope ts(c, HPQ_TRACE_PUSH);
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        hipLaunchKernelGGL((k_push_beam<decltype(O)::value>), dim3(unsigned(std::max<long>(1, (count + 255) / 256))),
+                           dim3(256), 0, c->stream, g, *b, first, count, store, L.Psi, L.Ez, L.Bx, L.By, L.Bz, dt,
+                           n_subcycles, int(do_z_push), qm, min_z, bc, S.idx[in], S.cnt + in, S.idx[out], S.cnt + out);
+    });
+    HPQ_HIP(hipGetLastError());
+    S.cur = out;
 }
 
 // ---------------------------------------------------------------------------------

@@ -105,6 +105,25 @@ class Simulation:
         self._bdesc = _abi.Beam(*[self.beam[k].data_ptr() for k in ("x", "y", "z", "ux", "uy", "uz", "w", "valid")],
                                 C.cast(self._boff, C.POINTER(C.c_int64)), n, b.charge, b.mass)
 
+    def _beam_state(self):
+        if "nsub" not in self.beam:
+            self.beam["nsub"] = torch.zeros(max(self.beam_host.n, 1), dtype=torch.int32, device=self.device)
+        self._bsdesc = _abi.BeamState(*[self.beam[k].data_ptr() for k in
+                                        ("x", "y", "z", "ux", "uy", "uz", "w", "valid", "nsub")],
+                                      C.cast(self._boff, C.POINTER(C.c_int64)), self.beam_host.n,
+                                      self.beam_host.charge, self.beam_host.mass)
+        return self._bsdesc
+
+    def beam_begin_step(self):
+        """Reset the beam sub-cycle counters and the slipped-particle list (hpq_beam_begin_step)."""
+        _abi.check(self.ctx, self.L.hpq_beam_begin_step(self.ctx, C.byref(self._beam_state())))
+
+    def advance_beam_slice(self, islice: int, dt: float, n_subcycles: int = 10, do_z_push: bool = True):
+        """AdvanceBeamParticlesSlice + shiftSlippedParticles for one slice (hpq_advance_beam_slice)."""
+        _abi.check(self.ctx, self.L.hpq_advance_beam_slice(self.ctx, C.byref(self._beam_state()), islice,
+                                                           C.c_void_p(self.store.data_ptr()), float(dt),
+                                                           int(n_subcycles), int(do_z_push)))
+
     def init_plasma(self):
         if self.np:
             _abi.check(self.ctx, self.L.hpq_init_plasma(self.ctx, C.byref(self._profile), C.byref(self._pdesc)))

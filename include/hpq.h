@@ -95,6 +95,17 @@ typedef struct hpq_beam {
     double charge, mass;
 } hpq_beam;
 
+/* Beam particles the beam push may move: hpq_beam's arrays, mutable, plus the sub-cycle
+ * counter of every particle (BeamIdx::nsubcycles, src/particles/beam/BeamParticleContainer.H:23-41).
+ * off stays a host array: particles of slice s are [off[s], off[s+1]). */
+typedef struct hpq_beam_state {
+    double *x, *y, *z, *ux, *uy, *uz, *w;
+    int32_t *valid, *nsubcycles;
+    const int64_t *off;
+    int64_t n;
+    double charge, mass;
+} hpq_beam_state;
+
 /* Adjoints of plasma quantities for the vector-Jacobian products below: device arrays of
  * n doubles (n of the matching hpq_plasma), accumulated with +=. */
 typedef struct hpq_plasma_adj {
@@ -184,6 +195,19 @@ int hpq_adjoint_forward(hpq_ctx *ctx, const hpq_plasma *p, const hpq_beam *b, do
  * mg_tol: relative tolerance of the adjoint Bx/By solves. */
 int hpq_adjoint_backward(hpq_ctx *ctx, const hpq_plasma *p, const hpq_beam *b, double *store,
                          const double *ez_seed, double *beam_w_adj, double mg_tol);
+
+/* BeamParticleContainer::initializeSlice's counter reset and an empty slipped-particle list,
+ * before the beam pushes of a step (src/particles/beam/BeamParticleContainer.cpp:353-390). */
+int hpq_beam_begin_step(hpq_ctx *ctx, hpq_beam_state *b);
+
+/* AdvanceBeamParticlesSlice (src/particles/pusher/BeamParticleAdvance.cpp:20-336; no radiation
+ * reaction, spin tracking or external fields) followed by shiftSlippedParticles
+ * (src/particles/sorting/SliceSort.cpp:13-70): pushes the particles of `slice` and those that
+ * slipped into it from the slice above, n_subcycles sub-steps of dt_step / n_subcycles, with
+ * the fields of the slice store. A particle that falls below the slice stops; with sub-cycles
+ * left it is listed for the next slice's push. Slices are pushed in decreasing order. */
+int hpq_advance_beam_slice(hpq_ctx *ctx, hpq_beam_state *b, int slice, const double *store, double dt_step,
+                           int n_subcycles, int do_z_push);
 
 /* FFTPoissonSolverDirichlet*::SolvePoissonEquation
  * (src/fields/fft_poisson_solver/FFTPoissonSolverDirichletDirect.cpp:101-139,

@@ -120,6 +120,15 @@ void explicit_deposit(const Ctx& c, const Plasma& p, double* F);
 void deposit_beam(const Ctx& c, const Beam& b, int slice, double* F, int cjx, int cjy, int cjz);
 void push_plasma(const Ctx& c, Plasma& p, const double* F, bool temp_slice, int cPsi, int cEz,
                  int cBx, int cBy, int cBz);
+// mutable beam arrays for the push (caller-owned)
+struct BeamRef {
+    double *x, *y, *z, *ux, *uy, *uz, *w;
+    int32_t *valid, *nsub;
+    double charge, mass;
+};
+void push_beam_slice(const Ctx& c, const double* F, const BeamRef& b, long first, long count, const int32_t* slip_in,
+                     long n_in, std::vector<int32_t>& slip_out, int slice, double dt_step, int n_subcycles,
+                     bool do_z_push);
 void dst1(const double* in, double* out, int n, int stride_in, int stride_out);
 void poisson_dirichlet(const Geom& g, const double* staging, double* lhs);  // staging: nx*ny
 void solve_poisson_psi_ez_bz(const Ctx& c, double* F);

@@ -81,6 +81,11 @@ def lib():
         L.orc_explicit_deposit.argtypes = [vp]
         L.orc_deposit_beam.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_push_plasma.argtypes = [vp, C.c_int]
+        i32p = C.POINTER(C.c_int32)
+        L.orc_push_beam_slice.argtypes = [vp] + [dp] * 7 + [i32p, i32p, C.c_double, C.c_double, C.c_long, C.c_long,
+                                                          i32p, C.c_long, i32p, C.c_long, C.c_int, C.c_double, C.c_int,
+                                                          C.c_int]
+        L.orc_push_beam_slice.restype = C.c_long
         L.orc_solve_poisson_psi_ez_bz.argtypes = [vp]
         L.orc_solve_poisson_bxby.argtypes = [vp]
         L.orc_initialize_sxsy_with_beam.argtypes = [vp]
@@ -222,6 +227,20 @@ class OracleSim:
 
     def push_plasma(self, temp_slice=False):
         lib().orc_push_plasma(self.h, int(temp_slice))
+
+    def push_beam_slice(self, b, slice_, first, count, slip_in, dt, n_subcycles, do_z_push=True):
+        """AdvanceBeamParticlesSlice on the dict of numpy arrays b (x, y, z, ux, uy, uz, w,
+        valid, nsub, modified in place) with this simulation's store; returns the indices
+        listed as slipped into the next slice."""
+        i32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        slip_in = np.ascontiguousarray(slip_in, dtype=np.int32)
+        cap = b["x"].shape[0]
+        out = np.zeros(max(cap, 1), np.int32)
+        n = lib().orc_push_beam_slice(self.h, *[_dp(b[k]) for k in ("x", "y", "z", "ux", "uy", "uz", "w")],
+                                      i32(b["valid"]), i32(b["nsub"]), b.get("charge", -1.0), b.get("mass", 1.0),
+                                      first, count, i32(slip_in), slip_in.shape[0], i32(out), cap, slice_, dt,
+                                      n_subcycles, int(do_z_push))
+        return out[:n].copy()
 
     def solve_poisson_psi_ez_bz(self):
         lib().orc_solve_poisson_psi_ez_bz(self.h)

@@ -287,4 +287,90 @@ void push_plasma(const Ctx& c, Plasma& p, const double* F, bool temp_slice, int 
     }
 }
 
+// AdvanceBeamParticlesSlice, src/particles/pusher/BeamParticleAdvance.cpp:20-336 (no
+// radiation reaction, spin tracking or external fields), for the particles [first, first+count)
+// and then the listed ones that slipped into this slice. A particle below min_z (the lower end
+// of the slice) stops sub-cycling (:152-155); with sub-cycles left it is listed for the next
+// slice, the role of shiftSlippedParticles (src/particles/sorting/SliceSort.cpp:13-70).
+void push_beam_slice(const Ctx& c, const double* F, const BeamRef& b, long first, long count, const int32_t* slip_in,
+                     long n_in, std::vector<int32_t>& slip_out, int slice, double dt_step, int n_subcycles,
+                     bool do_z_push) {
+    const Geom& g = c.g;
+    const Layout& L = c.L;
+    const int order = c.o.depos_order;
+    const double clight = 1.0, inv_c2 = 1.0 / (1.0 * 1.0);
+    const double qm = b.charge / b.mass;
+    const double dt = dt_step / n_subcycles;
+    const double min_z = g.zlo + slice * g.dz;
+    const double* Psi = F + L.Psi * g.plane;
+    const double* Ez = F + L.Ez * g.plane;
+    const double* Bx = F + L.Bx * g.plane;
+    const double* By = F + L.By * g.plane;
+    const double* Bz = F + L.Bz * g.plane;
+    const int lo = -g.ng, hix = g.nx - 1 + g.ng, hiy = g.ny - 1 + g.ng;
+    auto push = [&](long ip) {
+        if (!b.valid[ip]) return;
+        double xp = b.x[ip], yp = b.y[ip], zp = b.z[ip];
+        double ux = b.ux[ip], uy = b.uy[ip], uz = b.uz[ip];
+        int i = b.nsub[ip];
+        for (; i < n_subcycles; i++) {
+            if (zp < min_z) break;
+            const double gammap_inv = 1.0 / std::sqrt(1.0 + (ux * ux + uy * uy + uz * uz) * inv_c2);
+            xp += dt * 0.5 * ux * gammap_inv;
+            yp += dt * 0.5 * uy * gammap_inv;
+            if (enforce_bc(c.o, xp, yp, ux, uy)) {
+                b.w[ip] = 0.0;
+                b.valid[ip] = 0;
+                return;
+            }
+            double ExmByp = 0, EypBxp = 0, Ezp = 0, Bxp = 0, Byp = 0, Bzp = 0;
+            const double xm = (xp - g.xoff) * g.dxi, ym = (yp - g.yoff) * g.dyi;
+            for (int iy = 0; iy <= order + 1; ++iy) {
+                for (int ix = 0; ix <= order + 1; ++ix) {
+                    const DSF sy = dshape(1, order, ym, iy);
+                    const DSF sx = dshape(1, order, xm, ix);
+                    if (sx.cell < lo || sx.cell > hix || sy.cell < lo || sy.cell > hiy) continue;
+                    const long k = g.idx(sx.cell, sy.cell);
+                    ExmByp += (sx.ds * sy.s) * Psi[k] * g.dxi;
+                    EypBxp += (sx.s * sy.ds) * Psi[k] * g.dyi;
+                    Ezp += (sx.s * sy.s) * Ez[k];
+                    Bxp += (sx.s * sy.s) * Bx[k];
+                    Byp += (sx.s * sy.s) * By[k];
+                    Bzp += (sx.s * sy.s) * Bz[k];
+                }
+            }
+            const double ux_next = ux + dt * qm * (ExmByp + (clight - uz * gammap_inv) * Byp + uy * gammap_inv * Bzp);
+            const double uy_next = uy + dt * qm * (EypBxp + (uz * gammap_inv - clight) * Bxp - ux * gammap_inv * Bzp);
+            const double ux_i = (ux_next + ux) * 0.5;
+            const double uy_i = (uy_next + uy) * 0.5;
+            const double uz_i = uz + dt * 0.5 * qm * Ezp;
+            const double gamma_i_inv = 1.0 / std::sqrt(1.0 + (ux_i * ux_i + uy_i * uy_i + uz_i * uz_i) * inv_c2);
+            const double uz_next = uz + dt * qm * (Ezp + (ux_i * Byp - uy_i * Bxp) * gamma_i_inv);
+            const double gamma_next_inv =
+                1.0 / std::sqrt(1.0 + (ux_next * ux_next + uy_next * uy_next + uz_next * uz_next) * inv_c2);
+            xp += dt * 0.5 * ux_next * gamma_next_inv;
+            yp += dt * 0.5 * uy_next * gamma_next_inv;
+            if (do_z_push) zp += dt * (uz_next * gamma_next_inv - clight);
+            ux = ux_next;
+            uy = uy_next;
+            uz = uz_next;
+        }
+        if (enforce_bc(c.o, xp, yp, ux, uy)) {
+            b.w[ip] = 0.0;
+            b.valid[ip] = 0;
+            return;
+        }
+        b.x[ip] = xp;
+        b.y[ip] = yp;
+        b.z[ip] = zp;
+        b.nsub[ip] = i;
+        b.ux[ip] = ux;
+        b.uy[ip] = uy;
+        b.uz[ip] = uz;
+        if (zp < min_z && i < n_subcycles) slip_out.push_back(int32_t(ip));
+    };
+    for (long t = 0; t < count; ++t) push(first + t);
+    for (long t = 0; t < n_in; ++t) push(slip_in[t]);
+}
+
 }  // namespace orc

@@ -424,4 +424,18 @@ void orc_dshape(int type, int order, double xmid, int ix, double* out) {
     out[0] = r.s; out[1] = r.ds; out[2] = r.cell;
 }
 
+// AdvanceBeamParticlesSlice on caller arrays with the store of the simulation; returns the
+// number of particles listed in slip_out (capacity n_slip_cap)
+long orc_push_beam_slice(void* h, double* x, double* y, double* z, double* ux, double* uy, double* uz, double* w,
+                         int32_t* valid, int32_t* nsub, double charge, double mass, long first, long count,
+                         const int32_t* slip_in, long n_in, int32_t* slip_out, long n_slip_cap, int slice, double dt,
+                         int n_subcycles, int do_z_push) {
+    Sim* s = static_cast<Sim*>(h);
+    BeamRef b{x, y, z, ux, uy, uz, w, valid, nsub, charge, mass};
+    std::vector<int32_t> out;
+    push_beam_slice(s->c, s->F.data(), b, first, count, slip_in, n_in, out, slice, dt, n_subcycles, do_z_push != 0);
+    const long n = std::min<long>(long(out.size()), n_slip_cap);
+    for (long k = 0; k < n; ++k) slip_out[k] = out[k];
+    return long(out.size());
+}
 }  // extern "C"

@@ -337,3 +337,81 @@ def test_deposit_single_particle_brute_force():
         for i in range(-2, cfg.nx + 2):
             want[j + 2, i + 2] = -1.7 * bspline(2, xm - i) * bspline(2, ym - j)
     assert rel_err(st, want) < 1e-14
+
+
+# ----------------------------------------------------------------------------- beam push
+
+def _beam_arrays(n, x, y, z, ux, uy, uz):
+    f = lambda v: np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (n,)).copy())
+    return {"x": f(x), "y": f(y), "z": f(z), "ux": f(ux), "uy": f(uy), "uz": f(uz), "w": np.ones(n),
+            "valid": np.ones(n, np.int32), "nsub": np.zeros(n, np.int32), "charge": -1.0, "mass": 1.0}
+
+
+def _beam_oracle(fields):
+    cfg = small_config(n=31, nz=8)
+    sim = orc.OracleSim(cfg)
+    st = np.zeros_like(sim.store())
+    for name, v in fields.items():
+        st[sim.layout[name]] = v
+    sim.set_store(st)
+    return cfg, sim
+
+
+def test_beam_push_free_streaming():
+    """No fields: momenta are constant, x advances by dt ux / gamma, z by dt (uz / gamma - 1)."""
+    cfg, sim = _beam_oracle({})
+    rng = np.random.default_rng(1)
+    n, s, dt, nsub = 50, 5, 2.0, 10
+    z0 = cfg.zlo + (s + 0.9) * cfg.dz
+    b = _beam_arrays(n, rng.uniform(-2, 2, n), rng.uniform(-2, 2, n), z0, rng.normal(0, 3, n), rng.normal(0, 3, n),
+                     rng.uniform(200, 2000, n))
+    b0 = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in b.items()}
+    slipped = sim.push_beam_slice(b, s, 0, n, [], dt, nsub)
+    g = 1.0 / np.sqrt(1 + b0["ux"] ** 2 + b0["uy"] ** 2 + b0["uz"] ** 2)
+    assert slipped.size == 0
+    for k in ("ux", "uy", "uz"):
+        assert np.array_equal(b[k], b0[k])
+    assert rel_err(b["x"] - b0["x"], dt * b0["ux"] * g) < 1e-12
+    assert rel_err(b["z"] - b0["z"], dt * (b0["uz"] * g - 1.0)) < 1e-9
+    assert np.all(b["nsub"] == nsub)
+
+
+def test_beam_push_uniform_ez():
+    """Uniform Ez, no B: uz gains dt q/m Ez per sub-cycle exactly; x follows the half-step
+    velocities of the scheme."""
+    E0 = 0.3
+    cfg, sim = _beam_oracle({"Ez": E0})
+    n, s, dt, nsub = 20, 3, 1.5, 7
+    rng = np.random.default_rng(2)
+    b = _beam_arrays(n, rng.uniform(-1, 1, n), 0.0, cfg.zlo + (s + 0.95) * cfg.dz, rng.normal(0, 1, n), 0.0, 50.0)
+    b0 = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in b.items()}
+    sim.push_beam_slice(b, s, 0, n, [], dt, nsub, do_z_push=False)
+    h = dt / nsub
+    qm = -1.0
+    uz = b0["uz"].copy()
+    x = b0["x"].copy()
+    for _ in range(nsub):
+        g0 = 1.0 / np.sqrt(1 + b0["ux"] ** 2 + uz ** 2)
+        uz_n = uz + h * qm * E0
+        g1 = 1.0 / np.sqrt(1 + b0["ux"] ** 2 + uz_n ** 2)
+        x = x + h * 0.5 * b0["ux"] * g0 + h * 0.5 * b0["ux"] * g1
+        uz = uz_n
+    assert rel_err(b["uz"], uz) < 1e-14
+    assert rel_err(b["x"] - b0["x"], x - b0["x"]) < 1e-12
+    assert np.array_equal(b["z"], b0["z"])
+
+
+def test_beam_push_slipped_particles_continue():
+    """Slow particles fall below their slice, stop, and finish their sub-cycles in the next
+    slice: the total displacement is that of n_subcycles uninterrupted sub-steps."""
+    cfg, sim = _beam_oracle({})
+    n, s, dt, nsub = 6, 4, 3.0, 12
+    lo = cfg.zlo + s * cfg.dz
+    b = _beam_arrays(n, 0.0, 0.0, lo + np.linspace(0.01, 0.2, n) * cfg.dz, 0.0, 0.0, 1.0)
+    b0 = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in b.items()}
+    slipped = sim.push_beam_slice(b, s, 0, n, [], dt, nsub)
+    assert slipped.size == n and np.all(b["nsub"][slipped] < nsub)
+    sim.push_beam_slice(b, s - 1, 0, 0, slipped, dt, nsub)
+    g = 1.0 / np.sqrt(2.0)
+    assert np.all(b["nsub"] == nsub)
+    assert rel_err(b["z"] - b0["z"], dt * (g - 1.0)) < 1e-12
