@@ -28,7 +28,7 @@ EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq
            "hpq_add_capture", "hpq_clear_captures", "hpq_stats", "hpq_trace_enable", "hpq_trace_read",
            "hpq_trace_reset", "hpq_bin_tiles", "hpq_deposit_current_vjp", "hpq_explicit_deposition_vjp",
            "hpq_advance_plasma_particles_vjp", "hpq_adjoint_forward", "hpq_adjoint_backward",
-           "hpq_beam_begin_step", "hpq_advance_beam_slice"]
+           "hpq_beam_begin_step", "hpq_advance_beam_slice", "hpq_sort_beam_slices", "hpq_run_step_with_beam"]
 
 TRACE = {"push": 0, "deposit": 1, "explicit_deposit": 2, "beam_deposit": 3, "mg_smooth": 4, "mg_coarse": 5,
          "dst": 6}
@@ -133,6 +133,8 @@ def lib():
         L.hpq_adjoint_forward.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, ip]
         L.hpq_beam_begin_step.argtypes = [vp, C.POINTER(BeamState)]
         L.hpq_advance_beam_slice.argtypes = [vp, C.POINTER(BeamState), ip, vp, C.c_double, ip, ip]
+        L.hpq_sort_beam_slices.argtypes = [vp, C.POINTER(BeamState), C.POINTER(C.c_int64)]
+        L.hpq_run_step_with_beam.argtypes = [vp, C.POINTER(Plasma), C.POINTER(BeamState), vp, C.c_double, ip, ip]
         L.hpq_adjoint_backward.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, vp, vp, C.c_double]
         _lib = L
     return _lib

@@ -12,6 +12,7 @@
 // (bin, chunk) pair its first output slot; scatter, one wavefront per chunk walking its
 // particles in index order, so ties keep their order.
 #include <algorithm>
+#include <vector>
 
 #include "internal.hpp"
 #include "shape.hpp"
@@ -189,17 +190,26 @@ void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t
         return;
     }
     // at most 4096 chunks of at least 2048 particles (one wavefront walks a chunk)
-    const long chunk = std::max(2048L, (n + 4095) / 4096);
-    const int nchunks = int((n + chunk - 1) / chunk);
     Bins& B = c->bins;
     grow(B.keys, B.keys_cap, n);
-    const long m = long(nbins) * nchunks;
-    grow(B.counts, B.counts_cap, m);
-    grow(B.tmp, B.tmp_cap, scan_tmp_size(m));
     dispatch_order(c->p.depos_order, [&](auto O) {
         hipLaunchKernelGGL((k_bin_keys<decltype(O)::value>), grid1d(n, 256), dim3(256), 0, c->stream, g, *p, tile, ntx,
                            nty, B.keys);
     });
+    HPQ_HIP(hipGetLastError());
+    sort_keys(c, n, nbins, perm, offsets);
+}
+
+// stable counting sort of the keys in bins.keys (n of them, values in [0, nbins)): the
+// permutation (sorted slot -> index) and, when offsets != null, the first slot of every bin
+// plus offsets[nbins] = n
+void sort_keys(hpq_ctx* c, long n, int nbins, int32_t* perm, int32_t* offsets) {
+    Bins& B = c->bins;
+    const long chunk = std::max(2048L, (n + 4095) / 4096);
+    const int nchunks = int((n + chunk - 1) / chunk);
+    const long m = long(nbins) * nchunks;
+    grow(B.counts, B.counts_cap, m);
+    grow(B.tmp, B.tmp_cap, scan_tmp_size(m));
     const size_t lds = sizeof(int) * size_t(nbins);
     hipLaunchKernelGGL(k_bin_count, dim3(nchunks), dim3(256), lds, c->stream, B.keys, n, nbins, chunk, nchunks,
                        B.counts);
@@ -210,6 +220,74 @@ void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t
         hipLaunchKernelGGL(k_bin_offsets, grid1d(nbins + 1, 256), dim3(256), 0, c->stream, B.counts, nbins, nchunks,
                            n, offsets);
     HPQ_HIP(hipGetLastError());
+}
+
+namespace {
+
+// BoxSorter key (src/particles/sorting/BoxSort.cpp:20-60): the zeta-slice of every valid
+// beam particle; outside the domain or invalid -> nz (sorted last, made invalid)
+__global__ void k_slice_keys(Geom g, hpq_beam_state b, int32_t* __restrict__ keys) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= b.n) return;
+    int key = g.nz;
+    if (b.valid[ip]) {
+        const double s = floor((b.z[ip] - g.zlo) * g.dzi);
+        if (s >= 0.0 && s < double(g.nz)) key = int(s);
+    }
+    keys[ip] = key;
+}
+
+template <class T>
+__global__ void k_gather(const T* __restrict__ src, T* __restrict__ dst, const int32_t* __restrict__ perm, long n) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[perm[t]];
+}
+
+// particles sorted past the last slice are outside the domain: invalid, zero weight
+__global__ void k_invalidate_tail(hpq_beam_state b, const int32_t* __restrict__ offsets, int nz) {
+    const long t = long(offsets[nz]) + long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= b.n) return;
+    b.valid[t] = 0;
+    b.w[t] = 0.0;
+}
+
+}  // namespace
+
+void sort_beam_slices(hpq_ctx* c, hpq_beam_state* b, int64_t* off_out) {
+    const Geom& g = c->g;
+    const long n = b->n;
+    Bins& B = c->bins;
+    const int nbins = g.nz + 1;
+    std::vector<int32_t> offs(nbins + 1, 0);
+    if (n > 0) {
+        if (n > INT32_MAX) throw HpqError(HPQ_E_INVALID, "sort_beam_slices: more than 2^31-1 particles");
+        grow(B.keys, B.keys_cap, n);
+        hipLaunchKernelGGL(k_slice_keys, grid1d(n, 256), dim3(256), 0, c->stream, g, *b, B.keys);
+        HPQ_HIP(hipGetLastError());
+        int32_t *perm = nullptr, *d_off = nullptr;
+        void* scratch = nullptr;
+        HPQ_HIP(hipMalloc((void**)&perm, sizeof(int32_t) * n));
+        HPQ_HIP(hipMalloc((void**)&d_off, sizeof(int32_t) * (nbins + 1)));
+        HPQ_HIP(hipMalloc(&scratch, sizeof(double) * n));
+        sort_keys(c, n, nbins, perm, d_off);
+        double* reals[7] = {b->x, b->y, b->z, b->ux, b->uy, b->uz, b->w};
+        for (double* r : reals) {
+            hipLaunchKernelGGL(k_gather<double>, grid1d(n, 256), dim3(256), 0, c->stream, r, (double*)scratch, perm, n);
+            HPQ_HIP(hipMemcpyAsync(r, scratch, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        }
+        for (int32_t* q : {b->valid, b->nsubcycles}) {
+            hipLaunchKernelGGL(k_gather<int32_t>, grid1d(n, 256), dim3(256), 0, c->stream, q, (int32_t*)scratch, perm, n);
+            HPQ_HIP(hipMemcpyAsync(q, scratch, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
+        }
+        hipLaunchKernelGGL(k_invalidate_tail, grid1d(n, 256), dim3(256), 0, c->stream, *b, d_off, g.nz);
+        HPQ_HIP(hipGetLastError());
+        HPQ_HIP(hipMemcpyAsync(offs.data(), d_off, sizeof(int32_t) * (nbins + 1), hipMemcpyDeviceToHost, c->stream));
+        HPQ_HIP(hipStreamSynchronize(c->stream));
+        (void)hipFree(perm);
+        (void)hipFree(d_off);
+        (void)hipFree(scratch);
+    }
+    for (int s = 0; s <= g.nz; ++s) off_out[s] = offs[s];
 }
 
 // the permutation the deposition kernels walk: 16x16-cell tiles, refreshed every

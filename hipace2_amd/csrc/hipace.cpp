@@ -742,7 +742,7 @@ static void check_beam_state(const hpq_ctx* c, const hpq_beam_state* b) {
     if (b->n > 0 && (!b->x || !b->y || !b->z || !b->ux || !b->uy || !b->uz || !b->w || !b->valid || !b->nsubcycles))
         throw HpqError(HPQ_E_INVALID, "beam state has a null array");
     if (!b->off) throw HpqError(HPQ_E_INVALID, "beam state has no slice offsets");
-    if (b->off[0] != 0 || b->off[c->g.nz] != b->n) throw HpqError(HPQ_E_INVALID, "beam slice offsets do not cover n");
+    if (b->off[0] != 0 || b->off[c->g.nz] > b->n) throw HpqError(HPQ_E_INVALID, "beam slice offsets exceed n");
     if (!(b->mass != 0.0)) throw HpqError(HPQ_E_INVALID, "beam mass is zero");
 }
 
@@ -763,6 +763,36 @@ int hpq_advance_beam_slice(hpq_ctx* c, hpq_beam_state* b, int slice, const doubl
         if (slice < 0 || slice >= c->g.nz) throw HpqError(HPQ_E_INVALID, "slice index out of range");
         if (n_subcycles < 1) throw HpqError(HPQ_E_INVALID, "n_subcycles must be >= 1");
         advance_beam(c, b, slice, store, dt_step, n_subcycles, do_z_push != 0);
+    });
+}
+
+int hpq_sort_beam_slices(hpq_ctx* c, hpq_beam_state* b, int64_t* off_out) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        if (!b || !off_out) throw HpqError(HPQ_E_INVALID, "null pointer");
+        if (b->n > 0 && (!b->x || !b->y || !b->z || !b->ux || !b->uy || !b->uz || !b->w || !b->valid || !b->nsubcycles))
+            throw HpqError(HPQ_E_INVALID, "beam state has a null array");
+        sort_beam_slices(c, b, off_out);
+    });
+}
+
+int hpq_run_step_with_beam(hpq_ctx* c, const hpq_plasma* p, hpq_beam_state* b, double* store, double dt_step,
+                           int n_subcycles, int do_z_push) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_store(c, store);
+        check_plasma(p);
+        check_beam_state(c, b);
+        if (n_subcycles < 1) throw HpqError(HPQ_E_INVALID, "n_subcycles must be >= 1");
+        const hpq_beam view{b->x, b->y, b->z, b->ux, b->uy, b->uz, b->w, b->valid, b->off, b->n, b->charge, b->mass};
+        beam_begin_step(c, b);
+        Hipace h(c, p, &view, store);
+        for (int isl = c->g.nz - 1; isl >= 0; --isl) {
+            h.SolveOneSlice(isl);
+            // AdvanceBeamParticlesSlice after the plasma push (src/Hipace.cpp:700-709)
+            advance_beam(c, b, isl, store, dt_step, n_subcycles, do_z_push != 0);
+        }
+        diag_finalize(c);
     });
 }
 

@@ -256,6 +256,8 @@ int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof);
 void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets);
 const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p);
 void bins_free(hpq_ctx* c);
+void sort_keys(hpq_ctx* c, long n, int nbins, int32_t* perm, int32_t* offsets);
+void sort_beam_slices(hpq_ctx* c, hpq_beam_state* b, int64_t* off_out);
 void deposit_vjp(hpq_ctx* c, const hpq_plasma* p, const double* adj, const int32_t* comps, hpq_plasma_adj* a);
 void explicit_deposition_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, double* adj, hpq_plasma_adj* a);
 void advance_plasma_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, const hpq_plasma_adj* out,

@@ -124,6 +124,30 @@ class Simulation:
                                                            C.c_void_p(self.store.data_ptr()), float(dt),
                                                            int(n_subcycles), int(do_z_push)))
 
+    def sort_beam(self):
+        """Re-sort the beam by zeta-slice on the device (hpq_sort_beam_slices); updates the
+        slice offsets used by deposition and push."""
+        _abi.check(self.ctx, self.L.hpq_sort_beam_slices(self.ctx, C.byref(self._beam_state()), self._boff))
+        self.beam_host.off = np.array(list(self._boff), dtype=np.int64)
+        self._bdesc.off = C.cast(self._boff, C.POINTER(C.c_int64))
+
+    def run_step_with_beam(self, dt: float, n_subcycles: int = 10, do_z_push: bool = True,
+                           init_plasma: bool = True):
+        """One time step with the beam pushed after every slice (hpq_run_step_with_beam)."""
+        self.begin_step(init_plasma)
+        _abi.check(self.ctx, self.L.hpq_run_step_with_beam(self.ctx, C.byref(self._pdesc),
+                                                           C.byref(self._beam_state()),
+                                                           C.c_void_p(self.store.data_ptr()), float(dt),
+                                                           int(n_subcycles), int(do_z_push)))
+
+    def run_steps(self, nsteps: int, dt: float, n_subcycles: int = 10, do_z_push: bool = True):
+        """Several time steps: beam re-sorted by slice between steps, plasma re-injected each
+        step (src/Hipace.cpp:426-480)."""
+        for step in range(nsteps):
+            if step > 0:
+                self.sort_beam()
+            self.run_step_with_beam(dt, n_subcycles, do_z_push)
+
     def init_plasma(self):
         if self.np:
             _abi.check(self.ctx, self.L.hpq_init_plasma(self.ctx, C.byref(self._profile), C.byref(self._pdesc)))

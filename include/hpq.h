@@ -209,6 +209,17 @@ int hpq_beam_begin_step(hpq_ctx *ctx, hpq_beam_state *b);
 int hpq_advance_beam_slice(hpq_ctx *ctx, hpq_beam_state *b, int slice, const double *store, double dt_step,
                            int n_subcycles, int do_z_push);
 
+/* BoxSorter::sortParticlesByBox (src/particles/sorting/BoxSort.cpp:20-60): sorts the beam
+ * arrays in place by zeta-slice (stable), particles outside the domain or invalid last (made
+ * invalid), and writes the slice offsets to off_out (host, nz+1 entries). Between steps. */
+int hpq_sort_beam_slices(hpq_ctx *ctx, hpq_beam_state *b, int64_t *off_out);
+
+/* One time step with a moving beam: hpq_beam_begin_step, then for slices nz-1 .. 0
+ * Hipace::SolveOneSlice followed by AdvanceBeamParticlesSlice (src/Hipace.cpp:557-728), the
+ * beam of every slice deposited from its current positions. After hpq_begin_step. */
+int hpq_run_step_with_beam(hpq_ctx *ctx, const hpq_plasma *p, hpq_beam_state *b, double *store,
+                           double dt_step, int n_subcycles, int do_z_push);
+
 /* FFTPoissonSolverDirichlet*::SolvePoissonEquation
  * (src/fields/fft_poisson_solver/FFTPoissonSolverDirichletDirect.cpp:101-139,
  * ...DirichletFast.cpp:291-328): solves Lap(lhs) = staging with zero Dirichlet values

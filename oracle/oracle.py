@@ -86,6 +86,9 @@ def lib():
                                                           i32p, C.c_long, i32p, C.c_long, C.c_int, C.c_double, C.c_int,
                                                           C.c_int]
         L.orc_push_beam_slice.restype = C.c_long
+        L.orc_sim_run_step_with_beam.argtypes = [vp, C.c_double, C.c_int, C.c_int]
+        L.orc_sim_sort_beam.argtypes = [vp]
+        L.orc_sim_get_beam.argtypes = [vp] + [dp] * 7 + [i32p, C.POINTER(C.c_int64)]
         L.orc_solve_poisson_psi_ez_bz.argtypes = [vp]
         L.orc_solve_poisson_bxby.argtypes = [vp]
         L.orc_initialize_sxsy_with_beam.argtypes = [vp]
@@ -171,6 +174,30 @@ class OracleSim:
 
     def begin_step(self):
         lib().orc_sim_begin_step(self.h)
+
+    def run_step_with_beam(self, dt, n_subcycles=10, do_z_push=True):
+        """One time step with the beam pushed after every slice (after begin_step)."""
+        lib().orc_sim_run_step_with_beam(self.h, dt, n_subcycles, int(do_z_push))
+
+    def sort_beam(self):
+        lib().orc_sim_sort_beam(self.h)
+
+    def run_steps(self, nsteps, dt, n_subcycles=10, do_z_push=True):
+        for step in range(nsteps):
+            if step > 0:
+                self.sort_beam()
+            self.begin_step()
+            self.run_step_with_beam(dt, n_subcycles, do_z_push)
+
+    def beam(self) -> dict:
+        n = self._beam[0].shape[0]
+        out = {k: np.zeros(n) for k in ("x", "y", "z", "ux", "uy", "uz", "w")}
+        out["valid"] = np.zeros(n, np.int32)
+        out["off"] = np.zeros(self.cfg.nz + 1, np.int64)
+        lib().orc_sim_get_beam(self.h, *[_dp(out[k]) for k in ("x", "y", "z", "ux", "uy", "uz", "w")],
+                               out["valid"].ctypes.data_as(C.POINTER(C.c_int32)),
+                               out["off"].ctypes.data_as(C.POINTER(C.c_int64)))
+        return out
 
     def solve_slice(self, islice: int):
         lib().orc_sim_solve_slice(self.h, islice)

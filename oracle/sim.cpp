@@ -438,4 +438,74 @@ long orc_push_beam_slice(void* h, double* x, double* y, double* z, double* ux, d
     for (long k = 0; k < n; ++k) slip_out[k] = out[k];
     return long(out.size());
 }
+// One time step with the beam pushed after every slice (src/Hipace.cpp:557-728): counters
+// reset, then SolveOneSlice and AdvanceBeamParticlesSlice per slice, slipped particles handed
+// to the next slice. After orc_sim_begin_step.
+void orc_sim_run_step_with_beam(void* h, double dt, int n_subcycles, int do_z_push) {
+    Sim* s = static_cast<Sim*>(h);
+    Beam& b = s->beam;
+    const long n = long(b.x.size());
+    std::vector<int32_t> nsub(n, 0), slip_in, slip_out;
+    BeamRef r{b.x.data(), b.y.data(), b.z.data(), b.ux.data(), b.uy.data(), b.uz.data(), b.w.data(), b.valid.data(),
+              nsub.data(), b.charge, b.mass};
+    for (int isl = s->c.g.nz - 1; isl >= 0; --isl) {
+        solve_one_slice(*s, isl);
+        slip_out.clear();
+        const long first = b.off[isl], count = b.off[isl + 1] - b.off[isl];
+        push_beam_slice(s->c, s->F.data(), r, first, count, slip_in.data(), long(slip_in.size()), slip_out, isl, dt,
+                        n_subcycles, do_z_push != 0);
+        slip_in.swap(slip_out);
+    }
+}
+
+// BoxSorter::sortParticlesByBox (src/particles/sorting/BoxSort.cpp:20-60): stable sort of the
+// beam by zeta-slice; invalid or outside the domain last, made invalid
+void orc_sim_sort_beam(void* h) {
+    Sim* s = static_cast<Sim*>(h);
+    Beam& b = s->beam;
+    const Geom& g = s->c.g;
+    const long n = long(b.x.size());
+    std::vector<int> key(n);
+    for (long i = 0; i < n; ++i) {
+        int k = g.nz;
+        if (b.valid[i]) {
+            const double sl = std::floor((b.z[i] - g.zlo) * g.dzi);
+            if (sl >= 0.0 && sl < double(g.nz)) k = int(sl);
+        }
+        key[i] = k;
+    }
+    std::vector<long> order(n);
+    for (long i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](long a, long c) { return key[a] < key[c]; });
+    auto perm = [&](std::vector<double>& v) {
+        std::vector<double> t(n);
+        for (long i = 0; i < n; ++i) t[i] = v[order[i]];
+        v.swap(t);
+    };
+    for (auto* v : {&b.x, &b.y, &b.z, &b.ux, &b.uy, &b.uz, &b.w}) perm(*v);
+    std::vector<int32_t> vv(n);
+    for (long i = 0; i < n; ++i) vv[i] = b.valid[order[i]];
+    b.valid.swap(vv);
+    b.off.assign(g.nz + 1, 0);
+    for (long i = 0; i < n; ++i)
+        if (key[order[i]] < g.nz) b.off[key[order[i]] + 1]++;
+    for (int sl = 0; sl < g.nz; ++sl) b.off[sl + 1] += b.off[sl];
+    for (long i = b.off[g.nz]; i < n; ++i) {
+        b.valid[i] = 0;
+        b.w[i] = 0.0;
+    }
+}
+
+// copies the simulation's beam out (arrays of n, off of nz+1)
+void orc_sim_get_beam(void* h, double* x, double* y, double* z, double* ux, double* uy, double* uz, double* w,
+                      int32_t* valid, int64_t* off) {
+    Sim* s = static_cast<Sim*>(h);
+    const Beam& b = s->beam;
+    const size_t n = b.x.size();
+    std::copy(b.x.begin(), b.x.end(), x); std::copy(b.y.begin(), b.y.end(), y); std::copy(b.z.begin(), b.z.end(), z);
+    std::copy(b.ux.begin(), b.ux.end(), ux); std::copy(b.uy.begin(), b.uy.end(), uy);
+    std::copy(b.uz.begin(), b.uz.end(), uz); std::copy(b.w.begin(), b.w.end(), w);
+    std::copy(b.valid.begin(), b.valid.begin() + n, valid);
+    std::copy(b.off.begin(), b.off.end(), off);
+}
 }  // extern "C"

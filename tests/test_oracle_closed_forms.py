@@ -415,3 +415,26 @@ def test_beam_push_slipped_particles_continue():
     g = 1.0 / np.sqrt(2.0)
     assert np.all(b["nsub"] == nsub)
     assert rel_err(b["z"] - b0["z"], dt * (g - 1.0)) < 1e-12
+
+
+def test_beam_steps_sort_invariants():
+    """Two time steps with a moving beam, then the slice sort: every valid particle lies in
+    its slice's range, the order inside a slice is the previous relative order, and the
+    wakefield has changed the transverse momenta."""
+    from hipace2_amd.inputs import BeamSpec, make_beam
+    cfg = small_config(n=15, nz=8)
+    spec = BeamSpec(injection_type="fixed_weight", profile="gaussian", density=0.5,
+                    position_std=(0.5, 0.5, 1.41), num_particles=400, u_mean=(0.0, 0.0, 2.0), seed=3)
+    beam = make_beam(cfg, spec)
+    sim = orc.OracleSim(cfg, beam)
+    sim.run_steps(2, dt=3.0, n_subcycles=5)
+    before = sim.beam()
+    sim.sort_beam()
+    b = sim.beam()
+    off = b["off"]
+    for s in range(cfg.nz):
+        zs = b["z"][off[s]:off[s + 1]]
+        assert np.all(np.floor((zs - cfg.zlo) / cfg.dz) == s)
+        assert np.all(b["valid"][off[s]:off[s + 1]] == 1)
+    assert np.all(b["valid"][off[-1]:] == 0)
+    assert not np.array_equal(before["ux"][before["valid"] == 1], beam.ux[beam.valid == 1])
