@@ -28,7 +28,8 @@ EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq
            "hpq_add_capture", "hpq_clear_captures", "hpq_stats", "hpq_trace_enable", "hpq_trace_read",
            "hpq_trace_reset", "hpq_bin_tiles", "hpq_deposit_current_vjp", "hpq_explicit_deposition_vjp",
            "hpq_advance_plasma_particles_vjp", "hpq_adjoint_forward", "hpq_adjoint_backward",
-           "hpq_beam_begin_step", "hpq_advance_beam_slice", "hpq_sort_beam_slices", "hpq_run_step_with_beam"]
+           "hpq_beam_begin_step", "hpq_advance_beam_slice", "hpq_sort_beam_slices", "hpq_run_step_with_beam",
+           "hpq_set_insitu"]
 
 TRACE = {"push": 0, "deposit": 1, "explicit_deposit": 2, "beam_deposit": 3, "mg_smooth": 4, "mg_coarse": 5,
          "dst": 6}
@@ -70,6 +71,17 @@ class PlasmaAdj(C.Structure):
 class Beam(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("x", "y", "z", "ux", "uy", "uz", "w", "valid")] + \
                [("off", C.POINTER(C.c_int64)), ("n", C.c_int64), ("charge", C.c_double), ("mass", C.c_double)]
+
+
+INSITU_FIELD_NR, INSITU_PLASMA_NR, INSITU_BEAM_NR = 10, 14, 22
+
+
+class Insitu(C.Structure):
+    _fields_ = [("field_rdata", C.c_void_p), ("field_sum", C.c_void_p),
+                ("plasma_rdata", C.c_void_p), ("plasma_idata", C.c_void_p), ("plasma_sum", C.c_void_p),
+                ("plasma_isum", C.c_void_p), ("plasma_radius", C.c_double),
+                ("beam_rdata", C.c_void_p), ("beam_idata", C.c_void_p), ("beam_sum", C.c_void_p),
+                ("beam_isum", C.c_void_p), ("beam_radius", C.c_double)]
 
 
 class PlasmaProfile(C.Structure):
@@ -121,6 +133,7 @@ def lib():
         L.hpq_run_slices.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, ip, ip]
         L.hpq_set_checksums.argtypes = [vp, vp]
         L.hpq_add_capture.argtypes = [vp, ip, vp]
+        L.hpq_set_insitu.argtypes = [vp, C.POINTER(Insitu)]
         L.hpq_clear_captures.argtypes = [vp]
         L.hpq_stats.argtypes = [vp, dp]
         L.hpq_trace_enable.argtypes = [vp, ip, ip]

@@ -49,26 +49,53 @@ constexpr int SE_T = 256, SE_DIAG = 24;
 struct SliceEnd {
     int ndiag;
     int diag[SE_DIAG];
+    int ins[7];       // in-situ planes: ExmBy, EypBx, Ez, Bx, By, Bz, jz_beam
     PlaneSet shift;   // dst <- src, in order
     PlaneSet zero;    // dst <- 0
 };
+constexpr int SE_INSITU = HPQ_INSITU_FIELD_NR;
 
-template <bool DIAG, bool SHIFT>
+template <bool DIAG, bool SHIFT, bool INSITU>
 __global__ __launch_bounds__(SE_T) void k_slice_end(Geom g, double* __restrict__ store, SliceEnd e,
-                                                    double* __restrict__ partial) {
+                                                    double* __restrict__ partial, double* __restrict__ ipartial) {
     __shared__ double red[SE_T / 64][SE_DIAG];
+    __shared__ double ired[SE_T / 64][SE_INSITU];
     const int plane = int(g.plane);
     const int t = blockIdx.x * SE_T + threadIdx.x;
+    bool inside = false;
+    if (t < plane) {
+        const int jj = t / g.ldx, ii = t - jj * g.ldx;
+        inside = ii >= g.ng && ii < g.ng + g.nx && jj >= g.ng && jj < g.ng + g.ny;
+    }
     double acc[SE_DIAG];
 #pragma unroll
     for (int k = 0; k < SE_DIAG; ++k) acc[k] = 0.0;
-    if (DIAG && t < plane) {
-        const int jj = t / g.ldx, ii = t - jj * g.ldx;
-        if (ii >= g.ng && ii < g.ng + g.nx && jj >= g.ng && jj < g.ng + g.ny) {
+    if (DIAG && inside) {
 #pragma unroll
-            for (int k = 0; k < SE_DIAG; ++k)
-                if (k < e.ndiag) acc[k] = fabs(store[long(e.diag[k]) * plane + t]);
-        }
+        for (int k = 0; k < SE_DIAG; ++k)
+            if (k < e.ndiag) acc[k] = fabs(store[long(e.diag[k]) * plane + t]);
+    }
+    // Fields::InSituComputeDiags, src/fields/Fields.cpp:1323-1336 (c = 1)
+    double q[SE_INSITU];
+#pragma unroll
+    for (int k = 0; k < SE_INSITU; ++k) q[k] = 0.0;
+    if (INSITU && inside) {
+        const double clight = 1.0;
+        const double exmby = store[long(e.ins[0]) * plane + t], eypbx = store[long(e.ins[1]) * plane + t];
+        const double ez = store[long(e.ins[2]) * plane + t], bx = store[long(e.ins[3]) * plane + t];
+        const double by = store[long(e.ins[4]) * plane + t], bz = store[long(e.ins[5]) * plane + t];
+        const double jzb = store[long(e.ins[6]) * plane + t];
+        const double ex = exmby + by * clight, ey = eypbx - bx * clight;
+        q[0] = ex * ex;
+        q[1] = ey * ey;
+        q[2] = ez * ez;
+        q[3] = bx * bx;
+        q[4] = by * by;
+        q[5] = bz * bz;
+        q[6] = exmby * exmby;
+        q[7] = eypbx * eypbx;
+        q[8] = jzb;
+        q[9] = ez * jzb;
     }
     if (SHIFT && t < plane) {
 #pragma unroll
@@ -81,17 +108,32 @@ __global__ __launch_bounds__(SE_T) void k_slice_end(Geom g, double* __restrict__
     if (DIAG) {
 #pragma unroll
         for (int k = 0; k < SE_DIAG; ++k) {
-            if (k >= e.ndiag) break;
-            double v = acc[k];
+            if (k < e.ndiag) {
+                double v = acc[k];
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v;
+            }
+        }
+    }
+    if (INSITU) {
+#pragma unroll
+        for (int k = 0; k < SE_INSITU; ++k) {
+            double v = q[k];
             for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v;
+            if ((threadIdx.x & 63) == 0) ired[threadIdx.x >> 6][k] = v;
         }
-        __syncthreads();
-        if (threadIdx.x < e.ndiag) {
-            double v = red[0][threadIdx.x];
-            for (int w = 1; w < SE_T / 64; ++w) v += red[w][threadIdx.x];
-            partial[long(blockIdx.x) * SE_DIAG + threadIdx.x] += v;
-        }
+    }
+    if (DIAG || INSITU) __syncthreads();
+    if (DIAG && threadIdx.x < e.ndiag) {
+        double v = red[0][threadIdx.x];
+        for (int w = 1; w < SE_T / 64; ++w) v += red[w][threadIdx.x];
+        partial[long(blockIdx.x) * SE_DIAG + threadIdx.x] += v;
+    }
+    // this slice's row of in-situ sums (overwritten every slice)
+    if (INSITU && threadIdx.x < SE_INSITU) {
+        double v = ired[0][threadIdx.x];
+        for (int w = 1; w < SE_T / 64; ++w) v += ired[w][threadIdx.x];
+        ipartial[long(blockIdx.x) * INSITU_STRIDE + threadIdx.x] = v;
     }
 }
 
@@ -247,18 +289,25 @@ void captures(hpq_ctx* c, const double* store, int islice) {
     }
 }
 
-void slice_end(hpq_ctx* c, double* store, const int* shift_dst, const int* shift_src, int nshift, const int* zero,
-               int nzero) {
+void slice_end(hpq_ctx* c, double* store, int islice, const int* shift_dst, const int* shift_src, int nshift,
+               const int* zero, int nzero) {
     const Geom& g = c->g;
+    const Layout& L = c->L;
     const bool diag = c->checksums != nullptr;
+    const bool insitu = c->insitu.field_rdata != nullptr;
     const bool shift = nshift > 0 || nzero > 0;
-    if (!diag && !shift) return;
+    if (!diag && !shift && !insitu) return;
     SliceEnd e{};
     std::vector<int> dc;
     diag_comps(c, dc);
     if (int(dc.size()) > SE_DIAG) throw HpqError(HPQ_E_UNSUPPORTED, "slice_end: too many diagnostic components");
     e.ndiag = int(dc.size());
     for (int k = 0; k < e.ndiag; ++k) e.diag[k] = dc[k];
+    const int ins[7] = {L.ExmBy, L.EypBx, L.Ez, L.Bx, L.By, L.Bz, L.jz_beam};
+    for (int k = 0; k < 7; ++k) {
+        if (insitu && ins[k] < 0) throw HpqError(HPQ_E_UNSUPPORTED, "Must use explicit solver for field insitu diagnostic");
+        e.ins[k] = ins[k];
+    }
     for (int k = 0; k < nshift; ++k) {
         e.shift.dst[e.shift.n] = shift_dst[k];
         e.shift.src[e.shift.n] = shift_src[k];
@@ -273,14 +322,25 @@ void slice_end(hpq_ctx* c, double* store, const int* shift_dst, const int* shift
         HPQ_HIP(hipMalloc(&c->diag_comp_of, sizeof(int) * SE_DIAG));
         HPQ_HIP(hipMemcpy(c->diag_comp_of, dc.data(), sizeof(int) * dc.size(), hipMemcpyHostToDevice));
     }
+    double* ipart = insitu ? insitu_scratch(c, nblk) : nullptr;
     auto launch = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(SE_T), 0, c->stream, g, store, e, c->diag_partial);
+        hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(SE_T), 0, c->stream, g, store, e, c->diag_partial, ipart);
     };
-    if (diag && shift) launch(k_slice_end<true, true>);
-    else if (diag) launch(k_slice_end<true, false>);
-    else launch(k_slice_end<false, true>);
+    if (insitu) {
+        if (diag && shift) launch(k_slice_end<true, true, true>);
+        else if (diag) launch(k_slice_end<true, false, true>);
+        else if (shift) launch(k_slice_end<false, true, true>);
+        else launch(k_slice_end<false, false, true>);
+    } else {
+        if (diag && shift) launch(k_slice_end<true, true, false>);
+        else if (diag) launch(k_slice_end<true, false, false>);
+        else launch(k_slice_end<false, true, false>);
+    }
     HPQ_HIP(hipGetLastError());
     c->diag_pending = c->diag_pending || diag;
+    if (insitu)
+        insitu_finalize(c, int(nblk), HPQ_INSITU_FIELD_NR, false, g.dx * g.dy * g.dz, 0u, c->insitu.field_rdata,
+                        c->insitu.field_sum, nullptr, nullptr, islice);
 }
 
 // adds the pending per-workgroup diagnostic sums into the caller's checksums

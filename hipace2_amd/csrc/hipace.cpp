@@ -112,7 +112,7 @@ public:
         const int s[2] = {L_.ions_rhomjz, L_.ions_rhomjz};
         plane_add(c_, s_, d, s, 2);
     }
-    // The field diagnostics of this slice (sum |F|, FillDiagnostics), and, when another
+    // The field diagnostics of this slice (sum |F|, FillDiagnostics, InSituComputeDiags), and, when another
     // slice follows in the same pass, ShiftSlices plus that slice's InitializeSlices: one
     // pass over the store. The plasma push that comes next reads none of these planes.
     void EndSlice(int islice, bool chain) {
@@ -121,9 +121,9 @@ public:
             int d[8], s[8], z[16];
             const int ns = ShiftCopies(d, s);
             const int nz = InitZeros(z);
-            slice_end(c_, s_, d, s, ns, z, nz);
+            slice_end(c_, s_, islice, d, s, ns, z, nz);
         } else {
-            slice_end(c_, s_, nullptr, nullptr, 0, nullptr, 0);
+            slice_end(c_, s_, islice, nullptr, nullptr, 0, nullptr, 0);
         }
     }
     // Fields::SolvePoissonPsiExmByEypBxEzBz, :841-957
@@ -216,6 +216,7 @@ public:
     void SolveOneSlice(int islice, bool initialized = false, bool chain = false,
                        const std::function<void()>& before_push = {}) {
         const hpq_params& P = c_->p;
+        insitu_plasma(c_, p_, islice);
         if (!initialized) fields_.InitializeSlices();
         if (P.explicit_solver) {
             DepositCurrent(c_, p_, s_, L_.jx, L_.jy, -1, L_.rho, L_.chi, L_.rhomjz);
@@ -235,6 +236,7 @@ public:
         } else {
             PredictorCorrectorLoopToSolveBxBy(islice);
         }
+        insitu_beam(c_, b_, islice);
         fields_.EndSlice(islice, chain);
         if (before_push) before_push();
         advance_plasma(c_, p_, s_, false);
@@ -489,6 +491,8 @@ void adjoint_backward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double
     caps.swap(c->captures);
     double* checksums = c->checksums;
     c->checksums = nullptr;
+    const hpq_insitu insitu = c->insitu;
+    c->insitu = hpq_insitu{};
     try {
         for (double* a : A.padj[0]) HPQ_HIP(hipMemsetAsync(a, 0, sizeof(double) * p->n, c->stream));
         HPQ_HIP(hipMemsetAsync(A.mu, 0, 2 * sizeof(double) * ps, c->stream));
@@ -545,10 +549,12 @@ void adjoint_backward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double
     } catch (...) {
         c->captures.swap(caps);
         c->checksums = checksums;
+        c->insitu = insitu;
         throw;
     }
     c->captures.swap(caps);
     c->checksums = checksums;
+    c->insitu = insitu;
 }
 
 }  // namespace
@@ -619,6 +625,7 @@ void hpq_destroy(hpq_ctx* c) {
     if (c->h_red) (void)hipHostFree(c->h_red);
     adjoint_free(c);
     beam_slip_free(c);
+    insitu_free(c);
     delete c;
 }
 
@@ -933,6 +940,27 @@ int hpq_set_checksums(hpq_ctx* c, double* checksums) {
     return guarded(c, [&] {
         diag_finalize(c);
         c->checksums = checksums;
+    });
+}
+
+int hpq_set_insitu(hpq_ctx* c, const hpq_insitu* d) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        if (!d) {
+            c->insitu = hpq_insitu{};
+            return;
+        }
+        if (d->field_rdata && !d->field_sum) throw HpqError(HPQ_E_INVALID, "insitu: field_sum missing");
+        if (d->field_rdata && !c->p.explicit_solver)
+            throw HpqError(HPQ_E_UNSUPPORTED, "Must use explicit solver for field insitu diagnostic");
+        if (d->plasma_rdata && (!d->plasma_idata || !d->plasma_sum || !d->plasma_isum))
+            throw HpqError(HPQ_E_INVALID, "insitu: plasma arrays missing");
+        if (d->beam_rdata && (!d->beam_idata || !d->beam_sum || !d->beam_isum))
+            throw HpqError(HPQ_E_INVALID, "insitu: beam arrays missing");
+        if ((d->plasma_rdata && !(d->plasma_radius >= 0.0)) || (d->beam_rdata && !(d->beam_radius >= 0.0)))
+            throw HpqError(HPQ_E_INVALID, "insitu: radius must be >= 0");
+        if (d->field_rdata || d->plasma_rdata || d->beam_rdata) insitu_reserve(c);
+        c->insitu = *d;
     });
 }
 

@@ -213,6 +213,9 @@ struct hpq_ctx {
     double* diag_partial = nullptr;  // per-workgroup diagnostic sums not yet in checksums
     int* diag_comp_of = nullptr;
     bool diag_pending = false;
+    hpq_insitu insitu{};             // caller-owned in-situ diagnostic arrays (all NULL: off)
+    double* insitu_partial = nullptr;  // per-workgroup in-situ sums of one slice
+    long insitu_cap = 0;               // workgroups insitu_partial holds
     std::vector<hpq::Capture> captures;
     double stats[4] = {0, 0, 0, 0};
     hpq::Trace trace[HPQ_TRACE_N];
@@ -247,9 +250,17 @@ void plane_add(hpq_ctx* c, double* store, const int* dst, const int* src, int nc
 void plane_lincomb(hpq_ctx* c, double* store, int dst, double a, int sa, double b, int sb);
 void sxsy_from_beam(hpq_ctx* c, double* store);
 void captures(hpq_ctx* c, const double* store, int islice);
-void slice_end(hpq_ctx* c, double* store, const int* shift_dst, const int* shift_src, int nshift, const int* zero,
-               int nzero);
+void slice_end(hpq_ctx* c, double* store, int islice, const int* shift_dst, const int* shift_src, int nshift,
+               const int* zero, int nzero);
 void diag_finalize(hpq_ctx* c);
+constexpr int INSITU_STRIDE = 24;  // doubles per workgroup in insitu_partial
+void insitu_reserve(hpq_ctx* c);
+double* insitu_scratch(hpq_ctx* c, long nblk);
+void insitu_finalize(hpq_ctx* c, int nblk, int nreal, bool count, double scale, uint32_t norm_mask, double* rdata,
+                     double* sum, int32_t* idata, int32_t* isum, int islice);
+void insitu_plasma(hpq_ctx* c, const hpq_plasma* p, int islice);
+void insitu_beam(hpq_ctx* c, const hpq_beam* b, int islice);
+void insitu_free(hpq_ctx* c);
 double rel_b_error(hpq_ctx* c, const double* store, int bx, int by, int bxi, int byi);
 void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p);
 int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof);

@@ -216,6 +216,71 @@ class Simulation:
         _abi.check(self.ctx, self.L.hpq_trace_read(self.ctx, _abi.TRACE[kind], C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    # ---- in-situ reduced diagnostics -------------------------------------------------------
+    def enable_insitu(self, fields: bool = True, plasma: bool = True, beam: bool = True,
+                      plasma_radius: float = math.inf, beam_radius: float = math.inf):
+        """Per-slice reductions on the device during the slice loop (hpq_set_insitu): the
+        fields after the field solves, the plasma at the start of each slice, the beam
+        particles of the slice before its push. plasmas/beams.insitu_radius cut the particles."""
+        nz = self.cfg.nz
+        f64 = dict(dtype=torch.float64, device=self.device)
+        i32 = dict(dtype=torch.int32, device=self.device)
+        buf: Dict[str, torch.Tensor] = {}
+        d = _abi.Insitu()
+        if fields:
+            buf["field_rdata"] = torch.zeros((_abi.INSITU_FIELD_NR, nz), **f64)
+            buf["field_sum"] = torch.zeros(_abi.INSITU_FIELD_NR, **f64)
+        for kind, nr, radius, on in (("plasma", _abi.INSITU_PLASMA_NR, plasma_radius, plasma),
+                                     ("beam", _abi.INSITU_BEAM_NR, beam_radius, beam)):
+            if on:
+                buf[kind + "_rdata"] = torch.zeros((nr, nz), **f64)
+                buf[kind + "_idata"] = torch.zeros(nz, **i32)
+                buf[kind + "_sum"] = torch.zeros(nr, **f64)
+                buf[kind + "_isum"] = torch.zeros(1, **i32)
+                setattr(d, kind + "_radius", float(radius))
+        for k, t in buf.items():
+            setattr(d, k, t.data_ptr())
+        _abi.check(self.ctx, self.L.hpq_set_insitu(self.ctx, C.byref(d)))
+        self._insitu, self._insitu_desc = buf, d
+
+    def disable_insitu(self):
+        _abi.check(self.ctx, self.L.hpq_set_insitu(self.ctx, None))
+        self._insitu = {}
+
+    def insitu_arrays(self) -> Dict[str, np.ndarray]:
+        """Host copies of the in-situ arrays: <group>_rdata (NR, nz), _idata (nz), _sum (NR), _isum (1)."""
+        torch.cuda.synchronize(self.device)
+        return {k: v.cpu().numpy() for k, v in getattr(self, "_insitu", {}).items()}
+
+    def reset_insitu(self):
+        for v in getattr(self, "_insitu", {}).values():
+            v.zero_()
+
+    def write_insitu(self, step: int, time: float, field_prefix: str = "diags/field_insitu",
+                     plasma_prefix: str = "diags/plasma_insitu", beam_prefix: str = "diags/insitu",
+                     plasma_name: str = "plasma", beam_name: str = "beam", rank: int = 0):
+        """Append this step's records (Fields/Plasma/BeamParticleContainer::InSituWriteToFile)
+        and clear the arrays, as the reference does after writing."""
+        import os
+        from . import insitu as io
+        a = self.insitu_arrays()
+        cfg = self.cfg
+        dxdydz = cfg.dx * cfg.dy * cfg.dz
+        if "field_rdata" in a:
+            os.makedirs(field_prefix, exist_ok=True)
+            io.append_record(io.file_name(field_prefix, "fields", rank),
+                             io.field_record(time, step, cfg.zlo, cfg.zhi, a["field_rdata"], a["field_sum"]))
+        for kind, prefix, name, charge, mass in (
+                ("plasma", plasma_prefix, plasma_name, cfg.plasma_charge, cfg.plasma_mass),
+                ("beam", beam_prefix, beam_name, self.beam_host.charge, self.beam_host.mass)):
+            if kind + "_rdata" in a:
+                os.makedirs(prefix, exist_ok=True)
+                io.append_record(io.file_name(prefix, name, rank),
+                                 io.species_record(kind, time, step, charge, mass, cfg.zlo, cfg.zhi, dxdydz,
+                                                   a[kind + "_rdata"], a[kind + "_idata"], a[kind + "_sum"],
+                                                   int(a[kind + "_isum"][0])))
+        self.reset_insitu()
+
     def plasma_host(self) -> Dict[str, np.ndarray]:
         torch.cuda.synchronize(self.device)
         return {k: v[: self.np].cpu().numpy() for k, v in self.plasma.items()}

@@ -281,6 +281,43 @@ int hpq_set_checksums(hpq_ctx *ctx, double *checksums);
 int hpq_add_capture(hpq_ctx *ctx, int comp, double *dst);
 int hpq_clear_captures(hpq_ctx *ctx);
 
+/* In-situ reduced diagnostics, per slice, into caller-owned device arrays; a group whose
+ * rdata is NULL is off. Quantity q of slice s goes to rdata[q*nz + s]; the sums over slices
+ * are added (+=) into sum/isum, which the caller clears after writing a record.
+ * - fields (Fields::InSituComputeDiags, src/fields/Fields.cpp:1289-1346), after the field
+ *   solves: [Ex^2] [Ey^2] [Ez^2] [Bx^2] [By^2] [Bz^2] [ExmBy^2] [EypBx^2] [jz_beam]
+ *   [Ez*jz_beam], each summed over valid cells times dx*dy*dz (explicit solver only);
+ * - plasma (PlasmaParticleContainer::InSituComputeDiags,
+ *   src/particles/plasma/PlasmaParticleContainer.cpp:442-527), at the start of the slice:
+ *   sum(w) [x] [x^2] [y] [y^2] [ux] [ux^2] [uy] [uy^2] [uz] [uz^2] [ga] [ga^2]
+ *   [(ga-1)*(1-vz)] with quasi-static weights w*ga/psi; idata = Np;
+ * - beam (BeamParticleContainer::InSituComputeDiags,
+ *   src/particles/beam/BeamParticleContainer.cpp:475-594), before the beam push, over the
+ *   particles of the slice: sum(w) [x] [x^2] [y] [y^2] [z] [z^2] [ux] [ux^2] [uy] [uy^2]
+ *   [uz] [uz^2] [x*ux] [y*uy] [z*uz] [x*uy] [y*ux] [ux/uz] [uy/uz] [ga] [ga^2]; idata = Np.
+ * Per-slice particle moments are divided by the slice's sum(w) (not sum(w) itself, nor
+ * the plasma's [(ga-1)*(1-vz)]); the sums are not. Only valid particles with
+ * x^2 + y^2 <= radius^2 count (insitu_radius; infinity takes all). */
+#define HPQ_INSITU_FIELD_NR 10
+#define HPQ_INSITU_PLASMA_NR 14
+#define HPQ_INSITU_BEAM_NR 22
+typedef struct hpq_insitu {
+    double *field_rdata;   /* [10*nz] */
+    double *field_sum;     /* [10] */
+    double *plasma_rdata;  /* [14*nz] */
+    int32_t *plasma_idata; /* [nz] */
+    double *plasma_sum;    /* [14] */
+    int32_t *plasma_isum;  /* [1] */
+    double plasma_radius;
+    double *beam_rdata;    /* [22*nz] */
+    int32_t *beam_idata;   /* [nz] */
+    double *beam_sum;      /* [22] */
+    int32_t *beam_isum;    /* [1] */
+    double beam_radius;
+} hpq_insitu;
+/* NULL turns every group off. */
+int hpq_set_insitu(hpq_ctx *ctx, const hpq_insitu *d);
+
 /* Kernel timing (the reference's HIPACE_PROFILE regions, src/utils/HipaceProfilerWrapper.H):
  * when enabled for a kernel family, every launch of it is bracketed by HIP events on the
  * context's stream; hpq_trace_read synchronises the stream and returns the summed kernel
