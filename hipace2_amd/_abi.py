@@ -61,7 +61,8 @@ ADJ_NAMES = ["x", "y", "ux", "uy", "psi", "ux_half_step", "uy_half_step", "psi_h
 
 class BeamState(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("x", "y", "z", "ux", "uy", "uz", "w", "valid", "nsubcycles")] + \
-               [("off", C.POINTER(C.c_int64)), ("n", C.c_int64), ("charge", C.c_double), ("mass", C.c_double)]
+               [("off", C.POINTER(C.c_int64)), ("n", C.c_int64), ("charge", C.c_double), ("mass", C.c_double),
+                ("external_fields", C.c_double * 24), ("use_external_fields", C.c_int)]
 
 
 class PlasmaAdj(C.Structure):

@@ -217,6 +217,22 @@ __device__ void push_beam_particle(const Geom& g, const hpq_beam_state& b, long 
                 Byp += (X.s * Y.s) * By[k];
                 Bzp += (X.s * Y.s) * Bz[k];
             }
+        if (b.use_external_fields) {
+            // ApplyExternalField, src/particles/pusher/ExternalFields.H:43-55
+            const double* e = b.external_fields;
+            const double Ex = ((e[0] + e[1] * xp) + e[2] * yp) + e[3] * zp;
+            const double Ey = ((e[4] + e[5] * xp) + e[6] * yp) + e[7] * zp;
+            const double Eze = ((e[8] + e[9] * xp) + e[10] * yp) + e[11] * zp;
+            const double Bxe = ((e[12] + e[13] * xp) + e[14] * yp) + e[15] * zp;
+            const double Bye = ((e[16] + e[17] * xp) + e[18] * yp) + e[19] * zp;
+            const double Bze = ((e[20] + e[21] * xp) + e[22] * yp) + e[23] * zp;
+            ExmByp += Ex - clight * Bye;
+            EypBxp += Ey + clight * Bxe;
+            Ezp += Eze;
+            Bxp += Bxe;
+            Byp += Bye;
+            Bzp += Bze;
+        }
         const double ux_next = ux + dt * qm * (ExmByp + (clight - uz * gammap_inv) * Byp + uy * gammap_inv * Bzp);
         const double uy_next = uy + dt * qm * (EypBxp + (uz * gammap_inv - clight) * Bxp - ux * gammap_inv * Bzp);
         const double ux_i = (ux_next + ux) * 0.5;

@@ -17,6 +17,7 @@ Beams are generated on the host (numpy) and handed to the slice loop sorted into
 """
 from __future__ import annotations
 
+import ast
 import dataclasses
 import math
 import re
@@ -74,6 +75,11 @@ class SliceConfig:
     # transverse Poisson solves: "fft" (DST-I by FFT, FP64) or "bf16x3" (DST-I as sine-matrix
     # GEMMs on the matrix cores, FP32 operands split into two BF16 terms each)
     poisson_solver: str = "fft"
+    # time step and beam push (hipace.dt; <beam>.n_subcycles, <beam>.do_z_push,
+    # src/particles/beam/BeamParticleContainer.cpp:42-90)
+    dt: float = 0.0
+    n_subcycles: int = 10
+    do_z_push: bool = True
 
     @property
     def dx(self) -> float:
@@ -116,6 +122,9 @@ class BeamSpec:
     mass: float = 1.0
     seed: int = 0
     min_density: float = 0.0
+    # beams.external_E/B(x,y,z,t) as (6, 4) affine coefficients (Ex, Ey, Ez, Bx, By, Bz) x
+    # (1, x, y, z), or None
+    external: Optional[np.ndarray] = None
 
 
 @dataclasses.dataclass
@@ -132,6 +141,7 @@ class BeamArrays:
     off: np.ndarray
     charge: float = -1.0
     mass: float = 1.0
+    external: Optional[np.ndarray] = None   # (6, 4) affine external fields, or None
 
     @property
     def n(self) -> int:
@@ -288,10 +298,13 @@ def make_beam(cfg: SliceConfig, spec: Optional[BeamSpec]) -> BeamArrays:
     if spec is None:
         return BeamArrays.empty(cfg.nz)
     if spec.injection_type == "fixed_ppc":
-        return make_fixed_ppc_beam(cfg, spec)
-    if spec.injection_type == "fixed_weight":
-        return make_fixed_weight_beam(cfg, spec)
-    raise ValueError(f"unsupported beam injection_type {spec.injection_type!r}")
+        b = make_fixed_ppc_beam(cfg, spec)
+    elif spec.injection_type == "fixed_weight":
+        b = make_fixed_weight_beam(cfg, spec)
+    else:
+        raise ValueError(f"unsupported beam injection_type {spec.injection_type!r}")
+    b.external = None if spec.external is None else np.array(spec.external, dtype=np.float64)
+    return b
 
 
 def merge_beams(cfg: SliceConfig, beams) -> BeamArrays:
@@ -301,6 +314,9 @@ def merge_beams(cfg: SliceConfig, beams) -> BeamArrays:
         return BeamArrays.empty(cfg.nz)
     if len(beams) == 1:
         return beams[0]
+    ext = [b.external for b in beams]
+    if any(e is not None for e in ext) and not all(e is not None and np.array_equal(e, ext[0]) for e in ext):
+        raise ValueError("merged beams must share their external fields")
     fields = ("x", "y", "z", "ux", "uy", "uz", "w", "valid")
     out = {f: [] for f in fields}
     off = np.zeros(cfg.nz + 1, np.int64)
@@ -312,7 +328,7 @@ def merge_beams(cfg: SliceConfig, beams) -> BeamArrays:
                 out[f].append(getattr(b, f)[a:e])
             cnt += e - a
         off[s + 1] = off[s] + cnt
-    return BeamArrays(*(np.concatenate(out[f]) for f in fields), off, beams[0].charge, beams[0].mass)
+    return BeamArrays(*(np.concatenate(out[f]) for f in fields), off, beams[0].charge, beams[0].mass, ext[0])
 
 
 # --------------------------------------------------------------------------------------
@@ -339,6 +355,68 @@ def _num(v: str) -> float:
     if re.fullmatch(r"[-+0-9.eE]+", v):
         return float(v)
     raise ValueError(f"only numeric values are supported here, got {v!r}")
+
+
+_BINOPS = {ast.Add: lambda a, b: a + b, ast.Sub: lambda a, b: a - b}
+
+
+def _affine(node, names):
+    """An expression of the AMReX parser's arithmetic as an affine form: array
+    (c, d_x, d_y, d_z, d_t) with value c + d_x x + d_y y + d_z z + d_t t. Products and
+    quotients need a constant factor (or divisor); powers need constant operands."""
+    if isinstance(node, ast.Expression):
+        return _affine(node.body, names)
+    if isinstance(node, ast.Constant) and isinstance(node.value, (int, float)):
+        return np.array([float(node.value), 0, 0, 0, 0])
+    if isinstance(node, ast.Name) and node.id in names:
+        v = np.zeros(5)
+        v[1 + names.index(node.id)] = 1.0
+        return v
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+        v = _affine(node.operand, names)
+        return -v if isinstance(node.op, ast.USub) else v
+    if isinstance(node, ast.BinOp):
+        a, b = _affine(node.left, names), _affine(node.right, names)
+        const = lambda v: not np.any(v[1:])
+        if type(node.op) in _BINOPS:
+            return _BINOPS[type(node.op)](a, b)
+        if isinstance(node.op, ast.Mult):
+            if const(a):
+                return a[0] * b
+            if const(b):
+                return a * b[0]
+        if isinstance(node.op, ast.Div) and const(b):
+            return a / b[0]
+        if isinstance(node.op, ast.Pow) and const(a) and const(b):
+            return np.array([a[0] ** b[0], 0, 0, 0, 0])
+        raise ValueError("external fields must be affine in x, y, z on this path")
+    raise ValueError(f"unsupported term in an external-field expression: {ast.dump(node)}")
+
+
+def parse_affine(expr: str) -> np.ndarray:
+    """(c0, cx, cy, cz) of an affine, time-independent function of (x, y, z) written in the
+    reference's parser syntax (`^` for powers)."""
+    tree = ast.parse(expr.strip().strip('"').replace("^", "**"), mode="eval")
+    v = _affine(tree, ["x", "y", "z", "t"])
+    if v[4] != 0.0:
+        raise ValueError("time-dependent external fields are not on this path")
+    return v[:4]
+
+
+def external_fields(e_exprs, b_exprs) -> Optional[np.ndarray]:
+    """(6, 4) coefficients of beams.external_E(x,y,z,t) and external_B(x,y,z,t)
+    (src/particles/beam/BeamParticleContainer.cpp:72-87); None when neither is given."""
+    if e_exprs is None and b_exprs is None:
+        return None
+    out = np.zeros((6, 4))
+    for k, exprs in ((0, e_exprs), (3, b_exprs)):
+        if exprs is None:
+            continue
+        if len(exprs) != 3:
+            raise ValueError("external fields take three components")
+        for q in range(3):
+            out[k + q] = parse_affine(exprs[q])
+    return out
 
 
 def parse_deck(text: str, overrides: Optional[Dict[str, str]] = None):
@@ -408,6 +486,12 @@ def parse_deck(text: str, overrides: Optional[Dict[str, str]] = None):
             spec.num_particles = int(_num(g("num_particles")))
         if g("element", "electron") != "electron":
             raise ValueError("electron beams only on this path")
+        cfg.n_subcycles = int(_num(g("n_subcycles", "10")))
+        cfg.do_z_push = _num(g("do_z_push", "1")) != 0
+        ext_e, ext_b = g("external_E(x,y,z,t)"), g("external_B(x,y,z,t)")
+        spec.external = external_fields(None if ext_e is None else _tokens(ext_e),
+                                        None if ext_b is None else _tokens(ext_b))
         beams.append(spec)
+    cfg.dt = _num(kv.get("hipace.dt", "0"))
     max_step = int(_num(kv.get("max_step", "0")))
     return cfg, beams, max_step

@@ -112,6 +112,11 @@ class Simulation:
                                         ("x", "y", "z", "ux", "uy", "uz", "w", "valid", "nsub")],
                                       C.cast(self._boff, C.POINTER(C.c_int64)), self.beam_host.n,
                                       self.beam_host.charge, self.beam_host.mass)
+        ext = self.beam_host.external
+        if ext is not None:
+            for k, v in enumerate(np.asarray(ext, dtype=np.float64).reshape(24)):
+                self._bsdesc.external_fields[k] = float(v)
+            self._bsdesc.use_external_fields = 1
         return self._bsdesc
 
     def beam_begin_step(self):

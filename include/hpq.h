@@ -104,6 +104,13 @@ typedef struct hpq_beam_state {
     const int64_t *off;
     int64_t n;
     double charge, mass;
+    /* beams.external_E(x,y,z,t) / external_B(x,y,z,t) (BeamParticleContainer.cpp:72-87), added
+     * to the gathered fields by the push as ApplyExternalField does
+     * (src/particles/pusher/ExternalFields.H:28-56), for fields affine in position: component k
+     * of (Ex, Ey, Ez, Bx, By, Bz) is ((e[4k] + e[4k+1]*x) + e[4k+2]*y) + e[4k+3]*z.
+     * use_external_fields = 0: none. */
+    double external_fields[24];
+    int use_external_fields;
 } hpq_beam_state;
 
 /* Adjoints of plasma quantities for the vector-Jacobian products below: device arrays of

@@ -80,6 +80,7 @@ struct Beam {
     std::vector<int32_t> valid;
     std::vector<int64_t> off;   // nz+1
     double charge = -1.0, mass = 1.0;
+    std::vector<double> ext;    // 24 affine external-field coefficients, or empty
 };
 
 enum ParticleBC { BC_REFLECTING = 0, BC_PERIODIC = 1, BC_ABSORBING = 2 };
@@ -125,6 +126,8 @@ struct BeamRef {
     double *x, *y, *z, *ux, *uy, *uz, *w;
     int32_t *valid, *nsub;
     double charge, mass;
+    // external fields affine in position, (Ex, Ey, Ez, Bx, By, Bz) x (1, x, y, z); null: none
+    const double* ext = nullptr;
 };
 void push_beam_slice(const Ctx& c, const double* F, const BeamRef& b, long first, long count, const int32_t* slip_in,
                      long n_in, std::vector<int32_t>& slip_out, int slice, double dt_step, int n_subcycles,

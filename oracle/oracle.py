@@ -84,7 +84,8 @@ def lib():
         i32p = C.POINTER(C.c_int32)
         L.orc_push_beam_slice.argtypes = [vp] + [dp] * 7 + [i32p, i32p, C.c_double, C.c_double, C.c_long, C.c_long,
                                                           i32p, C.c_long, i32p, C.c_long, C.c_int, C.c_double, C.c_int,
-                                                          C.c_int]
+                                                          C.c_int, dp]
+        L.orc_sim_set_beam_external.argtypes = [vp, dp]
         L.orc_push_beam_slice.restype = C.c_long
         L.orc_sim_run_step_with_beam.argtypes = [vp, C.c_double, C.c_int, C.c_int]
         L.orc_sim_sort_beam.argtypes = [vp]
@@ -160,6 +161,9 @@ class OracleSim:
         lib().orc_sim_set_beam(self.h, b.n, *[_dp(a) for a in self._beam],
                                self._bvalid.ctypes.data_as(C.POINTER(C.c_int32)),
                                self._boff.ctypes.data_as(C.POINTER(C.c_int64)), b.charge, b.mass)
+        ext = getattr(b, "external", None)
+        self._bext = None if ext is None else np.ascontiguousarray(ext, dtype=np.float64).reshape(24)
+        lib().orc_sim_set_beam_external(self.h, None if self._bext is None else _dp(self._bext))
 
     def capture(self, name: str) -> np.ndarray:
         """Record the valid region of component `name` at the diagnostics point of every slice."""
@@ -263,10 +267,12 @@ class OracleSim:
         slip_in = np.ascontiguousarray(slip_in, dtype=np.int32)
         cap = b["x"].shape[0]
         out = np.zeros(max(cap, 1), np.int32)
+        ext = b.get("external")
+        ext = None if ext is None else np.ascontiguousarray(ext, dtype=np.float64).reshape(24)
         n = lib().orc_push_beam_slice(self.h, *[_dp(b[k]) for k in ("x", "y", "z", "ux", "uy", "uz", "w")],
                                       i32(b["valid"]), i32(b["nsub"]), b.get("charge", -1.0), b.get("mass", 1.0),
                                       first, count, i32(slip_in), slip_in.shape[0], i32(out), cap, slice_, dt,
-                                      n_subcycles, int(do_z_push))
+                                      n_subcycles, int(do_z_push), None if ext is None else _dp(ext))
         return out[:n].copy()
 
     def solve_poisson_psi_ez_bz(self):

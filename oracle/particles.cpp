@@ -288,7 +288,8 @@ void push_plasma(const Ctx& c, Plasma& p, const double* F, bool temp_slice, int 
 }
 
 // AdvanceBeamParticlesSlice, src/particles/pusher/BeamParticleAdvance.cpp:20-336 (no
-// radiation reaction, spin tracking or external fields), for the particles [first, first+count)
+// radiation reaction or spin tracking; external fields affine in position, ApplyExternalField
+// src/particles/pusher/ExternalFields.H:28-56), for the particles [first, first+count)
 // and then the listed ones that slipped into this slice. A particle below min_z (the lower end
 // of the slice) stops sub-cycling (:152-155); with sub-cycles left it is listed for the next
 // slice, the role of shiftSlippedParticles (src/particles/sorting/SliceSort.cpp:13-70).
@@ -338,6 +339,19 @@ void push_beam_slice(const Ctx& c, const double* F, const BeamRef& b, long first
                     Byp += (sx.s * sy.s) * By[k];
                     Bzp += (sx.s * sy.s) * Bz[k];
                 }
+            }
+            if (b.ext) {
+                double f[6];
+                for (int q = 0; q < 6; ++q) {
+                    const double* e = b.ext + 4 * q;
+                    f[q] = ((e[0] + e[1] * xp) + e[2] * yp) + e[3] * zp;
+                }
+                ExmByp += f[0] - clight * f[4];
+                EypBxp += f[1] + clight * f[3];
+                Ezp += f[2];
+                Bxp += f[3];
+                Byp += f[4];
+                Bzp += f[5];
             }
             const double ux_next = ux + dt * qm * (ExmByp + (clight - uz * gammap_inv) * Byp + uy * gammap_inv * Bzp);
             const double uy_next = uy + dt * qm * (EypBxp + (uz * gammap_inv - clight) * Bxp - ux * gammap_inv * Bzp);

@@ -321,6 +321,14 @@ void orc_sim_set_beam(void* h, int64_t n, const double* x, const double* y, cons
     b.w.assign(w, w + n); b.valid.assign(valid, valid + n);
     b.off.assign(off, off + s->c.g.nz + 1);
     b.charge = charge; b.mass = mass;
+    b.ext.clear();
+}
+
+// beams.external_E / external_B as affine functions of position (24 coefficients; NULL: none)
+void orc_sim_set_beam_external(void* h, const double* ext) {
+    Beam& b = static_cast<Sim*>(h)->beam;
+    if (ext) b.ext.assign(ext, ext + 24);
+    else b.ext.clear();
 }
 
 void orc_sim_add_capture(void* h, int comp, double* buf) { static_cast<Sim*>(h)->captures.push_back({comp, buf}); }
@@ -429,9 +437,9 @@ void orc_dshape(int type, int order, double xmid, int ix, double* out) {
 long orc_push_beam_slice(void* h, double* x, double* y, double* z, double* ux, double* uy, double* uz, double* w,
                          int32_t* valid, int32_t* nsub, double charge, double mass, long first, long count,
                          const int32_t* slip_in, long n_in, int32_t* slip_out, long n_slip_cap, int slice, double dt,
-                         int n_subcycles, int do_z_push) {
+                         int n_subcycles, int do_z_push, const double* ext) {
     Sim* s = static_cast<Sim*>(h);
-    BeamRef b{x, y, z, ux, uy, uz, w, valid, nsub, charge, mass};
+    BeamRef b{x, y, z, ux, uy, uz, w, valid, nsub, charge, mass, ext};
     std::vector<int32_t> out;
     push_beam_slice(s->c, s->F.data(), b, first, count, slip_in, n_in, out, slice, dt, n_subcycles, do_z_push != 0);
     const long n = std::min<long>(long(out.size()), n_slip_cap);
@@ -447,7 +455,7 @@ void orc_sim_run_step_with_beam(void* h, double dt, int n_subcycles, int do_z_pu
     const long n = long(b.x.size());
     std::vector<int32_t> nsub(n, 0), slip_in, slip_out;
     BeamRef r{b.x.data(), b.y.data(), b.z.data(), b.ux.data(), b.uy.data(), b.uz.data(), b.w.data(), b.valid.data(),
-              nsub.data(), b.charge, b.mass};
+              nsub.data(), b.charge, b.mass, b.ext.empty() ? nullptr : b.ext.data()};
     for (int isl = s->c.g.nz - 1; isl >= 0; --isl) {
         solve_one_slice(*s, isl);
         slip_out.clear();

@@ -32,6 +32,15 @@ CASES = [
     ("blowout_wake.Serial", "blowout_wake", {"hipace.tile_size": "8"}, 2e-2),
     # max_step=1 on two ranks with dt = 0: the second step repeats the first one
     ("blowout_wake_explicit.2Rank", "blowout_wake", {"hipace.tile_size": "8", "max_step": "1"}, 1e-9),
+    ("beam_in_vacuum.normalized.Serial", "beam_in_vacuum",
+     {"hipace.tile_size": "8", "hipace.depos_order_xy": "0", "diagnostic.field_data": "all rho"}, 1e-9),
+    # 21 steps (0..20) of a beam in vacuum in an external focusing field, beam pushed after
+    # every slice; the checksums are those of the step-20 output
+    ("beam_evolution.1Rank", "beam_in_vacuum",
+     {"hipace.tile_size": "8", "amr.n_cell": "32 32 10", "max_step": "20", "geometry.prob_lo": "-2. -2. -2.",
+      "geometry.prob_hi": "2.  2.  2.", "hipace.dt": "3.", "diagnostic.output_period": "20",
+      "beam.density": "1.e-8", "beam.radius": "1.", "beam.ppc": "4 4 1",
+      "beams.external_E(x,y,z,t)": ".5*x .5*y 0."}, 1e-12),
 ]
 
 

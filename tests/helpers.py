@@ -21,10 +21,36 @@ def golden_cases():
 
 def deck_config(case):
     """(cfg, beam) for one golden case of tests/golden/reference_checksums.json."""
-    text = "\n".join(f"{k} = {v}" for k, v in case["deck"].items())
-    cfg, beams, _ = parse_deck(text, case["overrides"])
-    beam = merge_beams(cfg, [make_beam(cfg, b) for b in beams])
+    cfg, beam, _ = deck_run(case)
     return cfg, beam
+
+
+def deck_run(case):
+    """(cfg, beam, max_step) for one golden case."""
+    text = "\n".join(f"{k} = {v}" for k, v in case["deck"].items())
+    cfg, beams, max_step = parse_deck(text, case["overrides"])
+    beam = merge_beams(cfg, [make_beam(cfg, b) for b in beams])
+    return cfg, beam, max_step
+
+
+def run_golden(sim, cfg, max_step):
+    """Evolve's steps 0..max_step as the reference's test ran them; the output (checksums) is
+    that of the last step, whose fields do not depend on that step's beam push. With dt = 0
+    every step repeats the first, so one step stands for all. `sim` is a Simulation or an
+    OracleSim; afterwards its beam holds the state the last step's output was taken from."""
+    if cfg.dt == 0.0 or max_step == 0:
+        sim.run_step()
+        return
+    for step in range(max_step + 1):
+        if step > 0:
+            sim.sort_beam()
+        if step == max_step:
+            sim.run_step()
+        elif hasattr(sim, "ctx"):
+            sim.run_step_with_beam(cfg.dt, cfg.n_subcycles, cfg.do_z_push)
+        else:
+            sim.begin_step()
+            sim.run_step_with_beam(cfg.dt, cfg.n_subcycles, cfg.do_z_push)
 
 
 def small_config(n=63, nz=16, order=2, dtype=2, explicit=True, L=6.0):

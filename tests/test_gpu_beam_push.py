@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 KEYS = ("x", "y", "z", "ux", "uy", "uz", "w")
 
 
-def _setup(order, bc="Periodic", n_per_slice=300, seed=0):
+def _setup(order, bc="Periodic", n_per_slice=300, seed=0, external=None):
     from hipace2_amd.inputs import BeamArrays
     from hipace2_amd.simulation import Simulation
     from oracle.oracle import OracleSim
@@ -29,6 +29,7 @@ def _setup(order, bc="Periodic", n_per_slice=300, seed=0):
                       rng.normal(0, 2, n), rng.normal(0, 2, n),
                       np.where(rng.uniform(0, 1, n) < 0.3, rng.uniform(0.5, 3.0, n), rng.uniform(100, 3000, n)),
                       np.ones(n), np.ones(n, np.int32), np.arange(nz + 1, dtype=np.int64) * n_per_slice)
+    beam.external = external
     sim = Simulation(cfg, beam)
     orc = OracleSim(cfg)
     st = random_store(sim.ncomp, sim.ldy, sim.ldx, seed=seed + 1, scale=0.3)
@@ -38,12 +39,19 @@ def _setup(order, bc="Periodic", n_per_slice=300, seed=0):
     host["valid"] = beam.valid.copy()
     host["nsub"] = np.zeros(n, np.int32)
     host["charge"], host["mass"] = beam.charge, beam.mass
+    host["external"] = external
     return cfg, beam, sim, orc, host
 
 
-@pytest.mark.parametrize("order,bc", [(2, "Periodic"), (1, "Reflecting"), (3, "Absorbing"), (0, "Periodic")])
-def test_beam_push_matches_oracle(order, bc):
-    cfg, beam, sim, orc, host = _setup(order, bc)
+# external fields affine in x, y, z on every component (beams.external_E/B)
+EXTERNAL = np.array([[0.1, 0.5, -0.2, 0.05], [-0.1, 0.3, 0.5, 0.0], [0.02, 0.0, 0.1, -0.3],
+                     [0.0, 0.1, 0.2, 0.0], [0.05, -0.4, 0.0, 0.1], [0.3, 0.0, 0.0, 0.2]])
+
+
+@pytest.mark.parametrize("order,bc,ext", [(2, "Periodic", False), (1, "Reflecting", False), (3, "Absorbing", False),
+                                          (0, "Periodic", False), (2, "Periodic", True)])
+def test_beam_push_matches_oracle(order, bc, ext):
+    cfg, beam, sim, orc, host = _setup(order, bc, external=EXTERNAL if ext else None)
     sim.beam_begin_step()
     dt, nsub = 2.5, 10
     slip = np.zeros(0, np.int32)

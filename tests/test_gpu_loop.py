@@ -8,22 +8,32 @@ import pytest
 
 torch = pytest.importorskip("torch")
 
-from helpers import deck_config, golden_cases, rel_err, small_config  # noqa: E402
+from helpers import deck_run, golden_cases, rel_err, run_golden, small_config  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
 CASES = ["linear_wake.normalized.1Rank", "gaussian_linear_wake.normalized.1Rank",
-         "beam_in_vacuum.normalized.1Rank", "blowout_wake_explicit.2Rank"]
+         "beam_in_vacuum.normalized.1Rank", "beam_in_vacuum.normalized.Serial", "blowout_wake_explicit.2Rank",
+         "beam_evolution.1Rank"]
 
 
 @pytest.mark.parametrize("name", CASES)
 def test_reference_golden_checksums(name):
     from hipace2_amd.simulation import Simulation
     case = golden_cases()[name]
-    cfg, beam = deck_config(case)
+    cfg, beam, max_step = deck_run(case)
     sim = Simulation(cfg, beam)
-    sim.run_step()
+    run_golden(sim, cfg, max_step)
     got = sim.checksums()
+    if cfg.dt != 0.0 and max_step > 0:
+        # the beam after its pushes: sum |attribute| over valid particles (the reference's
+        # particle checksums), within the script's tolerance
+        torch.cuda.synchronize()
+        m = sim.beam["valid"][: beam.n].cpu().numpy().astype(bool)
+        assert int(m.sum()) == int(case["beam"]["charge"])
+        for k in ("x", "y", "z", "ux", "uy", "uz", "w"):
+            v = float(np.abs(sim.beam[k][: beam.n].cpu().numpy()[m]).sum())
+            assert math.isclose(v, case["beam"][k], rel_tol=max(case["rtol"], 1e-11)), (k, v, case["beam"][k])
     # the reference script's tolerance, floored at 1e-11: the GPU sums atomically in
     # a different order than the reference's CPU build
     rtol = max(case["rtol"], 1e-11)

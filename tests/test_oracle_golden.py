@@ -6,7 +6,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import deck_config, golden_cases
+from helpers import deck_config, deck_run, golden_cases, run_golden
 
 CASES = sorted(golden_cases().keys())
 
@@ -15,9 +15,9 @@ CASES = sorted(golden_cases().keys())
 def test_oracle_reproduces_reference_field_checksums(name):
     from oracle.oracle import OracleSim
     case = golden_cases()[name]
-    cfg, beam = deck_config(case)
+    cfg, beam, max_step = deck_run(case)
     orc = OracleSim(cfg, beam, nthreads=4)
-    orc.run_step()
+    run_golden(orc, cfg, max_step)
     got = orc.checksums()
     # blowout_wake.Serial was produced by an older build and carries its own 2e-2 tolerance;
     # every other golden is matched to 1e-12 (the goldens have 14 significant digits)
@@ -40,3 +40,26 @@ def test_beam_generator_matches_reference_beam_checksums(name):
     assert math.isclose(np.abs(beam.y[m]).sum(), g["y"], rel_tol=1e-12)
     assert math.isclose(np.abs(beam.z[m]).sum(), g["z"], rel_tol=1e-12)
     assert math.isclose(np.abs(beam.uz[m]).sum(), g["uz"], rel_tol=1e-12)
+
+
+def beam_checksums(b):
+    """sum |attribute| over the valid beam particles (the reference's particle checksums,
+    tests/checksum/backend/openpmd_backend.py:47-62)."""
+    m = np.asarray(b["valid"]).astype(bool)
+    return {k: float(np.abs(np.asarray(b[k])[m]).sum()) for k in ("x", "y", "z", "ux", "uy", "uz", "w")} | \
+        {"charge": float(m.sum())}
+
+
+def test_oracle_reproduces_beam_evolution():
+    """beam_evolution.1Rank: 20 beam pushes (10 sub-cycles each) in an external focusing field
+    E = (x/2, y/2, 0); the golden beam checksums pin the beam push against the reference."""
+    from oracle.oracle import OracleSim
+    case = golden_cases()["beam_evolution.1Rank"]
+    cfg, beam, max_step = deck_run(case)
+    assert max_step == 20 and cfg.dt == 3.0 and beam.external is not None
+    orc = OracleSim(cfg, beam, nthreads=4)
+    run_golden(orc, cfg, max_step)
+    got = beam_checksums(orc.beam())
+    for k, want in case["beam"].items():
+        if k in got:
+            assert math.isclose(got[k], want, rel_tol=1e-12), (k, got[k], want)
