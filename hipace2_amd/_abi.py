@@ -18,7 +18,10 @@ ERRORS = {1: "HPQ_E_INVALID", 2: "HPQ_E_HIP", 3: "HPQ_E_MG_NOT_CONVERGED", 4: "H
 LAYOUT_NAMES = ["ncomp", "next_jx_beam", "next_jy_beam", "chi", "Sy", "Sx", "ExmBy", "EypBx", "Ez", "Bx",
                 "By", "Bz", "Psi", "jx_beam", "jy_beam", "jz_beam", "jx", "jy", "rhomjz", "rho",
                 "prev_jx_beam", "prev_jy_beam", "ions_rhomjz", "next_jx", "next_jy", "jz", "prev_Bx",
-                "prev_By", "prev_jx", "prev_jy", "pciter_Bx", "pciter_By"]
+                "prev_By", "prev_jx", "prev_jy", "pciter_Bx", "pciter_By",
+                "salame_Ez_target", "salame_Ez_no_salame", "salame_Ez", "salame_jx", "salame_jy",
+                "salame_jz_beam", "salame_Bx", "salame_By", "salame_Sy", "salame_Sx", "salame_Sy_back",
+                "salame_Sx_back"]
 
 EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq_layout", "hpq_store_doubles",
            "hpq_deposit_current", "hpq_explicit_deposition", "hpq_deposit_current_slice",
@@ -29,7 +32,7 @@ EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq
            "hpq_trace_reset", "hpq_bin_tiles", "hpq_deposit_current_vjp", "hpq_explicit_deposition_vjp",
            "hpq_advance_plasma_particles_vjp", "hpq_adjoint_forward", "hpq_adjoint_backward",
            "hpq_beam_begin_step", "hpq_advance_beam_slice", "hpq_sort_beam_slices", "hpq_run_step_with_beam",
-           "hpq_set_insitu"]
+           "hpq_set_insitu", "hpq_set_salame"]
 
 TRACE = {"push": 0, "deposit": 1, "explicit_deposit": 2, "beam_deposit": 3, "mg_smooth": 4, "mg_coarse": 5,
          "dst": 6}
@@ -45,7 +48,8 @@ class Params(C.Structure):
                 ("particle_bc", C.c_int), ("plo", C.c_double * 2), ("phi", C.c_double * 2),
                 ("pc_tol", C.c_double), ("pc_maxiter", C.c_int), ("pc_mix", C.c_double),
                 ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double),
-                ("bin_period", C.c_int), ("precision", C.c_int), ("poisson_solver", C.c_int)]
+                ("bin_period", C.c_int), ("precision", C.c_int), ("poisson_solver", C.c_int),
+                ("salame", C.c_int)]
 
 PRECISION = {"fp64": 0, "mixed": 1}
 POISSON_SOLVER = {"fft": 0, "bf16x3": 1}
@@ -83,6 +87,15 @@ class Insitu(C.Structure):
                 ("plasma_isum", C.c_void_p), ("plasma_radius", C.c_double),
                 ("beam_rdata", C.c_void_p), ("beam_idata", C.c_void_p), ("beam_sum", C.c_void_p),
                 ("beam_isum", C.c_void_p), ("beam_radius", C.c_double)]
+
+
+SALAME_TARGET = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p)
+
+
+class Salame(C.Structure):
+    _fields_ = [("mask", C.c_void_p), ("w", C.c_void_p), ("valid", C.c_void_p), ("n_iter", C.c_int),
+                ("do_advance", C.c_int), ("relative_tolerance", C.c_double), ("target", SALAME_TARGET),
+                ("user", C.c_void_p), ("weight_factor", C.POINTER(C.c_double))]
 
 
 class PlasmaProfile(C.Structure):
@@ -135,6 +148,7 @@ def lib():
         L.hpq_set_checksums.argtypes = [vp, vp]
         L.hpq_add_capture.argtypes = [vp, ip, vp]
         L.hpq_set_insitu.argtypes = [vp, C.POINTER(Insitu)]
+        L.hpq_set_salame.argtypes = [vp, C.POINTER(Salame)]
         L.hpq_clear_captures.argtypes = [vp]
         L.hpq_stats.argtypes = [vp, dp]
         L.hpq_trace_enable.argtypes = [vp, ip, ip]

@@ -18,7 +18,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhpq.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["poisson.hip", "mg.hip", "particles.hip", "deposit.hip", "bins.hip", "dst_mfma.hip", "adjoint.hip", "insitu.hip", "fields.hip", "hipace.cpp"]
+SOURCES = ["poisson.hip", "mg.hip", "particles.hip", "deposit.hip", "bins.hip", "dst_mfma.hip", "adjoint.hip", "insitu.hip", "salame.hip", "fields.hip", "hipace.cpp"]
 HEADERS = ["internal.hpp", "shape.hpp", "push.hpp", "tangent.hpp", os.path.join("..", "..", "include", "hpq.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

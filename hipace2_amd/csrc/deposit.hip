@@ -354,7 +354,8 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
 // ::DepositCurrentSlice deposit functor, src/particles/deposition/BeamDepositCurrent.cpp:133-193
 template <int O>
 __global__ __launch_bounds__(DT) void k_deposit_beam(Geom g, hpq_beam b, long first, long count,
-                                                     double* __restrict__ store, int cjx, int cjy, int cjz) {
+                                                     double* __restrict__ store, int cjx, int cjy, int cjz,
+                                                     const int32_t* __restrict__ select, int select_value) {
     constexpr int S = O + 1;
     __shared__ double win[WIN_BEAM];
     __shared__ int red[7 * DT / 64];
@@ -373,6 +374,7 @@ __global__ __launch_bounds__(DT) void k_deposit_beam(Geom g, hpq_beam b, long fi
         if (t >= count) continue;
         const long ip = first + t;
         if (!b.valid[ip]) continue;
+        if (select && select[ip] != select_value) continue;
         act[q] = true;
         const double ux = b.ux[ip], uy = b.uy[ip], uz = b.uz[ip];
         const double gaminv = 1.0 / sqrt(1.0 + ux * ux * clightsq + uy * uy * clightsq + uz * uz * clightsq);
@@ -489,6 +491,12 @@ void explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store) {
 }
 
 void deposit_beam(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const int32_t* comps) {
+    deposit_beam_select(c, b, slice, store, comps, nullptr, 0);
+}
+
+// only the particles whose select entry equals select_value deposit (select NULL: all)
+void deposit_beam_select(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const int32_t* comps,
+                         const int32_t* select, int select_value) {
     if (!b || slice < 0 || slice >= c->g.nz || !b->off) return;
     const long first = b->off[slice];
     const long count = b->off[slice + 1] - first;
@@ -498,7 +506,7 @@ void deposit_beam(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const
     TraceScope ts(c, HPQ_TRACE_BEAM_DEPOSIT);
     dispatch_order(c->p.depos_order, [&](auto O) {
         hipLaunchKernelGGL((k_deposit_beam<decltype(O)::value>), grid1d(count, DCHUNK), dim3(DT), 0, c->stream, c->g,
-                           *b, first, count, store, comps[0], comps[1], comps[2]);
+                           *b, first, count, store, comps[0], comps[1], comps[2], select, select_value);
     });
     HPQ_HIP(hipGetLastError());
 }

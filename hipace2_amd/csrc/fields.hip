@@ -266,12 +266,14 @@ void sxsy_from_beam(hpq_ctx* c, double* store) {
 }
 
 // Components summed by the field diagnostics: every plane of the layout except the
-// scratch planes of the slice shift and of the predictor-corrector loop.
+// scratch planes of the slice shift, of the predictor-corrector loop and of SALAME.
 void diag_comps(const hpq_ctx* c, std::vector<int>& out) {
     const Layout& L = c->L;
     const int skip[] = {L.next_jx_beam, L.next_jy_beam, L.prev_jx_beam, L.prev_jy_beam, L.ions_rhomjz, L.next_jx,
                         L.next_jy,      L.prev_Bx,      L.prev_By,      L.prev_jx,      L.prev_jy,     L.pciter_Bx,
-                        L.pciter_By,    L.pcprev_Bx,    L.pcprev_By};
+                        L.pciter_By,    L.pcprev_Bx,    L.pcprev_By,   L.sal_Ez_target, L.sal_Ez_no_salame,
+                        L.sal_Ez,       L.sal_jx,       L.sal_jy,      L.sal_jz_beam, L.sal_Bx,
+                        L.sal_By,       L.sal_Sy,       L.sal_Sx,      L.sal_Sy_back, L.sal_Sx_back};
     out.clear();
     for (int k = 0; k < L.ncomp; ++k) {
         bool s = false;

@@ -48,6 +48,11 @@ void layout_init(hpq_ctx* c) {
         if (c->p.deposit_rho) L.rho = n++;
         L.prev_jx_beam = n++; L.prev_jy_beam = n++;
         if (ions) L.ions_rhomjz = n++;
+        if (c->p.salame) {
+            L.sal_Ez_target = n++; L.sal_Ez_no_salame = n++; L.sal_Ez = n++; L.sal_jx = n++; L.sal_jy = n++;
+            L.sal_jz_beam = n++; L.sal_Bx = n++; L.sal_By = n++; L.sal_Sy = n++; L.sal_Sx = n++;
+            L.sal_Sy_back = n++; L.sal_Sx_back = n++;
+        }
     } else {
         L.next_jx = n++; L.next_jy = n++;
         L.ExmBy = n++; L.EypBx = n++; L.Ez = n++; L.Bx = n++; L.By = n++; L.Bz = n++; L.Psi = n++;
@@ -229,10 +234,15 @@ public:
         fields_.AddRhoIons();
         fields_.SolvePoissonPsiExmByEypBxEzBz();
         if (P.explicit_solver) {
-            if (P.do_beam_jx_jy) DepositCurrentSlice(c_, b_, islice - 1, s_, L_.next_jx_beam, L_.next_jy_beam, -1);
+            if (P.do_beam_jx_jy) {
+                // SALAME beams deposit no jx, jy while SALAME runs (MultiBeam.cpp:50-56)
+                const int32_t comps[3] = {L_.next_jx_beam, L_.next_jy_beam, -1};
+                deposit_beam_select(c_, b_, islice - 1, s_, comps, c_->salame.mask, 0);
+            }
             InitializeSxSyWithBeam();
             explicit_deposition(c_, p_, s_);
             ExplicitMGSolveBxBy();
+            if (c_->salame.mask && salame_count(c_, b_, islice) > 0) SalameModule(islice);
         } else {
             PredictorCorrectorLoopToSolveBxBy(islice);
         }
@@ -247,16 +257,105 @@ public:
     // Hipace::InitializeSxSyWithBeam, :745-790
     void InitializeSxSyWithBeam() { sxsy_from_beam(c_, s_); }
 
-    // Hipace::ExplicitMGSolveBxBy, :794-933 (hpmg branch, level 0)
-    void ExplicitMGSolveBxBy() {
+    // Hipace::ExplicitMGSolveBxBy, :794-933 (hpmg branch, level 0); chi always from This,
+    // Bx/By and Sy/Sx from This or from the SALAME planes
+    void ExplicitMGSolveBxBy(bool salame_slice = false) {
         hpmg::MultiGrid mg(c_);
         const long ps = c_->g.plane;
+        const int bx = salame_slice ? L_.sal_Bx : L_.Bx, sy = salame_slice ? L_.sal_Sy : L_.Sy;
         int iters = 0;
-        const int st = mg.solve1(s_ + long(L_.Bx) * ps, s_ + long(L_.Sy) * ps, s_ + long(L_.chi) * ps, ps,
+        const int st = mg.solve1(s_ + long(bx) * ps, s_ + long(sy) * ps, s_ + long(L_.chi) * ps, ps,
                                  c_->p.mg_tol_rel, c_->p.mg_tol_abs, c_->p.mg_maxiter, &iters);
         c_->stats[1] += iters;
         if (st == HPQ_E_MG_DIVERGED) throw HpqError(st, "hpmg failing so lets stop here");
         if (st == HPQ_E_MG_NOT_CONVERGED) throw HpqError(st, "hpmg failed");
+    }
+
+    // Fields::SolvePoissonEz, src/fields/Fields.cpp:960-1006, on the SALAME planes
+    void SolvePoissonEzSalame() {
+        salame_rhs_ez(c_, s_);
+        double* lhs[3] = {s_ + long(L_.sal_Ez) * c_->g.plane, nullptr, nullptr};
+        poisson_solve_staged(c_, 1, lhs);
+    }
+
+    // SalameModule, src/salame/Salame.cpp:14-184 (level 0, no grid current)
+    void SalameModule(int islice) {
+        hpq_ctx* c = c_;
+        const Layout& L = L_;
+        const Geom& g = c->g;
+        const hpq_salame& S = c->salame;
+        auto fill0 = [&](std::initializer_list<int> comps) {
+            std::vector<int> v(comps);
+            plane_fill(c, s_, v.data(), int(v.size()), 0.0);
+        };
+        auto copy = [&](std::initializer_list<int> dst, std::initializer_list<int> src) {
+            std::vector<int> d(dst), s(src);
+            plane_copy(c, s_, d.data(), s.data(), int(d.size()));
+        };
+        // GetPosOffset(2, geom, domain), src/fields/Fields.H:72-77
+        const double zoff = 0.5 * (g.zlo + g.zhi - g.dz * double(0 + g.nz - 1));
+        // the Ez field from before SALAME started on this run of slices (:21-29)
+        if (islice + 1 != c->sal_last_slice) {
+            copy({L.sal_Ez_target}, {L.Ez});
+            c->sal_overloaded = false;
+            c->sal_zeta_initial = islice * g.dz + zoff;
+        }
+        c->sal_last_slice = islice;
+        // plasma-only Sx, Sy, kept for step 4 (:31-38)
+        fill0({L.Sy, L.Sx});
+        explicit_deposition(c, p_, s_);
+        copy({L.sal_Sy_back, L.sal_Sx_back}, {L.Sy, L.Sx});
+        double W = 1.0;
+        for (int iter = 0; iter < S.n_iter; ++iter) {
+            // STEP 1: Ez of the next slice with the present SALAME weights (:44-75)
+            advance_plasma(c, p_, s_, true);
+            copy({L.sal_jx, L.sal_jy}, {L.next_jx_beam, L.next_jy_beam});
+            DepositCurrent(c, p_, s_, L.sal_jx, L.sal_jy, -1, -1, -1, -1);
+            fill0({L.sal_Ez, L.sal_jz_beam, L.sal_Sy, L.sal_Sx, L.sal_Bx, L.sal_By});
+            SolvePoissonEzSalame();
+            copy({L.sal_Ez_no_salame}, {L.sal_Ez});
+            // STEP 2: Ez of the SALAME beam alone (:77-120)
+            const int32_t cz[3] = {-1, -1, L.sal_jz_beam};
+            deposit_beam_select(c, b_, islice, s_, cz, S.mask, 1);
+            salame_sxsy(c, s_);
+            ExplicitMGSolveBxBy(true);
+            fill0({L.sal_Ez, L.sal_jx, L.sal_jy});
+            if (S.do_advance) {
+                salame_only_advance(c, p_, s_);
+                DepositCurrent(c, p_, s_, L.sal_jx, L.sal_jy, -1, -1, -1, -1);
+            } else {
+                salame_jxjy_from_b(c, s_);
+            }
+            SolvePoissonEzSalame();
+            // STEP 3: the weight factor (:122-159, SalameGetW :332-407)
+            fill0({L.sal_jz_beam});
+            deposit_beam_select(c, b_, islice, s_, cz, S.mask, 1);
+            double sum[4];
+            salame_sums(c, s_, sum);
+            double sum_Ez_target = sum[0], sum_Ez_no_salame = sum[1], sum_Ez_only_salame = sum[2];
+            const double sum_jz = sum[3];
+            sum_Ez_target /= sum_jz;
+            sum_Ez_no_salame /= sum_jz;
+            sum_Ez_only_salame /= sum_jz;
+            const double zeta = (islice - 1) * g.dz + zoff;
+            if (S.target) sum_Ez_target = S.target(zeta, c->sal_zeta_initial, sum_Ez_target, S.user);
+            W = (sum_Ez_target - sum_Ez_no_salame) / sum_Ez_only_salame + 1.0;
+            if (W < 0 || c->sal_overloaded) {
+                W = 0;
+                iter = S.n_iter - 1;
+                c->sal_overloaded = true;
+            }
+            if (!c->sal_overloaded && iter >= 1 && std::abs(W - 1.) < S.relative_tolerance) iter = S.n_iter - 1;
+            salame_multiply(c, b_, islice, W);
+            // STEP 4: Bx, By with the new weights (:161-177)
+            fill0({L.jz_beam, L.Sy, L.Sx});
+            DepositCurrentSlice(c, b_, islice, s_, -1, -1, L.jz_beam);
+            InitializeSxSyWithBeam();
+            const int d[2] = {L.Sy, L.Sx}, sb[2] = {L.sal_Sy_back, L.sal_Sx_back};
+            plane_add(c, s_, d, sb, 2);
+            ExplicitMGSolveBxBy();
+        }
+        if (S.weight_factor) S.weight_factor[islice] = W;
     }
 
     // Hipace::PredictorCorrectorLoopToSolveBxBy, :936-1031
@@ -454,6 +553,7 @@ void adjoint_check(const hpq_ctx* c) {
     if (P.nx % 2 == 0 || P.ny % 2 == 0)
         throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: odd nx, ny (node-centred multigrid, a symmetric operator)");
     if (P.do_beam_jx_jy) throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: do_beam_jx_jy must be 0");
+    if (c->salame.mask) throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: SALAME must be off");
 }
 
 // the forward pass with checkpoints every k slices
@@ -577,6 +677,8 @@ int hpq_create(const hpq_params* p, hpq_ctx** out) {
             throw HpqError(HPQ_E_UNSUPPORTED, "depos_derivative_type must be 0..2");
         if (p->depos_order == 0 && p->derivative_type == 0)
             throw HpqError(HPQ_E_INVALID, "Analytic derivative with depos_order=0 would vanish");
+        if (p->salame && !p->explicit_solver)
+            throw HpqError(HPQ_E_UNSUPPORTED, "Cannot use SALAME algorithm with predictor-corrector solver");
         c->p = *p;
         Geom& g = c->g;
         g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.ng = 2;
@@ -626,6 +728,7 @@ void hpq_destroy(hpq_ctx* c) {
     adjoint_free(c);
     beam_slip_free(c);
     insitu_free(c);
+    salame_free(c);
     delete c;
 }
 
@@ -644,7 +747,9 @@ int hpq_layout(const hpq_ctx* c, int32_t* out) {
                                  L.Ez, L.Bx, L.By, L.Bz, L.Psi, L.jx_beam, L.jy_beam, L.jz_beam, L.jx, L.jy,
                                  L.rhomjz, L.rho, L.prev_jx_beam, L.prev_jy_beam, L.ions_rhomjz, L.next_jx,
                                  L.next_jy, L.jz, L.prev_Bx, L.prev_By, L.prev_jx, L.prev_jy, L.pciter_Bx,
-                                 L.pciter_By};
+                                 L.pciter_By, L.sal_Ez_target, L.sal_Ez_no_salame, L.sal_Ez, L.sal_jx,
+                                 L.sal_jy, L.sal_jz_beam, L.sal_Bx, L.sal_By, L.sal_Sy, L.sal_Sx,
+                                 L.sal_Sy_back, L.sal_Sx_back};
     for (int k = 0; k < HPQ_LAYOUT_N; ++k) out[k] = v[k];
     return HPQ_OK;
 }
@@ -961,6 +1066,24 @@ int hpq_set_insitu(hpq_ctx* c, const hpq_insitu* d) {
             throw HpqError(HPQ_E_INVALID, "insitu: radius must be >= 0");
         if (d->field_rdata || d->plasma_rdata || d->beam_rdata) insitu_reserve(c);
         c->insitu = *d;
+    });
+}
+
+int hpq_set_salame(hpq_ctx* c, const hpq_salame* s) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        c->sal_last_slice = -1;
+        c->sal_overloaded = false;
+        if (!s) {
+            c->salame = hpq_salame{};
+            return;
+        }
+        if (!c->p.salame || !c->p.explicit_solver)
+            throw HpqError(HPQ_E_UNSUPPORTED, "SALAME needs hpq_params.salame = 1 and the explicit solver");
+        if (!s->mask || !s->w || !s->valid) throw HpqError(HPQ_E_INVALID, "SALAME: mask, w and valid are required");
+        if (s->n_iter < 0) throw HpqError(HPQ_E_INVALID, "SALAME: n_iter must be >= 0");
+        salame_alloc(c);
+        c->salame = *s;
     });
 }
 

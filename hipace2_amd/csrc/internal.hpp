@@ -52,6 +52,9 @@ struct Layout {
     int next_jx = -1, next_jy = -1, jz = -1;
     int prev_Bx = -1, prev_By = -1, prev_jx = -1, prev_jy = -1;
     int pciter_Bx = -1, pciter_By = -1, pcprev_Bx = -1, pcprev_By = -1;
+    // WhichSlice::Salame (src/fields/Fields.cpp:111-115)
+    int sal_Ez_target = -1, sal_Ez_no_salame = -1, sal_Ez = -1, sal_jx = -1, sal_jy = -1, sal_jz_beam = -1,
+        sal_Bx = -1, sal_By = -1, sal_Sy = -1, sal_Sx = -1, sal_Sy_back = -1, sal_Sx_back = -1;
 };
 
 // Batched 1-D DST-I (RODFT00) along contiguous rows of length n.
@@ -214,6 +217,12 @@ struct hpq_ctx {
     int* diag_comp_of = nullptr;
     bool diag_pending = false;
     hpq_insitu insitu{};             // caller-owned in-situ diagnostic arrays (all NULL: off)
+    hpq_salame salame{};             // SALAME settings (mask NULL: off)
+    int sal_last_slice = -1;         // Hipace::m_salame_last_slice
+    bool sal_overloaded = false;     // Hipace::m_salame_overloaded
+    double sal_zeta_initial = 0.0;   // Hipace::m_salame_zeta_initial
+    double* sal_dev = nullptr;       // device: 4 sums of SalameGetW, particle count
+    double* sal_host = nullptr;      // pinned copy
     double* insitu_partial = nullptr;  // per-workgroup in-situ sums of one slice
     long insitu_cap = 0;               // workgroups insitu_partial holds
     std::vector<hpq::Capture> captures;
@@ -261,6 +270,17 @@ void insitu_finalize(hpq_ctx* c, int nblk, int nreal, bool count, double scale, 
 void insitu_plasma(hpq_ctx* c, const hpq_plasma* p, int islice);
 void insitu_beam(hpq_ctx* c, const hpq_beam* b, int islice);
 void insitu_free(hpq_ctx* c);
+void deposit_beam_select(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const int32_t* comps,
+                         const int32_t* select, int select_value);
+void salame_alloc(hpq_ctx* c);
+void salame_free(hpq_ctx* c);
+long salame_count(hpq_ctx* c, const hpq_beam* b, int slice);
+void salame_sxsy(hpq_ctx* c, double* store);
+void salame_rhs_ez(hpq_ctx* c, const double* store);
+void salame_only_advance(hpq_ctx* c, const hpq_plasma* p, double* store);
+void salame_jxjy_from_b(hpq_ctx* c, double* store);
+void salame_sums(hpq_ctx* c, const double* store, double out[4]);
+void salame_multiply(hpq_ctx* c, const hpq_beam* b, int slice, double W);
 double rel_b_error(hpq_ctx* c, const double* store, int bx, int by, int bxi, int byi);
 void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p);
 int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof);

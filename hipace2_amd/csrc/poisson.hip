@@ -390,6 +390,33 @@ void build_rhs_psi_ez_bz(hpq_ctx* c, const double* store) {
     HPQ_HIP(hipGetLastError());
 }
 
+// Right-hand side of Fields::SolvePoissonEz (src/fields/Fields.cpp:988-993) into the staging of
+// field 0: (d_x jx + d_y jy)/(ep0 c), the Ez term of k_rhs_psi_ez_bz on other planes
+template <class T>
+__global__ void k_rhs_ez(Geom g, const double* __restrict__ jx, const double* __restrict__ jy, T* __restrict__ st,
+                         long ld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= g.nx) return;
+    const double ep0 = 1.0, cl = 1.0;
+    const double dxih = 0.5 * g.dxi, dyih = 0.5 * g.dyi;
+    const double fe = 1.0 / (ep0 * cl);
+    st[long(j) * ld + i] = T(fe * ((jx[g.idx(i + 1, j)] - jx[g.idx(i - 1, j)]) * dxih) +
+                             fe * ((jy[g.idx(i, j + 1)] - jy[g.idx(i, j - 1)]) * dyih));
+}
+
+// SolvePoissonEz on the SALAME planes: RHS from their jx, jy
+void salame_rhs_ez(hpq_ctx* c, const double* store) {
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    with_staging(c, [&](auto* st, long ld, long) {
+        using T = std::remove_pointer_t<decltype(st)>;
+        hipLaunchKernelGGL(k_rhs_ez<T>, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g,
+                           store + L.sal_jx * g.plane, store + L.sal_jy * g.plane, st, ld);
+    });
+    HPQ_HIP(hipGetLastError());
+}
+
 // Right-hand sides of Fields::SolvePoissonBxBy (src/fields/Fields.cpp:1044-1068).
 template <class T>
 __global__ void k_rhs_bxby(Geom g, const double* __restrict__ jz, const double* __restrict__ pjx,

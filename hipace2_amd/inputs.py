@@ -80,6 +80,13 @@ class SliceConfig:
     dt: float = 0.0
     n_subcycles: int = 10
     do_z_push: bool = True
+    # SALAME (src/Hipace.cpp:137-143): a beam has do_salame; hipace.salame_n_iter,
+    # salame_do_advance, salame_relative_tolerance, salame_Ez_target(zeta,zeta_initial,Ez_initial)
+    salame: bool = False
+    salame_n_iter: int = 5
+    salame_do_advance: bool = True
+    salame_relative_tolerance: float = 1e-4
+    salame_target: str = "Ez_initial"
 
     @property
     def dx(self) -> float:
@@ -125,6 +132,7 @@ class BeamSpec:
     # beams.external_E/B(x,y,z,t) as (6, 4) affine coefficients (Ex, Ey, Ez, Bx, By, Bz) x
     # (1, x, y, z), or None
     external: Optional[np.ndarray] = None
+    do_salame: bool = False
 
 
 @dataclasses.dataclass
@@ -142,6 +150,7 @@ class BeamArrays:
     charge: float = -1.0
     mass: float = 1.0
     external: Optional[np.ndarray] = None   # (6, 4) affine external fields, or None
+    salame: Optional[np.ndarray] = None     # int32 per particle: 1 for a SALAME beam's particles
 
     @property
     def n(self) -> int:
@@ -304,6 +313,8 @@ def make_beam(cfg: SliceConfig, spec: Optional[BeamSpec]) -> BeamArrays:
     else:
         raise ValueError(f"unsupported beam injection_type {spec.injection_type!r}")
     b.external = None if spec.external is None else np.array(spec.external, dtype=np.float64)
+    if spec.do_salame:
+        b.salame = np.ones(b.n, np.int32)
     return b
 
 
@@ -319,6 +330,7 @@ def merge_beams(cfg: SliceConfig, beams) -> BeamArrays:
         raise ValueError("merged beams must share their external fields")
     fields = ("x", "y", "z", "ux", "uy", "uz", "w", "valid")
     out = {f: [] for f in fields}
+    sal = []
     off = np.zeros(cfg.nz + 1, np.int64)
     for s in range(cfg.nz):
         cnt = 0
@@ -326,9 +338,13 @@ def merge_beams(cfg: SliceConfig, beams) -> BeamArrays:
             a, e = int(b.off[s]), int(b.off[s + 1])
             for f in fields:
                 out[f].append(getattr(b, f)[a:e])
+            sal.append(b.salame[a:e] if b.salame is not None else np.zeros(e - a, np.int32))
             cnt += e - a
         off[s + 1] = off[s] + cnt
-    return BeamArrays(*(np.concatenate(out[f]) for f in fields), off, beams[0].charge, beams[0].mass, ext[0])
+    merged = BeamArrays(*(np.concatenate(out[f]) for f in fields), off, beams[0].charge, beams[0].mass, ext[0])
+    if any(b.salame is not None for b in beams):
+        merged.salame = np.concatenate(sal).astype(np.int32)
+    return merged
 
 
 # --------------------------------------------------------------------------------------
@@ -419,6 +435,42 @@ def external_fields(e_exprs, b_exprs) -> Optional[np.ndarray]:
     return out
 
 
+_FUNCS = {"sqrt": math.sqrt, "exp": math.exp, "log": math.log, "sin": math.sin, "cos": math.cos,
+          "tan": math.tan, "abs": abs, "min": min, "max": max, "pow": pow}
+
+
+def parse_function(expr: str, names):
+    """A function of `names` from an expression in the reference's parser syntax (numbers,
+    the names, + - * / and ^, parentheses and a few elementary functions), evaluated by
+    walking the syntax tree; nothing else is accepted."""
+    tree = ast.parse(expr.strip().strip('"').replace("^", "**"), mode="eval")
+    ops = {ast.Add: lambda a, b: a + b, ast.Sub: lambda a, b: a - b, ast.Mult: lambda a, b: a * b,
+           ast.Div: lambda a, b: a / b, ast.Pow: lambda a, b: a ** b}
+
+    def ev(node, env):
+        if isinstance(node, ast.Expression):
+            return ev(node.body, env)
+        if isinstance(node, ast.Constant) and isinstance(node.value, (int, float)):
+            return float(node.value)
+        if isinstance(node, ast.Name) and node.id in env:
+            return env[node.id]
+        if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+            v = ev(node.operand, env)
+            return -v if isinstance(node.op, ast.USub) else v
+        if isinstance(node, ast.BinOp) and type(node.op) in ops:
+            return ops[type(node.op)](ev(node.left, env), ev(node.right, env))
+        if isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id in _FUNCS \
+                and not node.keywords:
+            return float(_FUNCS[node.func.id](*[ev(a, env) for a in node.args]))
+        raise ValueError(f"unsupported term in {expr!r}")
+
+    # every name must be one of the arguments: check once on the tree
+    for node in ast.walk(tree):
+        if isinstance(node, ast.Name) and node.id not in names and node.id not in _FUNCS:
+            raise ValueError(f"unknown name {node.id!r} in {expr!r}")
+    return lambda *args: ev(tree, dict(zip(names, (float(a) for a in args))))
+
+
 def parse_deck(text: str, overrides: Optional[Dict[str, str]] = None):
     """Parse a reference input deck into (SliceConfig, [BeamSpec], max_step)."""
     kv = parse_kv(text)
@@ -487,11 +539,17 @@ def parse_deck(text: str, overrides: Optional[Dict[str, str]] = None):
         if g("element", "electron") != "electron":
             raise ValueError("electron beams only on this path")
         cfg.n_subcycles = int(_num(g("n_subcycles", "10")))
+        spec.do_salame = _num(g("do_salame", "0")) != 0
         cfg.do_z_push = _num(g("do_z_push", "1")) != 0
         ext_e, ext_b = g("external_E(x,y,z,t)"), g("external_B(x,y,z,t)")
         spec.external = external_fields(None if ext_e is None else _tokens(ext_e),
                                         None if ext_b is None else _tokens(ext_b))
         beams.append(spec)
     cfg.dt = _num(kv.get("hipace.dt", "0"))
+    cfg.salame = any(b.do_salame for b in beams)
+    cfg.salame_n_iter = int(_num(kv.get("hipace.salame_n_iter", "5")))
+    cfg.salame_do_advance = _num(kv.get("hipace.salame_do_advance", "1")) != 0
+    cfg.salame_relative_tolerance = _num(kv.get("hipace.salame_relative_tolerance", "1e-4"))
+    cfg.salame_target = kv.get("hipace.salame_Ez_target(zeta,zeta_initial,Ez_initial)", "Ez_initial").strip('"')
     max_step = int(_num(kv.get("max_step", "0")))
     return cfg, beams, max_step

@@ -39,6 +39,7 @@ def make_params(cfg: SliceConfig) -> _abi.Params:
     p.bin_period = cfg.bin_period
     p.precision = _abi.PRECISION[cfg.precision]
     p.poisson_solver = _abi.POISSON_SOLVER[cfg.poisson_solver]
+    p.salame = int(cfg.salame)
     return p
 
 
@@ -65,7 +66,7 @@ class Simulation:
         self.ctx = h
         _abi.check(self.ctx, rc)
         self.set_stream(torch.cuda.current_stream(self.device))
-        arr = (C.c_int32 * 32)()
+        arr = (C.c_int32 * len(_abi.LAYOUT_NAMES))()
         self.L.hpq_layout(self.ctx, arr)
         self.layout = dict(zip(_abi.LAYOUT_NAMES, list(arr)))
         self.ncomp = self.layout["ncomp"]
@@ -101,6 +102,8 @@ class Simulation:
         n = b.n
         self.beam = {k: dev(getattr(b, k) if n else np.zeros(1)) for k in ("x", "y", "z", "ux", "uy", "uz", "w")}
         self.beam["valid"] = dev(b.valid if n else np.zeros(1, np.int32), torch.int32)
+        if n and b.salame is not None:
+            self.beam["salame"] = dev(b.salame, torch.int32)
         self._boff = (C.c_int64 * (self.cfg.nz + 1))(*[int(v) for v in b.off])
         self._bdesc = _abi.Beam(*[self.beam[k].data_ptr() for k in ("x", "y", "z", "ux", "uy", "uz", "w", "valid")],
                                 C.cast(self._boff, C.POINTER(C.c_int64)), n, b.charge, b.mass)
@@ -147,11 +150,49 @@ class Simulation:
 
     def run_steps(self, nsteps: int, dt: float, n_subcycles: int = 10, do_z_push: bool = True):
         """Several time steps: beam re-sorted by slice between steps, plasma re-injected each
-        step (src/Hipace.cpp:426-480)."""
+        step (src/Hipace.cpp:426-480); with a SALAME beam, SALAME runs in the first step."""
         for step in range(nsteps):
             if step > 0:
                 self.sort_beam()
+            if self.cfg.salame and step == 0:
+                self.enable_salame()
             self.run_step_with_beam(dt, n_subcycles, do_z_push)
+            if self.cfg.salame and step == 0:
+                self.disable_salame()
+
+    # ---- SALAME -------------------------------------------------------------------------
+    def enable_salame(self, n_iter: Optional[int] = None, do_advance: Optional[bool] = None,
+                      relative_tolerance: Optional[float] = None, target=None):
+        """Run SALAME on the following slices (hpq_set_salame) with the deck's settings unless
+        given: the weights of the SALAME beam's particles (<beam>.do_salame) are rescaled on the
+        device, slice by slice, to flatten Ez. `target(zeta, zeta_initial, Ez_initial)` replaces
+        hipace.salame_Ez_target; the default keeps Ez_initial. The last weight factor of every
+        slice is in self.salame_weight_factor afterwards."""
+        from .inputs import parse_function
+        cfg = self.cfg
+        if "salame" not in self.beam:
+            raise ValueError("SALAME needs beam particles of a beam with do_salame")
+        if target is None and cfg.salame_target.replace(" ", "") != "Ez_initial":
+            target = parse_function(cfg.salame_target, ["zeta", "zeta_initial", "Ez_initial"])
+        d = _abi.Salame()
+        d.mask = self.beam["salame"].data_ptr()
+        d.w = self.beam["w"].data_ptr()
+        d.valid = self.beam["valid"].data_ptr()
+        d.n_iter = cfg.salame_n_iter if n_iter is None else int(n_iter)
+        d.do_advance = int(cfg.salame_do_advance if do_advance is None else do_advance)
+        d.relative_tolerance = cfg.salame_relative_tolerance if relative_tolerance is None else relative_tolerance
+        self._salame_cb = None
+        if target is not None:
+            self._salame_cb = _abi.SALAME_TARGET(lambda z, z0, e0, _u: float(target(z, z0, e0)))
+            d.target = self._salame_cb
+        self.salame_weight_factor = np.zeros(cfg.nz)
+        d.weight_factor = self.salame_weight_factor.ctypes.data_as(C.POINTER(C.c_double))
+        _abi.check(self.ctx, self.L.hpq_set_salame(self.ctx, C.byref(d)))
+        self._salame_desc = d
+
+    def disable_salame(self):
+        _abi.check(self.ctx, self.L.hpq_set_salame(self.ctx, None))
+        self._salame_desc = None
 
     def init_plasma(self):
         if self.np:

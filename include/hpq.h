@@ -74,6 +74,8 @@ typedef struct hpq_params {
                                             the order of floating-point sums. */
     int precision;                       /* HPQ_PREC_FP64 or HPQ_PREC_MIXED */
     int poisson_solver;                  /* HPQ_POISSON_FFT or HPQ_POISSON_BF16X3 */
+    int salame;                          /* a beam has <beam>.do_salame: allocate the SALAME
+                                            planes (explicit solver only) */
 } hpq_params;
 
 /* Plasma species, SoA device arrays in PlasmaIdx order
@@ -126,8 +128,8 @@ typedef struct hpq_plasma_profile {
     double u_mean[3];
 } hpq_plasma_profile;
 
-/* Names of the hpq_layout() entries, in order. */
-#define HPQ_LAYOUT_N 32
+/* Number of hpq_layout() entries. */
+#define HPQ_LAYOUT_N 44
 
 /* ---- context ---------------------------------------------------------------- */
 int hpq_create(const hpq_params *p, hpq_ctx **out);
@@ -136,7 +138,9 @@ const char *hpq_last_error(const hpq_ctx *ctx);
 int hpq_set_stream(hpq_ctx *ctx, void *hip_stream);
 /* out[HPQ_LAYOUT_N]: ncomp, next_jx_beam, next_jy_beam, chi, Sy, Sx, ExmBy, EypBx, Ez, Bx,
  * By, Bz, Psi, jx_beam, jy_beam, jz_beam, jx, jy, rhomjz, rho, prev_jx_beam, prev_jy_beam,
- * ions_rhomjz, next_jx, next_jy, jz, prev_Bx, prev_By, prev_jx, prev_jy, pciter_Bx, pciter_By
+ * ions_rhomjz, next_jx, next_jy, jz, prev_Bx, prev_By, prev_jx, prev_jy, pciter_Bx, pciter_By,
+ * then the SALAME planes (WhichSlice::Salame, src/fields/Fields.cpp:111-115): Ez_target,
+ * Ez_no_salame, Ez, jx, jy, jz_beam, Bx, By, Sy, Sx, Sy_back, Sx_back
  * (-1: not allocated). Replaces the Comps map, src/fields/Fields.H:27-56. */
 int hpq_layout(const hpq_ctx *ctx, int32_t *out);
 int64_t hpq_store_doubles(const hpq_ctx *ctx);
@@ -324,6 +328,32 @@ typedef struct hpq_insitu {
 } hpq_insitu;
 /* NULL turns every group off. */
 int hpq_set_insitu(hpq_ctx *ctx, const hpq_insitu *d);
+
+/* SALAME, the reference's beam-loading algorithm (SalameModule, src/salame/Salame.cpp:14-184,
+ * called from src/Hipace.cpp:673-678): on every slice that holds particles of a SALAME beam,
+ * after the Bx/By solve, rescale those particles' weights by W so that the Ez field seen by
+ * the beam on the next slice matches the target, iterating n_iter times (or until
+ * |W - 1| < relative_tolerance); SALAME beams deposit no jx/jy meanwhile
+ * (MultiBeam::DepositCurrentSlice, src/particles/beam/MultiBeam.cpp:41-58). The reference runs it
+ * in the first time step only: set it before that step and clear it (NULL) after.
+ * mask: device int32 per beam particle, 1 for particles of a SALAME beam (<beam>.do_salame).
+ * w, valid: the beam's weight and valid arrays (device), updated in place (W = 0 invalidates).
+ * target: hipace.salame_Ez_target(zeta, zeta_initial, Ez_initial); NULL is the default
+ * "Ez_initial". weight_factor (host, nz doubles, may be NULL) receives the last W of every
+ * slice SALAME ran on. Needs hpq_params.salame = 1 and the explicit solver. */
+typedef double (*hpq_salame_target_fn)(double zeta, double zeta_initial, double ez_initial, void *user);
+typedef struct hpq_salame {
+    const int32_t *mask;
+    double *w;
+    int32_t *valid;
+    int n_iter;                   /* hipace.salame_n_iter (5) */
+    int do_advance;               /* hipace.salame_do_advance (1) */
+    double relative_tolerance;    /* hipace.salame_relative_tolerance (1e-4) */
+    hpq_salame_target_fn target;
+    void *user;
+    double *weight_factor;
+} hpq_salame;
+int hpq_set_salame(hpq_ctx *ctx, const hpq_salame *s);
 
 /* Kernel timing (the reference's HIPACE_PROFILE regions, src/utils/HipaceProfilerWrapper.H):
  * when enabled for a kernel family, every launch of it is bracketed by HIP events on the

@@ -7,7 +7,7 @@ namespace orc {
 
 // Component layout, restating Fields::AllocData src/fields/Fields.cpp:70-165 for the
 // configurations this path covers (no laser, no SALAME, no MLMG, no individual rho).
-Layout make_layout(bool explicit_solver, bool deposit_rho, bool neutralize) {
+Layout make_layout(bool explicit_solver, bool deposit_rho, bool neutralize, bool salame) {
     Layout L;
     int n = 0;
     if (explicit_solver) {
@@ -18,6 +18,11 @@ Layout make_layout(bool explicit_solver, bool deposit_rho, bool neutralize) {
         if (deposit_rho) L.rho = n++;
         L.prev_jx_beam = n++; L.prev_jy_beam = n++;
         if (neutralize) L.ions_rhomjz = n++;
+        if (salame) {
+            L.sal_Ez_target = n++; L.sal_Ez_no_salame = n++; L.sal_Ez = n++; L.sal_jx = n++; L.sal_jy = n++;
+            L.sal_jz_beam = n++; L.sal_Bx = n++; L.sal_By = n++; L.sal_Sy = n++; L.sal_Sx = n++;
+            L.sal_Sy_back = n++; L.sal_Sx_back = n++;
+        }
     } else {
         L.next_jx = n++; L.next_jy = n++;
         L.ExmBy = n++; L.EypBx = n++; L.Ez = n++; L.Bx = n++; L.By = n++; L.Bz = n++; L.Psi = n++;
@@ -169,6 +174,24 @@ void solve_poisson_psi_ez_bz(const Ctx& c, double* F) {
             exmby[g.idx(i, j)] = -(psi[g.idx(i + 1, j)] - psi[g.idx(i - 1, j)]) * dxih;
             eypbx[g.idx(i, j)] = -(psi[g.idx(i, j + 1)] - psi[g.idx(i, j - 1)]) * dyih;
         }
+}
+
+// Fields::SolvePoissonEz, src/fields/Fields.cpp:960-1006 (level 0, Dirichlet): the Ez part
+// of solve_poisson_psi_ez_bz on other components
+void solve_poisson_ez(const Ctx& c, double* F, int cjx, int cjy, int cez) {
+    const Geom& g = c.g;
+    const double ep0 = 1.0, cl = 1.0;
+    const int nx = g.nx, ny = g.ny;
+    std::vector<double> st(size_t(nx) * ny);
+    const double* jx = F + cjx * g.plane;
+    const double* jy = F + cjy * g.plane;
+    const double dxih = 0.5 * g.dxi, dyih = 0.5 * g.dyi;
+    const double fe = 1.0 / (ep0 * cl);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            st[long(j) * nx + i] = fe * ((jx[g.idx(i + 1, j)] - jx[g.idx(i - 1, j)]) * dxih) +
+                                   fe * ((jy[g.idx(i, j + 1)] - jy[g.idx(i, j - 1)]) * dyih);
+    poisson_dirichlet(g, st.data(), F + cez * g.plane);
 }
 
 // Fields::SolvePoissonBxBy, src/fields/Fields.cpp:1009-1078 (predictor-corrector path).

@@ -56,8 +56,11 @@ struct Layout {
     int next_jx = -1, next_jy = -1, jz = -1;
     int prev_Bx = -1, prev_By = -1, prev_jx = -1, prev_jy = -1;
     int pciter_Bx = -1, pciter_By = -1, pcprev_Bx = -1, pcprev_By = -1;
+    // WhichSlice::Salame (src/fields/Fields.cpp:111-115)
+    int sal_Ez_target = -1, sal_Ez_no_salame = -1, sal_Ez = -1, sal_jx = -1, sal_jy = -1, sal_jz_beam = -1,
+        sal_Bx = -1, sal_By = -1, sal_Sy = -1, sal_Sx = -1, sal_Sy_back = -1, sal_Sx_back = -1;
 };
-Layout make_layout(bool explicit_solver, bool deposit_rho, bool neutralize);
+Layout make_layout(bool explicit_solver, bool deposit_rho, bool neutralize, bool salame = false);
 
 // Plasma species SoA (src/particles/plasma/PlasmaParticleContainer.H:21-46).
 struct Plasma {
@@ -81,6 +84,7 @@ struct Beam {
     std::vector<int64_t> off;   // nz+1
     double charge = -1.0, mass = 1.0;
     std::vector<double> ext;    // 24 affine external-field coefficients, or empty
+    std::vector<int32_t> salame;  // 1 for particles of a SALAME beam, or empty
 };
 
 enum ParticleBC { BC_REFLECTING = 0, BC_PERIODIC = 1, BC_ABSORBING = 2 };
@@ -118,7 +122,13 @@ struct Ctx {
 void deposit_plasma(const Ctx& c, Plasma& p, double* F, int cjx, int cjy, int cjz, int crho,
                     int cchi, int crhomjz, bool ions, long* n_qsa);
 void explicit_deposit(const Ctx& c, const Plasma& p, double* F);
-void deposit_beam(const Ctx& c, const Beam& b, int slice, double* F, int cjx, int cjy, int cjz);
+// select (may be null): only particles whose entry equals select_value deposit
+void deposit_beam(const Ctx& c, const Beam& b, int slice, double* F, int cjx, int cjy, int cjz,
+                  const int32_t* select = nullptr, int select_value = 0);
+// SalameOnlyAdvancePlasma, src/salame/Salame.cpp:258-330
+void salame_only_advance(const Ctx& c, Plasma& p, const double* F);
+// Fields::SolvePoissonEz, src/fields/Fields.cpp:960-1006: Ez from jx, jy (components of F)
+void solve_poisson_ez(const Ctx& c, double* F, int cjx, int cjy, int cez);
 void push_plasma(const Ctx& c, Plasma& p, const double* F, bool temp_slice, int cPsi, int cEz,
                  int cBx, int cBy, int cBz);
 // mutable beam arrays for the push (caller-owned)

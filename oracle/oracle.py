@@ -17,7 +17,10 @@ LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
 LAYOUT_NAMES = ["ncomp", "next_jx_beam", "next_jy_beam", "chi", "Sy", "Sx", "ExmBy", "EypBx", "Ez", "Bx",
                 "By", "Bz", "Psi", "jx_beam", "jy_beam", "jz_beam", "jx", "jy", "rhomjz", "rho",
                 "prev_jx_beam", "prev_jy_beam", "ions_rhomjz", "next_jx", "next_jy", "jz", "prev_Bx",
-                "prev_By", "prev_jx", "prev_jy", "pciter_Bx", "pciter_By"]
+                "prev_By", "prev_jx", "prev_jy", "pciter_Bx", "pciter_By",
+                "salame_Ez_target", "salame_Ez_no_salame", "salame_Ez", "salame_jx", "salame_jy",
+                "salame_jz_beam", "salame_Bx", "salame_By", "salame_Sy", "salame_Sx", "salame_Sy_back",
+                "salame_Sx_back"]
 
 PLASMA_REALS = ["x", "y", "w", "ux", "uy", "psi", "x_prev", "y_prev", "ux_half_step", "uy_half_step",
                 "psi_half_step"]
@@ -36,7 +39,7 @@ class OrcConfig(C.Structure):
                 ("density", C.c_double), ("ramp_a", C.c_double), ("plasma_radius", C.c_double),
                 ("u_mean", C.c_double * 3),
                 ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double),
-                ("neutralize", C.c_int), ("nthreads", C.c_int)]
+                ("neutralize", C.c_int), ("nthreads", C.c_int), ("salame", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -86,6 +89,8 @@ def lib():
                                                           i32p, C.c_long, i32p, C.c_long, C.c_int, C.c_double, C.c_int,
                                                           C.c_int, dp]
         L.orc_sim_set_beam_external.argtypes = [vp, dp]
+        L.orc_sim_set_salame.argtypes = [vp, C.POINTER(C.c_int32), C.c_int, C.c_int, C.c_double, SALAME_TARGET,
+                                         vp, dp]
         L.orc_push_beam_slice.restype = C.c_long
         L.orc_sim_run_step_with_beam.argtypes = [vp, C.c_double, C.c_int, C.c_int]
         L.orc_sim_sort_beam.argtypes = [vp]
@@ -104,6 +109,9 @@ def lib():
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+SALAME_TARGET = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p)
 
 
 def make_config(cfg, nthreads: int = 0) -> OrcConfig:
@@ -127,6 +135,7 @@ def make_config(cfg, nthreads: int = 0) -> OrcConfig:
     c.plasma_charge, c.plasma_mass, c.max_qsa = cfg.plasma_charge, cfg.plasma_mass, cfg.max_qsa
     c.neutralize = int(cfg.neutralize)
     c.nthreads = nthreads
+    c.salame = int(getattr(cfg, "salame", False))
     return c
 
 
@@ -137,7 +146,7 @@ class OracleSim:
         self.cfg = cfg
         self._c = make_config(cfg, nthreads)
         self.h = lib().orc_sim_create(C.byref(self._c))
-        arr = (C.c_int * 32)()
+        arr = (C.c_int * len(LAYOUT_NAMES))()
         lib().orc_sim_layout(self.h, arr)
         self.layout = dict(zip(LAYOUT_NAMES, list(arr)))
         self.ncomp = self.layout["ncomp"]
@@ -164,6 +173,19 @@ class OracleSim:
         ext = getattr(b, "external", None)
         self._bext = None if ext is None else np.ascontiguousarray(ext, dtype=np.float64).reshape(24)
         lib().orc_sim_set_beam_external(self.h, None if self._bext is None else _dp(self._bext))
+
+    def enable_salame(self, mask, n_iter=5, do_advance=True, relative_tolerance=1e-4, target=None):
+        """SalameModule on the following slices for the beam particles with mask 1; the last
+        weight factor of every slice goes to self.salame_weight_factor."""
+        self._sal_mask = np.ascontiguousarray(mask, dtype=np.int32)
+        self._sal_cb = SALAME_TARGET(lambda z, z0, e0, _u: float(target(z, z0, e0))) if target else SALAME_TARGET()
+        self.salame_weight_factor = np.zeros(self.cfg.nz)
+        lib().orc_sim_set_salame(self.h, self._sal_mask.ctypes.data_as(C.POINTER(C.c_int32)), int(n_iter),
+                                 int(do_advance), float(relative_tolerance), self._sal_cb, None,
+                                 _dp(self.salame_weight_factor))
+
+    def disable_salame(self):
+        lib().orc_sim_set_salame(self.h, None, 0, 0, 0.0, SALAME_TARGET(), None, None)
 
     def capture(self, name: str) -> np.ndarray:
         """Record the valid region of component `name` at the diagnostics point of every slice."""

@@ -139,7 +139,8 @@ void explicit_deposit(const Ctx& c, const Plasma& p, double* F) {
 
 // ::DepositCurrentSlice, src/particles/deposition/BeamDepositCurrent.cpp:22-196:
 // beam particles of one slice deposit jx, jy, jz (rhomjz of the beam not modelled).
-void deposit_beam(const Ctx& c, const Beam& b, int slice, double* F, int cjx, int cjy, int cjz) {
+void deposit_beam(const Ctx& c, const Beam& b, int slice, double* F, int cjx, int cjy, int cjz,
+                  const int32_t* select, int select_value) {
     const Geom& g = c.g;
     const int order = c.o.depos_order;
     const double clightinv = 1.0 / 1.0, clightsq = 1.0 / (1.0 * 1.0);
@@ -153,6 +154,7 @@ void deposit_beam(const Ctx& c, const Beam& b, int slice, double* F, int cjx, in
     if (slice < 0 || slice + 1 >= int(b.off.size())) return;
     for (int64_t ip = b.off[slice]; ip < b.off[slice + 1]; ++ip) {
         if (!b.valid[ip]) continue;
+        if (select && select[ip] != select_value) continue;
         const double ux = b.ux[ip], uy = b.uy[ip], uz = b.uz[ip];
         const double gaminv = 1.0 / std::sqrt(1.0 + ux * ux * clightsq + uy * uy * clightsq + uz * uz * clightsq);
         const double wq = q * b.w[ip] * invvol;
@@ -284,6 +286,36 @@ void push_plasma(const Ctx& c, Plasma& p, const double* F, bool temp_slice, int 
         }
         for (int s = 0; s < nsub / 2; ++s) substep();
         p.ux[ip] = ux; p.uy[ip] = uy; p.psi[ip] = psi;
+    }
+}
+
+// SalameOnlyAdvancePlasma, src/salame/Salame.cpp:258-330: ux, uy of every valid plasma particle
+// from the SALAME-only Bx, By gathered at (x_prev, y_prev) (doBxByGatherShapeN,
+// src/particles/particles_utils/FieldGather.H:183-216)
+void salame_only_advance(const Ctx& c, Plasma& p, const double* F) {
+    const Geom& g = c.g;
+    const Layout& L = c.L;
+    const int order = c.o.depos_order;
+    const double* Bx = F + L.sal_Bx * g.plane;
+    const double* By = F + L.sal_By * g.plane;
+    const double qm = p.charge / p.mass;
+    const double dz = g.dz;
+    const int lo = -g.ng, hix = g.nx - 1 + g.ng, hiy = g.ny - 1 + g.ng;
+    for (size_t ip = 0; ip < p.size(); ++ip) {
+        if (!p.valid[ip]) continue;
+        const double xmid = (p.x_prev[ip] - g.xoff) * g.dxi;
+        const double ymid = (p.y_prev[ip] - g.yoff) * g.dyi;
+        double Bxp = 0.0, Byp = 0.0;
+        for (int iy = 0; iy <= order; ++iy)
+            for (int ix = 0; ix <= order; ++ix) {
+                const SF sx = shape(order, xmid, ix), sy = shape(order, ymid, iy);
+                if (sx.cell < lo || sx.cell > hix || sy.cell < lo || sy.cell > hiy) continue;
+                const long k = g.idx(sx.cell, sy.cell);
+                Bxp += sx.s * sy.s * Bx[k];
+                Byp += sx.s * sy.s * By[k];
+            }
+        p.ux[ip] = 1.5 * dz * qm * Byp;
+        p.uy[ip] = -1.5 * dz * qm * Bxp;
     }
 }
 

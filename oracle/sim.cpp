@@ -31,6 +31,17 @@ struct Sim {
     int mg_status = 0;
     long pc_iters = 0;
     double pushes = 0;
+    // SALAME settings and state (Hipace::m_salame_*, src/Hipace.H:284-301)
+    bool salame_on = false;
+    int salame_n_iter = 5;
+    bool salame_do_advance = true;
+    double salame_tol = 1e-4;
+    double (*salame_target)(double, double, double, void*) = nullptr;
+    void* salame_user = nullptr;
+    double* salame_weight_factor = nullptr;
+    int salame_last_slice = -1;
+    bool salame_overloaded = false;
+    double salame_zeta_initial = 0.0;
 };
 
 static void set_plane(Sim& s, int comp, double v) {
@@ -163,7 +174,134 @@ static void pc_loop(Sim& s, int islice) {
     s.pc_iters += it;
 }
 
-// Hipace::SolveOneSlice, src/Hipace.cpp:557-728 (level 0, no laser/SALAME/ionisation).
+static void mg_solve_bxby(Sim& s, int bx, int sy) {
+    const Options& o = s.c.o;
+    double* F = s.F.data();
+    int it = 0;
+    const int st = mg_solve1(s.c.g, F + bx * s.c.g.plane, F + sy * s.c.g.plane, F + s.c.L.chi * s.c.g.plane,
+                             s.c.g.plane, o.mg_tol_rel, o.mg_tol_abs, o.mg_maxiter, &it);
+    s.mg_iters += it;
+    if (st) s.mg_status = st;
+}
+
+// SalameModule, src/salame/Salame.cpp:14-184 (level 0, no grid current), with
+// SalameInitializeSxSyWithBeam (:187-219), SalameGetJxJyFromBxBy (:222-256),
+// SalameGetW (:332-407) and SalameMultiplyBeamWeight (:409-437)
+static void salame_module(Sim& s, int islice) {
+    const Layout& L = s.c.L;
+    const Geom& g = s.c.g;
+    double* F = s.F.data();
+    Beam& b = s.beam;
+    const int32_t* mask = b.salame.data();
+    long nq = 0;
+    if (islice + 1 != s.salame_last_slice) {
+        copy_plane(s, L.sal_Ez_target, L.Ez);
+        s.salame_overloaded = false;
+        s.salame_zeta_initial = islice * g.dz + g.zoff;
+    }
+    s.salame_last_slice = islice;
+    set_plane(s, L.Sy, 0.); set_plane(s, L.Sx, 0.);
+    if (s.has_plasma) explicit_deposit(s.c, s.plasma, F);
+    copy_plane(s, L.sal_Sy_back, L.Sy); copy_plane(s, L.sal_Sx_back, L.Sx);
+    double W = 1.0;
+    for (int iter = 0; iter < s.salame_n_iter; ++iter) {
+        // STEP 1
+        if (s.has_plasma) push_plasma(s.c, s.plasma, F, true, L.Psi, L.Ez, L.Bx, L.By, L.Bz);
+        copy_plane(s, L.sal_jx, L.next_jx_beam); copy_plane(s, L.sal_jy, L.next_jy_beam);
+        if (s.has_plasma) deposit_plasma(s.c, s.plasma, F, L.sal_jx, L.sal_jy, -1, -1, -1, -1, false, &nq);
+        for (int comp : {L.sal_Ez, L.sal_jz_beam, L.sal_Sy, L.sal_Sx, L.sal_Bx, L.sal_By}) set_plane(s, comp, 0.);
+        solve_poisson_ez(s.c, F, L.sal_jx, L.sal_jy, L.sal_Ez);
+        copy_plane(s, L.sal_Ez_no_salame, L.sal_Ez);
+        // STEP 2
+        deposit_beam(s.c, b, islice, F, -1, -1, L.sal_jz_beam, mask, 1);
+        {
+            const double mu0 = 1.0;
+            const double dxih = 0.5 * g.dxi, dyih = 0.5 * g.dyi;
+            const double* jzb = F + L.sal_jz_beam * g.plane;
+            double* Sy = F + L.sal_Sy * g.plane;
+            double* Sx = F + L.sal_Sx * g.plane;
+            for (int j = 0; j < g.ny; ++j)
+                for (int i = 0; i < g.nx; ++i) {
+                    const double dx_jzb = (jzb[g.idx(i + 1, j)] - jzb[g.idx(i - 1, j)]) * dxih;
+                    const double dy_jzb = (jzb[g.idx(i, j + 1)] - jzb[g.idx(i, j - 1)]) * dyih;
+                    Sy[g.idx(i, j)] = mu0 * (-dy_jzb);
+                    Sx[g.idx(i, j)] = -mu0 * (-dx_jzb);
+                }
+        }
+        mg_solve_bxby(s, L.sal_Bx, L.sal_Sy);
+        for (int comp : {L.sal_Ez, L.sal_jx, L.sal_jy}) set_plane(s, comp, 0.);
+        if (s.salame_do_advance) {
+            if (s.has_plasma) {
+                salame_only_advance(s.c, s.plasma, F);
+                deposit_plasma(s.c, s.plasma, F, L.sal_jx, L.sal_jy, -1, -1, -1, -1, false, &nq);
+            }
+        } else {
+            const double dz = 1.5 * g.dz;
+            const double mu0_inv = 1.0 / 1.0;
+            const double* chi = F + L.chi * g.plane;
+            const double* Bx = F + L.sal_Bx * g.plane;
+            const double* By = F + L.sal_By * g.plane;
+            double* jx = F + L.sal_jx * g.plane;
+            double* jy = F + L.sal_jy * g.plane;
+            for (int j = 0; j < g.ny; ++j)
+                for (int i = 0; i < g.nx; ++i) {
+                    const long k = g.idx(i, j);
+                    jx[k] = dz * chi[k] * By[k] * mu0_inv;
+                    jy[k] = -dz * chi[k] * Bx[k] * mu0_inv;
+                }
+        }
+        solve_poisson_ez(s.c, F, L.sal_jx, L.sal_jy, L.sal_Ez);
+        // STEP 3
+        set_plane(s, L.sal_jz_beam, 0.);
+        deposit_beam(s.c, b, islice, F, -1, -1, L.sal_jz_beam, mask, 1);
+        double sum_Ez_target = 0, sum_Ez_no_salame = 0, sum_Ez_only_salame = 0, sum_jz = 0;
+        {
+            const double factor = g.dx * g.dy / (g.dx * g.dy);
+            const double* jz = F + L.sal_jz_beam * g.plane;
+            const double* et = F + L.sal_Ez_target * g.plane;
+            const double* en = F + L.sal_Ez_no_salame * g.plane;
+            const double* ez = F + L.sal_Ez * g.plane;
+            for (int j = 0; j < g.ny; ++j)
+                for (int i = 0; i < g.nx; ++i) {
+                    const long k = g.idx(i, j);
+                    sum_Ez_target += factor * jz[k] * et[k];
+                    sum_Ez_no_salame += factor * jz[k] * en[k];
+                    sum_Ez_only_salame += factor * jz[k] * ez[k];
+                    sum_jz += factor * jz[k];
+                }
+        }
+        sum_Ez_target /= sum_jz;
+        sum_Ez_no_salame /= sum_jz;
+        sum_Ez_only_salame /= sum_jz;
+        const double zeta = (islice - 1) * g.dz + g.zoff;
+        if (s.salame_target) sum_Ez_target = s.salame_target(zeta, s.salame_zeta_initial, sum_Ez_target, s.salame_user);
+        W = (sum_Ez_target - sum_Ez_no_salame) / sum_Ez_only_salame + 1.0;
+        if (W < 0 || s.salame_overloaded) {
+            W = 0;
+            iter = s.salame_n_iter - 1;
+            s.salame_overloaded = true;
+        }
+        if (!s.salame_overloaded && iter >= 1 && std::abs(W - 1.) < s.salame_tol) iter = s.salame_n_iter - 1;
+        for (int64_t ip = b.off[islice]; ip < b.off[islice + 1]; ++ip) {
+            if (mask[ip] != 1 || !b.valid[ip]) continue;
+            if (W == 0) {
+                b.valid[ip] = 0;
+                continue;
+            }
+            b.w[ip] *= W;
+        }
+        // STEP 4
+        for (int comp : {L.jz_beam, L.Sy, L.Sx}) set_plane(s, comp, 0.);
+        deposit_beam(s.c, b, islice, F, -1, -1, L.jz_beam);
+        initialize_sxsy_with_beam(s.c, F);
+        add_plane(s, L.Sy, L.sal_Sy_back); add_plane(s, L.Sx, L.sal_Sx_back);
+        mg_solve_bxby(s, L.Bx, L.Sy);
+    }
+    s.n_qsa += nq;
+    if (s.salame_weight_factor) s.salame_weight_factor[islice] = W;
+}
+
+// Hipace::SolveOneSlice, src/Hipace.cpp:557-728 (level 0, no laser/ionisation).
 static void solve_one_slice(Sim& s, int islice) {
     const Layout& L = s.c.L;
     const Options& o = s.c.o;
@@ -189,14 +327,18 @@ static void solve_one_slice(Sim& s, int islice) {
     }
     solve_poisson_psi_ez_bz(s.c, F);
     if (o.explicit_solver) {
-        if (o.do_beam_jx_jy) deposit_beam(s.c, s.beam, islice - 1, F, L.next_jx_beam, L.next_jy_beam, -1);
+        // SALAME beams deposit no jx, jy while SALAME runs (src/particles/beam/MultiBeam.cpp:50-56)
+        const int32_t* sal = s.salame_on ? s.beam.salame.data() : nullptr;
+        if (o.do_beam_jx_jy) deposit_beam(s.c, s.beam, islice - 1, F, L.next_jx_beam, L.next_jy_beam, -1, sal, 0);
         initialize_sxsy_with_beam(s.c, F);
         if (s.has_plasma) explicit_deposit(s.c, s.plasma, F);
-        int it = 0;
-        const int st = mg_solve1(s.c.g, F + L.Bx * s.c.g.plane, F + L.Sy * s.c.g.plane, F + L.chi * s.c.g.plane,
-                                 s.c.g.plane, o.mg_tol_rel, o.mg_tol_abs, o.mg_maxiter, &it);
-        s.mg_iters += it;
-        if (st) s.mg_status = st;
+        mg_solve_bxby(s, L.Bx, L.Sy);
+        // MultiBeam::isSalameNow (MultiBeam.cpp:128-140): particles of a SALAME beam on this slice
+        if (s.salame_on) {
+            long n = 0;
+            for (int64_t ip = s.beam.off[islice]; ip < s.beam.off[islice + 1]; ++ip) n += sal[ip] == 1;
+            if (n > 0) salame_module(s, islice);
+        }
     } else {
         pc_loop(s, islice);
     }
@@ -254,6 +396,7 @@ struct orc_config {
     double plasma_charge, plasma_mass, max_qsa;
     int neutralize;
     int nthreads;
+    int salame;
 };
 
 static Geom geom_from(const orc_config* cf) {
@@ -276,7 +419,7 @@ static Ctx ctx_from(const orc_config* cf) {
     o.plo[0] = cf->plo[0]; o.plo[1] = cf->plo[1]; o.phi[0] = cf->phi[0]; o.phi[1] = cf->phi[1];
     o.pc_tol = cf->pc_tol; o.pc_maxiter = cf->pc_maxiter; o.pc_mix = cf->pc_mix;
     o.nthreads = cf->nthreads;
-    c.L = make_layout(o.explicit_solver, o.deposit_rho, cf->has_plasma && cf->neutralize);
+    c.L = make_layout(o.explicit_solver, o.deposit_rho, cf->has_plasma && cf->neutralize, cf->salame != 0);
     return c;
 }
 
@@ -302,12 +445,14 @@ void orc_sim_destroy(void* h) { delete static_cast<Sim*>(h); }
 
 int orc_sim_ncomp(void* h) { return static_cast<Sim*>(h)->c.L.ncomp; }
 
-void orc_sim_layout(void* h, int* out /* 32 ints */) {
+void orc_sim_layout(void* h, int* out /* 44 ints */) {
     const Layout& L = static_cast<Sim*>(h)->c.L;
     const int v[] = {L.ncomp, L.next_jx_beam, L.next_jy_beam, L.chi, L.Sy, L.Sx, L.ExmBy, L.EypBx, L.Ez, L.Bx,
                      L.By, L.Bz, L.Psi, L.jx_beam, L.jy_beam, L.jz_beam, L.jx, L.jy, L.rhomjz, L.rho,
                      L.prev_jx_beam, L.prev_jy_beam, L.ions_rhomjz, L.next_jx, L.next_jy, L.jz, L.prev_Bx,
-                     L.prev_By, L.prev_jx, L.prev_jy, L.pciter_Bx, L.pciter_By};
+                     L.prev_By, L.prev_jx, L.prev_jy, L.pciter_Bx, L.pciter_By, L.sal_Ez_target,
+                     L.sal_Ez_no_salame, L.sal_Ez, L.sal_jx, L.sal_jy, L.sal_jz_beam, L.sal_Bx, L.sal_By,
+                     L.sal_Sy, L.sal_Sx, L.sal_Sy_back, L.sal_Sx_back};
     std::memcpy(out, v, sizeof(v));
 }
 
@@ -322,6 +467,23 @@ void orc_sim_set_beam(void* h, int64_t n, const double* x, const double* y, cons
     b.off.assign(off, off + s->c.g.nz + 1);
     b.charge = charge; b.mass = mass;
     b.ext.clear();
+}
+
+// SALAME on the following slices (mask: n per-particle flags of the present beam; NULL: off)
+void orc_sim_set_salame(void* h, const int32_t* mask, int n_iter, int do_advance, double tol,
+                        double (*target)(double, double, double, void*), void* user, double* weight_factor) {
+    Sim* s = static_cast<Sim*>(h);
+    s->salame_last_slice = -1;
+    s->salame_overloaded = false;
+    s->salame_on = mask != nullptr;
+    if (!mask) return;
+    s->beam.salame.assign(mask, mask + s->beam.x.size());
+    s->salame_n_iter = n_iter;
+    s->salame_do_advance = do_advance != 0;
+    s->salame_tol = tol;
+    s->salame_target = target;
+    s->salame_user = user;
+    s->salame_weight_factor = weight_factor;
 }
 
 // beams.external_E / external_B as affine functions of position (24 coefficients; NULL: none)

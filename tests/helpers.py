@@ -92,3 +92,32 @@ def rel_err(a, b):
     if den == 0.0:
         return num
     return num / den
+
+
+def salame_case(nz=48, do_salame=True, zmin=-2.0, zmax=-0.5):
+    """(cfg, beam): a drive beam and, behind it in the accelerating phase of its wake, a
+    flat-top witness that SALAME reshapes (<beam>.do_salame)."""
+    from hipace2_amd.inputs import BeamSpec
+    cfg = SliceConfig(nx=63, ny=63, nz=nz, xlo=-6, xhi=6, ylo=-6, yhi=6, zlo=-8.0, zhi=4.0)
+    cfg.salame = do_salame
+    drive = BeamSpec(injection_type="fixed_weight", profile="gaussian", density=2.0, position_mean=(0, 0, 2.0),
+                     position_std=(0.3, 0.3, 0.6), num_particles=20000, u_mean=(0, 0, 1000.0), seed=1)
+    wit = BeamSpec(injection_type="fixed_ppc", profile="flattop", density=0.5, zmin=zmin, zmax=zmax, radius=0.4,
+                   ppc=(2, 2, 1), u_mean=(0, 0, 1000.0), do_salame=do_salame)
+    return cfg, merge_beams(cfg, [make_beam(cfg, drive), make_beam(cfg, wit)])
+
+
+def salame_flatness(ez, jz, slices, target=None):
+    """Relative deviation, per SALAME slice s, of the jz(s)-weighted mean of Ez on the next
+    slice (s - 1) from the target: the jz(s)-weighted mean of the first SALAME slice's Ez
+    plane, or target(s) when given."""
+    s0 = max(slices)
+    out = {}
+    for s in sorted(slices, reverse=True):
+        if s == 0:
+            continue
+        j = jz[s]
+        got = float((j * ez[s - 1]).sum() / j.sum())
+        want = float((j * ez[s0]).sum() / j.sum()) if target is None else target(s)
+        out[s] = (got - want) / abs(want)
+    return out

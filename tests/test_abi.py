@@ -41,3 +41,11 @@ def test_product_does_not_reference_the_oracle():
             if f.endswith((".py", ".hip", ".cpp", ".hpp", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower(), os.path.join(dirpath, f)
+
+
+def test_layout_names_match_header():
+    """hpq_layout() writes HPQ_LAYOUT_N entries; the binding names every one of them."""
+    from hipace2_amd import _abi
+    text = open(os.path.join(ROOT, "include", "hpq.h")).read()
+    n = int(re.search(r"#define HPQ_LAYOUT_N (\d+)", text).group(1))
+    assert len(_abi.LAYOUT_NAMES) == n and len(set(_abi.LAYOUT_NAMES)) == n
