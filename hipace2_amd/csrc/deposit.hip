@@ -40,6 +40,11 @@ __device__ __forceinline__ bool in_store(const Geom& g, int i, int j) {
     return i >= -g.ng && i <= g.nx - 1 + g.ng && j >= -g.ng && j <= g.ny - 1 + g.ng;
 }
 
+// the S x S stencil starting at cell (i, j) lies inside the guarded plane
+__device__ __forceinline__ bool stencil_in_store(const Geom& g, int i, int j, int S) {
+    return i >= -g.ng && i + S - 1 <= g.nx - 1 + g.ng && j >= -g.ng && j + S - 1 <= g.ny - 1 + g.ng;
+}
+
 // window of cells [i0, i0 + w) x [j0, j0 + h) in LDS; w == 0: none
 struct Win {
     int i0, j0, w, h;
@@ -203,6 +208,24 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
         }
         const bool local = W.holds(ci[q], cj[q], S);
         const R cmmr = R(charge_mu0_mass_ratio);
+        if (local && stencil_in_store(g, sx[0].cell, sy[0].cell, S)) {
+            // the stencil's cells are consecutive and all in the window: no per-cell checks
+            const int k0 = W.at(sx[0].cell, sy[0].cell);
+#pragma unroll
+            for (int iy = 0; iy < S; ++iy) {
+#pragma unroll
+                for (int ix = 0; ix < S; ++ix) {
+                    const R cd = q_invvol[q] * sx[ix].s * sy[iy].s;
+                    const R v[6] = {cd * vx_c[q], cd * vy_c[q], cd * (gamma_psi[q] - R(1.0)) * clight,
+                                    cd * gamma_psi[q], cd * cmmr * psi_inv[q], cd};
+                    const int k = k0 + iy * W.w + ix;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a)
+                        if (slot[a] >= 0) WinAcc<R>::add(win, cells, slot[a], k, v[a]);
+                }
+            }
+            continue;
+        }
 #pragma unroll
         for (int iy = 0; iy < S; ++iy) {
 #pragma unroll
@@ -314,6 +337,34 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
             sy[k] = dshape<T, O, R>(ymid[q], k);
         }
         const bool local = W.holds(ci[q], cj[q], S);
+        // the contribution of cell (ix, iy) from the four field values there
+        auto contrib = [&](int ix, int iy, R Bz_v, R Ez_v, R ExmBy_v, R EypBx_v, R& sy_add, R& sx_add) {
+            const DsfR<R> X = sx[ix], Y = sy[iy];
+            const R A = fma(-Bz_v, pvx, fma(EypBx_v, gpy, fma(-ExmBy_v, vxvy, Ez_v * pvy)) * clight_inv);
+            const R B = fma(Bz_v, pvy, fma(-EypBx_v, vxvy, fma(ExmBy_v, gpx, Ez_v * pvx)) * clight_inv);
+            const R ss = X.s * Y.s, ds = X.ds * Y.s, sd = X.s * Y.ds;
+            sy_add = fma(ay * A, ss, fma(by, ds, cy * sd));
+            sx_add = fma(ax * B, ss, fma(bx, ds, cx * sd));
+        };
+        if (local && stencil_in_store(g, sx[0].cell, sy[0].cell, S)) {
+            // the stencil's cells are consecutive and all in the window: no per-cell checks
+            const int k0 = W.at(sx[0].cell, sy[0].cell);
+            const double* wf = win + 2 * cells;
+#pragma unroll
+            for (int iy = 0; iy < S; ++iy) {
+#pragma unroll
+                for (int ix = 0; ix < S; ++ix) {
+                    if (T == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) continue;
+                    const int kw = k0 + iy * W.w + ix;
+                    R sy_add, sx_add;
+                    contrib(ix, iy, R(wf[kw]), R(wf[cells + kw]), R(wf[2 * cells + kw]), R(wf[3 * cells + kw]),
+                            sy_add, sx_add);
+                    WinAcc<R>::add(win, cells, 0, kw, sy_add);
+                    WinAcc<R>::add(win, cells, 1, kw, sx_add);
+                }
+            }
+            continue;
+        }
 #pragma unroll
         for (int iy = 0; iy < S; ++iy) {
 #pragma unroll
@@ -330,11 +381,8 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
                 } else {
                     Bz_v = R(Bz[k]); Ez_v = R(Ez[k]); ExmBy_v = R(ExmBy[k]); EypBx_v = R(EypBx[k]);
                 }
-                const R A = fma(-Bz_v, pvx, fma(EypBx_v, gpy, fma(-ExmBy_v, vxvy, Ez_v * pvy)) * clight_inv);
-                const R B = fma(Bz_v, pvy, fma(-EypBx_v, vxvy, fma(ExmBy_v, gpx, Ez_v * pvx)) * clight_inv);
-                const R ss = X.s * Y.s, ds = X.ds * Y.s, sd = X.s * Y.ds;
-                const R sy_add = fma(ay * A, ss, fma(by, ds, cy * sd));
-                const R sx_add = fma(ax * B, ss, fma(bx, ds, cx * sd));
+                R sy_add, sx_add;
+                contrib(ix, iy, Bz_v, Ez_v, ExmBy_v, EypBx_v, sy_add, sx_add);
                 if (local) {
                     WinAcc<R>::add(win, cells, 0, kw, sy_add);
                     WinAcc<R>::add(win, cells, 1, kw, sx_add);
