@@ -67,6 +67,14 @@ void layout_init(hpq_ctx* c) {
     c->L = L;
 }
 
+// the second stream and its events, created on first use (default priority)
+void side_init(hpq_ctx* c) {
+    if (c->side) return;
+    HPQ_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&c->ev_start, &c->ev_beam, &c->ev_fields, &c->ev_end})
+        HPQ_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+}
+
 // ---------------------------------------------------------------------------------
 // Fields (src/fields/Fields.cpp)
 // ---------------------------------------------------------------------------------
@@ -215,17 +223,52 @@ public:
     Hipace(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store)
         : c_(c), p_(p), b_(b), s_(store), fields_(c, store), L_(c->L) {}
 
-    // Hipace::SolveOneSlice, :557-728 (level 0; no laser, SALAME, ionisation, collisions).
+    // Hipace::SolveOneSlice, :557-728 (level 0; no laser, ionisation, collisions).
     // initialized: the previous slice of the same pass has already run InitializeSlices for
-    // this one; chain: another slice follows in this pass.
+    // this one; chain: another slice follows in this pass. two: the beam deposition and the
+    // end-of-slice pass run on the context's second stream (explicit solver, chained slices):
+    //   side: [jz_beam of this slice, jx/jy_beam of the next]          after the planes are zeroed
+    //   main: [plasma deposit, Poisson solves] wait(beam) [Sx/Sy, explicit deposit, MG, SALAME]
+    //   side: wait(fields) [diagnostics, ShiftSlices, next InitializeSlices]
+    //   main: [plasma push] wait(end)
+    // The push reads Psi, Ez, Bx, By, Bz, which the end-of-slice pass neither writes nor
+    // zeroes, so the two overlap; the next slice's deposits wait for the zeroing.
     void SolveOneSlice(int islice, bool initialized = false, bool chain = false,
-                       const std::function<void()>& before_push = {}) {
+                       const std::function<void()>& before_push = {}, bool two = false) {
         const hpq_params& P = c_->p;
+        two = two && P.explicit_solver;
         insitu_plasma(c_, p_, islice);
         if (!initialized) fields_.InitializeSlices();
+        auto on_side = [&](auto&& f) {
+            hipStream_t main = c_->stream;
+            c_->stream = c_->side;
+            try {
+                f();
+            } catch (...) {
+                c_->stream = main;
+                throw;
+            }
+            c_->stream = main;
+        };
+        auto beam_deposits = [&] {
+            DepositCurrentSlice(c_, b_, islice, s_, -1, -1, L_.jz_beam);
+            if (P.do_beam_jx_jy) {
+                // SALAME beams deposit no jx, jy while SALAME runs (MultiBeam.cpp:50-56)
+                const int32_t comps[3] = {L_.next_jx_beam, L_.next_jy_beam, -1};
+                deposit_beam_select(c_, b_, islice - 1, s_, comps, c_->salame.mask, 0);
+            }
+        };
+        if (two) {
+            HPQ_HIP(hipEventRecord(c_->ev_start, c_->stream));
+            on_side([&] {
+                HPQ_HIP(hipStreamWaitEvent(c_->stream, c_->ev_start, 0));
+                beam_deposits();
+                HPQ_HIP(hipEventRecord(c_->ev_beam, c_->stream));
+            });
+        }
         if (P.explicit_solver) {
             DepositCurrent(c_, p_, s_, L_.jx, L_.jy, -1, L_.rho, L_.chi, L_.rhomjz);
-            DepositCurrentSlice(c_, b_, islice, s_, -1, -1, L_.jz_beam);
+            if (!two) DepositCurrentSlice(c_, b_, islice, s_, -1, -1, L_.jz_beam);
         } else {
             DepositCurrent(c_, p_, s_, L_.jx, L_.jy, L_.jz, L_.rho, -1, L_.rhomjz);
             DepositCurrentSlice(c_, b_, islice, s_, P.do_beam_jx_jy ? L_.jx : -1, P.do_beam_jx_jy ? L_.jy : -1,
@@ -234,8 +277,9 @@ public:
         fields_.AddRhoIons();
         fields_.SolvePoissonPsiExmByEypBxEzBz();
         if (P.explicit_solver) {
-            if (P.do_beam_jx_jy) {
-                // SALAME beams deposit no jx, jy while SALAME runs (MultiBeam.cpp:50-56)
+            if (two) {
+                HPQ_HIP(hipStreamWaitEvent(c_->stream, c_->ev_beam, 0));
+            } else if (P.do_beam_jx_jy) {
                 const int32_t comps[3] = {L_.next_jx_beam, L_.next_jy_beam, -1};
                 deposit_beam_select(c_, b_, islice - 1, s_, comps, c_->salame.mask, 0);
             }
@@ -247,10 +291,20 @@ public:
             PredictorCorrectorLoopToSolveBxBy(islice);
         }
         insitu_beam(c_, b_, islice);
-        fields_.EndSlice(islice, chain);
+        if (two) {
+            HPQ_HIP(hipEventRecord(c_->ev_fields, c_->stream));
+            on_side([&] {
+                HPQ_HIP(hipStreamWaitEvent(c_->stream, c_->ev_fields, 0));
+                fields_.EndSlice(islice, chain);
+                HPQ_HIP(hipEventRecord(c_->ev_end, c_->stream));
+            });
+        } else {
+            fields_.EndSlice(islice, chain);
+        }
         if (before_push) before_push();
         advance_plasma(c_, p_, s_, false);
         c_->stats[3] += double(p_->n);
+        if (two) HPQ_HIP(hipStreamWaitEvent(c_->stream, c_->ev_end, 0));
         if (!chain) fields_.ShiftSlices();
     }
 
@@ -729,6 +783,9 @@ void hpq_destroy(hpq_ctx* c) {
     beam_slip_free(c);
     insitu_free(c);
     salame_free(c);
+    for (hipEvent_t e : {c->ev_start, c->ev_beam, c->ev_fields, c->ev_end})
+        if (e) (void)hipEventDestroy(e);
+    if (c->side) (void)hipStreamDestroy(c->side);
     delete c;
 }
 
@@ -1035,7 +1092,9 @@ int hpq_run_slices(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* s
         check_plasma(p);
         if (first >= c->g.nz || last < 0 || first < last) throw HpqError(HPQ_E_INVALID, "bad slice range");
         Hipace h(c, p, b, store);
-        for (int isl = first; isl >= last; --isl) h.SolveOneSlice(isl, isl != first, isl != last);
+        const bool two = c->p.explicit_solver && first > last;
+        if (two) side_init(c);
+        for (int isl = first; isl >= last; --isl) h.SolveOneSlice(isl, isl != first, isl != last, {}, two);
         diag_finalize(c);
     });
 }

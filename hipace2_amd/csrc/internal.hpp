@@ -223,6 +223,10 @@ struct hpq_ctx {
     double sal_zeta_initial = 0.0;   // Hipace::m_salame_zeta_initial
     double* sal_dev = nullptr;       // device: 4 sums of SalameGetW, particle count
     double* sal_host = nullptr;      // pinned copy
+    // second stream of the chained slice loop: beam deposition and the end-of-slice pass run
+    // beside the plasma deposition / Poisson solves and the plasma push
+    hipStream_t side = nullptr;
+    hipEvent_t ev_start = nullptr, ev_beam = nullptr, ev_fields = nullptr, ev_end = nullptr;
     double* insitu_partial = nullptr;  // per-workgroup in-situ sums of one slice
     long insitu_cap = 0;               // workgroups insitu_partial holds
     std::vector<hpq::Capture> captures;
