@@ -43,8 +43,26 @@ DTYPE = {"fp64": "f64",
                   "(multigrid, particle state)"}
 
 
-def make_workload(n=511, nz=512, ppc=2, nbeam=4_000_000):
-    from hipace2_amd.inputs import BeamSpec, SliceConfig, make_beam
+# The large workload: 1023 x 1023 over [-10, 10]^2, 1024 slices, 2 x 2 ppc with the transverse
+# density ramp n(r) = n0 (1 + 0.1 r^2), drive of 1e7 particles at peak 5 n0, witness of 2e6
+LARGE = dict(n=1023, nz=1024, half_width=10.0, ppc=2, ramp=0.1, drive_peak=5.0, drive_n=10_000_000,
+             witness_n=2_000_000, witness_peak=1.0)
+
+
+def make_workload(n=511, nz=512, ppc=2, nbeam=4_000_000, large=False):
+    from hipace2_amd.inputs import BeamSpec, SliceConfig, make_beam, merge_beams
+    if large:
+        from hipace2_amd.adjoint import witness_beam
+        P = LARGE
+        L = P["half_width"]
+        cfg = SliceConfig(nx=P["n"], ny=P["n"], nz=P["nz"], xlo=-L, xhi=L, ylo=-L, yhi=L, zlo=-8.0, zhi=4.0,
+                          depos_order=2, derivative_type=2, explicit_solver=True, ppc=(P["ppc"], P["ppc"]),
+                          ramp_a=P["ramp"])
+        drive = make_beam(cfg, BeamSpec(injection_type="fixed_weight", profile="gaussian", density=P["drive_peak"],
+                                        position_std=(0.3, 0.3, 1.41), num_particles=P["drive_n"],
+                                        u_mean=(0.0, 0.0, 1000.0), seed=1234))
+        wit = witness_beam(cfg, P["witness_n"], -6.0, -4.0, 0.2, P["witness_peak"], 1000.0, seed=4321)
+        return cfg, merge_beams(cfg, [drive, wit])
     cfg = SliceConfig(nx=n, ny=n, nz=nz, xlo=-8.0, xhi=8.0, ylo=-8.0, yhi=8.0, zlo=-8.0, zhi=4.0,
                       depos_order=2, derivative_type=2, explicit_solver=True, ppc=(ppc, ppc))
     spec = BeamSpec(injection_type="fixed_weight", profile="gaussian", density=0.01,
@@ -72,7 +90,7 @@ def run_gpu(args, precision, capture):
     import torch
     from hipace2_amd.simulation import Simulation
 
-    cfg, beam = make_workload(args.n, args.nz, args.ppc, args.nbeam)
+    cfg, beam = make_workload(args.n, args.nz, args.ppc, args.nbeam, large=args.workload == "large")
     if args.bin_period is not None:
         cfg.bin_period = args.bin_period
     cfg.precision = precision
@@ -129,7 +147,7 @@ def run_gpu(args, precision, capture):
 def measured_traffic(kind, precision, args):
     """HBM bytes per launch of kernel family `kind` on this workload, from the committed
     counter summary, or None when it does not cover this kernel and workload."""
-    if (args.n, args.nz, args.ppc) != (511, 512, 2) or not os.path.exists(TRAFFIC_FILE):
+    if (args.n, args.nz, args.ppc) != (511, 512, 2) or args.workload != "standard" or not os.path.exists(TRAFFIC_FILE):
         return None
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
@@ -168,25 +186,41 @@ def run_adjoint(args):
     from hipace2_amd.inputs import BeamSpec, SliceConfig, make_beam
     from hipace2_amd.simulation import Simulation
 
-    n, nz = args.n, args.nz
-    cfg = SliceConfig(nx=n, ny=n, nz=nz, xlo=-8.0, xhi=8.0, ylo=-8.0, yhi=8.0, zlo=-8.0, zhi=4.0,
-                      depos_order=2, derivative_type=2, explicit_solver=True, ppc=(1, 1))
+    large = args.workload == "large"
+    if large:
+        P = LARGE
+        n, nz, L = P["n"], P["nz"], P["half_width"]
+        cfg = SliceConfig(nx=n, ny=n, nz=nz, xlo=-L, xhi=L, ylo=-L, yhi=L, zlo=-8.0, zhi=4.0, depos_order=2,
+                          derivative_type=2, explicit_solver=True, ppc=(P["ppc"], P["ppc"]), ramp_a=P["ramp"])
+    else:
+        n, nz = args.n, args.nz
+        cfg = SliceConfig(nx=n, ny=n, nz=nz, xlo=-8.0, xhi=8.0, ylo=-8.0, yhi=8.0, zlo=-8.0, zhi=4.0,
+                          depos_order=2, derivative_type=2, explicit_solver=True, ppc=(1, 1))
     cfg.do_beam_jx_jy = False
     cfg.mg_tol_rel = 1e-10
-    drive = make_beam(cfg, BeamSpec(injection_type="fixed_weight", profile="gaussian", density=0.01,
-                                    position_std=(0.3, 0.3, 1.41), num_particles=100_000,
+    drive = make_beam(cfg, BeamSpec(injection_type="fixed_weight", profile="gaussian",
+                                    density=LARGE["drive_peak"] if large else 0.01,
+                                    position_std=(0.3, 0.3, 1.41),
+                                    num_particles=LARGE["drive_n"] if large else 100_000,
                                     u_mean=(0.0, 0.0, 1000.0), seed=1234))
-    wit = witness_beam(cfg, 1_000_000, -6.0, -4.0, 0.2, 0.005, 1000.0, seed=4321)
+    wit = witness_beam(cfg, LARGE["witness_n"] if large else 1_000_000, -6.0, -4.0, 0.2,
+                       LARGE["witness_peak"] if large else 0.005, 1000.0, seed=4321)
     bl = BeamLoading(cfg, drive, wit, -6.0, -4.0)
     sim = Simulation(cfg, bl.beam)
     knots0 = torch.as_tensor(np.random.default_rng(99).uniform(0.5, 1.5, 64), device="cuda")
     wsl = bl.witness_slices()
+    # the drive's charge as a factor q on its weights (q = 1 is the nominal drive)
+    drive_w = torch.as_tensor(bl.base_w, device="cuda")
+
+    def weights(knots, q=None):
+        w = bl.weights(knots)
+        return w if q is None else w + (q - 1.0) * drive_w
 
     def loss_of(ez):
         return on_axis_ez_variance(ez, wsl)
 
-    def plain(knots):
-        w = bl.weights(knots)
+    def plain(knots, q=None):
+        w = weights(knots, q)
         sim.beam["w"][: w.numel()].copy_(w)
         ez = sim.capture("Ez")
         sim.begin_step()
@@ -202,9 +236,10 @@ def run_adjoint(args):
     t_plain = time.perf_counter() - t0
 
     knots = knots0.clone().requires_grad_(True)
+    q = torch.ones((), dtype=torch.float64, device="cuda", requires_grad=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ez = slice_loop_ez(bl.weights(knots), sim, checkpoint_every=32, mg_tol=1e-9)
+    ez = slice_loop_ez(weights(knots, q), sim, checkpoint_every=32, mg_tol=1e-9)
     loss = loss_of(ez)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
@@ -213,6 +248,7 @@ def run_adjoint(args):
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
     grad = knots.grad.detach().cpu().numpy()
+    grad_q = float(q.grad)
 
     # the same gradient with the forward (and its recomputation) on the mixed path: FP32
     # particle arithmetic and split-BF16 DST; the vector-Jacobian products stay FP64
@@ -239,11 +275,26 @@ def run_adjoint(args):
     g = grad[picks] if len(picks) else np.zeros(0)
     scale = float(np.abs(fd).max()) if fd.size else 0.0
     err = float(np.abs(g - fd).max() / scale) if scale > 0 else None
+    # the drive-charge derivative against central differences of q at two step sizes: in a
+    # strong blowout the loss is only C1 in the drive charge (order-2 shapes crossing cells),
+    # so the difference quotient converges at O(h) and the smaller step is the reference
+    fd_q, err_q = {}, None
+    if len(picks):
+        one = torch.ones((), dtype=torch.float64, device="cuda")
+        for hq in (1e-3, 1e-4):
+            fd_q[str(hq)] = (plain(knots0, one + hq) - plain(knots0, one - hq)) / (2 * hq)
+        ref = fd_q["0.0001"]
+        err_q = abs(grad_q - ref) / abs(ref) if ref else None
     print(f"adjoint: plain {t_plain:.3f} s, checkpointed forward {t_fwd:.3f} s, backward {t_bwd:.3f} s, "
-          f"FD max rel err {err}, mixed-forward gradient vs FP64 {mixed_err:.3e}", file=sys.stderr, flush=True)
-    return {"workload": f"{n}x{n}x{nz}, 1x1 ppc, drive 1e5 + witness 1e6 particles, 64 knots, loss Var(on-axis Ez) "
+          f"FD max rel err {err}, drive-charge derivative err {err_q}, mixed-forward gradient vs FP64 "
+          f"{mixed_err:.3e}", file=sys.stderr, flush=True)
+    plasma = (f"{LARGE['ppc']}x{LARGE['ppc']} ppc with n(r) = n0 (1 + {LARGE['ramp']} r^2), drive 1e7 (peak "
+              f"{LARGE['drive_peak']}) + witness 2e6 particles" if large else
+              "1x1 ppc, drive 1e5 + witness 1e6 particles")
+    return {"workload": f"{n}x{n}x{nz}, {plasma}, 64 knots and the drive charge, loss Var(on-axis Ez) "
                         "over the witness slices, checkpoint every 32 slices, MG tolerance 1e-10 (adjoint solves 1e-9, warm-started), FP64",
-            "loss": float(loss), "plain_forward_s": round(t_plain, 4), "checkpointed_forward_s": round(t_fwd, 4),
+            "drive_charge_grad": grad_q, "drive_charge_fd_by_step": fd_q, "drive_charge_rel_err_vs_fd_1e-4": err_q,
+            "loss": float(loss.detach()), "plain_forward_s": round(t_plain, 4), "checkpointed_forward_s": round(t_fwd, 4),
             "backward_s": round(t_bwd, 4), "ratio_to_forward": round((t_fwd + t_bwd) / t_plain, 3),
             "fd_knots": [int(k) for k in picks], "fd_max_rel_err": err,
             "mixed_forward_grad_rel_err_vs_fp64": mixed_err, "mixed_forward_plus_backward_s": round(t_mixed, 4),
@@ -305,6 +356,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the beam-loading gradient measurement")
     ap.add_argument("--adjoint-fd-knots", type=int, default=8,
                     help="knots checked against central differences (two forward steps each)")
+    ap.add_argument("--workload", choices=["standard", "large"], default="standard",
+                    help="standard: 511^2 x 512 (the metric's configuration); large: 1023^2 x 1024, 2x2 ppc with "
+                         "a density ramp, drive 1e7 + witness 2e6 particles")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device to device, the reference dispatch for HBM byte counters")
     args = ap.parse_args()
@@ -386,8 +440,9 @@ def main():
         "vs_baseline": None,
         "dtype": DTYPE[head],
         "data": "synthetic (lattice plasma, seeded Philox Gaussian beam)",
-        "config": {"workload": WORKLOAD if (args.n, args.nz, args.ppc) == (511, 512, 2) else
-                   f"{args.n}x{args.n}x{args.nz} slice loop, {args.ppc}x{args.ppc} ppc, shape 2, explicit MG",
+        "config": {"workload": WORKLOAD if (args.n, args.nz, args.ppc) == (511, 512, 2) and args.workload == "standard"
+                   else f"{cfg.nx}x{cfg.ny}x{cfg.nz} slice loop, {cfg.ppc[0]}x{cfg.ppc[1]} ppc"
+                        f"{', density ramp' if cfg.ramp_a else ''}, shape 2, explicit MG",
                    "precision": head, "poisson_solver": cfg.poisson_solver,
                    "nx": cfg.nx, "ny": cfg.ny, "slices": cfg.nz, "plasma_particles_per_slice": n_plasma,
                    "beam_particles": beam.n, "mg_vcycles_per_slice": round(stats["mg_iters"] / cfg.nz, 3),
