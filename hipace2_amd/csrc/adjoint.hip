@@ -143,27 +143,32 @@ __global__ __launch_bounds__(VT) void k_explicit_vjp(Geom g, hpq_plasma p, hpq_p
 // the tangents of NB of the inputs, starting at input k0: 11 inputs in three launches keeps
 // the tangent arithmetic in registers. The adjoints of G go to a scratch array.
 constexpr int PUSH_IN = 11;
+constexpr int PUSH_T0 = 2;   // first input that needs a tangent direction
 
 template <int N>
 struct PushOut {
     Tan<N> x, y, uxh, uyh, psih, ux, uy, psi;
     bool removed;
+    double rx, ry;   // d x / d x_prev, d y / d y_prev: -1 after a reflection, else 1
 };
 
 // EnforceBC (GetAndSetPosition.H:29-99) on tangents: folds and shifts by constants keep the
 // derivative, a reflection negates it
 template <int N>
-__device__ bool enforce_bc_tan(const BC& bc, Tan<N>& x, Tan<N>& y, Tan<N>& ux, Tan<N>& uy) {
+__device__ bool enforce_bc_tan(const BC& bc, Tan<N>& x, Tan<N>& y, Tan<N>& ux, Tan<N>& uy, double& rx,
+                               double& ry) {
+    rx = 1.0;
+    ry = 1.0;
     if (x.v < bc.plo0 || y.v < bc.plo1 || x.v > bc.phi0 || y.v > bc.phi1) {
         const double lx = bc.phi0 - bc.plo0, ly = bc.phi1 - bc.plo1;
         auto shift = [](Tan<N>& t, double nv) { t.v = nv; };
         if (bc.type == HPQ_BC_REFLECTING) {
             double xv = fmod(x.v - bc.plo0, 2 * lx); if (xv < 0) xv += 2 * lx; xv += bc.plo0;
             shift(x, xv);
-            if (x.v > bc.phi0) { x = 2 * bc.phi0 - x; ux = -ux; }
+            if (x.v > bc.phi0) { x = 2 * bc.phi0 - x; ux = -ux; rx = -1.0; }
             double yv = fmod(y.v - bc.plo1, 2 * ly); if (yv < 0) yv += 2 * ly; yv += bc.plo1;
             shift(y, yv);
-            if (y.v > bc.phi1) { y = 2 * bc.phi1 - y; uy = -uy; }
+            if (y.v > bc.phi1) { y = 2 * bc.phi1 - y; uy = -uy; ry = -1.0; }
             return false;
         } else if (bc.type == HPQ_BC_PERIODIC) {
             double xv = fmod(x.v - bc.plo0, lx); if (xv < 0) xv += lx; xv += bc.plo0;
@@ -242,7 +247,7 @@ __device__ PushOut<N> push_tan(const Geom& g, const hpq_plasma& p, long ip, cons
     xp = xp + dz * clight_inv * (ux * (1.0 / psi));
     yp = yp + dz * clight_inv * (uy * (1.0 / psi));
     PushOut<N> o;
-    o.removed = enforce_bc_tan(bc, xp, yp, ux, uy);
+    o.removed = enforce_bc_tan(bc, xp, yp, ux, uy, o.rx, o.ry);
     o.x = xp;
     o.y = yp;
     o.uxh = ux;
@@ -274,6 +279,12 @@ __global__ __launch_bounds__(VT) void k_push_vjp(Geom g, hpq_plasma p, const dou
         for (int k = 0; k < N; ++k)
             lam[k] = lx * o.x.d[k] + ly * o.y.d[k] + luxh * o.uxh.d[k] + luyh * o.uyh.d[k] + lph * o.psih.d[k] +
                      lux * o.ux.d[k] + luy * o.uy.d[k] + lps * o.psi.d[k];
+        // x_prev, y_prev enter the outputs only through x = x_prev + ..., y = y_prev + ... (their
+        // part through the gathered fields is k_push_vjp_gather's): no tangent pass needed
+        if (k0 == PUSH_T0) {
+            in.x[ip] += lx * o.rx;
+            in.y[ip] += ly * o.ry;
+        }
     }
     double* dst[5] = {in.x, in.y, in.ux_half_step, in.uy_half_step, in.psi_half_step};
 #pragma unroll
@@ -386,7 +397,7 @@ void advance_plasma_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, co
     }
     dispatch_order(c->p.depos_order, [&](auto O) {
         constexpr int o = decltype(O)::value;
-        for (int k0 = 0; k0 < PUSH_IN; k0 += NB)
+        for (int k0 = PUSH_T0; k0 < PUSH_IN; k0 += NB)
             hipLaunchKernelGGL((k_push_vjp<o, NB>), grid1d(p->n, VT), dim3(VT), 0, c->stream, c->g, *p, store, c->L,
                                qmc, bc, *out, *in, A.push_g, k0);
         hipLaunchKernelGGL((k_push_vjp_gather<o>), grid1d(p->n, VT), dim3(VT), 0, c->stream, c->g, *p, store, c->L,

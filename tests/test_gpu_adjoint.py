@@ -107,9 +107,11 @@ def test_explicit_deposition_vjp(order, dtype):
     _close(vjp, fd)
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
-def test_advance_plasma_particles_vjp(order):
+@pytest.mark.parametrize("order,bc", [(1, "Periodic"), (2, "Periodic"), (3, "Periodic"), (2, "Reflecting"),
+                                      (2, "Absorbing")])
+def test_advance_plasma_particles_vjp(order, bc):
     cfg = small_config(order=order, dtype=2)
+    cfg.particle_bc = bc
     sim, orc = _pair(cfg, n_particles=3000, seed=13)
     d0 = orc.plasma()
     # the random state clips some positions to exactly half a cell inside the box, a kink
@@ -117,6 +119,13 @@ def test_advance_plasma_particles_vjp(order):
     nudge = np.random.default_rng(14)
     for k, h in (("x_prev", cfg.dx), ("y_prev", cfg.dy)):
         d0[k] = d0[k] + h * nudge.uniform(0.01, 0.02, d0[k].shape)
+    # a tenth of the particles close to the upper x and y boundaries, moving out of the box
+    # within this push (reflected, wrapped or absorbed)
+    edge = slice(0, 300)
+    d0["x_prev"][edge] = cfg.xhi - nudge.uniform(0.005, 0.02, 300)
+    d0["y_prev"][edge] = cfg.yhi - nudge.uniform(0.005, 0.02, 300)
+    d0["ux_half_step"][edge] = nudge.uniform(0.4, 0.6, 300)
+    d0["uy_half_step"][edge] = nudge.uniform(0.4, 0.6, 300)
     orc.set_plasma(d0)
     sim.set_plasma(d0)
     # fields a tenth of the unit random scale keep every particle in a smooth regime
