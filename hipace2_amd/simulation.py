@@ -136,6 +136,7 @@ class Simulation:
         """Re-sort the beam by zeta-slice on the device (hpq_sort_beam_slices); updates the
         slice offsets used by deposition and push."""
         _abi.check(self.ctx, self.L.hpq_sort_beam_slices(self.ctx, C.byref(self._beam_state()), self._boff))
+        self._beam_sorted = True   # the per-particle SALAME flags no longer follow the particles
         self.beam_host.off = np.array(list(self._boff), dtype=np.int64)
         self._bdesc.off = C.cast(self._boff, C.POINTER(C.c_int64))
 
@@ -172,6 +173,8 @@ class Simulation:
         cfg = self.cfg
         if "salame" not in self.beam:
             raise ValueError("SALAME needs beam particles of a beam with do_salame")
+        if getattr(self, "_beam_sorted", False):
+            raise ValueError("SALAME runs in the first time step, before the beam is re-sorted by slice")
         if target is None and cfg.salame_target.replace(" ", "") != "Ez_initial":
             target = parse_function(cfg.salame_target, ["zeta", "zeta_initial", "Ez_initial"])
         d = _abi.Salame()

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = ["linear_wake.normalized.1Rank", "gaussian_linear_wake.normalized.1Rank",
          "beam_in_vacuum.normalized.1Rank", "beam_in_vacuum.normalized.Serial", "blowout_wake_explicit.2Rank",
-         "beam_evolution.1Rank"]
+         "blowout_wake.Serial", "beam_evolution.1Rank"]
 
 
 @pytest.mark.parametrize("name", CASES)

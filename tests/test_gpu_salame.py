@@ -65,3 +65,29 @@ def test_salame_needs_the_explicit_solver():
     cfg.explicit_solver = False
     with pytest.raises(_abi.HpqError, match="predictor-corrector"):
         Simulation(cfg, beam)
+
+
+def test_salame_first_step_of_a_moving_beam_run():
+    """run_steps: SALAME in the first time step only (MultiBeam::isSalameNow, step 0), the beam
+    pushed after every slice, then a second step with the reshaped weights; against the
+    oracle driven the same way."""
+    from hipace2_amd.simulation import Simulation
+    from oracle.oracle import OracleSim
+    cfg, beam = salame_case(nz=32)
+    sim, orc = Simulation(cfg, beam), OracleSim(cfg, beam, nthreads=4)
+    sim.run_steps(2, dt=2.0, n_subcycles=4)
+    orc.enable_salame(beam.salame)
+    orc.begin_step()
+    orc.run_step_with_beam(2.0, 4)
+    orc.disable_salame()
+    orc.sort_beam()
+    orc.begin_step()
+    orc.run_step_with_beam(2.0, 4)
+    torch.cuda.synchronize()
+    ob = orc.beam()
+    assert np.any(sim.salame_weight_factor != 0)
+    assert np.array_equal(np.asarray(sim.beam_host.off), ob["off"])
+    assert np.array_equal(sim.beam["valid"][: beam.n].cpu().numpy(), ob["valid"])
+    for k in ("w", "x", "y", "z", "ux", "uy", "uz"):
+        e = rel_err(sim.beam[k][: beam.n].cpu().numpy(), ob[k])
+        assert e <= 1e-9, (k, e)
