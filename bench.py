@@ -288,6 +288,7 @@ def run_adjoint(args):
     print(f"adjoint: plain {t_plain:.3f} s, checkpointed forward {t_fwd:.3f} s, backward {t_bwd:.3f} s, "
           f"FD max rel err {err}, drive-charge derivative err {err_q}, mixed-forward gradient vs FP64 "
           f"{mixed_err:.3e}", file=sys.stderr, flush=True)
+    loading = None if large or args.loading_iters <= 0 else beam_loading(args, cfg, bl, sim, knots0, weights, loss_of)
     plasma = (f"{LARGE['ppc']}x{LARGE['ppc']} ppc with n(r) = n0 (1 + {LARGE['ramp']} r^2), drive 1e7 (peak "
               f"{LARGE['drive_peak']}) + witness 2e6 particles" if large else
               "1x1 ppc, drive 1e5 + witness 1e6 particles")
@@ -298,7 +299,66 @@ def run_adjoint(args):
             "backward_s": round(t_bwd, 4), "ratio_to_forward": round((t_fwd + t_bwd) / t_plain, 3),
             "fd_knots": [int(k) for k in picks], "fd_max_rel_err": err,
             "mixed_forward_grad_rel_err_vs_fp64": mixed_err, "mixed_forward_plus_backward_s": round(t_mixed, 4),
-            "grad_norm": float(np.linalg.norm(grad))}
+            "grad_norm": float(np.linalg.norm(grad)), "beam_loading": loading}
+
+
+def beam_loading(args, cfg, bl, sim, knots0, weights, loss_of):
+    """The flatness of the witness's accelerating field reached by gradient descent on the 64
+    knots with the adjoint gradient (Adam, `loading_iters` forward + backward passes), against
+    SALAME, the reference's own beam-loading algorithm (src/salame/Salame.cpp), which rescales the
+    witness slice by slice in one step."""
+    import numpy as np
+    import torch
+    from hipace2_amd.adjoint import slice_loop_ez
+    from hipace2_amd.simulation import Simulation
+
+    knots = knots0.clone().requires_grad_(True)
+    opt = torch.optim.Adam([knots], lr=0.05)
+    history = []
+    t0 = time.perf_counter()
+    for _ in range(args.loading_iters):
+        opt.zero_grad()
+        loss = loss_of(slice_loop_ez(weights(knots), sim, checkpoint_every=32, mg_tol=1e-9))
+        loss.backward()
+        history.append(float(loss.detach()))
+        opt.step()
+        with torch.no_grad():
+            knots.clamp_(min=0.0)
+    t_opt = time.perf_counter() - t0
+    w = weights(knots.detach())
+    sim.beam["w"][: w.numel()].copy_(w)
+    ez = sim.capture("Ez")
+    sim.begin_step()
+    sim.run_slices()
+    torch.cuda.synchronize()
+    loss_adj = float(loss_of(ez))
+    sim.clear_captures()
+
+    # SALAME on the witness of the initial knots (its per-slice rescaling replaces the profile)
+    cfg_s = dataclasses.replace(cfg, salame=True)
+    beam = dataclasses.replace(bl.beam)
+    beam.w = weights(knots0).cpu().numpy()
+    mask = np.zeros(beam.n, np.int32)
+    mask[bl.witness_index] = 1
+    beam.salame = mask
+    sim_s = Simulation(cfg_s, beam)
+    ez = sim_s.capture("Ez")
+    sim_s.enable_salame()
+    t0 = time.perf_counter()
+    sim_s.begin_step()
+    sim_s.run_slices()
+    torch.cuda.synchronize()
+    t_sal = time.perf_counter() - t0
+    loss_sal = float(loss_of(ez))
+    n_sal = int(np.count_nonzero(sim_s.salame_weight_factor))
+    del sim_s
+    print(f"beam loading: Var(on-axis Ez) initial {history[0] if history else None:.4e}, adjoint descent "
+          f"{loss_adj:.4e} after {args.loading_iters} iterations ({t_opt:.1f} s), SALAME {loss_sal:.4e} "
+          f"({t_sal:.2f} s)", file=sys.stderr, flush=True)
+    return {"loss_initial": history[0] if history else None, "adjoint_descent_iterations": args.loading_iters,
+            "adjoint_descent_loss": loss_adj, "adjoint_descent_s": round(t_opt, 3),
+            "adjoint_descent_history": [float(f"{v:.5e}") for v in history],
+            "salame_loss": loss_sal, "salame_s": round(t_sal, 3), "salame_slices": n_sal}
 
 
 def cpu_baseline(args, cfg, beam, sim):
@@ -356,6 +416,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the beam-loading gradient measurement")
     ap.add_argument("--adjoint-fd-knots", type=int, default=8,
                     help="knots checked against central differences (two forward steps each)")
+    ap.add_argument("--loading-iters", type=int, default=0,
+                    help="adjoint gradient-descent iterations of the beam-loading comparison with SALAME (0: skip)")
     ap.add_argument("--workload", choices=["standard", "large"], default="standard",
                     help="standard: 511^2 x 512 (the metric's configuration); large: 1023^2 x 1024, 2x2 ppc with "
                          "a density ramp, drive 1e7 + witness 2e6 particles")

@@ -24,7 +24,7 @@ namespace {
 constexpr int SCAN_T = 256, SCAN_ITEMS = 16, SCAN_BLOCK = SCAN_T * SCAN_ITEMS;
 
 template <int O>
-__global__ __launch_bounds__(256) void k_bin_keys(Geom g, hpq_plasma p, int T, int ntx, int nty,
+__global__ __launch_bounds__(256) void k_bin_keys(Geom g, hpq_plasma p, int T, int TY, int ntx, int nty,
                                                   int32_t* __restrict__ keys) {
     const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (ip >= p.n) return;
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_bin_keys(Geom g, hpq_plasma p, int T, i
         // integer division truncates toward zero, as in the reference; the tile index is
         // clamped so a particle outside the guarded box still lands in a real bin
         const int tx = min(max((cx + g.ng) / T, 0), ntx - 1);
-        const int ty = min(max((cy + g.ng) / T, 0), nty - 1);
+        const int ty = min(max((cy + g.ng) / TY, 0), nty - 1);
         key = tx * nty + ty;
     }
     keys[ip] = key;
@@ -175,10 +175,15 @@ void bins_free(hpq_ctx* c) {
 }
 
 void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets) {
-    if (tile <= 0) throw HpqError(HPQ_E_INVALID, "bin_tiles: tile size must be positive");
+    bin_tiles_rect(c, p, tile, tile, perm, offsets);
+}
+
+// tiles of tile x tile_y cells
+void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int32_t* perm, int32_t* offsets) {
+    if (tile <= 0 || tile_y <= 0) throw HpqError(HPQ_E_INVALID, "bin_tiles: tile size must be positive");
     if (p->n > INT32_MAX) throw HpqError(HPQ_E_INVALID, "bin_tiles: more than 2^31-1 particles");
     const Geom& g = c->g;
-    const int ntx = (g.nx + 2 * g.ng + tile - 1) / tile, nty = (g.ny + 2 * g.ng + tile - 1) / tile;
+    const int ntx = (g.nx + 2 * g.ng + tile - 1) / tile, nty = (g.ny + 2 * g.ng + tile_y - 1) / tile_y;
     const int nbins = ntx * nty + 1;
     if (nbins > 16384) throw HpqError(HPQ_E_UNSUPPORTED, "bin_tiles: more than 16383 tiles");
     const long n = p->n;
@@ -193,8 +198,8 @@ void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t
     Bins& B = c->bins;
     grow(B.keys, B.keys_cap, n);
     dispatch_order(c->p.depos_order, [&](auto O) {
-        hipLaunchKernelGGL((k_bin_keys<decltype(O)::value>), grid1d(n, 256), dim3(256), 0, c->stream, g, *p, tile, ntx,
-                           nty, B.keys);
+        hipLaunchKernelGGL((k_bin_keys<decltype(O)::value>), grid1d(n, 256), dim3(256), 0, c->stream, g, *p, tile,
+                           tile_y, ntx, nty, B.keys);
     });
     HPQ_HIP(hipGetLastError());
     sort_keys(c, n, nbins, perm, offsets);
@@ -297,7 +302,7 @@ const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p) {
     if (c->p.bin_period <= 0 || p->n <= 0) return nullptr;
     if (B.perm_n != p->n || B.stale) {
         grow(B.perm, B.perm_cap, p->n);
-        bin_tiles(c, p, DEPOSIT_TILE, B.perm, nullptr);
+        bin_tiles_rect(c, p, DEPOSIT_TILE_X, DEPOSIT_TILE_Y, B.perm, nullptr);
         B.perm_n = p->n;
         B.stale = false;
     }

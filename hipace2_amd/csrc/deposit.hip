@@ -24,7 +24,7 @@ constexpr int DT = 256;                 // threads per workgroup
 constexpr int DPPT = 4;                 // particles per thread
 constexpr int DCHUNK = DT * DPPT;       // particles per workgroup
 // LDS window sizes (doubles). A chunk of 1024 tile-sorted particles at 2 x 2 per cell
-// covers one 16 x 16-cell tile: 19 x 19 cells for a 3-point stencil, 21 x 21 for the
+// covers one 32 x 8-cell tile: 35 x 11 cells for a 3-point stencil, 37 x 13 for the
 // 5-point explicit one; the windows leave room for particles that drifted since the sort
 // while keeping four workgroups per CU resident.
 constexpr int WIN_DEPOSIT = 3072;       // 4 planes x 768 cells, 24 KiB

@@ -142,7 +142,10 @@ struct Capture {
 
 // Tile sort of the plasma for deposition (bins.hip): scratch of the counting sort and the
 // permutation the deposition kernels walk.
-constexpr int DEPOSIT_TILE = 16;
+// Deposition tiles are 32 x 8 cells (256 cells, a 2 x 2-ppc chunk of 1024 particles): in
+// index order a tile's particles fill a wavefront's lanes one row of 32 cells at a time, so the
+// 32 lanes of a half-wavefront touch 32 consecutive LDS doubles.
+constexpr int DEPOSIT_TILE_X = 32, DEPOSIT_TILE_Y = 8;
 struct Bins {
     int32_t* keys = nullptr;
     long keys_cap = 0;
@@ -289,6 +292,7 @@ double rel_b_error(hpq_ctx* c, const double* store, int bx, int by, int bxi, int
 void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p);
 int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof);
 void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets);
+void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int32_t* perm, int32_t* offsets);
 const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p);
 void bins_free(hpq_ctx* c);
 void sort_keys(hpq_ctx* c, long n, int nbins, int32_t* perm, int32_t* offsets);

@@ -66,7 +66,7 @@ class SliceConfig:
     plasma_mass: float = 1.0
     max_qsa: float = 35.0
     neutralize: bool = True
-    # plasma deposition walks the particles sorted by 16x16-cell tiles, re-sorted every
+    # plasma deposition walks the particles sorted by 32x8-cell tiles, re-sorted every
     # bin_period slices (0: index order); only the order of floating-point sums depends on it
     bin_period: int = 4
     # "fp64": the drop-in path; "mixed": FP32 gather, push increments and deposition with

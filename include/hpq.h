@@ -68,7 +68,7 @@ typedef struct hpq_params {
     double plasma_charge, plasma_mass;   /* species charge and mass */
     double max_qsa;                      /* plasma.max_qsa_weighting_factor */
     int bin_period;                      /* plasma deposition walks the particles sorted by
-                                            16x16-cell tiles, re-sorted every bin_period
+                                            32x8-cell tiles, re-sorted every bin_period
                                             slices (plasmas.reorder_period plays this role
                                             in the reference); 0: index order. Changes only
                                             the order of floating-point sums. */
