@@ -228,7 +228,12 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
     constexpr int NPAR = HW * SH;                  // points of one parity
     constexpr int KP = (NPAR + NT - 1) / NT;       // points of one parity per thread
     constexpr int NLD = (LX * LY + NT - 1) / NT;   // start values loaded per thread
-    __shared__ double s[2][LY][LX];
+    // LDS holds the two colours of the tile apart: point (lx, ly) lives in
+    // s[c][(base + lx + ly) & 1][ly][lx >> 1], so a colour sweep and its neighbour reads walk
+    // consecutive doubles across the lanes (no two-way bank conflict from stride-2 rows)
+    constexpr int LXH = LX / 2;
+    static_assert(LX % 2 == 0, "a tile row holds whole point pairs");
+    __shared__ double s[2][2][LY][LXH];
     __shared__ double r[2][RH][RW];
     __shared__ double red[2][NT / 64];
     __shared__ double ra[ACF ? RH : 1][ACF ? RW : 1];
@@ -238,6 +243,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
     const int ti0 = L.vilo + int(blockIdx.x) * TX - HALO;
     const int tj0 = L.vjlo + int(blockIdx.y) * TY - HALO;
     const int base = (ti0 + tj0) & 1;
+    auto S = [&](int c, int ly, int lx) -> double& { return s[c][(base + lx + ly) & 1][ly][lx >> 1]; };
     const double facx = 1. / (L.dx * L.dx), facy = 1. / (L.dy * L.dy);
     // owned points: [parity][q]; pl = lx | ly << 8, or -1 outside the valid box / region
     double r0[2][KP], r1[2][KP], ci[2][KP];
@@ -322,8 +328,8 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                 a0 = interp_add<CC>(ti0 + lx, tj0 + ly, a0, c0);
                 a1 = interp_add<CC>(ti0 + lx, tj0 + ly, a1, c1);
             }
-            s[0][ly][lx] = a0;
-            s[1][ly][lx] = a1;
+            S(0, ly, lx) = a0;
+            S(1, ly, lx) = a1;
         }
     }
     double rmax = 0.0;
@@ -342,6 +348,8 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
     constexpr int KB = KP > 2 ? (KP + 1) / 2 : KP;
     auto sweep = [&](auto PARC) {
         constexpr int par = decltype(PARC)::value;
+        // a point of colour par has (base + lx + ly) & 1 == par; its neighbours the other one
+        constexpr int np = 1 - par;
 #pragma unroll
         for (int q0 = 0; q0 < KP; q0 += KB) {
             double w0[KB], e0[KB], s0[KB], n0[KB], w1[KB], e1[KB], s1[KB], n1[KB];
@@ -350,8 +358,9 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                 const int q = q0 + b;
                 if (q >= KP || pl[par][q] < 0) continue;
                 const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
-                w0[b] = s[0][ly][lx - 1]; e0[b] = s[0][ly][lx + 1]; s0[b] = s[0][ly - 1][lx]; n0[b] = s[0][ly + 1][lx];
-                w1[b] = s[1][ly][lx - 1]; e1[b] = s[1][ly][lx + 1]; s1[b] = s[1][ly - 1][lx]; n1[b] = s[1][ly + 1][lx];
+                const int xw = (lx - 1) >> 1, xe = (lx + 1) >> 1, xc = lx >> 1;
+                w0[b] = s[0][np][ly][xw]; e0[b] = s[0][np][ly][xe]; s0[b] = s[0][np][ly - 1][xc]; n0[b] = s[0][np][ly + 1][xc];
+                w1[b] = s[1][np][ly][xw]; e1[b] = s[1][np][ly][xe]; s1[b] = s[1][np][ly - 1][xc]; n1[b] = s[1][np][ly + 1][xc];
             }
             double u0[KB], u1[KB];
 #pragma unroll
@@ -370,8 +379,8 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                 const int q = q0 + b;
                 if (q >= KP || pl[par][q] < 0) continue;
                 const int lx = pl[par][q] & 255, ly = pl[par][q] >> 8;
-                s[0][ly][lx] = u0[b];
-                s[1][ly][lx] = u1[b];
+                s[0][par][ly][lx >> 1] = u0[b];
+                s[1][par][ly][lx >> 1] = u1[b];
             }
         }
         __syncthreads();
@@ -394,8 +403,8 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                 if (ring) {
                     const int i = ti0 + lx, j = tj0 + ly;
                     const double a = acf[at(L, i, j)];
-                    auto P0 = [&](int di, int dj) { return s[0][ly + dj][lx + di]; };
-                    auto P1 = [&](int di, int dj) { return s[1][ly + dj][lx + di]; };
+                    auto P0 = [&](int di, int dj) { return S(0, ly + dj, lx + di); };
+                    auto P1 = [&](int di, int dj) { return S(1, ly + dj, lx + di); };
                     const double g0 = residual1<CC>(L, i, j, P0, r0[par][q], a, facx, facy);
                     const double g1 = residual1<CC>(L, i, j, P1, r1[par][q], a, facx, facy);
                     if (out) m = fmax(m, fmax(fabs(g0), fabs(g1)));
@@ -406,8 +415,8 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
             }
             if (out) {
                 const long k = at(L, ti0 + lx, tj0 + ly);
-                phi_out[k] = s[0][ly][lx];
-                phi_out[k + L.cs] = s[1][ly][lx];
+                phi_out[k] = s[0][par][ly][lx >> 1];
+                phi_out[k + L.cs] = s[1][par][ly][lx >> 1];
             }
         }
     }

@@ -39,6 +39,16 @@ __device__ __forceinline__ double2 cmul(double2 x, double2 w) {
     return make_double2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
 }
 
+// LDS slot of FFT entry i: the low four bits (16 double2 = the 64 banks) are XORed with a
+// function of the higher bits, chosen so that the bit-reversed scatter of the load and every
+// butterfly stage of fft_lds touch 16 distinct bank groups per 16 lanes for M in [64, 4096]
+// (unswizzled, the scatter is a 16-way and the first two stage pairs 4-way conflicts). A
+// bijection on [0, M) for every power of two M.
+__device__ __forceinline__ int sw(int i) {
+    const int h = i >> 4;
+    return i ^ ((h * 5 ^ (h >> 4)) & 15);
+}
+
 // In-place radix-2 DIT FFT of the bit-reversed sequence a[0..M): the stages are taken in
 // pairs (a 4-point butterfly per thread and one barrier per pair); the arithmetic is that
 // of the radix-2 stages. tw[k] = exp(-2 pi i k / M).
@@ -52,15 +62,15 @@ __device__ void fft_lds(double2* a, int M, int logM, const double2* __restrict__
             const int i0 = ((b >> (s - 1)) << (s + 1)) + k;
             const int i1 = i0 + h, i2 = i0 + 2 * h, i3 = i0 + 3 * h;
             const double2 w1 = tw[k * st1], w2 = tw[k * st2], w3 = tw[(k + h) * st2];
-            const double2 a0 = a[i0], a1 = a[i1], a2 = a[i2], a3 = a[i3];
+            const double2 a0 = a[sw(i0)], a1 = a[sw(i1)], a2 = a[sw(i2)], a3 = a[sw(i3)];
             const double2 v1 = cmul(a1, w1), v3 = cmul(a3, w1);
             const double2 x0 = make_double2(a0.x + v1.x, a0.y + v1.y), x1 = make_double2(a0.x - v1.x, a0.y - v1.y);
             const double2 x2 = make_double2(a2.x + v3.x, a2.y + v3.y), x3 = make_double2(a2.x - v3.x, a2.y - v3.y);
             const double2 u2 = cmul(x2, w2), u3 = cmul(x3, w3);
-            a[i0] = make_double2(x0.x + u2.x, x0.y + u2.y);
-            a[i2] = make_double2(x0.x - u2.x, x0.y - u2.y);
-            a[i1] = make_double2(x1.x + u3.x, x1.y + u3.y);
-            a[i3] = make_double2(x1.x - u3.x, x1.y - u3.y);
+            a[sw(i0)] = make_double2(x0.x + u2.x, x0.y + u2.y);
+            a[sw(i2)] = make_double2(x0.x - u2.x, x0.y - u2.y);
+            a[sw(i1)] = make_double2(x1.x + u3.x, x1.y + u3.y);
+            a[sw(i3)] = make_double2(x1.x - u3.x, x1.y - u3.y);
         }
         __syncthreads();
     }
@@ -71,10 +81,10 @@ __device__ void fft_lds(double2* a, int M, int logM, const double2* __restrict__
             const int k = b & (half - 1);
             const int i0 = ((b >> (s - 1)) << s) + k;
             const int i1 = i0 + half;
-            const double2 u = a[i0];
-            const double2 v = cmul(a[i1], tw[k * step]);
-            a[i0] = make_double2(u.x + v.x, u.y + v.y);
-            a[i1] = make_double2(u.x - v.x, u.y - v.y);
+            const double2 u = a[sw(i0)];
+            const double2 v = cmul(a[sw(i1)], tw[k * step]);
+            a[sw(i0)] = make_double2(u.x + v.x, u.y + v.y);
+            a[sw(i1)] = make_double2(u.x - v.x, u.y - v.y);
         }
         __syncthreads();
     }
@@ -93,7 +103,7 @@ __device__ __forceinline__ void load_odd_extension(double2* a, int n, int M, int
             const double2 y = v(M - m);
             x = make_double2(-y.x, -y.y);
         }
-        a[__brev(unsigned(m)) >> (32 - logM)] = x;
+        a[sw(__brev(unsigned(m)) >> (32 - logM))] = x;
     }
     __syncthreads();
 }
@@ -128,7 +138,7 @@ __global__ __launch_bounds__(DST_T) void k_dst_rows_fft(Rows in, Rows out, int n
             if (m < M) {
                 const int src = (m >= 1 && m <= n) ? m : (m > N ? M - m : 0);
                 if (src) {
-                    const double2 F = a[src];
+                    const double2 F = a[sw(src)];
                     double o0 = -F.y, o1 = F.x;
                     if (s0) o0 *= s0[src - 1];
                     if (s1) o1 *= s1[src - 1];
@@ -141,7 +151,7 @@ __global__ __launch_bounds__(DST_T) void k_dst_rows_fft(Rows in, Rows out, int n
 #pragma unroll
         for (int q = 0; q < MAXQ; ++q) {
             const int m = threadIdx.x + q * DST_T;
-            if (m < M) a[__brev(unsigned(m)) >> (32 - logM)] = nv[q];
+            if (m < M) a[sw(__brev(unsigned(m)) >> (32 - logM))] = nv[q];
         }
         __syncthreads();
         fft_lds(a, M, logM, tw);
@@ -149,8 +159,11 @@ __global__ __launch_bounds__(DST_T) void k_dst_rows_fft(Rows in, Rows out, int n
     }
     double* q0 = row_ptr(out, r0);
     double* q1 = has1 ? row_ptr(out, r1) : nullptr;
-    for (int k = threadIdx.x; k < n; k += DST_T) {
-        const double2 F = a[k + 1];
+    // entries m = k + 1 in [1, n], lanes on aligned runs of m
+    for (int m = threadIdx.x; m <= n; m += DST_T) {
+        if (m == 0) continue;
+        const int k = m - 1;
+        const double2 F = a[sw(m)];
         double o0 = -F.y, o1 = F.x;
         if (s0) o0 *= s0[k];
         if (s1) o1 *= s1[k];
