@@ -18,9 +18,10 @@
 // the values a global sweep gives them) and writes back the interior; the residual, its
 // restriction to the next level and its max norm come out of the same launch.
 // Prolongation is fused into the load of the up-sweep. On level 0 the up-sweep and the
-// final smoothing of the cycle are one launch of eight sweeps; its last workgroup to
-// finish evaluates convergence and publishes it to pinned host memory, which the host
-// polls instead of synchronising the stream. Levels from the first one of at most 33x33
+// final smoothing of the cycle are one launch of eight sweeps that leaves every
+// workgroup's residual max norm; a one-workgroup launch after it reduces them, evaluates
+// convergence and publishes the result to pinned host memory, which the host polls
+// instead of synchronising the stream. Levels from the first one of at most 33x33
 // points down run as one workgroup that does the coarse part of the V-cycle with barriers
 // between sweeps, its arrays in LDS; the levels of at most 64 points (and the bottom
 // solve) run in its first wavefront alone, without workgroup barriers. For odd n the
@@ -130,8 +131,8 @@ __device__ void publish(const MGCtl* c, MGMirror* m, int seq) {
     __hip_atomic_store(&m->word, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// end of one V-cycle (solve_doit :1371-1417), run by the last workgroup of the level-0
-// launch with the max norm of the cycle's residual
+// end of one V-cycle (solve_doit :1371-1417), run by k_check with the max norm of the
+// cycle's residual
 __device__ void end_cycle(MGCtl* ctl, MGCtl* other, MGMirror* mirror, int seq, int maxiter, double norminf) {
     ctl->iters += 1;
     if (norminf <= ctl->res_target) {
@@ -975,7 +976,7 @@ int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, lon
     // average_down_acoef (:1641-1661): level 1 in the initial smoothing, levels 2..cb in the
     // first V-cycle's down-sweep, the coarse ones inside k_coarse
     // initial smoothing from the guess (buf[1]) into buf[0], the iterate after zero cycles;
-    // its last workgroup sets the convergence target
+    // k_target then sets the convergence target from its residual and rhs max norms
     smooth<4, false, true, false, true, true, true>(c, S.L0, L1, S.buf[0], S.rhs0, S.rcs, S.L0.acf, S.buf[1], nullptr,
                                                     L1.res, nullptr, M.d_norms);
     const Check chk0{S.ctl, S.other, M.d_mirror, M.d_norms, level0_blocks(S.L0), 0, maxiter, tol_rel, tol_abs};

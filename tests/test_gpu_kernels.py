@@ -132,7 +132,7 @@ def test_push_boundary_conditions(bc):
         assert np.array_equal(pg[k], po[k]), k
 
 
-@pytest.mark.parametrize("n", [63, 64, 48, 255])
+@pytest.mark.parametrize("n", [63, 64, 48, 255, 511, 1023])
 def test_poisson_psi_ez_bz(n):
     cfg = small_config(n=n)
     sim, orc = _pair(cfg, n_particles=0)
@@ -161,7 +161,7 @@ MG_RUNS = [(1e-4, 200), (1e-7, 200), (1e-10, 200), (1.0, 200), (1e-300, 1), (1e-
 ORACLE_RC = {0: 0, 1: 3, 2: 4}
 
 
-@pytest.mark.parametrize("n", [63, 64, 127, 32])
+@pytest.mark.parametrize("n", [63, 64, 127, 32, 255, 511])
 def test_mg_solve1_bit_exact(n):
     """Consecutive solves on one context (each starts from the previous solution) against
     the oracle: same return code, same V-cycle count, identical planes."""
