@@ -404,8 +404,13 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                 if (ring) {
                     const int i = ti0 + lx, j = tj0 + ly;
                     const double a = acf[at(L, i, j)];
-                    auto P0 = [&](int di, int dj) { return S(0, ly + dj, lx + di); };
-                    auto P1 = [&](int di, int dj) { return S(1, ly + dj, lx + di); };
+                    // the point has colour par, its four neighbours the other one
+                    auto P0 = [&](int di, int dj) {
+                        return (di | dj) ? s[0][1 - par][ly + dj][(lx + di) >> 1] : s[0][par][ly][lx >> 1];
+                    };
+                    auto P1 = [&](int di, int dj) {
+                        return (di | dj) ? s[1][1 - par][ly + dj][(lx + di) >> 1] : s[1][par][ly][lx >> 1];
+                    };
                     const double g0 = residual1<CC>(L, i, j, P0, r0[par][q], a, facx, facy);
                     const double g1 = residual1<CC>(L, i, j, P1, r1[par][q], a, facx, facy);
                     if (out) m = fmax(m, fmax(fabs(g0), fabs(g1)));
