@@ -76,64 +76,80 @@ __global__ __launch_bounds__(VT) void k_deposit_vjp(Geom g, hpq_plasma p, hpq_pl
 
 // ---- ::ExplicitDeposition (ExplicitDeposition.cpp:166-262) --------------------------------
 // particle inputs seeded (x, y, ux, uy, psi); the Sy, Sx adjoints are read from their
-// planes, the adjoints of Bz, Ez, ExmBy, EypBx are added to theirs
+// planes, the adjoints of Bz, Ez, ExmBy, EypBx are added to theirs.
+// The per-particle coefficients depend on (ux, uy, psi) only and the shape factors on x (or
+// y) only, so they carry three and one tangent directions instead of five: each product of a
+// coefficient with a shape-factor product is expanded by hand into its value, its three
+// momentum parts and its x and y parts. The terms dropped are the exact zeros a five-direction
+// product would add, so the sums (taken in the same order) are those of the full tangents.
 template <int O, int TD>
 __global__ __launch_bounds__(VT) void k_explicit_vjp(Geom g, hpq_plasma p, hpq_plasma_adj a, const double* __restrict__ store,
                                                      double* __restrict__ adj, Layout L, double charge_invvol_mu0,
                                                      double charge_mass_ratio) {
-    using T = Tan<5>;
+    using U = Tan<3>;   // directions ux, uy, psi
+    using X1 = Tan<1>;  // direction x (or y)
     constexpr int S = dshape_width<TD, O>();
     const long ip = long(blockIdx.x) * VT + threadIdx.x;
     if (ip >= p.n || !p.valid[ip]) return;
     const double clight = 1.0, clight_inv = 1.0 / 1.0;
-    const T x = T::seed(p.x[ip], 0), y = T::seed(p.y[ip], 1);
-    const T ux = T::seed(p.ux[ip], 2), uy = T::seed(p.uy[ip], 3), psi = T::seed(p.psi[ip], 4);
-    const T pinv = 1.0 / psi;
-    const T pvx = ux * pinv * clight_inv, pvy = uy * pinv * clight_inv;
+    const U ux = U::seed(p.ux[ip], 0), uy = U::seed(p.uy[ip], 1), psi = U::seed(p.psi[ip], 2);
+    const U pinv = 1.0 / psi;
+    const U pvx = ux * pinv * clight_inv, pvy = uy * pinv * clight_inv;
     const double cdm = charge_invvol_mu0 * p.w[ip];
-    const T xmid = (x - g.xoff) * g.dxi, ymid = (y - g.yoff) * g.dyi;
+    const X1 xmid = (X1::seed(p.x[ip], 0) - g.xoff) * g.dxi, ymid = (X1::seed(p.y[ip], 0) - g.yoff) * g.dyi;
     const double Aabssqp = 0.0;
-    const T gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * pinv * pinv + pvx * pvx + pvy * pvy + 1.0);
-    const T vxvy = pvx * pvy;
-    const T gpx = gamma_psi - pvx * pvx, gpy = gamma_psi - pvy * pvy;
-    const T qp = charge_mass_ratio * pinv;
-    const T ay = -cdm * qp, by = cdm * clight * g.dxi * vxvy, cy = -cdm * clight * g.dyi * (gpy - 1.0);
-    const T ax = cdm * qp, bx = cdm * clight * g.dxi * (gpx - 1.0), cx = -cdm * clight * g.dyi * vxvy;
+    const U gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * pinv * pinv + pvx * pvx + pvy * pvy + 1.0);
+    const U vxvy = pvx * pvy;
+    const U gpx = gamma_psi - pvx * pvx, gpy = gamma_psi - pvy * pvy;
+    const U qp = charge_mass_ratio * pinv;
+    const U ay = -cdm * qp, by = cdm * clight * g.dxi * vxvy, cy = -cdm * clight * g.dyi * (gpy - 1.0);
+    const U ax = cdm * qp, bx = cdm * clight * g.dxi * (gpx - 1.0), cx = -cdm * clight * g.dyi * vxvy;
     const double* Bz = store + long(L.Bz) * g.plane;
     const double* Ez = store + long(L.Ez) * g.plane;
     const double* ExmBy = store + long(L.ExmBy) * g.plane;
     const double* EypBx = store + long(L.EypBx) * g.plane;
     const double* aSy = adj + long(L.Sy) * g.plane;
     const double* aSx = adj + long(L.Sx) * g.plane;
-    T acc(0.0);
+    // accumulated derivative parts: x, y, then ux, uy, psi
+    double acc_x = 0.0, acc_y = 0.0, acc_u[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int iy = 0; iy < S; ++iy) {
-        const DsfR<T> Y = dshape<TD, O, T, T>(ymid, iy);
+        const DsfR<X1> Y = dshape<TD, O, X1, X1>(ymid, iy);
 #pragma unroll
         for (int ix = 0; ix < S; ++ix) {
             if (TD == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) continue;
-            const DsfR<T> X = dshape<TD, O, T, T>(xmid, ix);
+            const DsfR<X1> X = dshape<TD, O, X1, X1>(xmid, ix);
             if (!in_store(g, X.cell, Y.cell)) continue;
             const long k = g.idx(X.cell, Y.cell);
             const double sy_a = aSy[k], sx_a = aSx[k];
             if (sy_a == 0.0 && sx_a == 0.0) continue;
-            const T A = -Bz[k] * pvx + (EypBx[k] * gpy - ExmBy[k] * vxvy + Ez[k] * pvy) * clight_inv;
-            const T B = Bz[k] * pvy + (-EypBx[k] * vxvy + ExmBy[k] * gpx + Ez[k] * pvx) * clight_inv;
-            const T ss = X.s * Y.s, ds = X.ds * Y.s, sd = X.s * Y.ds;
-            acc += sy_a * (ay * A * ss + by * ds + cy * sd) + sx_a * (ax * B * ss + bx * ds + cx * sd);
+            const U A = -Bz[k] * pvx + (EypBx[k] * gpy - ExmBy[k] * vxvy + Ez[k] * pvy) * clight_inv;
+            const U B = Bz[k] * pvy + (-EypBx[k] * vxvy + ExmBy[k] * gpx + Ez[k] * pvx) * clight_inv;
+            // ss = X.s Y.s, ds = X.ds Y.s, sd = X.s Y.ds: values and x, y parts
+            const double ss = X.s.v * Y.s.v, ds = X.ds.v * Y.s.v, sd = X.s.v * Y.ds.v;
+            const double ss_x = X.s.d[0] * Y.s.v, ds_x = X.ds.d[0] * Y.s.v, sd_x = X.s.d[0] * Y.ds.v;
+            const double ss_y = X.s.v * Y.s.d[0], ds_y = X.ds.v * Y.s.d[0], sd_y = X.s.v * Y.ds.d[0];
+            const U ayA = ay * A, axB = ax * B;
+            // sy_a (ayA ss + by ds + cy sd) + sx_a (axB ss + bx ds + cx sd), part by part
+            acc_x += sy_a * (ayA.v * ss_x + by.v * ds_x + cy.v * sd_x) + sx_a * (axB.v * ss_x + bx.v * ds_x + cx.v * sd_x);
+            acc_y += sy_a * (ayA.v * ss_y + by.v * ds_y + cy.v * sd_y) + sx_a * (axB.v * ss_y + bx.v * ds_y + cx.v * sd_y);
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                acc_u[q] += sy_a * (ayA.d[q] * ss + by.d[q] * ds + cy.d[q] * sd) +
+                            sx_a * (axB.d[q] * ss + bx.d[q] * ds + cx.d[q] * sd);
             // the contributions are linear in the four fields of cell k
-            const double wy = sy_a * ay.v * ss.v, wx = sx_a * ax.v * ss.v;
+            const double wy = sy_a * ay.v * ss, wx = sx_a * ax.v * ss;
             atomic_add(adj + long(L.Bz) * g.plane + k, -wy * pvx.v + wx * pvy.v);
             atomic_add(adj + long(L.Ez) * g.plane + k, (wy * pvy.v + wx * pvx.v) * clight_inv);
             atomic_add(adj + long(L.ExmBy) * g.plane + k, (-wy * vxvy.v + wx * gpx.v) * clight_inv);
             atomic_add(adj + long(L.EypBx) * g.plane + k, (wy * gpy.v - wx * vxvy.v) * clight_inv);
         }
     }
-    a.x[ip] += acc.d[0];
-    a.y[ip] += acc.d[1];
-    a.ux[ip] += acc.d[2];
-    a.uy[ip] += acc.d[3];
-    a.psi[ip] += acc.d[4];
+    a.x[ip] += acc_x;
+    a.y[ip] += acc_y;
+    a.ux[ip] += acc_u[0];
+    a.uy[ip] += acc_u[1];
+    a.psi[ip] += acc_u[2];
 }
 
 // ---- AdvancePlasmaParticles (PlasmaParticleAdvance.cpp:84-240), not a temporary slice -----
