@@ -318,7 +318,10 @@ template <int O>
 __global__ __launch_bounds__(VT) void k_push_vjp_gather(Geom g, hpq_plasma p, const double* __restrict__ store,
                                                         Layout L, const double* __restrict__ gadj, hpq_plasma_adj in,
                                                         double* __restrict__ adj) {
-    using T = Tan<2>;
+    // the shape factors in x carry the x tangent only, those in y the y tangent only; the
+    // products below are expanded by hand into value, x and y parts (the dropped terms are the
+    // exact zeros of two-direction products, the sums keep their order)
+    using X1 = Tan<1>;
     const long ip = long(blockIdx.x) * VT + threadIdx.x;
     if (ip >= p.n || !p.valid[ip]) return;
     double lg[6];
@@ -330,37 +333,45 @@ __global__ __launch_bounds__(VT) void k_push_vjp_gather(Geom g, hpq_plasma p, co
     }
     if (!any) return;
     const double clight = 1.0;
-    const T xm = (T::seed(p.x_prev[ip], 0) - g.xoff) * g.dxi, ym = (T::seed(p.y_prev[ip], 1) - g.yoff) * g.dyi;
+    const X1 xm = (X1::seed(p.x_prev[ip], 0) - g.xoff) * g.dxi, ym = (X1::seed(p.y_prev[ip], 0) - g.yoff) * g.dyi;
     constexpr int W = O + 2;
-    DsfR<T> sx[W], sy[W];
+    DsfR<X1> sx[W], sy[W];
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        sx[k] = dshape<1, O, T, T>(xm, k);
-        sy[k] = dshape<1, O, T, T>(ym, k);
+        sx[k] = dshape<1, O, X1, X1>(xm, k);
+        sy[k] = dshape<1, O, X1, X1>(ym, k);
     }
     const double* F[5] = {store + long(L.Psi) * g.plane, store + long(L.Ez) * g.plane, store + long(L.Bx) * g.plane,
                           store + long(L.By) * g.plane, store + long(L.Bz) * g.plane};
-    T acc(0.0);
+    double acc_x = 0.0, acc_y = 0.0;
 #pragma unroll
     for (int iy = 0; iy < W; ++iy)
 #pragma unroll
         for (int ix = 0; ix < W; ++ix) {
-            const DsfR<T>& X = sx[ix];
-            const DsfR<T>& Y = sy[iy];
+            const DsfR<X1>& X = sx[ix];
+            const DsfR<X1>& Y = sy[iy];
             if (!in_store(g, X.cell, Y.cell)) continue;
             const long k = g.idx(X.cell, Y.cell);
-            const T ss = X.s * Y.s;
-            const T wpsi = lg[0] * (X.ds * Y.s) * g.dxi + lg[1] * (X.s * Y.ds) * g.dyi;
-            const T wez = lg[2] * ss, wbx = lg[3] * clight * ss, wby = lg[4] * clight * ss, wbz = lg[5] * ss;
-            acc += wpsi * F[0][k] + wez * F[1][k] + wbx * F[2][k] + wby * F[3][k] + wbz * F[4][k];
-            if (wpsi.v != 0.0) atomic_add(adj + long(L.Psi) * g.plane + k, wpsi.v);
-            if (wez.v != 0.0) atomic_add(adj + long(L.Ez) * g.plane + k, wez.v);
-            if (wbx.v != 0.0) atomic_add(adj + long(L.Bx) * g.plane + k, wbx.v);
-            if (wby.v != 0.0) atomic_add(adj + long(L.By) * g.plane + k, wby.v);
-            if (wbz.v != 0.0) atomic_add(adj + long(L.Bz) * g.plane + k, wbz.v);
+            const double ss = X.s.v * Y.s.v, ss_x = X.s.d[0] * Y.s.v, ss_y = X.s.v * Y.s.d[0];
+            const double ds = X.ds.v * Y.s.v, ds_x = X.ds.d[0] * Y.s.v, ds_y = X.ds.v * Y.s.d[0];
+            const double sd = X.s.v * Y.ds.v, sd_x = X.s.d[0] * Y.ds.v, sd_y = X.s.v * Y.ds.d[0];
+            // wpsi = lg0 (X.ds Y.s) dxi + lg1 (X.s Y.ds) dyi; wez, wbx, wby, wbz = lg * ss
+            const double wpsi = lg[0] * ds * g.dxi + lg[1] * sd * g.dyi;
+            const double wpsi_x = lg[0] * ds_x * g.dxi + lg[1] * sd_x * g.dyi;
+            const double wpsi_y = lg[0] * ds_y * g.dxi + lg[1] * sd_y * g.dyi;
+            const double c2 = lg[2], c3 = lg[3] * clight, c4 = lg[4] * clight, c5 = lg[5];
+            const double f0 = F[0][k], f1 = F[1][k], f2 = F[2][k], f3 = F[3][k], f4 = F[4][k];
+            acc_x += wpsi_x * f0 + (c2 * ss_x) * f1 + (c3 * ss_x) * f2 + (c4 * ss_x) * f3 + (c5 * ss_x) * f4;
+            acc_y += wpsi_y * f0 + (c2 * ss_y) * f1 + (c3 * ss_y) * f2 + (c4 * ss_y) * f3 + (c5 * ss_y) * f4;
+            const double wez = c2 * ss, wbx = c3 * ss, wby = c4 * ss, wbz = c5 * ss;
+            if (wpsi != 0.0) atomic_add(adj + long(L.Psi) * g.plane + k, wpsi);
+            if (wez != 0.0) atomic_add(adj + long(L.Ez) * g.plane + k, wez);
+            if (wbx != 0.0) atomic_add(adj + long(L.Bx) * g.plane + k, wbx);
+            if (wby != 0.0) atomic_add(adj + long(L.By) * g.plane + k, wby);
+            if (wbz != 0.0) atomic_add(adj + long(L.Bz) * g.plane + k, wbz);
         }
-    in.x[ip] += acc.d[0];
-    in.y[ip] += acc.d[1];
+    in.x[ip] += acc_x;
+    in.y[ip] += acc_y;
 }
 
 }  // namespace
