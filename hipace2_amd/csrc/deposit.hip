@@ -160,8 +160,28 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
     int slot[6];
     const int ns = assign_slots(comp, slot);
 
-    R q_invvol[DPPT], vx_c[DPPT], vy_c[DPPT], gamma_psi[DPPT], psi_inv[DPPT];
-    double xmid[DPPT], ymid[DPPT];
+    // the per-particle quantities, computed again from the particle arrays (cache hits) when
+    // the particle's turn comes: only the start cells stay in registers across the window choice
+    struct PQ {
+        R q_invvol, vx_c, vy_c, gamma_psi, psi_inv;
+        double xmid, ymid;
+    };
+    auto particle = [&](long ip) {
+        PQ v;
+        v.psi_inv = R(1.0) / R(p.psi[ip]);
+        const double xp = p.x[ip], yp = p.y[ip];
+        v.vx_c = R(p.ux[ip]) * v.psi_inv;
+        v.vy_c = R(p.uy[ip]) * v.psi_inv;
+        v.q_invvol = R(charge_invvol) * R(p.w[ip]);
+        v.xmid = (xp - g.xoff) * g.dxi;
+        v.ymid = (yp - g.yoff) * g.dyi;
+        const R Aabssqp = 0.0;
+        v.gamma_psi = R(0.5) * ((R(1.0) + R(0.5) * Aabssqp) * v.psi_inv * v.psi_inv +
+                                v.vx_c * v.vx_c * clightinv * clightinv + v.vy_c * v.vy_c * clightinv * clightinv +
+                                R(1.0));
+        return v;
+    };
+    long ipq[DPPT];
     int ci[DPPT], cj[DPPT];
     bool act[DPPT];
     const long base = long(blockIdx.x) * DCHUNK;
@@ -169,29 +189,21 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
     for (int q = 0; q < DPPT; ++q) {
         const long t = base + q * DT + threadIdx.x;
         act[q] = false;
+        ipq[q] = 0;
         if (t >= p.n) continue;
         const long ip = order ? long(order[t]) : t;
         if (!p.valid[ip]) continue;
-        psi_inv[q] = R(1.0) / R(p.psi[ip]);
-        const double xp = p.x[ip], yp = p.y[ip];
-        vx_c[q] = R(p.ux[ip]) * psi_inv[q];
-        vy_c[q] = R(p.uy[ip]) * psi_inv[q];
-        q_invvol[q] = R(charge_invvol) * R(p.w[ip]);
-        xmid[q] = (xp - g.xoff) * g.dxi;
-        ymid[q] = (yp - g.yoff) * g.dyi;
-        const R Aabssqp = 0.0;
-        gamma_psi[q] = R(0.5) * ((R(1.0) + R(0.5) * Aabssqp) * psi_inv[q] * psi_inv[q] +
-                                 vx_c[q] * vx_c[q] * clightinv * clightinv +
-                                 vy_c[q] * vy_c[q] * clightinv * clightinv + R(1.0));
-        if (gamma_psi[q] < R(0.0) || gamma_psi[q] > R(max_qsa) || psi_inv[q] < R(0.0)) {
+        const PQ v = particle(ip);
+        if (v.gamma_psi < R(0.0) || v.gamma_psi > R(max_qsa) || v.psi_inv < R(0.0)) {
             atomicAdd(qsa_count, 1);
             p.w[ip] = 0.0;
             p.valid[ip] = 0;
             continue;
         }
         act[q] = true;
-        ci[q] = shape<O, R>(xmid[q], 0).cell;
-        cj[q] = shape<O, R>(ymid[q], 0).cell;
+        ipq[q] = ip;
+        ci[q] = shape<O, R>(v.xmid, 0).cell;
+        cj[q] = shape<O, R>(v.ymid, 0).cell;
     }
     const Win W = choose_window(ci, cj, act, S, ns > 0 ? WIN_DEPOSIT / ns : 0, red);
     const int cells = W.w * W.h;
@@ -200,11 +212,12 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
 #pragma unroll
     for (int q = 0; q < DPPT; ++q) {
         if (!act[q]) continue;
+        const PQ pq = particle(ipq[q]);
         SfR<R> sx[S], sy[S];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
-            sx[k] = shape<O, R>(xmid[q], k);
-            sy[k] = shape<O, R>(ymid[q], k);
+            sx[k] = shape<O, R>(pq.xmid, k);
+            sy[k] = shape<O, R>(pq.ymid, k);
         }
         const bool local = W.holds(ci[q], cj[q], S);
         const R cmmr = R(charge_mu0_mass_ratio);
@@ -215,9 +228,9 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
             for (int iy = 0; iy < S; ++iy) {
 #pragma unroll
                 for (int ix = 0; ix < S; ++ix) {
-                    const R cd = q_invvol[q] * sx[ix].s * sy[iy].s;
-                    const R v[6] = {cd * vx_c[q], cd * vy_c[q], cd * (gamma_psi[q] - R(1.0)) * clight,
-                                    cd * gamma_psi[q], cd * cmmr * psi_inv[q], cd};
+                    const R cd = pq.q_invvol * sx[ix].s * sy[iy].s;
+                    const R v[6] = {cd * pq.vx_c, cd * pq.vy_c, cd * (pq.gamma_psi - R(1.0)) * clight,
+                                    cd * pq.gamma_psi, cd * cmmr * pq.psi_inv, cd};
                     const int k = k0 + iy * W.w + ix;
 #pragma unroll
                     for (int a = 0; a < 6; ++a)
@@ -232,9 +245,9 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
             for (int ix = 0; ix < S; ++ix) {
                 const int i = sx[ix].cell, j = sy[iy].cell;
                 if (!in_store(g, i, j)) continue;
-                const R cd = q_invvol[q] * sx[ix].s * sy[iy].s;
-                const R v[6] = {cd * vx_c[q], cd * vy_c[q], cd * (gamma_psi[q] - R(1.0)) * clight,
-                                cd * gamma_psi[q], cd * cmmr * psi_inv[q], cd};
+                const R cd = pq.q_invvol * sx[ix].s * sy[iy].s;
+                const R v[6] = {cd * pq.vx_c, cd * pq.vy_c, cd * (pq.gamma_psi - R(1.0)) * clight,
+                                cd * pq.gamma_psi, cd * cmmr * pq.psi_inv, cd};
                 if (local) {
                     const int k = W.at(i, j);
 #pragma unroll
@@ -264,8 +277,10 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
     __shared__ double win[WIN_EXPLICIT];
     __shared__ int red[7 * DT / 64];
     const R clight = 1.0, clight_inv = R(1.0) / R(1.0);
-    R vx[DPPT], vy[DPPT], psi_inv[DPPT], cdm_[DPPT];
-    double xmid[DPPT], ymid[DPPT];
+    // Only the start cells of the DPPT particles stay in registers across the window choice;
+    // each particle's momenta, weight and position are read again (cache hits) when its turn
+    // comes, which keeps the FP64 kernel within 128 VGPRs (four waves per SIMD).
+    long ipq[DPPT];
     int ci[DPPT], cj[DPPT];
     bool act[DPPT];
     const long base = long(blockIdx.x) * DCHUNK;
@@ -273,19 +288,14 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
     for (int q = 0; q < DPPT; ++q) {
         const long t = base + q * DT + threadIdx.x;
         act[q] = false;
+        ipq[q] = 0;
         if (t >= p.n) continue;
         const long ip = order ? long(order[t]) : t;
         if (!p.valid[ip]) continue;
         act[q] = true;
-        psi_inv[q] = R(1.0) / R(p.psi[ip]);
-        const double xp = p.x[ip], yp = p.y[ip];
-        vx[q] = R(p.ux[ip]) * psi_inv[q] * clight_inv;
-        vy[q] = R(p.uy[ip]) * psi_inv[q] * clight_inv;
-        cdm_[q] = R(charge_invvol_mu0) * R(p.w[ip]);
-        xmid[q] = (xp - g.xoff) * g.dxi;
-        ymid[q] = (yp - g.yoff) * g.dyi;
-        ci[q] = dshape<T, O, R>(xmid[q], 0).cell;
-        cj[q] = dshape<T, O, R>(ymid[q], 0).cell;
+        ipq[q] = ip;
+        ci[q] = dshape<T, O, R>((p.x[ip] - g.xoff) * g.dxi, 0).cell;
+        cj[q] = dshape<T, O, R>((p.y[ip] - g.yoff) * g.dyi, 0).cell;
     }
     // window planes: Sy, Sx (accumulated), then copies of Bz, Ez, ExmBy, EypBx
     const Win W = choose_window(ci, cj, act, S, WIN_EXPLICIT / 6, red);
@@ -318,9 +328,12 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
         // with A = -Bz vx + (Ez vy - ExmBy vx vy + EypBx (gamma/psi - vy^2)) / c and
         // B = Bz vy + (Ez vx + ExmBy (gamma/psi - vx^2) - EypBx vx vy) / c (no laser term), is
         // regrouped into per-particle coefficients times three shape-factor products.
+        const long ip = ipq[q];
+        const R pinv = R(1.0) / R(p.psi[ip]);
+        const R pvx = R(p.ux[ip]) * pinv * clight_inv, pvy = R(p.uy[ip]) * pinv * clight_inv;
+        const R cdm = R(charge_invvol_mu0) * R(p.w[ip]);
+        const double xmid_q = (p.x[ip] - g.xoff) * g.dxi, ymid_q = (p.y[ip] - g.yoff) * g.dyi;
         const R q_mass_ratio = R(charge_mass_ratio);
-        const R cdm = cdm_[q];
-        const R pvx = vx[q], pvy = vy[q], pinv = psi_inv[q];
         const R Aabssqp = 0.0;
         const R gamma_psi =
             R(0.5) * ((R(1.0) + R(0.5) * Aabssqp) * pinv * pinv + pvx * pvx + pvy * pvy + R(1.0));
@@ -333,8 +346,8 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
         DsfR<R> sx[S], sy[S];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
-            sx[k] = dshape<T, O, R>(xmid[q], k);
-            sy[k] = dshape<T, O, R>(ymid[q], k);
+            sx[k] = dshape<T, O, R>(xmid_q, k);
+            sy[k] = dshape<T, O, R>(ymid_q, k);
         }
         const bool local = W.holds(ci[q], cj[q], S);
         // the contribution of cell (ix, iy) from the four field values there
