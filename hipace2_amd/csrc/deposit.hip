@@ -160,28 +160,8 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
     int slot[6];
     const int ns = assign_slots(comp, slot);
 
-    // the per-particle quantities, computed again from the particle arrays (cache hits) when
-    // the particle's turn comes: only the start cells stay in registers across the window choice
-    struct PQ {
-        R q_invvol, vx_c, vy_c, gamma_psi, psi_inv;
-        double xmid, ymid;
-    };
-    auto particle = [&](long ip) {
-        PQ v;
-        v.psi_inv = R(1.0) / R(p.psi[ip]);
-        const double xp = p.x[ip], yp = p.y[ip];
-        v.vx_c = R(p.ux[ip]) * v.psi_inv;
-        v.vy_c = R(p.uy[ip]) * v.psi_inv;
-        v.q_invvol = R(charge_invvol) * R(p.w[ip]);
-        v.xmid = (xp - g.xoff) * g.dxi;
-        v.ymid = (yp - g.yoff) * g.dyi;
-        const R Aabssqp = 0.0;
-        v.gamma_psi = R(0.5) * ((R(1.0) + R(0.5) * Aabssqp) * v.psi_inv * v.psi_inv +
-                                v.vx_c * v.vx_c * clightinv * clightinv + v.vy_c * v.vy_c * clightinv * clightinv +
-                                R(1.0));
-        return v;
-    };
-    long ipq[DPPT];
+    R q_invvol[DPPT], vx_c[DPPT], vy_c[DPPT], gamma_psi[DPPT], psi_inv[DPPT];
+    double xmid[DPPT], ymid[DPPT];
     int ci[DPPT], cj[DPPT];
     bool act[DPPT];
     const long base = long(blockIdx.x) * DCHUNK;
@@ -189,21 +169,29 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
     for (int q = 0; q < DPPT; ++q) {
         const long t = base + q * DT + threadIdx.x;
         act[q] = false;
-        ipq[q] = 0;
         if (t >= p.n) continue;
         const long ip = order ? long(order[t]) : t;
         if (!p.valid[ip]) continue;
-        const PQ v = particle(ip);
-        if (v.gamma_psi < R(0.0) || v.gamma_psi > R(max_qsa) || v.psi_inv < R(0.0)) {
+        psi_inv[q] = R(1.0) / R(p.psi[ip]);
+        const double xp = p.x[ip], yp = p.y[ip];
+        vx_c[q] = R(p.ux[ip]) * psi_inv[q];
+        vy_c[q] = R(p.uy[ip]) * psi_inv[q];
+        q_invvol[q] = R(charge_invvol) * R(p.w[ip]);
+        xmid[q] = (xp - g.xoff) * g.dxi;
+        ymid[q] = (yp - g.yoff) * g.dyi;
+        const R Aabssqp = 0.0;
+        gamma_psi[q] = R(0.5) * ((R(1.0) + R(0.5) * Aabssqp) * psi_inv[q] * psi_inv[q] +
+                                 vx_c[q] * vx_c[q] * clightinv * clightinv +
+                                 vy_c[q] * vy_c[q] * clightinv * clightinv + R(1.0));
+        if (gamma_psi[q] < R(0.0) || gamma_psi[q] > R(max_qsa) || psi_inv[q] < R(0.0)) {
             atomicAdd(qsa_count, 1);
             p.w[ip] = 0.0;
             p.valid[ip] = 0;
             continue;
         }
         act[q] = true;
-        ipq[q] = ip;
-        ci[q] = shape<O, R>(v.xmid, 0).cell;
-        cj[q] = shape<O, R>(v.ymid, 0).cell;
+        ci[q] = shape<O, R>(xmid[q], 0).cell;
+        cj[q] = shape<O, R>(ymid[q], 0).cell;
     }
     const Win W = choose_window(ci, cj, act, S, ns > 0 ? WIN_DEPOSIT / ns : 0, red);
     const int cells = W.w * W.h;
@@ -212,12 +200,11 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
 #pragma unroll
     for (int q = 0; q < DPPT; ++q) {
         if (!act[q]) continue;
-        const PQ pq = particle(ipq[q]);
         SfR<R> sx[S], sy[S];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
-            sx[k] = shape<O, R>(pq.xmid, k);
-            sy[k] = shape<O, R>(pq.ymid, k);
+            sx[k] = shape<O, R>(xmid[q], k);
+            sy[k] = shape<O, R>(ymid[q], k);
         }
         const bool local = W.holds(ci[q], cj[q], S);
         const R cmmr = R(charge_mu0_mass_ratio);
@@ -228,9 +215,9 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
             for (int iy = 0; iy < S; ++iy) {
 #pragma unroll
                 for (int ix = 0; ix < S; ++ix) {
-                    const R cd = pq.q_invvol * sx[ix].s * sy[iy].s;
-                    const R v[6] = {cd * pq.vx_c, cd * pq.vy_c, cd * (pq.gamma_psi - R(1.0)) * clight,
-                                    cd * pq.gamma_psi, cd * cmmr * pq.psi_inv, cd};
+                    const R cd = q_invvol[q] * sx[ix].s * sy[iy].s;
+                    const R v[6] = {cd * vx_c[q], cd * vy_c[q], cd * (gamma_psi[q] - R(1.0)) * clight,
+                                    cd * gamma_psi[q], cd * cmmr * psi_inv[q], cd};
                     const int k = k0 + iy * W.w + ix;
 #pragma unroll
                     for (int a = 0; a < 6; ++a)
@@ -245,9 +232,9 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
             for (int ix = 0; ix < S; ++ix) {
                 const int i = sx[ix].cell, j = sy[iy].cell;
                 if (!in_store(g, i, j)) continue;
-                const R cd = pq.q_invvol * sx[ix].s * sy[iy].s;
-                const R v[6] = {cd * pq.vx_c, cd * pq.vy_c, cd * (pq.gamma_psi - R(1.0)) * clight,
-                                cd * pq.gamma_psi, cd * cmmr * pq.psi_inv, cd};
+                const R cd = q_invvol[q] * sx[ix].s * sy[iy].s;
+                const R v[6] = {cd * vx_c[q], cd * vy_c[q], cd * (gamma_psi[q] - R(1.0)) * clight,
+                                cd * gamma_psi[q], cd * cmmr * psi_inv[q], cd};
                 if (local) {
                     const int k = W.at(i, j);
 #pragma unroll
