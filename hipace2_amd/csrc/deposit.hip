@@ -29,7 +29,7 @@ constexpr int DCHUNK = DT * DPPT;       // particles per workgroup
 // while keeping four workgroups per CU resident.
 constexpr int WIN_DEPOSIT = 3072;       // 4 planes x 768 cells, 24 KiB
 constexpr int WIN_EXPLICIT = 4608;      // 6 planes x 768 cells, 36 KiB
-constexpr int WIN_BEAM = 3072;          // 3 planes x 1024 cells, 24 KiB
+constexpr int WIN_BEAM = 7680;          // 3 planes x 2560 cells, 60 KiB: a beam core of +-25 cells
 
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void lds_add(double* p, double v) {
