@@ -21,8 +21,14 @@ namespace hpq {
 namespace {
 
 constexpr int DT = 256;                 // threads per workgroup
-constexpr int DPPT = 4;                 // particles per thread
+// particles per thread: four for the current and beam depositions; two for the explicit one,
+// whose per-particle work (21 cells, four window reads and two LDS adds each) is longest, so
+// twice the workgroups hide its latency better (72 -> 59 us at 511^2, 2 x 2 ppc, while the
+// current deposition runs 12 % slower with two)
+constexpr int DPPT = 4;
+constexpr int DPPT_EXPLICIT = 2;
 constexpr int DCHUNK = DT * DPPT;       // particles per workgroup
+constexpr int DCHUNK_EXPLICIT = DT * DPPT_EXPLICIT;
 // LDS window sizes (doubles). A chunk of 1024 tile-sorted particles at 2 x 2 per cell
 // covers one 32 x 8-cell tile: 35 x 11 cells for a 3-point stencil, 37 x 13 for the
 // 5-point explicit one; the windows leave room for particles that drifted since the sort
@@ -56,10 +62,11 @@ struct Win {
 
 // Block-wide choice of the window for stencils of S x S cells starting at (ci, cj) for
 // the particles with act set; cap is the number of cells that fit. red: 7 * DT/64 ints.
+template <int PPT = DPPT>
 __device__ Win choose_window(const int* ci, const int* cj, const bool* act, int S, int cap, int* red) {
     int mni = INT_MAX, mnj = INT_MAX, mxi = INT_MIN, mxj = INT_MIN, si = 0, sj = 0, cnt = 0;
 #pragma unroll
-    for (int q = 0; q < DPPT; ++q) {
+    for (int q = 0; q < PPT; ++q) {
         if (!act[q]) continue;
         mni = min(mni, ci[q]); mxi = max(mxi, ci[q]);
         mnj = min(mnj, cj[q]); mxj = max(mxj, cj[q]);
@@ -264,15 +271,15 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
     __shared__ double win[WIN_EXPLICIT];
     __shared__ int red[7 * DT / 64];
     const R clight = 1.0, clight_inv = R(1.0) / R(1.0);
-    // Only the start cells of the DPPT particles stay in registers across the window choice;
+    // Only the start cells of a thread's particles stay in registers across the window choice;
     // each particle's momenta, weight and position are read again (cache hits) when its turn
     // comes, which keeps the FP64 kernel within 128 VGPRs (four waves per SIMD).
-    long ipq[DPPT];
-    int ci[DPPT], cj[DPPT];
-    bool act[DPPT];
-    const long base = long(blockIdx.x) * DCHUNK;
+    long ipq[DPPT_EXPLICIT];
+    int ci[DPPT_EXPLICIT], cj[DPPT_EXPLICIT];
+    bool act[DPPT_EXPLICIT];
+    const long base = long(blockIdx.x) * DCHUNK_EXPLICIT;
 #pragma unroll
-    for (int q = 0; q < DPPT; ++q) {
+    for (int q = 0; q < DPPT_EXPLICIT; ++q) {
         const long t = base + q * DT + threadIdx.x;
         act[q] = false;
         ipq[q] = 0;
@@ -285,7 +292,7 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
         cj[q] = dshape<T, O, R>((p.y[ip] - g.yoff) * g.dyi, 0).cell;
     }
     // window planes: Sy, Sx (accumulated), then copies of Bz, Ez, ExmBy, EypBx
-    const Win W = choose_window(ci, cj, act, S, WIN_EXPLICIT / 6, red);
+    const Win W = choose_window<DPPT_EXPLICIT>(ci, cj, act, S, WIN_EXPLICIT / 6, red);
     const int cells = W.w * W.h;
     const double* Bz = store + long(cBz) * g.plane;
     const double* Ez = store + long(cEz) * g.plane;
@@ -307,7 +314,7 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < DPPT; ++q) {
+    for (int q = 0; q < DPPT_EXPLICIT; ++q) {
         if (!act[q]) continue;
         // Per particle, the reference's Sy/Sx contribution of a cell
         //   Sy += cdm (-XsYs A q/psi + (-XdsYs dxi (-vx vy) - XsYds dyi (gamma/psi - vy^2 - 1)) c)
@@ -522,7 +529,7 @@ void explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store) {
     dispatch_order(c->p.depos_order, [&](auto O) {
         constexpr int o = decltype(O)::value;
         auto launch = [&](auto kern) {
-            hipLaunchKernelGGL(kern, grid1d(p->n, DCHUNK), dim3(DT), 0, c->stream, g, *p, order, store, L.Bz, L.Ez,
+            hipLaunchKernelGGL(kern, grid1d(p->n, DCHUNK_EXPLICIT), dim3(DT), 0, c->stream, g, *p, order, store, L.Bz, L.Ez,
                                L.ExmBy, L.EypBx, L.Sy, L.Sx, qimu0, qmr);
         };
         if (c->p.precision == HPQ_PREC_MIXED) {
