@@ -178,8 +178,9 @@ def run_adjoint(args):
     current profile is 64 piecewise-linear knots (seeded Uniform[0.5, 1.5]); loss = variance
     of on-axis Ez over the witness slices; checkpoint every 32 slices. Times the plain forward
     step, the checkpointed forward and the backward march, and checks the gradient against
-    central differences on `adjoint_fd_knots` knots. Multigrid tolerance 1e-10 in both
-    directions, so the loss is a smooth function of the knots."""
+    central differences on `adjoint_fd_knots` knots. Multigrid tolerance 1e-10 for the forward
+    solves, so the loss is a smooth function of the knots, and `adjoint_mg_tol` for the
+    warm-started adjoint solves."""
     import numpy as np
     import torch
     from hipace2_amd.adjoint import BeamLoading, on_axis_ez_variance, slice_loop_ez, witness_beam
@@ -239,7 +240,7 @@ def run_adjoint(args):
     q = torch.ones((), dtype=torch.float64, device="cuda", requires_grad=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ez = slice_loop_ez(weights(knots, q), sim, checkpoint_every=32, mg_tol=1e-9)
+    ez = slice_loop_ez(weights(knots, q), sim, checkpoint_every=32, mg_tol=args.adjoint_mg_tol)
     loss = loss_of(ez)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
@@ -257,7 +258,7 @@ def run_adjoint(args):
     knots_m = knots0.clone().requires_grad_(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    loss_of(slice_loop_ez(bl.weights(knots_m), sim_m, checkpoint_every=32, mg_tol=1e-9)).backward()
+    loss_of(slice_loop_ez(bl.weights(knots_m), sim_m, checkpoint_every=32, mg_tol=args.adjoint_mg_tol)).backward()
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t0
     grad_m = knots_m.grad.detach().cpu().numpy()
@@ -293,7 +294,7 @@ def run_adjoint(args):
               f"{LARGE['drive_peak']}) + witness 2e6 particles" if large else
               "1x1 ppc, drive 1e5 + witness 1e6 particles")
     return {"workload": f"{n}x{n}x{nz}, {plasma}, 64 knots and the drive charge, loss Var(on-axis Ez) "
-                        "over the witness slices, checkpoint every 32 slices, MG tolerance 1e-10 (adjoint solves 1e-9, warm-started), FP64",
+                        f"over the witness slices, checkpoint every 32 slices, MG tolerance 1e-10 (adjoint solves {args.adjoint_mg_tol:g}, warm-started), FP64",
             "drive_charge_grad": grad_q, "drive_charge_fd_by_step": fd_q, "drive_charge_rel_err_vs_fd_1e-4": err_q,
             "loss": float(loss.detach()), "plain_forward_s": round(t_plain, 4), "checkpointed_forward_s": round(t_fwd, 4),
             "backward_s": round(t_bwd, 4), "ratio_to_forward": round((t_fwd + t_bwd) / t_plain, 3),
@@ -318,7 +319,7 @@ def beam_loading(args, cfg, bl, sim, knots0, weights, loss_of):
     t0 = time.perf_counter()
     for _ in range(args.loading_iters):
         opt.zero_grad()
-        loss = loss_of(slice_loop_ez(weights(knots), sim, checkpoint_every=32, mg_tol=1e-9))
+        loss = loss_of(slice_loop_ez(weights(knots), sim, checkpoint_every=32, mg_tol=args.adjoint_mg_tol))
         loss.backward()
         history.append(float(loss.detach()))
         opt.step()
@@ -414,6 +415,8 @@ def main():
     ap.add_argument("--mixed-poisson", choices=["bf16x3", "fft"], default="bf16x3",
                     help="transverse Poisson solver of the mixed path")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the beam-loading gradient measurement")
+    ap.add_argument("--adjoint-mg-tol", type=float, default=1e-8,
+                    help="relative tolerance of the adjoint multigrid solves (warm-started)")
     ap.add_argument("--adjoint-fd-knots", type=int, default=8,
                     help="knots checked against central differences (two forward steps each)")
     ap.add_argument("--loading-iters", type=int, default=0,
