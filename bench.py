@@ -6,16 +6,18 @@ Workload (one step = one pass of the slice loop over all ζ-slices, inputs resid
   explicit Bx/By solver with multigrid; Gaussian electron drive beam with peak density
   0.01 n0, sigma_r = 0.3, sigma_z = 1.41, 4e6 macro-particles (seeded Philox), gamma = 1000.
 
-Two arithmetic paths are timed on the same workload: the FP64 plain path (the drop-in)
-and the mixed path (FP32 gather, push increments and deposition, the FP32 contributions
-summed in FP64 LDS words; DST-I as split-BF16 sine-matrix GEMMs on the matrix cores; FP64 multigrid
-and particle state). One more untimed step of each records
-Ez, Ex-By, jx and rho-jz on every slice, and the mixed path's norm-wise relative error
-against the FP64 path is reported.
+Three paths are timed on the same workload: the FP64 plain path (the drop-in: deposition
+summed in the reference CPU build's order, bit for bit reproducible; the headline `value`),
+the FP64 path with atomic deposition sums, and the mixed path (FP32 gather, push increments and
+deposition, the FP32 contributions summed in FP64 LDS words; DST-I as split-BF16 sine-matrix
+GEMMs on the matrix cores; FP64 multigrid and particle state). One more untimed step of each
+records Ez, Ex-By, jx and rho-jz on every slice, and the other paths' norm-wise relative
+errors against the FP64 path are reported.
 
-Prints one JSON line: slices/s of the mixed path (value) beside the FP64 path's, plasma
-particle pushes/s, the roofline of the dominant kernel (HIP-event timed inside the timed
-region) and the oracle timed on the host cores on a bounded sample of the same workload.
+Prints one JSON line: slices/s of the FP64 path (value), plasma particle pushes/s, the roofline
+of the kernel family that takes the most time and the achieved bandwidth of every family (HIP-
+event timed), and the oracle timed on the host cores (all cores of the box's share and one
+core) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -39,40 +41,46 @@ COPY_BYTES = 256 << 20
 # scaled by the same counters on a 256 MiB device copy (see the summary beside it)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_511x511x512.json")
 DTYPE = {"fp64": "f64",
+         "fp64_atomic": "f64",
          "mixed": "f32 (gather, push increments, deposition) + split-bf16 x3 / f32 accumulate (DST) + f64 "
                   "(multigrid, particle state)"}
 
 
-# The large workload: 1023 x 1023 over [-10, 10]^2, 1024 slices, 2 x 2 ppc with the transverse
-# density ramp n(r) = n0 (1 + 0.1 r^2), drive of 1e7 particles at peak 5 n0, witness of 2e6
-LARGE = dict(n=1023, nz=1024, half_width=10.0, ppc=2, ramp=0.1, drive_peak=5.0, drive_n=10_000_000,
-             witness_n=2_000_000, witness_peak=1.0)
-
-
 def make_workload(n=511, nz=512, ppc=2, nbeam=4_000_000, large=False):
-    from hipace2_amd.inputs import BeamSpec, SliceConfig, make_beam, merge_beams
+    """The throughput workload (configuration 3) or, large, configuration 5
+    (hipace2_amd/workloads.py)."""
+    from hipace2_amd import workloads
     if large:
-        from hipace2_amd.adjoint import witness_beam
-        P = LARGE
-        L = P["half_width"]
-        cfg = SliceConfig(nx=P["n"], ny=P["n"], nz=P["nz"], xlo=-L, xhi=L, ylo=-L, yhi=L, zlo=-8.0, zhi=4.0,
-                          depos_order=2, derivative_type=2, explicit_solver=True, ppc=(P["ppc"], P["ppc"]),
-                          ramp_a=P["ramp"])
-        drive = make_beam(cfg, BeamSpec(injection_type="fixed_weight", profile="gaussian", density=P["drive_peak"],
-                                        position_std=(0.3, 0.3, 1.41), num_particles=P["drive_n"],
-                                        u_mean=(0.0, 0.0, 1000.0), seed=1234))
-        wit = witness_beam(cfg, P["witness_n"], -6.0, -4.0, 0.2, P["witness_peak"], 1000.0, seed=4321)
-        return cfg, merge_beams(cfg, [drive, wit])
-    cfg = SliceConfig(nx=n, ny=n, nz=nz, xlo=-8.0, xhi=8.0, ylo=-8.0, yhi=8.0, zlo=-8.0, zhi=4.0,
-                      depos_order=2, derivative_type=2, explicit_solver=True, ppc=(ppc, ppc))
-    spec = BeamSpec(injection_type="fixed_weight", profile="gaussian", density=0.01,
-                    position_std=(0.3, 0.3, 1.41), num_particles=nbeam, u_mean=(0.0, 0.0, 1000.0), seed=1234)
-    return cfg, make_beam(cfg, spec)
+        return workloads.config5()
+    return workloads.config3(n, nz, ppc, nbeam)
 
 
-def bytes_per_launch(kind, cfg, n_plasma):
-    """Algorithmic HBM bytes of one launch (each operand read or written once)."""
+def mg_bytes_per_step(cfg, vcycles):
+    """Algorithmic HBM bytes of the multigrid in one step: (18 + 35 k) G 8 per slice with k
+    V-cycles on that slice, summed over the slices, (18 nz + 35 K) G 8 with K V-cycles in all:
+    18 G words for the solve's set-up (rhs, acf, the guess and the result of two components,
+    the first residual and its restriction) and 35 G words per V-cycle (four sweeps down and up
+    on every level of a hierarchy whose levels shrink by 4, plus the level-0 residual)."""
     G = cfg.nx * cfg.ny
+    return (18 * cfg.nz + 35 * vcycles) * G * 8
+
+
+def dst_bytes_per_launch(cfg):
+    """Algorithmic bytes of one batched row DST launch of the three-field Poisson solve: the x
+    passes read and write 3 G words, the fused y pass (forward, eigenvalues, inverse) reads
+    3 G + G and writes 3 G; averaged over the three launches of a solve."""
+    G = cfg.nx * cfg.ny
+    return (6 + 7 + 6) * G * 8 // 3
+
+
+def bytes_per_launch(kind, cfg, n_plasma, launches=None, vcycles=None):
+    """Algorithmic HBM bytes of one launch (each operand read or written once); for the
+    multigrid, whose launches differ by level, the step's total over its launch count."""
+    G = cfg.nx * cfg.ny
+    if kind == "mg":
+        return mg_bytes_per_step(cfg, vcycles) / launches if launches and vcycles is not None else None
+    if kind == "dst":
+        return dst_bytes_per_launch(cfg)
     if kind == "push":
         # read x_prev, y_prev, ux/uy/psi half-step (5 x 8 B) + valid (4 B); write x, y, ux, uy, psi
         # and the five half-step values (10 x 8 B); gather reads Psi, Ez, Bx, By, Bz once
@@ -86,20 +94,37 @@ def bytes_per_launch(kind, cfg, n_plasma):
     return None
 
 
-def run_gpu(args, precision, capture):
+# kernel families: trace kinds of the library (the multigrid's two kinds form one family)
+FAMILIES = {"push": ["push"], "deposit": ["deposit"], "explicit_deposit": ["explicit_deposit"],
+            "mg": ["mg_smooth", "mg_coarse"], "dst": ["dst"], "beam_deposit": ["beam_deposit"]}
+
+
+def family_times(traces):
+    """(total ms, launches) per kernel family from per-kind traces."""
+    out = {}
+    for fam, kinds in FAMILIES.items():
+        got = [traces[k] for k in kinds if k in traces]
+        if got:
+            out[fam] = (sum(t[0] for t in got), sum(t[1] for t in got))
+    return out
+
+
+def run_gpu(args, path, capture):
     import torch
     from hipace2_amd.simulation import Simulation
 
     cfg, beam = make_workload(args.n, args.nz, args.ppc, args.nbeam, large=args.workload == "large")
     if args.bin_period is not None:
         cfg.bin_period = args.bin_period
+    precision = "mixed" if path == "mixed" else "fp64"
     cfg.precision = precision
+    cfg.deposition = "atomic" if path in ("mixed", "fp64_atomic") else "ordered"
     if precision == "mixed":
         cfg.poisson_solver = args.mixed_poisson
     sim = Simulation(cfg, beam)
     sim.init_plasma()
     pristine = {k: v.clone() for k, v in sim.plasma.items()}
-    kinds = ["push", "deposit", "explicit_deposit", "mg_smooth", "mg_coarse", "dst", "beam_deposit"]
+    kinds = [k for ks in FAMILIES.values() for k in ks]
 
     def step():
         for k, v in pristine.items():
@@ -111,14 +136,16 @@ def run_gpu(args, precision, capture):
     # timed steps keep events only around the dominant family, so the timed region is not
     # slowed by tens of event records per slice.
     breakdown = {}
+    warm_stats = None
     if args.warmup:
         sim.trace(kinds, True)
         sim.trace_reset()
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
+        warm_stats = sim.stats()
         breakdown = {k: sim.trace_read(k) for k in kinds}
-        timed = [dominant_kind(cfg, sim.np, breakdown)]
+        timed = FAMILIES[dominant_family(family_times(breakdown))]
         sim.trace(kinds, False)
     else:
         timed = kinds
@@ -139,9 +166,9 @@ def run_gpu(args, precision, capture):
         caps = {k: sim.capture(k) for k in ERR_FIELDS}
         step()
         torch.cuda.synchronize()
-    print(f"{precision}: {cfg.nz * args.steps / elapsed:.1f} slices/s", file=sys.stderr, flush=True)
+    print(f"{path}: {cfg.nz * args.steps / elapsed:.1f} slices/s", file=sys.stderr, flush=True)
     return dict(cfg=cfg, beam=beam, sim=sim, elapsed=elapsed, traces=traces, stats=stats, breakdown=breakdown,
-                caps=caps)
+                warm_stats=warm_stats, caps=caps)
 
 
 def measured_traffic(kind, precision, args):
@@ -165,11 +192,27 @@ def errors_vs_fp64(mixed, fp64):
     return out
 
 
-def dominant_kind(cfg, n_plasma, traces):
-    """The kernel family with an algorithmic byte count that takes the most time."""
-    bounded = {k: ms for k, (ms, launches) in traces.items()
-               if launches and bytes_per_launch(k, cfg, n_plasma) is not None}
-    return max(bounded, key=bounded.get)
+def dominant_family(fam_times):
+    """The kernel family that takes the most time."""
+    timed = {k: ms for k, (ms, launches) in fam_times.items() if launches}
+    return max(timed, key=timed.get)
+
+
+def family_table(cfg, n_plasma, fam_times, steps, vcycles_per_step):
+    """Per family: launches, average launch time, algorithmic bytes per launch, achieved GB/s
+    and its fraction of the HBM peak."""
+    out = {}
+    for fam, (ms, launches) in fam_times.items():
+        if not launches:
+            continue
+        bpl = bytes_per_launch(fam, cfg, n_plasma, launches / steps, vcycles_per_step)
+        avg_us = 1e3 * ms / launches
+        row = {"total_ms": round(ms, 3), "launches": launches, "avg_us": round(avg_us, 2)}
+        if bpl is not None:
+            gbs = bpl / (avg_us * 1e-6) / 1e9
+            row.update(bytes_per_launch=int(bpl), achieved_GBs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4))
+        out[fam] = row
+    return out
 
 
 def run_adjoint(args):
@@ -183,30 +226,14 @@ def run_adjoint(args):
     warm-started adjoint solves."""
     import numpy as np
     import torch
-    from hipace2_amd.adjoint import BeamLoading, on_axis_ez_variance, slice_loop_ez, witness_beam
-    from hipace2_amd.inputs import BeamSpec, SliceConfig, make_beam
+    from hipace2_amd import workloads
+    from hipace2_amd.adjoint import on_axis_ez_variance, slice_loop_ez
     from hipace2_amd.simulation import Simulation
 
     large = args.workload == "large"
-    if large:
-        P = LARGE
-        n, nz, L = P["n"], P["nz"], P["half_width"]
-        cfg = SliceConfig(nx=n, ny=n, nz=nz, xlo=-L, xhi=L, ylo=-L, yhi=L, zlo=-8.0, zhi=4.0, depos_order=2,
-                          derivative_type=2, explicit_solver=True, ppc=(P["ppc"], P["ppc"]), ramp_a=P["ramp"])
-    else:
-        n, nz = args.n, args.nz
-        cfg = SliceConfig(nx=n, ny=n, nz=nz, xlo=-8.0, xhi=8.0, ylo=-8.0, yhi=8.0, zlo=-8.0, zhi=4.0,
-                          depos_order=2, derivative_type=2, explicit_solver=True, ppc=(1, 1))
-    cfg.do_beam_jx_jy = False
-    cfg.mg_tol_rel = 1e-10
-    drive = make_beam(cfg, BeamSpec(injection_type="fixed_weight", profile="gaussian",
-                                    density=LARGE["drive_peak"] if large else 0.01,
-                                    position_std=(0.3, 0.3, 1.41),
-                                    num_particles=LARGE["drive_n"] if large else 100_000,
-                                    u_mean=(0.0, 0.0, 1000.0), seed=1234))
-    wit = witness_beam(cfg, LARGE["witness_n"] if large else 1_000_000, -6.0, -4.0, 0.2,
-                       LARGE["witness_peak"] if large else 0.005, 1000.0, seed=4321)
-    bl = BeamLoading(cfg, drive, wit, -6.0, -4.0)
+    cfg, bl = workloads.config4(large=large)
+    n, nz = cfg.nx, cfg.nz
+    P = workloads.LARGE
     sim = Simulation(cfg, bl.beam)
     knots0 = torch.as_tensor(np.random.default_rng(99).uniform(0.5, 1.5, 64), device="cuda")
     wsl = bl.witness_slices()
@@ -235,6 +262,18 @@ def run_adjoint(args):
     t0 = time.perf_counter()
     plain(knots0)
     t_plain = time.perf_counter() - t0
+    # the same forward step at the production multigrid tolerance (hipace.MG_tolerance_rel
+    # default 1e-4), the other denominator of the cost ratio
+    sim_p = Simulation(dataclasses.replace(cfg, mg_tol_rel=1e-4), bl.beam)
+    sim_p.beam["w"][: bl.beam.n].copy_(weights(knots0))
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim_p.begin_step()
+        sim_p.run_slices()
+        torch.cuda.synchronize()
+        t_plain_prod = time.perf_counter() - t0
+    del sim_p
 
     knots = knots0.clone().requires_grad_(True)
     q = torch.ones((), dtype=torch.float64, device="cuda", requires_grad=True)
@@ -290,14 +329,16 @@ def run_adjoint(args):
           f"FD max rel err {err}, drive-charge derivative err {err_q}, mixed-forward gradient vs FP64 "
           f"{mixed_err:.3e}", file=sys.stderr, flush=True)
     loading = None if large or args.loading_iters <= 0 else beam_loading(args, cfg, bl, sim, knots0, weights, loss_of)
-    plasma = (f"{LARGE['ppc']}x{LARGE['ppc']} ppc with n(r) = n0 (1 + {LARGE['ramp']} r^2), drive 1e7 (peak "
-              f"{LARGE['drive_peak']}) + witness 2e6 particles" if large else
+    plasma = (f"{P['ppc']}x{P['ppc']} ppc with n(r) = n0 (1 + {P['ramp']} r^2), drive 1e7 (peak "
+              f"{P['drive_peak']}) + witness 2e6 particles" if large else
               "1x1 ppc, drive 1e5 + witness 1e6 particles")
     return {"workload": f"{n}x{n}x{nz}, {plasma}, 64 knots and the drive charge, loss Var(on-axis Ez) "
                         f"over the witness slices, checkpoint every 32 slices, MG tolerance 1e-10 (adjoint solves {args.adjoint_mg_tol:g}, warm-started), FP64",
             "drive_charge_grad": grad_q, "drive_charge_fd_by_step": fd_q, "drive_charge_rel_err_vs_fd_1e-5": err_q,
             "loss": float(loss.detach()), "plain_forward_s": round(t_plain, 4), "checkpointed_forward_s": round(t_fwd, 4),
             "backward_s": round(t_bwd, 4), "ratio_to_forward": round((t_fwd + t_bwd) / t_plain, 3),
+            "plain_forward_production_tol_s": round(t_plain_prod, 4),
+            "ratio_to_forward_production_tol": round((t_fwd + t_bwd) / t_plain_prod, 3),
             "fd_knots": [int(k) for k in picks], "fd_max_rel_err": err,
             "mixed_forward_grad_rel_err_vs_fp64": mixed_err, "mixed_forward_plus_backward_s": round(t_mixed, 4),
             "grad_norm": float(np.linalg.norm(grad)), "beam_loading": loading}
@@ -362,14 +403,54 @@ def beam_loading(args, cfg, bl, sim, knots0, weights, loss_of):
             "salame_loss": loss_sal, "salame_s": round(t_sal, 3), "salame_slices": n_sal}
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) and its logical CPUs."""
+    name = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
+def timing_oracle():
+    """The oracle built for timing on this host: -O3 -march=native (FMA contraction stays off,
+    so its results are those of the committed build), compiled into a temporary directory;
+    the committed build when the compiler is not available."""
+    import shutil
+    import subprocess
+    import tempfile
+    from oracle import oracle as orc_mod
+    src = os.path.join(ROOT, "oracle")
+    out = tempfile.mkdtemp(prefix="oracle_native_")
+    try:
+        for f in os.listdir(src):
+            if f.endswith((".cpp", ".hpp")) or f == "Makefile":
+                shutil.copy(os.path.join(src, f), out)
+        flags = "-O3 -march=native -std=c++17 -fPIC -fopenmp -ffp-contract=off -Wall -Wno-unused-function"
+        subprocess.run(["make", "-s", "-C", out, "-j8", f"CXXFLAGS={flags}"], check=True, capture_output=True,
+                       timeout=300)
+        lib = os.path.join(out, "lib", "liboracle.so")
+        orc_mod.use_library(lib)
+        return "-O3 -march=native"
+    except (OSError, subprocess.SubprocessError) as e:
+        print(f"cpu baseline: native oracle build failed ({e}); timing the committed -O3 build", file=sys.stderr)
+        return "-O3 (committed build)"
+
+
 def cpu_baseline(args, cfg, beam, sim):
-    """The oracle on the host cores, started from the GPU state at slice s0 (inside the beam)."""
-    import numpy as np
+    """The oracle (the CPU build's algorithms in C++/OpenMP, FP64) on the host cores, started
+    from the GPU state at slice s0 (inside the beam): on every core of the box's CPU share and
+    on one core."""
     import torch
     from oracle.oracle import OracleSim
-    from hipace2_amd.simulation import PLASMA_REALS
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+    flags = timing_oracle()
     s0 = min(cfg.nz - 1, int(round((0.44 - cfg.zlo) / cfg.dz)))
     # GPU state entering slice s0
     sim.trace(["push"], False)
@@ -377,23 +458,34 @@ def cpu_baseline(args, cfg, beam, sim):
     sim.begin_step(init_plasma=False)
     sim.run_slices(cfg.nz - 1, s0 + 1)
     torch.cuda.synchronize()
-    orc = OracleSim(cfg, beam, nthreads=threads)
-    orc.begin_step()
-    orc.set_store(sim.store_host())
-    p = sim.plasma_host()
-    orc.set_plasma(p)
-    done = 0
-    t0 = time.perf_counter()
-    budget = args.cpu_seconds
-    while done < args.cpu_max_slices:
-        orc.solve_slice(s0 - done)
-        done += 1
-        if time.perf_counter() - t0 > budget and done >= 2:
-            break
-    t = time.perf_counter() - t0
-    return {"value": done / t, "unit": "slices/s", "cores": threads, "kind": "port",
+    store, plasma = sim.store_host(), sim.plasma_host()
+
+    def timed(nthreads, budget, max_slices):
+        orc = OracleSim(cfg, beam, nthreads=nthreads)
+        orc.begin_step()
+        orc.set_store(store)
+        orc.set_plasma(plasma)
+        done = 0
+        t0 = time.perf_counter()
+        while done < max_slices:
+            orc.solve_slice(s0 - done)
+            done += 1
+            if time.perf_counter() - t0 > budget and done >= 1:
+                break
+        t = time.perf_counter() - t0
+        del orc
+        return done, t
+
+    done, t = timed(threads, args.cpu_seconds, args.cpu_max_slices)
+    done1, t1 = timed(1, args.cpu_seconds, args.cpu_max_slices)
+    model, logical = cpu_model()
+    return {"value": round(done / t, 4), "unit": "slices/s", "cores": threads, "kind": "port",
+            "one_core": {"value": round(done1 / t1, 4), "unit": "slices/s", "cores": 1,
+                         "sample": f"{done1} slices in {t1:.1f} s"},
+            "cpu_model": model, "host_logical_cpus": logical, "compiler_flags": flags,
             "sample": f"{done} consecutive slices of the same step starting at slice {s0} (zeta=+0.44, inside "
-                      f"the drive beam) from the GPU's state, oracle (C++/OpenMP FP64) on {threads} host threads"}
+                      f"the drive beam) from the GPU's state, {t:.1f} s; oracle (C++/OpenMP FP64, the CPU "
+                      f"build's algorithms and summation orders) on {threads} threads (the box's CPU share)"}
 
 
 def main():
@@ -405,13 +497,16 @@ def main():
     ap.add_argument("--nz", type=int, default=512)
     ap.add_argument("--ppc", type=int, default=2)
     ap.add_argument("--nbeam", type=int, default=4_000_000)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="host threads of the CPU baseline (the CPU share of one GPU on the box)")
     ap.add_argument("--cpu-max-slices", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bin-period", type=int, default=None,
                     help="slices between tile sorts of the plasma for deposition (0: index order)")
-    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
-                    help="arithmetic path(s) to time; 'both' also reports the mixed path's error")
+    ap.add_argument("--paths", default="fp64,fp64_atomic,mixed",
+                    help="paths to time (comma separated): fp64 (ordered deposition, the headline), fp64_atomic, "
+                         "mixed; the first is the headline, the others report their error against it")
     ap.add_argument("--mixed-poisson", choices=["bf16x3", "fft"], default="bf16x3",
                     help="transverse Poisson solver of the mixed path")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the beam-loading gradient measurement")
@@ -439,9 +534,12 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
-    runs = {p: run_gpu(args, p, capture=len(precs) == 2) for p in precs}
-    head = precs[-1]
+    paths = [q.strip() for q in args.paths.split(",") if q.strip()]
+    for q in paths:
+        if q not in ("fp64", "fp64_atomic", "mixed"):
+            raise SystemExit(f"bench.py: unknown path {q}")
+    runs = {q: run_gpu(args, q, capture=len(paths) > 1) for q in paths}
+    head = paths[0]
     r = runs[head]
     cfg, beam, sim, elapsed, traces, stats, breakdown = (r[k] for k in ("cfg", "beam", "sim", "elapsed", "traces",
                                                                         "stats", "breakdown"))
@@ -452,36 +550,39 @@ def main():
     # a slice as one push (src/particles/pusher/PlasmaParticleAdvance.cpp:70-73)
     pushes_per_s = stats["pushes"] * args.steps / elapsed
 
-    def table(tr):
-        return {k: {"total_ms": round(ms, 3), "launches": launches,
-                    "avg_us": round(1e3 * ms / launches, 2) if launches else 0.0} for k, (ms, launches) in tr.items()}
+    # every family over the warm-up steps (events around every launch), the dominant one over
+    # the timed steps (events around its launches only)
+    warm = family_table(cfg, n_plasma, family_times(breakdown), max(args.warmup, 1),
+                        r["warm_stats"]["mg_iters"] if r["warm_stats"] else None) if breakdown else {}
+    timed = family_table(cfg, n_plasma, family_times(traces), args.steps, stats["mg_iters"])
+    dom = dominant_family(family_times(traces))
+    row = timed[dom]
+    roofline = {"bound": "hbm", "achieved": row.get("achieved_GBs"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": row.get("frac"), "traffic": measured_traffic(dom, head, args),
+                "kernel": dom, "bytes_per_launch": row.get("bytes_per_launch"), "avg_launch_us": row["avg_us"],
+                "launches": row["launches"],
+                "byte_model": "mg: (18 nz + 35 K) G 8 per step over its launches (K V-cycles)" if dom == "mg" else
+                              "each operand read or written once"}
+    if warm:
+        print(f"kernel families over {args.warmup} warm-up step(s): " + json.dumps(warm), file=sys.stderr)
+    print("dominant family over the timed region: " + json.dumps(timed), file=sys.stderr)
 
-    rows = table(traces)
-    dom = dominant_kind(cfg, n_plasma, traces)
-    bpl = bytes_per_launch(dom, cfg, n_plasma)
-    avg_s = rows[dom]["avg_us"] * 1e-6
-    achieved = bpl / avg_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, head, args),
-                "kernel": dom, "bytes_per_launch": bpl, "avg_launch_us": round(rows[dom]["avg_us"], 2)}
-    if breakdown:
-        print(f"kernel times over {args.warmup} warm-up step(s): " + json.dumps(table(breakdown)), file=sys.stderr)
-    print("dominant kernel over the timed region: " + json.dumps(rows), file=sys.stderr)
+    def summary(q):
+        f = runs[q]
+        v = cfg.nz * args.steps / f["elapsed"]
+        return {"value": round(v, 2), "unit": "slices/s", "ms_per_step": round(1e3 * f["elapsed"] / args.steps, 2),
+                "speedup_vs_fp64": round(v / slices_per_s, 3),
+                "plasma_pushes_per_s": round(f["stats"]["pushes"] * args.steps / f["elapsed"], 1),
+                "mg_vcycles_per_slice": round(f["stats"]["mg_iters"] / cfg.nz, 3), "dtype": DTYPE[q]}
 
     extra = {}
-    if len(precs) == 2:
-        f = runs["fp64"]
-        f_slices = cfg.nz * args.steps / f["elapsed"]
-        extra["fp64"] = {"value": round(f_slices, 2), "unit": "slices/s",
-                         "ms_per_step": round(1e3 * f["elapsed"] / args.steps, 2),
-                         "plasma_pushes_per_s": round(f["stats"]["pushes"] * args.steps / f["elapsed"], 1),
-                         "mg_vcycles_per_slice": round(f["stats"]["mg_iters"] / cfg.nz, 3)}
-        extra["speedup_vs_fp64"] = round(slices_per_s / f_slices, 3)
-        extra["max_rel_err_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in
-                                        errors_vs_fp64(r["caps"], f["caps"]).items()}
-        for p in precs:
-            runs[p]["caps"] = None
-            runs[p]["sim"].clear_captures()
+    for q in paths[1:]:
+        extra[q] = summary(q)
+        extra[q]["max_rel_err_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in
+                                           errors_vs_fp64(runs[q]["caps"], r["caps"]).items()}
+    for q in paths:
+        runs[q]["caps"] = None
+        runs[q]["sim"].clear_captures()
 
     cpu = None
     if not args.no_cpu:
@@ -508,13 +609,15 @@ def main():
         "config": {"workload": WORKLOAD if (args.n, args.nz, args.ppc) == (511, 512, 2) and args.workload == "standard"
                    else f"{cfg.nx}x{cfg.ny}x{cfg.nz} slice loop, {cfg.ppc[0]}x{cfg.ppc[1]} ppc"
                         f"{', density ramp' if cfg.ramp_a else ''}, shape 2, explicit MG",
-                   "precision": head, "poisson_solver": cfg.poisson_solver,
+                   "path": head, "deposition": cfg.deposition, "poisson_solver": cfg.poisson_solver,
                    "nx": cfg.nx, "ny": cfg.ny, "slices": cfg.nz, "plasma_particles_per_slice": n_plasma,
                    "beam_particles": beam.n, "mg_vcycles_per_slice": round(stats["mg_iters"] / cfg.nz, 3),
                    "bin_period": cfg.bin_period},
         "plasma_pushes_per_s": round(pushes_per_s, 1),
         **extra,
         "roofline": roofline,
+        "kernel_families": timed,
+        "kernel_families_warmup": warm,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
     }

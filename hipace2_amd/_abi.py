@@ -49,9 +49,10 @@ class Params(C.Structure):
                 ("pc_tol", C.c_double), ("pc_maxiter", C.c_int), ("pc_mix", C.c_double),
                 ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double),
                 ("bin_period", C.c_int), ("precision", C.c_int), ("poisson_solver", C.c_int),
-                ("salame", C.c_int)]
+                ("salame", C.c_int), ("deposition", C.c_int), ("tile_size", C.c_int)]
 
 PRECISION = {"fp64": 0, "mixed": 1}
+DEPOSITION = {"ordered": 0, "atomic": 1}
 POISSON_SOLVER = {"fft": 0, "bf16x3": 1}
 
 
@@ -100,7 +101,7 @@ class Salame(C.Structure):
 
 class PlasmaProfile(C.Structure):
     _fields_ = [("ppc", C.c_int * 2), ("density", C.c_double), ("ramp_a", C.c_double), ("radius", C.c_double),
-                ("u_mean", C.c_double * 3)]
+                ("u_mean", C.c_double * 3), ("density_table", C.c_void_p)]
 
 
 class HpqError(RuntimeError):

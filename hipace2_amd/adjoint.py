@@ -16,12 +16,19 @@ from .simulation import Simulation
 
 class SliceLoopEz(torch.autograd.Function):
     """Ez[slice, j, i] over the valid box for every slice of one step, as a function of the
-    beam weights w (one per macro-particle, in the beam's slice-sorted order)."""
+    beam weights w (one per macro-particle, in the beam's slice-sorted order).
+
+    The checkpoints of the forward pass live in the Simulation's context, and the backward pass
+    recomputes from them with the beam weights in the Simulation: a backward is valid only for
+    the last forward run on that Simulation, with its weights unchanged. Each forward takes a
+    new generation number; backward raises if another forward ran since, or if the weights
+    changed."""
 
     @staticmethod
     def forward(ctx, beam_w: torch.Tensor, sim: Simulation, checkpoint_every: int = 32,
                 mg_tol: float = 1e-10):
-        sim.beam["w"][: beam_w.numel()].copy_(beam_w.detach().to(sim.beam["w"].dtype))
+        w = beam_w.detach().to(sim.beam["w"].dtype)
+        sim.beam["w"][: beam_w.numel()].copy_(w)
         ez = sim.capture("Ez")
         sim.begin_step()
         sim.adjoint_forward(checkpoint_every)
@@ -31,11 +38,18 @@ class SliceLoopEz(torch.autograd.Function):
         ctx.sim = sim
         ctx.mg_tol = mg_tol
         ctx.n = beam_w.numel()
+        ctx.generation = sim.adjoint_generation
+        ctx.w = w.clone()
         return out
 
     @staticmethod
     def backward(ctx, grad_ez: torch.Tensor):
         sim = ctx.sim
+        if sim.adjoint_generation != ctx.generation:
+            raise RuntimeError("slice_loop_ez: another forward pass ran on this Simulation since; its "
+                               "checkpoints replaced this one's")
+        if not torch.equal(sim.beam["w"][: ctx.n], ctx.w):
+            raise RuntimeError("slice_loop_ez: the beam weights changed between the forward and the backward pass")
         w_adj = torch.zeros(max(ctx.n, 1), dtype=torch.float64, device=sim.device)
         sim.adjoint_backward(grad_ez.detach().to(torch.float64).contiguous(), w_adj, ctx.mg_tol)
         return w_adj[: ctx.n], None, None, None

@@ -18,7 +18,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhpq.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["poisson.hip", "mg.hip", "particles.hip", "deposit.hip", "bins.hip", "dst_mfma.hip", "adjoint.hip", "insitu.hip", "salame.hip", "fields.hip", "hipace.cpp"]
+SOURCES = ["poisson.hip", "mg.hip", "particles.hip", "deposit.hip", "bins.hip", "dst_mfma.hip", "adjoint.hip", "insitu.hip", "salame.hip", "fields.hip", "gather.hip", "hipace.cpp"]
 HEADERS = ["internal.hpp", "shape.hpp", "push.hpp", "tangent.hpp", os.path.join("..", "..", "include", "hpq.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -41,16 +41,24 @@ def build(force: bool = False, verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     hdrs = [os.path.join(CSRC, h) for h in HEADERS]
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(OBJDIR, src + ".o")
         objs.append(obj)
         if force or _newer(obj, [path] + hdrs):
-            cmd = [HIPCC] + COMMON + DEVICE + ["-c", path, "-o", obj]
+            cmds.append([HIPCC] + COMMON + DEVICE + ["-c", path, "-o", obj])
+    # translation units compile in parallel (hipcc is single-threaded per file)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8"))))
+    for i in range(0, len(cmds), jobs):
+        procs = []
+        for cmd in cmds[i:i + jobs]:
             if verbose:
                 print(" ".join(cmd), file=sys.stderr)
-            subprocess.run(cmd, check=True)
+            procs.append((cmd, subprocess.Popen(cmd)))
+        failed = [cmd for cmd, pr in procs if pr.wait() != 0]
+        if failed:
+            raise subprocess.CalledProcessError(1, failed[0])
     if force or _newer(LIB, objs):
         tmp = LIB + ".tmp"
         cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs

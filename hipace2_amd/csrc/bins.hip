@@ -138,6 +138,17 @@ __global__ void k_bin_offsets(const int32_t* __restrict__ first, int nbins, int 
     if (b == nbins) offsets[b] = int32_t(n);
 }
 
+template <class T>
+void grow(T*& p, long& cap, long need) {
+    if (need <= cap) return;
+    if (p) HPQ_HIP(hipFree(p));
+    p = nullptr;
+    HPQ_HIP(hipMalloc(&p, sizeof(T) * size_t(need)));
+    cap = need;
+}
+
+}  // namespace
+
 // exclusive scan of a[0..n) in place; tmp holds the block sums of every level
 void scan_exclusive(hipStream_t s, int32_t* a, long n, int32_t* tmp) {
     const long nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
@@ -155,17 +166,6 @@ long scan_tmp_size(long n) {
     for (long nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK; nb > 1; nb = (nb + SCAN_BLOCK - 1) / SCAN_BLOCK) t += nb;
     return t + 1;
 }
-
-template <class T>
-void grow(T*& p, long& cap, long need) {
-    if (need <= cap) return;
-    if (p) HPQ_HIP(hipFree(p));
-    p = nullptr;
-    HPQ_HIP(hipMalloc(&p, sizeof(T) * size_t(need)));
-    cap = need;
-}
-
-}  // namespace
 
 void bins_free(hpq_ctx* c) {
     Bins& B = c->bins;

@@ -484,8 +484,18 @@ __global__ __launch_bounds__(DT) void k_deposit_beam(Geom g, hpq_beam b, long fi
 
 }  // namespace
 
+// the FP64 path in the reference CPU build's summation order (gather.hip)
+static bool ordered(const hpq_ctx* c) {
+    return c->p.deposition == HPQ_DEPOSIT_ORDERED && c->p.precision == HPQ_PREC_FP64;
+}
+
 void deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32_t* comps, bool ions) {
     if (p->n <= 0) return;
+    if ((comps[0] >= 0) != (comps[1] >= 0)) throw HpqError(HPQ_E_INVALID, "deposit_current: jx and jy go together");
+    if (ordered(c)) {
+        deposit_current_ordered(c, p, store, comps, ions);
+        return;
+    }
     const Geom& g = c->g;
     const double charge = ions ? -c->p.plasma_charge : c->p.plasma_charge;
     const double mu0 = 1.0;
@@ -524,6 +534,10 @@ void explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store) {
     const double qmr = c->p.plasma_charge / c->p.plasma_mass;
     const int T = c->p.derivative_type;
     if (T < 0 || T > 2) throw HpqError(HPQ_E_UNSUPPORTED, "depos_derivative_type must be 0..2");
+    if (ordered(c)) {
+        explicit_deposition_ordered(c, p, store);
+        return;
+    }
     const int32_t* order = deposit_order(c, p);
     TraceScope ts(c, HPQ_TRACE_EXPLICIT_DEPOSIT);
     dispatch_order(c->p.depos_order, [&](auto O) {
@@ -558,6 +572,10 @@ void deposit_beam_select(hpq_ctx* c, const hpq_beam* b, int slice, double* store
     if (count <= 0) return;
     if ((comps[0] >= 0) != (comps[1] >= 0))
         throw HpqError(HPQ_E_INVALID, "deposit_current_slice: jx and jy go together");
+    if (c->p.deposition == HPQ_DEPOSIT_ORDERED) {
+        deposit_beam_ordered(c, b, slice, store, comps, select, select_value);
+        return;
+    }
     TraceScope ts(c, HPQ_TRACE_BEAM_DEPOSIT);
     dispatch_order(c->p.depos_order, [&](auto O) {
         hipLaunchKernelGGL((k_deposit_beam<decltype(O)::value>), grid1d(count, DCHUNK), dim3(DT), 0, c->stream, c->g,

@@ -184,32 +184,47 @@ __global__ void k_capture(Geom g, const double* __restrict__ P, double* __restri
     dst[long(j) * g.nx + i] = P[g.idx(i, j)];
 }
 
-// partial sums of |B| and |B - B_iter| over valid cells
-__global__ void k_relb(Geom g, const double* __restrict__ Bx, const double* __restrict__ By,
-                       const double* __restrict__ Bxi, const double* __restrict__ Byi, double* __restrict__ out) {
-    __shared__ double r0[256], r1[256];
+// The two sums of ComputeRelBFieldError over valid cells in a fixed order, so that the
+// predictor-corrector loop repeats bit for bit: one wavefront per row, lane l adding the cells
+// i = l, l + 64, ... in turn, the lanes combined by a xor butterfly (offsets 32, 16, ..., 1;
+// every lane ends with the same value); then one wavefront does the same over the row sums.
+__device__ __forceinline__ double butterfly64(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_relb_rows(Geom g, const double* __restrict__ Bx, const double* __restrict__ By,
+                                                   const double* __restrict__ Bxi, const double* __restrict__ Byi,
+                                                   double* __restrict__ rows) {
+    const int j = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= g.ny) return;
+    const double factor = g.dx * g.dy / (g.dx * g.dy);
     double a = 0.0, b = 0.0;
-    const long n = long(g.nx) * g.ny;
-    for (long t = long(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += long(gridDim.x) * blockDim.x) {
-        const int j = int(t / g.nx), i = int(t - long(j) * g.nx);
+    for (int i = lane; i < g.nx; i += 64) {
         const long k = g.idx(i, j);
-        const double factor = g.dx * g.dy / (g.dx * g.dy);
         a += factor * sqrt(Bx[k] * Bx[k] + By[k] * By[k]);
         b += factor * sqrt((Bx[k] - Bxi[k]) * (Bx[k] - Bxi[k]) + (By[k] - Byi[k]) * (By[k] - Byi[k]));
     }
-    r0[threadIdx.x] = a;
-    r1[threadIdx.x] = b;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            r0[threadIdx.x] += r0[threadIdx.x + w];
-            r1[threadIdx.x] += r1[threadIdx.x + w];
-        }
-        __syncthreads();
+    a = butterfly64(a);
+    b = butterfly64(b);
+    if (lane == 0) {
+        rows[j] = a;
+        rows[g.ny + j] = b;
     }
+}
+
+__global__ __launch_bounds__(64) void k_relb_total(int ny, const double* __restrict__ rows, double* __restrict__ out) {
+    double a = 0.0, b = 0.0;
+    for (int j = threadIdx.x; j < ny; j += 64) {
+        a += rows[j];
+        b += rows[ny + j];
+    }
+    a = butterfly64(a);
+    b = butterfly64(b);
     if (threadIdx.x == 0) {
-        atomicAdd(out, r0[0]);
-        atomicAdd(out + 1, r1[0]);
+        out[0] = a;
+        out[1] = b;
     }
 }
 
@@ -359,9 +374,16 @@ void diag_finalize(hpq_ctx* c) {
 
 double rel_b_error(hpq_ctx* c, const double* store, int bx, int by, int bxi, int byi) {
     const Geom& g = c->g;
-    HPQ_HIP(hipMemsetAsync(c->d_red, 0, 2 * sizeof(double), c->stream));
-    hipLaunchKernelGGL(k_relb, dim3(64), dim3(256), 0, c->stream, g, store + long(bx) * g.plane,
-                       store + long(by) * g.plane, store + long(bxi) * g.plane, store + long(byi) * g.plane, c->d_red);
+    if (c->relb_cap < 2L * g.ny) {
+        if (c->relb_rows) HPQ_HIP(hipFree(c->relb_rows));
+        c->relb_rows = nullptr;
+        HPQ_HIP(hipMalloc(&c->relb_rows, sizeof(double) * 2 * g.ny));
+        c->relb_cap = 2L * g.ny;
+    }
+    hipLaunchKernelGGL(k_relb_rows, dim3((g.ny + 3) / 4), dim3(256), 0, c->stream, g, store + long(bx) * g.plane,
+                       store + long(by) * g.plane, store + long(bxi) * g.plane, store + long(byi) * g.plane,
+                       c->relb_rows);
+    hipLaunchKernelGGL(k_relb_total, dim3(1), dim3(64), 0, c->stream, g.ny, c->relb_rows, c->d_red);
     HPQ_HIP(hipGetLastError());
     HPQ_HIP(hipMemcpyAsync(c->h_red, c->d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HPQ_HIP(hipStreamSynchronize(c->stream));

@@ -461,6 +461,8 @@ namespace {
 
 template <class F>
 int guarded(hpq_ctx* c, F&& f) {
+    // the caller may have changed the particle arrays since the last call
+    if (c) c->cl_plasma.valid = false;
     try {
         f();
         return HPQ_OK;
@@ -531,6 +533,7 @@ void plasma_save(hpq_ctx* c, const hpq_plasma* p, PlasmaBuf& b) {
 }
 
 void plasma_restore(hpq_ctx* c, const PlasmaBuf& b, const hpq_plasma* p) {
+    c->cl_plasma.valid = false;
     double* a[PLASMA_REALS];
     plasma_reals(p, a);
     for (int k = 0; k < PLASMA_REALS; ++k)
@@ -612,6 +615,8 @@ void adjoint_check(const hpq_ctx* c) {
 
 // the forward pass with checkpoints every k slices
 void adjoint_forward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store, int k) {
+    // a forward pass that stops part way leaves no usable checkpoints
+    c->adj.ready = false;
     adjoint_check(c);
     adjoint_alloc(c, k, p->n);
     AdjointState& A = c->adj;
@@ -733,6 +738,17 @@ int hpq_create(const hpq_params* p, hpq_ctx** out) {
             throw HpqError(HPQ_E_INVALID, "Analytic derivative with depos_order=0 would vanish");
         if (p->salame && !p->explicit_solver)
             throw HpqError(HPQ_E_UNSUPPORTED, "Cannot use SALAME algorithm with predictor-corrector solver");
+        if (p->deposition != HPQ_DEPOSIT_ORDERED && p->deposition != HPQ_DEPOSIT_ATOMIC)
+            throw HpqError(HPQ_E_INVALID, "deposition must be HPQ_DEPOSIT_ORDERED or HPQ_DEPOSIT_ATOMIC");
+        {
+            // the widest stencil the ordered deposition walks: O + 1 cells for the current and
+            // beam deposition, the derivative stencil for the explicit one
+            const int o = p->depos_order, t = p->derivative_type;
+            const int wide = std::max(o + 1, t == 0 ? o + 1 : (t == 1 ? o + 2 : o + 3));
+            const int T = p->tile_size > 0 ? p->tile_size : 32;
+            if (p->tile_size < 0 || T < wide)
+                throw HpqError(HPQ_E_INVALID, "tile_size must be at least the width of the deposition stencil");
+        }
         c->p = *p;
         Geom& g = c->g;
         g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.ng = 2;
@@ -773,6 +789,10 @@ void hpq_destroy(hpq_ctx* c) {
     poisson_free(c);
     mg_free(c);
     bins_free(c);
+    cell_lists_free(c->cl_plasma);
+    cell_lists_free(c->cl_beam);
+    init_scratch_free(c);
+    if (c->relb_rows) (void)hipFree(c->relb_rows);
     if (c->diag_partial) (void)hipFree(c->diag_partial);
     if (c->diag_comp_of) (void)hipFree(c->diag_comp_of);
     if (c->d_count) (void)hipFree(c->d_count);
@@ -1038,7 +1058,13 @@ int hpq_mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf,
 
 int64_t hpq_plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof) {
     if (!c || !prof) return -1;
-    return plasma_count(c, prof);
+    hpq_ctx* mc = const_cast<hpq_ctx*>(c);
+    int64_t n = -1;
+    const int rc = guarded(mc, [&] {
+        if (prof->ppc[0] < 1 || prof->ppc[1] < 1) throw HpqError(HPQ_E_INVALID, "ppc must be >= 1");
+        n = plasma_count(mc, prof);
+    });
+    return rc == HPQ_OK ? n : -1;
 }
 
 int hpq_init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p) {

@@ -160,6 +160,30 @@ struct Bins {
     int calls = 0;               // plasma depositions since the last re-sort
 };
 
+// Per-start-cell particle lists of the ordered deposition (gather.hip): list offsets over a
+// grid of base cells, the particles of every list by increasing index, and the per-particle
+// values the contributions need. valid/base/x/n name the plasma state the lists describe, so
+// the explicit deposition can reuse the lists of the current deposition on the same state.
+struct CellLists {
+    int bxlo = 0, bylo = 0, wx = 0, wy = 0;
+    int T = 32, ntx = 1, nty = 1;
+    int32_t *off = nullptr, *tmp = nullptr, *key = nullptr, *slot = nullptr, *unsorted = nullptr, *perm = nullptr;
+    long off_cap = 0, tmp_cap = 0, key_cap = 0, slot_cap = 0, unsorted_cap = 0, perm_cap = 0;
+    double* rec = nullptr;
+    long rec_cap = 0;
+    bool valid = false;
+    int base = -1;
+    const double* x = nullptr;
+    long n = 0;
+};
+
+// Scratch of the plasma injection: per-cell particle counts and their exclusive scan.
+struct InitScratch {
+    int32_t* off = nullptr;
+    int32_t* tmp = nullptr;
+    long cap = 0;
+};
+
 // Plasma SoA copy (11 FP64 arrays in PlasmaIdx order + valid) for checkpoints.
 constexpr int PLASMA_REALS = 11;
 struct PlasmaBuf {
@@ -209,6 +233,11 @@ struct hpq_ctx {
     hpq::Poisson ps;
     hpq::MultiGrid mg;
     hpq::Bins bins;
+    hpq::CellLists cl_plasma;        // ordered deposition of the plasma (main stream)
+    hpq::CellLists cl_beam;          // ordered deposition of the beam (either stream)
+    hpq::InitScratch init;
+    double* relb_rows = nullptr;     // row sums of ComputeRelBFieldError (2 * ny)
+    long relb_cap = 0;
     hpq::AdjointState adj;
     hpq::BeamSlip slip;
     int* d_count = nullptr;          // QSA counter
@@ -290,12 +319,21 @@ void salame_sums(hpq_ctx* c, const double* store, double out[4]);
 void salame_multiply(hpq_ctx* c, const hpq_beam* b, int slice, double W);
 double rel_b_error(hpq_ctx* c, const double* store, int bx, int by, int bxi, int byi);
 void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p);
-int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof);
+void init_scratch_free(hpq_ctx* c);
+int64_t plasma_count(hpq_ctx* c, const hpq_plasma_profile* prof);
 void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets);
 void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int32_t* perm, int32_t* offsets);
 const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p);
 void bins_free(hpq_ctx* c);
 void sort_keys(hpq_ctx* c, long n, int nbins, int32_t* perm, int32_t* offsets);
+// exclusive scan of a[0..n) in place; tmp: scan_tmp_size(n) entries
+void scan_exclusive(hipStream_t s, int32_t* a, long n, int32_t* tmp);
+long scan_tmp_size(long n);
+void deposit_current_ordered(hpq_ctx* c, const hpq_plasma* p, double* store, const int32_t* comps, bool ions);
+void explicit_deposition_ordered(hpq_ctx* c, const hpq_plasma* p, double* store);
+void deposit_beam_ordered(hpq_ctx* c, const hpq_beam* b, int slice, double* store, const int32_t* comps,
+                          const int32_t* select, int select_value);
+void cell_lists_free(CellLists& L);
 void sort_beam_slices(hpq_ctx* c, hpq_beam_state* b, int64_t* off_out);
 void deposit_vjp(hpq_ctx* c, const hpq_plasma* p, const double* adj, const int32_t* comps, hpq_plasma_adj* a);
 void explicit_deposition_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, double* adj, hpq_plasma_adj* a);

@@ -294,6 +294,7 @@ __global__ void k_fill_i32(int32_t* a, long n, int32_t v) {
 }  // namespace
 
 void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool temp_slice) {
+    c->cl_plasma.valid = false;   // positions change: the deposition lists are out of date
     if (p->n <= 0) return;
     const Layout& L = c->L;
     const double qmc = c->p.plasma_charge / (c->p.plasma_mass * 1.0);
@@ -359,7 +360,7 @@ void advance_beam(hpq_ctx* c, hpq_beam_state* b, int slice, const double* store,
 // ---------------------------------------------------------------------------------
 // plasma injection, PlasmaParticleContainer::InitParticles
 // (src/particles/plasma/PlasmaParticleContainerInit.cpp:18-378): lattice positions
-// (0.5 + k)/ppc per cell, weight n(x,y)/ppc, order i_part outer and cells inner.
+// (0.5 + k)/ppc per cell, weight n(x,y)/ppc, in the CPU build's order (by cell, then i_part).
 // ---------------------------------------------------------------------------------
 namespace {
 
@@ -368,36 +369,52 @@ struct Prof {
     double density, ramp_a, radius;
     double u0, u1, u2;
     double blo0, blo1, bhi0, bhi1;
+    const double* table;   // density at every lattice position, or null
 };
 
-__host__ __device__ __forceinline__ bool inject(const Geom& g, const Prof& pr, int i, int j, int ipart, double& x,
-                                                double& y, double& w) {
+__device__ __forceinline__ bool inject(const Geom& g, const Prof& pr, int i, int j, int ipart, double& x, double& y,
+                                       double& w) {
     const int ixp = ipart % pr.ppc0, iyp = ipart / pr.ppc0;
     const double r0 = (0.5 + ixp) / pr.ppc0, r1 = (0.5 + iyp) / pr.ppc1;
     x = g.xlo + (i + r0 + 0.0) * g.dx;
     y = g.ylo + (j + r1 + 0.0) * g.dy;
     const double rsq = x * x + y * y;
-    const double dens = pr.density * (1.0 + pr.ramp_a * rsq);
+    const double dens = pr.table ? pr.table[(long(j) * g.nx + i) * (pr.ppc0 * pr.ppc1) + ipart]
+                                 : pr.density * (1.0 + pr.ramp_a * rsq);
     if (x >= pr.bhi0 || x < pr.blo0 || y >= pr.bhi1 || y < pr.blo1 || rsq > pr.radius * pr.radius || dens <= 0.0)
         return false;
     w = dens * (1.0 / (pr.ppc0 * pr.ppc1));
     return true;
 }
 
-// one thread per cell of one i_part; slot = offset of the cell among injected cells
-__global__ void k_init_plasma(Geom g, Prof pr, int ipart, const int* __restrict__ slot, long base, hpq_plasma p) {
+// particles the profile injects into cell (i, j)
+__global__ void k_init_count(Geom g, Prof pr, int32_t* __restrict__ cnt) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.nx * g.ny) return;
     const int j = t / g.nx, i = t - j * g.nx;
+    int n = 0;
     double x, y, w;
-    if (!inject(g, pr, i, j, ipart, x, y, w)) return;
-    const long k = base + slot[t];
+    for (int ip = 0; ip < pr.ppc0 * pr.ppc1; ++ip) n += inject(g, pr, i, j, ip, x, y, w) ? 1 : 0;
+    cnt[t] = n;
+}
+
+// one thread per cell writes its particles, i_part in order, from the cell's first slot
+__global__ void k_init_plasma(Geom g, Prof pr, const int32_t* __restrict__ off, hpq_plasma p) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nx * g.ny) return;
+    const int j = t / g.nx, i = t - j * g.nx;
+    long k = off[t];
     const double psi0 = sqrt(1. + pr.u0 * pr.u0 + pr.u1 * pr.u1 + pr.u2 * pr.u2) - pr.u2;
-    p.x[k] = x; p.y[k] = y; p.w[k] = w;
-    p.ux[k] = pr.u0 * 1.0; p.uy[k] = pr.u1 * 1.0; p.psi[k] = psi0;
-    p.x_prev[k] = x; p.y_prev[k] = y;
-    p.ux_half_step[k] = pr.u0 * 1.0; p.uy_half_step[k] = pr.u1 * 1.0; p.psi_half_step[k] = psi0;
-    p.valid[k] = 1;
+    for (int ip = 0; ip < pr.ppc0 * pr.ppc1; ++ip) {
+        double x, y, w;
+        if (!inject(g, pr, i, j, ip, x, y, w)) continue;
+        p.x[k] = x; p.y[k] = y; p.w[k] = w;
+        p.ux[k] = pr.u0 * 1.0; p.uy[k] = pr.u1 * 1.0; p.psi[k] = psi0;
+        p.x_prev[k] = x; p.y_prev[k] = y;
+        p.ux_half_step[k] = pr.u0 * 1.0; p.uy_half_step[k] = pr.u1 * 1.0; p.psi_half_step[k] = psi0;
+        p.valid[k] = 1;
+        ++k;
+    }
 }
 
 Prof make_prof(const hpq_ctx* c, const hpq_plasma_profile* prof) {
@@ -406,48 +423,57 @@ Prof make_prof(const hpq_ctx* c, const hpq_plasma_profile* prof) {
     pr.density = prof->density; pr.ramp_a = prof->ramp_a; pr.radius = prof->radius;
     pr.u0 = prof->u_mean[0]; pr.u1 = prof->u_mean[1]; pr.u2 = prof->u_mean[2];
     pr.blo0 = c->p.plo[0]; pr.blo1 = c->p.plo[1]; pr.bhi0 = c->p.phi[0]; pr.bhi1 = c->p.phi[1];
+    pr.table = prof->density_table;
     return pr;
 }
 
 }  // namespace
 
-int64_t plasma_count(const hpq_ctx* c, const hpq_plasma_profile* prof) {
-    const Prof pr = make_prof(c, prof);
-    int64_t n = 0;
-    double x, y, w;
-    for (int ip = 0; ip < pr.ppc0 * pr.ppc1; ++ip)
-        for (int j = 0; j < c->g.ny; ++j)
-            for (int i = 0; i < c->g.nx; ++i) n += inject(c->g, pr, i, j, ip, x, y, w) ? 1 : 0;
-    return n;
+// The lattice in the CPU build's order: cells x fastest, the i_part of a cell in order (the
+// GPU build's order, i_part outer, sorted by cell: SortParticlesByCell,
+// PlasmaParticleContainerInit.cpp:372-377). Slots: count per cell, exclusive scan, fill; the
+// total is read back once to check the capacity.
+// per-cell counts and their exclusive scan in init.off (ncell + 1 entries, the last the total)
+static void init_slots(hpq_ctx* c, const Prof& pr) {
+    const Geom& g = c->g;
+    const long ncell = long(g.nx) * g.ny;
+    InitScratch& S = c->init;
+    if (S.cap < ncell + 1) {
+        if (S.off) HPQ_HIP(hipFree(S.off));
+        if (S.tmp) HPQ_HIP(hipFree(S.tmp));
+        S.off = S.tmp = nullptr;
+        HPQ_HIP(hipMalloc(&S.off, sizeof(int32_t) * (ncell + 1)));
+        HPQ_HIP(hipMalloc(&S.tmp, sizeof(int32_t) * scan_tmp_size(ncell + 1)));
+        S.cap = ncell + 1;
+    }
+    hipLaunchKernelGGL(k_init_count, grid1d(ncell, 256), dim3(256), 0, c->stream, g, pr, S.off);
+    HPQ_HIP(hipMemsetAsync(S.off + ncell, 0, sizeof(int32_t), c->stream));
+    scan_exclusive(c->stream, S.off, ncell + 1, S.tmp);
+    HPQ_HIP(hipMemcpyAsync(c->h_count + 1, S.off + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HPQ_HIP(hipStreamSynchronize(c->stream));
+}
+
+int64_t plasma_count(hpq_ctx* c, const hpq_plasma_profile* prof) {
+    init_slots(c, make_prof(c, prof));
+    return int64_t(c->h_count[1]);
 }
 
 void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p) {
+    c->cl_plasma.valid = false;
     const Prof pr = make_prof(c, prof);
     const Geom& g = c->g;
-    if (plasma_count(c, prof) != p->n)
+    const long ncell = long(g.nx) * g.ny;
+    init_slots(c, pr);
+    if (long(c->h_count[1]) != p->n)
         throw HpqError(HPQ_E_INVALID, "hpq_init_plasma: plasma capacity does not match the profile");
-    // slots computed on the host (exclusive scan of the injection mask, x fastest)
-    std::vector<int> slot(size_t(g.nx) * g.ny);
-    int* d_slot = nullptr;
-    HPQ_HIP(hipMalloc(&d_slot, sizeof(int) * slot.size()));
-    long base = 0;
-    for (int ip = 0; ip < pr.ppc0 * pr.ppc1; ++ip) {
-        int cnt = 0;
-        double x, y, w;
-        for (int j = 0; j < g.ny; ++j)
-            for (int i = 0; i < g.nx; ++i) {
-                slot[size_t(j) * g.nx + i] = cnt;
-                cnt += inject(g, pr, i, j, ip, x, y, w) ? 1 : 0;
-            }
-        HPQ_HIP(hipMemcpy(d_slot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_init_plasma, grid1d(long(g.nx) * g.ny, 256), dim3(256), 0, c->stream, g, pr, ip, d_slot,
-                           base, *p);
-        HPQ_HIP(hipGetLastError());
-        HPQ_HIP(hipStreamSynchronize(c->stream));
-        base += cnt;
-    }
-    HPQ_HIP(hipFree(d_slot));
-    if (base != p->n) throw HpqError(HPQ_E_INVALID, "hpq_init_plasma: plasma capacity does not match the profile");
+    hipLaunchKernelGGL(k_init_plasma, grid1d(ncell, 256), dim3(256), 0, c->stream, g, pr, c->init.off, *p);
+    HPQ_HIP(hipGetLastError());
+}
+
+void init_scratch_free(hpq_ctx* c) {
+    if (c->init.off) (void)hipFree(c->init.off);
+    if (c->init.tmp) (void)hipFree(c->init.tmp);
+    c->init = InitScratch{};
 }
 
 }  // namespace hpq

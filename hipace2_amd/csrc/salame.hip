@@ -170,6 +170,7 @@ void salame_jxjy_from_b(hpq_ctx* c, double* store) {
 }
 
 void salame_only_advance(hpq_ctx* c, const hpq_plasma* p, double* store) {
+    c->cl_plasma.valid = false;
     if (p->n <= 0) return;
     const Geom& g = c->g;
     const Layout& L = c->L;

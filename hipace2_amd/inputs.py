@@ -60,13 +60,21 @@ class SliceConfig:
     ppc: Tuple[int, int] = (1, 1)
     density: float = 1.0
     ramp_a: float = 0.0
+    # <plasma>.density(x,y,z) when it is an expression rather than a number: evaluated at every
+    # lattice position (z = 0, the start of the run) instead of density * (1 + ramp_a r^2)
+    density_expr: Optional[str] = None
     plasma_radius: float = math.inf
     u_mean: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     plasma_charge: float = -1.0
     plasma_mass: float = 1.0
     max_qsa: float = 35.0
     neutralize: bool = True
-    # plasma deposition walks the particles sorted by 32x8-cell tiles, re-sorted every
+    # order of the FP64 deposition sums: "ordered" (the reference CPU build's tiled order, bit
+    # for bit reproducible) or "atomic" (LDS windows + global atomics, hardware order)
+    deposition: str = "ordered"
+    # hipace.tile_size (src/Hipace.cpp:160): tiles of the CPU build's deposition order
+    tile_size: int = 32
+    # atomic deposition: walks the particles sorted by 32x8-cell tiles, re-sorted every
     # bin_period slices (0: index order); only the order of floating-point sums depends on it
     bin_period: int = 4
     # "fp64": the drop-in path; "mixed": FP32 gather, push increments and deposition with
@@ -471,6 +479,52 @@ def parse_function(expr: str, names):
     return lambda *args: ev(tree, dict(zip(names, (float(a) for a in args))))
 
 
+_NP_FUNCS = {"sqrt": np.sqrt, "exp": np.exp, "log": np.log, "sin": np.sin, "cos": np.cos, "tan": np.tan,
+             "abs": np.abs, "min": np.minimum, "max": np.maximum, "pow": np.power}
+
+
+def parse_function_np(expr: str, names):
+    """parse_function for numpy arrays: the same syntax and checks, elementwise evaluation in
+    the tree's order (every operation rounds as the scalar evaluation does)."""
+    parse_function(expr, names)
+    tree = ast.parse(expr.strip().strip('"').replace("^", "**"), mode="eval")
+    ops = {ast.Add: np.add, ast.Sub: np.subtract, ast.Mult: np.multiply, ast.Div: np.divide, ast.Pow: np.power}
+
+    def ev(node, env):
+        if isinstance(node, ast.Expression):
+            return ev(node.body, env)
+        if isinstance(node, ast.Constant):
+            return np.float64(node.value)
+        if isinstance(node, ast.Name):
+            return env[node.id]
+        if isinstance(node, ast.UnaryOp):
+            v = ev(node.operand, env)
+            return np.negative(v) if isinstance(node.op, ast.USub) else v
+        if isinstance(node, ast.BinOp):
+            return ops[type(node.op)](ev(node.left, env), ev(node.right, env))
+        return _NP_FUNCS[node.func.id](*[ev(a, env) for a in node.args])
+
+    return lambda *args: ev(tree, dict(zip(names, (np.asarray(a, dtype=np.float64) for a in args))))
+
+
+def lattice_density(cfg: "SliceConfig", z: float = 0.0) -> np.ndarray:
+    """<plasma>.density(x,y,z) at every lattice position of the injection, in the layout of
+    hpq_plasma_profile.density_table: entry (j*nx + i)*nppc + i_part, i_part = ix + iy*ppc_x,
+    x = xlo + (i + (0.5 + ix)/ppc_x)*dx (ParticleUtil.H:65-104; the same rounding as the
+    injection kernel)."""
+    px, py = cfg.ppc
+    f = parse_function_np(cfg.density_expr, ["x", "y", "z"])
+    ip = np.arange(px * py)
+    r0 = (0.5 + (ip % px)) / px
+    r1 = (0.5 + (ip // px)) / py
+    i = np.arange(cfg.nx, dtype=np.float64)
+    j = np.arange(cfg.ny, dtype=np.float64)
+    x = cfg.xlo + (i[None, :, None] + r0[None, None, :] + 0.0) * cfg.dx
+    y = cfg.ylo + (j[:, None, None] + r1[None, None, :] + 0.0) * cfg.dy
+    x, y = np.broadcast_arrays(x, y)
+    return np.ascontiguousarray(np.broadcast_to(f(x, y, np.full(x.shape, z)), x.shape)).reshape(-1)
+
+
 def parse_deck(text: str, overrides: Optional[Dict[str, str]] = None):
     """Parse a reference input deck into (SliceConfig, [BeamSpec], max_step)."""
     kv = parse_kv(text)
@@ -493,6 +547,7 @@ def parse_deck(text: str, overrides: Optional[Dict[str, str]] = None):
     cfg.pc_maxiter = int(_num(kv.get("hipace.predcorr_max_iterations", "30")))
     cfg.pc_mix = _num(kv.get("hipace.predcorr_B_mixing_factor", "0.05"))
     cfg.do_beam_jx_jy = _num(kv.get("hipace.do_beam_jx_jy_deposition", "1")) != 0
+    cfg.tile_size = int(_num(kv.get("hipace.tile_size", "32")))
     cfg.particle_bc = kv.get("boundary.particle", "Absorbing").strip('"')
     fd = kv.get("diagnostic.field_data", "")
     cfg.deposit_rho = ("rho" in _tokens(fd)) or _num(kv.get("hipace.deposit_rho", "0")) != 0
@@ -502,7 +557,11 @@ def parse_deck(text: str, overrides: Optional[Dict[str, str]] = None):
             raise ValueError("one plasma species on this path")
         p = pnames[0]
         dens = kv.get(f"{p}.density(x,y,z)", kv.get(f"{p}.density", "1.")).strip('"')
-        cfg.density = _num(dens)
+        try:
+            cfg.density = _num(dens)
+        except ValueError:
+            parse_function(dens, ["x", "y", "z"])   # rejects anything but the parser syntax
+            cfg.density_expr = dens
         cfg.ppc = tuple(int(_num(t)) for t in _tokens(kv.get(f"{p}.ppc", "1 1")))
         um = kv.get(f"{p}.u_mean", "0 0 0")
         cfg.u_mean = tuple(_num(t) for t in _tokens(um))

@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from .inputs import BC_CODES, BeamArrays, SliceConfig
+from .inputs import BC_CODES, BeamArrays, SliceConfig, lattice_density
 
 PLASMA_REALS = ["x", "y", "w", "ux", "uy", "psi", "x_prev", "y_prev", "ux_half_step", "uy_half_step",
                 "psi_half_step"]
@@ -40,6 +40,8 @@ def make_params(cfg: SliceConfig) -> _abi.Params:
     p.precision = _abi.PRECISION[cfg.precision]
     p.poisson_solver = _abi.POISSON_SOLVER[cfg.poisson_solver]
     p.salame = int(cfg.salame)
+    p.deposition = _abi.DEPOSITION[cfg.deposition]
+    p.tile_size = cfg.tile_size
     return p
 
 
@@ -74,6 +76,10 @@ class Simulation:
         self.store = torch.zeros((self.ncomp, self.ldy, self.ldx), dtype=torch.float64, device=self.device)
         # plasma SoA
         self._profile = make_profile(cfg)
+        if cfg.has_plasma and cfg.density_expr is not None:
+            # <plasma>.density(x,y,z) evaluated at every lattice position
+            self._density_table = torch.as_tensor(lattice_density(cfg)).to(self.device)
+            self._profile.density_table = self._density_table.data_ptr()
         n = int(self.L.hpq_plasma_count(self.ctx, C.byref(self._profile))) if cfg.has_plasma else 0
         self.np = n
         self.plasma = {k: torch.zeros(max(n, 1), dtype=torch.float64, device=self.device) for k in PLASMA_REALS}
@@ -390,8 +396,12 @@ class Simulation:
             self.ctx, C.byref(self._pdesc), C.c_void_p(self.store.data_ptr()), C.byref(o), C.byref(i),
             C.c_void_p(store_adj.data_ptr())))
 
+    adjoint_generation = 0
+
     def adjoint_forward(self, checkpoint_every: int = 32):
-        """One step's slices with checkpoints for the backward march (after begin_step)."""
+        """One step's slices with checkpoints for the backward march (after begin_step); every
+        call starts a new generation of checkpoints (adjoint_generation)."""
+        self.adjoint_generation += 1
         _abi.check(self.ctx, self.L.hpq_adjoint_forward(self.ctx, C.byref(self._pdesc), C.byref(self._bdesc),
                                                         C.c_void_p(self.store.data_ptr()), int(checkpoint_every)))
 

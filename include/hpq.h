@@ -41,6 +41,15 @@ enum { HPQ_BC_REFLECTING = 0, HPQ_BC_PERIODIC = 1, HPQ_BC_ABSORBING = 2 };
  * FP64 multigrid). */
 enum { HPQ_PREC_FP64 = 0, HPQ_PREC_MIXED = 1 };
 
+/* Order of the FP64 deposition sums (plasma current, explicit Sx/Sy, beam current). ORDERED:
+ * the order of the reference CPU build's tiled deposition (32-cell tiles in four colours,
+ * particles by index inside a tile; src/particles/deposition/DepositionUtil.H:204-253), every
+ * cell summed by one thread, no atomics: results repeat bit for bit, and equal a CPU build that
+ * sums in that order. ATOMIC: LDS windows flushed with global atomics, sums in hardware order
+ * (differences of a few ulp per cell between runs). The mixed-precision path always uses the
+ * atomic kernels. */
+enum { HPQ_DEPOSIT_ORDERED = 0, HPQ_DEPOSIT_ATOMIC = 1 };
+
 /* Transverse Dirichlet Poisson solver (fields.poisson_solver, src/fields/Fields.cpp:179-208):
  * DST-I by FFT in FP64, or DST-I as dense sine-matrix GEMMs on the matrix cores with each
  * FP32 operand split into two BF16 terms (hi + lo; three products hi*hi + hi*lo + lo*hi,
@@ -67,7 +76,7 @@ typedef struct hpq_params {
     double pc_mix;                       /* hipace.predcorr_B_mixing_factor */
     double plasma_charge, plasma_mass;   /* species charge and mass */
     double max_qsa;                      /* plasma.max_qsa_weighting_factor */
-    int bin_period;                      /* plasma deposition walks the particles sorted by
+    int bin_period;                      /* atomic deposition: walks the particles sorted by
                                             32x8-cell tiles, re-sorted every bin_period
                                             slices (plasmas.reorder_period plays this role
                                             in the reference); 0: index order. Changes only
@@ -76,6 +85,10 @@ typedef struct hpq_params {
     int poisson_solver;                  /* HPQ_POISSON_FFT or HPQ_POISSON_BF16X3 */
     int salame;                          /* a beam has <beam>.do_salame: allocate the SALAME
                                             planes (explicit solver only) */
+    int deposition;                      /* HPQ_DEPOSIT_ORDERED or HPQ_DEPOSIT_ATOMIC */
+    int tile_size;                       /* hipace.tile_size: tile width of the ordered deposition
+                                            (src/Hipace.cpp:160); 0: 32, the reference's default;
+                                            at least the widest deposition stencil */
 } hpq_params;
 
 /* Plasma species, SoA device arrays in PlasmaIdx order
@@ -121,11 +134,18 @@ typedef struct hpq_plasma_adj {
     double *x, *y, *ux, *uy, *psi, *ux_half_step, *uy_half_step, *psi_half_step;
 } hpq_plasma_adj;
 
-/* Plasma injection profile: n(x,y) = density * (1 + ramp_a (x^2+y^2)) inside `radius`. */
+/* Plasma injection profile: n(x,y) = density * (1 + ramp_a (x^2+y^2)) inside `radius`, or,
+ * when density_table is not NULL, the density the caller evaluated at every lattice position
+ * (<plasma>.density(x,y,z), a parser expression in the reference,
+ * src/particles/plasma/PlasmaParticleContainer.cpp:90-97): device array of nx*ny*ppc[0]*ppc[1]
+ * doubles, entry (j*nx + i)*ppc[0]*ppc[1] + i_part for the particle i_part = ix + iy*ppc[0] of
+ * cell (i, j), at x = xlo + (i + (0.5 + ix)/ppc[0])*dx, y likewise
+ * (ParticleUtil.H:65-104). Positions where the density is <= 0 inject nothing. */
 typedef struct hpq_plasma_profile {
     int ppc[2];
     double density, ramp_a, radius;
     double u_mean[3];
+    const double *density_table;
 } hpq_plasma_profile;
 
 /* Number of hpq_layout() entries. */
@@ -256,10 +276,13 @@ int hpq_mg_solve1(hpq_ctx *ctx, double *sol, const double *rhs, const double *ac
 
 /* ---- slice loop (Hipace::Evolve / SolveOneSlice) ---------------------------------- */
 
-/* Number of plasma particles the profile injects (PlasmaParticleContainerInit.cpp:143-176). */
+/* Number of plasma particles the profile injects (PlasmaParticleContainerInit.cpp:143-176);
+ * -1 on error. Synchronises the context's stream. */
 int64_t hpq_plasma_count(const hpq_ctx *ctx, const hpq_plasma_profile *prof);
 /* PlasmaParticleContainer::InitParticles (PlasmaParticleContainerInit.cpp:18-378): fills
- * p (capacity p->n must equal hpq_plasma_count) with the lattice in GPU order. */
+ * p (capacity p->n must equal hpq_plasma_count) with the lattice in the CPU build's order:
+ * cells x fastest, the particles of a cell in i_part order (the GPU build's injection order
+ * after SortParticlesByCell, :372-377). */
 int hpq_init_plasma(hpq_ctx *ctx, const hpq_plasma_profile *prof, const hpq_plasma *p);
 
 /* Tile binning of the plasma: the DenseBins of the CPU tiled deposition,

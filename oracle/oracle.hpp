@@ -9,6 +9,11 @@
 // order as the reference expression (compiled with -ffp-contract=off), so results
 // agree with the reference CPU build to rounding of the summation order only.
 //
+// Floating-point sums run in fixed orders: deposition in the CPU build's tiled four-colour order
+// (particles.cpp), the DST with rows paired as the product pairs them (fields.cpp), reductions
+// in the blocked orders the product uses. The product's FP64 path runs the same arithmetic, so
+// the two agree bit for bit.
+//
 // Parity pins (see tests/): golden checksums of the reference's own CPU/double CI
 // (tests/checksum/benchmarks_json/*.json), closed forms (DST eigenmodes, B-spline
 // moments, linear-field gather, uniform-field push), brute force on tiny inputs.
@@ -104,6 +109,7 @@ struct Options {
     int pc_maxiter = 30;          // predcorr_max_iterations
     double pc_mix = 0.05;         // predcorr_B_mixing_factor
     int nthreads = 0;             // OpenMP threads (0: default)
+    int tile_size = 32;           // hipace.tile_size (src/Hipace.H:240)
 };
 
 // ---- shape factors: src/particles/particles_utils/ShapeFactors.H ----
@@ -144,6 +150,8 @@ void push_beam_slice(const Ctx& c, const double* F, const BeamRef& b, long first
                      bool do_z_push);
 void dst1(const double* in, double* out, int n, int stride_in, int stride_out);
 void poisson_dirichlet(const Geom& g, const double* staging, double* lhs);  // staging: nx*ny
+// nf right-hand sides solved together (their rows paired as one list)
+void poisson_dirichlet_n(const Geom& g, int nf, const double* const* staging, double* const* lhs);
 void solve_poisson_psi_ez_bz(const Ctx& c, double* F);
 void solve_poisson_bxby(const Ctx& c, double* F);
 void initialize_sxsy_with_beam(const Ctx& c, double* F);

@@ -39,7 +39,7 @@ class OrcConfig(C.Structure):
                 ("density", C.c_double), ("ramp_a", C.c_double), ("plasma_radius", C.c_double),
                 ("u_mean", C.c_double * 3),
                 ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double),
-                ("neutralize", C.c_int), ("nthreads", C.c_int), ("salame", C.c_int)]
+                ("neutralize", C.c_int), ("nthreads", C.c_int), ("salame", C.c_int), ("tile_size", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -50,18 +50,29 @@ def build(force: bool = False) -> str:
 
 
 _lib = None
+_lib_path = LIB_PATH
+
+
+def use_library(path: str) -> None:
+    """Load the oracle from another build of the same sources (e.g. one compiled for the host
+    it runs on); must come before the first call into the oracle."""
+    global _lib_path
+    if _lib is not None and path != _lib_path:
+        raise RuntimeError("oracle library already loaded")
+    _lib_path = path
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if _lib_path == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(_lib_path)
         vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
         L.orc_sim_create.restype = vp
         L.orc_sim_create.argtypes = [C.POINTER(OrcConfig)]
         L.orc_sim_destroy.argtypes = [vp]
+        L.orc_sim_set_density_table.argtypes = [vp, dp]
         L.orc_sim_ncomp.argtypes = [vp]
         L.orc_sim_layout.argtypes = [vp, ip]
         L.orc_sim_set_beam.argtypes = [vp, C.c_int64] + [dp] * 7 + [C.POINTER(C.c_int32), C.POINTER(C.c_int64),
@@ -136,13 +147,16 @@ def make_config(cfg, nthreads: int = 0) -> OrcConfig:
     c.neutralize = int(cfg.neutralize)
     c.nthreads = nthreads
     c.salame = int(getattr(cfg, "salame", False))
+    c.tile_size = int(getattr(cfg, "tile_size", 32))
     return c
 
 
 class OracleSim:
     """One slice-loop simulation on the CPU (the checker)."""
 
-    def __init__(self, cfg, beam=None, nthreads: int = 0):
+    def __init__(self, cfg, beam=None, nthreads: int = 0, density_table=None):
+        """density_table: <plasma>.density(x,y,z) at every lattice position (the layout of
+        hpq_plasma_profile.density_table), required when cfg.density_expr is set."""
         self.cfg = cfg
         self._c = make_config(cfg, nthreads)
         self.h = lib().orc_sim_create(C.byref(self._c))
@@ -156,6 +170,14 @@ class OracleSim:
         self._beam = None
         if beam is not None:
             self.set_beam(beam)
+        self._density_table = None
+        if getattr(cfg, "density_expr", None) is not None and density_table is None:
+            raise ValueError("a density expression needs its lattice table")
+        if density_table is not None:
+            self._density_table = np.ascontiguousarray(density_table, dtype=np.float64)
+            if self._density_table.size != cfg.nx * cfg.ny * cfg.ppc[0] * cfg.ppc[1]:
+                raise ValueError("density table size")
+            lib().orc_sim_set_density_table(self.h, _dp(self._density_table))
 
     def __del__(self):
         try:

@@ -1,13 +1,59 @@
 // Oracle particle operations (TEST INFRASTRUCTURE ONLY, see oracle.hpp).
 #include "oracle.hpp"
+#include <algorithm>
 
 namespace orc {
 
-// ::DepositCurrent, src/particles/deposition/PlasmaDepositCurrent.cpp:22-259, deposit
-// functor :192-251, in the order of the simple particle loop of SharedMemoryDeposition
-// (src/particles/deposition/DepositionUtil.H:256-264). QSA violators get w=0 and are
-// invalidated (:234-241). `ions` flips the charge for the RhomJzIons background (:41).
-// Component indices < 0 are skipped.
+// The CPU build's deposition engine: SharedMemoryDeposition with tiling
+// (src/particles/deposition/DepositionUtil.H:204-253; Hipace::m_do_tiling is true in an OpenMP
+// build, tile width T = hipace.tile_size, 32 by default, src/Hipace.H:235-240). Particles are
+// binned by the tile of the lowest cell they deposit into, key ((cx - lo)/T) * ntile_y + (cy - lo)/T
+// with lo the first guard cell and C integer division, invalid particles last
+// (amrex::DenseBins::build, a stable counting sort); then the tiles are visited in four colours,
+// (even, even), (even, odd), (odd, even), (odd, odd) tile indices, and inside a tile the particles
+// in bin order. Tiles of one colour are at least T cells apart (T is at least the stencil width),
+// so they run in parallel without touching the same cell, and every cell receives its
+// contributions in one fixed order whatever the thread count. The tile index is clamped to the
+// tile grid (the reference indexes out of it for particles outside the guarded box, whose
+// stencils deposit nothing into the box).
+template <class StartF, class DepF>
+static void tiled_deposit(const Geom& g, int T, long n, StartF start, DepF dep) {
+    const int lo = -g.ng;
+    const int ntx = (g.nx + 2 * g.ng + T - 1) / T, nty = (g.ny + 2 * g.ng + T - 1) / T;
+    const int nbins = ntx * nty + 1;
+    std::vector<int> key(size_t(std::max(n, 0L)));
+    for (long ip = 0; ip < n; ++ip) {
+        int cx = 0, cy = 0;
+        if (start(ip, cx, cy)) {
+            const int tx = std::min(std::max((cx - lo) / T, 0), ntx - 1);
+            const int ty = std::min(std::max((cy - lo) / T, 0), nty - 1);
+            key[ip] = tx * nty + ty;
+        } else {
+            key[ip] = ntx * nty;
+        }
+    }
+    std::vector<long> off(size_t(nbins) + 1, 0), perm(size_t(std::max(n, 0L)));
+    for (long ip = 0; ip < n; ++ip) ++off[key[ip] + 1];
+    for (int b = 0; b < nbins; ++b) off[b + 1] += off[b];
+    {
+        std::vector<long> nxt(off.begin(), off.end() - 1);
+        for (long ip = 0; ip < n; ++ip) perm[nxt[key[ip]]++] = ip;
+    }
+    for (int px = 0; px < 2; ++px)
+        for (int py = 0; py < 2; ++py) {
+#pragma omp parallel for collapse(2) schedule(dynamic)
+            for (int tx = px; tx < ntx; tx += 2)
+                for (int ty = py; ty < nty; ty += 2) {
+                    const int t = tx * nty + ty;
+                    for (long k = off[t]; k < off[t + 1]; ++k) dep(perm[k]);
+                }
+        }
+}
+
+// ::DepositCurrent, src/particles/deposition/PlasmaDepositCurrent.cpp:22-259: start cell
+// (get_start_cell, :137-155) and deposit functor (:192-251) in the tiled order above. QSA
+// violators get w=0 and are invalidated (:234-241). `ions` flips the charge for the RhomJzIons
+// background (:41). Component indices < 0 are skipped.
 void deposit_plasma(const Ctx& c, Plasma& p, double* F, int cjx, int cjy, int cjz, int crho,
                     int cchi, int crhomjz, bool ions, long* n_qsa) {
     const Geom& g = c.g;
@@ -27,8 +73,13 @@ void deposit_plasma(const Ctx& c, Plasma& p, double* F, int cjx, int cjy, int cj
     double* pchi = cchi >= 0 ? F + cchi * g.plane : nullptr;
     double* prm = crhomjz >= 0 ? F + crhomjz * g.plane : nullptr;
     const int lo = -g.ng, hix = g.nx - 1 + g.ng, hiy = g.ny - 1 + g.ng;
-    for (size_t ip = 0; ip < p.size(); ++ip) {
-        if (!p.valid[ip]) continue;
+    auto start = [&](long ip, int& cx, int& cy) {
+        if (!p.valid[ip]) return false;
+        cx = shape(order, (p.x[ip] - g.xoff) * g.dxi, 0).cell;
+        cy = shape(order, (p.y[ip] - g.yoff) * g.dyi, 0).cell;
+        return true;
+    };
+    auto dep = [&](long ip) {
         const double psi_inv = 1.0 / p.psi[ip];
         const double xp = p.x[ip], yp = p.y[ip];
         const double vx_c = p.ux[ip] * psi_inv;
@@ -42,10 +93,11 @@ void deposit_plasma(const Ctx& c, Plasma& p, double* F, int cjx, int cjy, int cj
                                         vx_c * vx_c * clightinv * clightinv +
                                         vy_c * vy_c * clightinv * clightinv + 1.0);
         if (gamma_psi < 0.0 || gamma_psi > p.max_qsa || psi_inv < 0.0) {
+#pragma omp atomic
             ++nq;
             p.w[ip] = 0.0;
             p.valid[ip] = 0;
-            continue;
+            return;
         }
         for (int iy = 0; iy <= order; ++iy) {
             for (int ix = 0; ix <= order; ++ix) {
@@ -67,13 +119,15 @@ void deposit_plasma(const Ctx& c, Plasma& p, double* F, int cjx, int cjy, int cj
                 if (prm) prm[k] += wrhomjz;
             }
         }
-    }
+    };
+    tiled_deposit(g, c.o.tile_size, long(p.size()), start, dep);
     if (n_qsa) *n_qsa = nq;
 }
 
-// ::ExplicitDeposition, src/particles/deposition/ExplicitDeposition.cpp:20-265
-// (deposit functor :166-262, no laser): Sy, Sx from plasma using the cell-local
-// Bz, Ez, ExmBy, EypBx and derivative shape factors of type depos_derivative_type.
+// ::ExplicitDeposition, src/particles/deposition/ExplicitDeposition.cpp:20-265: start cell
+// (get_cell, :109-141) and deposit functor (:143-262, no laser) in the tiled order above: Sy,
+// Sx from plasma using the cell-local Bz, Ez, ExmBy, EypBx and derivative shape factors of type
+// depos_derivative_type.
 void explicit_deposit(const Ctx& c, const Plasma& p, double* F) {
     const Geom& g = c.g;
     const Layout& L = c.L;
@@ -90,8 +144,13 @@ void explicit_deposit(const Ctx& c, const Plasma& p, double* F) {
     double* Sx = F + L.Sx * g.plane;
     const int ns = order + dtype;
     const int lo = -g.ng, hix = g.nx - 1 + g.ng, hiy = g.ny - 1 + g.ng;
-    for (size_t ip = 0; ip < p.size(); ++ip) {
-        if (!p.valid[ip]) continue;
+    auto start = [&](long ip, int& cx, int& cy) {
+        if (!p.valid[ip]) return false;
+        cy = dshape(dtype, order, (p.y[ip] - g.yoff) * g.dyi, 0).cell;
+        cx = dshape(dtype, order, (p.x[ip] - g.xoff) * g.dxi, 0).cell;
+        return true;
+    };
+    auto dep = [&](long ip) {
         const double psi_inv = 1.0 / p.psi[ip];
         const double xp = p.x[ip], yp = p.y[ip];
         const double vx = p.ux[ip] * psi_inv * clight_inv;
@@ -134,11 +193,13 @@ void explicit_deposit(const Ctx& c, const Plasma& p, double* F) {
                        + sx.s * sy.ds * g.dyi * (-vx * vy)) * clight);
             }
         }
-    }
+    };
+    tiled_deposit(g, c.o.tile_size, long(p.size()), start, dep);
 }
 
-// ::DepositCurrentSlice, src/particles/deposition/BeamDepositCurrent.cpp:22-196:
-// beam particles of one slice deposit jx, jy, jz (rhomjz of the beam not modelled).
+// ::DepositCurrentSlice, src/particles/deposition/BeamDepositCurrent.cpp:22-196: the beam
+// particles of one slice deposit jx, jy, jz (rhomjz of the beam not modelled), start cell
+// (get_cell, :109-127) and deposit functor (:128-193) in the tiled order above.
 void deposit_beam(const Ctx& c, const Beam& b, int slice, double* F, int cjx, int cjy, int cjz,
                   const int32_t* select, int select_value) {
     const Geom& g = c.g;
@@ -152,9 +213,17 @@ void deposit_beam(const Ctx& c, const Beam& b, int slice, double* F, int cjx, in
     (void)clightinv;
     const int lo = -g.ng, hix = g.nx - 1 + g.ng, hiy = g.ny - 1 + g.ng;
     if (slice < 0 || slice + 1 >= int(b.off.size())) return;
-    for (int64_t ip = b.off[slice]; ip < b.off[slice + 1]; ++ip) {
-        if (!b.valid[ip]) continue;
-        if (select && select[ip] != select_value) continue;
+    const int64_t first = b.off[slice];
+    auto start = [&](long t, int& cx, int& cy) {
+        const int64_t ip = first + t;
+        if (!b.valid[ip]) return false;
+        if (select && select[ip] != select_value) return false;
+        cx = shape(order, (b.x[ip] - g.xoff) * g.dxi, 0).cell;
+        cy = shape(order, (b.y[ip] - g.yoff) * g.dyi, 0).cell;
+        return true;
+    };
+    auto dep = [&](long t) {
+        const int64_t ip = first + t;
         const double ux = b.ux[ip], uy = b.uy[ip], uz = b.uz[ip];
         const double gaminv = 1.0 / std::sqrt(1.0 + ux * ux * clightsq + uy * uy * clightsq + uz * uz * clightsq);
         const double wq = q * b.w[ip] * invvol;
@@ -172,7 +241,8 @@ void deposit_beam(const Ctx& c, const Beam& b, int slice, double* F, int cjx, in
                 if (pjz) pjz[k] += sx.s * sy.s * wqz;
             }
         }
-    }
+    };
+    tiled_deposit(g, c.o.tile_size, long(b.off[slice + 1] - first), start, dep);
 }
 
 // Forward-mode dual number (value, derivative) for the second-order Taylor step of

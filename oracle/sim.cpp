@@ -15,6 +15,8 @@ struct PlasmaProfile {
     double ramp_a = 0.0;    // n(x,y) = n0 (1 + ramp_a (x^2+y^2))
     double radius = INFINITY;
     double u_mean[3] = {0, 0, 0};
+    // the density at every lattice position, [(j*nx + i)*nppc + i_part], or empty
+    std::vector<double> table;
 };
 
 struct Sim {
@@ -58,8 +60,10 @@ static void add_plane(Sim& s, int dst, int src) {
 }
 
 // PlasmaParticleContainer::InitParticles, src/particles/plasma/PlasmaParticleContainerInit.cpp:18-378,
-// without fine patch / symmetrisation / hollow core; u_std = 0. The particle order is
-// i_part outer, cells inner (x fastest), as on GPU (:189-192, no CPU re-sort).
+// without fine patch / symmetrisation / hollow core; u_std = 0. The lattice is injected with
+// i_part outer and cells inner (:189-192); the CPU build then sorts the particles by cell,
+// x fastest, keeping the i_part order inside a cell (SortParticlesByCell, :372-377, a stable
+// amrex::DenseBins sort). That is the order here: cells (x fastest), then i_part.
 static void init_plasma(Sim& s) {
     const Geom& g = s.c.g;
     const PlasmaProfile& pr = s.prof;
@@ -68,21 +72,21 @@ static void init_plasma(Sim& s) {
     const double* blo = s.c.o.plo;
     const double* bhi = s.c.o.phi;
     std::vector<double> X, Y, W;
-    for (int ipart = 0; ipart < nppc; ++ipart) {
-        const int ixp = ipart % pr.ppc[0], iyp = ipart / pr.ppc[0];
-        const double r0 = (0.5 + ixp) / pr.ppc[0], r1 = (0.5 + iyp) / pr.ppc[1];
-        for (int j = 0; j < g.ny; ++j)
-            for (int i = 0; i < g.nx; ++i) {
+    for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i)
+            for (int ipart = 0; ipart < nppc; ++ipart) {
+                const int ixp = ipart % pr.ppc[0], iyp = ipart / pr.ppc[0];
+                const double r0 = (0.5 + ixp) / pr.ppc[0], r1 = (0.5 + iyp) / pr.ppc[1];
                 const double x = g.xlo + (i + r0 + 0.0) * g.dx;
                 const double y = g.ylo + (j + r1 + 0.0) * g.dy;
                 const double rsq = x * x + y * y;
-                const double dens = pr.density * (1.0 + pr.ramp_a * rsq);
+                const double dens = pr.table.empty() ? pr.density * (1.0 + pr.ramp_a * rsq)
+                                                     : pr.table[(size_t(j) * g.nx + i) * nppc + ipart];
                 if (x >= bhi[0] || x < blo[0] || y >= bhi[1] || y < blo[1] || rsq > pr.radius * pr.radius ||
                     dens <= 0.0)
                     continue;
                 X.push_back(x); Y.push_back(y); W.push_back(dens * scale_fac);
             }
-    }
     Plasma& p = s.plasma;
     p.resize(X.size());
     const double u0 = pr.u_mean[0], u1 = pr.u_mean[1], u2 = pr.u_mean[2];
@@ -113,19 +117,51 @@ static void accumulate_diag(Sim& s, int islice) {
     }
 }
 
-// Fields::ComputeRelBFieldError, src/fields/Fields.cpp:1229-1286
+// Sum of 64 lane values in the order of a xor butterfly (offsets 32, 16, ..., 1): every lane
+// ends with the same value; this is lane 0's.
+static double butterfly64(double v[64]) {
+    for (int off = 32; off > 0; off >>= 1) {
+        double t[64];
+        for (int l = 0; l < 64; ++l) t[l] = v[l] + v[l ^ off];
+        for (int l = 0; l < 64; ++l) v[l] = t[l];
+    }
+    return v[0];
+}
+
+// Fields::ComputeRelBFieldError, src/fields/Fields.cpp:1229-1286: the two sums over valid
+// cells in a fixed order (the reference's ReduceOps order depends on its thread count): per
+// row, lane l of 64 adds the cells i = l, l + 64, ... in turn, the lanes are combined by
+// butterfly64; the row sums are combined the same way, lane l adding rows l, l + 64, ...
 static double rel_b_error(Sim& s, int bx, int by, int bxi, int byi) {
     const Geom& g = s.c.g;
     const double* Bx = s.F.data() + bx * g.plane; const double* By = s.F.data() + by * g.plane;
     const double* Bxi = s.F.data() + bxi * g.plane; const double* Byi = s.F.data() + byi * g.plane;
-    double nB = 0, nD = 0;
     const double factor = g.dx * g.dy / (g.dx * g.dy);
-    for (int j = 0; j < g.ny; ++j)
-        for (int i = 0; i < g.nx; ++i) {
-            const long k = g.idx(i, j);
-            nB += factor * std::sqrt(Bx[k] * Bx[k] + By[k] * By[k]);
-            nD += factor * std::sqrt((Bx[k] - Bxi[k]) * (Bx[k] - Bxi[k]) + (By[k] - Byi[k]) * (By[k] - Byi[k]));
+    std::vector<double> rb(g.ny), rd(g.ny);
+    for (int j = 0; j < g.ny; ++j) {
+        double a[64], b[64];
+        for (int l = 0; l < 64; ++l) {
+            a[l] = 0.0;
+            b[l] = 0.0;
+            for (int i = l; i < g.nx; i += 64) {
+                const long k = g.idx(i, j);
+                a[l] += factor * std::sqrt(Bx[k] * Bx[k] + By[k] * By[k]);
+                b[l] += factor * std::sqrt((Bx[k] - Bxi[k]) * (Bx[k] - Bxi[k]) + (By[k] - Byi[k]) * (By[k] - Byi[k]));
+            }
         }
+        rb[j] = butterfly64(a);
+        rd[j] = butterfly64(b);
+    }
+    double a[64], b[64];
+    for (int l = 0; l < 64; ++l) {
+        a[l] = 0.0;
+        b[l] = 0.0;
+        for (int j = l; j < g.ny; j += 64) {
+            a[l] += rb[j];
+            b[l] += rd[j];
+        }
+    }
+    const double nB = butterfly64(a), nD = butterfly64(b);
     return (nB > 0.) ? nD / nB : 0.;
 }
 
@@ -397,6 +433,7 @@ struct orc_config {
     int neutralize;
     int nthreads;
     int salame;
+    int tile_size;
 };
 
 static Geom geom_from(const orc_config* cf) {
@@ -419,6 +456,7 @@ static Ctx ctx_from(const orc_config* cf) {
     o.plo[0] = cf->plo[0]; o.plo[1] = cf->plo[1]; o.phi[0] = cf->phi[0]; o.phi[1] = cf->phi[1];
     o.pc_tol = cf->pc_tol; o.pc_maxiter = cf->pc_maxiter; o.pc_mix = cf->pc_mix;
     o.nthreads = cf->nthreads;
+    o.tile_size = cf->tile_size > 0 ? cf->tile_size : 32;
     c.L = make_layout(o.explicit_solver, o.deposit_rho, cf->has_plasma && cf->neutralize, cf->salame != 0);
     return c;
 }
@@ -442,6 +480,16 @@ void* orc_sim_create(const orc_config* cf) {
 }
 
 void orc_sim_destroy(void* h) { delete static_cast<Sim*>(h); }
+
+// <plasma>.density(x,y,z) at every lattice position (nx*ny*nppc doubles, cell (i, j) particle
+// i_part at (j*nx + i)*nppc + i_part); NULL: the analytic profile
+void orc_sim_set_density_table(void* h, const double* table) {
+    Sim* s = static_cast<Sim*>(h);
+    const Geom& g = s->c.g;
+    const size_t n = size_t(g.nx) * g.ny * s->prof.ppc[0] * s->prof.ppc[1];
+    if (table) s->prof.table.assign(table, table + n);
+    else s->prof.table.clear();
+}
 
 int orc_sim_ncomp(void* h) { return static_cast<Sim*>(h)->c.L.ncomp; }
 

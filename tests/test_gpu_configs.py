@@ -1,0 +1,203 @@
+"""The five benchmark workloads (hipace2_amd/workloads.py) on the GPU at their full sizes.
+
+1. Linear wake, predictor-corrector: the whole step against the oracle, bit for bit.
+2. Blowout, explicit solver: step 0 against the oracle bit for bit (every slice's fields, the
+   plasma, the V-cycle and QSA counts), then ten steps with the beam pushed: the beam and the
+   last step's field checksums against the oracle's ten steps, bit for bit.
+3. Throughput workload: the FP64 path repeats bit for bit; the mixed path (FP32 particle
+   arithmetic, split-BF16 DST) stays within 1e-4 of it in Ez.
+4. Beam-loading gradient: the adjoint against central differences of the forward loss on two
+   knots, within 1e-3.
+5. Large workload: the ramped plasma injection against the oracle's bit for bit, and four
+   slices inside the drive from the GPU's state on both, bit for bit.
+The oracle (CPU, OpenMP) runs these sizes in seconds to about two minutes.
+"""
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from helpers import rel_err  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+NAMES_PC = ["Ez", "Psi", "Bx", "By", "Bz", "jx", "jz", "rhomjz"]
+NAMES_EX = ["Ez", "Psi", "Bx", "By", "Bz", "jx", "rhomjz", "chi", "Sx", "Sy"]
+PLASMA = ("x", "y", "w", "ux", "uy", "psi", "x_prev", "y_prev", "ux_half_step", "uy_half_step", "psi_half_step",
+          "valid")
+
+
+def _threads():
+    import os
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _equal_captures(cg, co, names):
+    for k in names:
+        g = cg[k].cpu().numpy()
+        o = co[k]
+        bad = [s for s in range(g.shape[0]) if not np.array_equal(g[s], o[s])]
+        assert not bad, (k, "differing slices", bad[-1], bad[0], rel_err(g, o))
+
+
+def _equal_plasma(sim, orc):
+    pg, po = sim.plasma_host(), orc.plasma()
+    for k in PLASMA:
+        assert np.array_equal(pg[k], po[k]), (k, rel_err(pg[k], po[k]))
+
+
+@pytest.mark.timeout(600)
+def test_config1_full_step_against_oracle():
+    from hipace2_amd import workloads
+    from hipace2_amd.simulation import Simulation
+    from oracle.oracle import OracleSim
+    cfg, beam = workloads.config1()
+    assert beam.n == 100_000 and cfg.nz == 100 and not cfg.explicit_solver
+    sim, orc = Simulation(cfg, beam), OracleSim(cfg, beam, nthreads=_threads())
+    cg = {k: sim.capture(k) for k in NAMES_PC}
+    co = {k: orc.capture(k) for k in NAMES_PC}
+    sim.run_step()
+    orc.run_step()
+    torch.cuda.synchronize()
+    _equal_captures(cg, co, NAMES_PC)
+    _equal_plasma(sim, orc)
+    sg, so = sim.stats(), orc.stats()
+    assert sg["pc_iters"] == so["pc_iters"] and sg["n_qsa"] == so["n_qsa"]
+
+
+@pytest.mark.timeout(900)
+def test_config2_blowout_step0_and_ten_steps_against_oracle():
+    from hipace2_amd import workloads
+    from hipace2_amd.simulation import Simulation
+    from oracle.oracle import OracleSim
+    cfg, beam = workloads.config2()
+    assert beam.n == 1_000_000 and cfg.nx == 255 and cfg.nz == 256
+    sim, orc = Simulation(cfg, beam), OracleSim(cfg, beam, nthreads=_threads())
+    cg = {k: sim.capture(k) for k in NAMES_EX}
+    co = {k: orc.capture(k) for k in NAMES_EX}
+    sim.run_step()
+    orc.run_step()
+    torch.cuda.synchronize()
+    _equal_captures(cg, co, NAMES_EX)
+    _equal_plasma(sim, orc)
+    sg, so = sim.stats(), orc.stats()
+    assert sg["mg_iters"] == so["mg_iters"] and sg["n_qsa"] == so["n_qsa"]
+    assert so["n_qsa"] >= 0
+    sim.clear_captures()
+    orc.clear_captures()
+    # ten steps with the beam pushed after every slice (the step's fields do not depend on the
+    # push that follows them); the beam sorted into slices between steps
+    for step in range(workloads.CONFIG2_STEPS):
+        if step > 0:
+            sim.sort_beam()
+            orc.sort_beam()
+        sim.run_step_with_beam(cfg.dt, cfg.n_subcycles, cfg.do_z_push)
+        orc.begin_step()
+        orc.run_step_with_beam(cfg.dt, cfg.n_subcycles, cfg.do_z_push)
+    torch.cuda.synchronize()
+    bg, bo = {k: v[: beam.n].cpu().numpy() for k, v in sim.beam.items() if k != "nsub"}, orc.beam()
+    for k in ("x", "y", "z", "ux", "uy", "uz", "w", "valid"):
+        assert np.array_equal(bg[k], bo[k]), (k, rel_err(bg[k], bo[k]))
+    assert float(np.abs(bg["ux"]).max()) > 0.0
+    assert np.array_equal(np.asarray(sim.beam_host.off), bo["off"])
+    # sum |F| over the last step's slices: the GPU adds the cells in another order (1e-15)
+    cs_g, cs_o = sim.checksums(), orc.checksums()
+    for k, v in cs_o.items():
+        assert math.isclose(cs_g[k], v, rel_tol=1e-12, abs_tol=1e-300), (k, cs_g[k], v)
+
+
+@pytest.mark.timeout(600)
+def test_config3_fp64_reproducible_and_mixed_within_1e4():
+    from hipace2_amd import workloads
+    from hipace2_amd.simulation import Simulation
+    cfg, beam = workloads.config3()
+    assert beam.n == 4_000_000 and cfg.nx == 511 and cfg.nz == 512 and cfg.ppc == (2, 2)
+    sim = Simulation(cfg, beam)
+    ez1 = sim.capture("Ez")
+    sim.run_step()
+    torch.cuda.synchronize()
+    e1 = ez1.clone()
+    sim.run_step()
+    torch.cuda.synchronize()
+    # the FP64 path with ordered deposition repeats bit for bit
+    assert torch.equal(ez1, e1)
+    assert float(e1.abs().max()) > 0
+    del sim
+    cfg_m = dataclasses_replace(cfg, precision="mixed", poisson_solver="bf16x3")
+    sim_m = Simulation(cfg_m, beam)
+    ezm = sim_m.capture("Ez")
+    sim_m.run_step()
+    torch.cuda.synchronize()
+    err = float((ezm - e1).abs().max() / e1.abs().max())
+    assert err <= 1e-4, err
+
+
+def dataclasses_replace(cfg, **kw):
+    import dataclasses
+    return dataclasses.replace(cfg, **kw)
+
+
+@pytest.mark.timeout(600)
+def test_config4_gradient_against_central_differences():
+    from hipace2_amd import workloads
+    from hipace2_amd.adjoint import on_axis_ez_variance, slice_loop_ez
+    from hipace2_amd.simulation import Simulation
+    cfg, bl = workloads.config4()
+    assert cfg.nx == 511 and cfg.nz == 512 and bl.beam.n > 1_000_000
+    sim = Simulation(cfg, bl.beam)
+    knots0 = torch.as_tensor(np.random.default_rng(99).uniform(0.5, 1.5, 64), device="cuda")
+    wsl = bl.witness_slices()
+
+    def plain(knots):
+        sim.beam["w"][: bl.beam.n].copy_(bl.weights(knots))
+        ez = sim.capture("Ez")
+        sim.begin_step()
+        sim.run_slices()
+        torch.cuda.synchronize()
+        out = float(on_axis_ez_variance(ez, wsl))
+        sim.clear_captures()
+        return out
+
+    knots = knots0.clone().requires_grad_(True)
+    loss = on_axis_ez_variance(slice_loop_ez(bl.weights(knots), sim, checkpoint_every=32, mg_tol=1e-9), wsl)
+    loss.backward()
+    grad = knots.grad.detach().cpu().numpy()
+    for k in (10, 41):
+        h = 1e-3 * float(knots0[k])
+        e = torch.zeros_like(knots0)
+        e[k] = h
+        fd = (plain(knots0 + e) - plain(knots0 - e)) / (2 * h)
+        assert abs(grad[k] - fd) <= 1e-3 * np.abs(grad).max(), (k, grad[k], fd)
+
+
+@pytest.mark.timeout(900)
+def test_config5_ramp_injection_and_slices_against_oracle():
+    from hipace2_amd import workloads
+    from hipace2_amd.simulation import Simulation
+    from oracle.oracle import OracleSim
+    cfg, beam = workloads.config5()
+    assert cfg.nx == 1023 and cfg.ramp_a == 0.1 and beam.n > 11_000_000
+    sim = Simulation(cfg, beam)
+    sim.init_plasma()
+    orc = OracleSim(cfg, beam, nthreads=_threads())
+    orc.begin_step()
+    _equal_plasma(sim, orc)
+    n = sim.np
+    assert n == orc.plasma()["x"].size and n > 4_000_000
+    # four slices inside the drive (zeta ~ +0.4) from the GPU's state, on both
+    s0 = int(round((0.44 - cfg.zlo) / cfg.dz))
+    sim.begin_step(init_plasma=False)
+    sim.run_slices(cfg.nz - 1, s0 + 1)
+    torch.cuda.synchronize()
+    orc.set_store(sim.store_host())
+    orc.set_plasma(sim.plasma_host())
+    for s in range(s0, s0 - 4, -1):
+        sim.solve_slice(s)
+        orc.solve_slice(s)
+    torch.cuda.synchronize()
+    g, o = sim.store_host(), orc.store()
+    for k in ("Ez", "Psi", "Bx", "By", "Bz", "chi", "Sx", "Sy", "jx", "rhomjz"):
+        assert np.array_equal(g[sim.layout[k]], o[orc.layout[k]]), (k, rel_err(g[sim.layout[k]], o[orc.layout[k]]))
+    _equal_plasma(sim, orc)
