@@ -227,3 +227,26 @@ def test_adjoint_loop_checkpoint_period_invariance():
         grads.append(w.grad.clone())
     for g in grads[1:]:
         assert float((g - grads[0]).abs().max()) <= 1e-9 * float(grads[0].abs().max())
+
+
+def test_adjoint_backward_of_a_replaced_forward_raises():
+    """A backward is valid only for the last forward pass on its Simulation with the weights it
+    ran with: forward(w1), forward(w2), then the first one's backward must raise, and so must a
+    backward after the weights changed in place."""
+    from hipace2_amd.adjoint import slice_loop_ez
+    cfg, beam, sim = _adjoint_sim(nz=12, nbeam=4000)
+    L = _losses(cfg)["linear"]
+    w1 = sim.beam["w"][: beam.n].clone().requires_grad_(True)
+    w2 = (1.1 * sim.beam["w"][: beam.n]).clone().requires_grad_(True)
+    loss1 = L(slice_loop_ez(w1, sim, checkpoint_every=4))
+    loss2 = L(slice_loop_ez(w2, sim, checkpoint_every=4))
+    with pytest.raises(RuntimeError, match="another forward pass"):
+        loss1.backward()
+    sim.beam["w"][0] += 1.0
+    with pytest.raises(RuntimeError, match="weights changed"):
+        loss2.backward()
+    sim.beam["w"][0] -= 1.0
+    # a fresh forward after the failures still differentiates
+    w3 = w2.detach().clone().requires_grad_(True)
+    L(slice_loop_ez(w3, sim, checkpoint_every=4)).backward()
+    assert torch.isfinite(w3.grad).all() and float(w3.grad.abs().max()) > 0.0

@@ -54,7 +54,8 @@ def test_config1_full_step_against_oracle():
     from hipace2_amd.simulation import Simulation
     from oracle.oracle import OracleSim
     cfg, beam = workloads.config1()
-    assert beam.n == 100_000 and cfg.nz == 100 and not cfg.explicit_solver
+    # 1e5 particles drawn; the Gaussian's tail beyond zeta = 4 (0.24 %) lies outside the box
+    assert 99_500 < beam.n <= 100_000 and cfg.nz == 100 and not cfg.explicit_solver
     sim, orc = Simulation(cfg, beam), OracleSim(cfg, beam, nthreads=_threads())
     cg = {k: sim.capture(k) for k in NAMES_PC}
     co = {k: orc.capture(k) for k in NAMES_PC}
@@ -73,7 +74,7 @@ def test_config2_blowout_step0_and_ten_steps_against_oracle():
     from hipace2_amd.simulation import Simulation
     from oracle.oracle import OracleSim
     cfg, beam = workloads.config2()
-    assert beam.n == 1_000_000 and cfg.nx == 255 and cfg.nz == 256
+    assert 999_900 < beam.n <= 1_000_000 and cfg.nx == 255 and cfg.nz == 256
     sim, orc = Simulation(cfg, beam), OracleSim(cfg, beam, nthreads=_threads())
     cg = {k: sim.capture(k) for k in NAMES_EX}
     co = {k: orc.capture(k) for k in NAMES_EX}
@@ -113,7 +114,8 @@ def test_config3_fp64_reproducible_and_mixed_within_1e4():
     from hipace2_amd import workloads
     from hipace2_amd.simulation import Simulation
     cfg, beam = workloads.config3()
-    assert beam.n == 4_000_000 and cfg.nx == 511 and cfg.nz == 512 and cfg.ppc == (2, 2)
+    # 4e6 particles drawn; the tail beyond zeta = 4 (0.23 %) lies outside the box
+    assert 3_980_000 < beam.n <= 4_000_000 and cfg.nx == 511 and cfg.nz == 512 and cfg.ppc == (2, 2)
     sim = Simulation(cfg, beam)
     ez1 = sim.capture("Ez")
     sim.run_step()
