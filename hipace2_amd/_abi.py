@@ -164,7 +164,7 @@ def lib():
         L.hpq_advance_beam_slice.argtypes = [vp, C.POINTER(BeamState), ip, vp, C.c_double, ip, ip]
         L.hpq_sort_beam_slices.argtypes = [vp, C.POINTER(BeamState), C.POINTER(C.c_int64)]
         L.hpq_run_step_with_beam.argtypes = [vp, C.POINTER(Plasma), C.POINTER(BeamState), vp, C.c_double, ip, ip]
-        L.hpq_adjoint_backward.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, vp, vp, C.c_double]
+        L.hpq_adjoint_backward.argtypes = [vp, C.POINTER(Plasma), C.POINTER(Beam), vp, vp, vp, vp, C.c_double]
         _lib = L
     return _lib
 

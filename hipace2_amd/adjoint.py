@@ -14,9 +14,14 @@ import torch
 from .simulation import Simulation
 
 
-class SliceLoopEz(torch.autograd.Function):
-    """Ez[slice, j, i] over the valid box for every slice of one step, as a function of the
-    beam weights w (one per macro-particle, in the beam's slice-sorted order).
+ADJOINT_FIELDS = ("Ez", "jz_beam")
+
+
+class SliceLoop(torch.autograd.Function):
+    """Fields of one step on every slice (the valid box, [slice, j, i]) as a function of the
+    beam weights w (one per macro-particle, in the beam's slice-sorted order): Ez, and
+    jz_beam when asked for, so that a loss may depend on both (the beam's energy gain,
+    sum Ez * jz_beam dx dy dz, is one).
 
     The checkpoints of the forward pass live in the Simulation's context, and the backward pass
     recomputes from them with the beam weights in the Simulation: a backward is valid only for
@@ -26,38 +31,62 @@ class SliceLoopEz(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, beam_w: torch.Tensor, sim: Simulation, checkpoint_every: int = 32,
-                mg_tol: float = 1e-10):
+                mg_tol: float = 1e-10, fields: tuple = ("Ez",)):
+        for f in fields:
+            if f not in ADJOINT_FIELDS:
+                raise ValueError(f"slice_loop: no adjoint for field {f!r} (have {ADJOINT_FIELDS})")
         w = beam_w.detach().to(sim.beam["w"].dtype)
         sim.beam["w"][: beam_w.numel()].copy_(w)
-        ez = sim.capture("Ez")
+        caps = {f: sim.capture(f) for f in fields}
         sim.begin_step()
         sim.adjoint_forward(checkpoint_every)
         torch.cuda.synchronize(sim.device)
-        out = ez.clone()
+        outs = tuple(caps[f].clone() for f in fields)
         sim.clear_captures()
+        ctx.set_materialize_grads(False)
         ctx.sim = sim
         ctx.mg_tol = mg_tol
         ctx.n = beam_w.numel()
+        ctx.fields = tuple(fields)
         ctx.generation = sim.adjoint_generation
         ctx.w = w.clone()
-        return out
+        return outs
 
     @staticmethod
-    def backward(ctx, grad_ez: torch.Tensor):
+    def backward(ctx, *grads):
         sim = ctx.sim
         if sim.adjoint_generation != ctx.generation:
-            raise RuntimeError("slice_loop_ez: another forward pass ran on this Simulation since; its "
+            raise RuntimeError("slice_loop: another forward pass ran on this Simulation since; its "
                                "checkpoints replaced this one's")
         if not torch.equal(sim.beam["w"][: ctx.n], ctx.w):
-            raise RuntimeError("slice_loop_ez: the beam weights changed between the forward and the backward pass")
+            raise RuntimeError("slice_loop: the beam weights changed between the forward and the backward pass")
+        seeds = {f: (g.detach().to(torch.float64).contiguous() if g is not None else None)
+                 for f, g in zip(ctx.fields, grads)}
         w_adj = torch.zeros(max(ctx.n, 1), dtype=torch.float64, device=sim.device)
-        sim.adjoint_backward(grad_ez.detach().to(torch.float64).contiguous(), w_adj, ctx.mg_tol)
-        return w_adj[: ctx.n], None, None, None
+        sim.adjoint_backward(seeds.get("Ez"), w_adj, ctx.mg_tol, jz_beam_seed=seeds.get("jz_beam"))
+        return w_adj[: ctx.n], None, None, None, None
+
+
+def slice_loop(beam_w: torch.Tensor, sim: Simulation, checkpoint_every: int = 32, mg_tol: float = 1e-10,
+               fields: tuple = ("Ez", "jz_beam")):
+    """The captured fields (a tuple in the order of `fields`) of one step, differentiable in
+    the beam weights."""
+    return SliceLoop.apply(beam_w, sim, checkpoint_every, mg_tol, tuple(fields))
 
 
 def slice_loop_ez(beam_w: torch.Tensor, sim: Simulation, checkpoint_every: int = 32,
                   mg_tol: float = 1e-10) -> torch.Tensor:
-    return SliceLoopEz.apply(beam_w, sim, checkpoint_every, mg_tol)
+    return SliceLoop.apply(beam_w, sim, checkpoint_every, mg_tol, ("Ez",))[0]
+
+
+def energy_gain(ez: torch.Tensor, jz_beam: torch.Tensor, slices: slice, cfg) -> torch.Tensor:
+    """Energy the beam gains per unit time over the given slices: the sum over those slices of
+    the in-situ quantity [Ez*jz_beam] = sum over valid cells of Ez * jz_beam * dx dy dz
+    (Fields::InSituComputeDiags, src/fields/Fields.cpp:1320-1346). jz_beam = q w v_z S(x) S(y)
+    is negative for electrons moving forward, so a decelerating field (Ez > 0) gives a
+    positive product: the quantity is the power the beam delivers to the wake, and the
+    witness gains energy when it is negative."""
+    return -(ez[slices] * jz_beam[slices]).sum() * (cfg.dx * cfg.dy * cfg.dz)
 
 
 def knot_profile(z: torch.Tensor, knots: torch.Tensor, z0: float, z1: float) -> torch.Tensor:
@@ -120,18 +149,33 @@ class BeamLoading:
         self.witness_index = np.concatenate(idx) if idx else np.zeros(0, np.int64)
         self.witness_z = witness.z
         self.witness_w0 = witness.w
+        didx = [np.arange(int(self.beam.off[s]), int(self.beam.off[s]) + int(drive.off[s + 1] - drive.off[s]))
+                for s in range(cfg.nz)]
+        self.drive_index = np.concatenate(didx) if didx else np.zeros(0, np.int64)
+        self.drive_z = drive.z
+        self.drive_w0 = drive.w
         self.base_w = self.beam.w.copy()
         self.base_w[self.witness_index] = 0.0
         self.s0 = int(np.floor((z0 - cfg.zlo) / cfg.dz))
         self.s1 = int(np.floor((z1 - cfg.zlo) / cfg.dz))
 
-    def weights(self, knots: torch.Tensor) -> torch.Tensor:
+    def weights(self, knots: torch.Tensor, drive_knots: Optional[torch.Tensor] = None,
+                drive_range: tuple = (-4.0, 4.0)) -> torch.Tensor:
+        """Weights of the merged beam: the witness's are w0 * knot_profile(zeta; knots) on
+        [z0, z1]; with drive_knots, the drive's current profile is shaped the same way,
+        w0 * knot_profile(zeta; drive_knots) on drive_range (constant beyond its ends)."""
         dev = knots.device
         base = torch.as_tensor(self.base_w, device=dev)
         zw = torch.as_tensor(self.witness_z, device=dev)
         w0 = torch.as_tensor(self.witness_w0, device=dev)
         idx = torch.as_tensor(self.witness_index, device=dev)
-        return base.index_put((idx,), w0 * knot_profile(zw, knots, self.z0, self.z1))
+        w = base.index_put((idx,), w0 * knot_profile(zw, knots, self.z0, self.z1))
+        if drive_knots is not None:
+            zd = torch.as_tensor(self.drive_z, device=dev)
+            d0 = torch.as_tensor(self.drive_w0, device=dev)
+            didx = torch.as_tensor(self.drive_index, device=dev)
+            w = w.index_put((didx,), d0 * knot_profile(zd, drive_knots, drive_range[0], drive_range[1]))
+        return w
 
     def witness_slices(self) -> slice:
         return slice(max(self.s0, 0), self.s1 + 1)

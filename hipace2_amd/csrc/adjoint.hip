@@ -582,12 +582,17 @@ void adj_seed(hpq_ctx* c, const double* seed, double* adj) {
     HPQ_HIP(hipGetLastError());
 }
 
-void adj_beam_weights(hpq_ctx* c, const hpq_beam* b, int slice, double* adj, double* w_adj) {
+void adj_beam_weights(hpq_ctx* c, const hpq_beam* b, int slice, double* adj, const double* jzb_seed,
+                      double* w_adj) {
     const Geom& g = c->g;
     const Layout& L = c->L;
-    // the jz_b adjoint goes to the jz_beam plane of the adjoint store
+    // the jz_b adjoint goes to the jz_beam plane of the adjoint store: through Sx/Sy, plus
+    // the loss's own dependence on jz_beam (valid cells)
     hipLaunchKernelGGL(k_adj_sxsy_jzb, dim3((g.ldx + 127) / 128, g.ldy), dim3(128), 0, c->stream, g,
                        adj + L.Sy * g.plane, adj + L.Sx * g.plane, adj + L.jz_beam * g.plane);
+    if (jzb_seed)
+        hipLaunchKernelGGL(k_adj_seed, dim3((g.nx + 127) / 128, g.ny), dim3(128), 0, c->stream, g, jzb_seed,
+                           adj + L.jz_beam * g.plane);
     HPQ_HIP(hipGetLastError());
     if (!b || !b->off || !w_adj) return;
     const long first = b->off[slice];

@@ -637,7 +637,7 @@ void adjoint_forward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double*
 // the backward march; seed: d loss / d Ez per slice ([slice][j][i], valid cells) or null;
 // w_adj: beam weight adjoints, accumulated
 void adjoint_backward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store, const double* seed,
-                      double* w_adj, double mg_tol) {
+                      const double* jzb_seed, double* w_adj, double mg_tol) {
     adjoint_check(c);
     AdjointState& A = c->adj;
     if (!A.ready || A.np != p->n) throw HpqError(HPQ_E_INVALID, "adjoint: no matching forward pass");
@@ -698,7 +698,7 @@ void adjoint_backward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double
                                        c->stream));
                 adj_chi(c, st, adj);
                 explicit_deposition_vjp(c, &sp, st, adj, &in);
-                adj_beam_weights(c, b, isl, adj, w_adj);
+                adj_beam_weights(c, b, isl, adj, jzb_seed ? jzb_seed + long(isl) * g.nx * g.ny : nullptr, w_adj);
                 adj_poisson(c, adj);
                 const int32_t comps[6] = {L.jx, L.jy, -1, -1, L.chi, L.rhomjz};
                 deposit_vjp(c, &sp, adj, comps, &in);
@@ -996,13 +996,15 @@ int hpq_adjoint_forward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, doub
 }
 
 int hpq_adjoint_backward(hpq_ctx* c, const hpq_plasma* p, const hpq_beam* b, double* store, const double* ez_seed,
-                         double* beam_w_adj, double mg_tol) {
+                         const double* jz_beam_seed, double* beam_w_adj, double mg_tol) {
     if (!c) return HPQ_E_INVALID;
     return guarded(c, [&] {
         check_store(c, store);
         check_plasma(p);
         if (!(mg_tol > 0.0)) throw HpqError(HPQ_E_INVALID, "mg_tol must be positive");
-        adjoint_backward(c, p, b, store, ez_seed, beam_w_adj, mg_tol);
+        if (jz_beam_seed && (!b || !beam_w_adj))
+            throw HpqError(HPQ_E_INVALID, "a jz_beam seed needs the beam and its weight adjoints");
+        adjoint_backward(c, p, b, store, ez_seed, jz_beam_seed, beam_w_adj, mg_tol);
     });
 }
 

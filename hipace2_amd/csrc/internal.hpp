@@ -342,7 +342,8 @@ void advance_plasma_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, co
 void adj_poisson(hpq_ctx* c, double* adj);
 void adj_chi(hpq_ctx* c, const double* store, double* adj);
 void adj_seed(hpq_ctx* c, const double* seed, double* adj);
-void adj_beam_weights(hpq_ctx* c, const hpq_beam* b, int slice, double* adj, double* w_adj);
+void adj_beam_weights(hpq_ctx* c, const hpq_beam* b, int slice, double* adj, const double* jzb_seed,
+                      double* w_adj);
 void adjoint_free(hpq_ctx* c);
 void beam_begin_step(hpq_ctx* c, hpq_beam_state* b);
 void advance_beam(hpq_ctx* c, hpq_beam_state* b, int slice, const double* store, double dt_step, int n_subcycles,

@@ -21,9 +21,10 @@
 // final smoothing of the cycle are one launch of eight sweeps that leaves every
 // workgroup's residual max norm; a one-workgroup launch after it reduces them, evaluates
 // convergence and publishes the result to pinned host memory, which the host polls
-// instead of synchronising the stream. Levels from the first one of at most 33x33
+// instead of synchronising the stream. Levels from the first one of at most 65x65
 // points down run as one workgroup that does the coarse part of the V-cycle with barriers
-// between sweeps, its arrays in LDS; the levels of at most 64 points (and the bottom
+// between sweeps, its arrays in LDS (acf, rhs and correction per level: 147 KiB from a
+// 65x65 level down); the levels of at most 64 points (and the bottom
 // solve) run in its first wavefront alone, without workgroup barriers. For odd n the
 // level-0 arrays are the caller's planes themselves (the node-centred level-0 box with
 // its ghost ring has the slice store's guarded shape).
@@ -41,7 +42,7 @@ namespace {
 constexpr int TX = 32, TY = 16;                // output tile of the fine-level kernel
 constexpr int NT = 256;                        // threads per workgroup
 constexpr int RW = TX + 2, RH = TY + 2;        // residual region: tile + 1 ring
-constexpr int COARSE_MAX_POINTS = 33 * 33;
+constexpr int COARSE_MAX_POINTS = 65 * 65;
 constexpr int CTX = 16, CTY = 32;              // coarse workgroup: CTX x CTY threads
 constexpr int COARSE_THREADS = CTX * CTY;
 constexpr size_t COARSE_LDS_MAX = 150 * 1024;
@@ -488,7 +489,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
 // component (a second one follows it when the arrays live in global memory).
 struct CLev {
     int ilo, ihi, jlo, jhi, vilo, vihi, vjlo, vjhi, ldx, cs;
-    int acf, c0i, res, cor, rescor;
+    int acf, res, cor;
     double facx, facy;
 };
 struct CArgs {
@@ -545,30 +546,33 @@ struct WaveMode {
     }
 };
 
+// one colour of gs1; the diagonal inverse is recomputed from acf (the same operations
+// as the reference's gs1, so the same value)
 template <class Mode, bool CC>
 __device__ __forceinline__ void c_sweep(double* w, const CLev& L, int color) {
     Mode::color(L, color, [&](int i, int j) {
         const int k = lat(L, i, j);
         const double* p = w + L.cor;
         auto P = [&](int a, int b) { return p[k + b * L.ldx + a]; };
-        w[L.cor + k] = gs_point<CC>(L, i, j, P, w[L.res + k], w[L.c0i + k], L.facx, L.facy);
+        const double ci = c0_inverse<CC>(L, i, j, w[L.acf + k], L.facx, L.facy);
+        w[L.cor + k] = gs_point<CC>(L, i, j, P, w[L.res + k], ci, L.facx, L.facy);
     });
     Mode::sync();
 }
 
-// down-sweep step of level L: four sweeps from zero, residual, restriction into C.res
+// down-sweep step of level L: four sweeps from zero, then the residual restricted into
+// C.res; each coarse point evaluates the fine residuals it averages (the values a stored
+// residual array would hold: every fine point the restriction reads is a valid point)
 template <class Mode, bool CC>
 __device__ void c_down(double* w, const CLev& L, const CLev& C) {
     for (int color = 0; color < 4; ++color) c_sweep<Mode, CC>(w, L, color);
-    Mode::valid(L, [&](int i, int j) {
-        const int k = lat(L, i, j);
-        const double* p = w + L.cor;
-        auto P = [&](int di, int dj) { return p[k + dj * L.ldx + di]; };
-        w[L.rescor + k] = residual1<CC>(L, i, j, P, w[L.res + k], w[L.acf + k], L.facx, L.facy);
-    });
-    Mode::sync();
     Mode::valid(C, [&](int i, int j) {
-        auto fa = [&](int a, int b) { return w[L.rescor + lat(L, a, b)]; };
+        auto fa = [&](int a, int b) {
+            const int k = lat(L, a, b);
+            const double* p = w + L.cor;
+            auto P = [&](int di, int dj) { return p[k + dj * L.ldx + di]; };
+            return residual1<CC>(L, a, b, P, w[L.res + k], w[L.acf + k], L.facx, L.facy);
+        };
         w[C.res + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
     });
     Mode::sync();
@@ -594,7 +598,7 @@ __device__ void c_up(double* w, const CLev& L, const CLev& C) {
 template <bool CC, bool IN_LDS>
 __global__ __launch_bounds__(COARSE_THREADS) void k_coarse(const CArgs A, double* __restrict__ gwork,
                                                            const MGCtl* __restrict__ ctl) {
-    constexpr int NB = 8;   // loads in flight per thread
+    constexpr int NB = 18;   // loads in flight per thread (a 65 x 65 level's acf and res in one round)
     extern __shared__ double sm[];
     const int comp = blockIdx.x;
     double* w = IN_LDS ? sm : gwork + long(comp) * A.work_doubles;
@@ -624,20 +628,14 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse(const CArgs A, double
         for (int t = threadIdx.x; t < L.cs; t += COARSE_THREADS) w[L.cor + t] = 0.0;
     }
     __syncthreads();
-    // average_down_acoef (:1641-1661) and the diagonal inverses
-    for (int l = 0; l < nl; ++l) {
+    // average_down_acoef (:1641-1661)
+    for (int l = 0; l + 1 < nl; ++l) {
         const CLev& L = A.lv[l];
-        BlockMode::valid(L, [&](int i, int j) {
-            const int k = lat(L, i, j);
-            w[L.c0i + k] = c0_inverse<CC>(L, i, j, w[L.acf + k], L.facx, L.facy);
+        const CLev& C = A.lv[l + 1];
+        BlockMode::valid(C, [&](int i, int j) {
+            auto fa = [&](int a, int b) { return w[L.acf + lat(L, a, b)]; };
+            w[C.acf + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
         });
-        if (l + 1 < nl) {
-            const CLev& C = A.lv[l + 1];
-            BlockMode::valid(C, [&](int i, int j) {
-                auto fa = [&](int a, int b) { return w[L.acf + lat(L, a, b)]; };
-                w[C.acf + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
-            });
-        }
         __syncthreads();
     }
     const int nb = A.nblock;
@@ -706,10 +704,8 @@ void coarse_args(MultiGrid& M) {
         L.ldx = G.ldx;
         L.cs = int(G.cs);
         L.acf = off; off += L.cs;
-        L.c0i = off; off += L.cs;
         L.res = off; off += L.cs;
         L.cor = off; off += L.cs;
-        L.rescor = off; off += L.cs;
         L.facx = 1. / (G.dx * G.dx);
         L.facy = 1. / (G.dy * G.dy);
     }

@@ -405,13 +405,18 @@ class Simulation:
         _abi.check(self.ctx, self.L.hpq_adjoint_forward(self.ctx, C.byref(self._pdesc), C.byref(self._bdesc),
                                                         C.c_void_p(self.store.data_ptr()), int(checkpoint_every)))
 
-    def adjoint_backward(self, ez_seed: Optional[torch.Tensor], beam_w_adj: torch.Tensor, mg_tol: float = 1e-10):
-        """Adds d loss / d (beam weights) to beam_w_adj given d loss / d Ez ([slice][j][i])."""
-        if ez_seed is not None:
-            assert ez_seed.is_contiguous() and ez_seed.numel() == self.cfg.nz * self.cfg.ny * self.cfg.nx
-        seed = C.c_void_p(ez_seed.data_ptr()) if ez_seed is not None else None
+    def adjoint_backward(self, ez_seed: Optional[torch.Tensor], beam_w_adj: torch.Tensor, mg_tol: float = 1e-10,
+                         jz_beam_seed: Optional[torch.Tensor] = None):
+        """Adds d loss / d (beam weights) to beam_w_adj given d loss / d Ez and d loss / d jz_beam
+        ([slice][j][i] each; None: zero)."""
+        seeds = []
+        for t in (ez_seed, jz_beam_seed):
+            if t is not None:
+                assert t.is_contiguous() and t.dtype == torch.float64
+                assert t.numel() == self.cfg.nz * self.cfg.ny * self.cfg.nx
+            seeds.append(C.c_void_p(t.data_ptr()) if t is not None else None)
         _abi.check(self.ctx, self.L.hpq_adjoint_backward(self.ctx, C.byref(self._pdesc), C.byref(self._bdesc),
-                                                         C.c_void_p(self.store.data_ptr()), seed,
+                                                         C.c_void_p(self.store.data_ptr()), seeds[0], seeds[1],
                                                          C.c_void_p(beam_w_adj.data_ptr()), float(mg_tol)))
 
     def advance_plasma_particles(self, temp_slice: bool = False):

@@ -219,13 +219,17 @@ int hpq_advance_plasma_particles_vjp(hpq_ctx *ctx, const hpq_plasma *p, const do
 int hpq_adjoint_forward(hpq_ctx *ctx, const hpq_plasma *p, const hpq_beam *b, double *store,
                         int checkpoint_every);
 
-/* Adjoint slice loop, step 2: for a loss L(Ez) with ez_seed = dL/dEz on every slice
- * (nz*ny*nx doubles, [slice][j][i] over the valid box; null: zero), adds dL/dw of every
- * beam macro-particle weight to beam_w_adj (b->n doubles). Marches the slices in reverse,
+/* Adjoint slice loop, step 2: for a loss L(Ez, jz_beam) with ez_seed = dL/dEz and
+ * jz_beam_seed = dL/d(jz_beam) on every slice (each nz*ny*nx doubles, [slice][j][i] over the
+ * valid box, the planes hpq_add_capture records; null: zero), adds dL/dw of every beam
+ * macro-particle weight to beam_w_adj (b->n doubles). A jz_beam seed differentiates
+ * quantities such as the beam's energy gain, sum Ez*jz_beam dx dy dz per slice (the in-situ
+ * [Ez*jz_beam], src/fields/Fields.cpp:1320-1346). Marches the slices in reverse,
  * recomputing each checkpoint segment; the plasma and the store are scratch afterwards.
  * mg_tol: relative tolerance of the adjoint Bx/By solves. */
 int hpq_adjoint_backward(hpq_ctx *ctx, const hpq_plasma *p, const hpq_beam *b, double *store,
-                         const double *ez_seed, double *beam_w_adj, double mg_tol);
+                         const double *ez_seed, const double *jz_beam_seed, double *beam_w_adj,
+                         double mg_tol);
 
 /* BeamParticleContainer::initializeSlice's counter reset and an empty slipped-particle list,
  * before the beam pushes of a step (src/particles/beam/BeamParticleContainer.cpp:353-390). */

@@ -216,6 +216,11 @@ class OracleSim:
         lib().orc_sim_add_capture(self.h, self.layout[name], _dp(buf))
         return buf
 
+    def clear_captures(self):
+        """Stop recording every captured component."""
+        lib().orc_sim_clear_captures(self.h)
+        self._captures.clear()
+
     def run_step(self, first=None, last=0) -> float:
         first = self.cfg.nz - 1 if first is None else first
         return lib().orc_sim_run_step(self.h, first, last)

@@ -250,3 +250,61 @@ def test_adjoint_backward_of_a_replaced_forward_raises():
     w3 = w2.detach().clone().requires_grad_(True)
     L(slice_loop_ez(w3, sim, checkpoint_every=4)).backward()
     assert torch.isfinite(w3.grad).all() and float(w3.grad.abs().max()) > 0.0
+
+
+def _loading_sim(n=63, nz=40):
+    from hipace2_amd import workloads  # noqa: F401
+    from hipace2_amd.adjoint import BeamLoading, witness_beam
+    from hipace2_amd.inputs import BeamSpec, make_beam
+    from hipace2_amd.simulation import Simulation
+    cfg = small_config(n=n, nz=nz, order=2, explicit=True)
+    cfg.do_beam_jx_jy = False
+    cfg.mg_tol_rel = 1e-13
+    drive = make_beam(cfg, BeamSpec(injection_type="fixed_weight", profile="gaussian", density=0.5,
+                                    position_std=(0.3, 0.3, 1.41), num_particles=20000,
+                                    u_mean=(0.0, 0.0, 1000.0), seed=5))
+    wit = witness_beam(cfg, 20000, -6.0, -4.0, 0.2, 0.3, 1000.0, seed=6)
+    bl = BeamLoading(cfg, drive, wit, -6.0, -4.0)
+    return cfg, bl, Simulation(cfg, bl.beam)
+
+
+def test_energy_gain_gradient_wrt_witness_and_drive_knots():
+    """The witness's energy gain (-sum over the witness slices of [Ez*jz_beam], the in-situ
+    quantity of Fields.cpp:1320-1346) as a function of 8 witness knots and 8 drive knots: the
+    adjoint gradient (through Ez and through jz_beam) against central differences."""
+    from hipace2_amd.adjoint import energy_gain, slice_loop
+    cfg, bl, sim = _loading_sim()
+    wsl = bl.witness_slices()
+    rng = np.random.default_rng(77)
+    k0 = torch.as_tensor(rng.uniform(0.5, 1.5, 8), device="cuda")
+    d0 = torch.as_tensor(rng.uniform(0.5, 1.5, 8), device="cuda")
+
+    def plain(k, d):
+        sim.beam["w"][: bl.beam.n].copy_(bl.weights(k, d))
+        ez, jz = sim.capture("Ez"), sim.capture("jz_beam")
+        sim.begin_step()
+        sim.run_slices()
+        torch.cuda.synchronize()
+        out = float(energy_gain(ez, jz, wsl, cfg))
+        sim.clear_captures()
+        return out
+
+    k = k0.clone().requires_grad_(True)
+    d = d0.clone().requires_grad_(True)
+    ez, jz = slice_loop(bl.weights(k, d), sim, checkpoint_every=8, mg_tol=1e-12)
+    loss = energy_gain(ez, jz, wsl, cfg)
+    loss.backward()
+    assert float(loss) != 0.0
+    gk, gd = k.grad.cpu().numpy(), d.grad.cpu().numpy()
+    scale = max(np.abs(gk).max(), np.abs(gd).max())
+    for which, g in (("witness", gk), ("drive", gd)):
+        for i in (1, 4, 6):
+            h = 1e-4
+            e = torch.zeros(8, dtype=torch.float64, device="cuda")
+            e[i] = h
+            if which == "witness":
+                fd = (plain(k0 + e, d0) - plain(k0 - e, d0)) / (2 * h)
+            else:
+                fd = (plain(k0, d0 + e) - plain(k0, d0 - e)) / (2 * h)
+            print(which, i, g[i], fd)
+            assert abs(g[i] - fd) <= 1e-5 * scale, (which, i, g[i], fd)

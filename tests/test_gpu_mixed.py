@@ -35,7 +35,7 @@ def _mixed(cfg):
 @pytest.mark.parametrize("order", [0, 1, 2, 3])
 def test_mixed_deposit_current(order, mode):
     cfg = _mixed(small_config(order=order, dtype=2 if order > 0 else 1))
-    cfg.bin_period, n = mode
+    cfg.deposition, cfg.bin_period, n = mode
     sim, orc = _pair(cfg, n_particles=n)
     comps = [k if sim.layout[k] >= 0 else -1 for k in ("jx", "jy", "jz", "rho", "chi", "rhomjz")]
     names = [k for k in comps if k != -1]
@@ -59,7 +59,7 @@ def test_mixed_deposit_current(order, mode):
 @pytest.mark.parametrize("mode", DEPOSIT_MODES)
 def test_mixed_explicit_deposition(order, dtype, mode):
     cfg = _mixed(small_config(order=order, dtype=dtype))
-    cfg.bin_period, n = mode
+    cfg.deposition, cfg.bin_period, n = mode
     sim, orc = _pair(cfg, n_particles=n)
     sim.explicit_deposition()
     orc.explicit_deposit()
