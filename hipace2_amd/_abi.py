@@ -32,7 +32,7 @@ EXPORTS = ["hpq_create", "hpq_destroy", "hpq_last_error", "hpq_set_stream", "hpq
            "hpq_trace_reset", "hpq_bin_tiles", "hpq_deposit_current_vjp", "hpq_explicit_deposition_vjp",
            "hpq_advance_plasma_particles_vjp", "hpq_adjoint_forward", "hpq_adjoint_backward",
            "hpq_beam_begin_step", "hpq_advance_beam_slice", "hpq_sort_beam_slices", "hpq_run_step_with_beam",
-           "hpq_set_insitu", "hpq_set_salame"]
+           "hpq_set_insitu", "hpq_set_salame", "hpq_bin_tiles_noalloc"]
 
 TRACE = {"push": 0, "deposit": 1, "explicit_deposit": 2, "beam_deposit": 3, "mg_smooth": 4, "mg_coarse": 5,
          "dst": 6}
@@ -150,6 +150,8 @@ def lib():
         L.hpq_add_capture.argtypes = [vp, ip, vp]
         L.hpq_set_insitu.argtypes = [vp, C.POINTER(Insitu)]
         L.hpq_set_salame.argtypes = [vp, C.POINTER(Salame)]
+        L.hpq_bin_tiles.argtypes = [vp, C.POINTER(Plasma), ip, vp, vp]
+        L.hpq_bin_tiles_noalloc.argtypes = [vp, C.POINTER(Plasma), ip, vp, vp]
         L.hpq_clear_captures.argtypes = [vp]
         L.hpq_stats.argtypes = [vp, dp]
         L.hpq_trace_enable.argtypes = [vp, ip, ip]

@@ -12,6 +12,8 @@
 // (bin, chunk) pair its first output slot; scatter, one wavefront per chunk walking its
 // particles in index order, so ties keep their order.
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "internal.hpp"
@@ -205,26 +207,122 @@ void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int32
     sort_keys(c, n, nbins, perm, offsets);
 }
 
-// stable counting sort of the keys in bins.keys (n of them, values in [0, nbins)): the
+// chunking of the counting sort: at most 4096 chunks of at least 2048 particles (one
+// wavefront walks a chunk)
+static void sort_chunks(long n, long& chunk, int& nchunks) {
+    chunk = std::max(2048L, (n + 4095) / 4096);
+    nchunks = int((n + chunk - 1) / chunk);
+}
+
+// stable counting sort of keys[0..n) (values in [0, nbins)) with caller-provided work
+// arrays: counts (nbins * nchunks entries), tmp (scan_tmp_size of that); writes the
 // permutation (sorted slot -> index) and, when offsets != null, the first slot of every bin
 // plus offsets[nbins] = n
+static void sort_keys_into(hipStream_t s, const int32_t* keys, long n, int nbins, int32_t* counts, int32_t* tmp,
+                           int32_t* perm, int32_t* offsets) {
+    long chunk;
+    int nchunks;
+    sort_chunks(n, chunk, nchunks);
+    const long m = long(nbins) * nchunks;
+    const size_t lds = sizeof(int) * size_t(nbins);
+    hipLaunchKernelGGL(k_bin_count, dim3(nchunks), dim3(256), lds, s, keys, n, nbins, chunk, nchunks, counts);
+    scan_exclusive(s, counts, m, tmp);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nchunks), dim3(64), lds, s, keys, n, nbins, chunk, nchunks, counts, perm);
+    if (offsets)
+        hipLaunchKernelGGL(k_bin_offsets, grid1d(nbins + 1, 256), dim3(256), 0, s, counts, nbins, nchunks, n,
+                           offsets);
+    HPQ_HIP(hipGetLastError());
+}
+
+// stable counting sort of the keys in bins.keys (n of them, values in [0, nbins)), work
+// arrays held by the context
 void sort_keys(hpq_ctx* c, long n, int nbins, int32_t* perm, int32_t* offsets) {
     Bins& B = c->bins;
-    const long chunk = std::max(2048L, (n + 4095) / 4096);
-    const int nchunks = int((n + chunk - 1) / chunk);
+    long chunk;
+    int nchunks;
+    sort_chunks(n, chunk, nchunks);
     const long m = long(nbins) * nchunks;
     grow(B.counts, B.counts_cap, m);
     grow(B.tmp, B.tmp_cap, scan_tmp_size(m));
-    const size_t lds = sizeof(int) * size_t(nbins);
-    hipLaunchKernelGGL(k_bin_count, dim3(nchunks), dim3(256), lds, c->stream, B.keys, n, nbins, chunk, nchunks,
-                       B.counts);
-    scan_exclusive(c->stream, B.counts, m, B.tmp);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(nchunks), dim3(64), lds, c->stream, B.keys, n, nbins, chunk, nchunks,
-                       B.counts, perm);
-    if (offsets)
-        hipLaunchKernelGGL(k_bin_offsets, grid1d(nbins + 1, 256), dim3(256), 0, c->stream, B.counts, nbins, nchunks,
-                           n, offsets);
+    sort_keys_into(c->stream, B.keys, n, nbins, B.counts, B.tmp, perm, offsets);
+}
+
+// ---------------------------------------------------------------------------------
+// Tile binning without allocation (hpq_bin_tiles_noalloc): the same kernels, their work
+// arrays (keys, per-chunk histograms, scan partials) in one file-scope device array. Every
+// device holds its own copy of the array from the library's load to the end of the process;
+// its address on a device is looked up once, when the first context is created there
+// (bins_noalloc_setup), and only that address is handed to kernels. All calls on one device
+// share the array: calls that overlap in time on one device (two streams, two host
+// threads) race. Nothing in it needs clearing: every entry a call reads was written earlier
+// in the same call.
+// ---------------------------------------------------------------------------------
+namespace {
+
+constexpr long NOALLOC_MAX_PARTICLES = 1L << 22;   // keys
+constexpr long NOALLOC_MAX_COUNTS = 1L << 22;      // bins x chunks of the counting sort
+constexpr long NOALLOC_TMP = 4096;                 // >= scan_tmp_size(NOALLOC_MAX_COUNTS) = 1025
+constexpr long NOALLOC_INTS = NOALLOC_MAX_PARTICLES + NOALLOC_MAX_COUNTS + NOALLOC_TMP;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+__device__ __align__(256) int32_t scratch[NOALLOC_INTS];
+
+std::mutex g_noalloc_mu;
+int32_t* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+}  // namespace
+
+void bins_noalloc_setup(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        throw HpqError(HPQ_E_UNSUPPORTED, "bin_tiles_noalloc: device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw HpqError(HPQ_E_HIP, std::string("bin_tiles_noalloc: hipGetSymbolAddress of the scratch array failed: ") +
+                                      hipGetErrorString(e));
+    g_noalloc_ptr[device] = static_cast<int32_t*>(ptr);
+}
+
+void bin_tiles_noalloc(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets) {
+    if (tile <= 0) throw HpqError(HPQ_E_INVALID, "bin_tiles: tile size must be positive");
+    const Geom& g = c->g;
+    const int ntx = (g.nx + 2 * g.ng + tile - 1) / tile, nty = (g.ny + 2 * g.ng + tile - 1) / tile;
+    const int nbins = ntx * nty + 1;
+    if (nbins > 16384) throw HpqError(HPQ_E_UNSUPPORTED, "bin_tiles: more than 16383 tiles");
+    const long n = p->n;
+    if (n > NOALLOC_MAX_PARTICLES)
+        throw HpqError(HPQ_E_INVALID, "bin_tiles_noalloc: more than 4194304 particles (the scratch array's limit); "
+                                      "use hpq_bin_tiles");
+    long chunk;
+    int nchunks;
+    sort_chunks(std::max(n, 1L), chunk, nchunks);
+    if (long(nbins) * nchunks > NOALLOC_MAX_COUNTS)
+        throw HpqError(HPQ_E_INVALID, "bin_tiles_noalloc: tiles x particle chunks exceed 4194304 (the scratch "
+                                      "array's limit); use larger tiles or hpq_bin_tiles");
+    int32_t* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        base = (c->device >= 0 && c->device < NOALLOC_MAX_DEVICES) ? g_noalloc_ptr[c->device] : nullptr;
+    }
+    if (!base) throw HpqError(HPQ_E_HIP, "bin_tiles_noalloc: the scratch array of this device was not set up");
+    if (n == 0) {
+        if (offsets)
+            hipLaunchKernelGGL(k_bin_offsets, grid1d(nbins + 1, 256), dim3(256), 0, c->stream, nullptr, nbins, 1, 0L,
+                               offsets);
+        HPQ_HIP(hipGetLastError());
+        return;
+    }
+    int32_t* keys = base;
+    int32_t* counts = base + NOALLOC_MAX_PARTICLES;
+    int32_t* tmp = counts + NOALLOC_MAX_COUNTS;
+    dispatch_order(c->p.depos_order, [&](auto O) {
+        hipLaunchKernelGGL((k_bin_keys<decltype(O)::value>), grid1d(n, 256), dim3(256), 0, c->stream, g, *p, tile,
+                           tile, ntx, nty, keys);
+    });
     HPQ_HIP(hipGetLastError());
+    sort_keys_into(c->stream, keys, n, nbins, counts, tmp, perm, offsets);
 }
 
 namespace {

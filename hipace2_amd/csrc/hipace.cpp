@@ -761,6 +761,8 @@ int hpq_create(const hpq_params* p, hpq_ctx** out) {
         g.xoff = 0.5 * (g.xlo + g.xhi - g.dx * double(g.nx - 1));
         g.yoff = 0.5 * (g.ylo + g.yhi - g.dy * double(g.ny - 1));
         layout_init(c);
+        HPQ_HIP(hipGetDevice(&c->device));
+        bins_noalloc_setup(c->device);
         HPQ_HIP(hipMalloc(&c->d_count, sizeof(int) * 4));
         HPQ_HIP(hipHostMalloc(&c->h_count, sizeof(int) * 4, hipHostMallocDefault));
         HPQ_HIP(hipMalloc(&c->d_red, sizeof(double) * 8));
@@ -1086,6 +1088,15 @@ int hpq_bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile_size, int32_t* perm,
         check_plasma(p);
         if (!perm && p->n > 0) throw HpqError(HPQ_E_INVALID, "perm is null");
         bin_tiles(c, p, tile_size, perm, offsets);
+    });
+}
+
+int hpq_bin_tiles_noalloc(hpq_ctx* c, const hpq_plasma* p, int tile_size, int32_t* perm, int32_t* offsets) {
+    if (!c) return HPQ_E_INVALID;
+    return guarded(c, [&] {
+        check_plasma(p);
+        if (!perm && p->n > 0) throw HpqError(HPQ_E_INVALID, "perm is null");
+        bin_tiles_noalloc(c, p, tile_size, perm, offsets);
     });
 }
 

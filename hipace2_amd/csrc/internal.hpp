@@ -230,6 +230,7 @@ struct hpq_ctx {
     hpq::Geom g;
     hpq::Layout L;
     hipStream_t stream = nullptr;
+    int device = -1;                 // HIP device current when the context was created
     hpq::Poisson ps;
     hpq::MultiGrid mg;
     hpq::Bins bins;
@@ -326,6 +327,9 @@ void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int32
 const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p);
 void bins_free(hpq_ctx* c);
 void sort_keys(hpq_ctx* c, long n, int nbins, int32_t* perm, int32_t* offsets);
+// tile binning with its work arrays in a file-scope device array (no allocation)
+void bins_noalloc_setup(int device);
+void bin_tiles_noalloc(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets);
 // exclusive scan of a[0..n) in place; tmp: scan_tmp_size(n) entries
 void scan_exclusive(hipStream_t s, int32_t* a, long n, int32_t* tmp);
 long scan_tmp_size(long n);

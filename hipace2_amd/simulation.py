@@ -455,6 +455,15 @@ class Simulation:
                                                   C.c_void_p(perm.data_ptr()), C.c_void_p(offsets.data_ptr())))
         return perm[:n], offsets
 
+    def bin_tiles_noalloc(self, tile: int, perm: torch.Tensor, offsets: torch.Tensor):
+        """hpq_bin_tiles_noalloc on the current plasma into caller-allocated int32 device
+        tensors (perm: at least max(n, 1) entries; offsets: ntile_x * ntile_y + 2)."""
+        assert perm.dtype == torch.int32 and offsets.dtype == torch.int32
+        assert perm.numel() >= max(self.np, 1)
+        _abi.check(self.ctx, self.L.hpq_bin_tiles_noalloc(self.ctx, C.byref(self._pdesc), tile,
+                                                          C.c_void_p(perm.data_ptr()),
+                                                          C.c_void_p(offsets.data_ptr())))
+
     def poisson_dirichlet(self, staging: torch.Tensor, lhs: torch.Tensor):
         assert staging.is_contiguous() and staging.numel() == self.cfg.nx * self.cfg.ny
         assert lhs.is_contiguous() and lhs.numel() == self.ldx * self.ldy

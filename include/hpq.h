@@ -298,6 +298,17 @@ int hpq_init_plasma(hpq_ctx *ctx, const hpq_plasma_profile *prof, const hpq_plas
  * first slot of every bin, then p->n. At most 16383 tiles. */
 int hpq_bin_tiles(hpq_ctx *ctx, const hpq_plasma *p, int tile_size, int32_t *perm, int32_t *offsets);
 
+/* hpq_bin_tiles without allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). Same arguments and results as hpq_bin_tiles; its work arrays (tile keys,
+ * per-chunk histograms, scan partials) live in one static device array of 32 MiB per device,
+ * held from the library's load to the end of the process, instead of context memory. Limits:
+ * p->n <= 4194304 particles and (tiles + 1) * ceil(p->n / 2048) <= 4194304; larger inputs
+ * return HPQ_E_INVALID. Every call on one device shares that array: calls that overlap in
+ * time on one device (several streams or host threads) race. Calls no hipMalloc/hipFree and
+ * no hipMemset. */
+int hpq_bin_tiles_noalloc(hpq_ctx *ctx, const hpq_plasma *p, int tile_size, int32_t *perm, int32_t *offsets);
+
 /* Start of a time step (src/Hipace.cpp:403-471): zero the slice store and deposit the
  * neutralising ion background of p (MultiPlasma::DepositNeutralizingBackground). */
 int hpq_begin_step(hpq_ctx *ctx, const hpq_plasma *p, double *store);
