@@ -69,6 +69,30 @@ def build(force: bool = False, verbose: bool = False) -> str:
     return LIB
 
 
+# host-side AddressSanitizer + UndefinedBehaviorSanitizer (device code is built as usual);
+# used by tests/test_sanitizers.py with clang's ASan runtime preloaded
+SANITIZE_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined"]
+
+
+def build_sanitized(lib: str, objdir: str) -> str:
+    """Builds a copy of the library with the host code sanitized into `lib`."""
+    os.makedirs(objdir, exist_ok=True)
+    cmds, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(objdir, src + ".o")
+        objs.append(obj)
+        cmds.append([HIPCC] + COMMON + DEVICE + SANITIZE_HOST + ["-c", os.path.join(CSRC, src), "-o", obj])
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8"))))
+    for i in range(0, len(cmds), jobs):
+        procs = [(cmd, subprocess.Popen(cmd, stderr=subprocess.DEVNULL)) for cmd in cmds[i:i + jobs]]
+        failed = [cmd for cmd, pr in procs if pr.wait() != 0]
+        if failed:
+            raise subprocess.CalledProcessError(1, failed[0])
+    subprocess.run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", lib] + objs, check=True)
+    return lib
+
+
 if __name__ == "__main__":
     build(force="--force" in sys.argv, verbose=True)
     print(LIB)
