@@ -21,10 +21,9 @@
 // final smoothing of the cycle are one launch of eight sweeps that leaves every
 // workgroup's residual max norm; a one-workgroup launch after it reduces them, evaluates
 // convergence and publishes the result to pinned host memory, which the host polls
-// instead of synchronising the stream. Levels from the first one of at most 65x65
+// instead of synchronising the stream. Levels from the first one of at most 33x33
 // points down run as one workgroup that does the coarse part of the V-cycle with barriers
-// between sweeps, its arrays in LDS (acf, rhs and correction per level: 147 KiB from a
-// 65x65 level down); the levels of at most 64 points (and the bottom
+// between sweeps, its arrays in LDS; the levels of at most 64 points (and the bottom
 // solve) run in its first wavefront alone, without workgroup barriers. For odd n the
 // level-0 arrays are the caller's planes themselves (the node-centred level-0 box with
 // its ghost ring has the slice store's guarded shape).
@@ -42,7 +41,7 @@ namespace {
 constexpr int TX = 32, TY = 16;                // output tile of the fine-level kernel
 constexpr int NT = 256;                        // threads per workgroup
 constexpr int RW = TX + 2, RH = TY + 2;        // residual region: tile + 1 ring
-constexpr int COARSE_MAX_POINTS = 65 * 65;
+constexpr int COARSE_MAX_POINTS = 33 * 33;
 constexpr int CTX = 16, CTY = 32;              // coarse workgroup: CTX x CTY threads
 constexpr int COARSE_THREADS = CTX * CTY;
 constexpr size_t COARSE_LDS_MAX = 150 * 1024;
@@ -202,6 +201,72 @@ __global__ __launch_bounds__(NT) void k_target(Check chk) {
     if (threadIdx.x == 0) end_initial(chk.ctl, chk.other, chk.tol_rel, chk.tol_abs, resnorm0, rhsnorm0);
 }
 
+// Convergence bookkeeping carried by the first launch of a V-cycle instead of a launch of
+// its own. TARGET (before cycle 0): the convergence target from the initial residual and
+// rhs norms, as k_target sets it. CHECK (before cycle m >= 1): the end of cycle m - 1, as
+// k_check evaluates it, with m cycles done. Every workgroup of the launch reduces the
+// level-0 norms and reaches the same decision; workgroup 0 records it in the control block.
+struct Fold {
+    enum : int { NONE = 0, TARGET = 1, CHECK = 2 };
+    int mode = NONE;
+    int m = 0, maxiter = 0, nblk = 0;
+    double tol_rel = 0.0, tol_abs = 0.0;
+    const double* norms = nullptr;
+    MGCtl* ctl = nullptr;
+    MGCtl* other = nullptr;
+};
+
+// the folded decision: whether the solve is finished before this launch's work
+__device__ bool fold_decide(const Fold& f, const MGCtl* gate, double* red) {
+    if (f.mode == Fold::NONE) return gate ? gate->done != 0 : false;
+    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
+    if (f.mode == Fold::TARGET) {
+        const double resnorm0 = max_norms(f.norms, f.nblk, 0, red);
+        const double rhsnorm0 = max_norms(f.norms, f.nblk, 1, red);
+        const double max_norm = (rhsnorm0 >= resnorm0) ? rhsnorm0 : resnorm0;
+        const double target = fmax(f.tol_abs, fmax(f.tol_rel, 1.e-16) * max_norm);
+        const bool done = resnorm0 <= target;
+        if (lead) {
+            f.ctl->max_norm = max_norm;
+            f.ctl->res_target = target;
+            if (done) {
+                f.ctl->done = 1;
+                clear_ctl(f.other);
+            }
+        }
+        return done;
+    }
+    // CHECK: the control block's done flag is set by this solve's earlier launches, or by
+    // workgroup 0 of this one with the decision every workgroup reaches here; one thread
+    // reads it for the whole workgroup (after the barriers of the reduction), so all of its
+    // threads take the same branch
+    __shared__ int done_before;
+    const double norminf = max_norms(f.norms, f.nblk, 0, red);
+    if (threadIdx.x == 0) done_before = f.ctl->done;
+    __syncthreads();
+    if (done_before) return true;
+    int status = 0;
+    bool done = false;
+    if (norminf <= f.ctl->res_target) {
+        done = true;
+    } else if (norminf > 1.e20 * f.ctl->max_norm) {
+        done = true;
+        status = HPQ_E_MG_DIVERGED;
+    } else if (f.m >= f.maxiter) {
+        done = true;
+        status = HPQ_E_MG_NOT_CONVERGED;
+    }
+    if (lead) {
+        f.ctl->iters = f.m;
+        if (done) {
+            f.ctl->status = status;
+            f.ctl->done = 1;
+            clear_ctl(f.other);
+        }
+    }
+    return done;
+}
+
 // NSW red-black sweeps (colours 0,1,2,3,0,1,...) of one level, LDS-tiled, optionally
 // followed by the residual and its restriction to the coarse points this tile owns (res_c).
 // Start: zero (ZERO), phi_in + I(cor_c) (INTERP) or phi_in. Result -> phi_out.
@@ -221,7 +286,8 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                                                const double* __restrict__ rhs, long rcs,
                                                const double* __restrict__ acf, const double* __restrict__ phi_in,
                                                const double* __restrict__ cor_c, double* __restrict__ res_c,
-                                               const MGCtl* __restrict__ gate, double* __restrict__ norms) {
+                                               const MGCtl* __restrict__ gate, double* __restrict__ norms,
+                                               Fold fold) {
     static_assert(NSW % 2 == 0, "sweeps come in red-black pairs");
     constexpr int HALO = NSW + 2;
     constexpr int LX = TX + 2 * HALO, LY = TY + 2 * HALO;
@@ -241,7 +307,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
     __shared__ double ra[ACF ? RH : 1][ACF ? RW : 1];
     static_assert(!NORMS || RES, "the norms come from the residual");
     static_assert(!ACF || RES, "acf is restricted with the residual");
-    const int gate_done = gate ? gate->done : 0;
+    const int gate_done = (fold.mode == Fold::NONE) ? (gate ? gate->done : 0) : 0;
     const int ti0 = L.vilo + int(blockIdx.x) * TX - HALO;
     const int tj0 = L.vjlo + int(blockIdx.y) * TY - HALO;
     const int base = (ti0 + tj0) & 1;
@@ -306,6 +372,8 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
         }
     }
     if (gate_done) return;
+    // a folded convergence decision: its norm loads are issued after the tile's own
+    if (fold.mode != Fold::NONE && fold_decide(fold, nullptr, &red[0][0])) return;
     if (INTERP) {
 #pragma unroll
         for (int it = 0; it < NCL; ++it) {
@@ -489,7 +557,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
 // component (a second one follows it when the arrays live in global memory).
 struct CLev {
     int ilo, ihi, jlo, jhi, vilo, vihi, vjlo, vjhi, ldx, cs;
-    int acf, res, cor;
+    int acf, c0i, res, cor, rescor;
     double facx, facy;
 };
 struct CArgs {
@@ -546,33 +614,30 @@ struct WaveMode {
     }
 };
 
-// one colour of gs1; the diagonal inverse is recomputed from acf (the same operations
-// as the reference's gs1, so the same value)
 template <class Mode, bool CC>
 __device__ __forceinline__ void c_sweep(double* w, const CLev& L, int color) {
     Mode::color(L, color, [&](int i, int j) {
         const int k = lat(L, i, j);
         const double* p = w + L.cor;
         auto P = [&](int a, int b) { return p[k + b * L.ldx + a]; };
-        const double ci = c0_inverse<CC>(L, i, j, w[L.acf + k], L.facx, L.facy);
-        w[L.cor + k] = gs_point<CC>(L, i, j, P, w[L.res + k], ci, L.facx, L.facy);
+        w[L.cor + k] = gs_point<CC>(L, i, j, P, w[L.res + k], w[L.c0i + k], L.facx, L.facy);
     });
     Mode::sync();
 }
 
-// down-sweep step of level L: four sweeps from zero, then the residual restricted into
-// C.res; each coarse point evaluates the fine residuals it averages (the values a stored
-// residual array would hold: every fine point the restriction reads is a valid point)
+// down-sweep step of level L: four sweeps from zero, residual, restriction into C.res
 template <class Mode, bool CC>
 __device__ void c_down(double* w, const CLev& L, const CLev& C) {
     for (int color = 0; color < 4; ++color) c_sweep<Mode, CC>(w, L, color);
+    Mode::valid(L, [&](int i, int j) {
+        const int k = lat(L, i, j);
+        const double* p = w + L.cor;
+        auto P = [&](int di, int dj) { return p[k + dj * L.ldx + di]; };
+        w[L.rescor + k] = residual1<CC>(L, i, j, P, w[L.res + k], w[L.acf + k], L.facx, L.facy);
+    });
+    Mode::sync();
     Mode::valid(C, [&](int i, int j) {
-        auto fa = [&](int a, int b) {
-            const int k = lat(L, a, b);
-            const double* p = w + L.cor;
-            auto P = [&](int di, int dj) { return p[k + dj * L.ldx + di]; };
-            return residual1<CC>(L, a, b, P, w[L.res + k], w[L.acf + k], L.facx, L.facy);
-        };
+        auto fa = [&](int a, int b) { return w[L.rescor + lat(L, a, b)]; };
         w[C.res + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
     });
     Mode::sync();
@@ -598,7 +663,7 @@ __device__ void c_up(double* w, const CLev& L, const CLev& C) {
 template <bool CC, bool IN_LDS>
 __global__ __launch_bounds__(COARSE_THREADS) void k_coarse(const CArgs A, double* __restrict__ gwork,
                                                            const MGCtl* __restrict__ ctl) {
-    constexpr int NB = 18;   // loads in flight per thread (a 65 x 65 level's acf and res in one round)
+    constexpr int NB = 8;   // loads in flight per thread
     extern __shared__ double sm[];
     const int comp = blockIdx.x;
     double* w = IN_LDS ? sm : gwork + long(comp) * A.work_doubles;
@@ -628,14 +693,20 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse(const CArgs A, double
         for (int t = threadIdx.x; t < L.cs; t += COARSE_THREADS) w[L.cor + t] = 0.0;
     }
     __syncthreads();
-    // average_down_acoef (:1641-1661)
-    for (int l = 0; l + 1 < nl; ++l) {
+    // average_down_acoef (:1641-1661) and the diagonal inverses
+    for (int l = 0; l < nl; ++l) {
         const CLev& L = A.lv[l];
-        const CLev& C = A.lv[l + 1];
-        BlockMode::valid(C, [&](int i, int j) {
-            auto fa = [&](int a, int b) { return w[L.acf + lat(L, a, b)]; };
-            w[C.acf + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
+        BlockMode::valid(L, [&](int i, int j) {
+            const int k = lat(L, i, j);
+            w[L.c0i + k] = c0_inverse<CC>(L, i, j, w[L.acf + k], L.facx, L.facy);
         });
+        if (l + 1 < nl) {
+            const CLev& C = A.lv[l + 1];
+            BlockMode::valid(C, [&](int i, int j) {
+                auto fa = [&](int a, int b) { return w[L.acf + lat(L, a, b)]; };
+                w[C.acf + lat(C, i, j)] = restrict_from<CC>(fa, i, j);
+            });
+        }
         __syncthreads();
     }
     const int nb = A.nblock;
@@ -704,8 +775,10 @@ void coarse_args(MultiGrid& M) {
         L.ldx = G.ldx;
         L.cs = int(G.cs);
         L.acf = off; off += L.cs;
+        L.c0i = off; off += L.cs;
         L.res = off; off += L.cs;
         L.cor = off; off += L.cs;
+        L.rescor = off; off += L.cs;
         L.facx = 1. / (G.dx * G.dx);
         L.facy = 1. / (G.dy * G.dy);
     }
@@ -840,39 +913,54 @@ struct Solve {
     double* buf[2];       // iterate after an even / odd number of V-cycles
     MGCtl* ctl;
     MGCtl* other;
+    double tol_rel, tol_abs;
 };
 
 template <int NSW, bool ZERO, bool RES, bool INTERP, bool RHSMAX = false, bool NORMS = false, bool ACF = false>
 void smooth(hpq_ctx* c, const MGLevel& L, const MGLevel& C, double* phi_out, const double* rhs, long rcs,
             const double* acf, const double* phi_in, const double* cor_c, double* res_c, const MGCtl* gate,
-            double* norms = nullptr) {
+            double* norms = nullptr, const Fold& fold = Fold{}) {
     TraceScope ts(c, HPQ_TRACE_MG_SMOOTH);
     const dim3 grid((L.vihi - L.vilo + 1 + TX - 1) / TX, (L.vjhi - L.vjlo + 1 + TY - 1) / TY);
     if (L.cc)
         hipLaunchKernelGGL((k_smooth<NSW, true, ZERO, RES, INTERP, RHSMAX, NORMS, ACF>), grid, dim3(NT), 0, c->stream, L,
-                           C, phi_out, rhs, rcs, acf, phi_in, cor_c, res_c, gate, norms);
+                           C, phi_out, rhs, rcs, acf, phi_in, cor_c, res_c, gate, norms, fold);
     else
         hipLaunchKernelGGL((k_smooth<NSW, false, ZERO, RES, INTERP, RHSMAX, NORMS, ACF>), grid, dim3(NT), 0, c->stream,
-                           L, C, phi_out, rhs, rcs, acf, phi_in, cor_c, res_c, gate, norms);
+                           L, C, phi_out, rhs, rcs, acf, phi_in, cor_c, res_c, gate, norms, fold);
 }
 
 // MultiGrid::vcycle (:1430-1512), V-cycle number m of the solve. Level 0's residual and
 // its restriction to level 1 come from the end of the previous cycle (or the initial
 // residual); every kernel returns at once when the control block says the solve is
 // finished. Cycle m runs only if cycles 0..m-1 ran, so the host knows that it starts from
-// buf[m & 1] and ends in buf[(m + 1) & 1].
-void vcycle(hpq_ctx* c, const Solve& S, int m, int maxiter) {
+// buf[m & 1] and ends in buf[(m + 1) & 1]. fold: the convergence target (m = 0) or the end
+// of cycle m - 1 is evaluated by the cycle's first launch (level 1 down); check: the cycle
+// ends with a k_check launch that publishes its result to the host.
+void vcycle(hpq_ctx* c, const Solve& S, int m, int maxiter, bool fold, bool check) {
     MultiGrid& M = c->mg;
     MGCtl* ctl = S.ctl;
     const int cb = M.coarse_begin;
     for (int l = 1; l < cb; ++l) {
         MGLevel& L = M.lev[l];
         MGLevel& C = M.lev[l + 1];
+        Fold f;
+        if (fold && l == 1) {
+            f.mode = m == 0 ? Fold::TARGET : Fold::CHECK;
+            f.m = m;
+            f.maxiter = maxiter;
+            f.nblk = level0_blocks(S.L0);
+            f.tol_rel = S.tol_rel;
+            f.tol_abs = S.tol_abs;
+            f.norms = M.d_norms;
+            f.ctl = ctl;
+            f.other = S.other;
+        }
         if (m == 0)
             smooth<4, true, true, false, false, false, true>(c, L, C, L.cor, L.res, L.cs, L.acf, nullptr, nullptr,
-                                                             C.res, ctl);
+                                                             C.res, ctl, nullptr, f);
         else
-            smooth<4, true, true, false>(c, L, C, L.cor, L.res, L.cs, L.acf, nullptr, nullptr, C.res, ctl);
+            smooth<4, true, true, false>(c, L, C, L.cor, L.res, L.cs, L.acf, nullptr, nullptr, C.res, ctl, nullptr, f);
     }
     {
         TraceScope ts(c, HPQ_TRACE_MG_COARSE);
@@ -909,8 +997,10 @@ void vcycle(hpq_ctx* c, const Solve& S, int m, int maxiter) {
     const double* corc = (cb == 1) ? L1.cor : L1.rescor;
     smooth<8, false, true, true, false, true>(c, S.L0, L1, S.buf[(m + 1) & 1], S.rhs0, S.rcs, S.L0.acf, S.buf[m & 1],
                                               corc, L1.res, ctl, M.d_norms);
-    const Check chk{ctl, S.other, M.d_mirror, M.d_norms, level0_blocks(S.L0), ++M.seq, maxiter, 0.0, 0.0};
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(NT), 0, c->stream, chk);
+    if (check) {
+        const Check chk{ctl, S.other, M.d_mirror, M.d_norms, level0_blocks(S.L0), ++M.seq, maxiter, 0.0, 0.0};
+        hipLaunchKernelGGL(k_check, dim3(1), dim3(NT), 0, c->stream, chk);
+    }
 }
 
 struct Published {
@@ -980,14 +1070,26 @@ int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, lon
     // k_target then sets the convergence target from its residual and rhs max norms
     smooth<4, false, true, false, true, true, true>(c, S.L0, L1, S.buf[0], S.rhs0, S.rcs, S.L0.acf, S.buf[1], nullptr,
                                                     L1.res, nullptr, M.d_norms);
-    const Check chk0{S.ctl, S.other, M.d_mirror, M.d_norms, level0_blocks(S.L0), 0, maxiter, tol_rel, tol_abs};
-    hipLaunchKernelGGL(k_target, dim3(1), dim3(NT), 0, c->stream, chk0);
+    S.tol_rel = tol_rel;
+    S.tol_abs = tol_abs;
+    // the convergence bookkeeping rides on the first launch of the next V-cycle when there
+    // is a tiled level 1 to carry it; the last cycle of every batch publishes with k_check
+    const bool can_fold = M.coarse_begin > 1 && maxiter > 0;
+    if (!can_fold) {
+        const Check chk0{S.ctl, S.other, M.d_mirror, M.d_norms, level0_blocks(S.L0), 0, maxiter, tol_rel, tol_abs};
+        hipLaunchKernelGGL(k_target, dim3(1), dim3(NT), 0, c->stream, chk0);
+    }
     HPQ_HIP(hipGetLastError());
     int batch = std::max(1, std::min(M.last_iters, maxiter));
     int issued = 0;
+    bool checked = !can_fold;   // the previous cycle's end (or the target) has been evaluated
     Published pub{};
     while (true) {
-        for (int b = 0; b < batch && issued < maxiter; ++b, ++issued) vcycle(c, S, issued, maxiter);
+        for (int b = 0; b < batch && issued < maxiter; ++b, ++issued) {
+            const bool last = b + 1 == batch || issued + 1 == maxiter;
+            vcycle(c, S, issued, maxiter, !checked, last || !can_fold);
+            checked = last || !can_fold;
+        }
         HPQ_HIP(hipGetLastError());
         if (issued == 0) break;   // maxiter == 0
 
