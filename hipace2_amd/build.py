@@ -2,7 +2,8 @@
 of include/hpq.h) for gfx950 with hipcc, in-tree.
 
 The FP64 path is compiled with -ffp-contract=off so that every multiply and add rounds
-separately, as the reference CPU build computes them; FP64 atomics use the hardware
+separately, as the reference CPU build computes them; the FP32 kernels of the mixed-precision
+path (the *_f32.hip units) are compiled with FMA contraction. FP64 atomics use the hardware
 global_atomic_add_f64 (-munsafe-fp-atomics; all buffers are coarse-grained device memory).
 Packed FP32 instructions are disabled for all device code.
 """
@@ -18,7 +19,9 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhpq.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["poisson.hip", "mg.hip", "particles.hip", "deposit.hip", "bins.hip", "dst_mfma.hip", "adjoint.hip", "insitu.hip", "salame.hip", "fields.hip", "gather.hip", "hipace.cpp"]
+SOURCES = ["poisson.hip", "mg.hip", "particles.hip", "deposit.hip", "bins.hip", "dst_mfma.hip", "adjoint.hip", "insitu.hip", "salame.hip", "fields.hip", "gather.hip", "particles_f32.hip", "deposit_f32.hip", "hipace.cpp"]
+# sources a translation unit includes besides the headers
+INCLUDES = {"particles_f32.hip": ["particles.hip"], "deposit_f32.hip": ["deposit.hip"]}
 HEADERS = ["internal.hpp", "shape.hpp", "push.hpp", "tangent.hpp", os.path.join("..", "..", "include", "hpq.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -28,6 +31,13 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-Wno-unused-variable", "-Wno-unused-command-line-argument"]
 DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _flags(src: str):
+    """Compile flags of one translation unit: FMA contraction for the FP32 units only."""
+    if src.endswith("_f32.hip"):
+        return [("-ffp-contract=fast" if f == "-ffp-contract=off" else f) for f in COMMON]
+    return COMMON
 
 
 def _newer(target: str, deps) -> bool:
@@ -46,8 +56,9 @@ def build(force: bool = False, verbose: bool = False) -> str:
         path = os.path.join(CSRC, src)
         obj = os.path.join(OBJDIR, src + ".o")
         objs.append(obj)
-        if force or _newer(obj, [path] + hdrs):
-            cmds.append([HIPCC] + COMMON + DEVICE + ["-c", path, "-o", obj])
+        deps = [path] + hdrs + [os.path.join(CSRC, d) for d in INCLUDES.get(src, [])]
+        if force or _newer(obj, deps):
+            cmds.append([HIPCC] + _flags(src) + DEVICE + ["-c", path, "-o", obj])
     # translation units compile in parallel (hipcc is single-threaded per file)
     jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8"))))
     for i in range(0, len(cmds), jobs):
@@ -82,7 +93,7 @@ def build_sanitized(lib: str, objdir: str) -> str:
     for src in SOURCES:
         obj = os.path.join(objdir, src + ".o")
         objs.append(obj)
-        cmds.append([HIPCC] + COMMON + DEVICE + SANITIZE_HOST + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + _flags(src) + DEVICE + SANITIZE_HOST + ["-c", os.path.join(CSRC, src), "-o", obj])
     jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8"))))
     for i in range(0, len(cmds), jobs):
         procs = [(cmd, subprocess.Popen(cmd, stderr=subprocess.DEVNULL)) for cmd in cmds[i:i + jobs]]

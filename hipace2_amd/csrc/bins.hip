@@ -24,13 +24,18 @@ namespace hpq {
 namespace {
 
 constexpr int SCAN_T = 256, SCAN_ITEMS = 16, SCAN_BLOCK = SCAN_T * SCAN_ITEMS;
+// bins of one counting sort: its per-chunk histograms live in LDS (128 KiB at this size)
+constexpr long SORT_MAX_BINS = 32768;
 
+// split > 1 (the atomic deposition's walk order): each tile's particles in `split` runs by
+// particle index modulo split, so the particles a lattice injection put in one cell fall in
+// different runs and neighbouring lanes of a deposition kernel add into different cells
 template <int O>
-__global__ __launch_bounds__(256) void k_bin_keys(Geom g, hpq_plasma p, int T, int TY, int ntx, int nty,
+__global__ __launch_bounds__(256) void k_bin_keys(Geom g, hpq_plasma p, int T, int TY, int ntx, int nty, int split,
                                                   int32_t* __restrict__ keys) {
     const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (ip >= p.n) return;
-    int key = ntx * nty;
+    int key = ntx * nty * split;
     if (p.valid[ip]) {
         const double xmid = (p.x[ip] - g.xoff) * g.dxi;
         const double ymid = (p.y[ip] - g.yoff) * g.dyi;
@@ -39,7 +44,7 @@ __global__ __launch_bounds__(256) void k_bin_keys(Geom g, hpq_plasma p, int T, i
         // clamped so a particle outside the guarded box still lands in a real bin
         const int tx = min(max((cx + g.ng) / T, 0), ntx - 1);
         const int ty = min(max((cy + g.ng) / TY, 0), nty - 1);
-        key = tx * nty + ty;
+        key = (tx * nty + ty) * split + int(ip % split);
     }
     keys[ip] = key;
 }
@@ -177,17 +182,21 @@ void bins_free(hpq_ctx* c) {
 }
 
 void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets) {
-    bin_tiles_rect(c, p, tile, tile, perm, offsets);
+    bin_tiles_rect(c, p, tile, tile, 1, perm, offsets);
 }
 
-// tiles of tile x tile_y cells
-void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int32_t* perm, int32_t* offsets) {
-    if (tile <= 0 || tile_y <= 0) throw HpqError(HPQ_E_INVALID, "bin_tiles: tile size must be positive");
+// tiles of tile x tile_y cells, each split in `split` runs by particle index modulo split
+void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int split, int32_t* perm,
+                    int32_t* offsets) {
+    if (tile <= 0 || tile_y <= 0 || split <= 0) throw HpqError(HPQ_E_INVALID, "bin_tiles: tile size must be positive");
     if (p->n > INT32_MAX) throw HpqError(HPQ_E_INVALID, "bin_tiles: more than 2^31-1 particles");
     const Geom& g = c->g;
     const int ntx = (g.nx + 2 * g.ng + tile - 1) / tile, nty = (g.ny + 2 * g.ng + tile_y - 1) / tile_y;
-    const int nbins = ntx * nty + 1;
-    if (nbins > 16384) throw HpqError(HPQ_E_UNSUPPORTED, "bin_tiles: more than 16383 tiles");
+    if (long(ntx) * nty * split + 1 > SORT_MAX_BINS)
+        throw HpqError(HPQ_E_UNSUPPORTED, split > 1 ? "bin_tiles: more than 32767 tile runs"
+                                                    : "bin_tiles: more than 16383 tiles");
+    if (split == 1 && ntx * nty + 1 > 16384) throw HpqError(HPQ_E_UNSUPPORTED, "bin_tiles: more than 16383 tiles");
+    const int nbins = ntx * nty * split + 1;
     const long n = p->n;
     if (n == 0) {
         if (offsets)
@@ -201,7 +210,7 @@ void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int32
     grow(B.keys, B.keys_cap, n);
     dispatch_order(c->p.depos_order, [&](auto O) {
         hipLaunchKernelGGL((k_bin_keys<decltype(O)::value>), grid1d(n, 256), dim3(256), 0, c->stream, g, *p, tile,
-                           tile_y, ntx, nty, B.keys);
+                           tile_y, ntx, nty, split, B.keys);
     });
     HPQ_HIP(hipGetLastError());
     sort_keys(c, n, nbins, perm, offsets);
@@ -225,6 +234,15 @@ static void sort_keys_into(hipStream_t s, const int32_t* keys, long n, int nbins
     sort_chunks(n, chunk, nchunks);
     const long m = long(nbins) * nchunks;
     const size_t lds = sizeof(int) * size_t(nbins);
+    if (lds > 64 * 1024) {
+        static std::once_flag once;
+        std::call_once(once, [] {
+            HPQ_HIP(hipFuncSetAttribute((const void*)k_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        int(sizeof(int) * SORT_MAX_BINS)));
+            HPQ_HIP(hipFuncSetAttribute((const void*)k_bin_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        int(sizeof(int) * SORT_MAX_BINS)));
+        });
+    }
     hipLaunchKernelGGL(k_bin_count, dim3(nchunks), dim3(256), lds, s, keys, n, nbins, chunk, nchunks, counts);
     scan_exclusive(s, counts, m, tmp);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nchunks), dim3(64), lds, s, keys, n, nbins, chunk, nchunks, counts, perm);
@@ -319,7 +337,7 @@ void bin_tiles_noalloc(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm,
     int32_t* tmp = counts + NOALLOC_MAX_COUNTS;
     dispatch_order(c->p.depos_order, [&](auto O) {
         hipLaunchKernelGGL((k_bin_keys<decltype(O)::value>), grid1d(n, 256), dim3(256), 0, c->stream, g, *p, tile,
-                           tile, ntx, nty, keys);
+                           tile, ntx, nty, 1, keys);
     });
     HPQ_HIP(hipGetLastError());
     sort_keys_into(c->stream, keys, n, nbins, counts, tmp, perm, offsets);
@@ -400,7 +418,7 @@ const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p) {
     if (c->p.bin_period <= 0 || p->n <= 0) return nullptr;
     if (B.perm_n != p->n || B.stale) {
         grow(B.perm, B.perm_cap, p->n);
-        bin_tiles_rect(c, p, DEPOSIT_TILE_X, DEPOSIT_TILE_Y, B.perm, nullptr);
+        bin_tiles_rect(c, p, DEPOSIT_TILE_X, DEPOSIT_TILE_Y, std::max(1, c->inj_ppc), B.perm, nullptr);
         B.perm_n = p->n;
         B.stale = false;
     }

@@ -484,6 +484,14 @@ __global__ __launch_bounds__(DT) void k_deposit_beam(Geom g, hpq_beam b, long fi
 
 }  // namespace
 
+// the FP32 depositions, in a translation unit of their own (deposit_f32.hip) built with FMA
+// contraction; this one keeps every FP64 multiply and add separately rounded
+void deposit_plasma_f32(hpq_ctx* c, const hpq_plasma* p, const int32_t* order, double* store, const int* cm,
+                        double charge_invvol, double cmr);
+void explicit_deposit_f32(hpq_ctx* c, const hpq_plasma* p, const int32_t* order, double* store, double qimu0,
+                          double qmr);
+
+#ifndef HPQ_F32_TU
 // the FP64 path in the reference CPU build's summation order (gather.hip)
 static bool ordered(const hpq_ctx* c) {
     return c->p.deposition == HPQ_DEPOSIT_ORDERED && c->p.precision == HPQ_PREC_FP64;
@@ -510,17 +518,17 @@ void deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32
         c->bins.stale = true;
     }
     const int32_t* order = deposit_order(c, p);
-    const bool mixed = c->p.precision == HPQ_PREC_MIXED;
     TraceScope ts(c, HPQ_TRACE_DEPOSIT);
-    dispatch_order(c->p.depos_order, [&](auto O) {
-        constexpr int o = decltype(O)::value;
-        auto launch = [&](auto kern) {
-            hipLaunchKernelGGL(kern, grid1d(p->n, DCHUNK), dim3(DT), 0, c->stream, g, *p, order, store, cm(0), cm(1),
-                               cm(2), cm(3), cm(4), cm(5), charge_invvol, cmr, c->p.max_qsa, c->d_count);
-        };
-        if (mixed) launch(k_deposit_plasma<o, float>);
-        else launch(k_deposit_plasma<o, double>);
-    });
+    if (c->p.precision == HPQ_PREC_MIXED) {
+        const int cms[6] = {cm(0), cm(1), cm(2), cm(3), cm(4), cm(5)};
+        deposit_plasma_f32(c, p, order, store, cms, charge_invvol, cmr);
+    } else {
+        dispatch_order(c->p.depos_order, [&](auto O) {
+            hipLaunchKernelGGL((k_deposit_plasma<decltype(O)::value, double>), grid1d(p->n, DCHUNK), dim3(DT), 0,
+                               c->stream, g, *p, order, store, cm(0), cm(1), cm(2), cm(3), cm(4), cm(5), charge_invvol,
+                               cmr, c->p.max_qsa, c->d_count);
+        });
+    }
     HPQ_HIP(hipGetLastError());
 }
 
@@ -540,22 +548,20 @@ void explicit_deposition(hpq_ctx* c, const hpq_plasma* p, double* store) {
     }
     const int32_t* order = deposit_order(c, p);
     TraceScope ts(c, HPQ_TRACE_EXPLICIT_DEPOSIT);
-    dispatch_order(c->p.depos_order, [&](auto O) {
-        constexpr int o = decltype(O)::value;
-        auto launch = [&](auto kern) {
-            hipLaunchKernelGGL(kern, grid1d(p->n, DCHUNK_EXPLICIT), dim3(DT), 0, c->stream, g, *p, order, store, L.Bz, L.Ez,
-                               L.ExmBy, L.EypBx, L.Sy, L.Sx, qimu0, qmr);
-        };
-        if (c->p.precision == HPQ_PREC_MIXED) {
-            if (T == 0) launch(k_explicit_deposit<o, 0, float>);
-            else if (T == 1) launch(k_explicit_deposit<o, 1, float>);
-            else launch(k_explicit_deposit<o, 2, float>);
-        } else {
+    if (c->p.precision == HPQ_PREC_MIXED) {
+        explicit_deposit_f32(c, p, order, store, qimu0, qmr);
+    } else {
+        dispatch_order(c->p.depos_order, [&](auto O) {
+            constexpr int o = decltype(O)::value;
+            auto launch = [&](auto kern) {
+                hipLaunchKernelGGL(kern, grid1d(p->n, DCHUNK_EXPLICIT), dim3(DT), 0, c->stream, g, *p, order, store,
+                                   L.Bz, L.Ez, L.ExmBy, L.EypBx, L.Sy, L.Sx, qimu0, qmr);
+            };
             if (T == 0) launch(k_explicit_deposit<o, 0, double>);
             else if (T == 1) launch(k_explicit_deposit<o, 1, double>);
             else launch(k_explicit_deposit<o, 2, double>);
-        }
-    });
+        });
+    }
     HPQ_HIP(hipGetLastError());
 }
 
@@ -583,5 +589,7 @@ void deposit_beam_select(hpq_ctx* c, const hpq_beam* b, int slice, double* store
     });
     HPQ_HIP(hipGetLastError());
 }
+
+#endif  // HPQ_F32_TU
 
 }  // namespace hpq

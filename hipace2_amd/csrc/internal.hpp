@@ -171,6 +171,8 @@ struct CellLists {
     long off_cap = 0, tmp_cap = 0, key_cap = 0, slot_cap = 0, unsorted_cap = 0, perm_cap = 0;
     double* rec = nullptr;
     long rec_cap = 0;
+    int32_t* ovf = nullptr;      // overflow lists of the staged gathers
+    long ovf_cap = 0;
     bool valid = false;
     int base = -1;
     const double* x = nullptr;
@@ -231,6 +233,7 @@ struct hpq_ctx {
     hpq::Layout L;
     hipStream_t stream = nullptr;
     int device = -1;                 // HIP device current when the context was created
+    int inj_ppc = 1;                 // particles per cell of the last hpq_init_plasma lattice
     hpq::Poisson ps;
     hpq::MultiGrid mg;
     hpq::Bins bins;
@@ -323,7 +326,8 @@ void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p
 void init_scratch_free(hpq_ctx* c);
 int64_t plasma_count(hpq_ctx* c, const hpq_plasma_profile* prof);
 void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets);
-void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int32_t* perm, int32_t* offsets);
+void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int split, int32_t* perm,
+                    int32_t* offsets);
 const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p);
 void bins_free(hpq_ctx* c);
 void sort_keys(hpq_ctx* c, long n, int nbins, int32_t* perm, int32_t* offsets);

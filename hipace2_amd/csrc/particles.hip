@@ -293,6 +293,11 @@ __global__ void k_fill_i32(int32_t* a, long n, int32_t v) {
 
 }  // namespace
 
+// the FP32 push, in a translation unit of its own (particles_f32.hip) built with FMA
+// contraction; this one keeps every FP64 multiply and add separately rounded
+void push_plasma_f32(hpq_ctx* c, const hpq_plasma* p, const double* store, double qmc, const BC& bc, bool temp_slice);
+
+#ifndef HPQ_F32_TU
 void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool temp_slice) {
     c->cl_plasma.valid = false;   // positions change: the deposition lists are out of date
     if (p->n <= 0) return;
@@ -300,12 +305,14 @@ void advance_plasma(hpq_ctx* c, const hpq_plasma* p, const double* store, bool t
     const double qmc = c->p.plasma_charge / (c->p.plasma_mass * 1.0);
     BC bc{c->p.particle_bc, c->p.plo[0], c->p.plo[1], c->p.phi[0], c->p.phi[1]};
     TraceScope ts(c, HPQ_TRACE_PUSH);
-    dispatch_order(c->p.depos_order, [&](auto O) {
-        auto kern = c->p.precision == HPQ_PREC_MIXED ? k_push_plasma<decltype(O)::value, float>
-                                                     : k_push_plasma<decltype(O)::value, double>;
-        hipLaunchKernelGGL(kern, grid1d(p->n, 256), dim3(256), 0, c->stream, c->g, *p,
-                           store, L.Psi, L.Ez, L.Bx, L.By, L.Bz, qmc, bc, int(temp_slice));
-    });
+    if (c->p.precision == HPQ_PREC_MIXED) {
+        push_plasma_f32(c, p, store, qmc, bc, temp_slice);
+    } else {
+        dispatch_order(c->p.depos_order, [&](auto O) {
+            hipLaunchKernelGGL((k_push_plasma<decltype(O)::value, double>), grid1d(p->n, 256), dim3(256), 0, c->stream,
+                               c->g, *p, store, L.Psi, L.Ez, L.Bx, L.By, L.Bz, qmc, bc, int(temp_slice));
+        });
+    }
     HPQ_HIP(hipGetLastError());
 }
 
@@ -468,6 +475,9 @@ void init_plasma(hpq_ctx* c, const hpq_plasma_profile* prof, const hpq_plasma* p
         throw HpqError(HPQ_E_INVALID, "hpq_init_plasma: plasma capacity does not match the profile");
     hipLaunchKernelGGL(k_init_plasma, grid1d(ncell, 256), dim3(256), 0, c->stream, g, pr, c->init.off, *p);
     HPQ_HIP(hipGetLastError());
+    // a cell's particles are consecutive: the atomic deposition walks them in that many runs
+    c->inj_ppc = prof->ppc[0] * prof->ppc[1];
+    c->bins.stale = true;
 }
 
 void init_scratch_free(hpq_ctx* c) {
@@ -475,5 +485,6 @@ void init_scratch_free(hpq_ctx* c) {
     if (c->init.tmp) (void)hipFree(c->init.tmp);
     c->init = InitScratch{};
 }
+#endif  // HPQ_F32_TU
 
 }  // namespace hpq
