@@ -120,7 +120,7 @@ def run_gpu(args, path, capture):
     cfg.precision = precision
     cfg.deposition = "atomic" if path in ("mixed", "fp64_atomic") else "ordered"
     if precision == "mixed":
-        cfg.poisson_solver = args.mixed_poisson
+        cfg.poisson_solver = mixed_poisson(args, cfg)
     sim = Simulation(cfg, beam)
     sim.init_plasma()
     pristine = {k: v.clone() for k, v in sim.plasma.items()}
@@ -169,6 +169,14 @@ def run_gpu(args, path, capture):
     print(f"{path}: {cfg.nz * args.steps / elapsed:.1f} slices/s", file=sys.stderr, flush=True)
     return dict(cfg=cfg, beam=beam, sim=sim, elapsed=elapsed, traces=traces, stats=stats, breakdown=breakdown,
                 warm_stats=warm_stats, caps=caps)
+
+
+def mixed_poisson(args, cfg):
+    """The mixed path's transverse solver: the split-BF16 sine-matrix GEMMs up to n = 511, the
+    FFT beyond, where the O(n^3) products lose to the FFT's O(n^2 log n) (both measured)."""
+    if args.mixed_poisson != "auto":
+        return args.mixed_poisson
+    return "bf16x3" if max(cfg.nx, cfg.ny) <= 511 else "fft"
 
 
 def measured_traffic(kind, precision, args):
@@ -292,7 +300,7 @@ def run_adjoint(args):
 
     # the same gradient with the forward (and its recomputation) on the mixed path: FP32
     # particle arithmetic and split-BF16 DST; the vector-Jacobian products stay FP64
-    cfg_m = dataclasses.replace(cfg, precision="mixed", poisson_solver="bf16x3")
+    cfg_m = dataclasses.replace(cfg, precision="mixed", poisson_solver=mixed_poisson(args, cfg))
     sim_m = Simulation(cfg_m, bl.beam)
     knots_m = knots0.clone().requires_grad_(True)
     torch.cuda.synchronize()
@@ -507,8 +515,8 @@ def main():
     ap.add_argument("--paths", default="fp64,fp64_atomic,mixed",
                     help="paths to time (comma separated): fp64 (ordered deposition, the headline), fp64_atomic, "
                          "mixed; the first is the headline, the others report their error against it")
-    ap.add_argument("--mixed-poisson", choices=["bf16x3", "fft"], default="bf16x3",
-                    help="transverse Poisson solver of the mixed path")
+    ap.add_argument("--mixed-poisson", choices=["auto", "bf16x3", "fft"], default="auto",
+                    help="transverse Poisson solver of the mixed path (auto: bf16x3 up to n = 511, fft beyond)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the beam-loading gradient measurement")
     ap.add_argument("--adjoint-mg-tol", type=float, default=1e-8,
                     help="relative tolerance of the adjoint multigrid solves (warm-started)")
