@@ -166,13 +166,23 @@ __device__ __forceinline__ void walk_ordered(const Geom& g, const CLView& cl, in
 constexpr int GS = 16;             // cells per side of a gather block
 constexpr int GS_MAXW = GS + 5;    // widest window side (stencils of at most 6 cells)
 constexpr int GS_CAP = 6144;       // staged list entries of the plasma gathers (24 KiB)
-constexpr int GS_CAP_BEAM = 24576; // of the beam gather (96 KiB): a beam's core is dense
+constexpr int GS_CAP_REC = 2560;   // of the plasma fast passes, with their records (150 KiB)
 template <int CAP>
 struct StagedLists {
     int off[GS_MAXW][GS_MAXW + 1];   // window list bounds, in staged entries
     int base[GS_MAXW];               // record position = staged entry + base[row]
     int total;                       // staged entries (more than CAP: not staged)
     int idx[CAP];                    // particle index of every staged entry
+};
+// staged lists with the NF record fields of every staged entry (NF = 0: indices only)
+template <int CAP, int NF>
+struct StagedRecs {
+    StagedLists<CAP> l;
+    double rec[NF][CAP];
+};
+template <int CAP>
+struct StagedRecs<CAP, 0> {
+    StagedLists<CAP> l;
 };
 template <int CAP>
 struct StagedView {
@@ -187,9 +197,10 @@ __device__ __forceinline__ int gather_nbx(const Geom& g) { return (g.ldx + GS - 
 __device__ __forceinline__ int gather_nby(const Geom& g) { return (g.ldy + GS - 1) / GS; }
 
 // stages the rows of lists below block (i0, j0) for stencils of S cells starting DELTA from
-// the base cell; every thread of the workgroup calls it
-template <int S, int DELTA, int CAP>
-__device__ __forceinline__ void stage_lists(const CLView& cl, int i0, int j0, StagedLists<CAP>& sl) {
+// the base cell, and calls copy(t, pos) for every staged entry t (record position pos); every
+// thread of the workgroup calls it
+template <int S, int DELTA, int CAP, class Copy>
+__device__ __forceinline__ void stage_lists(const CLView& cl, int i0, int j0, StagedLists<CAP>& sl, Copy&& copy) {
     constexpr int NW = GS + S - 1;
     static_assert(NW <= GS_MAXW, "window wider than the staging area");
     const int col0 = i0 - S + 1 - DELTA - cl.bxlo;
@@ -218,15 +229,30 @@ __device__ __forceinline__ void stage_lists(const CLView& cl, int i0, int j0, St
     __syncthreads();
     const int total = sl.total;
     if (total <= CAP) {
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            // the row holding staged entry t: the last row starting at or before t
-            int lo = 0, hi = NW - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (sl.off[mid][0] <= t) lo = mid;
-                else hi = mid - 1;
+        // U entries per thread at a time: their rows first, then all their loads in flight
+        constexpr int U = 4;
+        for (int t0 = threadIdx.x; t0 < total; t0 += U * blockDim.x) {
+            int pos[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + u * int(blockDim.x);
+                // the row holding staged entry t: the last row starting at or before t
+                int lo = 0, hi = NW - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (sl.off[mid][0] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                pos[u] = t < total ? t + sl.base[lo] : -1;
             }
-            sl.idx[t] = cl.perm[t + sl.base[lo]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + u * int(blockDim.x);
+                if (pos[u] >= 0) {
+                    sl.idx[t] = cl.perm[pos[u]];
+                    copy(t, pos[u]);
+                }
+            }
         }
     }
     __syncthreads();
@@ -316,9 +342,9 @@ __device__ __forceinline__ bool raster_ordered(const WindowColours<S>& wc, const
 // list ovf (ovf[0] blocks in ovf[1..]) and left to the second pass. MERGE: the blockIdx.x-th
 // listed block, each cell merging its lists (from LDS when they fit, from global memory when
 // they do not). The fast pass keeps few registers; the merge's state stays in the second.
-template <bool FAST, int S, int DELTA, int CAP>
+template <bool FAST, int S, int DELTA, int CAP, class Copy>
 __device__ __forceinline__ bool gather_setup(const Geom& g, const CLView& cl, StagedLists<CAP>& sl,
-                                             int32_t* __restrict__ ovf, int& b, int& i0, int& j0) {
+                                             int32_t* __restrict__ ovf, int& b, int& i0, int& j0, Copy&& copy) {
     const int nbx = gather_nbx(g);
     if constexpr (FAST) {
         b = xcd_block(int(blockIdx.x), nbx * gather_nby(g));
@@ -328,7 +354,7 @@ __device__ __forceinline__ bool gather_setup(const Geom& g, const CLView& cl, St
     }
     i0 = (b % nbx) * GS - g.ng;
     j0 = (b / nbx) * GS - g.ng;
-    stage_lists<S, DELTA>(cl, i0, j0, sl);
+    stage_lists<S, DELTA>(cl, i0, j0, sl, copy);
     if (FAST && sl.total > CAP) {
         if (threadIdx.x == 0) ovf[1 + atomicAdd(&ovf[0], 1)] = b;
         return false;
@@ -350,26 +376,32 @@ __device__ __forceinline__ bool block_raster_ordered(const Geom& g, const CLView
     return true;
 }
 
-// The walk of cell (i, j) = block cell (li, lj) in the CPU build's order, add(load(pos), ix, iy)
+// The walk of cell (i, j) = block cell (li, lj) in the CPU build's order, add(load(at), ix, iy)
 // for every contribution. FAST: the lists in raster order (stencil offsets known at compile
-// time, records read ahead by unrolling). MERGE: the merge of the lists, from LDS when the
-// block's rows were staged, from global memory otherwise.
-template <int S, int DELTA, bool FAST, int CAP, class Load, class Add>
+// time, records read ahead by unrolling); `at` is the staged entry when ENTRY (records staged
+// in LDS), the record position otherwise. MERGE: the merge of the lists, from LDS when the
+// block's rows were staged, from global memory otherwise; `at` is the record position.
+template <int S, int DELTA, bool FAST, bool ENTRY, int CAP, class Load, class Add>
 __device__ __forceinline__ void walk_cell(const Geom& g, const CLView& cl, const StagedLists<CAP>& sl, int i, int j,
                                           int li, int lj, Load&& load, Add&& add) {
     using Rec = decltype(load(0, 0, 0));
     const StagedView<CAP> src{sl, li, lj};
     if constexpr (FAST) {
+        // colours at run time (most windows hold one), lists unrolled: one copy of the
+        // contribution code per stencil offset
         const WindowColours<S> wc(g, cl, i, j);
-#pragma unroll
+#pragma unroll 1
         for (int c = 0; c < 4; ++c) {
+            const unsigned long long mk = c == 0 ? wc.mask[0] : c == 1 ? wc.mask[1] : c == 2 ? wc.mask[2] : wc.mask[3];
+            if (!mk) continue;
 #pragma unroll
             for (int k = 0; k < S * S; ++k) {
                 const int kx = k % S, ky = k / S;
-                if (!((wc.mask[c] >> k) & 1ull)) continue;
+                if (!((mk >> k) & 1ull)) continue;
                 const int a = src.bound(kx, ky), e = src.bound(kx + 1, ky);
-#pragma unroll 2
-                for (int h = a; h < e; ++h) add(load(src.pos(h, ky), S - 1 - kx, S - 1 - ky), S - 1 - kx, S - 1 - ky);
+#pragma unroll S <= 3 ? 2 : 1
+                for (int h = a; h < e; ++h)
+                    add(load(ENTRY ? h : src.pos(h, ky), S - 1 - kx, S - 1 - ky), S - 1 - kx, S - 1 - ky);
             }
         }
     } else {
@@ -596,26 +628,64 @@ __device__ __forceinline__ PlasmaRec load_plasma_rec(const double* __restrict__ 
     return {rec[pos], rec[n + pos], rec[2 * n + pos], rec[3 * n + pos], rec[4 * n + pos], rec[5 * n + pos],
             rec[6 * n + pos]};
 }
-struct BeamRec {
-    double wqx, wqy, wqz, sx, sy;
+// The staging area of a plasma gather: the fast pass stages the records with the lists (the
+// block's window particles are read from global memory once, each then serves every cell of
+// its stencil from LDS); the merge pass stages list indices only, for its larger blocks.
+template <bool FAST>
+using PlasmaStage = StagedRecs<FAST ? GS_CAP_REC : GS_CAP, FAST ? PLASMA_NREC : 0>;
+template <bool FAST>
+struct PlasmaRecs {
+    PlasmaStage<FAST>& st;
+    const double* __restrict__ rec;
+    long n;
+    // staged entry t <- record position pos
+    __device__ __forceinline__ void operator()(int t, int pos) const {
+        if constexpr (FAST) {
+#pragma unroll
+            for (int f = 0; f < PLASMA_NREC; ++f) st.rec[f][t] = rec[f * n + pos];
+        }
+    }
+    // the record of staged entry `at` (FAST) or of record position `at`
+    __device__ __forceinline__ PlasmaRec load(int at) const {
+        if constexpr (FAST) {
+            return {st.rec[0][at], st.rec[1][at], st.rec[2][at], st.rec[3][at], st.rec[4][at], st.rec[5][at],
+                    st.rec[6][at]};
+        } else {
+            return load_plasma_rec(rec, n, at);
+        }
+    }
 };
 
+// threads of a plasma gather workgroup: the fast pass runs two threads per cell, each with
+// its own share of the cell's sums (the cell's walk is the same for both), so each SIMD holds
+// two wavefronts with the workgroup's LDS; the merge pass one thread per cell
+template <bool FAST>
+constexpr int plasma_gather_threads() { return FAST ? 2 * GS * GS : GS * GS; }
+
 // ::DepositCurrent's per-cell sums, PlasmaDepositCurrent.cpp:199-283: comps jx, jy, jz, rho,
-// chi, rhomjz (-1: skip; jx and jy go together)
+// chi, rhomjz (-1: skip; jx and jy go together). Fast pass: the first thread of a cell sums
+// jx, jy and jz, the second rho, chi and rhomjz.
 template <int O, bool STAGED>
-__global__ __launch_bounds__(GS * GS) void k_gather_current(Geom g, CLView cl, const double* __restrict__ rec, long n,
-                                                            double* __restrict__ store, int cjx, int cjy, int cjz,
-                                                            int crho, int cchi, int crm,
-                                                            double charge_mu0_mass_ratio, int32_t* __restrict__ ovf) {
-    constexpr int S = O + 1, DELTA = shape_delta<O>(), CAP = GS_CAP;
-    __shared__ StagedLists<CAP> sl;
+__global__ __launch_bounds__(plasma_gather_threads<STAGED>()) void k_gather_current(
+    Geom g, CLView cl, const double* __restrict__ rec, long n, double* __restrict__ store, int cjx, int cjy, int cjz,
+    int crho, int cchi, int crm, double charge_mu0_mass_ratio, int32_t* __restrict__ ovf) {
+    constexpr int S = O + 1, DELTA = shape_delta<O>();
+    __shared__ PlasmaStage<STAGED> st;
+    auto& sl = st.l;
+    const PlasmaRecs<STAGED> pr{st, rec, n};
     int b, i0, j0;
-    if (!gather_setup<STAGED, S, DELTA>(g, cl, sl, ovf, b, i0, j0)) return;
-    const int li = int(threadIdx.x % GS), lj = int(threadIdx.x / GS);
+    if (!gather_setup<STAGED, S, DELTA>(g, cl, sl, ovf, b, i0, j0, pr)) return;
+    const int cell = int(threadIdx.x) % (GS * GS), half = int(threadIdx.x) / (GS * GS);
+    const int li = cell % GS, lj = cell / GS;
     const int i = i0 + li, j = j0 + lj;
     const bool in = i <= g.nx - 1 + g.ng && j <= g.ny - 1 + g.ng;
-    if (STAGED && !block_raster_ordered<S>(g, cl, sl, in, i, j, li, lj, b, ovf)) return;
+    if (STAGED && !block_raster_ordered<S>(g, cl, sl, in && half == 0, i, j, li, lj, b, ovf)) return;
     if (!in) return;
+    if (STAGED) {
+        if (half == 0) crho = cchi = crm = -1;
+        else cjx = cjy = cjz = -1;
+        if (cjx < 0 && cjz < 0 && crho < 0 && cchi < 0 && crm < 0) return;
+    }
     const long k = g.idx(i, j);
     const long ps = g.plane;
     double ajx = cjx >= 0 ? store[cjx * ps + k] : 0.0, ajy = cjy >= 0 ? store[cjy * ps + k] : 0.0;
@@ -624,7 +694,7 @@ __global__ __launch_bounds__(GS * GS) void k_gather_current(Geom g, CLView cl, c
     int hits = 0;
     const double clight = 1.0;
     const double q_mu0_mass_ratio = charge_mu0_mass_ratio;
-    walk_cell<S, DELTA, STAGED>(g, cl, sl, i, j, li, lj, [&](int pos, int, int) { return load_plasma_rec(rec, n, pos); },
+    walk_cell<S, DELTA, STAGED, STAGED>(g, cl, sl, i, j, li, lj, [&](int at, int, int) { return pr.load(at); },
                                   [&](const PlasmaRec& r, int ix, int iy) {
                                       const double sx = shape<O>(r.xmid, ix).s;
                                       const double sy = shape<O>(r.ymid, iy).s;
@@ -651,21 +721,24 @@ __global__ __launch_bounds__(GS * GS) void k_gather_current(Geom g, CLView cl, c
 }
 
 // ::ExplicitDeposition's per-cell sums, ExplicitDeposition.cpp:186-262 (no laser), with the
-// cell's own Bz, Ez, ExmBy, EypBx
+// cell's own Bz, Ez, ExmBy, EypBx. Fast pass: the first thread of a cell sums Sy, the second Sx.
 template <int T, int O, bool STAGED>
-__global__ __launch_bounds__(GS * GS) void k_gather_explicit(Geom g, CLView cl, const double* __restrict__ rec, long n,
-                                                             double* __restrict__ store, int cBz, int cEz, int cExmBy,
-                                                             int cEypBx, int cSy, int cSx, double charge_mass_ratio,
-                                                             int32_t* __restrict__ ovf) {
-    constexpr int S = dshape_width<T, O>(), DELTA = dshape_delta<T, O>(), CAP = GS_CAP;
-    __shared__ StagedLists<CAP> sl;
+__global__ __launch_bounds__(plasma_gather_threads<STAGED>()) void k_gather_explicit(
+    Geom g, CLView cl, const double* __restrict__ rec, long n, double* __restrict__ store, int cBz, int cEz, int cExmBy,
+    int cEypBx, int cSy, int cSx, double charge_mass_ratio, int32_t* __restrict__ ovf) {
+    constexpr int S = dshape_width<T, O>(), DELTA = dshape_delta<T, O>();
+    __shared__ PlasmaStage<STAGED> st;
+    auto& sl = st.l;
+    const PlasmaRecs<STAGED> pr{st, rec, n};
     int b, i0, j0;
-    if (!gather_setup<STAGED, S, DELTA>(g, cl, sl, ovf, b, i0, j0)) return;
-    const int li = int(threadIdx.x % GS), lj = int(threadIdx.x / GS);
+    if (!gather_setup<STAGED, S, DELTA>(g, cl, sl, ovf, b, i0, j0, pr)) return;
+    const int cell = int(threadIdx.x) % (GS * GS), half = int(threadIdx.x) / (GS * GS);
+    const int li = cell % GS, lj = cell / GS;
     const int i = i0 + li, j = j0 + lj;
     const bool in = i <= g.nx - 1 + g.ng && j <= g.ny - 1 + g.ng;
-    if (STAGED && !block_raster_ordered<S>(g, cl, sl, in, i, j, li, lj, b, ovf)) return;
+    if (STAGED && !block_raster_ordered<S>(g, cl, sl, in && half == 0, i, j, li, lj, b, ovf)) return;
     if (!in) return;
+    const bool do_sy = !STAGED || half == 0, do_sx = !STAGED || half == 1;
     const long k = g.idx(i, j);
     const long ps = g.plane;
     const double Bz_v = store[cBz * ps + k], Ez_v = store[cEz * ps + k];
@@ -675,8 +748,8 @@ __global__ __launch_bounds__(GS * GS) void k_gather_explicit(Geom g, CLView cl, 
     const double clight = 1.0, clight_inv = 1.0 / 1.0;
     const double dxi = g.dxi, dyi = g.dyi;
     const double q_mass_ratio = charge_mass_ratio;
-    walk_cell<S, DELTA, STAGED>(
-        g, cl, sl, i, j, li, lj, [&](int pos, int, int) { return load_plasma_rec(rec, n, pos); },
+    walk_cell<S, DELTA, STAGED, STAGED>(
+        g, cl, sl, i, j, li, lj, [&](int at, int, int) { return pr.load(at); },
         [&](const PlasmaRec& r, int ix, int iy) {
             if (T == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) return;
             const double charge_density_mu0 = r.q_invvol, vx = r.vx_c, vy = r.vy_c;
@@ -684,72 +757,441 @@ __global__ __launch_bounds__(GS * GS) void k_gather_explicit(Geom g, CLView cl, 
             const Dsf sy = dshape<T, O>(r.ymid, iy);
             const Dsf sx = dshape<T, O>(r.xmid, ix);
             const double AabssqDxp = 0.0, AabssqDyp = 0.0;
-            aSy += charge_density_mu0 * (
-                -sx.s * sy.s * (
-                    -Bz_v * vx
-                    + (Ez_v * vy
-                    + ExmBy_v * (-vx * vy)
-                    + EypBx_v * (gamma_psi - vy * vy)) * clight_inv
-                    - 0.25 * AabssqDyp * q_mass_ratio * psi_inv
-                ) * q_mass_ratio * psi_inv
-                + (-sx.ds * sy.s * dxi * (-vx * vy)
-                   - sx.s * sy.ds * dyi * (gamma_psi - vy * vy - 1.0)) * clight);
-            aSx += charge_density_mu0 * (
-                +sx.s * sy.s * (
-                    +Bz_v * vy
-                    + (Ez_v * vx
-                    + ExmBy_v * (gamma_psi - vx * vx)
-                    + EypBx_v * (-vx * vy)) * clight_inv
-                    - 0.25 * AabssqDxp * q_mass_ratio * psi_inv
-                ) * q_mass_ratio * psi_inv
-                + (+sx.ds * sy.s * dxi * (gamma_psi - vx * vx - 1.0)
-                   + sx.s * sy.ds * dyi * (-vx * vy)) * clight);
+            if (do_sy) {
+                aSy += charge_density_mu0 * (
+                    -sx.s * sy.s * (
+                        -Bz_v * vx
+                        + (Ez_v * vy
+                        + ExmBy_v * (-vx * vy)
+                        + EypBx_v * (gamma_psi - vy * vy)) * clight_inv
+                        - 0.25 * AabssqDyp * q_mass_ratio * psi_inv
+                    ) * q_mass_ratio * psi_inv
+                    + (-sx.ds * sy.s * dxi * (-vx * vy)
+                       - sx.s * sy.ds * dyi * (gamma_psi - vy * vy - 1.0)) * clight);
+            }
+            if (do_sx) {
+                aSx += charge_density_mu0 * (
+                    +sx.s * sy.s * (
+                        +Bz_v * vy
+                        + (Ez_v * vx
+                        + ExmBy_v * (gamma_psi - vx * vx)
+                        + EypBx_v * (-vx * vy)) * clight_inv
+                        - 0.25 * AabssqDxp * q_mass_ratio * psi_inv
+                    ) * q_mass_ratio * psi_inv
+                    + (+sx.ds * sy.s * dxi * (gamma_psi - vx * vx - 1.0)
+                       + sx.s * sy.ds * dyi * (-vx * vy)) * clight);
+            }
             ++hits;
         });
     if (hits == 0) return;
-    store[cSy * ps + k] = aSy;
-    store[cSx * ps + k] = aSx;
+    if (do_sy) store[cSy * ps + k] = aSy;
+    if (do_sx) store[cSx * ps + k] = aSx;
 }
 
-// ::DepositCurrentSlice's per-cell sums, BeamDepositCurrent.cpp:160-193
-template <int O, bool STAGED>
-__global__ __launch_bounds__(GS * GS) void k_gather_beam(Geom g, CLView cl, const double* __restrict__ rec, long n,
-                                                         double* __restrict__ store, int cjx, int cjy, int cjz,
-                                                         int32_t* __restrict__ ovf) {
-    constexpr int S = O + 1, DELTA = shape_delta<O>(), CAP = GS_CAP_BEAM;
-    __shared__ StagedLists<CAP> sl;
+// A cell's window lists as one sequence in the CPU build's order (colour by colour, the lists
+// of a colour in raster order), the lists of the mask `skip` left out. next() moves to the
+// next non-empty list: its entries [h, e) and stencil offset (ix, iy); false at the end. One
+// element per loop iteration keeps the lanes of a wavefront together however their list
+// lengths differ; the offset is then a run-time value.
+template <int S, int CAP>
+struct FlatWalk {
+    // lists left of the current colour, and of the colours after it (shifted down as the
+    // walk moves on a colour)
+    unsigned long long rem, m1, m2, m3;
+    StagedView<CAP> src;
+    int h = 0, e = 0, ix = 0, iy = 0;
+    __device__ __forceinline__ FlatWalk(const WindowColours<S>& w, const StagedView<CAP>& v, unsigned long long skip)
+        : rem(w.mask[0] & ~skip), m1(w.mask[1] & ~skip), m2(w.mask[2] & ~skip), m3(w.mask[3] & ~skip), src(v) {}
+    __device__ __forceinline__ bool next() {
+        while (true) {
+            while (rem == 0ull) {
+                if ((m1 | m2 | m3) == 0ull) return false;
+                rem = m1;
+                m1 = m2;
+                m2 = m3;
+                m3 = 0ull;
+            }
+            const int kk = __ffsll((long long)rem) - 1;
+            rem &= rem - 1ull;
+            const int kx = kk % S, ky = kk / S;
+            h = src.bound(kx, ky);
+            e = src.bound(kx + 1, ky);
+            if (h < e) {
+                ix = S - 1 - kx;
+                iy = S - 1 - ky;
+                return true;
+            }
+        }
+    }
+};
+
+// The order-2 shape factor (ShapeFactors.H compute_single_shape_factor, order 2) with the
+// offset at run time: with t = xmid - floor(xmid + 1/2), offsets 0 and 2 are (0.5 u) u for
+// u = 0.5 + sigma t, sigma = -1 and +1 (sigma t is exact and 0.5 + (-t) is 0.5 - t), offset 1
+// is 0.75 - t t: the reference's expressions bit for bit.
+__device__ __forceinline__ double shape2_rt(double t, double sigma, bool mid) {
+    const double u = 0.5 + sigma * t;
+    const double a = 0.5 * u * u;
+    const double b = 0.75 - t * t;
+    return mid ? b : a;
+}
+
+// The centred (type 2) order-2 derivative shape factors as one quadratic per stencil offset
+// q (ShapeFactors.H single_derivative_shape_factor, centred, order 2): with
+// t = (xmid + 1/2) - floor(xmid + 1/2) in [0, 1), s = (A2 t^2 + A1 t) + A0 and the reference's
+// sdx = (B2 t^2 + B1 t) + B0 (ds = -sdx); row q = {A2, A1, A0, B2, B1, B0}. Evaluated in this
+// form every row gives the reference's expression for its offset bit for bit: the
+// coefficients 0 and +-1 multiply exactly, x + (-y) is x - y, and a vanishing term adds +0 to
+// a value that is +0 or keeps its own sign. So a walk can take the offset at run time.
+__constant__ double CENTRED2[5][6] = {{0.0, 0.0, 0.0, -0.25, 0.5, -0.25},
+                                      {0.5, -1.0, 0.5, 0.5, -0.5, -0.25},
+                                      {-1.0, 1.0, 0.5, 0.0, -0.5, 0.25},
+                                      {0.5, 0.0, 0.0, -0.5, 0.5, 0.25},
+                                      {0.0, 0.0, 0.0, 0.25, 0.0, 0.0}};
+struct Quad {
+    double a2, a1, a0, b2, b1, b0;
+};
+
+// The fast pass of ::ExplicitDeposition for the centred order-2 shape (the default
+// derivative type): the same per-cell sums as k_gather_explicit<2, 2, true>, but each cell
+// walks its window as one sequence (colour by colour, the lists of a colour in raster order,
+// the four corner lists skipped), one element per iteration with the stencil offset at run
+// time and the shape factors from the rows of CENTRED2, so the lanes of a wavefront stay
+// together however their list lengths differ (the two wavefronts per SIMD cover each other's
+// LDS reads). The staged records hold t of x and y in place of the cell-scaled positions.
+__global__ __launch_bounds__(2 * GS * GS) void k_gather_explicit_c2(
+    Geom g, CLView cl, const double* __restrict__ rec, long n, double* __restrict__ store, int cBz, int cEz, int cExmBy,
+    int cEypBx, int cSy, int cSx, double charge_mass_ratio, int32_t* __restrict__ ovf) {
+    constexpr int T = 2, O = 2, S = dshape_width<T, O>(), DELTA = dshape_delta<T, O>();
+    constexpr unsigned long long CORNERS = 1ull | (1ull << (S - 1)) | (1ull << (S * (S - 1))) | (1ull << (S * S - 1));
+    __shared__ PlasmaStage<true> st;
+    __shared__ Quad quad[S];
+    auto& sl = st.l;
+    if (threadIdx.x < S) {
+        const int q = int(threadIdx.x);
+        quad[q] = Quad{CENTRED2[q][0], CENTRED2[q][1], CENTRED2[q][2], CENTRED2[q][3], CENTRED2[q][4], CENTRED2[q][5]};
+    }
+    auto copy = [&](int t, int pos) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) st.rec[f][t] = rec[f * n + pos];
+        const double xm = rec[5 * n + pos], ym = rec[6 * n + pos];
+        const double xf = floor(xm + 0.5), yf = floor(ym + 0.5);
+        st.rec[5][t] = xm + 0.5 - xf;
+        st.rec[6][t] = ym + 0.5 - yf;
+    };
     int b, i0, j0;
-    if (!gather_setup<STAGED, S, DELTA>(g, cl, sl, ovf, b, i0, j0)) return;
-    const int li = int(threadIdx.x % GS), lj = int(threadIdx.x / GS);
+    if (!gather_setup<true, S, DELTA>(g, cl, sl, ovf, b, i0, j0, copy)) return;
+    const int cell = int(threadIdx.x) % (GS * GS), half = int(threadIdx.x) / (GS * GS);
+    const int li = cell % GS, lj = cell / GS;
     const int i = i0 + li, j = j0 + lj;
     const bool in = i <= g.nx - 1 + g.ng && j <= g.ny - 1 + g.ng;
-    if (STAGED && !block_raster_ordered<S>(g, cl, sl, in, i, j, li, lj, b, ovf)) return;
+    if (!block_raster_ordered<S>(g, cl, sl, in && half == 0, i, j, li, lj, b, ovf)) return;
     if (!in) return;
     const long k = g.idx(i, j);
     const long ps = g.plane;
-    double ajx = cjx >= 0 ? store[cjx * ps + k] : 0.0, ajy = cjy >= 0 ? store[cjy * ps + k] : 0.0;
-    double ajz = cjz >= 0 ? store[cjz * ps + k] : 0.0;
+    const double Bz_v = store[cBz * ps + k], Ez_v = store[cEz * ps + k];
+    const double ExmBy_v = store[cExmBy * ps + k], EypBx_v = store[cEypBx * ps + k];
+    const int cout = half == 0 ? cSy : cSx;
+    double acc = store[cout * ps + k];
+    const double clight = 1.0, clight_inv = 1.0 / 1.0;
+    const double dxi = g.dxi, dyi = g.dyi;
+    const double q_mass_ratio = charge_mass_ratio;
+    const double AabssqDxp = 0.0, AabssqDyp = 0.0;
+    const WindowColours<S> wc(g, cl, i, j);
+    const StagedView<GS_CAP_REC> src{sl, li, lj};
+    FlatWalk<S, GS_CAP_REC> w(wc, src, CORNERS);
+    auto load = [&](int at) {
+        return PlasmaRec{st.rec[0][at], st.rec[1][at], st.rec[2][at], st.rec[3][at],
+                         st.rec[4][at], st.rec[5][at], st.rec[6][at]};
+    };
     int hits = 0;
-    walk_cell<S, DELTA, STAGED>(
-        g, cl, sl, i, j, li, lj,
-        [&](int pos, int ix, int iy) {
-            return BeamRec{rec[pos], rec[n + pos], rec[2 * n + pos], rec[(3 + ix) * n + pos],
-                           rec[(3 + O + 1 + iy) * n + pos]};
-        },
-        [&](const BeamRec& r, int, int) {
-            if (cjx >= 0) {
-                ajx += r.sx * r.sy * r.wqx;
-                ajy += r.sx * r.sy * r.wqy;
+    auto run = [&](auto HALF) {
+        constexpr bool SY = decltype(HALF)::value == 0;
+        Quad qx{}, qy{};
+        while (true) {
+            if (w.h == w.e) {
+                if (!w.next()) break;
+                qx = quad[w.ix];
+                qy = quad[w.iy];
             }
-            if (cjz >= 0) ajz += r.sx * r.sy * r.wqz;
+            const PlasmaRec r = load(w.h++);
+            const double charge_density_mu0 = r.q_invvol, vx = r.vx_c, vy = r.vy_c;
+            const double gamma_psi = r.gamma_psi, psi_inv = r.psi_inv;
+            const double tx = r.xmid, ty = r.ymid;
+            const double tx2 = tx * tx, ty2 = ty * ty;
+            const double sx_s = (qx.a2 * tx2 + qx.a1 * tx) + qx.a0;
+            const double sx_ds = -((qx.b2 * tx2 + qx.b1 * tx) + qx.b0);
+            const double sy_s = (qy.a2 * ty2 + qy.a1 * ty) + qy.a0;
+            const double sy_ds = -((qy.b2 * ty2 + qy.b1 * ty) + qy.b0);
+            if constexpr (SY) {
+                acc += charge_density_mu0 * (
+                    -sx_s * sy_s * (
+                        -Bz_v * vx
+                        + (Ez_v * vy
+                        + ExmBy_v * (-vx * vy)
+                        + EypBx_v * (gamma_psi - vy * vy)) * clight_inv
+                        - 0.25 * AabssqDyp * q_mass_ratio * psi_inv
+                    ) * q_mass_ratio * psi_inv
+                    + (-sx_ds * sy_s * dxi * (-vx * vy)
+                       - sx_s * sy_ds * dyi * (gamma_psi - vy * vy - 1.0)) * clight);
+            } else {
+                acc += charge_density_mu0 * (
+                    +sx_s * sy_s * (
+                        +Bz_v * vy
+                        + (Ez_v * vx
+                        + ExmBy_v * (gamma_psi - vx * vx)
+                        + EypBx_v * (-vx * vy)) * clight_inv
+                        - 0.25 * AabssqDxp * q_mass_ratio * psi_inv
+                    ) * q_mass_ratio * psi_inv
+                    + (+sx_ds * sy_s * dxi * (gamma_psi - vx * vx - 1.0)
+                       + sx_s * sy_ds * dyi * (-vx * vy)) * clight);
+            }
             ++hits;
-        });
+        }
+    };
+    if (half == 0) run(std::integral_constant<int, 0>{});
+    else run(std::integral_constant<int, 1>{});
+    if (hits == 0) return;
+    store[cout * ps + k] = acc;
+}
+
+// The fast pass of ::DepositCurrent for order 2: the per-cell sums of
+// k_gather_current<2, true>, each cell walking its window as one sequence with the offset at
+// run time (shape2_rt). The staged records hold t of x and y in place of the cell-scaled
+// positions. The first thread of a cell sums jx, jy and jz, the second rho, chi and rhomjz.
+__global__ __launch_bounds__(2 * GS * GS) void k_gather_current_o2(
+    Geom g, CLView cl, const double* __restrict__ rec, long n, double* __restrict__ store, int cjx, int cjy, int cjz,
+    int crho, int cchi, int crm, double charge_mu0_mass_ratio, int32_t* __restrict__ ovf) {
+    constexpr int O = 2, S = O + 1, DELTA = shape_delta<O>();
+    __shared__ PlasmaStage<true> st;
+    auto& sl = st.l;
+    auto copy = [&](int t, int pos) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) st.rec[f][t] = rec[f * n + pos];
+        const double xm = rec[5 * n + pos], ym = rec[6 * n + pos];
+        st.rec[5][t] = xm - floor(xm + 0.5);
+        st.rec[6][t] = ym - floor(ym + 0.5);
+    };
+    int b, i0, j0;
+    if (!gather_setup<true, S, DELTA>(g, cl, sl, ovf, b, i0, j0, copy)) return;
+    const int cell = int(threadIdx.x) % (GS * GS), half = int(threadIdx.x) / (GS * GS);
+    const int li = cell % GS, lj = cell / GS;
+    const int i = i0 + li, j = j0 + lj;
+    const bool in = i <= g.nx - 1 + g.ng && j <= g.ny - 1 + g.ng;
+    if (!block_raster_ordered<S>(g, cl, sl, in && half == 0, i, j, li, lj, b, ovf)) return;
+    if (!in) return;
+    if (half == 0) crho = cchi = crm = -1;
+    else cjx = cjy = cjz = -1;
+    if (cjx < 0 && cjz < 0 && crho < 0 && cchi < 0 && crm < 0) return;
+    const long k = g.idx(i, j);
+    const long ps = g.plane;
+    double ajx = cjx >= 0 ? store[cjx * ps + k] : 0.0, ajy = cjx >= 0 ? store[cjy * ps + k] : 0.0;
+    double ajz = cjz >= 0 ? store[cjz * ps + k] : 0.0, arho = crho >= 0 ? store[crho * ps + k] : 0.0;
+    double achi = cchi >= 0 ? store[cchi * ps + k] : 0.0, arm = crm >= 0 ? store[crm * ps + k] : 0.0;
+    const double clight = 1.0;
+    const double q_mu0_mass_ratio = charge_mu0_mass_ratio;
+    const WindowColours<S> wc(g, cl, i, j);
+    const StagedView<GS_CAP_REC> src{sl, li, lj};
+    FlatWalk<S, GS_CAP_REC> w(wc, src, 0ull);
+    double sgx = 0.0, sgy = 0.0;
+    bool midx = false, midy = false;
+    int hits = 0;
+    while (true) {
+        if (w.h == w.e) {
+            if (!w.next()) break;
+            sgx = w.ix == 0 ? -1.0 : 1.0;
+            sgy = w.iy == 0 ? -1.0 : 1.0;
+            midx = w.ix == 1;
+            midy = w.iy == 1;
+        }
+        const int at = w.h++;
+        const double q_invvol = st.rec[0][at], tx = st.rec[5][at], ty = st.rec[6][at];
+        const double sx = shape2_rt(tx, sgx, midx);
+        const double sy = shape2_rt(ty, sgy, midy);
+        const double charge_density = q_invvol * sx * sy;
+        if (cjx >= 0) {
+            ajx += charge_density * st.rec[1][at];
+            ajy += charge_density * st.rec[2][at];
+        }
+        if (cjz >= 0) ajz += charge_density * (st.rec[3][at] - 1.0) * clight;
+        if (crho >= 0) arho += charge_density * st.rec[3][at];
+        if (cchi >= 0) achi += charge_density * q_mu0_mass_ratio * st.rec[4][at];
+        if (crm >= 0) arm += charge_density;
+        ++hits;
+    }
     if (hits == 0) return;
     if (cjx >= 0) {
         store[cjx * ps + k] = ajx;
         store[cjy * ps + k] = ajy;
     }
     if (cjz >= 0) store[cjz * ps + k] = ajz;
+    if (crho >= 0) store[crho * ps + k] = arho;
+    if (cchi >= 0) store[cchi * ps + k] = achi;
+    if (crm >= 0) store[crm * ps + k] = arm;
+}
+
+// ---- beam: per-cell contribution sequences -----------------------------------------------
+// A beam's core puts hundreds of particles in a cell, in random index order, so merging a
+// cell's window lists one element at a time would leave one thread per cell walking thousands
+// of entries. Instead every (particle, stencil offset) pair finds its own place in its target
+// cell's sequence at once: the sequence is the window's lists colour by colour, each colour
+// merged by particle index, so a pair's place is the length of the lists of earlier colours
+// plus, for every list of its own colour, the number of its entries with a smaller index (a
+// binary search; the pair's own list gives its list rank). The pair writes its contribution
+// there; then each cell adds its sequence in order to the plane's value, one thread per cell,
+// or one wavefront per cell where the sequence is long.
+constexpr int BEAM_LONG = 32;   // longer sequences: a wavefront per cell
+
+// sequence length of every plane cell: the entries of its window lists (thread per cell)
+template <int O>
+__global__ __launch_bounds__(256) void k_beam_seg_count(Geom g, CLView cl, int32_t* __restrict__ seg) {
+    constexpr int S = O + 1, DELTA = shape_delta<O>();
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t > g.plane) return;
+    if (t == g.plane) {
+        seg[t] = 0;
+        return;
+    }
+    const int i = int(t % g.ldx) - g.ng, j = int(t / g.ldx) - g.ng;
+    const int col0 = i - S + 1 - DELTA - cl.bxlo, row0 = j - S + 1 - DELTA - cl.bylo;
+    int len = 0;
+#pragma unroll
+    for (int ky = 0; ky < S; ++ky) {
+        const long r = long(row0 + ky) * cl.wx;
+        len += cl.off[r + col0 + S] - cl.off[r + col0];
+    }
+    seg[t] = len;
+}
+
+// entries of perm[a, b) (increasing) below v
+__device__ __forceinline__ int count_below(const int32_t* __restrict__ perm, int a, int b, int v) {
+    int lo = a, hi = b;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (perm[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - a;
+}
+
+// pair (list entry pos, stencil offset q): its place in the target cell's sequence and its
+// contribution (BeamDepositCurrent.cpp:160-193: wq v times the two shape factors)
+template <int O>
+__global__ __launch_bounds__(256) void k_beam_pairs(Geom g, CLView cl, const int32_t* __restrict__ key,
+                                                    const double* __restrict__ rec, long n,
+                                                    const int32_t* __restrict__ seg, double* __restrict__ contrib,
+                                                    long m) {
+    constexpr int S = O + 1, DELTA = shape_delta<O>();
+    const long tid = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long pl = tid / (S * S);
+    const int q = int(tid - pl * (S * S));
+    if (pl >= cl.off[long(cl.wx) * cl.wy]) return;
+    const int pos = int(pl);
+    const int p = cl.perm[pos];
+    const int list = key[p];
+    const int ix = q % S, iy = q / S;
+    const int i = list % cl.wx + cl.bxlo + DELTA + ix, j = list / cl.wx + cl.bylo + DELTA + iy;
+    if (i < -g.ng || i > g.nx - 1 + g.ng || j < -g.ng || j > g.ny - 1 + g.ng) return;
+    const WindowColours<S> wc(g, cl, i, j);
+    const int kp = (S - 1 - iy) * S + (S - 1 - ix);
+    int cp = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        if ((wc.mask[c] >> kp) & 1ull) cp = c;
+    const int col0 = i - S + 1 - DELTA - cl.bxlo, row0 = j - S + 1 - DELTA - cl.bylo;
+    int rank = pos - cl.off[list];
+#pragma unroll
+    for (int k = 0; k < S * S; ++k) {
+        if (k == kp) continue;
+        int c = 0;
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc)
+            if ((wc.mask[cc] >> k) & 1ull) c = cc;
+        if (c > cp) continue;
+        const long L = long(row0 + k / S) * cl.wx + col0 + k % S;
+        const int a = cl.off[L], e = cl.off[L + 1];
+        rank += c < cp ? e - a : count_below(cl.perm, a, e, p);
+    }
+    const long out = long(seg[g.idx(i, j)]) + rank;
+    const double sx = rec[(3 + ix) * n + pos], sy = rec[(3 + O + 1 + iy) * n + pos];
+    contrib[out] = sx * sy * rec[pos];
+    contrib[m + out] = sx * sy * rec[n + pos];
+    contrib[2 * m + out] = sx * sy * rec[2 * n + pos];
+}
+
+// the sums of cells with short sequences (thread per cell); longer ones are listed in longc
+// (count at longc[0]) for k_beam_sum_long. jx and jy go together, as in the deposit functor.
+__global__ __launch_bounds__(256) void k_beam_sum_short(Geom g, const int32_t* __restrict__ seg,
+                                                        const double* __restrict__ contrib, long m,
+                                                        double* __restrict__ store, int cjx, int cjy, int cjz,
+                                                        int32_t* __restrict__ longc) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= g.plane) return;
+    const int a = seg[t], e = seg[t + 1];
+    if (a == e) return;
+    if (e - a > BEAM_LONG) {
+        longc[1 + atomicAdd(&longc[0], 1)] = int32_t(t);
+        return;
+    }
+    const long ps = g.plane;
+    if (cjx >= 0) {
+        double ajx = store[cjx * ps + t], ajy = store[cjy * ps + t];
+        for (int h = a; h < e; ++h) {
+            ajx += contrib[h];
+            ajy += contrib[m + h];
+        }
+        store[cjx * ps + t] = ajx;
+        store[cjy * ps + t] = ajy;
+    }
+    if (cjz >= 0) {
+        double ajz = store[cjz * ps + t];
+        for (int h = a; h < e; ++h) ajz += contrib[2 * m + h];
+        store[cjz * ps + t] = ajz;
+    }
+}
+
+// the sums of the listed cells, a wavefront per cell: chunks of 64 contributions per component
+// are read by the whole wavefront into LDS (the next chunk already in flight), and lanes 0-2
+// add the jx, jy and jz chunks in sequence order
+__global__ __launch_bounds__(64) void k_beam_sum_long(Geom g, const int32_t* __restrict__ seg,
+                                                      const double* __restrict__ contrib, long m,
+                                                      double* __restrict__ store, int cjx, int cjy, int cjz,
+                                                      const int32_t* __restrict__ longc) {
+    __shared__ double buf[3][64];
+    const int lane = int(threadIdx.x);
+    const long ps = g.plane;
+    const int comp = lane == 0 ? cjx : lane == 1 ? (cjx >= 0 ? cjy : -1) : lane == 2 ? cjz : -1;
+    const int nl = longc[0];
+    for (int w = int(blockIdx.x); w < nl; w += int(gridDim.x)) {
+        const long t = longc[1 + w];
+        const int a = seg[t], e = seg[t + 1];
+        double acc = comp >= 0 ? store[comp * ps + t] : 0.0;
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        if (a + lane < e) {
+            v0 = contrib[a + lane];
+            v1 = contrib[m + a + lane];
+            v2 = contrib[2 * m + a + lane];
+        }
+        for (int h0 = a; h0 < e; h0 += 64) {
+            __syncthreads();
+            buf[0][lane] = v0;
+            buf[1][lane] = v1;
+            buf[2][lane] = v2;
+            __syncthreads();
+            const int hn = h0 + 64 + lane;
+            if (hn < e) {
+                v0 = contrib[hn];
+                v1 = contrib[m + hn];
+                v2 = contrib[2 * m + hn];
+            }
+            if (comp >= 0) {
+                const int cnt = min(64, e - h0);
+                for (int u = 0; u < cnt; ++u) acc += buf[lane][u];
+            }
+        }
+        if (comp >= 0) store[comp * ps + t] = acc;
+        __syncthreads();
+    }
 }
 
 template <class T>
@@ -825,7 +1267,7 @@ int shape_base_of(int order) {
 
 void cell_lists_free(CellLists& L) {
     for (void* q : {(void*)L.off, (void*)L.tmp, (void*)L.key, (void*)L.slot, (void*)L.unsorted, (void*)L.perm,
-                    (void*)L.rec, (void*)L.ovf})
+                    (void*)L.rec, (void*)L.ovf, (void*)L.seg, (void*)L.segtmp, (void*)L.longc, (void*)L.contrib})
         if (q) (void)hipFree(q);
     L = CellLists{};
 }
@@ -856,8 +1298,13 @@ void deposit_current_ordered(hpq_ctx* c, const hpq_plasma* p, double* store, con
         hipLaunchKernelGGL(k_lists_order_plasma, grid1d(p->n, 256), dim3(256), 0, c->stream, g, *p, charge_invvol,
                            L.key, L.off, L.unsorted, L.perm, L.rec, p->n);
         // the fast pass, then the blocks it listed (merged)
-        hipLaunchKernelGGL((k_gather_current<o, true>), gather_grid(g), dim3(GS * GS), 0, c->stream, g, view(L),
-                           L.rec, p->n, store, comps[0], comps[1], comps[2], comps[3], comps[4], comps[5], cmr, L.ovf);
+        if constexpr (o == 2)
+            hipLaunchKernelGGL(k_gather_current_o2, gather_grid(g), dim3(2 * GS * GS), 0, c->stream, g, view(L), L.rec,
+                               p->n, store, comps[0], comps[1], comps[2], comps[3], comps[4], comps[5], cmr, L.ovf);
+        else
+            hipLaunchKernelGGL((k_gather_current<o, true>), gather_grid(g), dim3(plasma_gather_threads<true>()), 0,
+                               c->stream, g, view(L), L.rec, p->n, store, comps[0], comps[1], comps[2], comps[3],
+                               comps[4], comps[5], cmr, L.ovf);
         hipLaunchKernelGGL((k_gather_current<o, false>), gather_grid(g), dim3(GS * GS), 0, c->stream, g, view(L),
                            L.rec, p->n, store, comps[0], comps[1], comps[2], comps[3], comps[4], comps[5], cmr, L.ovf);
     });
@@ -906,8 +1353,13 @@ void explicit_deposition_ordered(hpq_ctx* c, const hpq_plasma* p, double* store)
             // reused by several explicit depositions)
             int32_t* ovf = L.ovf + gather_blocks(g) + 1;
             hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, ovf, 1L);
-            hipLaunchKernelGGL((k_gather_explicit<t, o, true>), gather_grid(g), dim3(GS * GS), 0, c->stream, g,
-                               view(L), L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy, Ly.Sx, qmr, ovf);
+            if constexpr (t == 2 && o == 2)
+                hipLaunchKernelGGL(k_gather_explicit_c2, gather_grid(g), dim3(2 * GS * GS), 0, c->stream, g, view(L),
+                                   L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy, Ly.Sx, qmr, ovf);
+            else
+                hipLaunchKernelGGL((k_gather_explicit<t, o, true>), gather_grid(g), dim3(plasma_gather_threads<true>()), 0,
+                                   c->stream, g, view(L), L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy,
+                                   Ly.Sx, qmr, ovf);
             hipLaunchKernelGGL((k_gather_explicit<t, o, false>), gather_grid(g), dim3(GS * GS), 0, c->stream, g,
                                view(L), L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy, Ly.Sx, qmr, ovf);
         };
@@ -927,11 +1379,18 @@ void deposit_beam_ordered(hpq_ctx* c, const hpq_beam* b, int slice, double* stor
     const long first = b->off[slice];
     const long count = b->off[slice + 1] - first;
     if (count <= 0) return;
-    if (count > INT32_MAX) throw HpqError(HPQ_E_INVALID, "ordered deposition: more than 2^31-1 particles");
+    const int S = c->p.depos_order + 1;
+    if (count > INT32_MAX / (S * S)) throw HpqError(HPQ_E_INVALID, "ordered beam deposition: too many particles in a slice");
     lists_geometry(c, L);
     lists_reserve(g, L, count, BEAM_NREC);
+    grow(L.seg, L.seg_cap, g.plane + 1);
+    grow(L.segtmp, L.segtmp_cap, scan_tmp_size(g.plane + 1));
+    grow(L.longc, L.longc_cap, g.plane + 1);
+    const long m = long(S) * S * count;
+    grow(L.contrib, L.contrib_cap, 3 * m);
     TraceScope ts(c, HPQ_TRACE_BEAM_DEPOSIT);
     lists_clear(g, c->stream, L);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, L.longc, 1L);
     dispatch_order(c->p.depos_order, [&](auto O) {
         constexpr int o = decltype(O)::value;
         hipLaunchKernelGGL(k_keys_beam<o>, grid1d(count, 256), dim3(256), 0, c->stream, g, *b, first, count, view(L),
@@ -940,11 +1399,15 @@ void deposit_beam_ordered(hpq_ctx* c, const hpq_beam* b, int slice, double* stor
         lists_scatter(c->stream, L, count);
         hipLaunchKernelGGL(k_lists_order_beam<o>, grid1d(count, 256), dim3(256), 0, c->stream, g, *b, first, count,
                            L.key, L.off, L.unsorted, L.perm, L.rec);
-        hipLaunchKernelGGL((k_gather_beam<o, true>), gather_grid(g), dim3(GS * GS), 0, c->stream, g, view(L),
-                           L.rec, count, store, comps[0], comps[1], comps[2], L.ovf);
-        hipLaunchKernelGGL((k_gather_beam<o, false>), gather_grid(g), dim3(GS * GS), 0, c->stream, g, view(L),
-                           L.rec, count, store, comps[0], comps[1], comps[2], L.ovf);
+        hipLaunchKernelGGL(k_beam_seg_count<o>, grid1d(g.plane + 1, 256), dim3(256), 0, c->stream, g, view(L), L.seg);
+        scan_exclusive(c->stream, L.seg, g.plane + 1, L.segtmp);
+        hipLaunchKernelGGL(k_beam_pairs<o>, grid1d(m, 256), dim3(256), 0, c->stream, g, view(L), L.key, L.rec, count,
+                           L.seg, L.contrib, m);
     });
+    hipLaunchKernelGGL(k_beam_sum_short, grid1d(g.plane, 256), dim3(256), 0, c->stream, g, L.seg, L.contrib, m, store,
+                       comps[0], comps[1], comps[2], L.longc);
+    hipLaunchKernelGGL(k_beam_sum_long, dim3(2048), dim3(64), 0, c->stream, g, L.seg, L.contrib, m, store, comps[0],
+                       comps[1], comps[2], L.longc);
     HPQ_HIP(hipGetLastError());
 }
 

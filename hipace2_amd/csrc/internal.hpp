@@ -173,6 +173,12 @@ struct CellLists {
     long rec_cap = 0;
     int32_t* ovf = nullptr;      // overflow lists of the staged gathers
     long ovf_cap = 0;
+    // the beam's per-cell contribution sequences: segment offsets per plane cell, their scan
+    // scratch, the contributions (3 arrays), the cells summed by a wavefront each
+    int32_t *seg = nullptr, *segtmp = nullptr, *longc = nullptr;
+    long seg_cap = 0, segtmp_cap = 0, longc_cap = 0;
+    double* contrib = nullptr;
+    long contrib_cap = 0;
     bool valid = false;
     int base = -1;
     const double* x = nullptr;
