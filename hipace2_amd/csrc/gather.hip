@@ -167,12 +167,12 @@ constexpr int GS = 16;             // cells per side of a gather block
 constexpr int GS_MAXW = GS + 5;    // widest window side (stencils of at most 6 cells)
 constexpr int GS_CAP = 6144;       // staged list entries of the plasma gathers (24 KiB)
 constexpr int GS_CAP_REC = 2560;   // of the plasma fast passes, with their records (150 KiB)
-template <int CAP>
+template <int CAP, bool IDX = true>
 struct StagedLists {
     int off[GS_MAXW][GS_MAXW + 1];   // window list bounds, in staged entries
     int base[GS_MAXW];               // record position = staged entry + base[row]
     int total;                       // staged entries (more than CAP: not staged)
-    int idx[CAP];                    // particle index of every staged entry
+    int idx[IDX ? CAP : 1];          // particle index of every staged entry (IDX)
 };
 // staged lists with the NF record fields of every staged entry (NF = 0: indices only)
 template <int CAP, int NF>
@@ -199,8 +199,8 @@ __device__ __forceinline__ int gather_nby(const Geom& g) { return (g.ldy + GS - 
 // stages the rows of lists below block (i0, j0) for stencils of S cells starting DELTA from
 // the base cell, and calls copy(t, pos) for every staged entry t (record position pos); every
 // thread of the workgroup calls it
-template <int S, int DELTA, int CAP, class Copy>
-__device__ __forceinline__ void stage_lists(const CLView& cl, int i0, int j0, StagedLists<CAP>& sl, Copy&& copy) {
+template <int S, int DELTA, int CAP, bool IDX, class Copy>
+__device__ __forceinline__ void stage_lists(const CLView& cl, int i0, int j0, StagedLists<CAP, IDX>& sl, Copy&& copy) {
     constexpr int NW = GS + S - 1;
     static_assert(NW <= GS_MAXW, "window wider than the staging area");
     const int col0 = i0 - S + 1 - DELTA - cl.bxlo;
@@ -249,7 +249,7 @@ __device__ __forceinline__ void stage_lists(const CLView& cl, int i0, int j0, St
             for (int u = 0; u < U; ++u) {
                 const int t = t0 + u * int(blockDim.x);
                 if (pos[u] >= 0) {
-                    sl.idx[t] = cl.perm[pos[u]];
+                    if constexpr (IDX) sl.idx[t] = cl.perm[pos[u]];
                     copy(t, pos[u]);
                 }
             }
@@ -342,8 +342,8 @@ __device__ __forceinline__ bool raster_ordered(const WindowColours<S>& wc, const
 // list ovf (ovf[0] blocks in ovf[1..]) and left to the second pass. MERGE: the blockIdx.x-th
 // listed block, each cell merging its lists (from LDS when they fit, from global memory when
 // they do not). The fast pass keeps few registers; the merge's state stays in the second.
-template <bool FAST, int S, int DELTA, int CAP, class Copy>
-__device__ __forceinline__ bool gather_setup(const Geom& g, const CLView& cl, StagedLists<CAP>& sl,
+template <bool FAST, int S, int DELTA, int CAP, bool IDX, class Copy>
+__device__ __forceinline__ bool gather_setup(const Geom& g, const CLView& cl, StagedLists<CAP, IDX>& sl,
                                              int32_t* __restrict__ ovf, int& b, int& i0, int& j0, Copy&& copy) {
     const int nbx = gather_nbx(g);
     if constexpr (FAST) {
@@ -788,19 +788,55 @@ __global__ __launch_bounds__(plasma_gather_threads<STAGED>()) void k_gather_expl
     if (do_sx) store[cSx * ps + k] = aSx;
 }
 
+// The staging area of the flattened fast passes: the window lists' bounds and the records of
+// their entries, without the particle indices (their raster check reads those from global
+// memory). Entry t's record sits at slot t + t / 4: a 2 x 2 lattice puts four particles in a
+// list, so unskewed the lanes of a wavefront (neighbouring cells) would read entries four
+// apart and meet on a quarter of the LDS banks.
+constexpr int FLAT_SLOTS = 2864;               // record slots per field (157 KiB for 7 fields)
+constexpr int FLAT_CAP = FLAT_SLOTS * 4 / 5;   // staged entries: t + t / 4 < FLAT_SLOTS
+__device__ __forceinline__ int flat_slot(int t) { return t + (t >> 2); }
+struct FlatStage {
+    StagedLists<FLAT_CAP, false> l;
+    double rec[PLASMA_NREC][FLAT_SLOTS];
+};
+struct FlatView {
+    const StagedLists<FLAT_CAP, false>& s;
+    int li, lj;
+    __device__ __forceinline__ int bound(int kx, int ky) const { return s.off[lj + ky][li + kx]; }
+};
+
+// fast pass of the flattened gathers: whether every cell of the block can walk its lists in
+// raster order, the lists' bounds and indices read from global memory (every thread of the
+// workgroup calls this; a block that cannot is listed for the merge pass)
+template <int S, int DELTA>
+__device__ __forceinline__ bool block_raster_ordered_global(const Geom& g, const CLView& cl, bool in, int i, int j,
+                                                            int b, int32_t* __restrict__ ovf) {
+    bool bad = false;
+    if (in) {
+        const GlobalLists src{cl, j - S + 1 - DELTA - cl.bylo, i - S + 1 - DELTA - cl.bxlo};
+        bad = !raster_ordered<S>(WindowColours<S>(g, cl, i, j), src);
+    }
+    if (__syncthreads_or(bad)) {
+        if (threadIdx.x == 0) ovf[1 + atomicAdd(&ovf[0], 1)] = b;
+        return false;
+    }
+    return true;
+}
+
 // A cell's window lists as one sequence in the CPU build's order (colour by colour, the lists
 // of a colour in raster order), the lists of the mask `skip` left out. next() moves to the
 // next non-empty list: its entries [h, e) and stencil offset (ix, iy); false at the end. One
 // element per loop iteration keeps the lanes of a wavefront together however their list
 // lengths differ; the offset is then a run-time value.
-template <int S, int CAP>
+template <int S, class View>
 struct FlatWalk {
     // lists left of the current colour, and of the colours after it (shifted down as the
     // walk moves on a colour)
     unsigned long long rem, m1, m2, m3;
-    StagedView<CAP> src;
+    View src;
     int h = 0, e = 0, ix = 0, iy = 0;
-    __device__ __forceinline__ FlatWalk(const WindowColours<S>& w, const StagedView<CAP>& v, unsigned long long skip)
+    __device__ __forceinline__ FlatWalk(const WindowColours<S>& w, const View& v, unsigned long long skip)
         : rem(w.mask[0] & ~skip), m1(w.mask[1] & ~skip), m2(w.mask[2] & ~skip), m3(w.mask[3] & ~skip), src(v) {}
     __device__ __forceinline__ bool next() {
         while (true) {
@@ -864,7 +900,7 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_explicit_c2(
     int cEypBx, int cSy, int cSx, double charge_mass_ratio, int32_t* __restrict__ ovf) {
     constexpr int T = 2, O = 2, S = dshape_width<T, O>(), DELTA = dshape_delta<T, O>();
     constexpr unsigned long long CORNERS = 1ull | (1ull << (S - 1)) | (1ull << (S * (S - 1))) | (1ull << (S * S - 1));
-    __shared__ PlasmaStage<true> st;
+    __shared__ FlatStage st;
     __shared__ Quad quad[S];
     auto& sl = st.l;
     if (threadIdx.x < S) {
@@ -873,11 +909,11 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_explicit_c2(
     }
     auto copy = [&](int t, int pos) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) st.rec[f][t] = rec[f * n + pos];
+        for (int f = 0; f < 5; ++f) st.rec[f][flat_slot(t)] = rec[f * n + pos];
         const double xm = rec[5 * n + pos], ym = rec[6 * n + pos];
         const double xf = floor(xm + 0.5), yf = floor(ym + 0.5);
-        st.rec[5][t] = xm + 0.5 - xf;
-        st.rec[6][t] = ym + 0.5 - yf;
+        st.rec[5][flat_slot(t)] = xm + 0.5 - xf;
+        st.rec[6][flat_slot(t)] = ym + 0.5 - yf;
     };
     int b, i0, j0;
     if (!gather_setup<true, S, DELTA>(g, cl, sl, ovf, b, i0, j0, copy)) return;
@@ -885,7 +921,7 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_explicit_c2(
     const int li = cell % GS, lj = cell / GS;
     const int i = i0 + li, j = j0 + lj;
     const bool in = i <= g.nx - 1 + g.ng && j <= g.ny - 1 + g.ng;
-    if (!block_raster_ordered<S>(g, cl, sl, in && half == 0, i, j, li, lj, b, ovf)) return;
+    if (!block_raster_ordered_global<S, DELTA>(g, cl, in && half == 0, i, j, b, ovf)) return;
     if (!in) return;
     const long k = g.idx(i, j);
     const long ps = g.plane;
@@ -898,9 +934,10 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_explicit_c2(
     const double q_mass_ratio = charge_mass_ratio;
     const double AabssqDxp = 0.0, AabssqDyp = 0.0;
     const WindowColours<S> wc(g, cl, i, j);
-    const StagedView<GS_CAP_REC> src{sl, li, lj};
-    FlatWalk<S, GS_CAP_REC> w(wc, src, CORNERS);
-    auto load = [&](int at) {
+    const FlatView src{sl, li, lj};
+    FlatWalk<S, FlatView> w(wc, src, CORNERS);
+    auto load = [&](int t) {
+        const int at = flat_slot(t);
         return PlasmaRec{st.rec[0][at], st.rec[1][at], st.rec[2][at], st.rec[3][at],
                          st.rec[4][at], st.rec[5][at], st.rec[6][at]};
     };
@@ -963,14 +1000,14 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_current_o2(
     Geom g, CLView cl, const double* __restrict__ rec, long n, double* __restrict__ store, int cjx, int cjy, int cjz,
     int crho, int cchi, int crm, double charge_mu0_mass_ratio, int32_t* __restrict__ ovf) {
     constexpr int O = 2, S = O + 1, DELTA = shape_delta<O>();
-    __shared__ PlasmaStage<true> st;
+    __shared__ FlatStage st;
     auto& sl = st.l;
     auto copy = [&](int t, int pos) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) st.rec[f][t] = rec[f * n + pos];
+        for (int f = 0; f < 5; ++f) st.rec[f][flat_slot(t)] = rec[f * n + pos];
         const double xm = rec[5 * n + pos], ym = rec[6 * n + pos];
-        st.rec[5][t] = xm - floor(xm + 0.5);
-        st.rec[6][t] = ym - floor(ym + 0.5);
+        st.rec[5][flat_slot(t)] = xm - floor(xm + 0.5);
+        st.rec[6][flat_slot(t)] = ym - floor(ym + 0.5);
     };
     int b, i0, j0;
     if (!gather_setup<true, S, DELTA>(g, cl, sl, ovf, b, i0, j0, copy)) return;
@@ -978,7 +1015,7 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_current_o2(
     const int li = cell % GS, lj = cell / GS;
     const int i = i0 + li, j = j0 + lj;
     const bool in = i <= g.nx - 1 + g.ng && j <= g.ny - 1 + g.ng;
-    if (!block_raster_ordered<S>(g, cl, sl, in && half == 0, i, j, li, lj, b, ovf)) return;
+    if (!block_raster_ordered_global<S, DELTA>(g, cl, in && half == 0, i, j, b, ovf)) return;
     if (!in) return;
     if (half == 0) crho = cchi = crm = -1;
     else cjx = cjy = cjz = -1;
@@ -991,8 +1028,8 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_current_o2(
     const double clight = 1.0;
     const double q_mu0_mass_ratio = charge_mu0_mass_ratio;
     const WindowColours<S> wc(g, cl, i, j);
-    const StagedView<GS_CAP_REC> src{sl, li, lj};
-    FlatWalk<S, GS_CAP_REC> w(wc, src, 0ull);
+    const FlatView src{sl, li, lj};
+    FlatWalk<S, FlatView> w(wc, src, 0ull);
     double sgx = 0.0, sgy = 0.0;
     bool midx = false, midy = false;
     int hits = 0;
@@ -1004,7 +1041,7 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_current_o2(
             midx = w.ix == 1;
             midy = w.iy == 1;
         }
-        const int at = w.h++;
+        const int at = flat_slot(w.h++);
         const double q_invvol = st.rec[0][at], tx = st.rec[5][at], ty = st.rec[6][at];
         const double sx = shape2_rt(tx, sgx, midx);
         const double sy = shape2_rt(ty, sgy, midy);
