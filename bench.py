@@ -37,8 +37,8 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = "511x511x512 blowout-geometry slice loop, 2x2 ppc, shape 2, explicit MG"
 ERR_FIELDS = ("Ez", "ExmBy", "jx", "rhomjz")
 COPY_BYTES = 256 << 20
-# HBM bytes per launch of the dominant kernels, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs
-# scaled by the same counters on a 256 MiB device copy (see the summary beside it)
+# HBM bytes per launch of every kernel family and path, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by the same counters on a 256 MiB device copy (the commands are in it)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_511x511x512.json")
 DTYPE = {"fp64": "f64",
          "fp64_atomic": "f64",
@@ -179,14 +179,15 @@ def mixed_poisson(args, cfg):
     return "bf16x3" if max(cfg.nx, cfg.ny) <= 511 else "fft"
 
 
-def measured_traffic(kind, precision, args):
-    """HBM bytes per launch of kernel family `kind` on this workload, from the committed
-    counter summary, or None when it does not cover this kernel and workload."""
+def measured_traffic(kind, path, args):
+    """HBM bytes per launch of kernel family `kind` on path `path` (fp64: ordered deposition,
+    fp64_atomic, mixed) of this workload, from the committed counter summary, or None when it
+    does not cover this family, path and workload."""
     if (args.n, args.nz, args.ppc) != (511, 512, 2) or args.workload != "standard" or not os.path.exists(TRAFFIC_FILE):
         return None
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
-    e = t.get("kernels", {}).get(f"{kind}/{precision}")
+    e = t.get("families", {}).get(f"{kind}/{path}")
     return None if e is None else int(e["hbm_bytes_per_launch"])
 
 
