@@ -136,6 +136,81 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_add(int32_t* __restrict__ a, lo
         if (base + q < n) a[base + q] += o;
 }
 
+// The two-launch scan for up to SCAN_FLAT_MAX blocks: block totals, then each block scans its
+// elements from the sum of the totals before it (read by the whole block). A thread's
+// SCAN_ITEMS consecutive elements are read as four 16-byte loads when the array is aligned.
+constexpr long SCAN_FLAT_MAX = 4096;
+__device__ __forceinline__ void scan_load(const int32_t* __restrict__ a, long base, long n, bool vec,
+                                          int (&v)[SCAN_ITEMS]) {
+    if (vec && base + SCAN_ITEMS <= n) {
+        const int4* p = reinterpret_cast<const int4*>(a + base);
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS / 4; ++q) {
+            const int4 x = p[q];
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS; ++q) v[q] = (base + q < n) ? a[base + q] : 0;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const int32_t* __restrict__ a, long n, int vec,
+                                                       int32_t* __restrict__ sums) {
+    __shared__ int wsum[SCAN_T / 64];
+    const long base = long(blockIdx.x) * SCAN_BLOCK + long(threadIdx.x) * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    scan_load(a, base, n, vec != 0, v);
+    int tot = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) tot += v[q];
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < SCAN_T / 64; ++w) t += wsum[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_down(int32_t* __restrict__ a, long n, int vec,
+                                                     const int32_t* __restrict__ sums) {
+    __shared__ int wsum[SCAN_T / 64];
+    __shared__ int wpre[SCAN_T / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the sum of the totals of the blocks before this one
+    int pre = 0;
+    for (int k = threadIdx.x; k < int(blockIdx.x); k += SCAN_T) pre += sums[k];
+    for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o);
+    if (lane == 0) wpre[wv] = pre;
+    const long base = long(blockIdx.x) * SCAN_BLOCK + long(threadIdx.x) * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    scan_load(a, base, n, vec != 0, v);
+    int tot = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) tot += v[q];
+    int inc = tot;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < SCAN_T / 64; ++w) off += wpre[w];
+    for (int w = 0; w < wv; ++w) off += wsum[w];
+    int run = off + inc - tot;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        if (base + q < n) a[base + q] = run;
+        run += v[q];
+    }
+}
+
 // bin starts: offsets[b] = first slot of bin b (its chunk-0 entry), offsets[nbins] = n;
 // all zero when there are no particles (first == nullptr)
 __global__ void k_bin_offsets(const int32_t* __restrict__ first, int nbins, int nchunks, long n,
@@ -161,6 +236,12 @@ void scan_exclusive(hipStream_t s, int32_t* a, long n, int32_t* tmp) {
     const long nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
     if (nb <= 1) {
         hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(SCAN_T), 0, s, a, n, nullptr);
+        return;
+    }
+    if (nb <= SCAN_FLAT_MAX) {
+        const int vec = (reinterpret_cast<uintptr_t>(a) & 15) == 0 ? 1 : 0;
+        hipLaunchKernelGGL(k_scan_reduce, dim3(unsigned(nb)), dim3(SCAN_T), 0, s, a, n, vec, tmp);
+        hipLaunchKernelGGL(k_scan_down, dim3(unsigned(nb)), dim3(SCAN_T), 0, s, a, n, vec, tmp);
         return;
     }
     hipLaunchKernelGGL(k_scan_block, dim3(unsigned(nb)), dim3(SCAN_T), 0, s, a, n, tmp);

@@ -193,20 +193,24 @@ struct StagedView {
     __device__ __forceinline__ int pos(int h, int ky) const { return h + s.base[lj + ky]; }
 };
 
+// blocks of GS x BH cells: BH = GS for the merge passes and the per-list walks, FLAT_BH for the
+// flat-walk fast passes (two of their workgroups fit in a CU's LDS)
+constexpr int FLAT_BH = 8;
 __device__ __forceinline__ int gather_nbx(const Geom& g) { return (g.ldx + GS - 1) / GS; }
-__device__ __forceinline__ int gather_nby(const Geom& g) { return (g.ldy + GS - 1) / GS; }
+template <int BH>
+__device__ __forceinline__ int gather_nby(const Geom& g) { return (g.ldy + BH - 1) / BH; }
 
-// stages the rows of lists below block (i0, j0) for stencils of S cells starting DELTA from
-// the base cell, and calls copy(t, pos) for every staged entry t (record position pos); every
-// thread of the workgroup calls it
-template <int S, int DELTA, int CAP, bool IDX, class Copy>
+// stages the rows of lists below block (i0, j0) of GS x BH cells for stencils of S cells
+// starting DELTA from the base cell, and calls copy(t, pos) for every staged entry t (record
+// position pos); every thread of the workgroup calls it
+template <int S, int DELTA, int BH, int CAP, bool IDX, class Copy>
 __device__ __forceinline__ void stage_lists(const CLView& cl, int i0, int j0, StagedLists<CAP, IDX>& sl, Copy&& copy) {
-    constexpr int NW = GS + S - 1;
-    static_assert(NW <= GS_MAXW, "window wider than the staging area");
+    constexpr int NW = GS + S - 1, NH = BH + S - 1;
+    static_assert(NW <= GS_MAXW && NH <= GS_MAXW, "window wider than the staging area");
     const int col0 = i0 - S + 1 - DELTA - cl.bxlo;
     const int row0 = j0 - S + 1 - DELTA - cl.bylo;
     // list bounds; lists outside the list grid are empty
-    for (int e = threadIdx.x; e < NW * (NW + 1); e += blockDim.x) {
+    for (int e = threadIdx.x; e < NH * (NW + 1); e += blockDim.x) {
         const int r = e / (NW + 1), c = e - r * (NW + 1);
         const int row = row0 + r;
         sl.off[r][c] = (row >= 0 && row < cl.wy) ? cl.off[row * cl.wx + min(max(col0 + c, 0), cl.wx)] : 0;
@@ -214,7 +218,7 @@ __device__ __forceinline__ void stage_lists(const CLView& cl, int i0, int j0, St
     __syncthreads();
     if (threadIdx.x == 0) {
         int run = 0;
-        for (int r = 0; r < NW; ++r) {
+        for (int r = 0; r < NH; ++r) {
             const int a = sl.off[r][0];
             sl.base[r] = a - run;
             run += sl.off[r][NW] - a;
@@ -222,7 +226,7 @@ __device__ __forceinline__ void stage_lists(const CLView& cl, int i0, int j0, St
         sl.total = run;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < NW * (NW + 1); e += blockDim.x) {
+    for (int e = threadIdx.x; e < NH * (NW + 1); e += blockDim.x) {
         const int r = e / (NW + 1), c = e - r * (NW + 1);
         sl.off[r][c] -= sl.base[r];
     }
@@ -237,7 +241,7 @@ __device__ __forceinline__ void stage_lists(const CLView& cl, int i0, int j0, St
             for (int u = 0; u < U; ++u) {
                 const int t = t0 + u * int(blockDim.x);
                 // the row holding staged entry t: the last row starting at or before t
-                int lo = 0, hi = NW - 1;
+                int lo = 0, hi = NH - 1;
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
                     if (sl.off[mid][0] <= t) lo = mid;
@@ -342,19 +346,19 @@ __device__ __forceinline__ bool raster_ordered(const WindowColours<S>& wc, const
 // list ovf (ovf[0] blocks in ovf[1..]) and left to the second pass. MERGE: the blockIdx.x-th
 // listed block, each cell merging its lists (from LDS when they fit, from global memory when
 // they do not). The fast pass keeps few registers; the merge's state stays in the second.
-template <bool FAST, int S, int DELTA, int CAP, bool IDX, class Copy>
+template <bool FAST, int S, int DELTA, int BH, int CAP, bool IDX, class Copy>
 __device__ __forceinline__ bool gather_setup(const Geom& g, const CLView& cl, StagedLists<CAP, IDX>& sl,
                                              int32_t* __restrict__ ovf, int& b, int& i0, int& j0, Copy&& copy) {
     const int nbx = gather_nbx(g);
     if constexpr (FAST) {
-        b = xcd_block(int(blockIdx.x), nbx * gather_nby(g));
+        b = xcd_block(int(blockIdx.x), nbx * gather_nby<BH>(g));
     } else {
         if (int(blockIdx.x) >= ovf[0]) return false;
         b = ovf[1 + blockIdx.x];
     }
     i0 = (b % nbx) * GS - g.ng;
-    j0 = (b / nbx) * GS - g.ng;
-    stage_lists<S, DELTA>(cl, i0, j0, sl, copy);
+    j0 = (b / nbx) * BH - g.ng;
+    stage_lists<S, DELTA, BH>(cl, i0, j0, sl, copy);
     if (FAST && sl.total > CAP) {
         if (threadIdx.x == 0) ovf[1 + atomicAdd(&ovf[0], 1)] = b;
         return false;
@@ -659,14 +663,14 @@ struct PlasmaRecs {
 // threads of a plasma gather workgroup: the fast pass runs two threads per cell, each with
 // its own share of the cell's sums (the cell's walk is the same for both), so each SIMD holds
 // two wavefronts with the workgroup's LDS; the merge pass one thread per cell
-template <bool FAST>
-constexpr int plasma_gather_threads() { return FAST ? 2 * GS * GS : GS * GS; }
+template <bool FAST, int BH = GS>
+constexpr int plasma_gather_threads() { return FAST ? 2 * GS * BH : GS * BH; }
 
 // ::DepositCurrent's per-cell sums, PlasmaDepositCurrent.cpp:199-283: comps jx, jy, jz, rho,
 // chi, rhomjz (-1: skip; jx and jy go together). Fast pass: the first thread of a cell sums
 // jx, jy and jz, the second rho, chi and rhomjz.
-template <int O, bool STAGED>
-__global__ __launch_bounds__(plasma_gather_threads<STAGED>()) void k_gather_current(
+template <int O, bool STAGED, int BH = GS>
+__global__ __launch_bounds__((plasma_gather_threads<STAGED, BH>())) void k_gather_current(
     Geom g, CLView cl, const double* __restrict__ rec, long n, double* __restrict__ store, int cjx, int cjy, int cjz,
     int crho, int cchi, int crm, double charge_mu0_mass_ratio, int32_t* __restrict__ ovf) {
     constexpr int S = O + 1, DELTA = shape_delta<O>();
@@ -674,8 +678,8 @@ __global__ __launch_bounds__(plasma_gather_threads<STAGED>()) void k_gather_curr
     auto& sl = st.l;
     const PlasmaRecs<STAGED> pr{st, rec, n};
     int b, i0, j0;
-    if (!gather_setup<STAGED, S, DELTA>(g, cl, sl, ovf, b, i0, j0, pr)) return;
-    const int cell = int(threadIdx.x) % (GS * GS), half = int(threadIdx.x) / (GS * GS);
+    if (!gather_setup<STAGED, S, DELTA, BH>(g, cl, sl, ovf, b, i0, j0, pr)) return;
+    const int cell = int(threadIdx.x) % (GS * BH), half = int(threadIdx.x) / (GS * BH);
     const int li = cell % GS, lj = cell / GS;
     const int i = i0 + li, j = j0 + lj;
     const bool in = i <= g.nx - 1 + g.ng && j <= g.ny - 1 + g.ng;
@@ -722,8 +726,8 @@ __global__ __launch_bounds__(plasma_gather_threads<STAGED>()) void k_gather_curr
 
 // ::ExplicitDeposition's per-cell sums, ExplicitDeposition.cpp:186-262 (no laser), with the
 // cell's own Bz, Ez, ExmBy, EypBx. Fast pass: the first thread of a cell sums Sy, the second Sx.
-template <int T, int O, bool STAGED>
-__global__ __launch_bounds__(plasma_gather_threads<STAGED>()) void k_gather_explicit(
+template <int T, int O, bool STAGED, int BH = GS>
+__global__ __launch_bounds__((plasma_gather_threads<STAGED, BH>())) void k_gather_explicit(
     Geom g, CLView cl, const double* __restrict__ rec, long n, double* __restrict__ store, int cBz, int cEz, int cExmBy,
     int cEypBx, int cSy, int cSx, double charge_mass_ratio, int32_t* __restrict__ ovf) {
     constexpr int S = dshape_width<T, O>(), DELTA = dshape_delta<T, O>();
@@ -731,8 +735,8 @@ __global__ __launch_bounds__(plasma_gather_threads<STAGED>()) void k_gather_expl
     auto& sl = st.l;
     const PlasmaRecs<STAGED> pr{st, rec, n};
     int b, i0, j0;
-    if (!gather_setup<STAGED, S, DELTA>(g, cl, sl, ovf, b, i0, j0, pr)) return;
-    const int cell = int(threadIdx.x) % (GS * GS), half = int(threadIdx.x) / (GS * GS);
+    if (!gather_setup<STAGED, S, DELTA, BH>(g, cl, sl, ovf, b, i0, j0, pr)) return;
+    const int cell = int(threadIdx.x) % (GS * BH), half = int(threadIdx.x) / (GS * BH);
     const int li = cell % GS, lj = cell / GS;
     const int i = i0 + li, j = j0 + lj;
     const bool in = i <= g.nx - 1 + g.ng && j <= g.ny - 1 + g.ng;
@@ -788,14 +792,14 @@ __global__ __launch_bounds__(plasma_gather_threads<STAGED>()) void k_gather_expl
     if (do_sx) store[cSx * ps + k] = aSx;
 }
 
-// The staging area of the flattened fast passes: the window lists' bounds and the records of
+// The staging area of the flat-walk fast passes: the window lists' bounds and the records of
 // their entries, without the particle indices (their raster check reads those from global
-// memory). Entry t's record sits at slot t + t / 4: a 2 x 2 lattice puts four particles in a
-// list, so unskewed the lanes of a wavefront (neighbouring cells) would read entries four
-// apart and meet on a quarter of the LDS banks.
-constexpr int FLAT_SLOTS = 2864;               // record slots per field (157 KiB for 7 fields)
-constexpr int FLAT_CAP = FLAT_SLOTS * 4 / 5;   // staged entries: t + t / 4 < FLAT_SLOTS
-__device__ __forceinline__ int flat_slot(int t) { return t + (t >> 2); }
+// memory). Sized so that two workgroups (blocks of GS x FLAT_BH cells) fit in a CU's LDS: one
+// stages its window while the other walks. A 2 x 2 lattice fills 960 of the 1400 entries of a
+// centred order-2 window (20 x 12 lists).
+constexpr int FLAT_CAP = 1400;                  // staged entries (77 KiB of records)
+constexpr int FLAT_SLOTS = FLAT_CAP;
+__device__ __forceinline__ int flat_slot(int t) { return t; }
 struct FlatStage {
     StagedLists<FLAT_CAP, false> l;
     double rec[PLASMA_NREC][FLAT_SLOTS];
@@ -895,7 +899,7 @@ struct Quad {
 // time and the shape factors from the rows of CENTRED2, so the lanes of a wavefront stay
 // together however their list lengths differ (the two wavefronts per SIMD cover each other's
 // LDS reads). The staged records hold t of x and y in place of the cell-scaled positions.
-__global__ __launch_bounds__(2 * GS * GS) void k_gather_explicit_c2(
+__global__ __launch_bounds__(2 * GS * FLAT_BH) void k_gather_explicit_c2(
     Geom g, CLView cl, const double* __restrict__ rec, long n, double* __restrict__ store, int cBz, int cEz, int cExmBy,
     int cEypBx, int cSy, int cSx, double charge_mass_ratio, int32_t* __restrict__ ovf) {
     constexpr int T = 2, O = 2, S = dshape_width<T, O>(), DELTA = dshape_delta<T, O>();
@@ -916,8 +920,8 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_explicit_c2(
         st.rec[6][flat_slot(t)] = ym + 0.5 - yf;
     };
     int b, i0, j0;
-    if (!gather_setup<true, S, DELTA>(g, cl, sl, ovf, b, i0, j0, copy)) return;
-    const int cell = int(threadIdx.x) % (GS * GS), half = int(threadIdx.x) / (GS * GS);
+    if (!gather_setup<true, S, DELTA, FLAT_BH>(g, cl, sl, ovf, b, i0, j0, copy)) return;
+    const int cell = int(threadIdx.x) % (GS * FLAT_BH), half = int(threadIdx.x) / (GS * FLAT_BH);
     const int li = cell % GS, lj = cell / GS;
     const int i = i0 + li, j = j0 + lj;
     const bool in = i <= g.nx - 1 + g.ng && j <= g.ny - 1 + g.ng;
@@ -996,7 +1000,7 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_explicit_c2(
 // k_gather_current<2, true>, each cell walking its window as one sequence with the offset at
 // run time (shape2_rt). The staged records hold t of x and y in place of the cell-scaled
 // positions. The first thread of a cell sums jx, jy and jz, the second rho, chi and rhomjz.
-__global__ __launch_bounds__(2 * GS * GS) void k_gather_current_o2(
+__global__ __launch_bounds__(2 * GS * FLAT_BH) void k_gather_current_o2(
     Geom g, CLView cl, const double* __restrict__ rec, long n, double* __restrict__ store, int cjx, int cjy, int cjz,
     int crho, int cchi, int crm, double charge_mu0_mass_ratio, int32_t* __restrict__ ovf) {
     constexpr int O = 2, S = O + 1, DELTA = shape_delta<O>();
@@ -1010,8 +1014,8 @@ __global__ __launch_bounds__(2 * GS * GS) void k_gather_current_o2(
         st.rec[6][flat_slot(t)] = ym - floor(ym + 0.5);
     };
     int b, i0, j0;
-    if (!gather_setup<true, S, DELTA>(g, cl, sl, ovf, b, i0, j0, copy)) return;
-    const int cell = int(threadIdx.x) % (GS * GS), half = int(threadIdx.x) / (GS * GS);
+    if (!gather_setup<true, S, DELTA, FLAT_BH>(g, cl, sl, ovf, b, i0, j0, copy)) return;
+    const int cell = int(threadIdx.x) % (GS * FLAT_BH), half = int(threadIdx.x) / (GS * FLAT_BH);
     const int li = cell % GS, lj = cell / GS;
     const int i = i0 + li, j = j0 + lj;
     const bool in = i <= g.nx - 1 + g.ng && j <= g.ny - 1 + g.ng;
@@ -1258,13 +1262,16 @@ CLView view(const CellLists& L) {
 }
 
 // gather blocks over the guarded plane
-long gather_blocks(const Geom& g) { return long((g.ldx + GS - 1) / GS) * ((g.ldy + GS - 1) / GS); }
+long gather_blocks(const Geom& g, int bh) { return long((g.ldx + GS - 1) / GS) * ((g.ldy + bh - 1) / bh); }
+
+// stride of the two overflow lists (the blocks of height FLAT_BH are the most numerous)
+long ovf_stride(const Geom& g) { return gather_blocks(g, FLAT_BH) + 1; }
 
 void lists_reserve(const Geom& g, CellLists& L, long n, int nrec) {
     const long ns = long(L.wx) * L.wy;
     // overflow lists of the gathers, [count, blocks...] each: the current (or beam)
     // deposition's at ovf, the explicit deposition's at ovf + gather_blocks + 1
-    grow(L.ovf, L.ovf_cap, 2 * (gather_blocks(g) + 1));
+    grow(L.ovf, L.ovf_cap, 2 * ovf_stride(g));
     grow(L.off, L.off_cap, ns + 1);
     grow(L.tmp, L.tmp_cap, scan_tmp_size(ns + 1));
     grow(L.key, L.key_cap, n);
@@ -1286,10 +1293,10 @@ void lists_scatter(hipStream_t s, CellLists& L, long n) {
 void lists_clear(const Geom& g, hipStream_t s, CellLists& L) {
     const long ns = long(L.wx) * L.wy;
     hipLaunchKernelGGL(k_zero_int, grid1d(ns + 1, 256), dim3(256), 0, s, L.off, ns + 1);
-    hipLaunchKernelGGL(k_zero_ovf, dim3(1), dim3(64), 0, s, L.ovf, gather_blocks(g) + 1);
+    hipLaunchKernelGGL(k_zero_ovf, dim3(1), dim3(64), 0, s, L.ovf, ovf_stride(g));
 }
 
-dim3 gather_grid(const Geom& g) { return dim3(unsigned(gather_blocks(g))); }
+dim3 gather_grid(const Geom& g, int bh) { return dim3(unsigned(gather_blocks(g, bh))); }
 
 int shape_base_of(int order) {
     switch (order) {
@@ -1334,16 +1341,22 @@ void deposit_current_ordered(hpq_ctx* c, const hpq_plasma* p, double* store, con
         lists_scatter(c->stream, L, p->n);
         hipLaunchKernelGGL(k_lists_order_plasma, grid1d(p->n, 256), dim3(256), 0, c->stream, g, *p, charge_invvol,
                            L.key, L.off, L.unsorted, L.perm, L.rec, p->n);
-        // the fast pass, then the blocks it listed (merged)
-        if constexpr (o == 2)
-            hipLaunchKernelGGL(k_gather_current_o2, gather_grid(g), dim3(2 * GS * GS), 0, c->stream, g, view(L), L.rec,
-                               p->n, store, comps[0], comps[1], comps[2], comps[3], comps[4], comps[5], cmr, L.ovf);
-        else
-            hipLaunchKernelGGL((k_gather_current<o, true>), gather_grid(g), dim3(plasma_gather_threads<true>()), 0,
+        // the fast pass, then the blocks it listed (merged, in the fast pass's block shape)
+        if constexpr (o == 2) {
+            hipLaunchKernelGGL(k_gather_current_o2, gather_grid(g, FLAT_BH), dim3(2 * GS * FLAT_BH), 0, c->stream, g,
+                               view(L), L.rec, p->n, store, comps[0], comps[1], comps[2], comps[3], comps[4], comps[5],
+                               cmr, L.ovf);
+            hipLaunchKernelGGL((k_gather_current<o, false, FLAT_BH>), gather_grid(g, FLAT_BH), dim3(GS * FLAT_BH), 0,
                                c->stream, g, view(L), L.rec, p->n, store, comps[0], comps[1], comps[2], comps[3],
                                comps[4], comps[5], cmr, L.ovf);
-        hipLaunchKernelGGL((k_gather_current<o, false>), gather_grid(g), dim3(GS * GS), 0, c->stream, g, view(L),
-                           L.rec, p->n, store, comps[0], comps[1], comps[2], comps[3], comps[4], comps[5], cmr, L.ovf);
+        } else {
+            hipLaunchKernelGGL((k_gather_current<o, true>), gather_grid(g, GS), dim3(plasma_gather_threads<true>()), 0,
+                               c->stream, g, view(L), L.rec, p->n, store, comps[0], comps[1], comps[2], comps[3],
+                               comps[4], comps[5], cmr, L.ovf);
+            hipLaunchKernelGGL((k_gather_current<o, false>), gather_grid(g, GS), dim3(GS * GS), 0, c->stream, g,
+                               view(L), L.rec, p->n, store, comps[0], comps[1], comps[2], comps[3], comps[4], comps[5],
+                               cmr, L.ovf);
+        }
     });
     HPQ_HIP(hipGetLastError());
     L.valid = !ions;
@@ -1388,17 +1401,21 @@ void explicit_deposition_ordered(hpq_ctx* c, const hpq_plasma* p, double* store)
             }
             // the explicit deposition's overflow list (its count cleared here: the lists may be
             // reused by several explicit depositions)
-            int32_t* ovf = L.ovf + gather_blocks(g) + 1;
+            int32_t* ovf = L.ovf + ovf_stride(g);
             hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, ovf, 1L);
-            if constexpr (t == 2 && o == 2)
-                hipLaunchKernelGGL(k_gather_explicit_c2, gather_grid(g), dim3(2 * GS * GS), 0, c->stream, g, view(L),
-                                   L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy, Ly.Sx, qmr, ovf);
-            else
-                hipLaunchKernelGGL((k_gather_explicit<t, o, true>), gather_grid(g), dim3(plasma_gather_threads<true>()), 0,
-                                   c->stream, g, view(L), L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy,
+            if constexpr (t == 2 && o == 2) {
+                hipLaunchKernelGGL(k_gather_explicit_c2, gather_grid(g, FLAT_BH), dim3(2 * GS * FLAT_BH), 0, c->stream, g,
+                                   view(L), L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy, Ly.Sx, qmr, ovf);
+                hipLaunchKernelGGL((k_gather_explicit<t, o, false, FLAT_BH>), gather_grid(g, FLAT_BH),
+                                   dim3(GS * FLAT_BH), 0, c->stream, g, view(L), L.rec, p->n, store, Ly.Bz, Ly.Ez,
+                                   Ly.ExmBy, Ly.EypBx, Ly.Sy, Ly.Sx, qmr, ovf);
+            } else {
+                hipLaunchKernelGGL((k_gather_explicit<t, o, true>), gather_grid(g, GS), dim3(plasma_gather_threads<true>()),
+                                   0, c->stream, g, view(L), L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy,
                                    Ly.Sx, qmr, ovf);
-            hipLaunchKernelGGL((k_gather_explicit<t, o, false>), gather_grid(g), dim3(GS * GS), 0, c->stream, g,
-                               view(L), L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy, Ly.Sx, qmr, ovf);
+                hipLaunchKernelGGL((k_gather_explicit<t, o, false>), gather_grid(g, GS), dim3(GS * GS), 0, c->stream, g,
+                                   view(L), L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy, Ly.Sx, qmr, ovf);
+            }
         };
         if (T == 0) run(std::integral_constant<int, 0>{});
         else if (T == 1) run(std::integral_constant<int, 1>{});
