@@ -547,11 +547,16 @@ __global__ __launch_bounds__(256) void k_zero_int(int32_t* __restrict__ a, long 
     if (t < n) a[t] = 0;
 }
 
-// the two overflow counts (ovf[0], ovf[nb]) of a fresh set of lists
-__global__ void k_zero_ovf(int32_t* __restrict__ ovf, long nb) {
-    if (threadIdx.x == 0) {
+// a fresh set of lists: the per-start-cell counts off[0, n), the two overflow counts
+// (ovf[0], ovf[stride]) and, for the beam, the count of long sequences (longc[0])
+__global__ __launch_bounds__(256) void k_lists_clear(int32_t* __restrict__ off, long n, int32_t* __restrict__ ovf,
+                                                     long stride, int32_t* __restrict__ longc) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) off[t] = 0;
+    if (t == 0) {
         ovf[0] = 0;
-        ovf[nb] = 0;
+        ovf[stride] = 0;
+        if (longc) longc[0] = 0;
     }
 }
 
@@ -1290,10 +1295,11 @@ void lists_scatter(hipStream_t s, CellLists& L, long n) {
     HPQ_HIP(hipGetLastError());
 }
 
-void lists_clear(const Geom& g, hipStream_t s, CellLists& L) {
+void lists_clear(const Geom& g, hipStream_t s, CellLists& L, int32_t* longc = nullptr) {
     const long ns = long(L.wx) * L.wy;
-    hipLaunchKernelGGL(k_zero_int, grid1d(ns + 1, 256), dim3(256), 0, s, L.off, ns + 1);
-    hipLaunchKernelGGL(k_zero_ovf, dim3(1), dim3(64), 0, s, L.ovf, ovf_stride(g));
+    hipLaunchKernelGGL(k_lists_clear, grid1d(ns + 1, 256), dim3(256), 0, s, L.off, ns + 1, L.ovf, ovf_stride(g),
+                       longc);
+    L.explicit_ovf_clear = true;
 }
 
 dim3 gather_grid(const Geom& g, int bh) { return dim3(unsigned(gather_blocks(g, bh))); }
@@ -1402,7 +1408,8 @@ void explicit_deposition_ordered(hpq_ctx* c, const hpq_plasma* p, double* store)
             // the explicit deposition's overflow list (its count cleared here: the lists may be
             // reused by several explicit depositions)
             int32_t* ovf = L.ovf + ovf_stride(g);
-            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, ovf, 1L);
+            if (!L.explicit_ovf_clear) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, ovf, 1L);
+            L.explicit_ovf_clear = false;
             if constexpr (t == 2 && o == 2) {
                 hipLaunchKernelGGL(k_gather_explicit_c2, gather_grid(g, FLAT_BH), dim3(2 * GS * FLAT_BH), 0, c->stream, g,
                                    view(L), L.rec, p->n, store, Ly.Bz, Ly.Ez, Ly.ExmBy, Ly.EypBx, Ly.Sy, Ly.Sx, qmr, ovf);
@@ -1443,8 +1450,7 @@ void deposit_beam_ordered(hpq_ctx* c, const hpq_beam* b, int slice, double* stor
     const long m = long(S) * S * count;
     grow(L.contrib, L.contrib_cap, 3 * m);
     TraceScope ts(c, HPQ_TRACE_BEAM_DEPOSIT);
-    lists_clear(g, c->stream, L);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, L.longc, 1L);
+    lists_clear(g, c->stream, L, L.longc);
     dispatch_order(c->p.depos_order, [&](auto O) {
         constexpr int o = decltype(O)::value;
         hipLaunchKernelGGL(k_keys_beam<o>, grid1d(count, 256), dim3(256), 0, c->stream, g, *b, first, count, view(L),

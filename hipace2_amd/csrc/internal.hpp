@@ -173,6 +173,7 @@ struct CellLists {
     long rec_cap = 0;
     int32_t* ovf = nullptr;      // overflow lists of the staged gathers
     long ovf_cap = 0;
+    bool explicit_ovf_clear = false;   // the explicit deposition's overflow count is zero
     // the beam's per-cell contribution sequences: segment offsets per plane cell, their scan
     // scratch, the contributions (3 arrays), the cells summed by a wavefront each
     int32_t *seg = nullptr, *segtmp = nullptr, *longc = nullptr;
