@@ -119,6 +119,9 @@ def run_gpu(args, path, capture):
     precision = "mixed" if path == "mixed" else "fp64"
     cfg.precision = precision
     cfg.deposition = "atomic" if path in ("mixed", "fp64_atomic") else "ordered"
+    # the atomic paths reorder the plasma in place by deposition tile (plasmas.reorder_period);
+    # the ordered path keeps the index order whose sums it reproduces
+    cfg.reorder_period = args.reorder_period if cfg.deposition == "atomic" else 0
     if precision == "mixed":
         cfg.poisson_solver = mixed_poisson(args, cfg)
     sim = Simulation(cfg, beam)
@@ -518,6 +521,8 @@ def main():
                          "mixed; the first is the headline, the others report their error against it")
     ap.add_argument("--mixed-poisson", choices=["auto", "bf16x3", "fft"], default="auto",
                     help="transverse Poisson solver of the mixed path (auto: bf16x3 up to n = 511, fft beyond)")
+    ap.add_argument("--reorder-period", type=int, default=4,
+                    help="plasmas.reorder_period of the atomic paths (fp64_atomic, mixed); 0: never")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the beam-loading gradient measurement")
     ap.add_argument("--adjoint-mg-tol", type=float, default=1e-8,
                     help="relative tolerance of the adjoint multigrid solves (warm-started)")
@@ -582,7 +587,8 @@ def main():
         return {"value": round(v, 2), "unit": "slices/s", "ms_per_step": round(1e3 * f["elapsed"] / args.steps, 2),
                 "speedup_vs_fp64": round(v / slices_per_s, 3),
                 "plasma_pushes_per_s": round(f["stats"]["pushes"] * args.steps / f["elapsed"], 1),
-                "mg_vcycles_per_slice": round(f["stats"]["mg_iters"] / cfg.nz, 3), "dtype": DTYPE[q]}
+                "mg_vcycles_per_slice": round(f["stats"]["mg_iters"] / cfg.nz, 3), "dtype": DTYPE[q],
+                "reorder_period": f["cfg"].reorder_period}
 
     extra = {}
     for q in paths[1:]:

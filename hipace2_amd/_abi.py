@@ -49,7 +49,8 @@ class Params(C.Structure):
                 ("pc_tol", C.c_double), ("pc_maxiter", C.c_int), ("pc_mix", C.c_double),
                 ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double),
                 ("bin_period", C.c_int), ("precision", C.c_int), ("poisson_solver", C.c_int),
-                ("salame", C.c_int), ("deposition", C.c_int), ("tile_size", C.c_int)]
+                ("salame", C.c_int), ("deposition", C.c_int), ("tile_size", C.c_int),
+                ("reorder_period", C.c_int)]
 
 PRECISION = {"fp64": 0, "mixed": 1}
 DEPOSITION = {"ordered": 0, "atomic": 1}

@@ -257,7 +257,7 @@ long scan_tmp_size(long n) {
 
 void bins_free(hpq_ctx* c) {
     Bins& B = c->bins;
-    for (void* q : {(void*)B.keys, (void*)B.perm, (void*)B.counts, (void*)B.tmp})
+    for (void* q : {(void*)B.keys, (void*)B.perm, (void*)B.counts, (void*)B.tmp, (void*)B.reorder})
         if (q) (void)hipFree(q);
     B = Bins{};
 }
@@ -445,6 +445,35 @@ __global__ void k_gather(const T* __restrict__ src, T* __restrict__ dst, const i
     if (t < n) dst[t] = src[perm[t]];
 }
 
+// the plasma's eleven FP64 arrays and valid flags gathered into staging in the order perm,
+// then copied back (PlasmaIdx order of hpq_plasma)
+struct PlasmaArrays {
+    double* r[11];
+    int32_t* valid;
+};
+__device__ __forceinline__ PlasmaArrays plasma_arrays(const hpq_plasma& p) {
+    return {{p.x, p.y, p.w, p.ux, p.uy, p.psi, p.x_prev, p.y_prev, p.ux_half_step, p.uy_half_step, p.psi_half_step},
+            p.valid};
+}
+__global__ __launch_bounds__(256) void k_reorder_gather(hpq_plasma p, const int32_t* __restrict__ perm,
+                                                        double* __restrict__ st) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= p.n) return;
+    const PlasmaArrays a = plasma_arrays(p);
+    const long s = perm[t];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) st[k * p.n + t] = a.r[k][s];
+    reinterpret_cast<int32_t*>(st + 11 * p.n)[t] = a.valid[s];
+}
+__global__ __launch_bounds__(256) void k_reorder_back(hpq_plasma p, const double* __restrict__ st) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= p.n) return;
+    const PlasmaArrays a = plasma_arrays(p);
+#pragma unroll
+    for (int k = 0; k < 11; ++k) a.r[k][t] = st[k * p.n + t];
+    a.valid[t] = reinterpret_cast<const int32_t*>(st + 11 * p.n)[t];
+}
+
 // particles sorted past the last slice are outside the domain: invalid, zero weight
 __global__ void k_invalidate_tail(hpq_beam_state b, const int32_t* __restrict__ offsets, int nz) {
     const long t = long(offsets[nz]) + long(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -492,11 +521,11 @@ void sort_beam_slices(hpq_ctx* c, hpq_beam_state* b, int64_t* off_out) {
     for (int s = 0; s <= g.nz; ++s) off_out[s] = offs[s];
 }
 
-// the permutation the deposition kernels walk: 16x16-cell tiles, refreshed every
-// bin_period slices by the slice driver
+// the permutation the deposition kernels walk: 32x8-cell tiles, refreshed every
+// bin_period slices by the slice driver; none when the plasma is reordered in place
 const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p) {
     Bins& B = c->bins;
-    if (c->p.bin_period <= 0 || p->n <= 0) return nullptr;
+    if (c->p.bin_period <= 0 || p->n <= 0 || plasma_reordered(c)) return nullptr;
     if (B.perm_n != p->n || B.stale) {
         grow(B.perm, B.perm_cap, p->n);
         bin_tiles_rect(c, p, DEPOSIT_TILE_X, DEPOSIT_TILE_Y, std::max(1, c->inj_ppc), B.perm, nullptr);
@@ -504,6 +533,29 @@ const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p) {
         B.stale = false;
     }
     return B.perm;
+}
+
+// PlasmaParticleContainer::ReorderParticles (src/particles/plasma/PlasmaParticleContainer.cpp:
+// 197-208; SortParticlesForDeposition on GPU builds): the plasma arrays in place in the order
+// of the deposition tiles (the tile sort of deposit_order), so the atomic depositions read
+// consecutive particles from consecutive addresses
+void reorder_plasma(hpq_ctx* c, const hpq_plasma* p) {
+    const long n = p->n;
+    if (n <= 0) return;
+    Bins& B = c->bins;
+    grow(B.perm, B.perm_cap, n);
+    // from injection order the first reorder of a step splits each tile by a particle's index
+    // within its cell (one particle per cell in every run of a tile, so the lanes of a
+    // wavefront add to different LDS words); later ones keep the order inside a tile (stable)
+    bin_tiles_rect(c, p, DEPOSIT_TILE_X, DEPOSIT_TILE_Y, B.reordered ? 1 : std::max(1, c->inj_ppc), B.perm, nullptr);
+    B.perm_n = -1;   // the permutation describes the old order
+    B.stale = true;
+    B.reordered = true;
+    // 11 FP64 arrays and the int32 flags (half a double each, rounded up)
+    grow(B.reorder, B.reorder_cap, 11 * n + (n + 1) / 2);
+    hipLaunchKernelGGL(k_reorder_gather, grid1d(n, 256), dim3(256), 0, c->stream, *p, B.perm, B.reorder);
+    hipLaunchKernelGGL(k_reorder_back, grid1d(n, 256), dim3(256), 0, c->stream, *p, B.reorder);
+    HPQ_HIP(hipGetLastError());
 }
 
 }  // namespace hpq

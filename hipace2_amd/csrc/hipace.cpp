@@ -238,6 +238,8 @@ public:
         const hpq_params& P = c_->p;
         two = two && P.explicit_solver;
         insitu_plasma(c_, p_, islice);
+        // MultiPlasma::ReorderParticles (Hipace.cpp:594-595)
+        if (plasma_reordered(c_) && islice % P.reorder_period == 0) reorder_plasma(c_, p_);
         if (!initialized) fields_.InitializeSlices();
         auto on_side = [&](auto&& f) {
             hipStream_t main = c_->stream;
@@ -611,6 +613,8 @@ void adjoint_check(const hpq_ctx* c) {
         throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: odd nx, ny (node-centred multigrid, a symmetric operator)");
     if (P.do_beam_jx_jy) throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: do_beam_jx_jy must be 0");
     if (c->salame.mask) throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: SALAME must be off");
+    if (plasma_reordered(c))
+        throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: plasma reordering (reorder_period) must be off");
 }
 
 // the forward pass with checkpoints every k slices
@@ -740,6 +744,7 @@ int hpq_create(const hpq_params* p, hpq_ctx** out) {
             throw HpqError(HPQ_E_UNSUPPORTED, "Cannot use SALAME algorithm with predictor-corrector solver");
         if (p->deposition != HPQ_DEPOSIT_ORDERED && p->deposition != HPQ_DEPOSIT_ATOMIC)
             throw HpqError(HPQ_E_INVALID, "deposition must be HPQ_DEPOSIT_ORDERED or HPQ_DEPOSIT_ATOMIC");
+        if (p->reorder_period < 0) throw HpqError(HPQ_E_INVALID, "reorder_period must be >= 0");
         {
             // the widest stencil the ordered deposition walks: O + 1 cells for the current and
             // beam deposition, the derivative stencil for the explicit one
@@ -1109,6 +1114,7 @@ int hpq_begin_step(hpq_ctx* c, const hpq_plasma* p, double* store) {
         HPQ_HIP(hipMemsetAsync(c->d_count, 0, sizeof(int) * 4, c->stream));
         for (double& s : c->stats) s = 0;
         c->bins.stale = true;
+        c->bins.reordered = false;   // a step starts from the plasma in injection order
         if (c->L.ions_rhomjz >= 0) DepositCurrent(c, p, store, -1, -1, -1, -1, -1, c->L.ions_rhomjz, true);
     });
 }

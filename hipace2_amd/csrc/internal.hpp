@@ -158,6 +158,9 @@ struct Bins {
     long perm_n = -1;
     bool stale = true;           // re-sort before the next deposition
     int calls = 0;               // plasma depositions since the last re-sort
+    double* reorder = nullptr;   // staging of the plasma arrays while they are reordered
+    long reorder_cap = 0;
+    bool reordered = false;      // the plasma was reordered since the step began
 };
 
 // Per-start-cell particle lists of the ordered deposition (gather.hip): list offsets over a
@@ -335,6 +338,15 @@ int64_t plasma_count(hpq_ctx* c, const hpq_plasma_profile* prof);
 void bin_tiles(hpq_ctx* c, const hpq_plasma* p, int tile, int32_t* perm, int32_t* offsets);
 void bin_tiles_rect(hpq_ctx* c, const hpq_plasma* p, int tile, int tile_y, int split, int32_t* perm,
                     int32_t* offsets);
+// plasmas.reorder_period: the plasma arrays reordered in place by deposition tile
+void reorder_plasma(hpq_ctx* c, const hpq_plasma* p);
+// the depositions sum in the reference CPU build's order (FP64 path, ordered deposition)
+inline bool ordered_deposition(const hpq_ctx* c) {
+    return c->p.deposition == HPQ_DEPOSIT_ORDERED && c->p.precision == HPQ_PREC_FP64;
+}
+// the atomic depositions walk the plasma in index order (reordered in place every
+// reorder_period slices) instead of through the tile permutation
+inline bool plasma_reordered(const hpq_ctx* c) { return c->p.reorder_period > 0 && !ordered_deposition(c); }
 const int32_t* deposit_order(hpq_ctx* c, const hpq_plasma* p);
 void bins_free(hpq_ctx* c);
 void sort_keys(hpq_ctx* c, long n, int nbins, int32_t* perm, int32_t* offsets);

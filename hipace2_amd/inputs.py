@@ -77,6 +77,9 @@ class SliceConfig:
     # atomic deposition: walks the particles sorted by 32x8-cell tiles, re-sorted every
     # bin_period slices (0: index order); only the order of floating-point sums depends on it
     bin_period: int = 4
+    # plasmas.reorder_period (PlasmaParticleContainer.cpp:197-208): reorder the plasma arrays
+    # in place by deposition tile every reorder_period slices (atomic deposition only; 0: never)
+    reorder_period: int = 0
     # "fp64": the drop-in path; "mixed": FP32 gather, push increments and deposition with
     # FP32 contributions summed in FP64 LDS words (FP64 store, particle state and multigrid)
     precision: str = "fp64"
@@ -570,6 +573,7 @@ def parse_deck(text: str, overrides: Optional[Dict[str, str]] = None):
             raise ValueError("electron plasma only on this path")
         cfg.max_qsa = _num(kv.get(f"{p}.max_qsa_weighting_factor",
                                   kv.get("plasmas.max_qsa_weighting_factor", "35")))
+        cfg.reorder_period = int(_num(kv.get(f"{p}.reorder_period", kv.get("plasmas.reorder_period", "0"))))
         cfg.has_plasma = True
     else:
         cfg.has_plasma = False

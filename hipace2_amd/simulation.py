@@ -42,6 +42,7 @@ def make_params(cfg: SliceConfig) -> _abi.Params:
     p.salame = int(cfg.salame)
     p.deposition = _abi.DEPOSITION[cfg.deposition]
     p.tile_size = cfg.tile_size
+    p.reorder_period = cfg.reorder_period
     return p
 
 

@@ -76,11 +76,11 @@ typedef struct hpq_params {
     double pc_mix;                       /* hipace.predcorr_B_mixing_factor */
     double plasma_charge, plasma_mass;   /* species charge and mass */
     double max_qsa;                      /* plasma.max_qsa_weighting_factor */
-    int bin_period;                      /* atomic deposition: walks the particles sorted by
-                                            32x8-cell tiles, re-sorted every bin_period
-                                            slices (plasmas.reorder_period plays this role
-                                            in the reference); 0: index order. Changes only
-                                            the order of floating-point sums. */
+    int bin_period;                      /* atomic deposition without reordering: walks the
+                                            particles through a permutation sorted by 32x8-cell
+                                            tiles, re-sorted every bin_period slices; 0: index
+                                            order. Changes only the order of floating-point
+                                            sums. */
     int precision;                       /* HPQ_PREC_FP64 or HPQ_PREC_MIXED */
     int poisson_solver;                  /* HPQ_POISSON_FFT or HPQ_POISSON_BF16X3 */
     int salame;                          /* a beam has <beam>.do_salame: allocate the SALAME
@@ -89,6 +89,14 @@ typedef struct hpq_params {
     int tile_size;                       /* hipace.tile_size: tile width of the ordered deposition
                                             (src/Hipace.cpp:160); 0: 32, the reference's default;
                                             at least the widest deposition stencil */
+    int reorder_period;                  /* plasmas.reorder_period (PlasmaParticleContainer.cpp:
+                                            197-208, called at Hipace.cpp:594-595): every slice
+                                            with islice % reorder_period == 0, the plasma arrays
+                                            are reordered in place by deposition tile (32x8
+                                            cells), so the depositions read them contiguously;
+                                            0: never (the reference's default). Applied with
+                                            atomic deposition only: the ordered deposition keeps
+                                            the index order, whose sums it reproduces. */
 } hpq_params;
 
 /* Plasma species, SoA device arrays in PlasmaIdx order
