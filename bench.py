@@ -124,6 +124,7 @@ def run_gpu(args, path, capture):
     cfg.reorder_period = args.reorder_period if cfg.deposition == "atomic" else 0
     if precision == "mixed":
         cfg.poisson_solver = mixed_poisson(args, cfg)
+        cfg.mixed_sums = args.mixed_sums
     sim = Simulation(cfg, beam)
     sim.init_plasma()
     pristine = {k: v.clone() for k, v in sim.plasma.items()}
@@ -521,6 +522,8 @@ def main():
                          "mixed; the first is the headline, the others report their error against it")
     ap.add_argument("--mixed-poisson", choices=["auto", "bf16x3", "fft"], default="auto",
                     help="transverse Poisson solver of the mixed path (auto: bf16x3 up to n = 511, fft beyond)")
+    ap.add_argument("--mixed-sums", choices=["fp64", "kahan32"], default="fp64",
+                    help="window sums of the mixed path's FP32 depositions: FP64 LDS words, or compensated FP32")
     ap.add_argument("--reorder-period", type=int, default=4,
                     help="plasmas.reorder_period of the atomic paths (fp64_atomic, mixed); 0: never")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the beam-loading gradient measurement")
@@ -588,7 +591,8 @@ def main():
                 "speedup_vs_fp64": round(v / slices_per_s, 3),
                 "plasma_pushes_per_s": round(f["stats"]["pushes"] * args.steps / f["elapsed"], 1),
                 "mg_vcycles_per_slice": round(f["stats"]["mg_iters"] / cfg.nz, 3), "dtype": DTYPE[q],
-                "reorder_period": f["cfg"].reorder_period}
+                "reorder_period": f["cfg"].reorder_period,
+                **({"mixed_sums": f["cfg"].mixed_sums} if q == "mixed" else {})}
 
     extra = {}
     for q in paths[1:]:

@@ -105,24 +105,40 @@ __device__ Win choose_window(const int* ci, const int* cj, const bool* act, int 
     return {i0, j0, w, h};
 }
 
-// Accumulation in the window: every contribution, FP64 or the FP32 value of the mixed path,
-// is added to an FP64 LDS word. On gfx950 an FP32 LDS atomic add runs lane by lane (about
-// 0.2 T adds/s over the chip against 4.4 T adds/s for FP64, measured on MI355X), so the
-// mixed path widens its FP32 contributions instead; an FP64 sum of FP32 terms is exact to
-// FP64 rounding, tighter than a compensated (Kahan) FP32 sum would be.
-template <class R>
+// Accumulation in the window. SUM = HPQ_SUM_FP64: every contribution, FP64 or the FP32
+// value of the mixed path, is added to an FP64 LDS word (an FP64 sum of FP32 terms is exact
+// to FP64 rounding). SUM = HPQ_SUM_KAHAN32 (mixed path): each window word holds two FP32
+// values, the sum and its compensation; a contribution is added to the sum with an atomic
+// that returns the old value, and the rounding error of that addition (TwoSum: old + v equals
+// the new sum plus the error exactly) is added to the compensation; the flush adds the two.
+template <class R, int SUM = HPQ_SUM_FP64>
 struct WinAcc {
     static __device__ __forceinline__ void add(double* win, int cells, int s, int k, R v) {
-        lds_add(&win[s * cells + k], double(v));
+        if constexpr (SUM == HPQ_SUM_KAHAN32) {
+            float* f = reinterpret_cast<float*>(&win[s * cells + k]);
+            const float x = float(v);
+            const float old = __hip_atomic_fetch_add(&f[0], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const float sum = old + x;
+            const float bv = sum - old;
+            const float err = (old - (sum - bv)) + (x - bv);
+            if (err != 0.0f) __hip_atomic_fetch_add(&f[1], err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            lds_add(&win[s * cells + k], double(v));
+        }
     }
     static __device__ __forceinline__ double get(const double* win, int cells, int s, int k) {
-        return win[s * cells + k];
+        if constexpr (SUM == HPQ_SUM_KAHAN32) {
+            const float* f = reinterpret_cast<const float*>(&win[s * cells + k]);
+            return double(f[0]) + double(f[1]);
+        } else {
+            return win[s * cells + k];
+        }
     }
 };
 
 // adds the window to the store: quantity k (slot[k] >= 0) goes to component comp[k];
 // cells holding zero are skipped
-template <class R, int NK>
+template <class R, int NK, int SUM = HPQ_SUM_FP64>
 __device__ void flush_window(const Geom& g, const Win& W, const double* win, const int (&slot)[NK],
                              const int (&comp)[NK], double* store) {
     const int cells = W.w * W.h;
@@ -133,7 +149,7 @@ __device__ void flush_window(const Geom& g, const Win& W, const double* win, con
 #pragma unroll
         for (int q = 0; q < NK; ++q) {
             if (slot[q] < 0) continue;
-            const double v = WinAcc<R>::get(win, cells, slot[q], t);
+            const double v = WinAcc<R, SUM>::get(win, cells, slot[q], t);
             if (v != 0.0) atomic_add(store + long(comp[q]) * g.plane + k, v);
         }
     }
@@ -153,7 +169,7 @@ __device__ __forceinline__ int assign_slots(const int (&comp)[NK], int (&slot)[N
 
 // ::DepositCurrent deposit functor, src/particles/deposition/PlasmaDepositCurrent.cpp:182-251,
 // arithmetic in R. comp order: jx, jy, jz, rho, chi, rhomjz (-1: skip; jx and jy go together).
-template <int O, class R>
+template <int O, class R, int SUM = HPQ_SUM_FP64>
 __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, const int32_t* __restrict__ order,
                                                        double* __restrict__ store, int cjx, int cjy, int cjz, int crho,
                                                        int cchi, int crm, double charge_invvol,
@@ -228,7 +244,7 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
                     const int k = k0 + iy * W.w + ix;
 #pragma unroll
                     for (int a = 0; a < 6; ++a)
-                        if (slot[a] >= 0) WinAcc<R>::add(win, cells, slot[a], k, v[a]);
+                        if (slot[a] >= 0) WinAcc<R, SUM>::add(win, cells, slot[a], k, v[a]);
                 }
             }
             continue;
@@ -246,7 +262,7 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
                     const int k = W.at(i, j);
 #pragma unroll
                     for (int a = 0; a < 6; ++a)
-                        if (slot[a] >= 0) WinAcc<R>::add(win, cells, slot[a], k, v[a]);
+                        if (slot[a] >= 0) WinAcc<R, SUM>::add(win, cells, slot[a], k, v[a]);
                 } else {
                     const long k = g.idx(i, j);
 #pragma unroll
@@ -257,12 +273,12 @@ __global__ __launch_bounds__(DT) void k_deposit_plasma(Geom g, hpq_plasma p, con
         }
     }
     __syncthreads();
-    flush_window<R>(g, W, win, slot, comp, store);
+    flush_window<R, 6, SUM>(g, W, win, slot, comp, store);
 }
 
 // ::ExplicitDeposition deposit functor, src/particles/deposition/ExplicitDeposition.cpp:166-262,
 // arithmetic in R (the fields are read from the FP64 store and rounded to R)
-template <int O, int T, class R>
+template <int O, int T, class R, int SUM = HPQ_SUM_FP64>
 __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, const int32_t* __restrict__ order,
                                                          double* __restrict__ store, int cBz, int cEz, int cExmBy,
                                                          int cEypBx, int cSy, int cSx, double charge_invvol_mu0,
@@ -366,8 +382,8 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
                     R sy_add, sx_add;
                     contrib(ix, iy, R(wf[kw]), R(wf[cells + kw]), R(wf[2 * cells + kw]), R(wf[3 * cells + kw]),
                             sy_add, sx_add);
-                    WinAcc<R>::add(win, cells, 0, kw, sy_add);
-                    WinAcc<R>::add(win, cells, 1, kw, sx_add);
+                    WinAcc<R, SUM>::add(win, cells, 0, kw, sy_add);
+                    WinAcc<R, SUM>::add(win, cells, 1, kw, sx_add);
                 }
             }
             continue;
@@ -391,8 +407,8 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
                 R sy_add, sx_add;
                 contrib(ix, iy, Bz_v, Ez_v, ExmBy_v, EypBx_v, sy_add, sx_add);
                 if (local) {
-                    WinAcc<R>::add(win, cells, 0, kw, sy_add);
-                    WinAcc<R>::add(win, cells, 1, kw, sx_add);
+                    WinAcc<R, SUM>::add(win, cells, 0, kw, sy_add);
+                    WinAcc<R, SUM>::add(win, cells, 1, kw, sx_add);
                 } else {
                     atomic_add(Sy + k, double(sy_add));
                     atomic_add(Sx + k, double(sx_add));
@@ -403,7 +419,7 @@ __global__ __launch_bounds__(DT) void k_explicit_deposit(Geom g, hpq_plasma p, c
     __syncthreads();
     const int comp[2] = {cSy, cSx};
     const int slot[2] = {0, 1};
-    flush_window<R>(g, W, win, slot, comp, store);
+    flush_window<R, 2, SUM>(g, W, win, slot, comp, store);
 }
 
 // ::DepositCurrentSlice deposit functor, src/particles/deposition/BeamDepositCurrent.cpp:133-193

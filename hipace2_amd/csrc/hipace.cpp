@@ -745,6 +745,8 @@ int hpq_create(const hpq_params* p, hpq_ctx** out) {
         if (p->deposition != HPQ_DEPOSIT_ORDERED && p->deposition != HPQ_DEPOSIT_ATOMIC)
             throw HpqError(HPQ_E_INVALID, "deposition must be HPQ_DEPOSIT_ORDERED or HPQ_DEPOSIT_ATOMIC");
         if (p->reorder_period < 0) throw HpqError(HPQ_E_INVALID, "reorder_period must be >= 0");
+        if (p->mixed_sums != HPQ_SUM_FP64 && p->mixed_sums != HPQ_SUM_KAHAN32)
+            throw HpqError(HPQ_E_INVALID, "mixed_sums must be HPQ_SUM_FP64 or HPQ_SUM_KAHAN32");
         {
             // the widest stencil the ordered deposition walks: O + 1 cells for the current and
             // beam deposition, the derivative stencil for the explicit one

@@ -83,6 +83,9 @@ class SliceConfig:
     # "fp64": the drop-in path; "mixed": FP32 gather, push increments and deposition with
     # FP32 contributions summed in FP64 LDS words (FP64 store, particle state and multigrid)
     precision: str = "fp64"
+    # the mixed path's window sums: "fp64" (FP32 contributions into FP64 LDS words) or
+    # "kahan32" (compensated FP32 sums: an FP32 sum word and an FP32 error word per cell)
+    mixed_sums: str = "fp64"
     # transverse Poisson solves: "fft" (DST-I by FFT, FP64) or "bf16x3" (DST-I as sine-matrix
     # GEMMs on the matrix cores, FP32 operands split into two BF16 terms each)
     poisson_solver: str = "fft"

@@ -43,6 +43,7 @@ def make_params(cfg: SliceConfig) -> _abi.Params:
     p.deposition = _abi.DEPOSITION[cfg.deposition]
     p.tile_size = cfg.tile_size
     p.reorder_period = cfg.reorder_period
+    p.mixed_sums = _abi.MIXED_SUMS[cfg.mixed_sums]
     return p
 
 

@@ -50,6 +50,12 @@ enum { HPQ_PREC_FP64 = 0, HPQ_PREC_MIXED = 1 };
  * atomic kernels. */
 enum { HPQ_DEPOSIT_ORDERED = 0, HPQ_DEPOSIT_ATOMIC = 1 };
 
+/* Window sums of the mixed path's FP32 depositions: each FP32 contribution added to an FP64 LDS
+ * word (SUM_FP64, exact to FP64 rounding), or to an FP32 word with a second FP32 word that
+ * collects the rounding error of every addition (SUM_KAHAN32: the compensated FP32 tile sums;
+ * the error of each atomic addition is recovered exactly from its returned old value). */
+enum { HPQ_SUM_FP64 = 0, HPQ_SUM_KAHAN32 = 1 };
+
 /* Transverse Dirichlet Poisson solver (fields.poisson_solver, src/fields/Fields.cpp:179-208):
  * DST-I by FFT in FP64, or DST-I as dense sine-matrix GEMMs on the matrix cores with each
  * FP32 operand split into two BF16 terms (hi + lo; three products hi*hi + hi*lo + lo*hi,
@@ -97,6 +103,7 @@ typedef struct hpq_params {
                                             0: never (the reference's default). Applied with
                                             atomic deposition only: the ordered deposition keeps
                                             the index order, whose sums it reproduces. */
+    int mixed_sums;                      /* HPQ_SUM_FP64 or HPQ_SUM_KAHAN32 (mixed path) */
 } hpq_params;
 
 /* Plasma species, SoA device arrays in PlasmaIdx order

@@ -55,6 +55,36 @@ def test_mixed_deposit_current(order, mode):
     assert np.array_equal(pg["valid"], po["valid"])
 
 
+@pytest.mark.parametrize("order", [1, 2])
+def test_mixed_compensated_fp32_sums(order):
+    """HPQ_SUM_KAHAN32: FP32 window words with an FP32 compensation word each. The FP32
+    contributions are the same as with FP64 words; each addition's rounding error is recovered,
+    so the planes stay within the FP32 contributions' budget (2e-6) and within a small factor of
+    the FP64-word sums (the compensation word rounds in FP32 too: 2.8x on chi at order 1)."""
+    errs = {}
+    for sums in ("fp64", "kahan32"):
+        cfg = _mixed(small_config(order=order, dtype=2))
+        cfg.deposition, cfg.bin_period = "atomic", 1
+        cfg.mixed_sums = sums
+        sim, orc = _pair(cfg, n_particles=60000)
+        comps = [k if sim.layout[k] >= 0 else -1 for k in ("jx", "jy", "jz", "rho", "chi", "rhomjz")]
+        names = [k for k in comps if k != -1]
+        for name in names:
+            sim.store[sim.layout[name]].zero_()
+        st = orc.store()
+        for name in names:
+            st[orc.layout[name]] = 0
+        orc.set_store(st)
+        assert sim.deposit_current(comps) == orc.deposit_plasma(comps)
+        sim.explicit_deposition()
+        orc.explicit_deposit()
+        g, o = sim.store_host(), orc.store()
+        errs[sums] = {name: rel_err(g[sim.layout[name]], o[orc.layout[name]]) for name in names + ["Sx", "Sy"]}
+    for name, e in errs["kahan32"].items():
+        assert e <= 2e-6, (name, e)
+        assert e <= 10.0 * errs["fp64"][name] + 1e-9, (name, e, errs["fp64"][name])
+
+
 @pytest.mark.parametrize("order,dtype", [(1, 2), (2, 2), (2, 1), (2, 0), (3, 2), (0, 1)])
 @pytest.mark.parametrize("mode", DEPOSIT_MODES)
 def test_mixed_explicit_deposition(order, dtype, mode):
