@@ -287,6 +287,7 @@ public:
             }
             InitializeSxSyWithBeam();
             explicit_deposition(c_, p_, s_);
+            if (P.mg_guess == HPQ_MG_GUESS_LINEAR) mg_guess_linear(c_, s_ + long(L_.Bx) * c_->g.plane, islice);
             ExplicitMGSolveBxBy();
             if (c_->salame.mask && salame_count(c_, b_, islice) > 0) SalameModule(islice);
         } else {
@@ -615,6 +616,8 @@ void adjoint_check(const hpq_ctx* c) {
     if (c->salame.mask) throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: SALAME must be off");
     if (plasma_reordered(c))
         throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: plasma reordering (reorder_period) must be off");
+    if (P.mg_guess != HPQ_MG_GUESS_PREVIOUS)
+        throw HpqError(HPQ_E_UNSUPPORTED, "adjoint: mg_guess must be HPQ_MG_GUESS_PREVIOUS");
 }
 
 // the forward pass with checkpoints every k slices
@@ -747,6 +750,10 @@ int hpq_create(const hpq_params* p, hpq_ctx** out) {
         if (p->reorder_period < 0) throw HpqError(HPQ_E_INVALID, "reorder_period must be >= 0");
         if (p->mixed_sums != HPQ_SUM_FP64 && p->mixed_sums != HPQ_SUM_KAHAN32)
             throw HpqError(HPQ_E_INVALID, "mixed_sums must be HPQ_SUM_FP64 or HPQ_SUM_KAHAN32");
+        if (p->mg_guess != HPQ_MG_GUESS_PREVIOUS && p->mg_guess != HPQ_MG_GUESS_LINEAR)
+            throw HpqError(HPQ_E_INVALID, "mg_guess must be HPQ_MG_GUESS_PREVIOUS or HPQ_MG_GUESS_LINEAR");
+        if (p->mg_guess == HPQ_MG_GUESS_LINEAR && !p->explicit_solver)
+            throw HpqError(HPQ_E_UNSUPPORTED, "mg_guess applies to the explicit Bx/By solver");
         {
             // the widest stencil the ordered deposition walks: O + 1 cells for the current and
             // beam deposition, the derivative stencil for the explicit one
@@ -1117,6 +1124,7 @@ int hpq_begin_step(hpq_ctx* c, const hpq_plasma* p, double* store) {
         for (double& s : c->stats) s = 0;
         c->bins.stale = true;
         c->bins.reordered = false;   // a step starts from the plasma in injection order
+        c->mg.hist_slice = -1;       // no Bx, By history across steps
         if (c->L.ions_rhomjz >= 0) DepositCurrent(c, p, store, -1, -1, -1, -1, -1, c->L.ions_rhomjz, true);
     });
 }

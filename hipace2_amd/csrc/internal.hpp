@@ -133,6 +133,9 @@ struct MultiGrid {
     MGMirror* h_mirror = nullptr;    // pinned coherent host memory
     MGMirror* d_mirror = nullptr;    // its device address
     int seq = 0;
+    // HPQ_MG_GUESS_LINEAR: Bx, By of the slice before the last one solved (2 planes)
+    double* hist = nullptr;
+    int hist_slice = -1;             // the slice whose Bx, By hist holds; -1: none
 };
 
 struct Capture {
@@ -297,6 +300,10 @@ void build_rhs_bxby(hpq_ctx* c, const double* store);
 void exmby_eypbx(hpq_ctx* c, double* store);
 void mg_init(hpq_ctx* c);
 void mg_free(hpq_ctx* c);
+// The multigrid's initial guess for the Bx, By of slice islice (two planes from bxby, one
+// plane apart) under HPQ_MG_GUESS_LINEAR: 2 B(islice + 1) - B(islice + 2) when the history
+// holds slice islice + 2 and bxby slice islice + 1; otherwise bxby is kept and recorded.
+void mg_guess_linear(hpq_ctx* c, double* bxby, int islice);
 int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, long comp_stride, double tol_rel,
               double tol_abs, int maxiter, int* iters);
 void deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32_t* comps, bool ions);

@@ -50,6 +50,20 @@ constexpr int COARSE_MAX_LEVELS = 16;
 __host__ __device__ __forceinline__ long at(const MGLevel& L, int i, int j) {
     return long(j - L.jlo + 1) * L.ldx + (i - L.ilo + 1);
 }
+
+// Tile of this workgroup. Workgroups go to the eight XCDs in turn (linear id mod 8), and
+// each XCD has its own L2: the tiles are renumbered so that XCD k takes the k-th eighth of
+// the tiles in row-major order, a band of whole tile rows, and the halos that neighbouring
+// tiles read twice are mostly read from that XCD's L2 (ids beyond the last multiple of
+// eight keep their tile).
+__device__ __forceinline__ void xcd_tile(int& bx, int& by) {
+    const int gx = int(gridDim.x), n = gx * int(gridDim.y);
+    const int id = int(blockIdx.x) + int(blockIdx.y) * gx;
+    const int m = n & ~7;
+    const int t = id < m ? (id & 7) * (m >> 3) + (id >> 3) : id;
+    by = t / gx;
+    bx = t - by * gx;
+}
 template <class LV>
 __device__ __forceinline__ bool in_valid(const LV& L, int i, int j) {
     return i >= L.vilo && i <= L.vihi && j >= L.vjlo && j <= L.vjhi;
@@ -308,13 +322,16 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
     static_assert(!NORMS || RES, "the norms come from the residual");
     static_assert(!ACF || RES, "acf is restricted with the residual");
     const int gate_done = (fold.mode == Fold::NONE) ? (gate ? gate->done : 0) : 0;
-    const int ti0 = L.vilo + int(blockIdx.x) * TX - HALO;
-    const int tj0 = L.vjlo + int(blockIdx.y) * TY - HALO;
+    int bx, by;
+    xcd_tile(bx, by);
+    const int ti0 = L.vilo + bx * TX - HALO;
+    const int tj0 = L.vjlo + by * TY - HALO;
     const int base = (ti0 + tj0) & 1;
     auto S = [&](int c, int ly, int lx) -> double& { return s[c][(base + lx + ly) & 1][ly][lx >> 1]; };
     const double facx = 1. / (L.dx * L.dx), facy = 1. / (L.dy * L.dy);
-    // owned points: [parity][q]; pl = lx | ly << 8, or -1 outside the valid box / region
-    double r0[2][KP], r1[2][KP], ci[2][KP];
+    // owned points: [parity][q]; pl = lx | ly << 8, or -1 outside the valid box / region;
+    // acf stays in registers for the residual (ai), ci becomes 1/c0
+    double r0[2][KP], r1[2][KP], ci[2][KP], ai[2][RES ? KP : 1];
     int pl[2][KP];
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
@@ -323,6 +340,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
             const int e = threadIdx.x + q * NT;
             pl[par][q] = -1;
             r0[par][q] = r1[par][q] = ci[par][q] = 0.0;
+            if (RES) ai[par][q] = 0.0;
             if (e < NPAR) {
                 const int ly = 1 + e / HW;
                 const int lx = 1 + 2 * (e - (ly - 1) * HW) + ((par + base + ly + 1) & 1);
@@ -333,6 +351,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                     r0[par][q] = rhs[k];
                     r1[par][q] = rhs[k + rcs];
                     ci[par][q] = acf[k];   // acf for now, 1/c0 below
+                    if (RES) ai[par][q] = ci[par][q];
                 }
             }
         }
@@ -472,7 +491,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                 const bool ring = lx >= HALO - 1 && lx <= HALO + TX && ly >= HALO - 1 && ly <= HALO + TY;
                 if (ring) {
                     const int i = ti0 + lx, j = tj0 + ly;
-                    const double a = acf[at(L, i, j)];
+                    const double a = ai[par][RES ? q : 0];
                     // the point has colour par, its four neighbours the other one
                     auto P0 = [&](int di, int dj) {
                         return (di | dj) ? s[0][1 - par][ly + dj][(lx + di) >> 1] : s[0][par][ly][lx >> 1];
@@ -540,7 +559,7 @@ __global__ __launch_bounds__(NT) void k_smooth(MGLevel L, MGLevel C, double* __r
                 mm = fmax(mm, red[0][w]);
                 rr = fmax(rr, red[1][w]);
             }
-            const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
+            const unsigned b = by * gridDim.x + bx;
             norms[2 * b] = mm;
             norms[2 * b + 1] = rr;
         }
@@ -753,6 +772,16 @@ __global__ void k_mg_store(MGLevel L, int sh, Geom g, double* __restrict__ sol, 
     sol[gk + cstride] = src[k + L.cs];
 }
 
+// initial guess of the next solve: b = 2 b - h and h = old b (EXTRAP), or h = b (record only)
+template <bool EXTRAP>
+__global__ void k_guess_linear(double* __restrict__ b, double* __restrict__ h, long n) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double b1 = b[t];
+    if (EXTRAP) b[t] = 2.0 * b1 - h[t];
+    h[t] = b1;
+}
+
 bool coarsenable(bool cc, int lo, int hi, int min_width) {
     const int size = hi - lo + 1;
     if (size < 2 * min_width) return false;
@@ -885,8 +914,22 @@ void mg_init(hpq_ctx* c) {
     }
 }
 
+void mg_guess_linear(hpq_ctx* c, double* bxby, int islice) {
+    MultiGrid& M = c->mg;
+    const long n = 2 * c->g.plane;
+    if (!M.hist) HPQ_HIP(hipMalloc(&M.hist, sizeof(double) * n));
+    const dim3 grid(unsigned((n + 255) / 256));
+    if (M.hist_slice == islice + 2)
+        hipLaunchKernelGGL(k_guess_linear<true>, grid, dim3(256), 0, c->stream, bxby, M.hist, n);
+    else
+        hipLaunchKernelGGL(k_guess_linear<false>, grid, dim3(256), 0, c->stream, bxby, M.hist, n);
+    HPQ_HIP(hipGetLastError());
+    M.hist_slice = islice + 1;
+}
+
 void mg_free(hpq_ctx* c) {
     MultiGrid& M = c->mg;
+    if (M.hist) (void)hipFree(M.hist);
     for (auto& L : M.lev)
         for (void* p : {(void*)L.acf, (void*)L.res, (void*)L.cor, (void*)L.rescor})
             if (p) (void)hipFree(p);

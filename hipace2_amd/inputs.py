@@ -86,6 +86,9 @@ class SliceConfig:
     # the mixed path's window sums: "fp64" (FP32 contributions into FP64 LDS words) or
     # "kahan32" (compensated FP32 sums: an FP32 sum word and an FP32 error word per cell)
     mixed_sums: str = "fp64"
+    # the explicit solver's multigrid starts from "previous" (the previous slice's Bx, By, as
+    # the reference does) or "linear" (extrapolated from the two previous slices)
+    mg_guess: str = "previous"
     # transverse Poisson solves: "fft" (DST-I by FFT, FP64) or "bf16x3" (DST-I as sine-matrix
     # GEMMs on the matrix cores, FP32 operands split into two BF16 terms each)
     poisson_solver: str = "fft"

@@ -44,6 +44,7 @@ def make_params(cfg: SliceConfig) -> _abi.Params:
     p.tile_size = cfg.tile_size
     p.reorder_period = cfg.reorder_period
     p.mixed_sums = _abi.MIXED_SUMS[cfg.mixed_sums]
+    p.mg_guess = _abi.MG_GUESS[cfg.mg_guess]
     return p
 
 

@@ -56,6 +56,13 @@ enum { HPQ_DEPOSIT_ORDERED = 0, HPQ_DEPOSIT_ATOMIC = 1 };
  * the error of each atomic addition is recovered exactly from its returned old value). */
 enum { HPQ_SUM_FP64 = 0, HPQ_SUM_KAHAN32 = 1 };
 
+/* Initial guess of the explicit solver's multigrid for Bx, By. PREVIOUS: the previous
+ * slice's solution, which the slice store still holds (the reference: Hipace.cpp:920 solves
+ * in place). LINEAR: extrapolated from the two previous slices, 2 B(i+1) - B(i+2), as
+ * Fields::InitialBfieldGuess (Fields.cpp:1150-1171) extrapolates for the predictor-corrector
+ * loop; fewer V-cycles, results within the multigrid tolerance of PREVIOUS's. */
+enum { HPQ_MG_GUESS_PREVIOUS = 0, HPQ_MG_GUESS_LINEAR = 1 };
+
 /* Transverse Dirichlet Poisson solver (fields.poisson_solver, src/fields/Fields.cpp:179-208):
  * DST-I by FFT in FP64, or DST-I as dense sine-matrix GEMMs on the matrix cores with each
  * FP32 operand split into two BF16 terms (hi + lo; three products hi*hi + hi*lo + lo*hi,
@@ -104,6 +111,7 @@ typedef struct hpq_params {
                                             atomic deposition only: the ordered deposition keeps
                                             the index order, whose sums it reproduces. */
     int mixed_sums;                      /* HPQ_SUM_FP64 or HPQ_SUM_KAHAN32 (mixed path) */
+    int mg_guess;                        /* HPQ_MG_GUESS_PREVIOUS or HPQ_MG_GUESS_LINEAR */
 } hpq_params;
 
 /* Plasma species, SoA device arrays in PlasmaIdx order

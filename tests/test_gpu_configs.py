@@ -125,6 +125,7 @@ def test_config3_fp64_reproducible_and_mixed_within_1e4():
     e1 = ez1.clone()
     sim.run_step()
     torch.cuda.synchronize()
+    vcycles_fp64 = sim.stats()["mg_iters"]
     # the FP64 path with ordered deposition repeats bit for bit
     assert torch.equal(ez1, e1)
     assert float(e1.abs().max()) > 0
@@ -136,6 +137,20 @@ def test_config3_fp64_reproducible_and_mixed_within_1e4():
     torch.cuda.synchronize()
     err = float((ezm - e1).abs().max() / e1.abs().max())
     assert err <= 1e-4, err
+    del sim_m
+    # the mixed path with atomic deposition, the plasma reordered every 4 slices, and the
+    # multigrid started from Bx, By extrapolated from the two previous slices: the same
+    # accuracy bound (the extrapolation saves no V-cycles here: the slice-to-slice change of
+    # Bx, By is dominated by the beam's shot noise, which extrapolation does not predict)
+    cfg_b = dataclasses_replace(cfg_m, deposition="atomic", reorder_period=4, mg_guess="linear")
+    sim_b = Simulation(cfg_b, beam)
+    ezb = sim_b.capture("Ez")
+    sim_b.run_step()
+    torch.cuda.synchronize()
+    err_b = float((ezb - e1).abs().max() / e1.abs().max())
+    print(f"mixed (bench settings): Ez error {err_b:.3e}, V-cycles {sim_b.stats()['mg_iters']} vs {vcycles_fp64}")
+    assert err_b <= 1e-4, err_b
+    assert abs(sim_b.stats()["mg_iters"] - vcycles_fp64) <= 0.05 * vcycles_fp64
 
 
 def dataclasses_replace(cfg, **kw):
