@@ -126,6 +126,7 @@ def run_gpu(args, path, capture):
         cfg.poisson_solver = mixed_poisson(args, cfg)
         cfg.mixed_sums = args.mixed_sums
         cfg.mg_guess = args.mixed_mg_guess
+        cfg.mg_guess_iters = args.mixed_mg_guess_iters
     sim = Simulation(cfg, beam)
     sim.init_plasma()
     pristine = {k: v.clone() for k, v in sim.plasma.items()}
@@ -525,9 +526,12 @@ def main():
                     help="transverse Poisson solver of the mixed path (auto: bf16x3 up to n = 511, fft beyond)")
     ap.add_argument("--mixed-sums", choices=["fp64", "kahan32"], default="fp64",
                     help="window sums of the mixed path's FP32 depositions: FP64 LDS words, or compensated FP32")
-    ap.add_argument("--mixed-mg-guess", choices=["previous", "linear"], default="previous",
+    ap.add_argument("--mixed-mg-guess", choices=["previous", "linear", "dst"], default="dst",
                     help="initial guess of the mixed path's multigrid: the previous slice's Bx, By (as the FP64 "
-                         "paths and the reference), or extrapolated from the two previous slices")
+                         "paths and the reference), extrapolated from the two previous slices, or the previous "
+                         "slice's corrected by shifted FFT DST solves with chi replaced by its mean")
+    ap.add_argument("--mixed-mg-guess-iters", type=int, default=1,
+                    help="DST corrections per slice of --mixed-mg-guess dst")
     ap.add_argument("--reorder-period", type=int, default=4,
                     help="plasmas.reorder_period of the atomic paths (fp64_atomic, mixed); 0: never")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the beam-loading gradient measurement")
@@ -596,7 +600,8 @@ def main():
                 "plasma_pushes_per_s": round(f["stats"]["pushes"] * args.steps / f["elapsed"], 1),
                 "mg_vcycles_per_slice": round(f["stats"]["mg_iters"] / cfg.nz, 3), "dtype": DTYPE[q],
                 "reorder_period": f["cfg"].reorder_period,
-                **({"mixed_sums": f["cfg"].mixed_sums, "mg_guess": f["cfg"].mg_guess} if q == "mixed" else {})}
+                **({"mixed_sums": f["cfg"].mixed_sums, "mg_guess": f["cfg"].mg_guess,
+                    "mg_guess_iters": f["cfg"].mg_guess_iters} if q == "mixed" else {})}
 
     extra = {}
     for q in paths[1:]:

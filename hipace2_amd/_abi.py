@@ -50,12 +50,13 @@ class Params(C.Structure):
                 ("plasma_charge", C.c_double), ("plasma_mass", C.c_double), ("max_qsa", C.c_double),
                 ("bin_period", C.c_int), ("precision", C.c_int), ("poisson_solver", C.c_int),
                 ("salame", C.c_int), ("deposition", C.c_int), ("tile_size", C.c_int),
-                ("reorder_period", C.c_int), ("mixed_sums", C.c_int), ("mg_guess", C.c_int)]
+                ("reorder_period", C.c_int), ("mixed_sums", C.c_int), ("mg_guess", C.c_int),
+                ("mg_guess_iters", C.c_int)]
 
 PRECISION = {"fp64": 0, "mixed": 1}
 DEPOSITION = {"ordered": 0, "atomic": 1}
 MIXED_SUMS = {"fp64": 0, "kahan32": 1}
-MG_GUESS = {"previous": 0, "linear": 1}
+MG_GUESS = {"previous": 0, "linear": 1, "dst": 2}
 POISSON_SOLVER = {"fft": 0, "bf16x3": 1}
 
 

@@ -288,6 +288,7 @@ public:
             InitializeSxSyWithBeam();
             explicit_deposition(c_, p_, s_);
             if (P.mg_guess == HPQ_MG_GUESS_LINEAR) mg_guess_linear(c_, s_ + long(L_.Bx) * c_->g.plane, islice);
+            if (P.mg_guess == HPQ_MG_GUESS_DST) bxby_guess_dst(c_, s_, P.mg_guess_iters);
             ExplicitMGSolveBxBy();
             if (c_->salame.mask && salame_count(c_, b_, islice) > 0) SalameModule(islice);
         } else {
@@ -750,10 +751,17 @@ int hpq_create(const hpq_params* p, hpq_ctx** out) {
         if (p->reorder_period < 0) throw HpqError(HPQ_E_INVALID, "reorder_period must be >= 0");
         if (p->mixed_sums != HPQ_SUM_FP64 && p->mixed_sums != HPQ_SUM_KAHAN32)
             throw HpqError(HPQ_E_INVALID, "mixed_sums must be HPQ_SUM_FP64 or HPQ_SUM_KAHAN32");
-        if (p->mg_guess != HPQ_MG_GUESS_PREVIOUS && p->mg_guess != HPQ_MG_GUESS_LINEAR)
-            throw HpqError(HPQ_E_INVALID, "mg_guess must be HPQ_MG_GUESS_PREVIOUS or HPQ_MG_GUESS_LINEAR");
-        if (p->mg_guess == HPQ_MG_GUESS_LINEAR && !p->explicit_solver)
+        if (p->mg_guess != HPQ_MG_GUESS_PREVIOUS && p->mg_guess != HPQ_MG_GUESS_LINEAR &&
+            p->mg_guess != HPQ_MG_GUESS_DST)
+            throw HpqError(HPQ_E_INVALID, "mg_guess must be HPQ_MG_GUESS_PREVIOUS, _LINEAR or _DST");
+        if (p->mg_guess != HPQ_MG_GUESS_PREVIOUS && !p->explicit_solver)
             throw HpqError(HPQ_E_UNSUPPORTED, "mg_guess applies to the explicit Bx/By solver");
+        if (p->mg_guess == HPQ_MG_GUESS_DST) {
+            auto pow2 = [](int v) { return v > 1 && (v & (v - 1)) == 0 && v <= 2048; };
+            if (!pow2(p->nx + 1) || !pow2(p->ny + 1))
+                throw HpqError(HPQ_E_UNSUPPORTED, "mg_guess DST needs nx + 1 and ny + 1 powers of two");
+            if (p->mg_guess_iters < 1) throw HpqError(HPQ_E_INVALID, "mg_guess_iters must be >= 1");
+        }
         {
             // the widest stencil the ordered deposition walks: O + 1 cells for the current and
             // beam deposition, the derivative stencil for the explicit one

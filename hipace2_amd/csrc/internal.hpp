@@ -73,6 +73,9 @@ struct Poisson {
     int nx = 0, ny = 0;
     DstPlan px, py;
     double* eigT = nullptr;   // [i][j] eigenvalue matrix with the RODFT00 normalisation
+    double inv_norm = 0;      // 1 / the normalisation factor in eigT
+    double* chi_part = nullptr;       // HPQ_MG_GUESS_DST: partial sums of chi, then its mean
+    unsigned* chi_count = nullptr;
     double* bufA = nullptr;   // 3 * nx * ny
     double* bufB = nullptr;
     // sine-matrix GEMM path (HPQ_POISSON_BF16X3): sizes padded to 64, S split into BF16
@@ -304,6 +307,9 @@ void mg_free(hpq_ctx* c);
 // plane apart) under HPQ_MG_GUESS_LINEAR: 2 B(islice + 1) - B(islice + 2) when the history
 // holds slice islice + 2 and bxby slice islice + 1; otherwise bxby is kept and recorded.
 void mg_guess_linear(hpq_ctx* c, double* bxby, int islice);
+// HPQ_MG_GUESS_DST: iters corrections (Bx, By) += (Lap - mean chi)^-1 [(Sy, Sx) - (Lap - chi) (Bx, By)]
+// by the FFT DST solve, starting from the store's Bx, By (the previous slice's solution)
+void bxby_guess_dst(hpq_ctx* c, double* store, int iters);
 int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, long comp_stride, double tol_rel,
               double tol_abs, int maxiter, int* iters);
 void deposit_current(hpq_ctx* c, const hpq_plasma* p, double* store, const int32_t* comps, bool ions);

@@ -112,11 +112,17 @@ __device__ __forceinline__ void load_odd_extension(double2* a, int n, int M, int
 // RODFT00(a)_{k-1} = -Im F_k and RODFT00(b)_{k-1} = Re F_k), optionally times the
 // eigenvalue rows (scale != null). SYM: the result is transformed once more in LDS (the
 // inverse pass of the same direction), so the middle of the 2-D solve is one launch.
+// shift (device scalar, may be null): the scale entries s = f / lambda become
+// f / (lambda - shift) = s / (1 - shift s / f), f = 1 / inv_f, the solve of
+// (Lap - shift) u = rhs. add: the output is added to the destination.
 template <bool SYM>
 __global__ __launch_bounds__(DST_T) void k_dst_rows_fft(Rows in, Rows out, int nrows, int n, int M, int logM,
                                                         const double2* __restrict__ tw,
-                                                        const double* __restrict__ scale) {
+                                                        const double* __restrict__ scale,
+                                                        const double* __restrict__ shift, double inv_f, int add) {
     extern __shared__ double2 a[];
+    const double sh = shift ? *shift : 0.0;
+    auto shifted = [&](double s) { return shift ? s / (1.0 - sh * s * inv_f) : s; };
     const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
     const bool has1 = r1 < nrows;
     const double* p0 = row_ptr(in, r0);
@@ -140,8 +146,8 @@ __global__ __launch_bounds__(DST_T) void k_dst_rows_fft(Rows in, Rows out, int n
                 if (src) {
                     const double2 F = a[sw(src)];
                     double o0 = -F.y, o1 = F.x;
-                    if (s0) o0 *= s0[src - 1];
-                    if (s1) o1 *= s1[src - 1];
+                    if (s0) o0 *= shifted(s0[src - 1]);
+                    if (s1) o1 *= shifted(s1[src - 1]);
                     if (!has1) o1 = 0.0;
                     nv[q] = m <= n ? make_double2(o0, o1) : make_double2(-o0, -o1);
                 }
@@ -165,10 +171,15 @@ __global__ __launch_bounds__(DST_T) void k_dst_rows_fft(Rows in, Rows out, int n
         const int k = m - 1;
         const double2 F = a[sw(m)];
         double o0 = -F.y, o1 = F.x;
-        if (s0) o0 *= s0[k];
-        if (s1) o1 *= s1[k];
-        q0[k] = o0;
-        if (has1) q1[k] = o1;
+        if (s0) o0 *= shifted(s0[k]);
+        if (s1) o1 *= shifted(s1[k]);
+        if (add) {
+            q0[k] += o0;
+            if (has1) q1[k] += o1;
+        } else {
+            q0[k] = o0;
+            if (has1) q1[k] = o1;
+        }
     }
 }
 
@@ -216,17 +227,18 @@ __global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ in
 // RODFT00 of nrows rows (times the eigenvalue rows when scale != null); sym: followed by
 // the inverse transform of the same rows (fused in LDS on the FFT path)
 static void dst_rows(hpq_ctx* c, const DstPlan& P, const Rows& in, const Rows& out, int nrows, const double* scale,
-                     bool sym = false) {
+                     bool sym = false, const double* shift = nullptr, double inv_f = 0.0, bool add = false) {
     TraceScope ts(c, HPQ_TRACE_DST);
     if (P.fft) {
         const size_t lds = size_t(P.M) * sizeof(double2);
         if (sym)
             hipLaunchKernelGGL(k_dst_rows_fft<true>, dim3((nrows + 1) / 2), dim3(DST_T), lds, c->stream, in, out,
-                               nrows, P.n, P.M, P.logM, P.tw, scale);
+                               nrows, P.n, P.M, P.logM, P.tw, scale, shift, inv_f, int(add));
         else
             hipLaunchKernelGGL(k_dst_rows_fft<false>, dim3((nrows + 1) / 2), dim3(DST_T), lds, c->stream, in, out,
-                               nrows, P.n, P.M, P.logM, P.tw, scale);
+                               nrows, P.n, P.M, P.logM, P.tw, scale, shift, inv_f, int(add));
     } else {
+        if (shift || add) throw HpqError(HPQ_E_UNSUPPORTED, "shifted DST solve: n + 1 must be a power of two");
         const size_t lds = size_t(P.n) * sizeof(double);
         hipLaunchKernelGGL(k_dst_rows_direct, dim3(nrows), dim3(256), lds, c->stream, in, out, P.n, P.sintab, scale);
         if (sym) {
@@ -281,6 +293,7 @@ void poisson_init(hpq_ctx* c) {
     const double dxsq = c->g.dx * c->g.dx, dysq = c->g.dy * c->g.dy;
     const double sxf = M_PI / (2. * (nx + 1)), syf = M_PI / (2. * (ny + 1));
     const double norm_fac = 0.5 / (2 * ((nx + 1) * (ny + 1)));
+    S.inv_norm = 1.0 / norm_fac;
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j) {
             const double sx = std::sin((i + 1) * sxf) * std::sin((i + 1) * sxf);
@@ -299,7 +312,7 @@ void poisson_init(hpq_ctx* c) {
 void poisson_free(hpq_ctx* c) {
     Poisson& S = c->ps;
     for (void* p : {(void*)S.px.tw, (void*)S.px.sintab, (void*)S.py.tw, (void*)S.py.sintab, (void*)S.eigT,
-                    (void*)S.bufA, (void*)S.bufB})
+                    (void*)S.bufA, (void*)S.bufB, (void*)S.chi_part, (void*)S.chi_count})
         if (p) (void)hipFree(p);
     dst_mfma_free(c);
     S = Poisson{};
@@ -307,11 +320,10 @@ void poisson_free(hpq_ctx* c) {
 
 // Solves nfields staged right-hand sides held in ps.bufA as [f][j][i] and writes the
 // valid region of field f into lhs_planes[f] (guarded planes).
-void poisson_solve_staged(hpq_ctx* c, int nf, double* const* lhs_planes) {
-    if (c->p.poisson_solver == HPQ_POISSON_BF16X3) {
-        poisson_solve_mfma(c, nf, lhs_planes);
-        return;
-    }
+// the FFT path of the staged solve; shift (device scalar or null) solves (Lap - shift) u =
+// rhs, add adds u to the destination planes
+static void solve_fft(hpq_ctx* c, int nf, double* const* lhs_planes, const double* shift = nullptr,
+                      bool add = false) {
     Poisson& S = c->ps;
     const int nx = S.nx, ny = S.ny;
     const long fs = long(nx) * ny;
@@ -329,9 +341,9 @@ void poisson_solve_staged(hpq_ctx* c, int nf, double* const* lhs_planes) {
     // y pass, eigenvalue scaling, inverse y pass on rows i (one launch on the FFT path;
     // the direct path leaves its result in bufA as well)
     if (S.py.fft)
-        dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufA, ny, nx), nf * nx, S.eigT, true);
+        dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufA, ny, nx), nf * nx, S.eigT, true, shift, S.inv_norm);
     else
-        dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufB, ny, nx), nf * nx, S.eigT, true);
+        dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufB, ny, nx), nf * nx, S.eigT, true, shift, S.inv_norm);
     hipLaunchKernelGGL(k_transpose, dim3((ny + 31) / 32, (nx + 31) / 32, nf), dim3(32, 8), 0, c->stream, S.bufA,
                        S.bufB, nx, ny);
     // inverse x pass straight into the destination planes
@@ -339,7 +351,17 @@ void poisson_solve_staged(hpq_ctx* c, int nf, double* const* lhs_planes) {
     for (int f = 0; f < 3; ++f) out.base[f] = lhs_planes[f < nf ? f : 0] + c->g.idx(0, 0);
     out.stride = c->g.ldx;
     out.rows_per_field = ny;
-    dst_rows(c, S.px, rows(S.bufB, nx, ny), out, nf * ny, nullptr);
+    dst_rows(c, S.px, rows(S.bufB, nx, ny), out, nf * ny, nullptr, false, nullptr, 0.0, add);
+}
+
+// Solves nfields staged right-hand sides held in ps.bufA as [f][j][i] and writes the
+// valid region of field f into lhs_planes[f] (guarded planes).
+void poisson_solve_staged(hpq_ctx* c, int nf, double* const* lhs_planes) {
+    if (c->p.poisson_solver == HPQ_POISSON_BF16X3) {
+        poisson_solve_mfma(c, nf, lhs_planes);
+        return;
+    }
+    solve_fft(c, nf, lhs_planes);
 }
 
 // where the staged right-hand sides go: FP64 [f][j][i] for the FFT path, FP32 padded rows
@@ -476,6 +498,92 @@ void exmby_eypbx(hpq_ctx* c, double* store) {
     hipLaunchKernelGGL(k_exmby_eypbx, dim3((g.nx + 2 + 127) / 128, g.ny + 2), dim3(128), 0, c->stream, g,
                        store + L.Psi * g.plane, store + L.ExmBy * g.plane, store + L.EypBx * g.plane);
     HPQ_HIP(hipGetLastError());
+}
+
+}  // namespace hpq
+
+// ---------------------------------------------------------------------------------------
+// DST-preconditioned initial guess of the explicit Bx/By multigrid (HPQ_MG_GUESS_DST)
+// ---------------------------------------------------------------------------------------
+namespace hpq {
+namespace {
+
+constexpr int CHI_BLOCKS = 120, CHI_T = 256;
+
+// mean of chi over the valid cells: per-workgroup partial sums, the last workgroup to
+// finish adds them in workgroup order (a fixed order) and resets the counter
+__global__ __launch_bounds__(CHI_T) void k_chi_mean(Geom g, const double* __restrict__ chi,
+                                                    double* __restrict__ part, unsigned* __restrict__ count,
+                                                    double* __restrict__ mean) {
+    __shared__ double red[CHI_T / 64];
+    __shared__ bool last;
+    double s = 0.0;
+    const long n = long(g.nx) * g.ny;
+    for (long t = long(blockIdx.x) * CHI_T + threadIdx.x; t < n; t += long(gridDim.x) * CHI_T) {
+        const int j = int(t / g.nx), i = int(t - long(j) * g.nx);
+        s += chi[g.idx(i, j)];
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int w = 0; w < CHI_T / 64; ++w) b += red[w];
+        part[blockIdx.x] = b;
+        __threadfence();
+        last = atomicAdd(count, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double tot = 0.0;
+        for (unsigned b = 0; b < gridDim.x; ++b) tot += __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *mean = tot / double(n);
+        *count = 0u;
+    }
+}
+
+// residual of the explicit Bx/By system, (Sy, Sx) - (Lap - chi) (Bx, By) on the valid
+// cells (the guard cells of Bx, By hold the homogeneous Dirichlet values), staged as the
+// right-hand sides [f][j][i] of the shifted DST solve
+__global__ void k_bxby_residual(Geom g, const double* __restrict__ bx, const double* __restrict__ sy,
+                                const double* __restrict__ chi, double* __restrict__ st) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, f = blockIdx.z;
+    if (i >= g.nx) return;
+    const double* b = bx + long(f) * g.plane;
+    const double* s = sy + long(f) * g.plane;
+    const long k = g.idx(i, j);
+    const double fx = 1. / (g.dx * g.dx), fy = 1. / (g.dy * g.dy);
+    const double c = b[k];
+    const double lap = fx * (b[k - 1] + b[k + 1] - 2. * c) + fy * (b[k - g.ldx] + b[k + g.ldx] - 2. * c);
+    st[long(f) * g.nx * g.ny + long(j) * g.nx + i] = s[k] - (lap - chi[k] * c);
+}
+
+}  // namespace
+
+void bxby_guess_dst(hpq_ctx* c, double* store, int iters) {
+    Poisson& S = c->ps;
+    const Geom& g = c->g;
+    const Layout& L = c->L;
+    if (!S.px.fft || !S.py.fft) throw HpqError(HPQ_E_UNSUPPORTED, "mg_guess DST: nx + 1 and ny + 1 must be powers of two");
+    if (L.By != L.Bx + 1 || L.Sx != L.Sy + 1) throw HpqError(HPQ_E_UNSUPPORTED, "mg_guess DST: Bx, By / Sy, Sx not adjacent");
+    if (!S.chi_part) {
+        HPQ_HIP(hipMalloc(&S.chi_part, sizeof(double) * (CHI_BLOCKS + 1)));
+        HPQ_HIP(hipMalloc(&S.chi_count, sizeof(unsigned)));
+        HPQ_HIP(hipMemsetAsync(S.chi_count, 0, sizeof(unsigned), c->stream));
+    }
+    double* chi = store + long(L.chi) * g.plane;
+    double* mean = S.chi_part + CHI_BLOCKS;
+    hipLaunchKernelGGL(k_chi_mean, dim3(CHI_BLOCKS), dim3(CHI_T), 0, c->stream, g, chi, S.chi_part, S.chi_count, mean);
+    HPQ_HIP(hipGetLastError());
+    double* bx = store + long(L.Bx) * g.plane;
+    double* const lhs[3] = {bx, bx + g.plane, nullptr};
+    for (int it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL(k_bxby_residual, dim3((g.nx + 127) / 128, g.ny, 2), dim3(128), 0, c->stream, g, bx,
+                           store + long(L.Sy) * g.plane, chi, S.bufA);
+        HPQ_HIP(hipGetLastError());
+        solve_fft(c, 2, lhs, mean, true);
+    }
 }
 
 }  // namespace hpq

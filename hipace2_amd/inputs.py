@@ -87,8 +87,11 @@ class SliceConfig:
     # "kahan32" (compensated FP32 sums: an FP32 sum word and an FP32 error word per cell)
     mixed_sums: str = "fp64"
     # the explicit solver's multigrid starts from "previous" (the previous slice's Bx, By, as
-    # the reference does) or "linear" (extrapolated from the two previous slices)
+    # the reference does), "linear" (extrapolated from the two previous slices) or "dst" (the
+    # previous slice's Bx, By corrected mg_guess_iters times by the shifted DST solve with
+    # chi replaced by its mean)
     mg_guess: str = "previous"
+    mg_guess_iters: int = 1
     # transverse Poisson solves: "fft" (DST-I by FFT, FP64) or "bf16x3" (DST-I as sine-matrix
     # GEMMs on the matrix cores, FP32 operands split into two BF16 terms each)
     poisson_solver: str = "fft"

@@ -45,6 +45,7 @@ def make_params(cfg: SliceConfig) -> _abi.Params:
     p.reorder_period = cfg.reorder_period
     p.mixed_sums = _abi.MIXED_SUMS[cfg.mixed_sums]
     p.mg_guess = _abi.MG_GUESS[cfg.mg_guess]
+    p.mg_guess_iters = cfg.mg_guess_iters
     return p
 
 

@@ -60,8 +60,12 @@ enum { HPQ_SUM_FP64 = 0, HPQ_SUM_KAHAN32 = 1 };
  * slice's solution, which the slice store still holds (the reference: Hipace.cpp:920 solves
  * in place). LINEAR: extrapolated from the two previous slices, 2 B(i+1) - B(i+2), as
  * Fields::InitialBfieldGuess (Fields.cpp:1150-1171) extrapolates for the predictor-corrector
- * loop; fewer V-cycles, results within the multigrid tolerance of PREVIOUS's. */
-enum { HPQ_MG_GUESS_PREVIOUS = 0, HPQ_MG_GUESS_LINEAR = 1 };
+ * loop. DST: the previous slice's solution corrected mg_guess_iters times by the exact
+ * solve of the same system with chi replaced by its mean over the slice,
+ * B += (Lap - <chi>)^-1 [S - (Lap - chi) B], each a shifted FFT DST solve (needs nx + 1,
+ * ny + 1 powers of two). LINEAR and DST change only where the multigrid starts: it still
+ * iterates to its tolerance, so results agree with PREVIOUS's within that tolerance. */
+enum { HPQ_MG_GUESS_PREVIOUS = 0, HPQ_MG_GUESS_LINEAR = 1, HPQ_MG_GUESS_DST = 2 };
 
 /* Transverse Dirichlet Poisson solver (fields.poisson_solver, src/fields/Fields.cpp:179-208):
  * DST-I by FFT in FP64, or DST-I as dense sine-matrix GEMMs on the matrix cores with each
@@ -111,7 +115,8 @@ typedef struct hpq_params {
                                             atomic deposition only: the ordered deposition keeps
                                             the index order, whose sums it reproduces. */
     int mixed_sums;                      /* HPQ_SUM_FP64 or HPQ_SUM_KAHAN32 (mixed path) */
-    int mg_guess;                        /* HPQ_MG_GUESS_PREVIOUS or HPQ_MG_GUESS_LINEAR */
+    int mg_guess;                        /* HPQ_MG_GUESS_PREVIOUS, _LINEAR or _DST */
+    int mg_guess_iters;                  /* HPQ_MG_GUESS_DST: DST corrections per slice (>= 1) */
 } hpq_params;
 
 /* Plasma species, SoA device arrays in PlasmaIdx order

@@ -101,6 +101,32 @@ def test_slices_match_oracle(n, order, explicit, peak, deposition, tol):
         assert sg["pc_iters"] == so["pc_iters"]
 
 
+@pytest.mark.parametrize("guess,iters", [("linear", 1), ("dst", 1), ("dst", 2)])
+def test_mg_initial_guess_keeps_the_physics(guess, iters):
+    """hpq_params.mg_guess: the explicit solver's multigrid starts from another initial guess
+    and still iterates to its tolerance. With a tight tolerance (1e-11) every slice's fields
+    agree with the run that starts from the previous slice's solution (the reference's
+    guess) to 1e-8, and the DST-corrected guess, which solves the system exactly when chi
+    is uniform, needs fewer V-cycles."""
+    from hipace2_amd.simulation import Simulation
+    runs = {}
+    for g, it in (("previous", 1), (guess, iters)):
+        cfg = small_config(n=63, nz=48, order=2, explicit=True)
+        cfg.deposition, cfg.mg_tol_rel, cfg.mg_guess, cfg.mg_guess_iters = "atomic", 1e-11, g, it
+        beam = _drive_beam(cfg, 3.0, 0.3, 1.41, 20000, 1000.0)
+        sim = Simulation(cfg, beam)
+        caps = {k: sim.capture(k) for k in ("Ez", "Bx", "By", "jx", "rhomjz")}
+        sim.run_step()
+        torch.cuda.synchronize()
+        runs[g] = ({k: v.cpu().numpy() for k, v in caps.items()}, sim.stats())
+    (f0, s0), (f1, s1) = runs["previous"], runs[guess]
+    for k in f0:
+        assert rel_err(f1[k], f0[k]) <= 1e-8, (k, rel_err(f1[k], f0[k]))
+    print(f"{guess} x{iters}: V-cycles {s1['mg_iters']} vs {s0['mg_iters']}")
+    if guess == "dst":
+        assert s1["mg_iters"] < s0["mg_iters"], (s1["mg_iters"], s0["mg_iters"])
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_reorder_period_keeps_the_physics(precision):
     """plasmas.reorder_period with atomic deposition: the plasma arrays are reordered in place by
