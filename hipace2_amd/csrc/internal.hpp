@@ -74,8 +74,7 @@ struct Poisson {
     DstPlan px, py;
     double* eigT = nullptr;   // [i][j] eigenvalue matrix with the RODFT00 normalisation
     double inv_norm = 0;      // 1 / the normalisation factor in eigT
-    double* chi_part = nullptr;       // HPQ_MG_GUESS_DST: partial sums of chi, then its mean
-    unsigned* chi_count = nullptr;
+    double* chi_part = nullptr;       // HPQ_MG_GUESS_DST: sums of chi over each row
     double* bufA = nullptr;   // 3 * nx * ny
     double* bufB = nullptr;
     // sine-matrix GEMM path (HPQ_POISSON_BF16X3): sizes padded to 64, S split into BF16
@@ -123,6 +122,7 @@ struct MultiGrid {
     bool coarse_in_lds = false;
     size_t coarse_lds_bytes = 0;
     int last_iters = 1;     // V-cycles of the previous solve: first batch size of the next
+    bool last_zero = false; // the previous solve converged without a V-cycle
     double dx = 0, dy = 0;
     std::vector<MGLevel> lev;
     alignas(16) unsigned char coarse_args[2048];   // kernel argument block of the coarse kernel

@@ -208,11 +208,16 @@ __global__ __launch_bounds__(NT) void k_check(Check chk) {
     if (threadIdx.x == 0) end_cycle(chk.ctl, chk.other, chk.mirror, chk.seq, chk.maxiter, norminf);
 }
 
+// the convergence target; with seq != 0 the result (done after zero V-cycles or not) is
+// published to the host as well
 __global__ __launch_bounds__(NT) void k_target(Check chk) {
     __shared__ double red[NT / 64];
     const double resnorm0 = max_norms(chk.norms, chk.nblk, 0, red);
     const double rhsnorm0 = max_norms(chk.norms, chk.nblk, 1, red);
-    if (threadIdx.x == 0) end_initial(chk.ctl, chk.other, chk.tol_rel, chk.tol_abs, resnorm0, rhsnorm0);
+    if (threadIdx.x == 0) {
+        end_initial(chk.ctl, chk.other, chk.tol_rel, chk.tol_abs, resnorm0, rhsnorm0);
+        if (chk.seq) publish(chk.ctl, chk.mirror, chk.seq);
+    }
 }
 
 // Convergence bookkeeping carried by the first launch of a V-cycle instead of a launch of
@@ -1116,18 +1121,23 @@ int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, lon
     S.tol_rel = tol_rel;
     S.tol_abs = tol_abs;
     // the convergence bookkeeping rides on the first launch of the next V-cycle when there
-    // is a tiled level 1 to carry it; the last cycle of every batch publishes with k_check
+    // is a tiled level 1 to carry it; the last cycle of every batch publishes with k_check.
+    // After a solve that needed no V-cycle (an initial guess within the tolerance), the
+    // target is evaluated and published first, so that no cycle is enqueued speculatively.
     const bool can_fold = M.coarse_begin > 1 && maxiter > 0;
-    if (!can_fold) {
-        const Check chk0{S.ctl, S.other, M.d_mirror, M.d_norms, level0_blocks(S.L0), 0, maxiter, tol_rel, tol_abs};
+    const bool probe = M.last_zero && maxiter > 0;
+    if (!can_fold || probe) {
+        const int seq = probe ? ++M.seq : 0;
+        const Check chk0{S.ctl, S.other, M.d_mirror, M.d_norms, level0_blocks(S.L0), seq, maxiter, tol_rel, tol_abs};
         hipLaunchKernelGGL(k_target, dim3(1), dim3(NT), 0, c->stream, chk0);
     }
     HPQ_HIP(hipGetLastError());
     int batch = std::max(1, std::min(M.last_iters, maxiter));
     int issued = 0;
-    bool checked = !can_fold;   // the previous cycle's end (or the target) has been evaluated
+    bool checked = !can_fold || probe;   // the previous cycle's end (or the target) has been evaluated
     Published pub{};
-    while (true) {
+    const bool done0 = probe && (pub = wait_mirror(c, M.seq)).done;
+    while (!done0) {
         for (int b = 0; b < batch && issued < maxiter; ++b, ++issued) {
             const bool last = b + 1 == batch || issued + 1 == maxiter;
             vcycle(c, S, issued, maxiter, !checked, last || !can_fold);
@@ -1141,7 +1151,7 @@ int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, lon
         batch = 1;
     }
     int it = 0, status = 0;
-    if (issued) {
+    if (issued || done0) {
         it = pub.iters;
         status = pub.status;
     } else {
@@ -1152,6 +1162,7 @@ int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, lon
         status = h.done ? HPQ_OK : HPQ_E_MG_NOT_CONVERGED;
     }
     M.last_iters = std::max(1, it);
+    M.last_zero = (issued || done0) && it == 0 && status == HPQ_OK;
     if (iters) *iters = it;
     // the result is buf[it & 1]; in the aliased layout buf[1] is sol itself
     if (!(alias && (it & 1))) {
@@ -1160,7 +1171,7 @@ int mg_solve1(hpq_ctx* c, double* sol, const double* rhs, const double* acf, lon
                            S.buf[it & 1]);
     }
     HPQ_HIP(hipGetLastError());
-    if (issued) {
+    if (issued || done0) {
         M.ctl_pick ^= 1;
         M.ctl_dirty = false;
     }

@@ -112,17 +112,34 @@ __device__ __forceinline__ void load_odd_extension(double2* a, int n, int M, int
 // RODFT00(a)_{k-1} = -Im F_k and RODFT00(b)_{k-1} = Re F_k), optionally times the
 // eigenvalue rows (scale != null). SYM: the result is transformed once more in LDS (the
 // inverse pass of the same direction), so the middle of the 2-D solve is one launch.
-// shift (device scalar, may be null): the scale entries s = f / lambda become
-// f / (lambda - shift) = s / (1 - shift s / f), f = 1 / inv_f, the solve of
-// (Lap - shift) u = rhs. add: the output is added to the destination.
+// shift (parts != null): the shift sigma is the sum of the n parts times inv_count, added
+// in a fixed order by every workgroup; the scale entries s = f / lambda become
+// f / (lambda - sigma) = s / (1 - sigma s / f), f = 1 / inv_f, the solve of
+// (Lap - sigma) u = rhs. add: the output is added to the destination.
+struct Shift {
+    const double* parts = nullptr;
+    int n = 0;
+    double inv_count = 0.0, inv_f = 0.0;
+};
+
 template <bool SYM>
 __global__ __launch_bounds__(DST_T) void k_dst_rows_fft(Rows in, Rows out, int nrows, int n, int M, int logM,
                                                         const double2* __restrict__ tw,
-                                                        const double* __restrict__ scale,
-                                                        const double* __restrict__ shift, double inv_f, int add) {
+                                                        const double* __restrict__ scale, Shift shift, int add) {
     extern __shared__ double2 a[];
-    const double sh = shift ? *shift : 0.0;
-    auto shifted = [&](double s) { return shift ? s / (1.0 - sh * s * inv_f) : s; };
+    __shared__ double sigma_s;
+    double sh = 0.0;
+    if (shift.parts) {
+        if (threadIdx.x < 64) {
+            double t = 0.0;
+            for (int q = threadIdx.x; q < shift.n; q += 64) t += shift.parts[q];
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+            if (threadIdx.x == 0) sigma_s = t * shift.inv_count;
+        }
+        __syncthreads();
+        sh = sigma_s;
+    }
+    auto shifted = [&](double s) { return shift.parts ? s / (1.0 - sh * s * shift.inv_f) : s; };
     const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
     const bool has1 = r1 < nrows;
     const double* p0 = row_ptr(in, r0);
@@ -227,18 +244,18 @@ __global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ in
 // RODFT00 of nrows rows (times the eigenvalue rows when scale != null); sym: followed by
 // the inverse transform of the same rows (fused in LDS on the FFT path)
 static void dst_rows(hpq_ctx* c, const DstPlan& P, const Rows& in, const Rows& out, int nrows, const double* scale,
-                     bool sym = false, const double* shift = nullptr, double inv_f = 0.0, bool add = false) {
+                     bool sym = false, const Shift& shift = Shift{}, bool add = false) {
     TraceScope ts(c, HPQ_TRACE_DST);
     if (P.fft) {
         const size_t lds = size_t(P.M) * sizeof(double2);
         if (sym)
             hipLaunchKernelGGL(k_dst_rows_fft<true>, dim3((nrows + 1) / 2), dim3(DST_T), lds, c->stream, in, out,
-                               nrows, P.n, P.M, P.logM, P.tw, scale, shift, inv_f, int(add));
+                               nrows, P.n, P.M, P.logM, P.tw, scale, shift, int(add));
         else
             hipLaunchKernelGGL(k_dst_rows_fft<false>, dim3((nrows + 1) / 2), dim3(DST_T), lds, c->stream, in, out,
-                               nrows, P.n, P.M, P.logM, P.tw, scale, shift, inv_f, int(add));
+                               nrows, P.n, P.M, P.logM, P.tw, scale, shift, int(add));
     } else {
-        if (shift || add) throw HpqError(HPQ_E_UNSUPPORTED, "shifted DST solve: n + 1 must be a power of two");
+        if (shift.parts || add) throw HpqError(HPQ_E_UNSUPPORTED, "shifted DST solve: n + 1 must be a power of two");
         const size_t lds = size_t(P.n) * sizeof(double);
         hipLaunchKernelGGL(k_dst_rows_direct, dim3(nrows), dim3(256), lds, c->stream, in, out, P.n, P.sintab, scale);
         if (sym) {
@@ -312,7 +329,7 @@ void poisson_init(hpq_ctx* c) {
 void poisson_free(hpq_ctx* c) {
     Poisson& S = c->ps;
     for (void* p : {(void*)S.px.tw, (void*)S.px.sintab, (void*)S.py.tw, (void*)S.py.sintab, (void*)S.eigT,
-                    (void*)S.bufA, (void*)S.bufB, (void*)S.chi_part, (void*)S.chi_count})
+                    (void*)S.bufA, (void*)S.bufB, (void*)S.chi_part})
         if (p) (void)hipFree(p);
     dst_mfma_free(c);
     S = Poisson{};
@@ -320,10 +337,9 @@ void poisson_free(hpq_ctx* c) {
 
 // Solves nfields staged right-hand sides held in ps.bufA as [f][j][i] and writes the
 // valid region of field f into lhs_planes[f] (guarded planes).
-// the FFT path of the staged solve; shift (device scalar or null) solves (Lap - shift) u =
-// rhs, add adds u to the destination planes
-static void solve_fft(hpq_ctx* c, int nf, double* const* lhs_planes, const double* shift = nullptr,
-                      bool add = false) {
+// the FFT path of the staged solve; shift (parts of the shift, or none) solves
+// (Lap - shift) u = rhs, add adds u to the destination planes
+static void solve_fft(hpq_ctx* c, int nf, double* const* lhs_planes, Shift shift = Shift{}, bool add = false) {
     Poisson& S = c->ps;
     const int nx = S.nx, ny = S.ny;
     const long fs = long(nx) * ny;
@@ -341,9 +357,9 @@ static void solve_fft(hpq_ctx* c, int nf, double* const* lhs_planes, const doubl
     // y pass, eigenvalue scaling, inverse y pass on rows i (one launch on the FFT path;
     // the direct path leaves its result in bufA as well)
     if (S.py.fft)
-        dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufA, ny, nx), nf * nx, S.eigT, true, shift, S.inv_norm);
+        dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufA, ny, nx), nf * nx, S.eigT, true, shift);
     else
-        dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufB, ny, nx), nf * nx, S.eigT, true, shift, S.inv_norm);
+        dst_rows(c, S.py, rows(S.bufA, ny, nx), rows(S.bufB, ny, nx), nf * nx, S.eigT, true, shift);
     hipLaunchKernelGGL(k_transpose, dim3((ny + 31) / 32, (nx + 31) / 32, nf), dim3(32, 8), 0, c->stream, S.bufA,
                        S.bufB, nx, ny);
     // inverse x pass straight into the destination planes
@@ -351,7 +367,7 @@ static void solve_fft(hpq_ctx* c, int nf, double* const* lhs_planes, const doubl
     for (int f = 0; f < 3; ++f) out.base[f] = lhs_planes[f < nf ? f : 0] + c->g.idx(0, 0);
     out.stride = c->g.ldx;
     out.rows_per_field = ny;
-    dst_rows(c, S.px, rows(S.bufB, nx, ny), out, nf * ny, nullptr, false, nullptr, 0.0, add);
+    dst_rows(c, S.px, rows(S.bufB, nx, ny), out, nf * ny, nullptr, false, Shift{}, add);
 }
 
 // Solves nfields staged right-hand sides held in ps.bufA as [f][j][i] and writes the
@@ -508,55 +524,39 @@ void exmby_eypbx(hpq_ctx* c, double* store) {
 namespace hpq {
 namespace {
 
-constexpr int CHI_BLOCKS = 120, CHI_T = 256;
-
-// mean of chi over the valid cells: per-workgroup partial sums, the last workgroup to
-// finish adds them in workgroup order (a fixed order) and resets the counter
-__global__ __launch_bounds__(CHI_T) void k_chi_mean(Geom g, const double* __restrict__ chi,
-                                                    double* __restrict__ part, unsigned* __restrict__ count,
-                                                    double* __restrict__ mean) {
-    __shared__ double red[CHI_T / 64];
-    __shared__ bool last;
-    double s = 0.0;
-    const long n = long(g.nx) * g.ny;
-    for (long t = long(blockIdx.x) * CHI_T + threadIdx.x; t < n; t += long(gridDim.x) * CHI_T) {
-        const int j = int(t / g.nx), i = int(t - long(j) * g.nx);
-        s += chi[g.idx(i, j)];
-    }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double b = 0.0;
-        for (int w = 0; w < CHI_T / 64; ++w) b += red[w];
-        part[blockIdx.x] = b;
-        __threadfence();
-        last = atomicAdd(count, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        double tot = 0.0;
-        for (unsigned b = 0; b < gridDim.x; ++b) tot += __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        *mean = tot / double(n);
-        *count = 0u;
-    }
-}
+constexpr int RES_T = 256;   // threads of the residual kernel, one row per workgroup
 
 // residual of the explicit Bx/By system, (Sy, Sx) - (Lap - chi) (Bx, By) on the valid
 // cells (the guard cells of Bx, By hold the homogeneous Dirichlet values), staged as the
-// right-hand sides [f][j][i] of the shifted DST solve
-__global__ void k_bxby_residual(Geom g, const double* __restrict__ bx, const double* __restrict__ sy,
-                                const double* __restrict__ chi, double* __restrict__ st) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, f = blockIdx.z;
-    if (i >= g.nx) return;
+// right-hand sides [f][j][i] of the shifted DST solve. The workgroups of component 0 also
+// write the sum of chi over their row (a fixed order) to part[j]; the shifted solve adds
+// the rows' sums.
+__global__ __launch_bounds__(RES_T) void k_bxby_residual(Geom g, const double* __restrict__ bx,
+                                                         const double* __restrict__ sy,
+                                                         const double* __restrict__ chi, double* __restrict__ st,
+                                                         double* __restrict__ part) {
+    __shared__ double red[RES_T / 64];
+    const int j = blockIdx.x, f = blockIdx.y;
     const double* b = bx + long(f) * g.plane;
     const double* s = sy + long(f) * g.plane;
-    const long k = g.idx(i, j);
     const double fx = 1. / (g.dx * g.dx), fy = 1. / (g.dy * g.dy);
-    const double c = b[k];
-    const double lap = fx * (b[k - 1] + b[k + 1] - 2. * c) + fy * (b[k - g.ldx] + b[k + g.ldx] - 2. * c);
-    st[long(f) * g.nx * g.ny + long(j) * g.nx + i] = s[k] - (lap - chi[k] * c);
+    double cs = 0.0;
+    for (int i = threadIdx.x; i < g.nx; i += RES_T) {
+        const long k = g.idx(i, j);
+        const double c = b[k], cv = chi[k];
+        cs += cv;
+        const double lap = fx * (b[k - 1] + b[k + 1] - 2. * c) + fy * (b[k - g.ldx] + b[k + g.ldx] - 2. * c);
+        st[long(f) * g.nx * g.ny + long(j) * g.nx + i] = s[k] - (lap - cv * c);
+    }
+    if (f != 0) return;
+    for (int o = 32; o > 0; o >>= 1) cs += __shfl_xor(cs, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cs;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int w = 1; w < RES_T / 64; ++w) t += red[w];
+        part[j] = t;
+    }
 }
 
 }  // namespace
@@ -567,22 +567,20 @@ void bxby_guess_dst(hpq_ctx* c, double* store, int iters) {
     const Layout& L = c->L;
     if (!S.px.fft || !S.py.fft) throw HpqError(HPQ_E_UNSUPPORTED, "mg_guess DST: nx + 1 and ny + 1 must be powers of two");
     if (L.By != L.Bx + 1 || L.Sx != L.Sy + 1) throw HpqError(HPQ_E_UNSUPPORTED, "mg_guess DST: Bx, By / Sy, Sx not adjacent");
-    if (!S.chi_part) {
-        HPQ_HIP(hipMalloc(&S.chi_part, sizeof(double) * (CHI_BLOCKS + 1)));
-        HPQ_HIP(hipMalloc(&S.chi_count, sizeof(unsigned)));
-        HPQ_HIP(hipMemsetAsync(S.chi_count, 0, sizeof(unsigned), c->stream));
-    }
-    double* chi = store + long(L.chi) * g.plane;
-    double* mean = S.chi_part + CHI_BLOCKS;
-    hipLaunchKernelGGL(k_chi_mean, dim3(CHI_BLOCKS), dim3(CHI_T), 0, c->stream, g, chi, S.chi_part, S.chi_count, mean);
-    HPQ_HIP(hipGetLastError());
+    if (!S.chi_part) HPQ_HIP(hipMalloc(&S.chi_part, sizeof(double) * g.ny));
+    const double* chi = store + long(L.chi) * g.plane;
     double* bx = store + long(L.Bx) * g.plane;
     double* const lhs[3] = {bx, bx + g.plane, nullptr};
+    Shift sh;
+    sh.parts = S.chi_part;
+    sh.n = g.ny;
+    sh.inv_count = 1.0 / (double(g.nx) * g.ny);
+    sh.inv_f = S.inv_norm;
     for (int it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL(k_bxby_residual, dim3((g.nx + 127) / 128, g.ny, 2), dim3(128), 0, c->stream, g, bx,
-                           store + long(L.Sy) * g.plane, chi, S.bufA);
+        hipLaunchKernelGGL(k_bxby_residual, dim3(g.ny, 2), dim3(RES_T), 0, c->stream, g, bx,
+                           store + long(L.Sy) * g.plane, chi, S.bufA, S.chi_part);
         HPQ_HIP(hipGetLastError());
-        solve_fft(c, 2, lhs, mean, true);
+        solve_fft(c, 2, lhs, sh, true);
     }
 }
 
