@@ -246,6 +246,10 @@ def run_adjoint(args):
     from hipace2_amd.simulation import Simulation
 
     large = args.workload == "large"
+
+    def note(msg):   # progress of the long phases, on stderr
+        print(f"adjoint: {msg}", file=sys.stderr, flush=True)
+
     cfg, bl = workloads.config4(large=large)
     n, nz = cfg.nx, cfg.nz
     P = workloads.LARGE
@@ -277,6 +281,7 @@ def run_adjoint(args):
     t0 = time.perf_counter()
     plain(knots0)
     t_plain = time.perf_counter() - t0
+    note(f"plain forward {t_plain:.3f} s")
     # the same forward step at the production multigrid tolerance (hipace.MG_tolerance_rel
     # default 1e-4), the other denominator of the cost ratio
     sim_p = Simulation(dataclasses.replace(cfg, mg_tol_rel=1e-4), bl.beam)
@@ -289,6 +294,7 @@ def run_adjoint(args):
         torch.cuda.synchronize()
         t_plain_prod = time.perf_counter() - t0
     del sim_p
+    note(f"forward at the production multigrid tolerance {t_plain_prod:.3f} s")
 
     knots = knots0.clone().requires_grad_(True)
     q = torch.ones((), dtype=torch.float64, device="cuda", requires_grad=True)
@@ -304,6 +310,7 @@ def run_adjoint(args):
     t_bwd = time.perf_counter() - t0
     grad = knots.grad.detach().cpu().numpy()
     grad_q = float(q.grad)
+    note(f"checkpointed forward {t_fwd:.3f} s, backward {t_bwd:.3f} s")
 
     # the same gradient with the forward (and its recomputation) on the mixed path: FP32
     # particle arithmetic and split-BF16 DST; the vector-Jacobian products stay FP64
@@ -318,6 +325,7 @@ def run_adjoint(args):
     grad_m = knots_m.grad.detach().cpu().numpy()
     mixed_err = float(np.abs(grad_m - grad).max() / np.abs(grad).max())
     del sim_m
+    note(f"mixed-path forward + backward {t_mixed:.3f} s")
 
     picks = np.linspace(0, 63, args.adjoint_fd_knots).round().astype(int) if args.adjoint_fd_knots else []
     fd = []
@@ -326,6 +334,7 @@ def run_adjoint(args):
         e = torch.zeros_like(knots0)
         e[k] = h
         fd.append((plain(knots0 + e) - plain(knots0 - e)) / (2 * h))
+        note(f"central difference of knot {k} done")
     fd = np.array(fd)
     g = grad[picks] if len(picks) else np.zeros(0)
     scale = float(np.abs(fd).max()) if fd.size else 0.0
@@ -338,6 +347,7 @@ def run_adjoint(args):
         one = torch.ones((), dtype=torch.float64, device="cuda")
         for hq in (1e-3, 1e-4, 1e-5):
             fd_q[str(hq)] = (plain(knots0, one + hq) - plain(knots0, one - hq)) / (2 * hq)
+            note(f"central difference of the drive charge, step {hq:g}, done")
         ref = fd_q[str(1e-5)]
         err_q = abs(grad_q - ref) / abs(ref) if ref else None
     print(f"adjoint: plain {t_plain:.3f} s, checkpointed forward {t_fwd:.3f} s, backward {t_bwd:.3f} s, "
