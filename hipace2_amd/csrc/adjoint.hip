@@ -7,8 +7,12 @@
 // outputs are the particle's share of the product. Field values enter linearly, so their
 // adjoints are scattered to the cells with the forward weights. Branches (QSA removal,
 // boundary conditions, shape-factor pieces) follow the values and contribute no derivative.
-// Plain FP64 arithmetic throughout; one thread per particle in index order; field adjoints
-// are added to the adjoint planes with FP64 atomics.
+// Plain FP64 arithmetic throughout; one thread per particle in index order. The field
+// adjoints of a workgroup's particles are summed in an LDS window over the bounding box of
+// their stencils (consecutive plasma particles sit in neighbouring cells) and the window is
+// added to the adjoint planes with one FP64 atomic per touched cell; a workgroup whose box
+// does not fit adds every contribution with a global atomic.
+#include <climits>
 #include <cmath>
 
 #include "internal.hpp"
@@ -26,6 +30,59 @@ __device__ __forceinline__ bool in_store(const Geom& g, int i, int j) {
     return i >= -g.ng && i <= g.nx - 1 + g.ng && j >= -g.ng && j <= g.ny - 1 + g.ng;
 }
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void lds_add(double* p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+constexpr int VJ_T = 256;      // threads (particles) per workgroup of the scattering VJPs
+constexpr int VJ_WIN = 1280;   // window cells per adjoint plane
+
+// LDS window of cells [i0, i0 + w) x [j0, j0 + h) holding np adjoint planes; w == 0: none
+struct AdjWin {
+    int i0 = 0, j0 = 0, w = 0, h = 0;
+    __device__ int cells() const { return w * h; }
+};
+
+// the bounding box of the S x S stencils starting at (ci, cj) of the workgroup's active
+// particles, when it fits in cap cells (every thread of the workgroup calls this)
+__device__ AdjWin adj_window(int ci, int cj, bool act, int S, int cap, int* red) {
+    int mni = act ? ci : INT_MAX, mxi = act ? ci : INT_MIN, mnj = act ? cj : INT_MAX, mxj = act ? cj : INT_MIN;
+    for (int o = 32; o > 0; o >>= 1) {
+        mni = min(mni, __shfl_xor(mni, o)); mxi = max(mxi, __shfl_xor(mxi, o));
+        mnj = min(mnj, __shfl_xor(mnj, o)); mxj = max(mxj, __shfl_xor(mxj, o));
+    }
+    constexpr int NW = VJ_T / 64;
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wv] = mni; red[NW + wv] = mxi; red[2 * NW + wv] = mnj; red[3 * NW + wv] = mxj;
+    }
+    __syncthreads();
+    mni = red[0]; mxi = red[NW]; mnj = red[2 * NW]; mxj = red[3 * NW];
+    for (int w = 1; w < NW; ++w) {
+        mni = min(mni, red[w]); mxi = max(mxi, red[NW + w]);
+        mnj = min(mnj, red[2 * NW + w]); mxj = max(mxj, red[3 * NW + w]);
+    }
+    AdjWin W;
+    if (mni > mxi) return W;
+    const long bw = long(mxi) - mni + S, bh = long(mxj) - mnj + S;
+    if (bw * bh > cap) return W;
+    W.i0 = mni; W.j0 = mnj; W.w = int(bw); W.h = int(bh);
+    return W;
+}
+
+// adds the window's np planes to the adjoint planes comp[0..np) with one atomic per nonzero
+// cell (every thread of the workgroup calls this, after the accumulation's barrier)
+template <int NP>
+__device__ void adj_flush(const Geom& g, const AdjWin& W, const double* win, double* adj, const int* comp) {
+    const int cells = W.cells();
+    for (int t = threadIdx.x; t < NP * cells; t += VJ_T) {
+        const double v = win[t];
+        if (v == 0.0) continue;
+        const int q = t / cells, c = t - q * cells;
+        const int j = W.j0 + c / W.w, i = W.i0 + c % W.w;
+        atomic_add(adj + long(comp[q]) * g.plane + g.idx(i, j), v);
+    }
+}
 
 // ---- ::DepositCurrent (PlasmaDepositCurrent.cpp:182-251) ----------------------------------
 // inputs seeded: x, y, ux, uy, psi
@@ -83,73 +140,94 @@ __global__ __launch_bounds__(VT) void k_deposit_vjp(Geom g, hpq_plasma p, hpq_pl
 // momentum parts and its x and y parts. The terms dropped are the exact zeros a five-direction
 // product would add, so the sums (taken in the same order) are those of the full tangents.
 template <int O, int TD>
-__global__ __launch_bounds__(VT) void k_explicit_vjp(Geom g, hpq_plasma p, hpq_plasma_adj a, const double* __restrict__ store,
-                                                     double* __restrict__ adj, Layout L, double charge_invvol_mu0,
-                                                     double charge_mass_ratio) {
+__global__ __launch_bounds__(VJ_T) void k_explicit_vjp(Geom g, hpq_plasma p, hpq_plasma_adj a,
+                                                       const double* __restrict__ store, double* __restrict__ adj,
+                                                       Layout L, double charge_invvol_mu0, double charge_mass_ratio) {
     using U = Tan<3>;   // directions ux, uy, psi
     using X1 = Tan<1>;  // direction x (or y)
     constexpr int S = dshape_width<TD, O>();
-    const long ip = long(blockIdx.x) * VT + threadIdx.x;
-    if (ip >= p.n || !p.valid[ip]) return;
-    const double clight = 1.0, clight_inv = 1.0 / 1.0;
-    const U ux = U::seed(p.ux[ip], 0), uy = U::seed(p.uy[ip], 1), psi = U::seed(p.psi[ip], 2);
-    const U pinv = 1.0 / psi;
-    const U pvx = ux * pinv * clight_inv, pvy = uy * pinv * clight_inv;
-    const double cdm = charge_invvol_mu0 * p.w[ip];
-    const X1 xmid = (X1::seed(p.x[ip], 0) - g.xoff) * g.dxi, ymid = (X1::seed(p.y[ip], 0) - g.yoff) * g.dyi;
-    const double Aabssqp = 0.0;
-    const U gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * pinv * pinv + pvx * pvx + pvy * pvy + 1.0);
-    const U vxvy = pvx * pvy;
-    const U gpx = gamma_psi - pvx * pvx, gpy = gamma_psi - pvy * pvy;
-    const U qp = charge_mass_ratio * pinv;
-    const U ay = -cdm * qp, by = cdm * clight * g.dxi * vxvy, cy = -cdm * clight * g.dyi * (gpy - 1.0);
-    const U ax = cdm * qp, bx = cdm * clight * g.dxi * (gpx - 1.0), cx = -cdm * clight * g.dyi * vxvy;
-    const double* Bz = store + long(L.Bz) * g.plane;
-    const double* Ez = store + long(L.Ez) * g.plane;
-    const double* ExmBy = store + long(L.ExmBy) * g.plane;
-    const double* EypBx = store + long(L.EypBx) * g.plane;
-    const double* aSy = adj + long(L.Sy) * g.plane;
-    const double* aSx = adj + long(L.Sx) * g.plane;
-    // accumulated derivative parts: x, y, then ux, uy, psi
-    double acc_x = 0.0, acc_y = 0.0, acc_u[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int iy = 0; iy < S; ++iy) {
-        const DsfR<X1> Y = dshape<TD, O, X1, X1>(ymid, iy);
-#pragma unroll
-        for (int ix = 0; ix < S; ++ix) {
-            if (TD == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) continue;
-            const DsfR<X1> X = dshape<TD, O, X1, X1>(xmid, ix);
-            if (!in_store(g, X.cell, Y.cell)) continue;
-            const long k = g.idx(X.cell, Y.cell);
-            const double sy_a = aSy[k], sx_a = aSx[k];
-            if (sy_a == 0.0 && sx_a == 0.0) continue;
-            const U A = -Bz[k] * pvx + (EypBx[k] * gpy - ExmBy[k] * vxvy + Ez[k] * pvy) * clight_inv;
-            const U B = Bz[k] * pvy + (-EypBx[k] * vxvy + ExmBy[k] * gpx + Ez[k] * pvx) * clight_inv;
-            // ss = X.s Y.s, ds = X.ds Y.s, sd = X.s Y.ds: values and x, y parts
-            const double ss = X.s.v * Y.s.v, ds = X.ds.v * Y.s.v, sd = X.s.v * Y.ds.v;
-            const double ss_x = X.s.d[0] * Y.s.v, ds_x = X.ds.d[0] * Y.s.v, sd_x = X.s.d[0] * Y.ds.v;
-            const double ss_y = X.s.v * Y.s.d[0], ds_y = X.ds.v * Y.s.d[0], sd_y = X.s.v * Y.ds.d[0];
-            const U ayA = ay * A, axB = ax * B;
-            // sy_a (ayA ss + by ds + cy sd) + sx_a (axB ss + bx ds + cx sd), part by part
-            acc_x += sy_a * (ayA.v * ss_x + by.v * ds_x + cy.v * sd_x) + sx_a * (axB.v * ss_x + bx.v * ds_x + cx.v * sd_x);
-            acc_y += sy_a * (ayA.v * ss_y + by.v * ds_y + cy.v * sd_y) + sx_a * (axB.v * ss_y + bx.v * ds_y + cx.v * sd_y);
-#pragma unroll
-            for (int q = 0; q < 3; ++q)
-                acc_u[q] += sy_a * (ayA.d[q] * ss + by.d[q] * ds + cy.d[q] * sd) +
-                            sx_a * (axB.d[q] * ss + bx.d[q] * ds + cx.d[q] * sd);
-            // the contributions are linear in the four fields of cell k
-            const double wy = sy_a * ay.v * ss, wx = sx_a * ax.v * ss;
-            atomic_add(adj + long(L.Bz) * g.plane + k, -wy * pvx.v + wx * pvy.v);
-            atomic_add(adj + long(L.Ez) * g.plane + k, (wy * pvy.v + wx * pvx.v) * clight_inv);
-            atomic_add(adj + long(L.ExmBy) * g.plane + k, (-wy * vxvy.v + wx * gpx.v) * clight_inv);
-            atomic_add(adj + long(L.EypBx) * g.plane + k, (wy * gpy.v - wx * vxvy.v) * clight_inv);
-        }
+    constexpr int NP = 4;   // adjoint planes: Bz, Ez, ExmBy, EypBx
+    __shared__ double win[NP * VJ_WIN];
+    __shared__ int red[4 * VJ_T / 64];
+    const long ip = long(blockIdx.x) * VJ_T + threadIdx.x;
+    const bool act = ip < p.n && p.valid[ip];
+    int ci = 0, cj = 0;
+    if (act) {
+        ci = dshape<TD, O>((p.x[ip] - g.xoff) * g.dxi, 0).cell;
+        cj = dshape<TD, O>((p.y[ip] - g.yoff) * g.dyi, 0).cell;
     }
-    a.x[ip] += acc_x;
-    a.y[ip] += acc_y;
-    a.ux[ip] += acc_u[0];
-    a.uy[ip] += acc_u[1];
-    a.psi[ip] += acc_u[2];
+    const AdjWin W = adj_window(ci, cj, act, S, VJ_WIN, red);
+    const int cells = W.cells();
+    for (int t = threadIdx.x; t < NP * cells; t += VJ_T) win[t] = 0.0;
+    __syncthreads();
+    const int comp[NP] = {L.Bz, L.Ez, L.ExmBy, L.EypBx};
+    auto scatter = [&](int q, int i, int j, double v) {
+        if (W.w) lds_add(&win[q * cells + (j - W.j0) * W.w + (i - W.i0)], v);
+        else atomic_add(adj + long(comp[q]) * g.plane + g.idx(i, j), v);
+    };
+    if (act) {
+        const double clight = 1.0, clight_inv = 1.0 / 1.0;
+        const U ux = U::seed(p.ux[ip], 0), uy = U::seed(p.uy[ip], 1), psi = U::seed(p.psi[ip], 2);
+        const U pinv = 1.0 / psi;
+        const U pvx = ux * pinv * clight_inv, pvy = uy * pinv * clight_inv;
+        const double cdm = charge_invvol_mu0 * p.w[ip];
+        const X1 xmid = (X1::seed(p.x[ip], 0) - g.xoff) * g.dxi, ymid = (X1::seed(p.y[ip], 0) - g.yoff) * g.dyi;
+        const double Aabssqp = 0.0;
+        const U gamma_psi = 0.5 * ((1.0 + 0.5 * Aabssqp) * pinv * pinv + pvx * pvx + pvy * pvy + 1.0);
+        const U vxvy = pvx * pvy;
+        const U gpx = gamma_psi - pvx * pvx, gpy = gamma_psi - pvy * pvy;
+        const U qp = charge_mass_ratio * pinv;
+        const U ay = -cdm * qp, by = cdm * clight * g.dxi * vxvy, cy = -cdm * clight * g.dyi * (gpy - 1.0);
+        const U ax = cdm * qp, bx = cdm * clight * g.dxi * (gpx - 1.0), cx = -cdm * clight * g.dyi * vxvy;
+        const double* Bz = store + long(L.Bz) * g.plane;
+        const double* Ez = store + long(L.Ez) * g.plane;
+        const double* ExmBy = store + long(L.ExmBy) * g.plane;
+        const double* EypBx = store + long(L.EypBx) * g.plane;
+        const double* aSy = adj + long(L.Sy) * g.plane;
+        const double* aSx = adj + long(L.Sx) * g.plane;
+        // accumulated derivative parts: x, y, then ux, uy, psi
+        double acc_x = 0.0, acc_y = 0.0, acc_u[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int iy = 0; iy < S; ++iy) {
+            const DsfR<X1> Y = dshape<TD, O, X1, X1>(ymid, iy);
+#pragma unroll
+            for (int ix = 0; ix < S; ++ix) {
+                if (TD == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) continue;
+                const DsfR<X1> X = dshape<TD, O, X1, X1>(xmid, ix);
+                if (!in_store(g, X.cell, Y.cell)) continue;
+                const long k = g.idx(X.cell, Y.cell);
+                const double sy_a = aSy[k], sx_a = aSx[k];
+                if (sy_a == 0.0 && sx_a == 0.0) continue;
+                const U A = -Bz[k] * pvx + (EypBx[k] * gpy - ExmBy[k] * vxvy + Ez[k] * pvy) * clight_inv;
+                const U B = Bz[k] * pvy + (-EypBx[k] * vxvy + ExmBy[k] * gpx + Ez[k] * pvx) * clight_inv;
+                // ss = X.s Y.s, ds = X.ds Y.s, sd = X.s Y.ds: values and x, y parts
+                const double ss = X.s.v * Y.s.v, ds = X.ds.v * Y.s.v, sd = X.s.v * Y.ds.v;
+                const double ss_x = X.s.d[0] * Y.s.v, ds_x = X.ds.d[0] * Y.s.v, sd_x = X.s.d[0] * Y.ds.v;
+                const double ss_y = X.s.v * Y.s.d[0], ds_y = X.ds.v * Y.s.d[0], sd_y = X.s.v * Y.ds.d[0];
+                const U ayA = ay * A, axB = ax * B;
+                // sy_a (ayA ss + by ds + cy sd) + sx_a (axB ss + bx ds + cx sd), part by part
+                acc_x += sy_a * (ayA.v * ss_x + by.v * ds_x + cy.v * sd_x) + sx_a * (axB.v * ss_x + bx.v * ds_x + cx.v * sd_x);
+                acc_y += sy_a * (ayA.v * ss_y + by.v * ds_y + cy.v * sd_y) + sx_a * (axB.v * ss_y + bx.v * ds_y + cx.v * sd_y);
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    acc_u[q] += sy_a * (ayA.d[q] * ss + by.d[q] * ds + cy.d[q] * sd) +
+                                sx_a * (axB.d[q] * ss + bx.d[q] * ds + cx.d[q] * sd);
+                // the contributions are linear in the four fields of cell k
+                const double wy = sy_a * ay.v * ss, wx = sx_a * ax.v * ss;
+                scatter(0, X.cell, Y.cell, -wy * pvx.v + wx * pvy.v);
+                scatter(1, X.cell, Y.cell, (wy * pvy.v + wx * pvx.v) * clight_inv);
+                scatter(2, X.cell, Y.cell, (-wy * vxvy.v + wx * gpx.v) * clight_inv);
+                scatter(3, X.cell, Y.cell, (wy * gpy.v - wx * vxvy.v) * clight_inv);
+            }
+        }
+        a.x[ip] += acc_x;
+        a.y[ip] += acc_y;
+        a.ux[ip] += acc_u[0];
+        a.uy[ip] += acc_u[1];
+        a.psi[ip] += acc_u[2];
+    }
+    __syncthreads();
+    if (W.w) adj_flush<NP>(g, W, win, adj, comp);
 }
 
 // ---- AdvancePlasmaParticles (PlasmaParticleAdvance.cpp:84-240), not a temporary slice -----
@@ -315,63 +393,84 @@ __global__ __launch_bounds__(VT) void k_push_vjp(Geom g, hpq_plasma p, const dou
 // through the shape factors: scatter the G adjoints with the forward's weights and add
 // their position part to the x_prev, y_prev adjoints (B*c scaled by c).
 template <int O>
-__global__ __launch_bounds__(VT) void k_push_vjp_gather(Geom g, hpq_plasma p, const double* __restrict__ store,
-                                                        Layout L, const double* __restrict__ gadj, hpq_plasma_adj in,
-                                                        double* __restrict__ adj) {
+__global__ __launch_bounds__(VJ_T) void k_push_vjp_gather(Geom g, hpq_plasma p, const double* __restrict__ store,
+                                                          Layout L, const double* __restrict__ gadj,
+                                                          hpq_plasma_adj in, double* __restrict__ adj) {
     // the shape factors in x carry the x tangent only, those in y the y tangent only; the
     // products below are expanded by hand into value, x and y parts (the dropped terms are the
     // exact zeros of two-direction products, the sums keep their order)
     using X1 = Tan<1>;
-    const long ip = long(blockIdx.x) * VT + threadIdx.x;
-    if (ip >= p.n || !p.valid[ip]) return;
-    double lg[6];
-    bool any = false;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        lg[q] = gadj[long(q) * p.n + ip];
-        any = any || lg[q] != 0.0;
-    }
-    if (!any) return;
-    const double clight = 1.0;
-    const X1 xm = (X1::seed(p.x_prev[ip], 0) - g.xoff) * g.dxi, ym = (X1::seed(p.y_prev[ip], 0) - g.yoff) * g.dyi;
     constexpr int W = O + 2;
-    DsfR<X1> sx[W], sy[W];
+    constexpr int NP = 5;   // adjoint planes: Psi, Ez, Bx, By, Bz
+    __shared__ double win[NP * VJ_WIN];
+    __shared__ int red[4 * VJ_T / 64];
+    const long ip = long(blockIdx.x) * VJ_T + threadIdx.x;
+    double lg[6] = {0, 0, 0, 0, 0, 0};
+    bool act = ip < p.n && p.valid[ip];
+    if (act) {
+        bool any = false;
 #pragma unroll
-    for (int k = 0; k < W; ++k) {
-        sx[k] = dshape<1, O, X1, X1>(xm, k);
-        sy[k] = dshape<1, O, X1, X1>(ym, k);
-    }
-    const double* F[5] = {store + long(L.Psi) * g.plane, store + long(L.Ez) * g.plane, store + long(L.Bx) * g.plane,
-                          store + long(L.By) * g.plane, store + long(L.Bz) * g.plane};
-    double acc_x = 0.0, acc_y = 0.0;
-#pragma unroll
-    for (int iy = 0; iy < W; ++iy)
-#pragma unroll
-        for (int ix = 0; ix < W; ++ix) {
-            const DsfR<X1>& X = sx[ix];
-            const DsfR<X1>& Y = sy[iy];
-            if (!in_store(g, X.cell, Y.cell)) continue;
-            const long k = g.idx(X.cell, Y.cell);
-            const double ss = X.s.v * Y.s.v, ss_x = X.s.d[0] * Y.s.v, ss_y = X.s.v * Y.s.d[0];
-            const double ds = X.ds.v * Y.s.v, ds_x = X.ds.d[0] * Y.s.v, ds_y = X.ds.v * Y.s.d[0];
-            const double sd = X.s.v * Y.ds.v, sd_x = X.s.d[0] * Y.ds.v, sd_y = X.s.v * Y.ds.d[0];
-            // wpsi = lg0 (X.ds Y.s) dxi + lg1 (X.s Y.ds) dyi; wez, wbx, wby, wbz = lg * ss
-            const double wpsi = lg[0] * ds * g.dxi + lg[1] * sd * g.dyi;
-            const double wpsi_x = lg[0] * ds_x * g.dxi + lg[1] * sd_x * g.dyi;
-            const double wpsi_y = lg[0] * ds_y * g.dxi + lg[1] * sd_y * g.dyi;
-            const double c2 = lg[2], c3 = lg[3] * clight, c4 = lg[4] * clight, c5 = lg[5];
-            const double f0 = F[0][k], f1 = F[1][k], f2 = F[2][k], f3 = F[3][k], f4 = F[4][k];
-            acc_x += wpsi_x * f0 + (c2 * ss_x) * f1 + (c3 * ss_x) * f2 + (c4 * ss_x) * f3 + (c5 * ss_x) * f4;
-            acc_y += wpsi_y * f0 + (c2 * ss_y) * f1 + (c3 * ss_y) * f2 + (c4 * ss_y) * f3 + (c5 * ss_y) * f4;
-            const double wez = c2 * ss, wbx = c3 * ss, wby = c4 * ss, wbz = c5 * ss;
-            if (wpsi != 0.0) atomic_add(adj + long(L.Psi) * g.plane + k, wpsi);
-            if (wez != 0.0) atomic_add(adj + long(L.Ez) * g.plane + k, wez);
-            if (wbx != 0.0) atomic_add(adj + long(L.Bx) * g.plane + k, wbx);
-            if (wby != 0.0) atomic_add(adj + long(L.By) * g.plane + k, wby);
-            if (wbz != 0.0) atomic_add(adj + long(L.Bz) * g.plane + k, wbz);
+        for (int q = 0; q < 6; ++q) {
+            lg[q] = gadj[long(q) * p.n + ip];
+            any = any || lg[q] != 0.0;
         }
-    in.x[ip] += acc_x;
-    in.y[ip] += acc_y;
+        act = any;
+    }
+    int ci = 0, cj = 0;
+    if (act) {
+        ci = dshape<1, O>((p.x_prev[ip] - g.xoff) * g.dxi, 0).cell;
+        cj = dshape<1, O>((p.y_prev[ip] - g.yoff) * g.dyi, 0).cell;
+    }
+    const AdjWin Wn = adj_window(ci, cj, act, W, VJ_WIN, red);
+    const int cells = Wn.cells();
+    for (int t = threadIdx.x; t < NP * cells; t += VJ_T) win[t] = 0.0;
+    __syncthreads();
+    const int comp[NP] = {L.Psi, L.Ez, L.Bx, L.By, L.Bz};
+    auto scatter = [&](int q, int i, int j, double v) {
+        if (Wn.w) lds_add(&win[q * cells + (j - Wn.j0) * Wn.w + (i - Wn.i0)], v);
+        else atomic_add(adj + long(comp[q]) * g.plane + g.idx(i, j), v);
+    };
+    if (act) {
+        const double clight = 1.0;
+        const X1 xm = (X1::seed(p.x_prev[ip], 0) - g.xoff) * g.dxi, ym = (X1::seed(p.y_prev[ip], 0) - g.yoff) * g.dyi;
+        DsfR<X1> sx[W], sy[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            sx[k] = dshape<1, O, X1, X1>(xm, k);
+            sy[k] = dshape<1, O, X1, X1>(ym, k);
+        }
+        const double* F[5] = {store + long(L.Psi) * g.plane, store + long(L.Ez) * g.plane,
+                              store + long(L.Bx) * g.plane, store + long(L.By) * g.plane, store + long(L.Bz) * g.plane};
+        double acc_x = 0.0, acc_y = 0.0;
+#pragma unroll
+        for (int iy = 0; iy < W; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < W; ++ix) {
+                const DsfR<X1>& X = sx[ix];
+                const DsfR<X1>& Y = sy[iy];
+                if (!in_store(g, X.cell, Y.cell)) continue;
+                const long k = g.idx(X.cell, Y.cell);
+                const double ss = X.s.v * Y.s.v, ss_x = X.s.d[0] * Y.s.v, ss_y = X.s.v * Y.s.d[0];
+                const double ds = X.ds.v * Y.s.v, ds_x = X.ds.d[0] * Y.s.v, ds_y = X.ds.v * Y.s.d[0];
+                const double sd = X.s.v * Y.ds.v, sd_x = X.s.d[0] * Y.ds.v, sd_y = X.s.v * Y.ds.d[0];
+                // wpsi = lg0 (X.ds Y.s) dxi + lg1 (X.s Y.ds) dyi; wez, wbx, wby, wbz = lg * ss
+                const double wpsi = lg[0] * ds * g.dxi + lg[1] * sd * g.dyi;
+                const double wpsi_x = lg[0] * ds_x * g.dxi + lg[1] * sd_x * g.dyi;
+                const double wpsi_y = lg[0] * ds_y * g.dxi + lg[1] * sd_y * g.dyi;
+                const double c2 = lg[2], c3 = lg[3] * clight, c4 = lg[4] * clight, c5 = lg[5];
+                const double f0 = F[0][k], f1 = F[1][k], f2 = F[2][k], f3 = F[3][k], f4 = F[4][k];
+                acc_x += wpsi_x * f0 + (c2 * ss_x) * f1 + (c3 * ss_x) * f2 + (c4 * ss_x) * f3 + (c5 * ss_x) * f4;
+                acc_y += wpsi_y * f0 + (c2 * ss_y) * f1 + (c3 * ss_y) * f2 + (c4 * ss_y) * f3 + (c5 * ss_y) * f4;
+                const double w5[5] = {wpsi, c2 * ss, c3 * ss, c4 * ss, c5 * ss};
+#pragma unroll
+                for (int q = 0; q < 5; ++q)
+                    if (w5[q] != 0.0) scatter(q, X.cell, Y.cell, w5[q]);
+            }
+        in.x[ip] += acc_x;
+        in.y[ip] += acc_y;
+    }
+    __syncthreads();
+    if (Wn.w) adj_flush<NP>(g, Wn, win, adj, comp);
 }
 
 }  // namespace
@@ -400,7 +499,8 @@ void explicit_deposition_vjp(hpq_ctx* c, const hpq_plasma* p, const double* stor
     dispatch_order(c->p.depos_order, [&](auto O) {
         constexpr int o = decltype(O)::value;
         auto launch = [&](auto kern) {
-            hipLaunchKernelGGL(kern, grid1d(p->n, VT), dim3(VT), 0, c->stream, g, *p, *a, store, adj, c->L, qimu0, qmr);
+            hipLaunchKernelGGL(kern, grid1d(p->n, VJ_T), dim3(VJ_T), 0, c->stream, g, *p, *a, store, adj, c->L, qimu0,
+                               qmr);
         };
         if (T == 0) launch(k_explicit_vjp<o, 0>);
         else if (T == 1) launch(k_explicit_vjp<o, 1>);
@@ -427,7 +527,7 @@ void advance_plasma_vjp(hpq_ctx* c, const hpq_plasma* p, const double* store, co
         for (int k0 = PUSH_T0; k0 < PUSH_IN; k0 += NB)
             hipLaunchKernelGGL((k_push_vjp<o, NB>), grid1d(p->n, VT), dim3(VT), 0, c->stream, c->g, *p, store, c->L,
                                qmc, bc, *out, *in, A.push_g, k0);
-        hipLaunchKernelGGL((k_push_vjp_gather<o>), grid1d(p->n, VT), dim3(VT), 0, c->stream, c->g, *p, store, c->L,
+        hipLaunchKernelGGL((k_push_vjp_gather<o>), grid1d(p->n, VJ_T), dim3(VJ_T), 0, c->stream, c->g, *p, store, c->L,
                            A.push_g, *in, adj);
     });
     HPQ_HIP(hipGetLastError());
