@@ -42,6 +42,7 @@ COPY_BYTES = 256 << 20
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_511x511x512.json")
 DTYPE = {"fp64": "f64",
          "fp64_atomic": "f64",
+         "fp64_dst": "f64",
          "mixed": "f32 (gather, push increments, deposition) + split-bf16 x3 / f32 accumulate (DST) + f64 "
                   "(multigrid, particle state)"}
 
@@ -118,7 +119,7 @@ def run_gpu(args, path, capture):
         cfg.bin_period = args.bin_period
     precision = "mixed" if path == "mixed" else "fp64"
     cfg.precision = precision
-    cfg.deposition = "atomic" if path in ("mixed", "fp64_atomic") else "ordered"
+    cfg.deposition = "atomic" if path in ("mixed", "fp64_atomic", "fp64_dst") else "ordered"
     # the atomic paths reorder the plasma in place by deposition tile (plasmas.reorder_period);
     # the ordered path keeps the index order whose sums it reproduces
     cfg.reorder_period = args.reorder_period if cfg.deposition == "atomic" else 0
@@ -127,6 +128,9 @@ def run_gpu(args, path, capture):
         cfg.mixed_sums = args.mixed_sums
         cfg.mg_guess = args.mixed_mg_guess
         cfg.mg_guess_iters = args.mixed_mg_guess_iters
+    if path == "fp64_dst":
+        # FP64 with atomic deposition and the mixed path's DST-corrected multigrid guess
+        cfg.mg_guess, cfg.mg_guess_iters = "dst", args.mixed_mg_guess_iters
     sim = Simulation(cfg, beam)
     sim.init_plasma()
     pristine = {k: v.clone() for k, v in sim.plasma.items()}
@@ -531,7 +535,8 @@ def main():
                     help="slices between tile sorts of the plasma for deposition (0: index order)")
     ap.add_argument("--paths", default="fp64,fp64_atomic,mixed",
                     help="paths to time (comma separated): fp64 (ordered deposition, the headline), fp64_atomic, "
-                         "mixed; the first is the headline, the others report their error against it")
+                         "mixed, fp64_dst (atomic deposition with the mixed path's DST-corrected multigrid guess); the "
+                         "first is the headline, the others report their error against it")
     ap.add_argument("--mixed-poisson", choices=["auto", "bf16x3", "fft"], default="auto",
                     help="transverse Poisson solver of the mixed path (auto: bf16x3 up to n = 511, fft beyond)")
     ap.add_argument("--mixed-sums", choices=["fp64", "kahan32"], default="fp64",
@@ -571,7 +576,7 @@ def main():
         del src, dst
     paths = [q.strip() for q in args.paths.split(",") if q.strip()]
     for q in paths:
-        if q not in ("fp64", "fp64_atomic", "mixed"):
+        if q not in ("fp64", "fp64_atomic", "mixed", "fp64_dst"):
             raise SystemExit(f"bench.py: unknown path {q}")
     runs = {q: run_gpu(args, q, capture=len(paths) > 1) for q in paths}
     head = paths[0]
@@ -611,7 +616,7 @@ def main():
                 "mg_vcycles_per_slice": round(f["stats"]["mg_iters"] / cfg.nz, 3), "dtype": DTYPE[q],
                 "reorder_period": f["cfg"].reorder_period,
                 **({"mixed_sums": f["cfg"].mixed_sums, "mg_guess": f["cfg"].mg_guess,
-                    "mg_guess_iters": f["cfg"].mg_guess_iters} if q == "mixed" else {})}
+                    "mg_guess_iters": f["cfg"].mg_guess_iters} if q in ("mixed", "fp64_dst") else {})}
 
     extra = {}
     for q in paths[1:]:
