@@ -116,12 +116,18 @@ def test_mixed_push(order):
         assert np.array_equal(pg["valid"], po["valid"])
 
 
-@pytest.mark.parametrize("n,order,peak", [(63, 2, 1.0), (64, 1, 0.5)])
-def test_mixed_slices_vs_fp64_oracle(n, order, peak):
+@pytest.mark.parametrize("n,order,peak,guess", [(63, 2, 1.0, "previous"), (64, 1, 0.5, "previous"),
+                                               (63, 2, 1.0, "dst")])
+def test_mixed_slices_vs_fp64_oracle(n, order, peak, guess):
+    """The mixed path against the FP64 oracle, every slice within 1e-4; also with the
+    multigrid started from the DST-corrected guess (the benchmark's mixed path). That guess
+    changes where the multigrid stops inside its tolerance (1e-4 of the residual): Ez stays
+    within 1e-4, the other fields within 3e-4 (FP32 arithmetic alone stays near 1e-6 here)."""
     from hipace2_amd.inputs import BeamSpec, make_beam
     from hipace2_amd.simulation import Simulation
     from oracle.oracle import OracleSim
     cfg = _mixed(small_config(n=n, nz=48, order=order, explicit=True))
+    cfg.mg_guess = guess
     spec = BeamSpec(injection_type="fixed_weight", profile="gaussian", density=peak,
                     position_mean=(0.0, 0.0, 0.0), position_std=(0.3, 0.3, 1.41),
                     num_particles=20000, u_mean=(0.0, 0.0, 1000.0), seed=0)
@@ -137,7 +143,7 @@ def test_mixed_slices_vs_fp64_oracle(n, order, peak):
     errs = {k: rel_err(cg[k].cpu().numpy(), co[k]) for k in names}
     print("mixed vs FP64 oracle:", errs)
     for k, e in errs.items():
-        assert e <= 1e-4, (k, e)
+        assert e <= (1e-4 if guess == "previous" or k == "Ez" else 3e-4), (k, e)
 
 
 # Split-BF16 sine-matrix DST (poisson_solver="bf16x3"): operands carry ~16 significant bits
