@@ -47,12 +47,15 @@ DTYPE = {"fp64": "f64",
                   "(multigrid, particle state)"}
 
 
-def make_workload(n=511, nz=512, ppc=2, nbeam=4_000_000, large=False):
-    """The throughput workload (configuration 3) or, large, configuration 5
-    (hipace2_amd/workloads.py)."""
+def make_workload(n=511, nz=512, ppc=2, nbeam=4_000_000, kind="standard"):
+    """The throughput workload (configuration 3), the largest (configuration 5) or the strong
+    blowout (configuration 2: 255^2 x 256, 1 x 1 ppc, drive peak 10 n0; one pass of the slice
+    loop per step) of hipace2_amd/workloads.py."""
     from hipace2_amd import workloads
-    if large:
+    if kind == "large":
         return workloads.config5()
+    if kind == "blowout":
+        return workloads.config2()
     return workloads.config3(n, nz, ppc, nbeam)
 
 
@@ -114,7 +117,7 @@ def run_gpu(args, path, capture):
     import torch
     from hipace2_amd.simulation import Simulation
 
-    cfg, beam = make_workload(args.n, args.nz, args.ppc, args.nbeam, large=args.workload == "large")
+    cfg, beam = make_workload(args.n, args.nz, args.ppc, args.nbeam, kind=args.workload)
     if args.bin_period is not None:
         cfg.bin_period = args.bin_period
     precision = "mixed" if path == "mixed" else "fp64"
@@ -559,9 +562,10 @@ def main():
                     help="knots checked against central differences (two forward steps each)")
     ap.add_argument("--loading-iters", type=int, default=0,
                     help="adjoint gradient-descent iterations of the beam-loading comparison with SALAME (0: skip)")
-    ap.add_argument("--workload", choices=["standard", "large"], default="standard",
+    ap.add_argument("--workload", choices=["standard", "large", "blowout"], default="standard",
                     help="standard: 511^2 x 512 (the metric's configuration); large: 1023^2 x 1024, 2x2 ppc with "
-                         "a density ramp, drive 1e7 + witness 2e6 particles")
+                         "a density ramp, drive 1e7 + witness 2e6 particles; blowout: 255^2 x 256, 1x1 ppc, "
+                         "drive peak 10 n0 (the adjoint measurement stays the 511^2 one)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device to device, the reference dispatch for HBM byte counters")
     args = ap.parse_args()
