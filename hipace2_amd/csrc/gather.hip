@@ -574,7 +574,16 @@ __device__ __forceinline__ int list_position(long t, const int32_t* __restrict__
                                              const int32_t* __restrict__ unsorted) {
     const int a = off[key[t]], b = off[key[t] + 1];
     int r = 0;
-    for (int u = a; u < b; ++u) r += unsorted[u] < int32_t(t) ? 1 : 0;
+    int u = a;
+    // eight loads in flight at a time: a beam core's lists hold hundreds of particles
+    for (; u + 8 <= b; u += 8) {
+        int32_t q[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] = unsorted[u + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r += q[k] < int32_t(t) ? 1 : 0;
+    }
+    for (; u < b; ++u) r += unsorted[u] < int32_t(t) ? 1 : 0;
     return a + r;
 }
 
@@ -1203,6 +1212,9 @@ __global__ __launch_bounds__(64) void k_beam_sum_long(Geom g, const int32_t* __r
                                                       const double* __restrict__ contrib, long m,
                                                       double* __restrict__ store, int cjx, int cjy, int cjz,
                                                       const int32_t* __restrict__ longc) {
+    // LCH chunks of 64 contributions in flight: each chunk is read by the whole wavefront while
+    // three lanes add the previous ones in sequence order
+    constexpr int LCH = 4;
     __shared__ double buf[3][64];
     const int lane = int(threadIdx.x);
     const long ps = g.plane;
@@ -1212,27 +1224,47 @@ __global__ __launch_bounds__(64) void k_beam_sum_long(Geom g, const int32_t* __r
         const long t = longc[1 + w];
         const int a = seg[t], e = seg[t + 1];
         double acc = comp >= 0 ? store[comp * ps + t] : 0.0;
-        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-        if (a + lane < e) {
-            v0 = contrib[a + lane];
-            v1 = contrib[m + a + lane];
-            v2 = contrib[2 * m + a + lane];
-        }
-        for (int h0 = a; h0 < e; h0 += 64) {
-            __syncthreads();
-            buf[0][lane] = v0;
-            buf[1][lane] = v1;
-            buf[2][lane] = v2;
-            __syncthreads();
-            const int hn = h0 + 64 + lane;
-            if (hn < e) {
-                v0 = contrib[hn];
-                v1 = contrib[m + hn];
-                v2 = contrib[2 * m + hn];
+        double v[LCH][3];
+#pragma unroll
+        for (int d = 0; d < LCH; ++d) {
+            const int h = a + 64 * d + lane;
+            v[d][0] = v[d][1] = v[d][2] = 0.0;
+            if (h < e) {
+                v[d][0] = contrib[h];
+                v[d][1] = contrib[m + h];
+                v[d][2] = contrib[2 * m + h];
             }
-            if (comp >= 0) {
-                const int cnt = min(64, e - h0);
-                for (int u = 0; u < cnt; ++u) acc += buf[lane][u];
+        }
+        for (int h0 = a; h0 < e; h0 += 64 * LCH) {
+#pragma unroll
+            for (int d = 0; d < LCH; ++d) {
+                const int hc = h0 + 64 * d;
+                if (hc >= e) break;
+                __syncthreads();
+                buf[0][lane] = v[d][0];
+                buf[1][lane] = v[d][1];
+                buf[2][lane] = v[d][2];
+                __syncthreads();
+                const int hn = hc + 64 * LCH + lane;
+                if (hn < e) {
+                    v[d][0] = contrib[hn];
+                    v[d][1] = contrib[m + hn];
+                    v[d][2] = contrib[2 * m + hn];
+                }
+                if (comp >= 0) {
+                    // in sequence order; sixteen LDS reads in flight ahead of the adds
+                    const int cnt = min(64, e - hc);
+                    const double* row = buf[lane];
+                    int u = 0;
+                    for (; u + 16 <= cnt; u += 16) {
+                        double q[16];
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) q[k] = row[u + k];
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) acc += q[k];
+                    }
+                    for (; u < cnt; ++u) acc += row[u];
+                }
             }
         }
         if (comp >= 0) store[comp * ps + t] = acc;
