@@ -82,11 +82,15 @@ struct GlobalLists {
 // Calls f(pos, ix, iy) for every particle of the lists whose start cell lies in the S x S
 // window below cell (i, j), in the CPU build's order: colour by colour, and inside the window
 // part of one colour by increasing particle index (a merge of the lists). pos is the
-// particle's record position, (ix, iy) its stencil offset that lands on (i, j).
-template <int S, class Src, class F>
+// particle's record position, (ix, iy) its stencil offset that lands on (i, j). The lists of
+// the mask SKIP (bit ky * S + kx), whose contributions the caller drops, are left out. A list's
+// end is read from the source when the list is taken, not kept: the merge's registers bound
+// how many wavefronts share a SIMD.
+template <int S, unsigned long long SKIP = 0ull, class Src, class F>
 __device__ __forceinline__ void walk_ordered(const Geom& g, const CLView& cl, int i, int j, const Src& src, F&& f) {
     const int x0 = i - S + 1, y0 = j - S + 1;
-    int h[S * S], e[S * S];
+    int h[S * S];
+    bool ne[S * S];   // list non-empty
 #pragma unroll
     for (int ky = 0; ky < S; ++ky) {
         int o[S + 1];
@@ -95,7 +99,7 @@ __device__ __forceinline__ void walk_ordered(const Geom& g, const CLView& cl, in
 #pragma unroll
         for (int kx = 0; kx < S; ++kx) {
             h[ky * S + kx] = o[kx];
-            e[ky * S + kx] = o[kx + 1];
+            ne[ky * S + kx] = o[kx] < o[kx + 1];
         }
     }
     // window split at tile boundaries: columns [x0, xs) in tile txa, [xs, i] in tile txb
@@ -119,13 +123,14 @@ __device__ __forceinline__ void walk_ordered(const Geom& g, const CLView& cl, in
 #pragma unroll
         for (int k = 0; k < S * S; ++k) {
             const int sx = x0 + k % S, sy = y0 + k / S;
-            const bool act = sx >= ax && sx <= bx && sy >= ay && sy <= by && h[k] < e[k];
+            const bool act = !((SKIP >> k) & 1ull) && sx >= ax && sx <= bx && sy >= ay && sy <= by && ne[k];
             v[k] = act ? src.idx(h[k]) : INT_MAX;
         }
         while (true) {
             int m1 = INT_MAX, m2 = INT_MAX, km = -1;
 #pragma unroll
             for (int k = 0; k < S * S; ++k) {
+                if ((SKIP >> k) & 1ull) continue;
                 if (v[k] < m1) {
                     m2 = m1;
                     m1 = v[k];
@@ -135,12 +140,13 @@ __device__ __forceinline__ void walk_ordered(const Geom& g, const CLView& cl, in
                 }
             }
             if (km < 0) break;
-            int hc = 0, ec = 0;
+            int hc = 0;
 #pragma unroll
             for (int k = 0; k < S * S; ++k)
-                if (k == km) { hc = h[k]; ec = e[k]; }
-            const int ix = S - 1 - km % S, iy = S - 1 - km / S;
-            const int kyl = km / S;
+                if (!((SKIP >> k) & 1ull) && k == km) hc = h[k];
+            const int kyl = km / S, kxl = km - kyl * S;
+            const int ec = src.bound(kxl + 1, kyl);
+            const int ix = S - 1 - kxl, iy = S - 1 - kyl;
             // the list holding the smallest index goes on while it stays below every other head
             int p = m1;
             while (true) {
@@ -151,7 +157,7 @@ __device__ __forceinline__ void walk_ordered(const Geom& g, const CLView& cl, in
             }
 #pragma unroll
             for (int k = 0; k < S * S; ++k)
-                if (k == km) { h[k] = hc; v[k] = p; }
+                if (!((SKIP >> k) & 1ull) && k == km) { h[k] = hc; v[k] = p; }
         }
     }
 }
@@ -385,7 +391,7 @@ __device__ __forceinline__ bool block_raster_ordered(const Geom& g, const CLView
 // time, records read ahead by unrolling); `at` is the staged entry when ENTRY (records staged
 // in LDS), the record position otherwise. MERGE: the merge of the lists, from LDS when the
 // block's rows were staged, from global memory otherwise; `at` is the record position.
-template <int S, int DELTA, bool FAST, bool ENTRY, int CAP, class Load, class Add>
+template <int S, int DELTA, bool FAST, bool ENTRY, unsigned long long SKIP = 0ull, int CAP, class Load, class Add>
 __device__ __forceinline__ void walk_cell(const Geom& g, const CLView& cl, const StagedLists<CAP>& sl, int i, int j,
                                           int li, int lj, Load&& load, Add&& add) {
     using Rec = decltype(load(0, 0, 0));
@@ -401,7 +407,7 @@ __device__ __forceinline__ void walk_cell(const Geom& g, const CLView& cl, const
 #pragma unroll
             for (int k = 0; k < S * S; ++k) {
                 const int kx = k % S, ky = k / S;
-                if (!((mk >> k) & 1ull)) continue;
+                if (!((mk >> k) & 1ull) || ((SKIP >> k) & 1ull)) continue;
                 const int a = src.bound(kx, ky), e = src.bound(kx + 1, ky);
 #pragma unroll S <= 3 ? 2 : 1
                 for (int h = a; h < e; ++h)
@@ -411,10 +417,10 @@ __device__ __forceinline__ void walk_cell(const Geom& g, const CLView& cl, const
     } else {
         auto f = pipelined<Rec>(load, add);
         if (sl.total <= CAP) {
-            walk_ordered<S>(g, cl, i, j, src, f);
+            walk_ordered<S, SKIP>(g, cl, i, j, src, f);
         } else {
             const GlobalLists gsrc{cl, j - S + 1 - DELTA - cl.bylo, i - S + 1 - DELTA - cl.bxlo};
-            walk_ordered<S>(g, cl, i, j, gsrc, f);
+            walk_ordered<S, SKIP>(g, cl, i, j, gsrc, f);
         }
         f.finish();
     }
@@ -766,7 +772,11 @@ __global__ __launch_bounds__((plasma_gather_threads<STAGED, BH>())) void k_gathe
     const double clight = 1.0, clight_inv = 1.0 / 1.0;
     const double dxi = g.dxi, dyi = g.dyi;
     const double q_mass_ratio = charge_mass_ratio;
-    walk_cell<S, DELTA, STAGED, STAGED>(
+    // the centred derivative's corner offsets contribute nothing (see the add below): the
+    // merge leaves their lists out
+    constexpr unsigned long long CORNERS =
+        T == 2 ? (1ull | (1ull << (S - 1)) | (1ull << (S * (S - 1))) | (1ull << (S * S - 1))) : 0ull;
+    walk_cell<S, DELTA, STAGED, STAGED, CORNERS>(
         g, cl, sl, i, j, li, lj, [&](int at, int, int) { return pr.load(at); },
         [&](const PlasmaRec& r, int ix, int iy) {
             if (T == 2 && (ix == 0 || ix == O + 2) && (iy == 0 || iy == O + 2)) return;
